@@ -1,0 +1,661 @@
+// hpo.cpp -- the oracle's linear algebra, solver and overlap restatements and the C ABI
+// the test harness loads.  TEST INFRASTRUCTURE ONLY (see hpo.h).
+//
+// Loop orders follow the reference exactly; rows are split across OpenMP threads where
+// the reference uses rayon par_chunks_mut (results per element do not depend on the
+// thread count, as in the reference).
+#include "hpo.h"
+#include "hpo_core.hpp"
+
+#include <algorithm>
+#include <cfloat>
+#include <cmath>
+#include <cstring>
+#include <vector>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+using namespace hpo;
+using D = DD<double>;
+using M4 = MF<double>;
+
+static inline D ld(const double* p, size_t i) { return D{p[2 * i], p[2 * i + 1]}; }
+static inline void st(double* p, size_t i, D v) { p[2 * i] = v.hi; p[2 * i + 1] = v.lo; }
+static inline M4 ldm(const double* p, size_t i) { M4 r; std::memcpy(r.l, p + 4 * i, 32); return r; }
+static inline void stm(double* p, size_t i, const M4& v) { std::memcpy(p + 4 * i, v.l, 32); }
+static inline hpo_dd to_c(D v) { return hpo_dd{v.hi, v.lo}; }
+static inline D from_c(hpo_dd v) { return D{v.hi, v.lo}; }
+static inline hpo_mf4 to_cm(const M4& v) { hpo_mf4 r; std::memcpy(r.l, v.l, 32); return r; }
+static inline M4 from_cm(const hpo_mf4& v) { M4 r; std::memcpy(r.l, v.l, 32); return r; }
+
+extern "C" {
+
+int hpo_num_threads(void) {
+#ifdef _OPENMP
+  return omp_get_max_threads();
+#else
+  return 1;
+#endif
+}
+
+// ------------------------------------------------------------------ dd scalar ops
+hpo_dd hpo_dd_add(hpo_dd a, hpo_dd b) { return to_c(add(from_c(a), from_c(b))); }
+hpo_dd hpo_dd_sub(hpo_dd a, hpo_dd b) { return to_c(sub(from_c(a), from_c(b))); }
+hpo_dd hpo_dd_mul(hpo_dd a, hpo_dd b) { return to_c(mul(from_c(a), from_c(b))); }
+hpo_dd hpo_dd_div(hpo_dd a, hpo_dd b) { return to_c(div(from_c(a), from_c(b))); }
+hpo_dd hpo_dd_add_f64(hpo_dd a, double b) { return to_c(add_f(from_c(a), b)); }
+hpo_dd hpo_dd_mul_f64(hpo_dd a, double b) { return to_c(mul_f(from_c(a), b)); }
+hpo_dd hpo_dd_div_f64(hpo_dd a, double b) { return to_c(div_f(from_c(a), b)); }
+hpo_dd hpo_dd_sqrt(hpo_dd a) { return to_c(dsqrt(from_c(a))); }
+hpo_dd hpo_dd_recip(hpo_dd a) { return to_c(recip(from_c(a))); }
+hpo_dd hpo_dd_exp(hpo_dd a) { return to_c(dexp(from_c(a))); }
+hpo_dd hpo_dd_ln(hpo_dd a) { return to_c(dln(from_c(a))); }
+
+// op: 0 add, 1 sub, 2 mul, 3 div, 4 add_f64 (b = hi of b), 5 mul_f64, 6 div_f64
+void hpo_dd_binop_n(int op, const double* a, const double* b, double* out, size_t n) {
+  for (size_t i = 0; i < n; ++i) {
+    D x = ld(a, i), y = ld(b, i), r{0, 0};
+    switch (op) {
+      case 0: r = add(x, y); break;
+      case 1: r = sub(x, y); break;
+      case 2: r = mul(x, y); break;
+      case 3: r = div(x, y); break;
+      case 4: r = add_f(x, y.hi); break;
+      case 5: r = mul_f(x, y.hi); break;
+      case 6: r = div_f(x, y.hi); break;
+    }
+    st(out, i, r);
+  }
+}
+// op: 0 sqrt, 1 recip, 2 exp, 3 ln, 4 neg, 5 abs
+void hpo_dd_unop_n(int op, const double* a, double* out, size_t n) {
+  for (size_t i = 0; i < n; ++i) {
+    D x = ld(a, i), r{0, 0};
+    switch (op) {
+      case 0: r = dsqrt(x); break;
+      case 1: r = recip(x); break;
+      case 2: r = dexp(x); break;
+      case 3: r = dln(x); break;
+      case 4: r = neg(x); break;
+      case 5: r = dabs(x); break;
+    }
+    st(out, i, r);
+  }
+}
+
+// ------------------------------------------------------------------ MF4 scalar ops
+void hpo_mf4_renormalize(double* limbs4) { renormalize(limbs4); }
+void hpo_mf4_renormalize_from8(const double* in8, double* out4) { renormalize_from8(in8, out4); }
+hpo_mf4 hpo_mf4_add(hpo_mf4 a, hpo_mf4 b) { return to_cm(mf_add(from_cm(a), from_cm(b))); }
+hpo_mf4 hpo_mf4_sub(hpo_mf4 a, hpo_mf4 b) { return to_cm(mf_sub(from_cm(a), from_cm(b))); }
+hpo_mf4 hpo_mf4_mul(hpo_mf4 a, hpo_mf4 b) { return to_cm(mf_mul(from_cm(a), from_cm(b))); }
+hpo_mf4 hpo_mf4_mul_f64(hpo_mf4 a, double b) { return to_cm(mf_mul_f(from_cm(a), b)); }
+hpo_mf4 hpo_mf4_recip(hpo_mf4 a) { return to_cm(mf_recip(from_cm(a))); }
+hpo_mf4 hpo_mf4_div(hpo_mf4 a, hpo_mf4 b) { return to_cm(mf_div(from_cm(a), from_cm(b))); }
+hpo_mf4 hpo_mf4_sqrt(hpo_mf4 a) { return to_cm(mf_sqrt(from_cm(a))); }
+hpo_mf4 hpo_mf4_exp(hpo_mf4 a) { return to_cm(mf_exp(from_cm(a))); }
+
+// op: 0 add, 1 sub, 2 mul, 3 div, 4 mul_f64 (b = limb 0 of b)
+void hpo_mf4_binop_n(int op, const double* a, const double* b, double* out, size_t n) {
+  for (size_t i = 0; i < n; ++i) {
+    M4 x = ldm(a, i), y = ldm(b, i), r = mf_zero<double>();
+    switch (op) {
+      case 0: r = mf_add(x, y); break;
+      case 1: r = mf_sub(x, y); break;
+      case 2: r = mf_mul(x, y); break;
+      case 3: r = mf_div(x, y); break;
+      case 4: r = mf_mul_f(x, y.l[0]); break;
+    }
+    stm(out, i, r);
+  }
+}
+// op: 0 sqrt, 1 recip, 2 exp, 3 renormalize
+void hpo_mf4_unop_n(int op, const double* a, double* out, size_t n) {
+  for (size_t i = 0; i < n; ++i) {
+    M4 x = ldm(a, i), r = x;
+    switch (op) {
+      case 0: r = mf_sqrt(x); break;
+      case 1: r = mf_recip(x); break;
+      case 2: r = mf_exp(x); break;
+      case 3: renormalize(r.l); break;
+    }
+    stm(out, i, r);
+  }
+}
+
+// ------------------------------------------------------------------ dd linalg
+// gemm: crates/quad/src/linalg.rs:150-165 (row-parallel; per element a sequential dd sum)
+void hpo_gemm_rows(const double* a, const double* b, double* c, size_t m, size_t n, size_t k,
+                   size_t row0, size_t nrows) {
+  (void)m;
+#pragma omp parallel for schedule(dynamic, 1)
+  for (long ii = 0; ii < (long)nrows; ++ii) {
+    size_t i = row0 + (size_t)ii;
+    for (size_t j = 0; j < n; ++j) {
+      D s{0.0, 0.0};
+      for (size_t p = 0; p < k; ++p) s = add(s, mul(ld(a, i * k + p), ld(b, p * n + j)));
+      st(c, i * n + j, s);
+    }
+  }
+}
+void hpo_gemm(const double* a, const double* b, double* c, size_t m, size_t n, size_t k) {
+  hpo_gemm_rows(a, b, c, m, n, k, 0, m);
+}
+// gemm_atb: linalg.rs:168-182 (A stored k x m)
+void hpo_gemm_atb_rows(const double* a, const double* b, double* c, size_t m, size_t n, size_t k,
+                       size_t row0, size_t nrows) {
+#pragma omp parallel for schedule(dynamic, 1)
+  for (long ii = 0; ii < (long)nrows; ++ii) {
+    size_t i = row0 + (size_t)ii;
+    for (size_t j = 0; j < n; ++j) {
+      D s{0.0, 0.0};
+      for (size_t p = 0; p < k; ++p) s = add(s, mul(ld(a, p * m + i), ld(b, p * n + j)));
+      st(c, i * n + j, s);
+    }
+  }
+}
+void hpo_gemm_atb(const double* a, const double* b, double* c, size_t m, size_t n, size_t k) {
+  hpo_gemm_atb_rows(a, b, c, m, n, k, 0, m);
+}
+
+// cholesky (unblocked): linalg.rs:42-96.  Only the lower triangle is written.
+int hpo_cholesky(double* a, size_t n, size_t* pivot, double* value) {
+  std::vector<D> row_j(n);
+  for (size_t j = 0; j < n; ++j) {
+    D sum{0.0, 0.0};
+    for (size_t k = 0; k < j; ++k) {
+      D ljk = ld(a, j * n + k);
+      sum = add(sum, mul(ljk, ljk));
+    }
+    D diag = sub(ld(a, j * n + j), sum);
+    if (diag.hi <= 0.0) {
+      if (pivot) *pivot = j;
+      if (value) *value = diag.hi + diag.lo;
+      return 1;
+    }
+    D ljj = dsqrt(diag);
+    st(a, j * n + j, ljj);
+    D ljj_inv = recip(ljj);
+    for (size_t k = 0; k < j; ++k) row_j[k] = ld(a, j * n + k);
+    size_t remaining = n - j - 1;
+    if (remaining == 0) continue;
+#pragma omp parallel for schedule(static) if (remaining >= 64 && j >= 4)
+    for (long idx = 0; idx < (long)remaining; ++idx) {
+      size_t i = j + 1 + (size_t)idx;
+      D s{0.0, 0.0};
+      for (size_t k = 0; k < j; ++k) s = add(s, mul(ld(a, i * n + k), row_j[k]));
+      st(a, i * n + j, mul(sub(ld(a, i * n + j), s), ljj_inv));
+    }
+  }
+  return 0;
+}
+
+// trsm_right_lt: linalg.rs:224-235 (l is jb x jb, b is m x jb, both dense row-major)
+static void trsm_right_lt(const D* l, size_t jb, D* b, size_t m) {
+  for (size_t j = 0; j < jb; ++j) {
+    D ljj_inv = recip(l[j * jb + j]);
+    for (size_t i = 0; i < m; ++i) {
+      D s = b[i * jb + j];
+      for (size_t k = 0; k < j; ++k) s = sub(s, mul(b[i * jb + k], l[j * jb + k]));
+      b[i * jb + j] = mul(s, ljj_inv);
+    }
+  }
+}
+
+// cholesky_blocked: linalg.rs:238-293
+int hpo_cholesky_blocked(double* a, size_t n, size_t nb, size_t* pivot, double* value) {
+  if (n == 0) return 0;
+  if (nb == 0) nb = 64;
+  std::vector<D> diag(nb * nb), panel(n * nb);
+  for (size_t jblk = 0; jblk < n; jblk += nb) {
+    size_t jb = std::min(nb, n - jblk);
+    for (size_t i = 0; i < jb; ++i)
+      for (size_t j = 0; j < jb; ++j) diag[i * jb + j] = ld(a, (jblk + i) * n + jblk + j);
+    {
+      std::vector<double> tmp(2 * jb * jb);
+      for (size_t i = 0; i < jb * jb; ++i) { tmp[2 * i] = diag[i].hi; tmp[2 * i + 1] = diag[i].lo; }
+      size_t pv = 0; double vv = 0;
+      if (hpo_cholesky(tmp.data(), jb, &pv, &vv)) {
+        if (pivot) *pivot = pv + jblk;
+        if (value) *value = vv;
+        return 1;
+      }
+      for (size_t i = 0; i < jb * jb; ++i) diag[i] = D{tmp[2 * i], tmp[2 * i + 1]};
+    }
+    for (size_t i = 0; i < jb; ++i)
+      for (size_t j = 0; j < jb; ++j) st(a, (jblk + i) * n + jblk + j, diag[i * jb + j]);
+    size_t remaining = n - jblk - jb;
+    if (remaining == 0) continue;
+    for (size_t i = 0; i < remaining; ++i)
+      for (size_t j = 0; j < jb; ++j) panel[i * jb + j] = ld(a, (jblk + jb + i) * n + jblk + j);
+    trsm_right_lt(diag.data(), jb, panel.data(), remaining);
+    for (size_t i = 0; i < remaining; ++i)
+      for (size_t j = 0; j < jb; ++j) st(a, (jblk + jb + i) * n + jblk + j, panel[i * jb + j]);
+    size_t t0 = jblk + jb;
+#pragma omp parallel for schedule(dynamic, 4) if (remaining >= 64)
+    for (long ii = 0; ii < (long)remaining; ++ii) {
+      size_t i = (size_t)ii;
+      for (size_t cj = 0; cj <= i; ++cj) {
+        D s{0.0, 0.0};
+        for (size_t k = 0; k < jb; ++k) s = add(s, mul(panel[i * jb + k], panel[cj * jb + k]));
+        size_t idx = (t0 + i) * n + t0 + cj;
+        st(a, idx, sub(ld(a, idx), s));
+      }
+    }
+  }
+  return 0;
+}
+
+// forward_solve / backward_solve / solve_cholesky: linalg.rs:99-126
+void hpo_forward_solve(const double* l, const double* b, double* y, size_t n) {
+  std::vector<D> v(n);
+  for (size_t i = 0; i < n; ++i) v[i] = ld(b, i);
+  for (size_t i = 0; i < n; ++i) {
+    for (size_t j = 0; j < i; ++j) v[i] = sub(v[i], mul(ld(l, i * n + j), v[j]));
+    v[i] = div(v[i], ld(l, i * n + i));
+  }
+  for (size_t i = 0; i < n; ++i) st(y, i, v[i]);
+}
+void hpo_backward_solve(const double* l, const double* y, double* x, size_t n) {
+  std::vector<D> v(n);
+  for (size_t i = 0; i < n; ++i) v[i] = ld(y, i);
+  for (size_t ii = n; ii-- > 0;) {
+    for (size_t j = ii + 1; j < n; ++j) v[ii] = sub(v[ii], mul(ld(l, j * n + ii), v[j]));
+    v[ii] = div(v[ii], ld(l, ii * n + ii));
+  }
+  for (size_t i = 0; i < n; ++i) st(x, i, v[i]);
+}
+void hpo_solve_cholesky(const double* l, const double* b, double* x, size_t n) {
+  std::vector<double> y(2 * n);
+  hpo_forward_solve(l, b, y.data(), n);
+  hpo_backward_solve(l, y.data(), x, n);
+}
+// matvec: linalg.rs:129-147
+void hpo_matvec(const double* a, const double* x, double* y, size_t n) {
+#pragma omp parallel for schedule(static) if (n >= 64)
+  for (long ii = 0; ii < (long)n; ++ii) {
+    size_t i = (size_t)ii;
+    D s{0.0, 0.0};
+    for (size_t j = 0; j < n; ++j) s = add(s, mul(ld(a, i * n + j), ld(x, j)));
+    st(y, i, s);
+  }
+}
+// cond_estimate: linalg.rs:185-194
+double hpo_cond_estimate(const double* l, size_t n) {
+  double dmax = 0.0, dmin = DBL_MAX;
+  for (size_t i = 0; i < n; ++i) {
+    D v = ld(l, i * n + i);
+    double d = std::fabs(v.hi + v.lo);
+    dmax = std::fmax(dmax, d);
+    dmin = std::fmin(dmin, d);
+  }
+  double r = dmax / dmin;
+  return r * r;
+}
+
+// ------------------------------------------------------------------ MF4 linalg
+// mf_gemm: multifloat/linalg.rs:39-61
+void hpo_mf4_gemm_rows(const double* a, const double* b, double* c, size_t m, size_t n, size_t k,
+                       size_t row0, size_t nrows) {
+  (void)m;
+#pragma omp parallel for schedule(dynamic, 1)
+  for (long ii = 0; ii < (long)nrows; ++ii) {
+    size_t i = row0 + (size_t)ii;
+    for (size_t j = 0; j < n; ++j) {
+      M4 s = mf_zero<double>();
+      for (size_t p = 0; p < k; ++p) s = mf_add(s, mf_mul(ldm(a, i * k + p), ldm(b, p * n + j)));
+      stm(c, i * n + j, s);
+    }
+  }
+}
+void hpo_mf4_gemm(const double* a, const double* b, double* c, size_t m, size_t n, size_t k) {
+  hpo_mf4_gemm_rows(a, b, c, m, n, k, 0, m);
+}
+// mf_gemm_atb: multifloat/linalg.rs:64-85
+void hpo_mf4_gemm_atb(const double* a, const double* b, double* c, size_t m, size_t n, size_t k) {
+#pragma omp parallel for schedule(dynamic, 1)
+  for (long ii = 0; ii < (long)m; ++ii) {
+    size_t i = (size_t)ii;
+    for (size_t j = 0; j < n; ++j) {
+      M4 s = mf_zero<double>();
+      for (size_t p = 0; p < k; ++p) s = mf_add(s, mf_mul(ldm(a, p * m + i), ldm(b, p * n + j)));
+      stm(c, i * n + j, s);
+    }
+  }
+}
+// mf_cholesky: multifloat/linalg.rs:89-141.  Returns 0 ok, 1 not positive definite.
+int hpo_mf4_cholesky(double* a, size_t n) {
+  std::vector<M4> row_j(n);
+  for (size_t j = 0; j < n; ++j) {
+    M4 sum = mf_zero<double>();
+    for (size_t k = 0; k < j; ++k) { M4 v = ldm(a, j * n + k); sum = mf_add(sum, mf_mul(v, v)); }
+    M4 dg = mf_sub(ldm(a, j * n + j), sum);
+    if (dg.l[0] <= 0.0) return 1;
+    M4 ljj = mf_sqrt(dg);
+    stm(a, j * n + j, ljj);
+    M4 inv = mf_recip(ljj);
+    for (size_t k = 0; k < j; ++k) row_j[k] = ldm(a, j * n + k);
+    size_t remaining = n - j - 1;
+    if (remaining == 0) continue;
+#pragma omp parallel for schedule(static) if (remaining >= 64 && j >= 4)
+    for (long idx = 0; idx < (long)remaining; ++idx) {
+      size_t i = j + 1 + (size_t)idx;
+      M4 s = mf_zero<double>();
+      for (size_t k = 0; k < j; ++k) s = mf_add(s, mf_mul(ldm(a, i * n + k), row_j[k]));
+      stm(a, i * n + j, mf_mul(mf_sub(ldm(a, i * n + j), s), inv));
+    }
+  }
+  return 0;
+}
+// mf_cholesky_solve: multifloat/linalg.rs:145-172
+void hpo_mf4_cholesky_solve(const double* l, const double* b, double* x, size_t n) {
+  std::vector<M4> y(n);
+  for (size_t i = 0; i < n; ++i) y[i] = ldm(b, i);
+  for (size_t i = 0; i < n; ++i) {
+    for (size_t j = 0; j < i; ++j) y[i] = mf_sub(y[i], mf_mul(ldm(l, i * n + j), y[j]));
+    y[i] = mf_div(y[i], ldm(l, i * n + i));
+  }
+  for (size_t ii = n; ii-- > 0;) {
+    for (size_t j = ii + 1; j < n; ++j) y[ii] = mf_sub(y[ii], mf_mul(ldm(l, j * n + ii), y[j]));
+    y[ii] = mf_div(y[ii], ldm(l, ii * n + ii));
+  }
+  for (size_t i = 0; i < n; ++i) stm(x, i, y[i]);
+}
+// mf_gemv: multifloat/linalg.rs:17-36 (A is n x m)
+void hpo_mf4_gemv(const double* a, const double* x, double* y, size_t n, size_t m) {
+#pragma omp parallel for schedule(static)
+  for (long ii = 0; ii < (long)n; ++ii) {
+    size_t i = (size_t)ii;
+    M4 s = mf_zero<double>();
+    for (size_t j = 0; j < m; ++j) s = mf_add(s, mf_mul(ldm(a, i * m + j), ldm(x, j)));
+    stm(y, i, s);
+  }
+}
+
+// ------------------------------------------------------------------ solver crate
+// norm_f128: solver/src/lib.rs:243-249
+double hpo_norm_f128(const double* x, size_t n) {
+  D s{0.0, 0.0};
+  for (size_t i = 0; i < n; ++i) { D v = ld(x, i); s = add(s, mul(v, v)); }
+  D r = dsqrt(s);
+  return r.hi + r.lo;
+}
+// residual_norm_f128_from_f128: lib.rs:237-241
+static double residual_norm_dd(const double* a, const double* x, const double* b, size_t n) {
+  std::vector<double> ax(2 * n), r(2 * n);
+  hpo_matvec(a, x, ax.data(), n);
+  for (size_t i = 0; i < n; ++i) st(r.data(), i, sub(ld(b, i), ld(ax.data(), i)));
+  return hpo_norm_f128(r.data(), n);
+}
+// solve_pure_f128: lib.rs:127-144
+static void solve_pure_f128(const double* a, const double* b, size_t n, hpo_solution* sol,
+                            double* x_dd, double* x_f64) {
+  std::vector<double> l(a, a + 2 * n * n);
+  size_t pv = 0; double vv = 0;
+  if (hpo_cholesky(l.data(), n, &pv, &vv)) { sol->error = 1; sol->pivot = pv; return; }
+  sol->kappa = hpo_cond_estimate(l.data(), n);
+  hpo_solve_cholesky(l.data(), b, x_dd, n);
+  for (size_t i = 0; i < n; ++i) x_f64[i] = x_dd[2 * i] + x_dd[2 * i + 1];
+  sol->residual_norm = residual_norm_dd(a, x_dd, b, n);
+  sol->strategy = 2;
+  sol->refinement_iters = 0;
+  sol->error = 0;
+}
+// iterative_refinement: lib.rs:75-125
+static void iterative_refinement(const double* a, const double* b, const std::vector<double>& x0,
+                                 size_t n, hpo_solution* sol, double* x_dd, double* x_f64) {
+  std::vector<double> l(a, a + 2 * n * n);
+  size_t pv = 0; double vv = 0;
+  if (hpo_cholesky(l.data(), n, &pv, &vv)) { sol->error = 1; sol->pivot = pv; return; }
+  sol->kappa = hpo_cond_estimate(l.data(), n);
+  for (size_t i = 0; i < n; ++i) { x_dd[2 * i] = x0[i]; x_dd[2 * i + 1] = 0.0; }
+  double bnorm = hpo_norm_f128(b, n);
+  double best = DBL_MAX;
+  int iters = 0;
+  std::vector<double> ax(2 * n), r(2 * n), dx(2 * n);
+  for (int it = 0; it < 30; ++it) {
+    hpo_matvec(a, x_dd, ax.data(), n);
+    for (size_t i = 0; i < n; ++i) st(r.data(), i, sub(ld(b, i), ld(ax.data(), i)));
+    double rnorm = hpo_norm_f128(r.data(), n);
+    iters += 1;
+    if (rnorm < 1e-28 * bnorm) break;
+    if (iters > 3 && rnorm > 0.5 * best) break;
+    if (rnorm < best) best = rnorm;
+    hpo_solve_cholesky(l.data(), r.data(), dx.data(), n);
+    for (size_t i = 0; i < n; ++i) st(x_dd, i, add(ld(x_dd, i), ld(dx.data(), i)));
+  }
+  for (size_t i = 0; i < n; ++i) x_f64[i] = x_dd[2 * i] + x_dd[2 * i + 1];
+  sol->residual_norm = residual_norm_dd(a, x_dd, b, n);
+  sol->strategy = 1;
+  sol->refinement_iters = iters;
+  sol->error = 0;
+}
+// cholesky_f64 / solve_cholesky_f64 / cond_estimate_f64: lib.rs:169-224 (non-macOS path)
+static bool cholesky_f64(const double* a, size_t n, std::vector<double>& l) {
+  l.assign(a, a + n * n);
+  for (size_t j = 0; j < n; ++j) {
+    double sum = 0.0;
+    for (size_t k = 0; k < j; ++k) sum += l[j * n + k] * l[j * n + k];
+    double diag = l[j * n + j] - sum;
+    if (diag <= 0.0) return false;
+    double ljj = std::sqrt(diag);
+    l[j * n + j] = ljj;
+    double inv = 1.0 / ljj;
+    for (size_t i = j + 1; i < n; ++i) {
+      double s = 0.0;
+      for (size_t k = 0; k < j; ++k) s += l[i * n + k] * l[j * n + k];
+      l[i * n + j] = (l[i * n + j] - s) * inv;
+    }
+    for (size_t i = 0; i < j; ++i) l[i * n + j] = 0.0;
+  }
+  return true;
+}
+static void solve_cholesky_f64(const std::vector<double>& l, const double* b, size_t n, std::vector<double>& y) {
+  y.assign(b, b + n);
+  for (size_t i = 0; i < n; ++i) {
+    for (size_t j = 0; j < i; ++j) y[i] -= l[i * n + j] * y[j];
+    y[i] /= l[i * n + i];
+  }
+  for (size_t ii = n; ii-- > 0;) {
+    for (size_t j = ii + 1; j < n; ++j) y[ii] -= l[j * n + ii] * y[j];
+    y[ii] /= l[ii * n + ii];
+  }
+}
+static double cond_estimate_f64(const std::vector<double>& l, size_t n) {
+  double dmax = 0.0, dmin = DBL_MAX;
+  for (size_t i = 0; i < n; ++i) {
+    double d = std::fabs(l[i * n + i]);
+    dmax = std::fmax(dmax, d);
+    dmin = std::fmin(dmin, d);
+  }
+  double r = dmax / dmin;
+  return r * r;
+}
+// solve_spd: lib.rs:38-68
+void hpo_solve_spd(const double* a, const double* b, size_t n, hpo_solution* sol, double* x_dd, double* x_f64) {
+  std::memset(sol, 0, sizeof(*sol));
+  std::vector<double> a128(2 * n * n, 0.0), b128(2 * n, 0.0);
+  for (size_t i = 0; i < n * n; ++i) a128[2 * i] = a[i];
+  for (size_t i = 0; i < n; ++i) b128[2 * i] = b[i];
+  std::vector<double> l64, x64;
+  if (cholesky_f64(a, n, l64)) {
+    solve_cholesky_f64(l64, b, n, x64);
+    double kappa = cond_estimate_f64(l64, n);
+    if (kappa < 1e10) {
+      for (size_t i = 0; i < n; ++i) { x_dd[2 * i] = x64[i]; x_dd[2 * i + 1] = 0.0; x_f64[i] = x64[i]; }
+      sol->kappa = kappa;
+      sol->residual_norm = residual_norm_dd(a128.data(), x_dd, b128.data(), n);
+      sol->strategy = 0;
+      sol->refinement_iters = 0;
+      return;
+    }
+    iterative_refinement(a128.data(), b128.data(), x64, n, sol, x_dd, x_f64);
+    return;
+  }
+  solve_pure_f128(a128.data(), b128.data(), n, sol, x_dd, x_f64);
+}
+// solve_spd_f128: lib.rs:71-73
+void hpo_solve_spd_f128(const double* a_dd, const double* b_dd, size_t n, hpo_solution* sol, double* x_dd, double* x_f64) {
+  std::memset(sol, 0, sizeof(*sol));
+  solve_pure_f128(a_dd, b_dd, n, sol, x_dd, x_f64);
+}
+
+// ------------------------------------------------------------------ overlap crate
+static const double PI_F64 = 3.141592653589793;
+// overlap_primitive_f64: overlap/src/integral.rs:15-25
+double hpo_overlap_primitive_f64(double ai, const double* ci, double aj, const double* cj) {
+  double gamma = ai + aj;
+  double mu = ai * aj / gamma;
+  double dx = ci[0] - cj[0], dy = ci[1] - cj[1], dz = ci[2] - cj[2];
+  double r2 = dx * dx + dy * dy + dz * dz;  // integral.rs:354-360
+  double sq = std::sqrt(PI_F64 / gamma);
+  double prefactor = (sq * sq) * sq;         // powi(3)
+  return prefactor * std::exp(-mu * r2);
+}
+// overlap_primitive_f128: integral.rs:29-56 (f64 pi, f64 centre differences, libm exp)
+hpo_dd hpo_overlap_primitive_f128(double ai, const double* ci, double aj, const double* cj) {
+  D a = dd_from(ai), b = dd_from(aj);
+  D gamma = add(a, b);
+  D mu = div(mul(a, b), gamma);
+  D dx = dd_from(ci[0] - cj[0]), dy = dd_from(ci[1] - cj[1]), dz = dd_from(ci[2] - cj[2]);
+  D r2 = add(add(mul(dx, dx), mul(dy, dy)), mul(dz, dz));
+  D pi = dd_from(PI_F64);
+  D ratio = div(pi, gamma);
+  D prefactor = mul(ratio, dsqrt(ratio));
+  D nmr2 = mul(sub(D{0.0, 0.0}, mu), r2);
+  D exp_hi = dd_from(std::exp(nmr2.hi));
+  D exp_val = mul(exp_hi, add(D{1.0, 0.0}, dd_from(nmr2.lo)));
+  return to_c(mul(prefactor, exp_val));
+}
+// screen_shell_pair for single-primitive S shells: screen.rs:57-101
+int hpo_screen_s_pair(double ai, const double* ci, double aj, const double* cj, double diffuse_threshold,
+                      double ratio_threshold, double eps_skip, double safe_distance) {
+  double dx = ci[0] - cj[0], dy = ci[1] - cj[1], dz = ci[2] - cj[2];
+  double r2 = dx * dx + dy * dy + dz * dz;
+  double amin_a = std::fmin(INFINITY, ai), amin_b = std::fmin(INFINITY, aj);
+  double alpha_sum = amin_a + amin_b;
+  double gamma = amin_a * amin_b / alpha_sum;
+  double exponent = gamma * r2;
+  if (exponent > -std::log(eps_skip)) return 2;
+  bool ad = amin_a < diffuse_threshold, bd = amin_b < diffuse_threshold;
+  if (!ad && !bd) return 0;
+  double amax_a = std::fmax(0.0, ai), amax_b = std::fmax(0.0, aj);
+  double ratio = std::fmax(amax_a / amin_b, amax_b / amin_a);
+  double r = std::sqrt(r2);
+  if (ratio > ratio_threshold && r < safe_distance) return 1;
+  if (ad && bd && r < safe_distance) return 1;
+  return 0;
+}
+// assemble_overlap_mixed over single-primitive S shells: assemble.rs:35-85 with
+// overlap_contracted_{f64,f128} s-s branches (integral.rs:93-142)
+void hpo_assemble_overlap_s(const double* centers, const double* alpha, const double* coeff, size_t n,
+                            const double* params4, double* s, size_t* counts) {
+  std::memset(s, 0, sizeof(double) * n * n);
+  counts[0] = counts[1] = counts[2] = 0;
+  for (size_t i = 0; i < n; ++i)
+    for (size_t j = 0; j <= i; ++j) {
+      const double* ci = centers + 3 * i;
+      const double* cj = centers + 3 * j;
+      int tier = hpo_screen_s_pair(alpha[i], ci, alpha[j], cj, params4[0], params4[1], params4[2], params4[3]);
+      double v = 0.0;
+      if (tier == 0) {
+        double sp = hpo_overlap_primitive_f64(alpha[i], ci, alpha[j], cj);
+        v = coeff[i] * coeff[j] * sp;
+      } else if (tier == 1) {
+        D sp = from_c(hpo_overlap_primitive_f128(alpha[i], ci, alpha[j], cj));
+        D sum = add(D{0.0, 0.0}, mul(mul(dd_from(coeff[i]), dd_from(coeff[j])), sp));
+        v = sum.hi + sum.lo;
+      }
+      counts[tier] += 1;
+      if (tier == 2) continue;
+      s[i * n + j] = v;
+      if (i != j) s[j * n + i] = v;
+    }
+}
+
+// Normalised s-type overlap built from the reference's dd operations.  The reference's
+// dd exp (dd.rs:119) wraps the exponent field for k < -1022, so for arguments below
+// -708 the entry is returned as 0 (the true value is below 1e-307, under a dd ulp of
+// the unit diagonal by ~275 orders of magnitude).
+hpo_dd hpo_overlap_s_dd_entry(double ai, const double* ci, double aj, const double* cj) {
+  D a = dd_from(ai), b = dd_from(aj);
+  D g = add(a, b);
+  D ab = mul(a, b);
+  D p = div(ab, g);
+  D ratio = div(mul_f(ab, 4.0), mul(g, g));
+  D t = dsqrt(ratio);
+  D pref = mul(t, dsqrt(t));
+  D d[3];
+  for (int c = 0; c < 3; ++c) two_sum(ci[c], -cj[c], d[c].hi, d[c].lo);
+  D r2 = add(add(mul(d[0], d[0]), mul(d[1], d[1])), mul(d[2], d[2]));
+  D arg = neg(mul(p, r2));
+  if (arg.hi < -708.0) return hpo_dd{0.0, 0.0};
+  return to_c(mul(pref, dexp(arg)));
+}
+void hpo_overlap_s_dd(const double* centers, const double* alpha, size_t n, double* s) {
+#pragma omp parallel for schedule(dynamic, 8)
+  for (long ii = 0; ii < (long)n; ++ii) {
+    size_t i = (size_t)ii;
+    for (size_t j = 0; j <= i; ++j) {
+      hpo_dd v = hpo_overlap_s_dd_entry(alpha[i], centers + 3 * i, alpha[j], centers + 3 * j);
+      s[2 * (i * n + j)] = v.hi; s[2 * (i * n + j) + 1] = v.lo;
+      s[2 * (j * n + i)] = v.hi; s[2 * (j * n + i) + 1] = v.lo;
+    }
+  }
+}
+
+}  // extern "C"
+
+// ------------------------------------------------------------------ op counting
+namespace {
+thread_local long g_ops = 0;
+struct CD {
+  double v;
+  CD() : v(0) {}
+  CD(double x) : v(x) {}
+};
+inline CD operator+(CD a, CD b) { ++g_ops; return CD(a.v + b.v); }
+inline CD operator-(CD a, CD b) { ++g_ops; return CD(a.v - b.v); }
+inline CD operator*(CD a, CD b) { ++g_ops; return CD(a.v * b.v); }
+inline CD operator/(CD a, CD b) { ++g_ops; return CD(a.v / b.v); }
+inline CD operator-(CD a) { return CD(-a.v); }
+// found by argument-dependent lookup from the templates in hpo_core.hpp
+inline CD fma_(CD a, CD b, CD c) { ++g_ops; return CD(std::fma(a.v, b.v, c.v)); }
+inline CD sqrt_(CD a) { ++g_ops; return CD(std::sqrt(a.v)); }
+inline double val(CD a) { return a.v; }
+}  // namespace
+
+extern "C" {
+long hpo_count_dd_mac(void) {
+  DD<CD> s{CD(0.3), CD(1e-18)}, a{CD(1.1), CD(3e-17)}, b{CD(-0.7), CD(2e-17)};
+  g_ops = 0;
+  s = add(s, mul(a, b));
+  return g_ops;
+}
+long hpo_count_mf4_mac(const double* s4, const double* a4, const double* b4) {
+  MF<CD> s, a, b;
+  for (int i = 0; i < 4; ++i) { s.l[i] = CD(s4[i]); a.l[i] = CD(a4[i]); b.l[i] = CD(b4[i]); }
+  g_ops = 0;
+  s = mf_add(s, mf_mul(a, b));
+  return g_ops;
+}
+long hpo_count_overlap_entry(double ai, const double* ci, double aj, const double* cj) {
+  g_ops = 0;
+  DD<CD> a = dd_from(CD(ai)), b = dd_from(CD(aj));
+  DD<CD> g = add(a, b);
+  DD<CD> ab = mul(a, b);
+  DD<CD> p = div(ab, g);
+  DD<CD> ratio = div(mul_f(ab, CD(4.0)), mul(g, g));
+  DD<CD> t = dsqrt(ratio);
+  DD<CD> pref = mul(t, dsqrt(t));
+  DD<CD> d[3];
+  for (int c = 0; c < 3; ++c) two_sum(CD(ci[c]), CD(-cj[c]), d[c].hi, d[c].lo);
+  DD<CD> r2 = add(add(mul(d[0], d[0]), mul(d[1], d[1])), mul(d[2], d[2]));
+  DD<CD> arg = neg(mul(p, r2));
+  if (arg.hi.v < -708.0) return g_ops;
+  DD<CD> r = mul(pref, dexp(arg));
+  (void)r;
+  return g_ops;
+}
+}

@@ -1,0 +1,131 @@
+/*
+ * hpo.h -- the oracle: a CPU restatement, op for op, of the hyperprec (m0at/quadprec)
+ * reference algorithms on the extended-precision path.
+ *
+ * TEST INFRASTRUCTURE ONLY.  Only tests/, __graft_entry__.smoke() and bench.py's
+ * cpu_baseline leg may load this library, and only as the checker / CPU baseline.
+ * The product library (quadprec_amd/) never links, loads or calls it.
+ *
+ * Arithmetic contract: IEEE binary64, round to nearest, compiled with
+ * -ffp-contract=off so that no a*b+c is fused except where the reference writes
+ * mul_add (the TwoProd error term, crates/quad/src/dd.rs:33).
+ *
+ * Layout: a double-double value is {hi, lo} (crates/quad/src/dd.rs:8-11); arrays of
+ * them are interleaved (AoS), which is how a Rust &[f128] lies in memory.  A
+ * MultiFloat<4> value is 4 doubles, most significant first
+ * (crates/quad/src/multifloat/core.rs:6-8).
+ *
+ * Parity pinning (see the test files named here):
+ *   dd ops            tests/test_oracle_dd.py  (mpmath + reference unit tests)
+ *   MF4 ops           tests/test_oracle_mf4.py (mpmath + reference unit tests)
+ *   linalg / solver   tests/test_oracle_linalg.py (reference tests, README Hilbert table)
+ *   overlap           tests/test_oracle_overlap.py (mpmath closed form, reference tests)
+ */
+#ifndef HPO_H
+#define HPO_H
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct { double hi, lo; } hpo_dd;
+typedef struct { double l[4]; } hpo_mf4;
+
+/* ---- dd scalar ops (crates/quad/src/dd.rs) ---- */
+hpo_dd hpo_dd_add(hpo_dd a, hpo_dd b);          /* dd.rs:147-157 */
+hpo_dd hpo_dd_sub(hpo_dd a, hpo_dd b);          /* dd.rs:159-165 */
+hpo_dd hpo_dd_mul(hpo_dd a, hpo_dd b);          /* dd.rs:175-183 */
+hpo_dd hpo_dd_div(hpo_dd a, hpo_dd b);          /* dd.rs:185-195 */
+hpo_dd hpo_dd_add_f64(hpo_dd a, double b);      /* dd.rs:228-235 */
+hpo_dd hpo_dd_mul_f64(hpo_dd a, double b);      /* dd.rs:243-250 */
+hpo_dd hpo_dd_div_f64(hpo_dd a, double b);      /* dd.rs:252-260 */
+hpo_dd hpo_dd_sqrt(hpo_dd a);                   /* dd.rs:61-78 */
+hpo_dd hpo_dd_recip(hpo_dd a);                  /* dd.rs:80-83 */
+hpo_dd hpo_dd_exp(hpo_dd a);                    /* dd.rs:106-121 */
+hpo_dd hpo_dd_ln(hpo_dd a);                     /* dd.rs:124-137 */
+/* array forms for the Python harness: out[i] = op(a[i], b[i]) */
+void hpo_dd_binop_n(int op, const double* a, const double* b, double* out, size_t n);
+void hpo_dd_unop_n(int op, const double* a, double* out, size_t n);
+
+/* ---- MultiFloat<4> ops (crates/quad/src/multifloat) ---- */
+void hpo_mf4_renormalize(double* limbs4);                        /* core.rs:57-93 */
+void hpo_mf4_renormalize_from8(const double* in8, double* out4); /* core.rs:99-168 */
+hpo_mf4 hpo_mf4_add(hpo_mf4 a, hpo_mf4 b);   /* add.rs:19-106 */
+hpo_mf4 hpo_mf4_sub(hpo_mf4 a, hpo_mf4 b);   /* add.rs:108-114 */
+hpo_mf4 hpo_mf4_mul(hpo_mf4 a, hpo_mf4 b);   /* mul.rs:12-111 */
+hpo_mf4 hpo_mf4_mul_f64(hpo_mf4 a, double b);/* mul.rs:124-202 */
+hpo_mf4 hpo_mf4_recip(hpo_mf4 a);            /* div.rs:14-41 */
+hpo_mf4 hpo_mf4_div(hpo_mf4 a, hpo_mf4 b);   /* div.rs:44-77 */
+hpo_mf4 hpo_mf4_sqrt(hpo_mf4 a);             /* math.rs:22-40 */
+hpo_mf4 hpo_mf4_exp(hpo_mf4 a);              /* transcendental.rs:179-243 */
+void hpo_mf4_binop_n(int op, const double* a, const double* b, double* out, size_t n);
+void hpo_mf4_unop_n(int op, const double* a, double* out, size_t n);
+
+/* ---- dd linear algebra (crates/quad/src/linalg.rs) ---- */
+void hpo_gemm(const double* a, const double* b, double* c, size_t m, size_t n, size_t k);      /* 150-165 */
+void hpo_gemm_rows(const double* a, const double* b, double* c, size_t m, size_t n, size_t k,
+                   size_t row0, size_t nrows);   /* rows [row0,row0+nrows) of hpo_gemm */
+void hpo_gemm_atb(const double* a, const double* b, double* c, size_t m, size_t n, size_t k);  /* 168-182 */
+void hpo_gemm_atb_rows(const double* a, const double* b, double* c, size_t m, size_t n, size_t k,
+                       size_t row0, size_t nrows);
+/* return 0 on success, else 1 with *pivot / *value set (CholeskyError, linalg.rs:196-208) */
+int hpo_cholesky(double* a, size_t n, size_t* pivot, double* value);                          /* 42-96 */
+int hpo_cholesky_blocked(double* a, size_t n, size_t nb, size_t* pivot, double* value);       /* 238-293 */
+void hpo_forward_solve(const double* l, const double* b, double* y, size_t n);                 /* 99-108 */
+void hpo_backward_solve(const double* l, const double* y, double* x, size_t n);                /* 111-120 */
+void hpo_solve_cholesky(const double* l, const double* b, double* x, size_t n);                /* 123-126 */
+void hpo_matvec(const double* a, const double* x, double* y, size_t n);                        /* 129-147 */
+double hpo_cond_estimate(const double* l, size_t n);                                           /* 185-194 */
+
+/* ---- MF4 linear algebra (crates/quad/src/multifloat/linalg.rs) ---- */
+void hpo_mf4_gemm(const double* a, const double* b, double* c, size_t m, size_t n, size_t k);      /* 39-61 */
+void hpo_mf4_gemm_rows(const double* a, const double* b, double* c, size_t m, size_t n, size_t k,
+                       size_t row0, size_t nrows);
+void hpo_mf4_gemm_atb(const double* a, const double* b, double* c, size_t m, size_t n, size_t k);  /* 64-85 */
+int hpo_mf4_cholesky(double* a, size_t n);                                                         /* 89-141 */
+void hpo_mf4_cholesky_solve(const double* l, const double* b, double* x, size_t n);                /* 145-172 */
+void hpo_mf4_gemv(const double* a, const double* x, double* y, size_t n, size_t m);                /* 17-36 */
+
+/* ---- solver crate (crates/solver/src/lib.rs) ---- */
+typedef struct {
+  double kappa;
+  double residual_norm;
+  int strategy;            /* 0 Float64, 1 MixedPrecision, 2 Float128 (lib.rs:25-30) */
+  int refinement_iters;
+  int error;               /* 0 ok, 1 NotPositiveDefinite */
+  size_t pivot;
+} hpo_solution;
+/* x_dd: n dd values (AoS); x_f64: n doubles */
+void hpo_solve_spd(const double* a, const double* b, size_t n, hpo_solution* sol, double* x_dd, double* x_f64); /* 38-68 */
+void hpo_solve_spd_f128(const double* a_dd, const double* b_dd, size_t n, hpo_solution* sol, double* x_dd, double* x_f64); /* 71-73 */
+double hpo_norm_f128(const double* x_dd, size_t n);   /* 243-249 */
+
+/* ---- overlap crate (crates/overlap/src) ---- */
+double hpo_overlap_primitive_f64(double ai, const double* ci, double aj, const double* cj);   /* integral.rs:15-25 */
+hpo_dd hpo_overlap_primitive_f128(double ai, const double* ci, double aj, const double* cj);  /* integral.rs:29-56 */
+/* screen_shell_pair for single-primitive S shells: 0 F64, 1 F128, 2 Skip (screen.rs:57-101) */
+int hpo_screen_s_pair(double ai, const double* ci, double aj, const double* cj,
+                      double diffuse_threshold, double ratio_threshold, double eps_skip, double safe_distance);
+/* assemble_overlap_mixed over single-primitive S shells (assemble.rs:35-85); counts[3] = f64, f128, skipped */
+void hpo_assemble_overlap_s(const double* centers, const double* alpha, const double* coeff, size_t n,
+                            const double* params4, double* s_f64, size_t* counts);
+/* normalised s-type overlap in dd, built from the reference's dd ops:
+ *   S = (4 a b / (a+b)^2)^{3/4} exp(-(a b/(a+b)) |Ri-Rj|^2), Ri-Rj by TwoSum.  Full n x n, AoS dd. */
+hpo_dd hpo_overlap_s_dd_entry(double ai, const double* ci, double aj, const double* cj);
+void hpo_overlap_s_dd(const double* centers, const double* alpha, size_t n, double* s_dd);
+
+/* ---- counting: FP64 ops per unit of work, from an instrumented restatement ---- */
+/* returns the number of binary64 operations (+,-,*,/,fma,sqrt) one dd MAC s += a*b costs (29) */
+long hpo_count_dd_mac(void);
+/* number of binary64 ops of one MF4 MAC s = s + a*b on given operands (data dependent) */
+long hpo_count_mf4_mac(const double* s4, const double* a4, const double* b4);
+long hpo_count_overlap_entry(double ai, const double* ci, double aj, const double* cj);
+
+int hpo_num_threads(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,315 @@
+"""ctypes bindings of the oracle (oracle/libhpo.so) for the test suite and bench's
+CPU-baseline leg.  The oracle is the checker: a CPU restatement, op for op, of the
+hyperprec reference algorithms.  Product code never imports this module.
+
+Array conventions (numpy, float64):
+  dd arrays  : shape (..., 2)  [hi, lo]         (Rust &[f128] memory layout)
+  MF4 arrays : shape (..., 4)  limbs, most significant first
+"""
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+ORACLE_DIR = os.path.join(ROOT, "oracle")
+LIB_PATH = os.path.join(ORACLE_DIR, "libhpo.so")
+
+_lib = None
+_dp = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
+_sz = C.c_size_t
+
+
+class DD(C.Structure):
+    _fields_ = [("hi", C.c_double), ("lo", C.c_double)]
+
+
+class MF4(C.Structure):
+    _fields_ = [("l", C.c_double * 4)]
+
+
+class Solution(C.Structure):
+    _fields_ = [("kappa", C.c_double), ("residual_norm", C.c_double), ("strategy", C.c_int),
+                ("refinement_iters", C.c_int), ("error", C.c_int), ("pivot", C.c_size_t)]
+
+
+def build():
+    subprocess.run(["make", "-s", "-C", ORACLE_DIR], check=True)
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            build()
+        L = C.CDLL(LIB_PATH)
+        for name in ("hpo_dd_add", "hpo_dd_sub", "hpo_dd_mul", "hpo_dd_div"):
+            getattr(L, name).restype = DD
+            getattr(L, name).argtypes = [DD, DD]
+        for name in ("hpo_dd_add_f64", "hpo_dd_mul_f64", "hpo_dd_div_f64"):
+            getattr(L, name).restype = DD
+            getattr(L, name).argtypes = [DD, C.c_double]
+        for name in ("hpo_dd_sqrt", "hpo_dd_recip", "hpo_dd_exp", "hpo_dd_ln"):
+            getattr(L, name).restype = DD
+            getattr(L, name).argtypes = [DD]
+        L.hpo_dd_binop_n.argtypes = [C.c_int, _dp, _dp, _dp, _sz]
+        L.hpo_dd_unop_n.argtypes = [C.c_int, _dp, _dp, _sz]
+        L.hpo_mf4_binop_n.argtypes = [C.c_int, _dp, _dp, _dp, _sz]
+        L.hpo_mf4_unop_n.argtypes = [C.c_int, _dp, _dp, _sz]
+        L.hpo_mf4_renormalize.argtypes = [_dp]
+        L.hpo_mf4_renormalize_from8.argtypes = [_dp, _dp]
+        for name in ("hpo_gemm", "hpo_gemm_atb", "hpo_mf4_gemm", "hpo_mf4_gemm_atb"):
+            getattr(L, name).argtypes = [_dp, _dp, _dp, _sz, _sz, _sz]
+        for name in ("hpo_gemm_rows", "hpo_gemm_atb_rows", "hpo_mf4_gemm_rows"):
+            getattr(L, name).argtypes = [_dp, _dp, _dp, _sz, _sz, _sz, _sz, _sz]
+        L.hpo_cholesky.argtypes = [_dp, _sz, C.POINTER(C.c_size_t), C.POINTER(C.c_double)]
+        L.hpo_cholesky_blocked.argtypes = [_dp, _sz, _sz, C.POINTER(C.c_size_t), C.POINTER(C.c_double)]
+        for name in ("hpo_forward_solve", "hpo_backward_solve", "hpo_solve_cholesky", "hpo_matvec",
+                     "hpo_mf4_cholesky_solve"):
+            getattr(L, name).argtypes = [_dp, _dp, _dp, _sz]
+        L.hpo_cond_estimate.argtypes = [_dp, _sz]
+        L.hpo_cond_estimate.restype = C.c_double
+        L.hpo_mf4_cholesky.argtypes = [_dp, _sz]
+        L.hpo_mf4_gemv.argtypes = [_dp, _dp, _dp, _sz, _sz]
+        for name in ("hpo_solve_spd", "hpo_solve_spd_f128"):
+            getattr(L, name).argtypes = [_dp, _dp, _sz, C.POINTER(Solution), _dp, _dp]
+        L.hpo_norm_f128.argtypes = [_dp, _sz]
+        L.hpo_norm_f128.restype = C.c_double
+        L.hpo_overlap_primitive_f64.argtypes = [C.c_double, _dp, C.c_double, _dp]
+        L.hpo_overlap_primitive_f64.restype = C.c_double
+        L.hpo_overlap_primitive_f128.argtypes = [C.c_double, _dp, C.c_double, _dp]
+        L.hpo_overlap_primitive_f128.restype = DD
+        L.hpo_screen_s_pair.argtypes = [C.c_double, _dp, C.c_double, _dp] + [C.c_double] * 4
+        L.hpo_screen_s_pair.restype = C.c_int
+        L.hpo_assemble_overlap_s.argtypes = [_dp, _dp, _dp, _sz, _dp, _dp,
+                                             np.ctypeslib.ndpointer(dtype=np.uintp, flags="C_CONTIGUOUS")]
+        L.hpo_overlap_s_dd_entry.argtypes = [C.c_double, _dp, C.c_double, _dp]
+        L.hpo_overlap_s_dd_entry.restype = DD
+        L.hpo_overlap_s_dd.argtypes = [_dp, _dp, _sz, _dp]
+        L.hpo_count_dd_mac.restype = C.c_long
+        L.hpo_count_mf4_mac.argtypes = [_dp, _dp, _dp]
+        L.hpo_count_mf4_mac.restype = C.c_long
+        L.hpo_count_overlap_entry.argtypes = [C.c_double, _dp, C.c_double, _dp]
+        L.hpo_count_overlap_entry.restype = C.c_long
+        L.hpo_num_threads.restype = C.c_int
+        _lib = L
+    return _lib
+
+
+def _c(x):
+    return np.ascontiguousarray(x, dtype=np.float64)
+
+
+# ---------------------------------------------------------------- dd scalar / array ops
+_BIN = {"add": 0, "sub": 1, "mul": 2, "div": 3, "add_f64": 4, "mul_f64": 5, "div_f64": 6}
+_UN = {"sqrt": 0, "recip": 1, "exp": 2, "ln": 3, "neg": 4, "abs": 5}
+
+
+def dd_binop(op, a, b):
+    a, b = _c(a), _c(b)
+    a, b = np.broadcast_arrays(a, b)
+    a, b = _c(a), _c(b)
+    out = np.empty_like(a)
+    lib().hpo_dd_binop_n(_BIN[op], a, b, out, a.size // 2)
+    return out
+
+
+def dd_unop(op, a):
+    a = _c(a)
+    out = np.empty_like(a)
+    lib().hpo_dd_unop_n(_UN[op], a, out, a.size // 2)
+    return out
+
+
+_MBIN = {"add": 0, "sub": 1, "mul": 2, "div": 3, "mul_f64": 4}
+_MUN = {"sqrt": 0, "recip": 1, "exp": 2, "renormalize": 3}
+
+
+def mf4_binop(op, a, b):
+    a, b = np.broadcast_arrays(_c(a), _c(b))
+    a, b = _c(a), _c(b)
+    out = np.empty_like(a)
+    lib().hpo_mf4_binop_n(_MBIN[op], a, b, out, a.size // 4)
+    return out
+
+
+def mf4_unop(op, a):
+    a = _c(a)
+    out = np.empty_like(a)
+    lib().hpo_mf4_unop_n(_MUN[op], a, out, a.size // 4)
+    return out
+
+
+# ---------------------------------------------------------------- linalg
+def gemm(a, b, m, n, k):
+    c = np.zeros((m * n, 2))
+    lib().hpo_gemm(_c(a), _c(b), c, m, n, k)
+    return c
+
+
+def gemm_rows(a, b, m, n, k, row0, nrows):
+    c = np.zeros((m * n, 2))
+    lib().hpo_gemm_rows(_c(a), _c(b), c, m, n, k, row0, nrows)
+    return c
+
+
+def gemm_atb(a, b, m, n, k):
+    c = np.zeros((m * n, 2))
+    lib().hpo_gemm_atb(_c(a), _c(b), c, m, n, k)
+    return c
+
+
+def gemm_atb_rows(a, b, m, n, k, row0, nrows):
+    c = np.zeros((m * n, 2))
+    lib().hpo_gemm_atb_rows(_c(a), _c(b), c, m, n, k, row0, nrows)
+    return c
+
+
+def mf4_gemm(a, b, m, n, k):
+    c = np.zeros((m * n, 4))
+    lib().hpo_mf4_gemm(_c(a), _c(b), c, m, n, k)
+    return c
+
+
+def mf4_gemm_rows(a, b, m, n, k, row0, nrows):
+    c = np.zeros((m * n, 4))
+    lib().hpo_mf4_gemm_rows(_c(a), _c(b), c, m, n, k, row0, nrows)
+    return c
+
+
+def mf4_gemm_atb(a, b, m, n, k):
+    c = np.zeros((m * n, 4))
+    lib().hpo_mf4_gemm_atb(_c(a), _c(b), c, m, n, k)
+    return c
+
+
+class CholeskyError(Exception):
+    def __init__(self, pivot, value):
+        super().__init__(f"Cholesky failed at pivot {pivot} (value={value:.4e})")
+        self.pivot, self.value = pivot, value
+
+
+def cholesky(a, n):
+    """Returns the factored copy; raises CholeskyError like linalg.rs:54-57."""
+    a = _c(a).copy()
+    pv, vv = C.c_size_t(0), C.c_double(0)
+    if lib().hpo_cholesky(a, n, C.byref(pv), C.byref(vv)):
+        raise CholeskyError(pv.value, vv.value)
+    return a
+
+
+def cholesky_blocked(a, n, nb):
+    a = _c(a).copy()
+    pv, vv = C.c_size_t(0), C.c_double(0)
+    if lib().hpo_cholesky_blocked(a, n, nb, C.byref(pv), C.byref(vv)):
+        raise CholeskyError(pv.value, vv.value)
+    return a
+
+
+def _vec_op(name, l, b, n):
+    out = np.zeros((n, 2))
+    getattr(lib(), name)(_c(l), _c(b), out, n)
+    return out
+
+
+def forward_solve(l, b, n):
+    return _vec_op("hpo_forward_solve", l, b, n)
+
+
+def backward_solve(l, y, n):
+    return _vec_op("hpo_backward_solve", l, y, n)
+
+
+def solve_cholesky(l, b, n):
+    return _vec_op("hpo_solve_cholesky", l, b, n)
+
+
+def matvec(a, x, n):
+    return _vec_op("hpo_matvec", a, x, n)
+
+
+def cond_estimate(l, n):
+    return lib().hpo_cond_estimate(_c(l), n)
+
+
+def mf4_cholesky(a, n):
+    a = _c(a).copy()
+    if lib().hpo_mf4_cholesky(a, n):
+        raise ValueError("Cholesky failed: matrix is not positive definite")
+    return a
+
+
+def mf4_cholesky_solve(l, b, n):
+    out = np.zeros((n, 4))
+    lib().hpo_mf4_cholesky_solve(_c(l), _c(b), out, n)
+    return out
+
+
+def mf4_gemv(a, x, n, m):
+    out = np.zeros((n, 4))
+    lib().hpo_mf4_gemv(_c(a), _c(x), out, n, m)
+    return out
+
+
+def _solve(name, a, b, n):
+    sol = Solution()
+    x = np.zeros((n, 2))
+    xf = np.zeros(n)
+    getattr(lib(), name)(_c(a), _c(b), n, C.byref(sol), x, xf)
+    return sol, x, xf
+
+
+def solve_spd(a, b, n):
+    return _solve("hpo_solve_spd", a, b, n)
+
+
+def solve_spd_f128(a, b, n):
+    return _solve("hpo_solve_spd_f128", a, b, n)
+
+
+def norm_f128(x):
+    x = _c(x)
+    return lib().hpo_norm_f128(x, x.size // 2)
+
+
+# ---------------------------------------------------------------- overlap
+def overlap_s_dd(centers, alpha):
+    n = len(alpha)
+    s = np.zeros((n * n, 2))
+    lib().hpo_overlap_s_dd(_c(centers), _c(alpha), n, s)
+    return s
+
+
+def overlap_s_dd_entry(ai, ci, aj, cj):
+    r = lib().hpo_overlap_s_dd_entry(ai, _c(ci), aj, _c(cj))
+    return np.array([r.hi, r.lo])
+
+
+def overlap_primitive_f64(ai, ci, aj, cj):
+    return lib().hpo_overlap_primitive_f64(ai, _c(ci), aj, _c(cj))
+
+
+def overlap_primitive_f128(ai, ci, aj, cj):
+    r = lib().hpo_overlap_primitive_f128(ai, _c(ci), aj, _c(cj))
+    return np.array([r.hi, r.lo])
+
+
+def assemble_overlap_s(centers, alpha, coeff, params=(0.1, 500.0, 1e-14, 12.0)):
+    n = len(alpha)
+    s = np.zeros(n * n)
+    counts = np.zeros(3, dtype=np.uintp)
+    lib().hpo_assemble_overlap_s(_c(centers), _c(alpha), _c(coeff), n, _c(params), s, counts)
+    return s, counts
+
+
+def count_dd_mac():
+    return lib().hpo_count_dd_mac()
+
+
+def count_mf4_mac(s, a, b):
+    return lib().hpo_count_mf4_mac(_c(s), _c(a), _c(b))
+
+
+def count_overlap_entry(ai, ci, aj, cj):
+    return lib().hpo_count_overlap_entry(ai, _c(ci), aj, _c(cj))
