@@ -1,0 +1,93 @@
+/*
+ * hyperprec_hip.h -- C ABI of the MI355X (gfx950) implementation of hyperprec's
+ * extended-precision path: double-double (f128) and MultiFloat<4> (f256) GEMM, the
+ * blocked Cholesky factorisation with its triangular solves, the solver crate's SPD
+ * solves and the Gaussian overlap-matrix build.
+ *
+ * Plain pointers and sizes only.  Two families of entry points:
+ *
+ *   hp_*      drop-in replacements for the reference's Rust functions.  Arrays are host
+ *             memory in the reference's layout: a dd value is {hi, lo} (two doubles,
+ *             crates/quad/src/dd.rs:8-11), an MF4 value is 4 doubles most significant
+ *             first (crates/quad/src/multifloat/core.rs:6-8), matrices are row-major.
+ *             Each call stages the operands through HBM on the calling thread's default
+ *             stream and returns when the results are back in host memory.
+ *
+ *   hp_dev_*  the same operations on operands already resident in HBM, as
+ *             structure-of-arrays planes (dd: a hi plane and a lo plane; MF4: four limb
+ *             planes, plane l of an r x c matrix at offset l*r*c), enqueued on the
+ *             caller's HIP stream (`stream` is a hipStream_t; NULL is the null stream).
+ *
+ * Every function returns a status code (HP_OK on success).  On failure
+ * hp_last_error() returns a message for the calling thread.  Shape errors, which the
+ * reference turns into assert_eq! panics (crates/quad/src/linalg.rs:151-153), are
+ * HP_ESHAPE; a Cholesky breakdown, which the reference reports as
+ * CholeskyError{pivot, value} (linalg.rs:196-208), is HP_ENOTSPD with the pivot and the
+ * f64 value of the failing diagonal written through the out-parameters.
+ *
+ * All functions are thread-safe; they keep no state between calls other than HIP's.
+ */
+#ifndef HYPERPREC_HIP_H
+#define HYPERPREC_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum hp_status {
+  HP_OK = 0,
+  HP_ESHAPE = 1,   /* inconsistent sizes (reference: assert_eq! panic) */
+  HP_ENOTSPD = 2,  /* Cholesky breakdown (reference: CholeskyError / SolveError) */
+  HP_EDEVICE = 3,  /* HIP runtime error or no usable gfx950 device */
+  HP_EINVAL = 4,   /* invalid argument (null pointer, unknown mode) */
+  HP_ENOMEM = 5    /* device allocation failed */
+};
+
+/* Message describing the last failure on the calling thread ("" if none). */
+const char* hp_last_error(void);
+/* Library ABI version (major*100 + minor). */
+int hp_version(void);
+
+/* ------------------------------------------------------------------ GEMM */
+
+/* C = A*B, A m x k, B k x n, C m x n, dd, row-major.
+ * Replaces gemm (crates/quad/src/linalg.rs:150-165).  Bit-identical to the reference. */
+int hp_gemm(const double* a, const double* b, double* c, size_t m, size_t n, size_t k);
+
+/* C = A^T * B with A stored k x m.  Replaces gemm_atb (linalg.rs:168-182). */
+int hp_gemm_atb(const double* a, const double* b, double* c, size_t m, size_t n, size_t k);
+
+/* MultiFloat<4> C = A*B.  Replaces mf_gemm::<4> (crates/quad/src/multifloat/linalg.rs:39-61). */
+int hp_mf_gemm(const double* a, const double* b, double* c, size_t m, size_t n, size_t k);
+
+/* MultiFloat<4> C = A^T * B, A stored k x m.  Replaces mf_gemm_atb::<4> (multifloat/linalg.rs:64-85). */
+int hp_mf_gemm_atb(const double* a, const double* b, double* c, size_t m, size_t n, size_t k);
+
+/* Device dd GEMM on SoA planes.  trans_a = 0: A is m x k (gemm); 1: A is k x m (gemm_atb).
+ * mode = HP_GEMM_PLAIN is the reference's per-element order (bit-identical). */
+enum hp_gemm_mode {
+  HP_GEMM_PLAIN = 0
+};
+int hp_dev_gemm(int trans_a, const double* a_hi, const double* a_lo, const double* b_hi, const double* b_lo,
+                double* c_hi, double* c_lo, size_t m, size_t n, size_t k, int mode, int slices, void* stream);
+
+/* Device MF4 GEMM on limb-major planes (see above). */
+int hp_dev_mf_gemm(int trans_a, const double* a, const double* b, double* c, size_t m, size_t n, size_t k,
+                   void* stream);
+
+/* ------------------------------------------------------------------ synthetic inputs */
+
+/* Deterministic workload generators (host memory).  SplitMix64 stream from `seed`;
+ * N(0,1) by Box-Muller.  dd: hi ~ N(0,1), lo = hi * U(-2^-53, 2^-53), then renormalised
+ * with quick_two_sum.  MF4: limb0 ~ N(0,1), limb i+1 = limb i * U(-2^-53, 2^-53), then
+ * renormalised (multifloat/core.rs:57-93).  Outputs are AoS like the hp_* entry points. */
+void hp_gen_dd_normal(uint64_t seed, size_t count, double* out_aos);
+void hp_gen_mf4_normal(uint64_t seed, size_t count, double* out_aos);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* HYPERPREC_HIP_H */

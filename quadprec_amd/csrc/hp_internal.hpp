@@ -1,0 +1,70 @@
+// hp_internal.hpp -- shared host-side plumbing of the HIP library: status codes, the
+// thread-local error message, HIP error checks and device scratch buffers.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstddef>
+#include <cstdio>
+#include <string>
+
+#include "../../include/hyperprec_hip.h"
+
+namespace hp {
+
+void set_error(const char* fmt, ...);
+
+#define HP_HIP(call)                                                                  \
+  do {                                                                                \
+    hipError_t _e = (call);                                                           \
+    if (_e != hipSuccess) {                                                           \
+      ::hp::set_error("%s failed: %s (%s:%d)", #call, hipGetErrorString(_e), __FILE__, \
+                      __LINE__);                                                      \
+      return HP_EDEVICE;                                                              \
+    }                                                                                 \
+  } while (0)
+
+#define HP_LAUNCH_CHECK()                                                             \
+  do {                                                                                \
+    hipError_t _e = hipGetLastError();                                                \
+    if (_e != hipSuccess) {                                                           \
+      ::hp::set_error("kernel launch failed: %s (%s:%d)", hipGetErrorString(_e),       \
+                      __FILE__, __LINE__);                                            \
+      return HP_EDEVICE;                                                              \
+    }                                                                                 \
+  } while (0)
+
+// RAII device buffer used by the host (drop-in) entry points.
+struct DevBuf {
+  void* p = nullptr;
+  size_t bytes = 0;
+  DevBuf() = default;
+  DevBuf(const DevBuf&) = delete;
+  DevBuf& operator=(const DevBuf&) = delete;
+  ~DevBuf() { if (p) (void)hipFree(p); }
+  int alloc(size_t nbytes) {
+    if (nbytes == 0) nbytes = 16;
+    hipError_t e = hipMalloc(&p, nbytes);
+    if (e != hipSuccess) {
+      p = nullptr;
+      set_error("hipMalloc(%zu) failed: %s", nbytes, hipGetErrorString(e));
+      return HP_ENOMEM;
+    }
+    bytes = nbytes;
+    return HP_OK;
+  }
+  double* d() const { return static_cast<double*>(p); }
+};
+
+// AoS <-> SoA conversion for the host entry points (the host array is a Rust &[f128]
+// layout: hi, lo interleaved).  `limbs` is 2 for dd, 4 for MF4.
+int upload_planes(const double* host_aos, size_t count, int limbs, double* dev_planes, hipStream_t s);
+int download_planes(const double* dev_planes, size_t count, int limbs, double* host_aos, hipStream_t s);
+
+// kernels' launchers (device pointers, SoA planes)
+int launch_gemm_dd_plain(int trans_a, const double* ahi, const double* alo, const double* bhi,
+                         const double* blo, double* chi, double* clo, size_t m, size_t n, size_t k,
+                         hipStream_t s);
+int launch_mf4_gemm(int trans_a, const double* a, const double* b, double* c, size_t m, size_t n,
+                    size_t k, hipStream_t s);
+
+}  // namespace hp
