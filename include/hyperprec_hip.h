@@ -78,6 +78,68 @@ int hp_dev_gemm(int trans_a, const double* a_hi, const double* a_lo, const doubl
 int hp_dev_mf_gemm(int trans_a, const double* a, const double* b, double* c, size_t m, size_t n, size_t k,
                    void* stream);
 
+/* ------------------------------------------------------------------ Cholesky and solves */
+
+/* In-place unblocked Cholesky A = L L^T of an n x n dd matrix (row-major, AoS).
+ * Replaces cholesky (crates/quad/src/linalg.rs:42-96): only the lower triangle is
+ * written, the upper keeps A.  Bit-identical to the reference.  On a non-positive pivot
+ * returns HP_ENOTSPD with *pivot / *value = CholeskyError{pivot, value}. */
+int hp_cholesky(double* a, size_t n, size_t* pivot, double* value);
+
+/* Blocked Cholesky with block size nb (0 -> 64).  Replaces cholesky_blocked
+ * (linalg.rs:238-293); bit-identical to the reference for the same nb. */
+int hp_cholesky_blocked(double* a, size_t n, size_t nb, size_t* pivot, double* value);
+
+/* Device form on dd planes: unblocked != 0 selects `cholesky`, else `cholesky_blocked(nb)`.
+ * Synchronises `stream` to read the pivot status. */
+int hp_dev_cholesky(double* a_hi, double* a_lo, size_t n, size_t nb, int unblocked, size_t* pivot, double* value,
+                    void* stream);
+
+/* L y = b, L^T x = y and both (linalg.rs:99-126); L is the dd factor (lower triangle),
+ * b / y / x are n dd values.  Reference operation order (bit-identical). */
+int hp_forward_solve(const double* l, const double* b, double* y, size_t n);
+int hp_backward_solve(const double* l, const double* y, double* x, size_t n);
+int hp_solve_cholesky(const double* l, const double* b, double* x, size_t n);
+
+/* y = A x in dd (linalg.rs:129-147), reference order. */
+int hp_matvec(const double* a, const double* x, double* y, size_t n);
+
+/* (max|L_ii| / min|L_ii|)^2 of a dd factor (linalg.rs:185-194). */
+double hp_cond_estimate(const double* l, size_t n);
+
+/* ------------------------------------------------------------------ SPD solver */
+
+/* Strategy (crates/solver/src/lib.rs:25-30) */
+enum hp_strategy { HP_STRATEGY_FLOAT64 = 0, HP_STRATEGY_MIXED = 1, HP_STRATEGY_FLOAT128 = 2 };
+
+/* Solution{kappa, residual_norm, strategy, refinement_iters} (lib.rs:10-23); x and x_f64
+ * are returned through the x_dd (n dd values) and x_f64 (n doubles) arguments.  On
+ * SolveError::NotPositiveDefinite{pivot} (lib.rs:251-265) the call returns HP_ENOTSPD
+ * and `pivot` is set. */
+typedef struct {
+  double kappa;
+  double residual_norm;
+  int strategy;
+  int refinement_iters;
+  size_t pivot;
+} hp_solution;
+
+/* solve_spd (lib.rs:38-68): f64 A and b; f64 Cholesky, then f64 / mixed / f128 strategy. */
+int hp_solve_spd(const double* a, const double* b, size_t n, hp_solution* sol, double* x_dd, double* x_f64);
+
+/* solve_spd_f128 (lib.rs:71-73): dd A and b, unblocked dd Cholesky and solves, no refinement. */
+int hp_solve_spd_f128(const double* a_dd, const double* b_dd, size_t n, hp_solution* sol, double* x_dd,
+                      double* x_f64);
+
+/* ------------------------------------------------------------------ overlap build */
+
+/* Normalised s-type Gaussian overlap in dd: S_uv = (4 a_u a_v/(a_u+a_v)^2)^{3/4}
+ * exp(-(a_u a_v/(a_u+a_v)) |R_u - R_v|^2), the single-primitive s-s case of
+ * assemble_overlap_mixed (crates/overlap/src/assemble.rs:35-85, integral.rs:15-142) built
+ * in dd without screening.  centers: n x 3 (x, y, z), alpha: n, s_dd: n x n dd (AoS). */
+int hp_overlap_s(const double* centers, const double* alpha, size_t n, double* s_dd);
+int hp_dev_overlap_s(const double* centers, const double* alpha, size_t n, double* s_hi, double* s_lo, void* stream);
+
 /* ------------------------------------------------------------------ synthetic inputs */
 
 /* Deterministic workload generators (host memory).  SplitMix64 stream from `seed`;
@@ -86,6 +148,11 @@ int hp_dev_mf_gemm(int trans_a, const double* a, const double* b, double* c, siz
  * renormalised (multifloat/core.rs:57-93).  Outputs are AoS like the hp_* entry points. */
 void hp_gen_dd_normal(uint64_t seed, size_t count, double* out_aos);
 void hp_gen_mf4_normal(uint64_t seed, size_t count, double* out_aos);
+/* Gaussian basis: the first n points (lexicographic, z fastest) of an L^3 lattice,
+ * L = ceil(n^{1/3}), with `spacing` bohr plus U(-jitter, jitter) per coordinate, and
+ * exponents 10^U(log10 alpha_lo, log10 alpha_hi).  centers: n x 3, alpha: n. */
+void hp_gen_overlap_basis(uint64_t seed, size_t n, double alpha_lo, double alpha_hi, double spacing, double jitter,
+                          double* centers, double* alpha);
 
 #ifdef __cplusplus
 }

@@ -34,6 +34,26 @@ class HyperprecError(RuntimeError):
     pass
 
 
+class Solution(C.Structure):
+    """Solution{kappa, residual_norm, strategy, refinement_iters} (crates/solver/src/lib.rs:10-23);
+    x and x_f64 are attached by solve_spd / solve_spd_f128."""
+    _fields_ = [("kappa", C.c_double), ("residual_norm", C.c_double), ("strategy", C.c_int),
+                ("refinement_iters", C.c_int), ("pivot", C.c_size_t)]
+
+
+class Strategy:
+    """crates/solver/src/lib.rs:25-30"""
+    Float64, MixedPrecision, Float128 = 0, 1, 2
+
+
+class SolveError(HyperprecError):
+    """SolveError::NotPositiveDefinite{pivot} (crates/solver/src/lib.rs:251-265)."""
+
+    def __init__(self, pivot):
+        super().__init__(f"Matrix is not positive definite (failed at pivot {pivot})")
+        self.pivot = pivot
+
+
 class ShapeError(HyperprecError, AssertionError):
     """The reference panics via assert_eq! on inconsistent sizes (linalg.rs:151-153)."""
 
@@ -61,6 +81,19 @@ def lib():
         getattr(L, name).argtypes = [_dp, _dp, _dp, _sz, _sz, _sz]
     L.hp_dev_gemm.argtypes = [C.c_int, _vp, _vp, _vp, _vp, _vp, _vp, _sz, _sz, _sz, C.c_int, C.c_int, _vp]
     L.hp_dev_mf_gemm.argtypes = [C.c_int, _vp, _vp, _vp, _sz, _sz, _sz, _vp]
+    L.hp_cholesky.argtypes = [_dp, _sz, C.POINTER(C.c_size_t), C.POINTER(C.c_double)]
+    L.hp_cholesky_blocked.argtypes = [_dp, _sz, _sz, C.POINTER(C.c_size_t), C.POINTER(C.c_double)]
+    L.hp_dev_cholesky.argtypes = [_vp, _vp, _sz, _sz, C.c_int, C.POINTER(C.c_size_t), C.POINTER(C.c_double), _vp]
+    for name in ("hp_forward_solve", "hp_backward_solve", "hp_solve_cholesky", "hp_matvec"):
+        getattr(L, name).argtypes = [_dp, _dp, _dp, _sz]
+    L.hp_cond_estimate.argtypes = [_dp, _sz]
+    L.hp_cond_estimate.restype = C.c_double
+    for name in ("hp_solve_spd", "hp_solve_spd_f128"):
+        getattr(L, name).argtypes = [_dp, _dp, _sz, C.POINTER(Solution), _dp, _dp]
+    L.hp_overlap_s.argtypes = [_dp, _dp, _sz, _dp]
+    L.hp_dev_overlap_s.argtypes = [_vp, _vp, _sz, _vp, _vp, _vp]
+    L.hp_gen_overlap_basis.argtypes = [C.c_uint64, _sz, C.c_double, C.c_double, C.c_double, C.c_double, _dp, _dp]
+    L.hp_gen_overlap_basis.restype = None
     L.hp_gen_dd_normal.argtypes = [C.c_uint64, _sz, _dp]
     L.hp_gen_dd_normal.restype = None
     L.hp_gen_mf4_normal.argtypes = [C.c_uint64, _sz, _dp]
@@ -120,6 +153,106 @@ def mf_gemm_atb(a, b, m, n, k, out=None):
     c = np.zeros((m * n, 4)) if out is None else out
     _check(lib().hp_mf_gemm_atb(a, b, c, m, n, k), "mf_gemm_atb")
     return c
+
+
+# ---------------------------------------------------------------- Cholesky and solves
+def _chol(fn, a, n, *extra):
+    a = _arr(a, 2, n * n, "a").copy()
+    pv, vv = C.c_size_t(0), C.c_double(0.0)
+    rc = fn(a, n, *extra, C.byref(pv), C.byref(vv))
+    if rc == HP_ENOTSPD:
+        raise CholeskyError(pv.value, vv.value)
+    _check(rc, "cholesky")
+    return a
+
+
+def cholesky(a, n):
+    """Unblocked dd Cholesky (crates/quad/src/linalg.rs:42-96); returns the factored copy
+    (lower triangle = L, upper keeps A).  Raises CholeskyError like the reference."""
+    return _chol(lib().hp_cholesky, a, n)
+
+
+def cholesky_blocked(a, n, nb=0):
+    """Blocked dd Cholesky (linalg.rs:238-293), block size nb (0 -> 64)."""
+    return _chol(lib().hp_cholesky_blocked, a, n, nb)
+
+
+def _vec(name, l, b, n):
+    l = _arr(l, 2, n * n, "l")
+    b = _arr(b, 2, n, "b")
+    out = np.zeros((n, 2))
+    _check(getattr(lib(), name)(l, b, out, n), name)
+    return out
+
+
+def forward_solve(l, b, n):
+    """L y = b (linalg.rs:99-108)."""
+    return _vec("hp_forward_solve", l, b, n)
+
+
+def backward_solve(l, y, n):
+    """L^T x = y (linalg.rs:111-120)."""
+    return _vec("hp_backward_solve", l, y, n)
+
+
+def solve_cholesky(l, b, n):
+    """L L^T x = b (linalg.rs:123-126)."""
+    return _vec("hp_solve_cholesky", l, b, n)
+
+
+def matvec(a, x, n):
+    """y = A x (linalg.rs:129-147)."""
+    return _vec("hp_matvec", a, x, n)
+
+
+def cond_estimate(l, n):
+    """(max|L_ii|/min|L_ii|)^2 (linalg.rs:185-194)."""
+    return lib().hp_cond_estimate(_arr(l, 2, n * n, "l"), n)
+
+
+def _solve(name, a, b, n, limbs):
+    a = _arr(a, limbs, n * n, "a")
+    b = _arr(b, limbs, n, "b")
+    sol = Solution()
+    x = np.zeros((n, 2))
+    xf = np.zeros(n)
+    rc = getattr(lib(), name)(a, b, n, C.byref(sol), x, xf)
+    if rc == HP_ENOTSPD:
+        raise SolveError(sol.pivot)
+    _check(rc, name)
+    sol.x = x
+    sol.x_f64 = xf
+    return sol
+
+
+def solve_spd(a, b, n):
+    """solve_spd (crates/solver/src/lib.rs:38-68): f64 A (n*n) and b (n)."""
+    return _solve("hp_solve_spd", a, b, n, 1)
+
+
+def solve_spd_f128(a, b, n):
+    """solve_spd_f128 (lib.rs:71-73): dd A and b."""
+    return _solve("hp_solve_spd_f128", a, b, n, 2)
+
+
+# ---------------------------------------------------------------- overlap
+def overlap_s(centers, alpha):
+    """Normalised s-type Gaussian overlap matrix in dd, (n*n, 2)."""
+    alpha = np.ascontiguousarray(alpha, dtype=np.float64)
+    n = alpha.size
+    centers = _arr(centers, 3, n, "centers")
+    s = np.zeros((n * n, 2))
+    _check(lib().hp_overlap_s(centers, alpha, n, s), "overlap_s")
+    return s
+
+
+def gen_overlap_basis(seed, n, alpha_lo=0.01, alpha_hi=10.0, spacing=1.4, jitter=0.05):
+    """Lattice Gaussians: first n points of an L^3 grid (L = ceil(n^(1/3))), spacing in
+    bohr plus U(+-jitter), exponents log-uniform in [alpha_lo, alpha_hi]."""
+    c = np.empty((n, 3))
+    a = np.empty(n)
+    lib().hp_gen_overlap_basis(seed, n, alpha_lo, alpha_hi, spacing, jitter, c, a)
+    return c, a
 
 
 # ---------------------------------------------------------------- generators

@@ -181,4 +181,18 @@ void hp_gen_mf4_normal(uint64_t seed, size_t count, double* out) {
   }
 }
 
+void hp_gen_overlap_basis(uint64_t seed, size_t n, double alpha_lo, double alpha_hi, double spacing, double jitter,
+                          double* centers, double* alpha) {
+  SplitMix64 u{seed};
+  size_t L = 1;
+  while (L * L * L < n) ++L;
+  for (size_t i = 0; i < n; ++i) {
+    size_t ix = i / (L * L), iy = (i / L) % L, iz = i % L;
+    size_t g[3] = {ix, iy, iz};
+    for (int c = 0; c < 3; ++c) centers[3 * i + c] = (double)g[c] * spacing + (2.0 * u.uniform() - 1.0) * jitter;
+  }
+  const double l0 = std::log10(alpha_lo), l1 = std::log10(alpha_hi);
+  for (size_t i = 0; i < n; ++i) alpha[i] = std::pow(10.0, l0 + (l1 - l0) * u.uniform());
+}
+
 }  // extern "C"

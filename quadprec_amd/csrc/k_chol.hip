@@ -1,0 +1,550 @@
+// k_chol.hip -- reference-order (bit-exact) Cholesky factorisations, triangular solves and
+// matvec for double-double and binary64.
+//
+// The reference's unblocked Cholesky (crates/quad/src/linalg.rs:42-96) computes every
+// L[i][c] as (a[i][c] - s) * inv(L[c][c]) with s = sum_{k<c} L[i][k]*L[c][k] accumulated in
+// increasing k from zero.  This file runs the same arithmetic right-looking: each (i, c)
+// keeps its partial sum S[i][c] in HBM, and once column k is final every S[i][c], c > k,
+// receives L[i][k]*L[c][k] -- still in increasing k, still one dd add per term -- so every
+// output bit equals the reference's, while the O(n^3) part runs as parallel tiles:
+//
+//   k_diag_factor   a <=64-wide diagonal block, column by column, in one workgroup (LDS)
+//   k_row_recur     the rows below a 64-wide panel: one wavefront per row, one lane per
+//                   panel column, the finished entry broadcast with v_readlane each step
+//   k_rank_update   the trailing lower triangle: 64x64 register-blocked tiles, the panel
+//                   staged through LDS, per element the k-ordered dd MAC chain
+//
+// cholesky_blocked (linalg.rs:238-293) uses the same kernels: its diagonal block is the
+// unblocked algorithm on the block, its trsm_right_lt (linalg.rs:224-235) is k_row_recur
+// in SUB mode (s = b; s -= b_ik*l_jk; b = s*recip(l_jj)), and its trailing update is
+// k_rank_update in SUMSUB mode (s = sum from zero; a -= s).
+//
+// Binary64 instantiations give the solver crate's cholesky_f64 / solve_cholesky_f64
+// (crates/solver/src/lib.rs:169-213).
+#include "hp_chol.hpp"
+#include "hp_internal.hpp"
+
+namespace hp {
+namespace {
+
+constexpr int DB = 64;  // diagonal block / panel width handled by one workgroup
+
+__device__ __forceinline__ double rl(double v, int lane) {  // v_readlane of a double
+  long long b = __builtin_bit_cast(long long, v);
+  int lo = __builtin_amdgcn_readlane((int)(b & 0xffffffffll), lane);
+  int hi = __builtin_amdgcn_readlane((int)(b >> 32), lane);
+  return __builtin_bit_cast(double, ((long long)hi << 32) | (unsigned int)lo);
+}
+__device__ __forceinline__ dd rl(dd v, int lane) { return mkdd(rl(v.hi, lane), rl(v.lo, lane)); }
+
+template <class A>
+struct Lds2 {  // a 64x65 tile of T in LDS as hi / lo planes
+  double h[DB][DB + 1];
+  double l[A::LIMBS == 2 ? DB : 1][DB + 1];
+};
+template <class A>
+__device__ __forceinline__ typename A::T lget(Lds2<A>& s, int i, int j) {
+  if constexpr (A::LIMBS == 2) return mkdd(s.h[i][j], s.l[i][j]);
+  else return s.h[i][j];
+}
+template <class A>
+__device__ __forceinline__ void lset(Lds2<A>& s, int i, int j, typename A::T v) {
+  if constexpr (A::LIMBS == 2) { s.h[i][j] = v.hi; s.l[i][j] = v.lo; }
+  else s.h[i][j] = v;
+}
+
+// ---------------------------------------------------------------- diagonal block
+// Factor a[r0:r0+jb, r0:r0+jb] (jb <= 64) with the unblocked algorithm.  S_INIT: the
+// partial sums start from S (unblocked emulation across panels) instead of zero.  On
+// success the lower triangle is written back and inv[j] = recip(L[j][j]); on a
+// breakdown the block is left as it was and err records the pivot (linalg.rs:54-57).
+template <class A, bool S_INIT>
+__global__ __launch_bounds__(256) void k_diag_factor(Mat a, Mat S, long r0, int jb, ErrRec* err, double* inv_hi,
+                                                     double* inv_lo) {
+  using T = typename A::T;
+  extern __shared__ __align__(16) unsigned char smem[];
+  Lds2<A>& L = *reinterpret_cast<Lds2<A>*>(smem);
+  Lds2<A>& Sa = *reinterpret_cast<Lds2<A>*>(smem + sizeof(Lds2<A>));
+  __shared__ int fail;
+  __shared__ double invs[2];
+  const int tid = threadIdx.x;
+  if (err->flag) return;
+  for (int idx = tid; idx < jb * jb; idx += blockDim.x) {
+    int i = idx / jb, c = idx % jb;
+    if (c <= i) {
+      lset<A>(L, i, c, A::ld(a, r0 + i, r0 + c));
+      lset<A>(Sa, i, c, S_INIT ? A::ld(S, r0 + i, r0 + c) : A::zero());
+    }
+  }
+  if (tid == 0) fail = 0;
+  __syncthreads();
+  for (int j = 0; j < jb; ++j) {
+    if (tid == 0) {
+      T diag = A::sub(lget<A>(L, j, j), lget<A>(Sa, j, j));
+      if (A::nonpos(diag)) {
+        err->flag = 1;
+        err->pivot = r0 + j;
+        err->value = A::f64(diag);
+        fail = 1;
+      } else {
+        T ljj = A::sqrt(diag);
+        lset<A>(L, j, j, ljj);
+        T inv = A::recip(ljj);
+        if constexpr (A::LIMBS == 2) { invs[0] = inv.hi; invs[1] = inv.lo; inv_hi[j] = inv.hi; inv_lo[j] = inv.lo; }
+        else { invs[0] = inv; inv_hi[j] = inv; }
+      }
+    }
+    __syncthreads();
+    if (fail) return;
+    T inv;
+    if constexpr (A::LIMBS == 2) inv = mkdd(invs[0], invs[1]);
+    else inv = invs[0];
+    for (int i = j + 1 + tid; i < jb; i += blockDim.x)
+      lset<A>(L, i, j, A::mul(A::sub(lget<A>(L, i, j), lget<A>(Sa, i, j)), inv));
+    __syncthreads();
+    const int m = jb - j - 1;
+    for (int idx = tid; idx < m * m; idx += blockDim.x) {
+      int ii = idx / m, cc = idx % m;
+      if (cc <= ii) {
+        int i = j + 1 + ii, c = j + 1 + cc;
+        lset<A>(Sa, i, c, A::add(lget<A>(Sa, i, c), A::mul(lget<A>(L, i, j), lget<A>(L, c, j))));
+      }
+    }
+    __syncthreads();
+  }
+  for (int idx = tid; idx < jb * jb; idx += blockDim.x) {
+    int i = idx / jb, c = idx % jb;
+    if (c <= i) A::st(a, r0 + i, r0 + c, lget<A>(L, i, c));
+  }
+}
+
+// ---------------------------------------------------------------- rows below a panel
+// Rows [row_begin, row_end) x panel columns [c0, c0+w), w <= 64, L = the factored
+// diagonal block a[c0:c0+w, c0:c0+w], inv[k] = recip(L[k][k]).
+//   SUB (trsm_right_lt, linalg.rs:224-235):  t = b_ij; t = t - b_ik*l_jk (k<j); b_ij = t*inv_j
+//   ACC (unblocked, linalg.rs:78-82):        t = S_ij; t = t + l_ik*l_jk (k<j); l_ij = (a_ij - t)*inv_j
+// One wavefront per row, lane j = panel column j.
+template <class A, bool ACC>
+__global__ __launch_bounds__(256) void k_row_recur(Mat a, Mat S, long c0, int w, long row_begin, long row_end,
+                                                   const double* inv_hi, const double* inv_lo, const ErrRec* err) {
+  using T = typename A::T;
+  extern __shared__ __align__(16) unsigned char smem[];
+  Lds2<A>& L = *reinterpret_cast<Lds2<A>*>(smem);
+  if (err->flag) return;
+  const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+  for (int idx = tid; idx < w * w; idx += blockDim.x) {
+    int i = idx / w, c = idx % w;
+    if (c < i) lset<A>(L, i, c, A::ld(a, c0 + i, c0 + c));
+  }
+  __syncthreads();
+  T inv = A::zero();
+  if (lane < w) inv = A::ldv(inv_hi, inv_lo, lane);
+  for (long row = row_begin + (long)blockIdx.x * 4 + wv; row < row_end; row += (long)gridDim.x * 4) {
+    T t = A::zero(), av = A::zero(), fin = A::zero();
+    if (lane < w) {
+      if (ACC) { t = A::ld(S, row, c0 + lane); av = A::ld(a, row, c0 + lane); }
+      else t = A::ld(a, row, c0 + lane);
+    }
+    for (int k = 0; k < w; ++k) {
+      T bk = ACC ? A::mul(A::sub(av, t), inv) : A::mul(t, inv);  // meaningful on lane k
+      if (lane == k) fin = bk;
+      T b = rl(bk, k);
+      if (lane > k && lane < w) {
+        T p = A::mul(b, lget<A>(L, lane, k));
+        t = ACC ? A::add(t, p) : A::sub(t, p);
+      }
+    }
+    if (lane < w) A::st(a, row, c0 + lane, fin);
+  }
+}
+
+// ---------------------------------------------------------------- rank-k updates
+// out(i, j), i in [ri0, ri0+M), j in [cj0, cj0+N), with operands
+//   p(i, k) = a(i, ka0+k),  q(j, k) = a(bj0 + j - cj0, ka0+k),  k in [0, K):
+//   SUMSUB (cholesky_blocked trailing, linalg.rs:268-279): s = 0; s += p*q (k order); a -= s
+//   ACC    (unblocked emulation, linalg.rs:78-82):         S += p*q (k order)
+//   SUBINC (trsm_right_lt across 64-column sub-panels, linalg.rs:228-231): a -= p*q (k order)
+// TRI restricts to j <= i (ri0 == cj0).  64x64 tiles, 4x4 register block per thread, the
+// operands staged through LDS in 16-deep k slabs.
+enum { RU_SUMSUB = 0, RU_ACC = 1, RU_SUBINC = 2 };
+constexpr int TB = 64, TK = 16;
+
+template <class A, int MODE, bool TRI>
+__global__ __launch_bounds__(256) void k_rank_update(Mat a, Mat S, long ri0, long M, long cj0, long N, long bj0,
+                                                     long ka0, long K, const ErrRec* err) {
+  using T = typename A::T;
+  const int ti = blockIdx.y, tc = blockIdx.x;
+  if (TRI && tc > ti) return;
+  if (err->flag) return;
+  __shared__ double As[2][TK][TB], Bs[2][TK][TB];
+  const int t = threadIdx.x, tx = t & 15, ty = t >> 4;
+  const long i0 = (long)ti * TB, j0 = (long)tc * TB;  // tile origin relative to ri0 / cj0
+  T acc[4][4];
+#pragma unroll
+  for (int r = 0; r < 4; ++r)
+#pragma unroll
+    for (int c = 0; c < 4; ++c) {
+      long li = i0 + ty * 4 + r, lj = j0 + tx * 4 + c;
+      bool in = li < M && lj < N && (!TRI || ri0 + li >= cj0 + lj);
+      T v = A::zero();
+      if (in && MODE == RU_ACC) v = A::ld(S, ri0 + li, cj0 + lj);
+      if (in && MODE == RU_SUBINC) v = A::ld(a, ri0 + li, cj0 + lj);
+      acc[r][c] = v;
+    }
+  for (long k0 = 0; k0 < K; k0 += TK) {
+    {  // thread t stages tile row t/4, k (t%4)*4..+3 of both operands
+      int r = t >> 2, kc = (t & 3) * 4;
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        long kk = k0 + kc + q;
+        T va = (i0 + r < M && kk < K) ? A::ld(a, ri0 + i0 + r, ka0 + kk) : A::zero();
+        T vb = (j0 + r < N && kk < K) ? A::ld(a, bj0 + j0 + r, ka0 + kk) : A::zero();
+        if constexpr (A::LIMBS == 2) {
+          As[0][kc + q][r] = va.hi; As[1][kc + q][r] = va.lo;
+          Bs[0][kc + q][r] = vb.hi; Bs[1][kc + q][r] = vb.lo;
+        } else {
+          As[0][kc + q][r] = va; Bs[0][kc + q][r] = vb;
+        }
+      }
+    }
+    __syncthreads();
+    const int kmax = (int)min((long)TK, K - k0);
+    for (int kk = 0; kk < kmax; ++kk) {
+      T av[4], bv[4];
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        if constexpr (A::LIMBS == 2) {
+          av[r] = mkdd(As[0][kk][ty * 4 + r], As[1][kk][ty * 4 + r]);
+          bv[r] = mkdd(Bs[0][kk][tx * 4 + r], Bs[1][kk][tx * 4 + r]);
+        } else {
+          av[r] = As[0][kk][ty * 4 + r];
+          bv[r] = Bs[0][kk][tx * 4 + r];
+        }
+      }
+#pragma unroll
+      for (int r = 0; r < 4; ++r)
+#pragma unroll
+        for (int c = 0; c < 4; ++c) {
+          T p = A::mul(av[r], bv[c]);
+          acc[r][c] = (MODE == RU_SUBINC) ? A::sub(acc[r][c], p) : A::add(acc[r][c], p);
+        }
+    }
+    __syncthreads();
+  }
+#pragma unroll
+  for (int r = 0; r < 4; ++r)
+#pragma unroll
+    for (int c = 0; c < 4; ++c) {
+      long li = i0 + ty * 4 + r, lj = j0 + tx * 4 + c;
+      if (li < M && lj < N && (!TRI || ri0 + li >= cj0 + lj)) {
+        long gi = ri0 + li, gj = cj0 + lj;
+        if (MODE == RU_ACC) A::st(S, gi, gj, acc[r][c]);
+        else if (MODE == RU_SUBINC) A::st(a, gi, gj, acc[r][c]);
+        else A::st(a, gi, gj, A::sub(A::ld(a, gi, gj), acc[r][c]));
+      }
+    }
+}
+
+// inv[j] = recip(L[c0+j][c0+j]) for j < w (trsm_right_lt, linalg.rs:226)
+template <class A>
+__global__ void k_diag_recip(Mat a, long c0, int w, double* inv_h, double* inv_l) {
+  int j = threadIdx.x;
+  if (j < w) A::stv(inv_h, inv_l, j, A::recip(A::ld(a, c0 + j, c0 + j)));
+}
+
+// ---------------------------------------------------------------- helpers
+template <class A>
+__global__ void k_zero_lower(Mat S, long r0, long len) {  // S[r0+i][r0+j] = 0 for j <= i
+  long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  long total = len * len;
+  for (; idx < total; idx += (long)gridDim.x * blockDim.x) {
+    long i = idx / len, j = idx % len;
+    if (j <= i) A::st(S, r0 + i, r0 + j, A::zero());
+  }
+}
+__global__ void k_zero_upper_f64(double* l, long n) {  // cholesky_f64 zeroes the upper triangle (lib.rs:190-192)
+  long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  for (; idx < n * n; idx += (long)gridDim.x * blockDim.x)
+    if (idx % n > idx / n) l[idx] = 0.0;
+}
+
+// ---------------------------------------------------------------- forward solve
+// linalg.rs:99-108: y_i = y_i - l_ij*y_j (j ascending), then y_i = y_i / l_ii.
+// Blocks of 64 rows: the diagonal block in one wavefront, then the rows below receive
+// the block's contributions in j order.
+template <class A>
+__global__ __launch_bounds__(64) void k_fwd_diag(Mat l, double* yh, double* yl, long j0, int bw) {
+  using T = typename A::T;
+  extern __shared__ __align__(16) unsigned char smem[];
+  Lds2<A>& L = *reinterpret_cast<Lds2<A>*>(smem);
+  const int lane = threadIdx.x;
+  for (int i = 0; i < bw; ++i)
+    if (lane <= i) lset<A>(L, i, lane, A::ld(l, j0 + i, j0 + lane));
+  __syncthreads();
+  T y = lane < bw ? A::ldv(yh, yl, j0 + lane) : A::zero();
+  for (int j = 0; j < bw; ++j) {
+    T yj = A::div(y, lget<A>(L, lane < bw ? lane : 0, lane < bw ? lane : 0));  // meaningful on lane j
+    if (lane == j) y = yj;
+    T b = rl(yj, j);
+    if (lane > j && lane < bw) y = A::sub(y, A::mul(lget<A>(L, lane, j), b));
+  }
+  if (lane < bw) A::stv(yh, yl, j0 + lane, y);
+}
+template <class A>
+__global__ __launch_bounds__(256) void k_fwd_update(Mat l, double* yh, double* yl, long j0, int bw, long n) {
+  using T = typename A::T;
+  __shared__ double Th[64][DB + 1], Tl[A::LIMBS == 2 ? 64 : 1][DB + 1];
+  __shared__ double Yh[DB], Yl[DB];
+  const int tid = threadIdx.x;
+  const long rbase = j0 + bw + (long)blockIdx.x * 64;
+  if (tid < bw) { Yh[tid] = yh[j0 + tid]; if constexpr (A::LIMBS == 2) Yl[tid] = yl[j0 + tid]; }
+  for (int idx = tid; idx < 64 * bw; idx += 256) {
+    int r = idx / bw, c = idx % bw;
+    long gi = rbase + r;
+    T v = gi < n ? A::ld(l, gi, j0 + c) : A::zero();
+    if constexpr (A::LIMBS == 2) { Th[r][c] = v.hi; Tl[r][c] = v.lo; }
+    else Th[r][c] = v;
+  }
+  __syncthreads();
+  if (tid < 64 && rbase + tid < n) {
+    long gi = rbase + tid;
+    T y = A::ldv(yh, yl, gi);
+    for (int j = 0; j < bw; ++j) {
+      T lv, yv;
+      if constexpr (A::LIMBS == 2) { lv = mkdd(Th[tid][j], Tl[tid][j]); yv = mkdd(Yh[j], Yl[j]); }
+      else { lv = Th[tid][j]; yv = Yh[j]; }
+      y = A::sub(y, A::mul(lv, yv));
+    }
+    A::stv(yh, yl, gi, y);
+  }
+}
+
+// ---------------------------------------------------------------- backward solve
+// linalg.rs:111-120: x_i = x_i - l_ji*x_j for j = i+1..n-1 ascending, then x_i / l_ii.
+// The chain of row i begins with x_{i+1}, the last value computed before it, so the
+// n(n-1)/2 dependent dd subtractions cannot be reordered without changing the result.
+// One workgroup: the products l_ji*x_j of a chunk are formed in parallel into LDS and
+// thread 0 runs the subtraction chain.
+template <class A>
+__global__ __launch_bounds__(256) void k_bwd_exact(Mat l, double* xh, double* xl, long n) {
+  using T = typename A::T;
+  __shared__ double Ph[256], Pl[A::LIMBS == 2 ? 256 : 1];
+  __shared__ double Xi[2];
+  const int tid = threadIdx.x;
+  for (long i = n - 1; i >= 0; --i) {
+    T s = A::zero();
+    if (tid == 0) s = A::ldv(xh, xl, i);
+    for (long jc = i + 1; jc < n; jc += 256) {
+      long j = jc + tid;
+      if (j < n) {
+        T p = A::mul(A::ld(l, j, i), A::ldv(xh, xl, j));
+        if constexpr (A::LIMBS == 2) { Ph[tid] = p.hi; Pl[tid] = p.lo; }
+        else Ph[tid] = p;
+      }
+      __syncthreads();
+      if (tid == 0) {
+        int cnt = (int)min((long)256, n - jc);
+        for (int q = 0; q < cnt; ++q) {
+          T p;
+          if constexpr (A::LIMBS == 2) p = mkdd(Ph[q], Pl[q]);
+          else p = Ph[q];
+          s = A::sub(s, p);
+        }
+      }
+      __syncthreads();
+    }
+    if (tid == 0) {
+      T x = A::div(s, A::ld(l, i, i));
+      A::stv(xh, xl, i, x);
+      __threadfence_block();
+    }
+    __syncthreads();
+  }
+}
+
+// ---------------------------------------------------------------- matvec
+// linalg.rs:129-147: y_i = sum_j a_ij*x_j, sequential from zero.  64 rows per
+// wavefront, the row tile staged through LDS so HBM reads stay coalesced.
+template <class A>
+__global__ __launch_bounds__(64) void k_matvec(Mat a, const double* xh, const double* xl, double* yh, double* yl,
+                                               long n) {
+  using T = typename A::T;
+  __shared__ double Th[64][65], Tl[A::LIMBS == 2 ? 64 : 1][65];
+  __shared__ double Xh[64], Xl[64];
+  const int lane = threadIdx.x;
+  const long r0 = (long)blockIdx.x * 64;
+  T s = A::zero();
+  for (long c0 = 0; c0 < n; c0 += 64) {
+    int cw = (int)min((long)64, n - c0);
+    for (int r = 0; r < 64; ++r) {
+      long gi = r0 + r;
+      T v = (gi < n && lane < cw) ? A::ld(a, gi, c0 + lane) : A::zero();
+      if constexpr (A::LIMBS == 2) { Th[r][lane] = v.hi; Tl[r][lane] = v.lo; }
+      else Th[r][lane] = v;
+    }
+    if (lane < cw) { Xh[lane] = xh[c0 + lane]; if constexpr (A::LIMBS == 2) Xl[lane] = xl[c0 + lane]; }
+    __syncthreads();
+    for (int q = 0; q < cw; ++q) {
+      T av, xv;
+      if constexpr (A::LIMBS == 2) { av = mkdd(Th[lane][q], Tl[lane][q]); xv = mkdd(Xh[q], Xl[q]); }
+      else { av = Th[lane][q]; xv = Xh[q]; }
+      s = A::add(s, A::mul(av, xv));
+    }
+    __syncthreads();
+  }
+  if (r0 + lane < n) A::stv(yh, yl, r0 + lane, s);
+}
+
+}  // namespace
+
+// ================================================================ host orchestration
+template <class A>
+static int set_lds_attrs() {
+  const int big = (int)(2 * sizeof(Lds2<A>));
+  const int one = (int)sizeof(Lds2<A>);
+  HP_HIP(hipFuncSetAttribute((const void*)k_diag_factor<A, true>, hipFuncAttributeMaxDynamicSharedMemorySize, big));
+  HP_HIP(hipFuncSetAttribute((const void*)k_diag_factor<A, false>, hipFuncAttributeMaxDynamicSharedMemorySize, big));
+  HP_HIP(hipFuncSetAttribute((const void*)k_row_recur<A, true>, hipFuncAttributeMaxDynamicSharedMemorySize, one));
+  HP_HIP(hipFuncSetAttribute((const void*)k_row_recur<A, false>, hipFuncAttributeMaxDynamicSharedMemorySize, one));
+  HP_HIP(hipFuncSetAttribute((const void*)k_fwd_diag<A>, hipFuncAttributeMaxDynamicSharedMemorySize, one));
+  return HP_OK;
+}
+template <class A>
+static int ensure_attrs() {
+  static int rc = set_lds_attrs<A>();  // once per process (function attributes are global)
+  return rc;
+}
+
+template <class A, int MODE, bool TRI>
+static int rank_update(Mat a, Mat S, long ri0, long M, long cj0, long N, long bj0, long ka0, long K, ErrRec* err,
+                       hipStream_t s) {
+  if (M <= 0 || N <= 0) return HP_OK;
+  dim3 grid((unsigned)((N + TB - 1) / TB), (unsigned)((M + TB - 1) / TB));
+  hipLaunchKernelGGL((k_rank_update<A, MODE, TRI>), grid, dim3(256), 0, s, a, S, ri0, M, cj0, N, bj0, ka0, K,
+                     (const ErrRec*)err);
+  HP_LAUNCH_CHECK();
+  return HP_OK;
+}
+
+template <class A, bool ACC>
+static int row_recur(Mat a, Mat S, long c0, int w, long rb, long re, double* inv_h, double* inv_l, ErrRec* err,
+                     hipStream_t s) {
+  if (re <= rb) return HP_OK;
+  unsigned g = (unsigned)std::min<long>((re - rb + 3) / 4, 2048);
+  hipLaunchKernelGGL((k_row_recur<A, ACC>), dim3(g), dim3(256), sizeof(Lds2<A>), s, a, S, c0, w, rb, re, inv_h,
+                     inv_l, (const ErrRec*)err);
+  HP_LAUNCH_CHECK();
+  return HP_OK;
+}
+
+// Unblocked Cholesky (linalg.rs:42-96) of a[r0:r0+len, r0:r0+len], S = partial sums.
+template <class A>
+static int factor_unblocked_region(Mat a, Mat S, long r0, long len, ErrRec* err, double* inv_h, double* inv_l,
+                                   hipStream_t s) {
+  {
+    long total = len * len;
+    unsigned g = (unsigned)std::min<long>((total + 255) / 256, 65536);
+    hipLaunchKernelGGL(k_zero_lower<A>, dim3(g ? g : 1), dim3(256), 0, s, S, r0, len);
+    HP_LAUNCH_CHECK();
+  }
+  for (long c0 = r0; c0 < r0 + len; c0 += DB) {
+    int w = (int)std::min<long>(DB, r0 + len - c0);
+    hipLaunchKernelGGL((k_diag_factor<A, true>), dim3(1), dim3(256), 2 * sizeof(Lds2<A>), s, a, S, c0, w, err, inv_h,
+                       inv_l);
+    HP_LAUNCH_CHECK();
+    long rb = c0 + w, re = r0 + len;
+    int rc;
+    if ((rc = row_recur<A, true>(a, S, c0, w, rb, re, inv_h, inv_l, err, s))) return rc;
+    if ((rc = rank_update<A, RU_ACC, true>(a, S, rb, re - rb, rb, re - rb, rb, c0, w, err, s))) return rc;
+  }
+  return HP_OK;
+}
+
+template <class A>
+int chol_exact(Mat a, long n, long nb, bool unblocked, Mat S, ErrRec* err, double* inv_h, double* inv_l,
+               hipStream_t s) {
+  int rc = ensure_attrs<A>();
+  if (rc) return rc;
+  if (n == 0) return HP_OK;
+  if (nb == 0) nb = 64;  // linalg.rs:241
+  if (unblocked || nb >= n) return factor_unblocked_region<A>(a, S, 0, n, err, inv_h, inv_l, s);
+  for (long jb0 = 0; jb0 < n; jb0 += nb) {
+    long jb = std::min(nb, n - jb0);
+    // diagonal block: the unblocked algorithm on the block (linalg.rs:250-255)
+    if (jb <= DB) {
+      hipLaunchKernelGGL((k_diag_factor<A, false>), dim3(1), dim3(256), 2 * sizeof(Lds2<A>), s, a, S, jb0, (int)jb,
+                         err, inv_h, inv_l);
+      HP_LAUNCH_CHECK();
+    } else if ((rc = factor_unblocked_region<A>(a, S, jb0, jb, err, inv_h, inv_l, s))) {
+      return rc;
+    }
+    long rem = n - jb0 - jb;
+    if (rem <= 0) continue;
+    // trsm_right_lt on the panel rows (linalg.rs:259-261) in 64-column sub-panels
+    for (long q0 = 0; q0 < jb; q0 += DB) {
+      int w = (int)std::min<long>(DB, jb - q0);
+      if (q0 > 0 &&
+          (rc = rank_update<A, RU_SUBINC, false>(a, S, jb0 + jb, rem, jb0 + q0, w, jb0 + q0, jb0, q0, err, s)))
+        return rc;
+      if (jb > DB) {
+        hipLaunchKernelGGL(k_diag_recip<A>, dim3(1), dim3(64), 0, s, a, jb0 + q0, w, inv_h, inv_l);
+        HP_LAUNCH_CHECK();
+      }
+      if ((rc = row_recur<A, false>(a, S, jb0 + q0, w, jb0 + jb, n, inv_h, inv_l, err, s))) return rc;
+    }
+    // trailing update (linalg.rs:263-288)
+    if ((rc = rank_update<A, RU_SUMSUB, true>(a, S, jb0 + jb, rem, jb0 + jb, rem, jb0 + jb, jb0, jb, err, s))) return rc;
+  }
+  return HP_OK;
+}
+
+template <class A>
+int fwd_exact(Mat l, double* yh, double* yl, long n, hipStream_t s) {
+  int rc = ensure_attrs<A>();
+  if (rc) return rc;
+  for (long j0 = 0; j0 < n; j0 += DB) {
+    int bw = (int)std::min<long>(DB, n - j0);
+    hipLaunchKernelGGL(k_fwd_diag<A>, dim3(1), dim3(64), sizeof(Lds2<A>), s, l, yh, yl, j0, bw);
+    HP_LAUNCH_CHECK();
+    long rem = n - j0 - bw;
+    if (rem > 0) {
+      hipLaunchKernelGGL(k_fwd_update<A>, dim3((unsigned)((rem + 63) / 64)), dim3(256), 0, s, l, yh, yl, j0, bw, n);
+      HP_LAUNCH_CHECK();
+    }
+  }
+  return HP_OK;
+}
+
+template <class A>
+int bwd_exact(Mat l, double* xh, double* xl, long n, hipStream_t s) {
+  if (n == 0) return HP_OK;
+  hipLaunchKernelGGL(k_bwd_exact<A>, dim3(1), dim3(256), 0, s, l, xh, xl, n);
+  HP_LAUNCH_CHECK();
+  return HP_OK;
+}
+
+template <class A>
+int matvec_exact(Mat a, const double* xh, const double* xl, double* yh, double* yl, long n, hipStream_t s) {
+  if (n == 0) return HP_OK;
+  hipLaunchKernelGGL(k_matvec<A>, dim3((unsigned)((n + 63) / 64)), dim3(64), 0, s, a, xh, xl, yh, yl, n);
+  HP_LAUNCH_CHECK();
+  return HP_OK;
+}
+
+int zero_upper_f64(double* l, long n, hipStream_t s) {
+  unsigned g = (unsigned)std::min<long>((n * n + 255) / 256, 65536);
+  hipLaunchKernelGGL(k_zero_upper_f64, dim3(g ? g : 1), dim3(256), 0, s, l, n);
+  HP_LAUNCH_CHECK();
+  return HP_OK;
+}
+
+template int chol_exact<ArDD>(Mat, long, long, bool, Mat, ErrRec*, double*, double*, hipStream_t);
+template int chol_exact<ArF64>(Mat, long, long, bool, Mat, ErrRec*, double*, double*, hipStream_t);
+template int fwd_exact<ArDD>(Mat, double*, double*, long, hipStream_t);
+template int fwd_exact<ArF64>(Mat, double*, double*, long, hipStream_t);
+template int bwd_exact<ArDD>(Mat, double*, double*, long, hipStream_t);
+template int bwd_exact<ArF64>(Mat, double*, double*, long, hipStream_t);
+template int matvec_exact<ArDD>(Mat, const double*, const double*, double*, double*, long, hipStream_t);
+template int matvec_exact<ArF64>(Mat, const double*, const double*, double*, double*, long, hipStream_t);
+
+}  // namespace hp
