@@ -67,9 +67,18 @@ int hp_mf_gemm(const double* a, const double* b, double* c, size_t m, size_t n, 
 int hp_mf_gemm_atb(const double* a, const double* b, double* c, size_t m, size_t n, size_t k);
 
 /* Device dd GEMM on SoA planes.  trans_a = 0: A is m x k (gemm); 1: A is k x m (gemm_atb).
- * mode = HP_GEMM_PLAIN is the reference's per-element order (bit-identical). */
+ *   HP_GEMM_PLAIN     the reference's per-element order on FP64 VALU (bit-identical);
+ *   HP_GEMM_SPLIT_I8  split precision on INT8 matrix cores: rows of A / columns of B are
+ *                     scaled to W-bit integers (W ~ 108 bits for `slices` = 31), the
+ *                     integer product is formed exactly from `slices` int8 GEMMs modulo
+ *                     coprime moduli <= 256 and rebuilt by the Chinese remainder theorem.
+ *                     Error <= 2^(1-W) (max_p|A_ip| sum_p|B_pj| + sum_p|A_ip| max_p|B_pj|)
+ *                     + 2^-106 |C_ij|.  slices in [8, 32]; 0 selects 31.
+ * The split modes keep a device workspace between calls (~3*slices*(m*k + k*n + m*n)/3
+ * bytes) and synchronise `stream` before returning. */
 enum hp_gemm_mode {
-  HP_GEMM_PLAIN = 0
+  HP_GEMM_PLAIN = 0,
+  HP_GEMM_SPLIT_I8 = 2
 };
 int hp_dev_gemm(int trans_a, const double* a_hi, const double* a_lo, const double* b_hi, const double* b_lo,
                 double* c_hi, double* c_lo, size_t m, size_t n, size_t k, int mode, int slices, void* stream);

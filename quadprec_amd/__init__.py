@@ -23,6 +23,7 @@ LIB_PATH = os.path.join(_HERE, "libhyperprec_hip.so")
 
 HP_OK, HP_ESHAPE, HP_ENOTSPD, HP_EDEVICE, HP_EINVAL, HP_ENOMEM = range(6)
 HP_GEMM_PLAIN = 0
+HP_GEMM_SPLIT_I8 = 2
 
 _lib = None
 _dp = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")

@@ -140,11 +140,12 @@ int hp_mf_gemm_atb(const double* a, const double* b, double* c, size_t m, size_t
 
 int hp_dev_gemm(int trans_a, const double* a_hi, const double* a_lo, const double* b_hi, const double* b_lo,
                 double* c_hi, double* c_lo, size_t m, size_t n, size_t k, int mode, int slices, void* stream) {
-  (void)slices;
   hipStream_t s = (hipStream_t)stream;
   switch (mode) {
     case HP_GEMM_PLAIN:
       return launch_gemm_dd_plain(trans_a, a_hi, a_lo, b_hi, b_lo, c_hi, c_lo, m, n, k, s);
+    case HP_GEMM_SPLIT_I8:
+      return launch_gemm_split_i8(trans_a, a_hi, a_lo, b_hi, b_lo, c_hi, c_lo, m, n, k, slices, s);
     default:
       set_error("hp_dev_gemm: unknown mode %d", mode);
       return HP_EINVAL;

@@ -64,6 +64,9 @@ int download_planes(const double* dev_planes, size_t count, int limbs, double* h
 int launch_gemm_dd_plain(int trans_a, const double* ahi, const double* alo, const double* bhi,
                          const double* blo, double* chi, double* clo, size_t m, size_t n, size_t k,
                          hipStream_t s);
+int launch_gemm_split_i8(int trans_a, const double* ahi, const double* alo, const double* bhi, const double* blo,
+                         double* chi, double* clo, size_t m, size_t n, size_t k, int R, hipStream_t s);
+int crt_bits_W(int R, size_t k);
 int launch_mf4_gemm(int trans_a, const double* a, const double* b, double* c, size_t m, size_t n,
                     size_t k, hipStream_t s);
 

@@ -1,0 +1,509 @@
+// k_split_i8.hip -- split-precision double-double GEMM on gfx950 INT8 matrix cores.
+//
+// Exact-integer (Ozaki-style) GEMM by residue arithmetic:
+//   1. every row i of A and column j of B gets a power-of-two scale 2^e_i / 2^f_j that
+//      bounds it; each dd entry becomes a W-bit integer  A'_ip = T(A_ip * 2^(W-e_i)),
+//      B'_pj = T(B_pj * 2^(W-f_j))  (T truncates hi and lo separately; error < 2 units of
+//      2^(e-W) per entry, W ~ 108-112 bits, i.e. ~2^-108 of the row / column maximum);
+//   2. for R pairwise-coprime moduli m_r <= 256 the residues A' mod m_r, B' mod m_r are
+//      int8 values in [-128, 127]; C'_r = (A'_r B'_r mod m_r) is an int8 x int8 -> int32
+//      GEMM on v_mfma_i32_32x32x32_i8 -- exact, |partial sums| <= k*2^14 < 2^31;
+//   3. C' = A'B' (exactly, |C'| < 2^(2W+13) < M/16, M = prod m_r) is rebuilt from its
+//      residues by the Chinese remainder theorem in 256-bit integer arithmetic and turned
+//      into a dd value times 2^(e_i + f_j - 2W).
+// The product of the integer matrices is exact, so the only errors are the W-bit
+// truncation of the inputs and the final rounding to dd: |C - C*|_ij <=
+// 2^(1-W) (max_p|A_ip| sum_p|B_pj| + sum_p|A_ip| max_p|B_pj|) + 2^-106 |C*_ij|.
+// Cost: R int8 GEMMs (2R int ops per dd MAC) instead of 29 FP64 VALU ops per dd MAC.
+//
+// Layouts: residue planes are [r][row][k] (k contiguous) for both operands, so A (m x k)
+// and B^T (n x k) are loaded identically; C residues are [r][i][j] bytes.
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+#include <mutex>
+#include <vector>
+
+#include "hp_arith.hpp"
+#include "hp_internal.hpp"
+
+namespace hp {
+
+constexpr int CRT_MAXR = 32;
+constexpr int CRT_LIMBS = 8;  // 256-bit integers as 8 x 32-bit limbs
+
+// The moduli: 2^8 and the largest pairwise-coprime values below it.
+static const uint32_t kModuli[CRT_MAXR] = {256, 255, 253, 251, 247, 241, 239, 233, 229, 227, 223,
+                                           211, 199, 197, 193, 191, 181, 179, 173, 167, 163, 157,
+                                           151, 149, 139, 137, 131, 127, 113, 109, 107, 103};
+
+struct CrtTable {
+  int R;                                   // number of moduli in use
+  int pad;
+  uint32_t mod[CRT_MAXR];
+  uint32_t yinv[CRT_MAXR];                 // (M / m_r)^-1 mod m_r
+  double inv_m[CRT_MAXR];                  // 1 / m_r
+  uint32_t Mr[CRT_MAXR][CRT_LIMBS];        // M / m_r, little-endian limbs
+  uint32_t M[CRT_LIMBS];                   // M
+  double c24[CRT_MAXR][5];                 // 2^(24q) mod m_r, q < 5
+};
+
+namespace {
+
+// ------------------------------------------------------------------ host big-int helpers
+void big_mul_small(uint32_t* x, uint32_t v) {
+  uint64_t carry = 0;
+  for (int i = 0; i < CRT_LIMBS; ++i) {
+    uint64_t t = (uint64_t)x[i] * v + carry;
+    x[i] = (uint32_t)t;
+    carry = t >> 32;
+  }
+}
+uint32_t big_mod_small(const uint32_t* x, uint32_t m) {
+  uint64_t r = 0;
+  for (int i = CRT_LIMBS - 1; i >= 0; --i) r = ((r << 32) | x[i]) % m;
+  return (uint32_t)r;
+}
+uint32_t inv_mod(uint32_t a, uint32_t m) {  // a^-1 mod m (gcd(a, m) = 1)
+  int64_t t = 0, nt = 1, r = m, nr = a % m;
+  while (nr) {
+    int64_t q = r / nr, tmp;
+    tmp = t - q * nt; t = nt; nt = tmp;
+    tmp = r - q * nr; r = nr; nr = tmp;
+  }
+  if (t < 0) t += m;
+  return (uint32_t)t;
+}
+
+void build_table(int R, CrtTable& T) {
+  std::memset(&T, 0, sizeof(T));
+  T.R = R;
+  uint32_t M[CRT_LIMBS] = {1, 0, 0, 0, 0, 0, 0, 0};
+  for (int r = 0; r < R; ++r) big_mul_small(M, kModuli[r]);
+  std::memcpy(T.M, M, sizeof(M));
+  for (int r = 0; r < R; ++r) {
+    uint32_t m = kModuli[r];
+    T.mod[r] = m;
+    T.inv_m[r] = 1.0 / (double)m;
+    uint32_t Mr[CRT_LIMBS] = {1, 0, 0, 0, 0, 0, 0, 0};
+    for (int q = 0; q < R; ++q)
+      if (q != r) big_mul_small(Mr, kModuli[q]);
+    std::memcpy(T.Mr[r], Mr, sizeof(Mr));
+    T.yinv[r] = inv_mod(big_mod_small(Mr, m), m);
+    uint64_t p = 1 % m;
+    for (int q = 0; q < 5; ++q) {
+      T.c24[r][q] = (double)p;
+      p = (p << 24) % m;
+    }
+  }
+}
+
+double log2_M(int R) {
+  double s = 0;
+  for (int r = 0; r < R; ++r) s += std::log2((double)kModuli[r]);
+  return s;
+}
+
+// ------------------------------------------------------------------ device helpers
+__device__ __forceinline__ uint32_t mod32(uint32_t x, uint32_t m, double inv_m) {
+  uint32_t q = (uint32_t)((double)x * inv_m);
+  int32_t r = (int32_t)(x - q * m);
+  if (r < 0) r += m;
+  if (r >= (int32_t)m) r -= m;
+  return (uint32_t)r;
+}
+// sign(v) * floor(|v|) as a 128-bit integer; |v| < 2^120
+__device__ __forceinline__ __int128 trunc128(double v) {
+  uint64_t b = __builtin_bit_cast(uint64_t, v);
+  int ex = (int)((b >> 52) & 0x7ff);
+  if (ex == 0) return 0;  // zero / subnormal: |v| < 1
+  uint64_t mn = (b & 0xfffffffffffffull) | (1ull << 52);
+  int e2 = ex - 1075;     // |v| = mn * 2^e2
+  unsigned __int128 u;
+  if (e2 >= 0) u = (unsigned __int128)mn << e2;
+  else if (e2 > -53) u = mn >> (-e2);
+  else u = 0;
+  __int128 x = (__int128)u;
+  return (b >> 63) ? -x : x;
+}
+__device__ __forceinline__ int8_t center(uint32_t r, uint32_t m) {
+  return (int8_t)((r > (m - 1) / 2) ? (int)r - (int)m : (int)r);
+}
+__device__ __forceinline__ int frexp_exp(double v) {  // |v| < 2^e, 0 for 0
+  if (v == 0.0) return -100000;
+  int e;
+  (void)__builtin_frexp(v, &e);
+  return e;
+}
+
+// ------------------------------------------------------------------ scale exponents
+// row scales: element (row, p) at base[row*ld + p] (contiguous p); one wavefront per row
+__global__ void k_scale_rows(const double* __restrict__ hi, long rows, long K, long ld, int* __restrict__ e) {
+  long row = (long)blockIdx.x * 4 + (threadIdx.x >> 6);
+  int lane = threadIdx.x & 63;
+  if (row >= rows) return;
+  int mx = -100000;
+  for (long p = lane; p < K; p += 64) mx = max(mx, frexp_exp(hi[row * ld + p]));
+  for (int o = 32; o > 0; o >>= 1) mx = max(mx, __shfl_xor(mx, o, 64));
+  if (lane == 0) e[row] = mx;
+}
+// column scales: element (col, p) at base[p*ld + col]; a thread per column and 256-row
+// chunk, combined with atomicMax (e[] preset to INT_MIN by k_fill_int)
+__global__ void k_fill_int(int* p, long n, int v) {
+  long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = v;
+}
+__global__ void k_scale_cols(const double* __restrict__ hi, long cols, long K, long ld, int* __restrict__ e) {
+  long col = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (col >= cols) return;
+  long p0 = (long)blockIdx.y * 256, p1 = min(K, p0 + 256);
+  int mx = -100000;
+  for (long p = p0; p < p1; ++p) mx = max(mx, frexp_exp(hi[p * ld + col]));
+  atomicMax(&e[col], mx);
+}
+
+// ------------------------------------------------------------------ residues
+// Element (row, p) of the logical rows x K operand -> out[r][row][p] (int8, centred).
+// CONTIG: the source element is at base[row*ld + p]; else at base[p*ld + row] (staged
+// through an LDS transpose so both the dd reads and the byte writes are coalesced).
+template <bool CONTIG>
+__global__ __launch_bounds__(256) void k_residues(const double* __restrict__ hi, const double* __restrict__ lo,
+                                                  long rows, long K, long ld, const int* __restrict__ escale, int W,
+                                                  const CrtTable* __restrict__ tab, int8_t* __restrict__ out) {
+  __shared__ double c24s[CRT_MAXR][5];
+  __shared__ uint32_t mods[CRT_MAXR];
+  __shared__ double mods_d[CRT_MAXR], invs[CRT_MAXR];
+  __shared__ double th[64][65], tl[64][65];
+  const int R = tab->R;
+  for (int i = threadIdx.x; i < R * 5; i += 256) c24s[i / 5][i % 5] = tab->c24[i / 5][i % 5];
+  if (threadIdx.x < R) {
+    mods[threadIdx.x] = tab->mod[threadIdx.x];
+    mods_d[threadIdx.x] = (double)tab->mod[threadIdx.x];
+    invs[threadIdx.x] = tab->inv_m[threadIdx.x];
+  }
+  const long row0 = (long)blockIdx.y * 64, p0 = (long)blockIdx.x * 64;
+  if (!CONTIG) {  // stage the 64 x 64 tile: source rows p (contiguous in `row`)
+    for (int idx = threadIdx.x; idx < 64 * 64; idx += 256) {
+      int pp = idx / 64, rr = idx % 64;
+      long gr = row0 + rr, gp = p0 + pp;
+      bool ok = gr < rows && gp < K;
+      th[rr][pp] = ok ? hi[gp * ld + gr] : 0.0;
+      tl[rr][pp] = ok ? lo[gp * ld + gr] : 0.0;
+    }
+  }
+  __syncthreads();
+  const size_t plane = (size_t)rows * K;
+  for (int idx = threadIdx.x; idx < 64 * 64; idx += 256) {
+    int rr = idx / 64, pp = idx % 64;
+    long gr = row0 + rr, gp = p0 + pp;
+    if (gr >= rows || gp >= K) continue;
+    double h, l;
+    if (CONTIG) { h = hi[gr * ld + gp]; l = lo[gr * ld + gp]; }
+    else { h = th[rr][pp]; l = tl[rr][pp]; }
+    int sc = W - max(escale[gr], -1100);
+    __int128 X = trunc128(__builtin_ldexp(h, sc)) + trunc128(__builtin_ldexp(l, sc));
+    const bool neg = X < 0;
+    unsigned __int128 U = neg ? (unsigned __int128)(-X) : (unsigned __int128)X;
+    double L[5];
+#pragma unroll
+    for (int q = 0; q < 5; ++q) L[q] = (double)(uint32_t)((U >> (24 * q)) & 0xffffffu);
+    size_t o = (size_t)gr * K + gp;
+    for (int r = 0; r < R; ++r) {
+      const double m = mods_d[r];
+      double v = L[0] * c24s[r][0] + L[1] * c24s[r][1] + L[2] * c24s[r][2] + L[3] * c24s[r][3] + L[4] * c24s[r][4];
+      double rr = v - m * __builtin_floor(v * invs[r]);  // exact: v < 2^35
+      if (rr < 0) rr += m;
+      if (rr >= m) rr -= m;
+      uint32_t res = (uint32_t)rr;
+      uint32_t mi = mods[r];
+      if (neg && res) res = mi - res;
+      out[(size_t)r * plane + o] = center(res, mi);
+    }
+  }
+}
+
+// ------------------------------------------------------------------ int8 GEMM mod m
+// Cr[i][j] = (sum_p Ar[i][p] * Br[j][p]) mod m_r for every modulus r = blockIdx.z.
+// 256 x 256 block tile, 4 wavefronts of 128 x 128 (4 x 4 tiles of v_mfma_i32_32x32x32_i8),
+// K in 32-deep stages double-buffered through LDS.
+typedef int v4i __attribute__((ext_vector_type(4)));
+typedef int v16i __attribute__((ext_vector_type(16)));
+constexpr int GM = 256, GN = 256, GK = 32, GKP = GK + 16;
+
+__global__ __launch_bounds__(256) void k_gemm_i8_mod(const int8_t* __restrict__ A, const int8_t* __restrict__ B,
+                                                     uint8_t* __restrict__ C, long M, long N, long K,
+                                                     const CrtTable* __restrict__ tab) {
+  __shared__ __align__(16) int8_t As[2][GM][GKP];
+  __shared__ __align__(16) int8_t Bs[2][GN][GKP];
+  const int r = blockIdx.z;
+  const uint32_t m = tab->mod[r];
+  const double inv_m = tab->inv_m[r];
+  A += (size_t)r * M * K;
+  B += (size_t)r * N * K;
+  C += (size_t)r * M * N;
+  const int t = threadIdx.x, lane = t & 63, w = t >> 6;
+  const int wm = w >> 1, wn = w & 1;
+  const long i0 = (long)blockIdx.y * GM, j0 = (long)blockIdx.x * GN;
+
+  v16i acc[4][4];
+#pragma unroll
+  for (int a = 0; a < 4; ++a)
+#pragma unroll
+    for (int b = 0; b < 4; ++b)
+#pragma unroll
+      for (int q = 0; q < 16; ++q) acc[a][b][q] = 0;
+
+  // each thread stages one 32-byte row segment of A and of B per stage
+  const bool kvec = (K % 16) == 0;
+  auto load_row = [&](const int8_t* base, long rowg, long nrows, long k0, v4i& x0, v4i& x1) {
+    x0 = v4i{0, 0, 0, 0};
+    x1 = v4i{0, 0, 0, 0};
+    if (rowg >= nrows) return;
+    const int8_t* p = base + rowg * K + k0;
+    if (kvec && k0 + GK <= K) {
+      x0 = *reinterpret_cast<const v4i*>(p);
+      x1 = *reinterpret_cast<const v4i*>(p + 16);
+    } else {
+      int8_t tmp[32];
+      for (int q = 0; q < 32; ++q) tmp[q] = (k0 + q < K) ? p[q] : (int8_t)0;
+      __builtin_memcpy(&x0, tmp, 16);
+      __builtin_memcpy(&x1, tmp + 16, 16);
+    }
+  };
+  v4i ra0, ra1, rb0, rb1;
+  const long nst = (K + GK - 1) / GK;
+  load_row(A, i0 + t, M, 0, ra0, ra1);
+  load_row(B, j0 + t, N, 0, rb0, rb1);
+  const long KCH = 65536;  // reduce partial sums mod m before |sum| could reach 2^31
+  for (long st = 0; st < nst; ++st) {
+    const int buf = st & 1;
+    *reinterpret_cast<v4i*>(&As[buf][t][0]) = ra0;
+    *reinterpret_cast<v4i*>(&As[buf][t][16]) = ra1;
+    *reinterpret_cast<v4i*>(&Bs[buf][t][0]) = rb0;
+    *reinterpret_cast<v4i*>(&Bs[buf][t][16]) = rb1;
+    __syncthreads();
+    if (st + 1 < nst) {
+      load_row(A, i0 + t, M, (st + 1) * GK, ra0, ra1);
+      load_row(B, j0 + t, N, (st + 1) * GK, rb0, rb1);
+    }
+    v4i af[4], bf[4];
+    const int kq = 16 * (lane >> 5);
+#pragma unroll
+    for (int a = 0; a < 4; ++a) af[a] = *reinterpret_cast<const v4i*>(&As[buf][wm * 128 + a * 32 + (lane & 31)][kq]);
+#pragma unroll
+    for (int b = 0; b < 4; ++b) bf[b] = *reinterpret_cast<const v4i*>(&Bs[buf][wn * 128 + b * 32 + (lane & 31)][kq]);
+#pragma unroll
+    for (int a = 0; a < 4; ++a)
+#pragma unroll
+      for (int b = 0; b < 4; ++b) acc[a][b] = __builtin_amdgcn_mfma_i32_32x32x32_i8(af[a], bf[b], acc[a][b], 0, 0, 0);
+    if (((st + 1) * GK) % KCH == 0 && st + 1 < nst) {
+#pragma unroll
+      for (int a = 0; a < 4; ++a)
+#pragma unroll
+        for (int b = 0; b < 4; ++b)
+#pragma unroll 1
+          for (int q = 0; q < 16; ++q) {
+            int v = acc[a][b][q];
+            int32_t rr = v - (int32_t)m * (int32_t)__builtin_floor((double)v * inv_m);
+            acc[a][b][q] = rr;
+          }
+    }
+    // the buffer written next stage is the other one; this barrier orders its reuse
+    __syncthreads();
+  }
+  // epilogue: residue in [0, m) as a byte.  32x32 MFMA output: lane l holds column l%32,
+  // register q holds row 8*(q/4) + 4*(l/32) + q%4.
+#pragma unroll
+  for (int a = 0; a < 4; ++a)
+#pragma unroll
+    for (int b = 0; b < 4; ++b)
+#pragma unroll
+      for (int q = 0; q < 16; ++q) {
+        long gi = i0 + wm * 128 + a * 32 + 8 * (q >> 2) + 4 * (lane >> 5) + (q & 3);
+        long gj = j0 + wn * 128 + b * 32 + (lane & 31);
+        if (gi < M && gj < N) {
+          int v = acc[a][b][q];
+          int32_t rr = v - (int32_t)m * (int32_t)__builtin_floor((double)v * inv_m);
+          if (rr < 0) rr += m;
+          if (rr >= (int32_t)m) rr -= m;
+          C[gi * N + gj] = (uint8_t)rr;
+        }
+      }
+}
+
+// ------------------------------------------------------------------ CRT reconstruction
+// C'_ij = sum_r u_r (M/m_r) - q M, u_r = c_r y_r mod m_r, q = round(sum_r u_r / m_r);
+// then C_ij = C'_ij 2^(e_i + f_j - 2W) rounded to dd.
+__device__ __forceinline__ dd u64_to_dd(uint64_t x) {  // exact
+  double h = (double)x;  // round to nearest; may equal 2^64
+  uint64_t hb = (h >= 18446744073709551616.0) ? 0 : (uint64_t)h;
+  int64_t rem = (h >= 18446744073709551616.0) ? -(int64_t)(0 - x) : (int64_t)(x - hb);
+  double l = (double)rem;
+  dd r;
+  quick_two_sum(h, l, r.hi, r.lo);
+  return r;
+}
+
+__global__ __launch_bounds__(256) void k_crt(const uint8_t* __restrict__ Cr, long M, long N,
+                                             const int* __restrict__ er, const int* __restrict__ fc, int W,
+                                             const CrtTable* __restrict__ tab, double* __restrict__ chi,
+                                             double* __restrict__ clo) {
+  __shared__ uint32_t Mr[CRT_MAXR][CRT_LIMBS];
+  __shared__ uint32_t Ml[CRT_LIMBS];
+  __shared__ uint32_t mods[CRT_MAXR], ys[CRT_MAXR];
+  __shared__ double invs[CRT_MAXR];
+  const int R = tab->R;
+  for (int i = threadIdx.x; i < R * CRT_LIMBS; i += blockDim.x) Mr[i / CRT_LIMBS][i % CRT_LIMBS] = tab->Mr[i / CRT_LIMBS][i % CRT_LIMBS];
+  if (threadIdx.x < CRT_LIMBS) Ml[threadIdx.x] = tab->M[threadIdx.x];
+  if (threadIdx.x < R) {
+    mods[threadIdx.x] = tab->mod[threadIdx.x];
+    ys[threadIdx.x] = tab->yinv[threadIdx.x];
+    invs[threadIdx.x] = tab->inv_m[threadIdx.x];
+  }
+  __syncthreads();
+  const size_t plane = (size_t)M * N;
+  for (size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x; idx < plane; idx += (size_t)gridDim.x * blockDim.x) {
+    uint64_t acc[CRT_LIMBS] = {0, 0, 0, 0, 0, 0, 0, 0};
+    double t = 0.0;
+    for (int r = 0; r < R; ++r) {
+      uint32_t m = mods[r];
+      uint32_t u = mod32((uint32_t)Cr[(size_t)r * plane + idx] * ys[r], m, invs[r]);
+      t += (double)u * invs[r];
+#pragma unroll
+      for (int q = 0; q < CRT_LIMBS; ++q) acc[q] += (uint64_t)u * Mr[r][q];
+    }
+    uint64_t qn = (uint64_t)__builtin_rint(t);
+    // S - qn*M with carries, 256-bit two's complement
+    uint32_t v[CRT_LIMBS];
+    int64_t carry = 0;
+#pragma unroll
+    for (int q = 0; q < CRT_LIMBS; ++q) {
+      int64_t s = (int64_t)(acc[q] & 0xffffffffull) - (int64_t)(qn * Ml[q]) + carry;
+      // acc[q] >> 32 contributes to the next limb
+      v[q] = (uint32_t)s;
+      carry = (s >> 32) + (int64_t)(acc[q] >> 32);
+    }
+    bool neg = (v[CRT_LIMBS - 1] >> 31) != 0;
+    if (neg) {  // magnitude
+      uint64_t c = 1;
+#pragma unroll
+      for (int q = 0; q < CRT_LIMBS; ++q) {
+        uint64_t s = (uint64_t)(uint32_t)~v[q] + c;
+        v[q] = (uint32_t)s;
+        c = s >> 32;
+      }
+    }
+    int top = CRT_LIMBS - 1;
+    while (top > 0 && v[top] == 0) --top;
+    dd res = mkdd(0.0);
+    if (top > 0 || v[0] != 0) {
+      // sum of the top six 32-bit limbs (192 bits) as two 64-bit words and two 32-bit words
+      const int base = er[idx / N] + fc[idx % N] - 2 * W;
+      for (int q = top; q >= 0 && q >= top - 5; q -= 2) {
+        uint64_t wv = (q >= 1) ? (((uint64_t)v[q] << 32) | v[q - 1]) : (uint64_t)v[q];
+        int sh = 32 * ((q >= 1) ? q - 1 : q) + base;
+        dd part = u64_to_dd(wv);
+        part.hi = __builtin_ldexp(part.hi, sh);
+        part.lo = __builtin_ldexp(part.lo, sh);
+        res = dd_add(res, part);
+      }
+      if (neg) res = dd_neg(res);
+    }
+    chi[idx] = res.hi;
+    clo[idx] = res.lo;
+  }
+}
+
+// ------------------------------------------------------------------ workspace cache
+struct SplitWork {
+  std::mutex mu;
+  void* buf = nullptr;
+  size_t bytes = 0;
+  CrtTable* tab = nullptr;
+  int tabR = -1;
+};
+SplitWork g_work;
+
+}  // namespace
+
+int crt_bits_W(int R, size_t k) {
+  int lk = 0;
+  while ((1ull << lk) < (unsigned long long)std::max<size_t>(k, 1)) ++lk;
+  return (int)((std::floor(log2_M(R)) - lk - 4) / 2);
+}
+
+int launch_gemm_split_i8(int trans_a, const double* ahi, const double* alo, const double* bhi, const double* blo,
+                         double* chi, double* clo, size_t m, size_t n, size_t k, int R, hipStream_t s) {
+  if (m == 0 || n == 0) return HP_OK;
+  if (R <= 0) R = 31;
+  if (R < 8 || R > CRT_MAXR) {
+    set_error("split int8 gemm: moduli count %d outside [8, %d]", R, CRT_MAXR);
+    return HP_EINVAL;
+  }
+  if (k == 0) {
+    HP_HIP(hipMemsetAsync(chi, 0, m * n * sizeof(double), s));
+    HP_HIP(hipMemsetAsync(clo, 0, m * n * sizeof(double), s));
+    return HP_OK;
+  }
+  const int W = crt_bits_W(R, k);
+  if (W < 20 || W > 120) {
+    set_error("split int8 gemm: k = %zu too large for %d moduli", k, R);
+    return HP_EINVAL;
+  }
+  const size_t resA = (size_t)R * m * k, resB = (size_t)R * n * k, resC = (size_t)R * m * n;
+  const size_t need = resA + resB + resC + (m + n) * sizeof(int) + 1024;
+  std::lock_guard<std::mutex> lock(g_work.mu);
+  if (g_work.bytes < need) {
+    if (g_work.buf) (void)hipFree(g_work.buf);
+    g_work.buf = nullptr;
+    g_work.bytes = 0;
+    HP_HIP(hipMalloc(&g_work.buf, need));
+    g_work.bytes = need;
+  }
+  if (!g_work.tab) HP_HIP(hipMalloc(&g_work.tab, sizeof(CrtTable)));
+  if (g_work.tabR != R) {
+    static CrtTable host;
+    build_table(R, host);
+    HP_HIP(hipMemcpy(g_work.tab, &host, sizeof(CrtTable), hipMemcpyHostToDevice));
+    g_work.tabR = R;
+  }
+  int8_t* rA = static_cast<int8_t*>(g_work.buf);
+  int8_t* rB = rA + resA;
+  uint8_t* rC = reinterpret_cast<uint8_t*>(rB + resB);
+  int* eA = reinterpret_cast<int*>(((uintptr_t)(rC + resC) + 15) & ~(uintptr_t)15);
+  int* fB = eA + m;
+  // scales: rows of A (contiguous unless gemm_atb), columns of B
+  hipLaunchKernelGGL(k_fill_int, dim3((unsigned)((m + n + 255) / 256)), dim3(256), 0, s, eA, (long)(m + n), -100000);
+  HP_LAUNCH_CHECK();
+  const unsigned kch = (unsigned)((k + 255) / 256);
+  if (!trans_a)
+    hipLaunchKernelGGL(k_scale_rows, dim3((unsigned)((m + 3) / 4)), dim3(256), 0, s, ahi, (long)m, (long)k, (long)k, eA);
+  else
+    hipLaunchKernelGGL(k_scale_cols, dim3((unsigned)((m + 255) / 256), kch), dim3(256), 0, s, ahi, (long)m, (long)k,
+                       (long)m, eA);
+  HP_LAUNCH_CHECK();
+  hipLaunchKernelGGL(k_scale_cols, dim3((unsigned)((n + 255) / 256), kch), dim3(256), 0, s, bhi, (long)n, (long)k,
+                     (long)n, fB);
+  HP_LAUNCH_CHECK();
+  dim3 ga((unsigned)((k + 63) / 64), (unsigned)((m + 63) / 64));
+  if (!trans_a)
+    hipLaunchKernelGGL(k_residues<true>, ga, dim3(256), 0, s, ahi, alo, (long)m, (long)k, (long)k, eA, W, g_work.tab, rA);
+  else
+    hipLaunchKernelGGL(k_residues<false>, ga, dim3(256), 0, s, ahi, alo, (long)m, (long)k, (long)m, eA, W, g_work.tab, rA);
+  HP_LAUNCH_CHECK();
+  dim3 gb((unsigned)((k + 63) / 64), (unsigned)((n + 63) / 64));
+  hipLaunchKernelGGL(k_residues<false>, gb, dim3(256), 0, s, bhi, blo, (long)n, (long)k, (long)n, fB, W, g_work.tab, rB);
+  HP_LAUNCH_CHECK();
+  dim3 gg((unsigned)((n + GN - 1) / GN), (unsigned)((m + GM - 1) / GM), (unsigned)R);
+  hipLaunchKernelGGL(k_gemm_i8_mod, gg, dim3(256), 0, s, rA, rB, rC, (long)m, (long)n, (long)k, g_work.tab);
+  HP_LAUNCH_CHECK();
+  size_t total = m * n;
+  unsigned gc = (unsigned)std::min<size_t>((total + 255) / 256, 1u << 16);
+  hipLaunchKernelGGL(k_crt, dim3(gc), dim3(256), 0, s, rC, (long)m, (long)n, eA, fB, W, g_work.tab, chi, clo);
+  HP_LAUNCH_CHECK();
+  // the workspace is reused by the next call: keep it owned until this one has run
+  HP_HIP(hipStreamSynchronize(s));
+  return HP_OK;
+}
+
+}  // namespace hp

@@ -1,0 +1,121 @@
+"""Split-precision dd GEMM (INT8 matrix cores + CRT) against the f256 oracle.
+
+The oracle is the MultiFloat<4> GEMM restatement (crates/quad/src/multifloat/linalg.rs:39-61
+on dd inputs promoted with from_f128, multifloat/conv.rs:120-131), accurate to ~2^-200.
+The bar is componentwise: |C - C*|_ij <= tol * (|A||B|)_ij with tol = 2^-100, and the
+split result must be at least as accurate as the plain dd GEMM on the same inputs up to a
+factor 4 (both are measured, the worst ratios are printed)."""
+import numpy as np
+import pytest
+
+import oracle_lib as O
+
+pytestmark = pytest.mark.gpu
+TOL = 2.0 ** -100
+
+
+@pytest.fixture(scope="module")
+def qp():
+    import quadprec_amd
+    quadprec_amd.lib()
+    return quadprec_amd
+
+
+def to_mf4(x):
+    out = np.zeros((x.shape[0], 4))
+    out[:, :2] = x
+    return out
+
+
+def dev_gemm(qp, a, b, m, n, k, mode, trans_a=False, slices=0):
+    import torch
+    da, db = qp.dev.to_planes(a, 2), qp.dev.to_planes(b, 2)
+    dc = torch.empty((2, m * n), dtype=torch.float64, device="cuda")
+    qp.dev.gemm(da, db, dc, m, n, k, trans_a=trans_a, mode=mode, slices=slices)
+    torch.cuda.synchronize()
+    return qp.dev.from_planes(dc)
+
+
+def comp_err(c, ref4, absab):
+    """max_ij |c - ref| / (|A||B|)_ij, with ref in 4 limbs."""
+    d = np.zeros(c.shape[0])
+    # (c.hi - r0) + (c.lo - r1) - r2 - r3, each difference is exact or nearly so
+    d = ((c[:, 0] - ref4[:, 0]) - ref4[:, 1] + c[:, 1]) - ref4[:, 2] - ref4[:, 3]
+    return float(np.max(np.abs(d) / np.maximum(absab, 1e-300)))
+
+
+def absprod(a, b, m, n, k, trans_a=False):
+    A = np.abs(a[:, 0]).reshape((k, m) if trans_a else (m, k))
+    if trans_a:
+        A = A.T
+    return (A @ np.abs(b[:, 0]).reshape(k, n)).ravel()
+
+
+@pytest.mark.parametrize("m,n,k", [(64, 64, 64), (100, 37, 513), (200, 130, 300), (1, 300, 17), (257, 5, 4)])
+def test_split_i8_vs_f256(qp, m, n, k):
+    a = qp.gen_dd_normal(31 + m, m * k)
+    b = qp.gen_dd_normal(32 + n, k * n)
+    ref = O.mf4_gemm(to_mf4(a), to_mf4(b), m, n, k)
+    ab = absprod(a, b, m, n, k)
+    c_split = dev_gemm(qp, a, b, m, n, k, qp.HP_GEMM_SPLIT_I8)
+    c_plain = dev_gemm(qp, a, b, m, n, k, qp.HP_GEMM_PLAIN)
+    e_split, e_plain = comp_err(c_split, ref, ab), comp_err(c_plain, ref, ab)
+    print(f"m={m} n={n} k={k}: split {e_split:.3e}  plain {e_plain:.3e}")
+    assert e_split <= TOL
+    assert e_split <= max(4 * e_plain, 2.0 ** -104)
+
+
+def test_split_i8_gemm_atb_and_dynamic_range(qp):
+    m, n, k = 90, 70, 150
+    a = qp.gen_dd_normal(5, k * m)   # stored k x m
+    b = qp.gen_dd_normal(6, k * n)
+    a[::7] *= 1e-150                 # rows of wildly different magnitude
+    b[3::11] *= 1e120
+    a[k * 0:k * 0 + m] = 0.0         # one zero row of the stored matrix
+    at = a.reshape(k, m, 2).transpose(1, 0, 2).reshape(m * k, 2)
+    ref = O.mf4_gemm(to_mf4(at), to_mf4(b), m, n, k)
+    ab = absprod(a, b, m, n, k, trans_a=True)
+    c = dev_gemm(qp, a, b, m, n, k, qp.HP_GEMM_SPLIT_I8, trans_a=True)
+    assert comp_err(c, ref, ab) <= TOL
+
+
+def test_split_i8_exact_on_small_integers(qp):
+    # integer inputs well inside W bits: the split product is exact
+    m, n, k = 33, 47, 129
+    rng = np.random.default_rng(1)
+    a = np.zeros((m * k, 2))
+    b = np.zeros((k * n, 2))
+    a[:, 0] = rng.integers(-1000, 1000, m * k)
+    b[:, 0] = rng.integers(-1000, 1000, k * n)
+    c = dev_gemm(qp, a, b, m, n, k, qp.HP_GEMM_SPLIT_I8)
+    ex = (a[:, 0].reshape(m, k).astype(np.int64) @ b[:, 0].reshape(k, n).astype(np.int64)).ravel()
+    assert np.array_equal(c[:, 0], ex.astype(np.float64)) and not c[:, 1].any()
+
+
+@pytest.mark.parametrize("slices", [24, 28, 32])
+def test_split_i8_slices_tradeoff(qp, slices):
+    m, n, k = 64, 64, 256
+    a = qp.gen_dd_normal(7, m * k)
+    b = qp.gen_dd_normal(8, k * n)
+    ref = O.mf4_gemm(to_mf4(a), to_mf4(b), m, n, k)
+    c = dev_gemm(qp, a, b, m, n, k, qp.HP_GEMM_SPLIT_I8, slices=slices)
+    e = comp_err(c, ref, absprod(a, b, m, n, k))
+    # W = floor((log2 M - 12 - 4) / 2) bits per input: 24 -> ~82, 28 -> ~96, 32 -> ~112
+    W = {24: 82, 28: 96, 32: 112}[slices]
+    assert e <= 2.0 ** (4 - W)
+
+
+def test_split_i8_full_size_sampled(qp):
+    """4096^3 split GEMM: a sample of 8 rows x 256 columns against the f256 oracle."""
+    m = n = k = 4096
+    a = qp.gen_dd_normal(1, m * k)
+    b = qp.gen_dd_normal(1 + 0x5EED, k * n)
+    c = dev_gemm(qp, a, b, m, n, k, qp.HP_GEMM_SPLIT_I8).reshape(m, n, 2)
+    rows = np.array([0, 511, 1024, 2047, 2048, 3000, 4000, 4095])
+    cols = np.arange(0, 4096, 16)
+    a_s = a.reshape(m, k, 2)[rows].reshape(-1, 2)
+    b_s = np.ascontiguousarray(b.reshape(k, n, 2)[:, cols]).reshape(-1, 2)
+    ref = O.mf4_gemm(to_mf4(a_s), to_mf4(b_s), len(rows), len(cols), k)
+    got = np.ascontiguousarray(c[rows][:, cols]).reshape(-1, 2)
+    ab = absprod(a_s, b_s, len(rows), len(cols), k)
+    assert comp_err(got, ref, ab) <= TOL
