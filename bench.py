@@ -1,16 +1,18 @@
 """Benchmark of the extended-precision hot path on one MI355X.
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W]
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu]
 
 A step is one double-double GEMM C = A*B at m = n = k = 4096 (hi limbs N(0,1),
-lo = hi*U(-2^-53, 2^-53), seed 1) on operands already resident in HBM.  Prints ONE JSON
-line.  `value` is dd-GEMM GFLOP/s = 2*m*n*k / (seconds per step), counting one dd
-multiply-add as 2 flops.
+lo = hi*U(-2^-53, 2^-53), seed 1) on operands already resident in HBM, computed by the
+split-precision path (INT8 matrix cores, 31-modulus CRT; its accuracy against the f256
+oracle is established by tests/test_gpu_split.py).  The plain dd kernel (reference op
+order, bit-identical) is timed on the same operands for the split-vs-plain ratio.
 
-The `roofline` object describes the dominant kernel, timed with HIP events recorded on
-the stream the kernel runs on.  The `cpu_baseline` object times the oracle (the C++
-op-for-op restatement of the reference's gemm, crates/quad/src/linalg.rs:150-165) on a
-bounded sample of rows of the same product on the host cores.
+Prints ONE JSON line.  `value` is dd-GEMM GFLOP/s = 2*m*n*k / (seconds per step),
+counting one dd multiply-add as 2 flops.  The `roofline` object describes the dominant
+kernel of the step (the int8 GEMMs), timed with HIP events recorded on the stream it runs
+on.  The `cpu_baseline` object times the oracle (the C++ op-for-op restatement of the
+reference's gemm, crates/quad/src/linalg.rs:150-165) on a bounded sample of rows.
 """
 import argparse
 import json
@@ -22,7 +24,9 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 M = N = K = 4096
+SLICES = 31                      # CRT moduli = int8 GEMMs per dd GEMM
 FP64_PEAK_TFLOPS = 78.6          # MI355X FP64 vector (= FP64 matrix) dense peak
+INT8_PEAK_TOPS = 5000.0          # MI355X dense INT8 matrix peak
 DD_MAC_OPS = 29                  # FP64 ops per reference dd MAC (dd add 20 + dd mul 9)
 
 
@@ -45,6 +49,21 @@ def cpu_baseline(a, b, target_s=12.0):
             "sample": f"{rows} of {M} rows of the 4096^3 dd gemm ({t:.1f} s), oracle C++ restatement, OpenMP rows"}
 
 
+def time_steps(step, steps, warmup, stream, torch):
+    for _ in range(warmup):
+        step()
+    torch.cuda.synchronize()
+    ev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(steps)]
+    t0 = time.perf_counter()
+    for s, e in ev:
+        s.record(stream)
+        step()
+        e.record(stream)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    return wall * 1e3 / steps, sum(s.elapsed_time(e) for s, e in ev) / steps
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -57,7 +76,6 @@ def main():
               "the multi-GPU layout is described in DESIGN.md and is not built.", file=sys.stderr)
         sys.exit(2)
 
-    import numpy as np
     import torch
     import quadprec_amd as qp
 
@@ -66,46 +84,59 @@ def main():
     b = qp.gen_dd_normal(1 + 0x5EED, K * N)
     da, db = qp.dev.to_planes(a, 2, dev), qp.dev.to_planes(b, 2, dev)
     dc = torch.empty((2, M * N), dtype=torch.float64, device=dev)
+    stream = torch.cuda.current_stream()
+    macs = M * N * K
 
-    def step():
+    # split path (headline); per-phase HIP events on the launch stream
+    qp.set_phase_timing(True)
+    phases = []
+
+    def split_step():
+        qp.dev.gemm(da, db, dc, M, N, K, mode=qp.HP_GEMM_SPLIT_I8, slices=SLICES)
+        phases.append(qp.phase_times())
+
+    ms_split, kern_split = time_steps(split_step, args.steps, args.warmup, stream, torch)
+    qp.set_phase_timing(False)
+    timed = phases[args.warmup:]
+    ph = [sum(p[i] for p in timed) / len(timed) for i in range(len(timed[0]))]  # prep, int8 gemms, crt
+    gemm_ms = ph[1]
+    achieved = 2.0 * SLICES * macs / (gemm_ms * 1e-3) / 1e12
+
+    # plain path (reference op order) on the same operands
+    def plain_step():
         qp.dev.gemm(da, db, dc, M, N, K, mode=qp.HP_GEMM_PLAIN)
 
-    for _ in range(args.warmup):
-        step()
-    torch.cuda.synchronize()
-    stream = torch.cuda.current_stream()
-    ev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(args.steps)]
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    for s, e in ev:
-        s.record(stream)
-        step()
-        e.record(stream)
-    torch.cuda.synchronize()
-    wall = time.perf_counter() - t0
-    kern_ms = sum(s.elapsed_time(e) for s, e in ev) / len(ev)
-    ms_per_step = wall * 1e3 / args.steps
-    macs = M * N * K
-    value = 2.0 * macs / (ms_per_step * 1e-3) / 1e9
-    achieved = 2.0 * DD_MAC_OPS * macs / (kern_ms * 1e-3) / 1e12
+    ms_plain, kern_plain = time_steps(plain_step, max(2, min(args.steps, 3)), 1, stream, torch)
+    plain_ach = 2.0 * DD_MAC_OPS * macs / (kern_plain * 1e-3) / 1e12
+
+    value = 2.0 * macs / (ms_split * 1e-3) / 1e9
     out = {
-        "metric": "dd-GEMM GFLOP/s @4096^3 (plain dd, reference op order)",
+        "metric": "dd-GEMM GFLOP/s @4096^3 (split precision)",
         "value": round(value, 3),
         "unit": "GFLOP/s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": round(ms_per_step, 4),
+        "ms_per_step": round(ms_split, 4),
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64 (double-double)",
+        "dtype": "int8 residues (31-modulus CRT) -> double-double",
         "data": "synthetic: seeded N(0,1) dd operands, generated on the host and resident in HBM",
-        "config": {"workload": "dd gemm C=A*B, m=n=k=4096, row-major, seed 1", "parallelism": "1 GPU"},
-        "roofline": {"bound": "valu", "kernel": "k_gemm_dd_plain", "achieved": round(achieved, 3),
-                     "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s", "frac": round(achieved / FP64_PEAK_TFLOPS, 4),
-                     "traffic": None, "kernel_ms": round(kern_ms, 4),
-                     "note": "29 FP64 VALU ops per dd MAC counted as FMA-equivalent flops (2 each)"},
+        "config": {"workload": "dd gemm C=A*B, m=n=k=4096, row-major, seed 1; split = 31 int8 GEMMs + CRT",
+                   "parallelism": "1 GPU"},
+        "roofline": {"bound": "mfma", "kernel": "k_gemm_i8_mod", "achieved": round(achieved, 2),
+                     "peak": INT8_PEAK_TOPS, "unit": "TOP/s (int8)", "frac": round(achieved / INT8_PEAK_TOPS, 4),
+                     "traffic": None, "kernel_ms": round(gemm_ms, 4),
+                     "k_stage": qp.split_gemm_variant(M, N, K, SLICES),
+                     "phases_ms": {"scale_and_residues": round(ph[0], 4), "int8_gemms": round(ph[1], 4),
+                                   "crt": round(ph[2], 4)}},
+        "plain": {"value": round(2.0 * macs / (ms_plain * 1e-3) / 1e9, 3), "unit": "GFLOP/s",
+                  "ms_per_step": round(ms_plain, 4), "kernel": "k_gemm_dd_plain",
+                  "roofline": {"bound": "valu", "achieved": round(plain_ach, 3), "peak": FP64_PEAK_TFLOPS,
+                               "unit": "TFLOP/s", "frac": round(plain_ach / FP64_PEAK_TFLOPS, 4),
+                               "note": "29 FP64 VALU ops per dd MAC, FMA-equivalent (2 flops each)"}},
+        "split_over_plain": round(ms_plain / ms_split, 2),
     }
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(a, b)

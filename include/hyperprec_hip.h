@@ -149,6 +149,20 @@ int hp_solve_spd_f128(const double* a_dd, const double* b_dd, size_t n, hp_solut
 int hp_overlap_s(const double* centers, const double* alpha, size_t n, double* s_dd);
 int hp_dev_overlap_s(const double* centers, const double* alpha, size_t n, double* s_hi, double* s_lo, void* stream);
 
+/* ------------------------------------------------------------------ measurement */
+
+/* Per-phase HIP-event timing of the multi-kernel entry points (no reference
+ * counterpart; for benchmarks).  When on, the next split GEMM records events on its
+ * stream at its phase boundaries (scaling + residues | int8 GEMMs | CRT) and
+ * hp_get_phase_times copies the elapsed milliseconds, returning the phase count. */
+void hp_set_phase_timing(int on);
+/* K-stage depth the split GEMM's autotuner chose for this shape class (0 = not tuned yet).
+ * The first split GEMM of a shape class times each variant on its real operands. */
+int hp_split_gemm_variant(size_t m, size_t n, size_t k, int slices);
+/* Pins the K-stage depth (32, 64 or 128) for a shape class; 0 clears it (re-tune). */
+int hp_split_gemm_set_variant(size_t m, size_t n, size_t k, int slices, int variant);
+int hp_get_phase_times(double* ms, int max);
+
 /* ------------------------------------------------------------------ synthetic inputs */
 
 /* Deterministic workload generators (host memory).  SplitMix64 stream from `seed`;

@@ -95,6 +95,14 @@ def lib():
     L.hp_dev_overlap_s.argtypes = [_vp, _vp, _sz, _vp, _vp, _vp]
     L.hp_gen_overlap_basis.argtypes = [C.c_uint64, _sz, C.c_double, C.c_double, C.c_double, C.c_double, _dp, _dp]
     L.hp_gen_overlap_basis.restype = None
+    L.hp_split_gemm_variant.argtypes = [_sz, _sz, _sz, C.c_int]
+    L.hp_split_gemm_variant.restype = C.c_int
+    L.hp_split_gemm_set_variant.argtypes = [_sz, _sz, _sz, C.c_int, C.c_int]
+    L.hp_split_gemm_set_variant.restype = C.c_int
+    L.hp_set_phase_timing.argtypes = [C.c_int]
+    L.hp_set_phase_timing.restype = None
+    L.hp_get_phase_times.argtypes = [_dp, C.c_int]
+    L.hp_get_phase_times.restype = C.c_int
     L.hp_gen_dd_normal.argtypes = [C.c_uint64, _sz, _dp]
     L.hp_gen_dd_normal.restype = None
     L.hp_gen_mf4_normal.argtypes = [C.c_uint64, _sz, _dp]
@@ -254,6 +262,28 @@ def gen_overlap_basis(seed, n, alpha_lo=0.01, alpha_hi=10.0, spacing=1.4, jitter
     a = np.empty(n)
     lib().hp_gen_overlap_basis(seed, n, alpha_lo, alpha_hi, spacing, jitter, c, a)
     return c, a
+
+
+# ---------------------------------------------------------------- measurement
+def set_phase_timing(on):
+    lib().hp_set_phase_timing(int(bool(on)))
+
+
+def split_gemm_variant(m, n, k, slices=0):
+    """K-stage depth the split GEMM autotuner chose for this shape class (0 = untuned)."""
+    return lib().hp_split_gemm_variant(m, n, k, slices)
+
+
+def split_gemm_set_variant(m, n, k, variant, slices=0):
+    """Pin the split GEMM's K-stage depth (32, 64, 128) for a shape class; 0 re-tunes."""
+    _check(lib().hp_split_gemm_set_variant(m, n, k, slices, variant), "split_gemm_set_variant")
+
+
+def phase_times():
+    """Milliseconds of each phase of the last multi-kernel call on this thread."""
+    buf = np.zeros(8)
+    n = lib().hp_get_phase_times(buf, 8)
+    return list(buf[:n])
 
 
 # ---------------------------------------------------------------- generators

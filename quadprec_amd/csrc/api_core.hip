@@ -65,6 +65,24 @@ int download_planes(const double* dev_planes, size_t count, int limbs, double* h
   return HP_OK;
 }
 
+static thread_local PhaseTimer g_timer;
+PhaseTimer& phase_timer() { return g_timer; }
+void PhaseTimer::mark(hipStream_t s) {
+  if (!on || n > MAXP) return;
+  if (!ev[n]) (void)hipEventCreate(&ev[n]);
+  (void)hipEventRecord(ev[n], s);
+  ++n;
+}
+void PhaseTimer::finish() {
+  if (!on) return;
+  for (int i = 0; i + 1 < n; ++i) {
+    float t = 0.f;
+    (void)hipEventSynchronize(ev[i + 1]);
+    (void)hipEventElapsedTime(&t, ev[i], ev[i + 1]);
+    ms[i] = t;
+  }
+}
+
 }  // namespace hp
 
 using namespace hp;
@@ -123,6 +141,19 @@ struct Normal {
 extern "C" {
 
 const char* hp_last_error(void) { return hp::g_err; }
+
+int hp_split_gemm_variant(size_t m, size_t n, size_t k, int slices) { return split_gemm_variant(m, n, k, slices); }
+int hp_split_gemm_set_variant(size_t m, size_t n, size_t k, int slices, int variant) {
+  return set_split_gemm_variant(m, n, k, slices, variant);
+}
+
+void hp_set_phase_timing(int on) { phase_timer().on = on != 0; }
+int hp_get_phase_times(double* ms, int max) {
+  PhaseTimer& t = phase_timer();
+  int cnt = t.n > 0 ? t.n - 1 : 0;
+  for (int i = 0; i < cnt && i < max; ++i) ms[i] = t.ms[i];
+  return cnt;
+}
 int hp_version(void) { return 100; }
 
 int hp_gemm(const double* a, const double* b, double* c, size_t m, size_t n, size_t k) {

@@ -55,6 +55,19 @@ struct DevBuf {
   double* d() const { return static_cast<double*>(p); }
 };
 
+// Optional per-phase HIP-event timing of multi-kernel entry points (hp_set_phase_timing).
+// Phase boundaries are recorded on the stream the kernels run on.
+struct PhaseTimer {
+  static constexpr int MAXP = 8;
+  bool on = false;
+  int n = 0;
+  hipEvent_t ev[MAXP + 1] = {};
+  double ms[MAXP] = {};
+  void mark(hipStream_t s);  // record the next boundary (no-op when off)
+  void finish();             // after synchronisation: elapsed times between boundaries
+};
+PhaseTimer& phase_timer();
+
 // AoS <-> SoA conversion for the host entry points (the host array is a Rust &[f128]
 // layout: hi, lo interleaved).  `limbs` is 2 for dd, 4 for MF4.
 int upload_planes(const double* host_aos, size_t count, int limbs, double* dev_planes, hipStream_t s);
@@ -67,6 +80,8 @@ int launch_gemm_dd_plain(int trans_a, const double* ahi, const double* alo, cons
 int launch_gemm_split_i8(int trans_a, const double* ahi, const double* alo, const double* bhi, const double* blo,
                          double* chi, double* clo, size_t m, size_t n, size_t k, int R, hipStream_t s);
 int crt_bits_W(int R, size_t k);
+int split_gemm_variant(size_t m, size_t n, size_t k, int R);
+int set_split_gemm_variant(size_t m, size_t n, size_t k, int R, int variant);
 int launch_mf4_gemm(int trans_a, const double* a, const double* b, double* c, size_t m, size_t n,
                     size_t k, hipStream_t s);
 

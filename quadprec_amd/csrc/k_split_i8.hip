@@ -21,6 +21,7 @@
 #include <cmath>
 #include <cstdint>
 #include <cstring>
+#include <map>
 #include <mutex>
 #include <vector>
 
@@ -224,17 +225,35 @@ __global__ __launch_bounds__(256) void k_residues(const double* __restrict__ hi,
 
 // ------------------------------------------------------------------ int8 GEMM mod m
 // Cr[i][j] = (sum_p Ar[i][p] * Br[j][p]) mod m_r for every modulus r = blockIdx.z.
-// 256 x 256 block tile, 4 wavefronts of 128 x 128 (4 x 4 tiles of v_mfma_i32_32x32x32_i8),
-// K in 32-deep stages double-buffered through LDS.
+// 256 x 256 block tile, 4 wavefronts of 128 x 128 (4 x 4 tiles of v_mfma_i32_32x32x32_i8,
+// accumulators in AGPRs); K advances in GK-deep stages double-buffered through LDS, the
+// next stage fetched from HBM into registers while the current one feeds the MFMAs.
+// Operand k-slot layout is the same for A and B, so only the output layout matters:
+// lane l holds column l%32, register q holds row 8*(q/4) + 4*(l/32) + q%4.
 typedef int v4i __attribute__((ext_vector_type(4)));
 typedef int v16i __attribute__((ext_vector_type(16)));
-constexpr int GM = 256, GN = 256, GK = 32, GKP = GK + 16;
+constexpr int GM = 256, GN = 256;
 
-__global__ __launch_bounds__(256) void k_gemm_i8_mod(const int8_t* __restrict__ A, const int8_t* __restrict__ B,
+// 16 bytes at p, bytes at index >= cnt replaced by 0 (static indexing only)
+__device__ __forceinline__ v4i load16_guarded(const int8_t* p, long cnt) {
+  v4i x = {0, 0, 0, 0};
+#pragma unroll
+  for (int q = 0; q < 16; ++q) {
+    uint32_t b = (q < cnt) ? (uint8_t)p[q] : 0u;
+    x[q >> 2] |= (int)(b << (8 * (q & 3)));
+  }
+  return x;
+}
+
+template <int GK>
+__global__ __launch_bounds__(512) void k_gemm_i8_mod(const int8_t* __restrict__ A, const int8_t* __restrict__ B,
                                                      uint8_t* __restrict__ C, long M, long N, long K,
                                                      const CrtTable* __restrict__ tab) {
-  __shared__ __align__(16) int8_t As[2][GM][GKP];
-  __shared__ __align__(16) int8_t Bs[2][GN][GKP];
+  constexpr int GKP = GK + 16;   // row pitch: keeps the 16-lane groups of ds_read_b128 on distinct banks
+  constexpr int NV = GK / 16;    // 16-byte vectors per row per stage
+  extern __shared__ __align__(16) int8_t smem[];
+  int8_t (*As)[GM][GKP] = reinterpret_cast<int8_t (*)[GM][GKP]>(smem);
+  int8_t (*Bs)[GN][GKP] = reinterpret_cast<int8_t (*)[GN][GKP]>(smem + 2 * GM * GKP);
   const int r = blockIdx.z;
   const uint32_t m = tab->mod[r];
   const double inv_m = tab->inv_m[r];
@@ -242,94 +261,88 @@ __global__ __launch_bounds__(256) void k_gemm_i8_mod(const int8_t* __restrict__ 
   B += (size_t)r * N * K;
   C += (size_t)r * M * N;
   const int t = threadIdx.x, lane = t & 63, w = t >> 6;
-  const int wm = w >> 1, wn = w & 1;
+  const int wm = w >> 1, wn = w & 1;  // 8 wavefronts: 4 x 2, each 64 rows x 128 columns
   const long i0 = (long)blockIdx.y * GM, j0 = (long)blockIdx.x * GN;
 
-  v16i acc[4][4];
+  v16i acc[2][4];
 #pragma unroll
-  for (int a = 0; a < 4; ++a)
+  for (int a = 0; a < 2; ++a)
 #pragma unroll
     for (int b = 0; b < 4; ++b)
 #pragma unroll
       for (int q = 0; q < 16; ++q) acc[a][b][q] = 0;
 
-  // each thread stages one 32-byte row segment of A and of B per stage
   const bool kvec = (K % 16) == 0;
-  auto load_row = [&](const int8_t* base, long rowg, long nrows, long k0, v4i& x0, v4i& x1) {
-    x0 = v4i{0, 0, 0, 0};
-    x1 = v4i{0, 0, 0, 0};
-    if (rowg >= nrows) return;
-    const int8_t* p = base + rowg * K + k0;
-    if (kvec && k0 + GK <= K) {
-      x0 = *reinterpret_cast<const v4i*>(p);
-      x1 = *reinterpret_cast<const v4i*>(p + 16);
-    } else {
-      int8_t tmp[32];
-      for (int q = 0; q < 32; ++q) tmp[q] = (k0 + q < K) ? p[q] : (int8_t)0;
-      __builtin_memcpy(&x0, tmp, 16);
-      __builtin_memcpy(&x1, tmp + 16, 16);
+  // threads 0..255 stage row t of the A tile, threads 256..511 row t-256 of the B tile
+  const bool isA = t < 256;
+  const long srow = (isA ? i0 : j0) + (t & 255);
+  const long slim = isA ? M : N;
+  const int8_t* ps = (isA ? A : B) + srow * K;
+  int8_t (*Ss)[GM][GKP] = isA ? As : Bs;
+  v4i rs[NV];
+  auto fetch = [&](long k0) {
+#pragma unroll
+    for (int v = 0; v < NV; ++v) {
+      long kk = k0 + 16 * v;
+      if (srow < slim && kvec && kk + 16 <= K) rs[v] = *reinterpret_cast<const v4i*>(ps + kk);
+      else rs[v] = (srow < slim && kk < K) ? load16_guarded(ps + kk, K - kk) : v4i{0, 0, 0, 0};
     }
   };
-  v4i ra0, ra1, rb0, rb1;
   const long nst = (K + GK - 1) / GK;
-  load_row(A, i0 + t, M, 0, ra0, ra1);
-  load_row(B, j0 + t, N, 0, rb0, rb1);
-  const long KCH = 65536;  // reduce partial sums mod m before |sum| could reach 2^31
+  fetch(0);
+  const long KCH = 65536;  // partial sums are reduced mod m before |sum| could reach 2^31
   for (long st = 0; st < nst; ++st) {
     const int buf = st & 1;
-    *reinterpret_cast<v4i*>(&As[buf][t][0]) = ra0;
-    *reinterpret_cast<v4i*>(&As[buf][t][16]) = ra1;
-    *reinterpret_cast<v4i*>(&Bs[buf][t][0]) = rb0;
-    *reinterpret_cast<v4i*>(&Bs[buf][t][16]) = rb1;
+#pragma unroll
+    for (int v = 0; v < NV; ++v) *reinterpret_cast<v4i*>(&Ss[buf][t & 255][16 * v]) = rs[v];
     __syncthreads();
-    if (st + 1 < nst) {
-      load_row(A, i0 + t, M, (st + 1) * GK, ra0, ra1);
-      load_row(B, j0 + t, N, (st + 1) * GK, rb0, rb1);
+    if (st + 1 < nst) fetch((st + 1) * GK);
+#pragma unroll
+    for (int ks = 0; ks < GK; ks += 32) {
+      const int kq = ks + 16 * (lane >> 5);
+      v4i af[2], bf[4];
+#pragma unroll
+      for (int a = 0; a < 2; ++a) af[a] = *reinterpret_cast<const v4i*>(&As[buf][wm * 64 + a * 32 + (lane & 31)][kq]);
+#pragma unroll
+      for (int b = 0; b < 4; ++b) bf[b] = *reinterpret_cast<const v4i*>(&Bs[buf][wn * 128 + b * 32 + (lane & 31)][kq]);
+#pragma unroll
+      for (int a = 0; a < 2; ++a)
+#pragma unroll
+        for (int b = 0; b < 4; ++b) acc[a][b] = __builtin_amdgcn_mfma_i32_32x32x32_i8(af[a], bf[b], acc[a][b], 0, 0, 0);
     }
-    v4i af[4], bf[4];
-    const int kq = 16 * (lane >> 5);
-#pragma unroll
-    for (int a = 0; a < 4; ++a) af[a] = *reinterpret_cast<const v4i*>(&As[buf][wm * 128 + a * 32 + (lane & 31)][kq]);
-#pragma unroll
-    for (int b = 0; b < 4; ++b) bf[b] = *reinterpret_cast<const v4i*>(&Bs[buf][wn * 128 + b * 32 + (lane & 31)][kq]);
-#pragma unroll
-    for (int a = 0; a < 4; ++a)
-#pragma unroll
-      for (int b = 0; b < 4; ++b) acc[a][b] = __builtin_amdgcn_mfma_i32_32x32x32_i8(af[a], bf[b], acc[a][b], 0, 0, 0);
     if (((st + 1) * GK) % KCH == 0 && st + 1 < nst) {
 #pragma unroll
-      for (int a = 0; a < 4; ++a)
+      for (int a = 0; a < 2; ++a)
 #pragma unroll
         for (int b = 0; b < 4; ++b)
-#pragma unroll 1
+#pragma unroll
           for (int q = 0; q < 16; ++q) {
             int v = acc[a][b][q];
-            int32_t rr = v - (int32_t)m * (int32_t)__builtin_floor((double)v * inv_m);
-            acc[a][b][q] = rr;
+            acc[a][b][q] = v - (int32_t)m * (int32_t)__builtin_floor((double)v * inv_m);
           }
     }
-    // the buffer written next stage is the other one; this barrier orders its reuse
-    __syncthreads();
+    __syncthreads();  // the other buffer is overwritten next stage
   }
-  // epilogue: residue in [0, m) as a byte.  32x32 MFMA output: lane l holds column l%32,
-  // register q holds row 8*(q/4) + 4*(l/32) + q%4.
 #pragma unroll
-  for (int a = 0; a < 4; ++a)
+  for (int a = 0; a < 2; ++a)
 #pragma unroll
-    for (int b = 0; b < 4; ++b)
+    for (int b = 0; b < 4; ++b) {
+      const long gj = j0 + wn * 128 + b * 32 + (lane & 31);
+      const long gib = i0 + wm * 64 + a * 32 + 4 * (lane >> 5);
 #pragma unroll
       for (int q = 0; q < 16; ++q) {
-        long gi = i0 + wm * 128 + a * 32 + 8 * (q >> 2) + 4 * (lane >> 5) + (q & 3);
-        long gj = j0 + wn * 128 + b * 32 + (lane & 31);
-        if (gi < M && gj < N) {
-          int v = acc[a][b][q];
-          int32_t rr = v - (int32_t)m * (int32_t)__builtin_floor((double)v * inv_m);
-          if (rr < 0) rr += m;
-          if (rr >= (int32_t)m) rr -= m;
-          C[gi * N + gj] = (uint8_t)rr;
-        }
+        long gi = gib + 8 * (q >> 2) + (q & 3);
+        int v = acc[a][b][q];
+        int32_t rr = v - (int32_t)m * (int32_t)__builtin_floor((double)v * inv_m);
+        if (rr < 0) rr += m;
+        if (rr >= (int32_t)m) rr -= m;
+        if (gi < M && gj < N) C[gi * N + gj] = (uint8_t)rr;
       }
+    }
 }
+
+template <int GK>
+constexpr size_t gemm_i8_lds() { return (size_t)2 * (GM + GN) * (GK + 16); }
 
 // ------------------------------------------------------------------ CRT reconstruction
 // C'_ij = sum_r u_r (M/m_r) - q M, u_r = c_r y_r mod m_r, q = round(sum_r u_r / m_r);
@@ -423,8 +436,36 @@ struct SplitWork {
   int tabR = -1;
 };
 SplitWork g_work;
+std::map<uint64_t, int> g_tuned;  // autotuned K-stage depth per shape class (under g_work.mu)
+
+uint64_t shape_key(size_t m, size_t n, size_t k, int R) {
+  auto bucket = [](size_t v) {  // next power of two
+    uint64_t b = 1;
+    while (b < v) b <<= 1;
+    return (uint64_t)__builtin_ctzll(b);
+  };
+  return (bucket(m) << 24) | (bucket(n) << 16) | (bucket(k) << 8) | (uint64_t)R;
+}
 
 }  // namespace
+
+int set_split_gemm_variant(size_t m, size_t n, size_t k, int R, int variant) {
+  if (variant != 0 && variant != 32 && variant != 64 && variant != 128) {
+    set_error("split gemm variant must be 0 (autotune), 32, 64 or 128");
+    return HP_EINVAL;
+  }
+  std::lock_guard<std::mutex> lock(g_work.mu);
+  uint64_t key = shape_key(m, n, k, R <= 0 ? 31 : R);
+  if (variant == 0) g_tuned.erase(key);
+  else g_tuned[key] = variant;
+  return HP_OK;
+}
+
+int split_gemm_variant(size_t m, size_t n, size_t k, int R) {
+  std::lock_guard<std::mutex> lock(g_work.mu);
+  auto it = g_tuned.find(shape_key(m, n, k, R <= 0 ? 31 : R));
+  return it == g_tuned.end() ? 0 : it->second;
+}
 
 int crt_bits_W(int R, size_t k) {
   int lk = 0;
@@ -460,7 +501,15 @@ int launch_gemm_split_i8(int trans_a, const double* ahi, const double* alo, cons
     HP_HIP(hipMalloc(&g_work.buf, need));
     g_work.bytes = need;
   }
-  if (!g_work.tab) HP_HIP(hipMalloc(&g_work.tab, sizeof(CrtTable)));
+  if (!g_work.tab) {
+    HP_HIP(hipMalloc(&g_work.tab, sizeof(CrtTable)));
+    HP_HIP(hipFuncSetAttribute((const void*)k_gemm_i8_mod<128>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                               (int)gemm_i8_lds<128>()));
+    HP_HIP(hipFuncSetAttribute((const void*)k_gemm_i8_mod<64>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                               (int)gemm_i8_lds<64>()));
+    HP_HIP(hipFuncSetAttribute((const void*)k_gemm_i8_mod<32>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                               (int)gemm_i8_lds<32>()));
+  }
   if (g_work.tabR != R) {
     static CrtTable host;
     build_table(R, host);
@@ -472,6 +521,9 @@ int launch_gemm_split_i8(int trans_a, const double* ahi, const double* alo, cons
   uint8_t* rC = reinterpret_cast<uint8_t*>(rB + resB);
   int* eA = reinterpret_cast<int*>(((uintptr_t)(rC + resC) + 15) & ~(uintptr_t)15);
   int* fB = eA + m;
+  PhaseTimer& tm = phase_timer();
+  tm.n = 0;
+  tm.mark(s);
   // scales: rows of A (contiguous unless gemm_atb), columns of B
   hipLaunchKernelGGL(k_fill_int, dim3((unsigned)((m + n + 255) / 256)), dim3(256), 0, s, eA, (long)(m + n), -100000);
   HP_LAUNCH_CHECK();
@@ -494,15 +546,64 @@ int launch_gemm_split_i8(int trans_a, const double* ahi, const double* alo, cons
   dim3 gb((unsigned)((k + 63) / 64), (unsigned)((n + 63) / 64));
   hipLaunchKernelGGL(k_residues<false>, gb, dim3(256), 0, s, bhi, blo, (long)n, (long)k, (long)n, fB, W, g_work.tab, rB);
   HP_LAUNCH_CHECK();
+  tm.mark(s);
   dim3 gg((unsigned)((n + GN - 1) / GN), (unsigned)((m + GM - 1) / GM), (unsigned)R);
-  hipLaunchKernelGGL(k_gemm_i8_mod, gg, dim3(256), 0, s, rA, rB, rC, (long)m, (long)n, (long)k, g_work.tab);
-  HP_LAUNCH_CHECK();
+  auto launch = [&](int variant) -> int {
+    if (variant == 128)
+      hipLaunchKernelGGL(k_gemm_i8_mod<128>, gg, dim3(512), gemm_i8_lds<128>(), s, rA, rB, rC, (long)m, (long)n,
+                         (long)k, g_work.tab);
+    else if (variant == 32)
+      hipLaunchKernelGGL(k_gemm_i8_mod<32>, gg, dim3(512), gemm_i8_lds<32>(), s, rA, rB, rC, (long)m, (long)n,
+                         (long)k, g_work.tab);
+    else
+      hipLaunchKernelGGL(k_gemm_i8_mod<64>, gg, dim3(512), gemm_i8_lds<64>(), s, rA, rB, rC, (long)m, (long)n,
+                         (long)k, g_work.tab);
+    HP_LAUNCH_CHECK();
+    return HP_OK;
+  };
+  // autotuner: the first call for a shape class times every K-stage depth on the real
+  // operands (HIP events on this stream) and keeps the fastest
+  const uint64_t key = shape_key(m, n, k, R);  // R already defaulted
+  int variant = 0;
+  {
+    auto it = g_tuned.find(key);
+    if (it != g_tuned.end()) variant = it->second;
+  }
+  if (!variant) {
+    const int cand[3] = {32, 64, 128};
+    float best = 1e30f;
+    hipEvent_t e0, e1;
+    HP_HIP(hipEventCreate(&e0));
+    HP_HIP(hipEventCreate(&e1));
+    for (int c : cand) {
+      if (c > 32 && (long)k < c) continue;
+      int rc = launch(c);  // warm
+      if (rc) return rc;
+      HP_HIP(hipEventRecord(e0, s));
+      if ((rc = launch(c))) return rc;
+      HP_HIP(hipEventRecord(e1, s));
+      HP_HIP(hipEventSynchronize(e1));
+      float ms = 0.f;
+      HP_HIP(hipEventElapsedTime(&ms, e0, e1));
+      if (ms < best) { best = ms; variant = c; }
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    g_tuned[key] = variant;
+    tm.n = 0;  // tuning runs are not part of the timed phases
+    tm.mark(s);
+    tm.mark(s);
+  }
+  if (int rc = launch(variant)) return rc;
+  tm.mark(s);
   size_t total = m * n;
   unsigned gc = (unsigned)std::min<size_t>((total + 255) / 256, 1u << 16);
   hipLaunchKernelGGL(k_crt, dim3(gc), dim3(256), 0, s, rC, (long)m, (long)n, eA, fB, W, g_work.tab, chi, clo);
   HP_LAUNCH_CHECK();
+  tm.mark(s);
   // the workspace is reused by the next call: keep it owned until this one has run
   HP_HIP(hipStreamSynchronize(s));
+  tm.finish();
   return HP_OK;
 }
 

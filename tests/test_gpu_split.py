@@ -65,6 +65,24 @@ def test_split_i8_vs_f256(qp, m, n, k):
     assert e_split <= max(4 * e_plain, 2.0 ** -104)
 
 
+@pytest.mark.parametrize("variant", [32, 64, 128])
+def test_split_i8_every_k_stage_variant(qp, variant):
+    m, n, k = 300, 270, 640
+    a = qp.gen_dd_normal(41, m * k)
+    b = qp.gen_dd_normal(42, k * n)
+    qp.split_gemm_set_variant(m, n, k, variant)
+    try:
+        c = dev_gemm(qp, a, b, m, n, k, qp.HP_GEMM_SPLIT_I8)
+        assert qp.split_gemm_variant(m, n, k) == variant
+    finally:
+        qp.split_gemm_set_variant(m, n, k, 0)
+    rows = np.arange(0, m, 37)
+    a_s = a.reshape(m, k, 2)[rows].reshape(-1, 2)
+    ref = O.mf4_gemm(to_mf4(a_s), to_mf4(b), len(rows), n, k)
+    got = np.ascontiguousarray(c.reshape(m, n, 2)[rows]).reshape(-1, 2)
+    assert comp_err(got, ref, absprod(a_s, b, len(rows), n, k)) <= TOL
+
+
 def test_split_i8_gemm_atb_and_dynamic_range(qp):
     m, n, k = 90, 70, 150
     a = qp.gen_dd_normal(5, k * m)   # stored k x m
