@@ -140,6 +140,29 @@ int hp_solve_spd(const double* a, const double* b, size_t n, hp_solution* sol, d
 int hp_solve_spd_f128(const double* a_dd, const double* b_dd, size_t n, hp_solution* sol, double* x_dd,
                       double* x_f64);
 
+/* Mixed-precision SPD solve to a target accuracy (extends iterative_refinement,
+ * lib.rs:75-125, with a quad-double residual).  S and b are dd; x is returned in dd.
+ * The factor is chosen by `factor_kind`:
+ *   HP_FACTOR_F64       binary64 Cholesky of S.hi (needs kappa(S) * 2^-53 < 1)
+ *   HP_FACTOR_DD        dd cholesky_blocked(nb = 64), reference order on FP64 VALU
+ *   HP_FACTOR_DD_SPLIT  dd blocked Cholesky (nb = 256) with the trailing updates on the
+ *                       INT8 matrix cores (split GEMM)
+ * Iterates r = b - S x (qd), x += solve(L, r) until max|dx| <= tol * max|x| or max_iter. */
+enum hp_factor_kind { HP_FACTOR_F64 = 0, HP_FACTOR_DD = 1, HP_FACTOR_DD_SPLIT = 2 };
+typedef struct {
+  int factor_kind;
+  int iterations;
+  double factor_ms;        /* wall time of the factorisation (device-synchronised) */
+  double refine_ms;        /* wall time of the refinement loop */
+  double last_correction;  /* max|dx| / max|x| of the last step */
+  size_t pivot;            /* failing pivot on HP_ENOTSPD */
+} hp_refine_stats;
+int hp_dev_solve_spd_refined(int factor_kind, const double* s_hi, const double* s_lo, const double* b_hi,
+                             const double* b_lo, size_t n, double tol, int max_iter, double* x_hi, double* x_lo,
+                             hp_refine_stats* stats, void* stream);
+int hp_solve_spd_refined(int factor_kind, const double* s_dd, const double* b_dd, size_t n, double tol, int max_iter,
+                         double* x_dd, hp_refine_stats* stats);
+
 /* ------------------------------------------------------------------ overlap build */
 
 /* Normalised s-type Gaussian overlap in dd: S_uv = (4 a_u a_v/(a_u+a_v)^2)^{3/4}

@@ -42,6 +42,14 @@ class Solution(C.Structure):
                 ("refinement_iters", C.c_int), ("pivot", C.c_size_t)]
 
 
+class RefineStats(C.Structure):
+    _fields_ = [("factor_kind", C.c_int), ("iterations", C.c_int), ("factor_ms", C.c_double),
+                ("refine_ms", C.c_double), ("last_correction", C.c_double), ("pivot", C.c_size_t)]
+
+
+HP_FACTOR_F64, HP_FACTOR_DD, HP_FACTOR_DD_SPLIT = 0, 1, 2
+
+
 class Strategy:
     """crates/solver/src/lib.rs:25-30"""
     Float64, MixedPrecision, Float128 = 0, 1, 2
@@ -91,6 +99,9 @@ def lib():
     L.hp_cond_estimate.restype = C.c_double
     for name in ("hp_solve_spd", "hp_solve_spd_f128"):
         getattr(L, name).argtypes = [_dp, _dp, _sz, C.POINTER(Solution), _dp, _dp]
+    L.hp_dev_solve_spd_refined.argtypes = [C.c_int, _vp, _vp, _vp, _vp, _sz, C.c_double, C.c_int, _vp, _vp,
+                                           C.POINTER(RefineStats), _vp]
+    L.hp_solve_spd_refined.argtypes = [C.c_int, _dp, _dp, _sz, C.c_double, C.c_int, _dp, C.POINTER(RefineStats)]
     L.hp_overlap_s.argtypes = [_dp, _dp, _sz, _dp]
     L.hp_dev_overlap_s.argtypes = [_vp, _vp, _sz, _vp, _vp, _vp]
     L.hp_gen_overlap_basis.argtypes = [C.c_uint64, _sz, C.c_double, C.c_double, C.c_double, C.c_double, _dp, _dp]
@@ -242,6 +253,20 @@ def solve_spd(a, b, n):
 def solve_spd_f128(a, b, n):
     """solve_spd_f128 (lib.rs:71-73): dd A and b."""
     return _solve("hp_solve_spd_f128", a, b, n, 2)
+
+
+def solve_spd_refined(s, b, n, factor_kind=HP_FACTOR_DD_SPLIT, tol=1e-30, max_iter=20):
+    """Mixed-precision SPD solve with quad-double residual refinement (dd S and b).
+    Returns (x as (n, 2) dd, RefineStats)."""
+    s = _arr(s, 2, n * n, "s")
+    b = _arr(b, 2, n, "b")
+    x = np.zeros((n, 2))
+    st = RefineStats()
+    rc = lib().hp_solve_spd_refined(factor_kind, s, b, n, tol, max_iter, x, C.byref(st))
+    if rc == HP_ENOTSPD:
+        raise SolveError(st.pivot)
+    _check(rc, "solve_spd_refined")
+    return x, st
 
 
 # ---------------------------------------------------------------- overlap

@@ -397,6 +397,34 @@ int hp_solve_spd(const double* a, const double* b, size_t n, hp_solution* sol, d
   return rc;
 }
 
+int hp_dev_solve_spd_refined(int factor_kind, const double* s_hi, const double* s_lo, const double* b_hi,
+                             const double* b_lo, size_t n, double tol, int max_iter, double* x_hi, double* x_lo,
+                             hp_refine_stats* stats, void* stream) {
+  hp_refine_stats local;
+  if (!stats) stats = &local;
+  return solve_refined(factor_kind, s_hi, s_lo, b_hi, b_lo, (long)n, tol, max_iter, x_hi, x_lo, stats,
+                       (hipStream_t)stream);
+}
+
+int hp_solve_spd_refined(int factor_kind, const double* s_dd, const double* b_dd, size_t n, double tol, int max_iter,
+                         double* x_dd, hp_refine_stats* stats) {
+  hp_refine_stats local;
+  if (!stats) stats = &local;
+  std::memset(stats, 0, sizeof(*stats));
+  if (n == 0) return HP_OK;
+  int rc = check_ptrs(s_dd, b_dd, x_dd);
+  if (rc) return rc;
+  hipStream_t s = hipStreamPerThread;
+  DevBuf ds;
+  DDVec b, x;
+  if ((rc = ds.alloc(2 * n * n * sizeof(double))) || (rc = b.alloc((long)n)) || (rc = x.alloc((long)n))) return rc;
+  if ((rc = upload_planes(s_dd, n * n, 2, ds.d(), s)) || (rc = upload_planes(b_dd, n, 2, b.hi(), s))) return rc;
+  rc = solve_refined(factor_kind, ds.d(), ds.d() + n * n, b.hi(), b.lo(), (long)n, tol, max_iter, x.hi(), x.lo(),
+                     stats, s);
+  if (rc) return rc;
+  return download_planes(x.hi(), n, 2, x_dd, s);
+}
+
 int hp_overlap_s(const double* centers, const double* alpha, size_t n, double* s_dd) {
   if (n == 0) return HP_OK;
   int rc = check_ptrs(centers, alpha, s_dd);

@@ -79,7 +79,19 @@ int launch_gemm_dd_plain(int trans_a, const double* ahi, const double* alo, cons
                          hipStream_t s);
 int launch_gemm_split_i8(int trans_a, const double* ahi, const double* alo, const double* bhi, const double* blo,
                          double* chi, double* clo, size_t m, size_t n, size_t k, int R, hipStream_t s);
+// operand of the split GEMM: logical element (row, p) at hi[row*ld + p] when contig, else
+// hi[p*ld + row]; A is m x k, B is given as its transpose (n x k)
+struct SplitOperand {
+  const double* hi;
+  const double* lo;
+  long ld;
+  bool contig;
+};
+int split_gemm_general(const SplitOperand& A, const SplitOperand& B, double* chi, double* clo, size_t ldc, size_t m,
+                       size_t n, size_t k, int R, hipStream_t s);
 int crt_bits_W(int R, size_t k);
+int solve_refined(int kind, const double* sh, const double* sl, const double* bh, const double* bl, long n, double tol,
+                  int max_iter, double* xh, double* xl, hp_refine_stats* st, hipStream_t s);
 int split_gemm_variant(size_t m, size_t n, size_t k, int R);
 int set_split_gemm_variant(size_t m, size_t n, size_t k, int R, int variant);
 int launch_mf4_gemm(int trans_a, const double* a, const double* b, double* c, size_t m, size_t n,

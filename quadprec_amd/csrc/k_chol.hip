@@ -460,6 +460,35 @@ static int factor_unblocked_region(Mat a, Mat S, long r0, long len, ErrRec* err,
   return HP_OK;
 }
 
+// One block column of cholesky_blocked (linalg.rs:247-261): the diagonal block
+// [c0, c0+jb) by the unblocked algorithm, then trsm_right_lt on rows [c0+jb, n).
+template <class A>
+int chol_panel_exact(Mat a, Mat S, long c0, long jb, long n, ErrRec* err, double* inv_h, double* inv_l,
+                     hipStream_t s) {
+  int rc = ensure_attrs<A>();
+  if (rc) return rc;
+  if (jb <= DB) {
+    hipLaunchKernelGGL((k_diag_factor<A, false>), dim3(1), dim3(256), 2 * sizeof(Lds2<A>), s, a, S, c0, (int)jb, err,
+                       inv_h, inv_l);
+    HP_LAUNCH_CHECK();
+  } else if ((rc = factor_unblocked_region<A>(a, S, c0, jb, err, inv_h, inv_l, s))) {
+    return rc;
+  }
+  long rem = n - c0 - jb;
+  if (rem <= 0) return HP_OK;
+  for (long q0 = 0; q0 < jb; q0 += DB) {
+    int w = (int)std::min<long>(DB, jb - q0);
+    if (q0 > 0 && (rc = rank_update<A, RU_SUBINC, false>(a, S, c0 + jb, rem, c0 + q0, w, c0 + q0, c0, q0, err, s)))
+      return rc;
+    if (jb > DB) {
+      hipLaunchKernelGGL(k_diag_recip<A>, dim3(1), dim3(64), 0, s, a, c0 + q0, w, inv_h, inv_l);
+      HP_LAUNCH_CHECK();
+    }
+    if ((rc = row_recur<A, false>(a, S, c0 + q0, w, c0 + jb, n, inv_h, inv_l, err, s))) return rc;
+  }
+  return HP_OK;
+}
+
 template <class A>
 int chol_exact(Mat a, long n, long nb, bool unblocked, Mat S, ErrRec* err, double* inv_h, double* inv_l,
                hipStream_t s) {
@@ -470,30 +499,12 @@ int chol_exact(Mat a, long n, long nb, bool unblocked, Mat S, ErrRec* err, doubl
   if (unblocked || nb >= n) return factor_unblocked_region<A>(a, S, 0, n, err, inv_h, inv_l, s);
   for (long jb0 = 0; jb0 < n; jb0 += nb) {
     long jb = std::min(nb, n - jb0);
-    // diagonal block: the unblocked algorithm on the block (linalg.rs:250-255)
-    if (jb <= DB) {
-      hipLaunchKernelGGL((k_diag_factor<A, false>), dim3(1), dim3(256), 2 * sizeof(Lds2<A>), s, a, S, jb0, (int)jb,
-                         err, inv_h, inv_l);
-      HP_LAUNCH_CHECK();
-    } else if ((rc = factor_unblocked_region<A>(a, S, jb0, jb, err, inv_h, inv_l, s))) {
-      return rc;
-    }
+    if ((rc = chol_panel_exact<A>(a, S, jb0, jb, n, err, inv_h, inv_l, s))) return rc;
     long rem = n - jb0 - jb;
-    if (rem <= 0) continue;
-    // trsm_right_lt on the panel rows (linalg.rs:259-261) in 64-column sub-panels
-    for (long q0 = 0; q0 < jb; q0 += DB) {
-      int w = (int)std::min<long>(DB, jb - q0);
-      if (q0 > 0 &&
-          (rc = rank_update<A, RU_SUBINC, false>(a, S, jb0 + jb, rem, jb0 + q0, w, jb0 + q0, jb0, q0, err, s)))
-        return rc;
-      if (jb > DB) {
-        hipLaunchKernelGGL(k_diag_recip<A>, dim3(1), dim3(64), 0, s, a, jb0 + q0, w, inv_h, inv_l);
-        HP_LAUNCH_CHECK();
-      }
-      if ((rc = row_recur<A, false>(a, S, jb0 + q0, w, jb0 + jb, n, inv_h, inv_l, err, s))) return rc;
-    }
     // trailing update (linalg.rs:263-288)
-    if ((rc = rank_update<A, RU_SUMSUB, true>(a, S, jb0 + jb, rem, jb0 + jb, rem, jb0 + jb, jb0, jb, err, s))) return rc;
+    if (rem > 0 &&
+        (rc = rank_update<A, RU_SUMSUB, true>(a, S, jb0 + jb, rem, jb0 + jb, rem, jb0 + jb, jb0, jb, err, s)))
+      return rc;
   }
   return HP_OK;
 }
@@ -539,6 +550,8 @@ int zero_upper_f64(double* l, long n, hipStream_t s) {
 }
 
 template int chol_exact<ArDD>(Mat, long, long, bool, Mat, ErrRec*, double*, double*, hipStream_t);
+template int chol_panel_exact<ArDD>(Mat, Mat, long, long, long, ErrRec*, double*, double*, hipStream_t);
+template int chol_panel_exact<ArF64>(Mat, Mat, long, long, long, ErrRec*, double*, double*, hipStream_t);
 template int chol_exact<ArF64>(Mat, long, long, bool, Mat, ErrRec*, double*, double*, hipStream_t);
 template int fwd_exact<ArDD>(Mat, double*, double*, long, hipStream_t);
 template int fwd_exact<ArF64>(Mat, double*, double*, long, hipStream_t);

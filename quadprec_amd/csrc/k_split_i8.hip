@@ -360,7 +360,7 @@ __device__ __forceinline__ dd u64_to_dd(uint64_t x) {  // exact
 __global__ __launch_bounds__(256) void k_crt(const uint8_t* __restrict__ Cr, long M, long N,
                                              const int* __restrict__ er, const int* __restrict__ fc, int W,
                                              const CrtTable* __restrict__ tab, double* __restrict__ chi,
-                                             double* __restrict__ clo) {
+                                             double* __restrict__ clo, long ldc) {
   __shared__ uint32_t Mr[CRT_MAXR][CRT_LIMBS];
   __shared__ uint32_t Ml[CRT_LIMBS];
   __shared__ uint32_t mods[CRT_MAXR], ys[CRT_MAXR];
@@ -422,8 +422,9 @@ __global__ __launch_bounds__(256) void k_crt(const uint8_t* __restrict__ Cr, lon
       }
       if (neg) res = dd_neg(res);
     }
-    chi[idx] = res.hi;
-    clo[idx] = res.lo;
+    const size_t co = (size_t)(idx / N) * ldc + idx % N;
+    chi[co] = res.hi;
+    clo[co] = res.lo;
   }
 }
 
@@ -473,8 +474,8 @@ int crt_bits_W(int R, size_t k) {
   return (int)((std::floor(log2_M(R)) - lk - 4) / 2);
 }
 
-int launch_gemm_split_i8(int trans_a, const double* ahi, const double* alo, const double* bhi, const double* blo,
-                         double* chi, double* clo, size_t m, size_t n, size_t k, int R, hipStream_t s) {
+int split_gemm_general(const SplitOperand& A, const SplitOperand& B, double* chi, double* clo, size_t ldc,
+                       size_t m, size_t n, size_t k, int R, hipStream_t s) {
   if (m == 0 || n == 0) return HP_OK;
   if (R <= 0) R = 31;
   if (R < 8 || R > CRT_MAXR) {
@@ -482,8 +483,8 @@ int launch_gemm_split_i8(int trans_a, const double* ahi, const double* alo, cons
     return HP_EINVAL;
   }
   if (k == 0) {
-    HP_HIP(hipMemsetAsync(chi, 0, m * n * sizeof(double), s));
-    HP_HIP(hipMemsetAsync(clo, 0, m * n * sizeof(double), s));
+    HP_HIP(hipMemset2DAsync(chi, ldc * sizeof(double), 0, n * sizeof(double), m, s));
+    HP_HIP(hipMemset2DAsync(clo, ldc * sizeof(double), 0, n * sizeof(double), m, s));
     return HP_OK;
   }
   const int W = crt_bits_W(R, k);
@@ -528,24 +529,23 @@ int launch_gemm_split_i8(int trans_a, const double* ahi, const double* alo, cons
   hipLaunchKernelGGL(k_fill_int, dim3((unsigned)((m + n + 255) / 256)), dim3(256), 0, s, eA, (long)(m + n), -100000);
   HP_LAUNCH_CHECK();
   const unsigned kch = (unsigned)((k + 255) / 256);
-  if (!trans_a)
-    hipLaunchKernelGGL(k_scale_rows, dim3((unsigned)((m + 3) / 4)), dim3(256), 0, s, ahi, (long)m, (long)k, (long)k, eA);
-  else
-    hipLaunchKernelGGL(k_scale_cols, dim3((unsigned)((m + 255) / 256), kch), dim3(256), 0, s, ahi, (long)m, (long)k,
-                       (long)m, eA);
-  HP_LAUNCH_CHECK();
-  hipLaunchKernelGGL(k_scale_cols, dim3((unsigned)((n + 255) / 256), kch), dim3(256), 0, s, bhi, (long)n, (long)k,
-                     (long)n, fB);
-  HP_LAUNCH_CHECK();
-  dim3 ga((unsigned)((k + 63) / 64), (unsigned)((m + 63) / 64));
-  if (!trans_a)
-    hipLaunchKernelGGL(k_residues<true>, ga, dim3(256), 0, s, ahi, alo, (long)m, (long)k, (long)k, eA, W, g_work.tab, rA);
-  else
-    hipLaunchKernelGGL(k_residues<false>, ga, dim3(256), 0, s, ahi, alo, (long)m, (long)k, (long)m, eA, W, g_work.tab, rA);
-  HP_LAUNCH_CHECK();
-  dim3 gb((unsigned)((k + 63) / 64), (unsigned)((n + 63) / 64));
-  hipLaunchKernelGGL(k_residues<false>, gb, dim3(256), 0, s, bhi, blo, (long)n, (long)k, (long)n, fB, W, g_work.tab, rB);
-  HP_LAUNCH_CHECK();
+  auto scales_and_residues = [&](const SplitOperand& X, long rows, int* e, int8_t* out) -> int {
+    if (X.contig)
+      hipLaunchKernelGGL(k_scale_rows, dim3((unsigned)((rows + 3) / 4)), dim3(256), 0, s, X.hi, rows, (long)k, X.ld, e);
+    else
+      hipLaunchKernelGGL(k_scale_cols, dim3((unsigned)((rows + 255) / 256), kch), dim3(256), 0, s, X.hi, rows, (long)k,
+                         X.ld, e);
+    HP_LAUNCH_CHECK();
+    dim3 g((unsigned)((k + 63) / 64), (unsigned)((rows + 63) / 64));
+    if (X.contig)
+      hipLaunchKernelGGL(k_residues<true>, g, dim3(256), 0, s, X.hi, X.lo, rows, (long)k, X.ld, e, W, g_work.tab, out);
+    else
+      hipLaunchKernelGGL(k_residues<false>, g, dim3(256), 0, s, X.hi, X.lo, rows, (long)k, X.ld, e, W, g_work.tab, out);
+    HP_LAUNCH_CHECK();
+    return HP_OK;
+  };
+  if (int rc = scales_and_residues(A, (long)m, eA, rA)) return rc;
+  if (int rc = scales_and_residues(B, (long)n, fB, rB)) return rc;
   tm.mark(s);
   dim3 gg((unsigned)((n + GN - 1) / GN), (unsigned)((m + GM - 1) / GM), (unsigned)R);
   auto launch = [&](int variant) -> int {
@@ -598,13 +598,21 @@ int launch_gemm_split_i8(int trans_a, const double* ahi, const double* alo, cons
   tm.mark(s);
   size_t total = m * n;
   unsigned gc = (unsigned)std::min<size_t>((total + 255) / 256, 1u << 16);
-  hipLaunchKernelGGL(k_crt, dim3(gc), dim3(256), 0, s, rC, (long)m, (long)n, eA, fB, W, g_work.tab, chi, clo);
+  hipLaunchKernelGGL(k_crt, dim3(gc), dim3(256), 0, s, rC, (long)m, (long)n, eA, fB, W, g_work.tab, chi, clo,
+                     (long)ldc);
   HP_LAUNCH_CHECK();
   tm.mark(s);
   // the workspace is reused by the next call: keep it owned until this one has run
   HP_HIP(hipStreamSynchronize(s));
   tm.finish();
   return HP_OK;
+}
+
+int launch_gemm_split_i8(int trans_a, const double* ahi, const double* alo, const double* bhi, const double* blo,
+                         double* chi, double* clo, size_t m, size_t n, size_t k, int R, hipStream_t s) {
+  SplitOperand A{ahi, alo, trans_a ? (long)m : (long)k, !trans_a};
+  SplitOperand B{bhi, blo, (long)n, false};
+  return split_gemm_general(A, B, chi, clo, n, m, n, k, R, s);
 }
 
 }  // namespace hp

@@ -49,6 +49,28 @@ def gemm(a, b, c, m, n, k, trans_a=False, mode=HP_GEMM_PLAIN, slices=0):
     return c
 
 
+def solve_spd_refined(s, b, x, n, factor_kind, tol=1e-30, max_iter=20):
+    """Mixed-precision SPD solve on planes: s (2, n*n), b (2, n), x (2, n) written."""
+    from . import RefineStats, SolveError, HP_ENOTSPD
+    import ctypes as C
+    _need(s, 2 * n * n, "s")
+    _need(b, 2 * n, "b")
+    _need(x, 2 * n, "x")
+    st = RefineStats()
+    rc = lib().hp_dev_solve_spd_refined(factor_kind, _ptr(s[0]), _ptr(s[1]), _ptr(b[0]), _ptr(b[1]), n, tol, max_iter,
+                                        _ptr(x[0]), _ptr(x[1]), C.byref(st), _stream())
+    if rc == HP_ENOTSPD:
+        raise SolveError(st.pivot)
+    _check(rc, "dev.solve_spd_refined")
+    return st
+
+
+def overlap_s(centers, alpha, s, n):
+    """dd overlap matrix into planes s (2, n*n); centers (n*3,), alpha (n,) device tensors."""
+    _check(lib().hp_dev_overlap_s(_ptr(centers), _ptr(alpha), n, _ptr(s[0]), _ptr(s[1]), _stream()), "dev.overlap_s")
+    return s
+
+
 def mf_gemm(a, b, c, m, n, k, trans_a=False):
     """MF4 C = op(A) B on limb planes: a (4, m*k), b (4, k*n), c (4, m*n)."""
     _need(a, 4 * m * k, "a")

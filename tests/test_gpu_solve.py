@@ -1,0 +1,91 @@
+"""Mixed-precision SPD solves with quad-double residual refinement (hp_solve_spd_refined),
+checked against an independent CPU solution x*:
+
+  x* = the oracle's dd cholesky_blocked (crates/quad/src/linalg.rs:238-293) and
+  solve_cholesky, refined with residuals b - S x evaluated in MultiFloat<4>
+  (crates/quad/src/multifloat/linalg.rs:17-36 mf_gemv) until the correction stagnates.
+
+S is the Gaussian overlap matrix of config 0/2 (lattice Gaussians, exponents log-uniform),
+b has dd N(0,1) entries.  The bar is ||x - x*||_inf / ||x*||_inf <= 1e-28 (kappa ~ 1e20)
+and <= 1e-30 (kappa ~ 1e8 .. 1e10)."""
+import numpy as np
+import pytest
+
+import oracle_lib as O
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def qp():
+    import quadprec_amd
+    quadprec_amd.lib()
+    return quadprec_amd
+
+
+def to_mf4(x):
+    out = np.zeros((x.shape[0], 4))
+    out[:, :2] = x
+    return out
+
+
+def cpu_reference(s, b, n, iters=8):
+    l = O.cholesky_blocked(s, n, 64)
+    x = O.solve_cholesky(l, b, n)
+    s4 = to_mf4(s)
+    for _ in range(iters):
+        ax = O.mf4_gemv(s4, to_mf4(x), n, n)
+        r4 = O.mf4_binop("sub", to_mf4(b), ax)
+        r = np.ascontiguousarray(r4[:, :2])
+        dx = O.solve_cholesky(l, r, n)
+        x = O.dd_binop("add", x, dx)
+        if np.max(np.abs(dx[:, 0])) <= 1e-34 * np.max(np.abs(x[:, 0])):
+            break
+    return x
+
+
+def rel_err(x, xs):
+    d = O.dd_binop("sub", x, xs)
+    return float(np.max(np.abs(d[:, 0] + d[:, 1])) / np.max(np.abs(xs[:, 0])))
+
+
+@pytest.mark.parametrize("lo,kinds,bar", [
+    (0.5, ("f64", "dd", "split"), 1e-30),   # kappa ~ 1e8: every factor converges
+    (0.01, ("dd", "split"), 1e-28),          # kappa ~ 1e20 (config 0's exponent range)
+])
+def test_refined_solve_vs_cpu_reference(qp, lo, kinds, bar):
+    n = 512
+    c, a = qp.gen_overlap_basis(0, n, lo, 10.0)
+    s = qp.overlap_s(c, a)
+    b = qp.gen_dd_normal(77, n)
+    xs = cpu_reference(s, b, n)
+    kind_id = {"f64": qp.HP_FACTOR_F64, "dd": qp.HP_FACTOR_DD, "split": qp.HP_FACTOR_DD_SPLIT}
+    for kind in kinds:
+        x, st = qp.solve_spd_refined(s, b, n, kind_id[kind], tol=bar * 1e-2, max_iter=30)
+        e = rel_err(x, xs)
+        print(f"alpha_lo={lo} factor={kind}: iters={st.iterations} err={e:.3e} last={st.last_correction:.2e} "
+              f"factor {st.factor_ms:.1f} ms refine {st.refine_ms:.1f} ms")
+        assert e <= bar, (kind, e)
+
+
+def test_refined_solve_f64_factor_fails_cleanly_on_indefinite(qp):
+    n = 64
+    s = np.zeros((n * n, 2))
+    for i in range(n):
+        s[i * n + i, 0] = 1.0
+    s[10 * n + 10, 0] = -1.0
+    with pytest.raises(qp.SolveError) as e:
+        qp.solve_spd_refined(s, qp.gen_dd_normal(1, n), n, qp.HP_FACTOR_DD_SPLIT)
+    assert e.value.pivot == 10
+
+
+def test_split_cholesky_factor_matches_plain_to_dd_accuracy(qp):
+    """The split factor differs from the reference-order one only by rounding: its
+    residual against S after one refinement-free solve is at the plain factor's level."""
+    n = 300
+    c, a = qp.gen_overlap_basis(3, n, 0.5, 10.0)
+    s = qp.overlap_s(c, a)
+    b = qp.gen_dd_normal(5, n)
+    x1, st1 = qp.solve_spd_refined(s, b, n, qp.HP_FACTOR_DD, tol=0.0, max_iter=1)
+    x2, st2 = qp.solve_spd_refined(s, b, n, qp.HP_FACTOR_DD_SPLIT, tol=0.0, max_iter=1)
+    assert rel_err(x2, x1) < 1e-20
