@@ -49,6 +49,50 @@ def cpu_baseline(a, b, target_s=12.0):
             "sample": f"{rows} of {M} rows of the 4096^3 dd gemm ({t:.1f} s), oracle C++ restatement, OpenMP rows"}
 
 
+SOLVE_N = 4096
+SOLVE_CASES = [  # (name, alpha_lo, spacing, target rel err, split-path factor, seed)
+    ("a_kappa_small", 0.5, 1.4, 1e-30, "f64", 2),
+    ("b_kappa_1e16", 0.01, 1.4, 1e-28, "split", 2),
+    ("b_extreme_kappa_1e25", 0.001, 0.7, 1e-28, "split", 2),
+]
+
+
+def bench_solves(qp, torch, dev):
+    """N=4096 overlap SPD solves to the target error: split path vs the plain dd factor,
+    both refined with quad-double residuals (time = factor + refinement, device-synced)."""
+    kinds = {"f64": qp.HP_FACTOR_F64, "dd": qp.HP_FACTOR_DD, "split": qp.HP_FACTOR_DD_SPLIT}
+    out = {}
+    n = SOLVE_N
+    for name, lo, sp, target, split_kind, seed in SOLVE_CASES:
+        c, a = qp.gen_overlap_basis(seed, n, lo, 10.0, sp)
+        dc = torch.from_numpy(c.ravel()).to(dev)
+        da = torch.from_numpy(a).to(dev)
+        S = torch.empty((2, n * n), dtype=torch.float64, device=dev)
+        qp.dev.overlap_s(dc, da, S, n)
+        b = qp.dev.to_planes(qp.gen_dd_normal(seed + 100, n), 2, dev)
+        x = torch.empty((2, n), dtype=torch.float64, device=dev)
+        res = {}
+        for label, kind in (("split", split_kind), ("plain", "dd")):
+            best = None
+            for _ in range(2):  # the first run also warms the autotuner and allocations
+                torch.cuda.synchronize()
+                t0 = time.perf_counter()
+                st = qp.dev.solve_spd_refined(S, b, x, n, kinds[kind], tol=target * 1e-2, max_iter=30)
+                torch.cuda.synchronize()
+                t = (time.perf_counter() - t0) * 1e3
+                if best is None or t < best[0]:
+                    best = (t, st.factor_ms, st.refine_ms, st.iterations, st.last_correction)
+            res[label] = {"factor": kind, "ms": round(best[0], 2), "factor_ms": round(best[1], 2),
+                          "refine_ms": round(best[2], 2), "iterations": best[3],
+                          "last_correction": float(f"{best[4]:.3e}")}
+        res["alpha"] = [lo, 10.0]
+        res["spacing_bohr"] = sp
+        res["target_rel_err"] = target
+        res["split_over_plain"] = round(res["plain"]["ms"] / res["split"]["ms"], 2)
+        out[name] = res
+    return out
+
+
 def time_steps(step, steps, warmup, stream, torch):
     for _ in range(warmup):
         step()
@@ -70,6 +114,7 @@ def main():
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--no-solve", action="store_true", help="skip the N=4096 SPD solve timings")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); "
@@ -138,6 +183,8 @@ def main():
                                "note": "29 FP64 VALU ops per dd MAC, FMA-equivalent (2 flops each)"}},
         "split_over_plain": round(ms_plain / ms_split, 2),
     }
+    if not args.no_solve:
+        out["spd_solve_n4096"] = bench_solves(qp, torch, dev)
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(a, b)
     print(json.dumps(out), flush=True)

@@ -102,6 +102,8 @@ def lib():
     L.hp_dev_solve_spd_refined.argtypes = [C.c_int, _vp, _vp, _vp, _vp, _sz, C.c_double, C.c_int, _vp, _vp,
                                            C.POINTER(RefineStats), _vp]
     L.hp_solve_spd_refined.argtypes = [C.c_int, _dp, _dp, _sz, C.c_double, C.c_int, _dp, C.POINTER(RefineStats)]
+    L.hp_set_split_factor_panel.argtypes = [C.c_int]
+    L.hp_set_split_factor_panel.restype = C.c_int
     L.hp_overlap_s.argtypes = [_dp, _dp, _sz, _dp]
     L.hp_dev_overlap_s.argtypes = [_vp, _vp, _sz, _vp, _vp, _vp]
     L.hp_gen_overlap_basis.argtypes = [C.c_uint64, _sz, C.c_double, C.c_double, C.c_double, C.c_double, _dp, _dp]
@@ -267,6 +269,11 @@ def solve_spd_refined(s, b, n, factor_kind=HP_FACTOR_DD_SPLIT, tol=1e-30, max_it
         raise SolveError(st.pivot)
     _check(rc, "solve_spd_refined")
     return x, st
+
+
+def set_split_factor_panel(nb):
+    """Panel width (64, 128, 256, 512) of the split-dd factorisation."""
+    _check(lib().hp_set_split_factor_panel(nb), "set_split_factor_panel")
 
 
 # ---------------------------------------------------------------- overlap

@@ -397,6 +397,15 @@ int hp_solve_spd(const double* a, const double* b, size_t n, hp_solution* sol, d
   return rc;
 }
 
+int hp_set_split_factor_panel(int nb) {
+  if (nb != 64 && nb != 128 && nb != 256 && nb != 512) {
+    set_error("split factor panel width must be 64, 128, 256 or 512");
+    return HP_EINVAL;
+  }
+  g_split_panel = nb;
+  return HP_OK;
+}
+
 int hp_dev_solve_spd_refined(int factor_kind, const double* s_hi, const double* s_lo, const double* b_hi,
                              const double* b_lo, size_t n, double tol, int max_iter, double* x_hi, double* x_lo,
                              hp_refine_stats* stats, void* stream) {

@@ -26,6 +26,7 @@ struct ArDD {
   using T = dd;
   static constexpr int LIMBS = 2;
   __device__ static T zero() { return mkdd(0.0, 0.0); }
+  __device__ static T one() { return mkdd(1.0, 0.0); }
   __device__ static T ld(const Mat& m, long i, long j) { long o = i * m.ld + j; return mkdd(m.hi[o], m.lo[o]); }
   __device__ static void st(const Mat& m, long i, long j, T v) { long o = i * m.ld + j; m.hi[o] = v.hi; m.lo[o] = v.lo; }
   __device__ static T ldv(const double* hi, const double* lo, long i) { return mkdd(hi[i], lo[i]); }
@@ -47,6 +48,7 @@ struct ArF64 {
   using T = double;
   static constexpr int LIMBS = 1;
   __device__ static T zero() { return 0.0; }
+  __device__ static T one() { return 1.0; }
   __device__ static T ld(const Mat& m, long i, long j) { return m.hi[i * m.ld + j]; }
   __device__ static void st(const Mat& m, long i, long j, T v) { m.hi[i * m.ld + j] = v; }
   __device__ static T ldv(const double* hi, const double*, long i) { return hi[i]; }

@@ -87,9 +87,15 @@ struct SplitOperand {
   long ld;
   bool contig;
 };
+struct SplitOpts {
+  bool lower_only = false;    // C symmetric: only j <= i is formed (m == n)
+  bool subtract = false;      // C -= A B^T instead of C = A B^T
+  bool same_operand = false;  // B is A (one set of residues)
+};
 int split_gemm_general(const SplitOperand& A, const SplitOperand& B, double* chi, double* clo, size_t ldc, size_t m,
-                       size_t n, size_t k, int R, hipStream_t s);
+                       size_t n, size_t k, int R, hipStream_t s, const SplitOpts& opt);
 int crt_bits_W(int R, size_t k);
+extern int g_split_panel;
 int solve_refined(int kind, const double* sh, const double* sl, const double* bh, const double* bl, long n, double tol,
                   int max_iter, double* xh, double* xl, hp_refine_stats* st, hipStream_t s);
 int split_gemm_variant(size_t m, size_t n, size_t k, int R);

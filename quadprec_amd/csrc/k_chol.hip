@@ -489,6 +489,35 @@ int chol_panel_exact(Mat a, Mat S, long c0, long jb, long n, ErrRec* err, double
   return HP_OK;
 }
 
+// Second stream and events for the factorisations' lookahead (default priority).
+struct Lookahead {
+  hipStream_t s2 = nullptr;
+  hipEvent_t evA = nullptr, evB = nullptr;
+  bool evB_live = false;
+  int init() {
+    HP_HIP(hipStreamCreateWithFlags(&s2, hipStreamNonBlocking));
+    HP_HIP(hipEventCreateWithFlags(&evA, hipEventDisableTiming));
+    HP_HIP(hipEventCreateWithFlags(&evB, hipEventDisableTiming));
+    return HP_OK;
+  }
+  ~Lookahead() {
+    if (s2) { (void)hipStreamSynchronize(s2); (void)hipStreamDestroy(s2); }
+    if (evA) (void)hipEventDestroy(evA);
+    if (evB) (void)hipEventDestroy(evB);
+  }
+};
+
+template <class A>
+int rank_update_sumsub(Mat a, Mat S, long ri0, long M, long cj0, long N, long ka0, long K, ErrRec* err,
+                       hipStream_t s) {
+  return rank_update<A, RU_SUMSUB, true>(a, S, ri0, M, cj0, N, cj0, ka0, K, err, s);
+}
+
+// cholesky_blocked with lookahead: after panel k, the next panel's columns receive panel
+// k's update first (stream s); the rest of the trailing update runs on a second stream
+// while panel k+1 is factored.  Every element still receives its updates in panel order
+// (the strip update of step k+1 waits for the rest update of step k), so the result is
+// bit-identical to the sequential loop of linalg.rs:247-291.
 template <class A>
 int chol_exact(Mat a, long n, long nb, bool unblocked, Mat S, ErrRec* err, double* inv_h, double* inv_l,
                hipStream_t s) {
@@ -497,15 +526,27 @@ int chol_exact(Mat a, long n, long nb, bool unblocked, Mat S, ErrRec* err, doubl
   if (n == 0) return HP_OK;
   if (nb == 0) nb = 64;  // linalg.rs:241
   if (unblocked || nb >= n) return factor_unblocked_region<A>(a, S, 0, n, err, inv_h, inv_l, s);
+  Lookahead la;
+  if ((rc = la.init())) return rc;
   for (long jb0 = 0; jb0 < n; jb0 += nb) {
     long jb = std::min(nb, n - jb0);
     if ((rc = chol_panel_exact<A>(a, S, jb0, jb, n, err, inv_h, inv_l, s))) return rc;
     long rem = n - jb0 - jb;
-    // trailing update (linalg.rs:263-288)
-    if (rem > 0 &&
-        (rc = rank_update<A, RU_SUMSUB, true>(a, S, jb0 + jb, rem, jb0 + jb, rem, jb0 + jb, jb0, jb, err, s)))
-      return rc;
+    if (rem <= 0) continue;
+    long t0 = jb0 + jb, w2 = std::min(nb, rem);
+    if (la.evB_live) HP_HIP(hipStreamWaitEvent(s, la.evB, 0));
+    // next panel's columns (linalg.rs:263-288 restricted to them)
+    if ((rc = rank_update<A, RU_SUMSUB, true>(a, S, t0, rem, t0, w2, t0, jb0, jb, err, s))) return rc;
+    if (rem > w2) {
+      HP_HIP(hipEventRecord(la.evA, s));
+      HP_HIP(hipStreamWaitEvent(la.s2, la.evA, 0));
+      long r0 = t0 + w2;
+      if ((rc = rank_update<A, RU_SUMSUB, true>(a, S, r0, n - r0, r0, n - r0, r0, jb0, jb, err, la.s2))) return rc;
+      HP_HIP(hipEventRecord(la.evB, la.s2));
+      la.evB_live = true;
+    }
   }
+  if (la.evB_live) HP_HIP(hipStreamWaitEvent(s, la.evB, 0));
   return HP_OK;
 }
 
@@ -552,6 +593,7 @@ int zero_upper_f64(double* l, long n, hipStream_t s) {
 template int chol_exact<ArDD>(Mat, long, long, bool, Mat, ErrRec*, double*, double*, hipStream_t);
 template int chol_panel_exact<ArDD>(Mat, Mat, long, long, long, ErrRec*, double*, double*, hipStream_t);
 template int chol_panel_exact<ArF64>(Mat, Mat, long, long, long, ErrRec*, double*, double*, hipStream_t);
+template int rank_update_sumsub<ArDD>(Mat, Mat, long, long, long, long, long, long, ErrRec*, hipStream_t);
 template int chol_exact<ArF64>(Mat, long, long, bool, Mat, ErrRec*, double*, double*, hipStream_t);
 template int fwd_exact<ArDD>(Mat, double*, double*, long, hipStream_t);
 template int fwd_exact<ArF64>(Mat, double*, double*, long, hipStream_t);

@@ -28,6 +28,8 @@ template <class A>
 int chol_panel_exact(Mat a, Mat S, long c0, long jb, long n, ErrRec* err, double* inv_h, double* inv_l, hipStream_t s);
 template <class A>
 int fwd_exact(Mat l, double* yh, double* yl, long n, hipStream_t s);
+template <class A>
+int rank_update_sumsub(Mat a, Mat S, long ri0, long M, long cj0, long N, long ka0, long K, ErrRec* err, hipStream_t s);
 
 namespace {
 
@@ -91,6 +93,106 @@ __global__ __launch_bounds__(256) void k_bwd_update(Mat l, double* yh, double* y
     if constexpr (A::LIMBS == 2) xv = mkdd(Xh[j], Xl[j]);
     else xv = Xh[j];
     y = A::sub(y, A::mul(A::ld(l, b0 + j, i), xv));
+  }
+  A::stv(yh, yl, i, y);
+}
+
+template <class A>
+__device__ __forceinline__ typename A::T Vg(double (*Vh)[BW + 1], double (*Vl)[BW + 1], int i, int c) {
+  if constexpr (A::LIMBS == 2) return mkdd(Vh[i][c], Vl[i][c]);
+  else return Vh[i][c];
+}
+
+// ---------------------------------------------------------------- blocked inverse solves
+// Correction solves with precomputed inverses of the 64 x 64 diagonal blocks of L: each
+// block step is one launch in which every workgroup forms x_b = inv(L_bb) y_b (a 64-long
+// dot per lane, no sequential chain) and then subtracts the block's contributions from its
+// own rows.  x is written to a separate vector so no workgroup reads a value another one
+// is overwriting.
+template <class A>
+__global__ __launch_bounds__(64) void k_diag_block_inverse(Mat l, long n, double* ih, double* il) {
+  // inv(L_bb) for block b = blockIdx.x, lane c computes column c by forward substitution
+  using T = typename A::T;
+  __shared__ double Lh[BW][BW + 1], Ll[A::LIMBS == 2 ? BW : 1][BW + 1];
+  __shared__ double Vh[BW][BW + 1], Vl[A::LIMBS == 2 ? BW : 1][BW + 1];
+  const long b0 = (long)blockIdx.x * BW;
+  const int bw = (int)min((long)BW, n - b0);
+  const int c = threadIdx.x;
+  for (int i = 0; i < bw; ++i)
+    if (c <= i) {
+      T v = A::ld(l, b0 + i, b0 + c);
+      if constexpr (A::LIMBS == 2) { Lh[i][c] = v.hi; Ll[i][c] = v.lo; }
+      else Lh[i][c] = v;
+    }
+  __syncthreads();
+  auto Lg = [&](int i, int k) -> T {
+    if constexpr (A::LIMBS == 2) return mkdd(Lh[i][k], Ll[i][k]);
+    else return Lh[i][k];
+  };
+  if (c < bw) {
+    for (int i = 0; i < bw; ++i) {
+      T v = A::zero();
+      if (i >= c) {
+        T s = (i == c) ? A::one() : A::zero();
+        for (int k = c; k < i; ++k) {
+          T vk;
+          if constexpr (A::LIMBS == 2) vk = mkdd(Vh[k][c], Vl[k][c]);
+          else vk = Vh[k][c];
+          s = A::sub(s, A::mul(Lg(i, k), vk));
+        }
+        v = A::div(s, Lg(i, i));
+      }
+      if constexpr (A::LIMBS == 2) { Vh[i][c] = v.hi; Vl[i][c] = v.lo; }
+      else Vh[i][c] = v;
+    }
+  }
+  __syncthreads();
+  // store as [block][i][c]
+  const size_t base = (size_t)blockIdx.x * BW * BW;
+  for (int i = 0; i < BW; ++i) {
+    T v = (i < bw && c < bw) ? Vg<A>(Vh, Vl, i, c) : A::zero();
+    A::stv(ih, il, (long)(base + (size_t)i * BW + c), v);
+  }
+}
+
+// one block step of L y = b (FWD) or L^T x = y (!FWD); rows handled: FWD rows >= b0+bw,
+// !FWD rows < b0.  x_b goes to xo; y is updated in place for the other rows.
+template <class A, bool FWD>
+__global__ __launch_bounds__(256) void k_inv_step(Mat l, const double* ih, const double* il, double* yh, double* yl,
+                                                  double* xoh, double* xol, long n, long b0, int bw) {
+  using T = typename A::T;
+  __shared__ double Yh[BW], Yl[BW], Xh[BW], Xl[BW];
+  const int t = threadIdx.x;
+  const size_t base = (size_t)(b0 / BW) * BW * BW;
+  if (t < bw) {
+    Yh[t] = yh[b0 + t];
+    if constexpr (A::LIMBS == 2) Yl[t] = yl[b0 + t];
+  }
+  __syncthreads();
+  if (t < bw) {  // x_t = sum_k inv[t][k] y_k (FWD) or sum_k inv[k][t] y_k (transposed)
+    T s = A::zero();
+    for (int k = 0; k < bw; ++k) {
+      size_t o = FWD ? base + (size_t)t * BW + k : base + (size_t)k * BW + t;
+      T yv;
+      if constexpr (A::LIMBS == 2) yv = mkdd(Yh[k], Yl[k]);
+      else yv = Yh[k];
+      s = A::add(s, A::mul(A::ldv(ih, il, (long)o), yv));
+    }
+    if constexpr (A::LIMBS == 2) { Xh[t] = s.hi; Xl[t] = s.lo; }
+    else Xh[t] = s;
+    if (blockIdx.x == 0) A::stv(xoh, xol, b0 + t, s);
+  }
+  __syncthreads();
+  long i = (long)blockIdx.x * 256 + t;
+  if (FWD) i += b0 + bw;
+  if (FWD ? (i >= n) : (i >= b0)) return;
+  T y = A::ldv(yh, yl, i);
+  for (int k = 0; k < bw; ++k) {
+    T xv;
+    if constexpr (A::LIMBS == 2) xv = mkdd(Xh[k], Xl[k]);
+    else xv = Xh[k];
+    T lv = FWD ? A::ld(l, i, b0 + k) : A::ld(l, b0 + k, i);
+    y = A::sub(y, A::mul(lv, xv));
   }
   A::stv(yh, yl, i, y);
 }
@@ -209,6 +311,41 @@ int bwd_fast(Mat l, double* yh, double* yl, long n, hipStream_t s) {
   return HP_OK;
 }
 
+// inv(L_bb) for all 64-row diagonal blocks: nblk x 64 x 64 values per plane
+template <class A>
+int diag_inverses(Mat l, long n, double* ih, double* il, hipStream_t s) {
+  long nblk = (n + BW - 1) / BW;
+  hipLaunchKernelGGL(k_diag_block_inverse<A>, dim3((unsigned)nblk), dim3(64), 0, s, l, n, ih, il);
+  HP_LAUNCH_CHECK();
+  return HP_OK;
+}
+// L L^T x = b with the block inverses; y (scratch, n) holds b on entry and is consumed
+template <class A>
+int solve_inv(Mat l, const double* ih, const double* il, double* yh, double* yl, double* xh, double* xl, long n,
+              hipStream_t s) {
+  long nblk = (n + BW - 1) / BW;
+  for (long b = 0; b < nblk; ++b) {  // forward: z = L^-1 y into x
+    long b0 = b * BW;
+    int bw = (int)std::min<long>(BW, n - b0);
+    long rows = n - b0 - bw;
+    unsigned g = (unsigned)std::max<long>(1, (rows + 255) / 256);
+    hipLaunchKernelGGL((k_inv_step<A, true>), dim3(g), dim3(256), 0, s, l, ih, il, yh, yl, xh, xl, n, b0, bw);
+    HP_LAUNCH_CHECK();
+  }
+  for (long b = nblk - 1; b >= 0; --b) {  // backward: x = L^-T z, z in x (copied to y as the working vector)
+    long b0 = b * BW;
+    int bw = (int)std::min<long>(BW, n - b0);
+    if (b == nblk - 1) {
+      HP_HIP(hipMemcpyAsync(yh, xh, n * sizeof(double), hipMemcpyDeviceToDevice, s));
+      if (A::LIMBS == 2) HP_HIP(hipMemcpyAsync(yl, xl, n * sizeof(double), hipMemcpyDeviceToDevice, s));
+    }
+    unsigned g = (unsigned)std::max<long>(1, (b0 + 255) / 256);
+    hipLaunchKernelGGL((k_inv_step<A, false>), dim3(g), dim3(256), 0, s, l, ih, il, yh, yl, xh, xl, n, b0, bw);
+    HP_LAUNCH_CHECK();
+  }
+  return HP_OK;
+}
+
 double absmax_dev(const double* v, long n, unsigned long long* scratch, hipStream_t s, int* rc) {
   *rc = HP_OK;
   if (hipMemsetAsync(scratch, 0, sizeof(unsigned long long), s) != hipSuccess) { *rc = HP_EDEVICE; return 0; }
@@ -225,28 +362,56 @@ double absmax_dev(const double* v, long n, unsigned long long* scratch, hipStrea
 
 }  // namespace
 
-// dd blocked Cholesky with the trailing update P P^T on the INT8 matrix cores.
-int chol_split(Mat a, long n, long nb, Mat S, ErrRec* err, double* inv_h, double* inv_l, double* tmp, hipStream_t s) {
+// dd blocked Cholesky with the trailing update P P^T on the INT8 matrix cores.  Per panel:
+// the panel (diagonal block + trsm) on FP64 VALU, the next panel's columns updated on
+// VALU, then the rest of the trailing matrix updated by one split SYRK (lower tiles only,
+// the subtraction fused into the CRT) on a second stream while the next panel is factored.
+int chol_split(Mat a, long n, long nb, Mat S, ErrRec* err, double* inv_h, double* inv_l, hipStream_t s) {
   if (nb <= 0) nb = 256;
+  hipStream_t s2 = nullptr;
+  hipEvent_t evA = nullptr, evB = nullptr;
+  bool evB_live = false;
+  HP_HIP(hipStreamCreateWithFlags(&s2, hipStreamNonBlocking));
+  struct Guard {
+    hipStream_t& s2; hipEvent_t& a; hipEvent_t& b;
+    ~Guard() {
+      if (s2) { (void)hipStreamSynchronize(s2); (void)hipStreamDestroy(s2); }
+      if (a) (void)hipEventDestroy(a);
+      if (b) (void)hipEventDestroy(b);
+    }
+  } guard{s2, evA, evB};
+  HP_HIP(hipEventCreateWithFlags(&evA, hipEventDisableTiming));
+  HP_HIP(hipEventCreateWithFlags(&evB, hipEventDisableTiming));
+  int rc = chol_panel_exact<ArDD>(a, S, 0, std::min(nb, n), n, err, inv_h, inv_l, s);
+  if (rc) return rc;
   for (long c0 = 0; c0 < n; c0 += nb) {
     long jb = std::min(nb, n - c0);
-    int rc = chol_panel_exact<ArDD>(a, S, c0, jb, n, err, inv_h, inv_l, s);
-    if (rc) return rc;
     long rem = n - c0 - jb;
-    if (rem <= 0) continue;
-    ErrRec e;
-    HP_HIP(hipMemcpyAsync(&e, err, sizeof(e), hipMemcpyDeviceToHost, s));
-    HP_HIP(hipStreamSynchronize(s));
-    if (e.flag) return HP_OK;  // reported by the caller
-    SplitOperand P{a.hi + (c0 + jb) * a.ld + c0, a.lo + (c0 + jb) * a.ld + c0, a.ld, true};
-    double* th = tmp;
-    double* tl = tmp + (size_t)rem * rem;
-    if ((rc = split_gemm_general(P, P, th, tl, (size_t)rem, (size_t)rem, (size_t)rem, (size_t)jb, 31, s))) return rc;
-    hipLaunchKernelGGL(k_sub_lower, dim3(g1(rem * rem)), dim3(256), 0, s, a.hi, a.lo, a.ld, c0 + jb, th, tl, rem);
-    HP_LAUNCH_CHECK();
+    if (rem <= 0) break;
+    long t0 = c0 + jb, w2 = std::min(nb, rem);
+    if (evB_live) HP_HIP(hipStreamWaitEvent(s, evB, 0));
+    if ((rc = rank_update_sumsub<ArDD>(a, S, t0, rem, t0, w2, c0, jb, err, s))) return rc;
+    HP_HIP(hipEventRecord(evA, s));
+    // next panel on s, concurrently with the rest update below
+    if ((rc = chol_panel_exact<ArDD>(a, S, t0, w2, n, err, inv_h, inv_l, s))) return rc;
+    if (rem > w2) {
+      HP_HIP(hipStreamWaitEvent(s2, evA, 0));
+      long r0 = t0 + w2, m = n - r0;
+      SplitOperand P{a.hi + r0 * a.ld + c0, a.lo + r0 * a.ld + c0, a.ld, true};
+      SplitOpts opt;
+      opt.lower_only = opt.subtract = opt.same_operand = true;
+      if ((rc = split_gemm_general(P, P, a.hi + r0 * a.ld + r0, a.lo + r0 * a.ld + r0, (size_t)a.ld, (size_t)m,
+                                   (size_t)m, (size_t)jb, 31, s2, opt)))
+        return rc;
+      HP_HIP(hipEventRecord(evB, s2));
+      evB_live = true;
+    }
   }
+  if (evB_live) HP_HIP(hipStreamWaitEvent(s, evB, 0));
   return HP_OK;
 }
+
+int g_split_panel = 128;  // panel width of the split-dd factorisation
 
 int solve_refined(int kind, const double* sh, const double* sl, const double* bh, const double* bl, long n, double tol,
                   int max_iter, double* xh, double* xl, hp_refine_stats* st, hipStream_t s) {
@@ -255,14 +420,18 @@ int solve_refined(int kind, const double* sh, const double* sl, const double* bh
   st->factor_kind = kind;
   if (n == 0) return HP_OK;
   const size_t nn = (size_t)n * n;
-  DevBuf L, Sw, tmp, inv, err, vec, sc;
+  DevBuf L, Sw, inv, err, vec, sc;
   int rc;
   const int limbs = (kind == HP_FACTOR_F64) ? 1 : 2;
+  DevBuf binv;
+  const long nblk = (n + BW - 1) / BW;
   if ((rc = L.alloc(nn * limbs * sizeof(double))) || (rc = Sw.alloc(nn * limbs * sizeof(double))) ||
       (rc = inv.alloc(2 * 64 * sizeof(double))) || (rc = err.alloc(sizeof(ErrRec))) ||
-      (rc = vec.alloc(6 * n * sizeof(double))) || (rc = sc.alloc(64)))
+      (rc = vec.alloc(8 * n * sizeof(double))) || (rc = sc.alloc(64)) ||
+      (rc = binv.alloc((size_t)nblk * BW * BW * limbs * sizeof(double))))
     return rc;
-  if (kind == HP_FACTOR_DD_SPLIT && (rc = tmp.alloc(2 * nn * sizeof(double)))) return rc;
+  double* ivh = binv.d();
+  double* ivl = limbs == 2 ? binv.d() + (size_t)nblk * BW * BW : nullptr;
   HP_HIP(hipMemsetAsync(err.p, 0, sizeof(ErrRec), s));
   HP_HIP(hipStreamSynchronize(s));
   auto t0 = clk::now();
@@ -274,13 +443,18 @@ int solve_refined(int kind, const double* sh, const double* sl, const double* bh
   switch (kind) {
     case HP_FACTOR_F64: rc = chol_exact<ArF64>(Lm, n, 64, false, Sm, er, inv.d(), inv.d() + 64, s); break;
     case HP_FACTOR_DD: rc = chol_exact<ArDD>(Lm, n, 64, false, Sm, er, inv.d(), inv.d() + 64, s); break;
-    case HP_FACTOR_DD_SPLIT: rc = chol_split(Lm, n, 256, Sm, er, inv.d(), inv.d() + 64, tmp.d(), s); break;
+    case HP_FACTOR_DD_SPLIT: rc = chol_split(Lm, n, g_split_panel, Sm, er, inv.d(), inv.d() + 64, s); break;
     default: set_error("solve_refined: unknown factor kind %d", kind); return HP_EINVAL;
   }
   if (rc) return rc;
   ErrRec e;
   HP_HIP(hipMemcpyAsync(&e, er, sizeof(e), hipMemcpyDeviceToHost, s));
   HP_HIP(hipStreamSynchronize(s));
+  if (!e.flag) {
+    rc = (kind == HP_FACTOR_F64) ? diag_inverses<ArF64>(Lm, n, ivh, ivl, s) : diag_inverses<ArDD>(Lm, n, ivh, ivl, s);
+    if (rc) return rc;
+    HP_HIP(hipStreamSynchronize(s));
+  }
   auto t1 = clk::now();
   st->factor_ms = std::chrono::duration<double, std::milli>(t1 - t0).count();
   if (e.flag) {
@@ -293,6 +467,9 @@ int solve_refined(int kind, const double* sh, const double* sl, const double* bh
   double* dh = rh + 2 * n;
   double* dl = rh + 3 * n;
   double* d64 = rh + 4 * n;
+  double* w64 = rh + 5 * n;
+  double* wh = rh + 6 * n;
+  double* wl = rh + 7 * n;
   unsigned long long* scr = (unsigned long long*)sc.p;
   HP_HIP(hipMemsetAsync(xh, 0, n * sizeof(double), s));
   HP_HIP(hipMemsetAsync(xl, 0, n * sizeof(double), s));
@@ -310,14 +487,15 @@ int solve_refined(int kind, const double* sh, const double* sl, const double* bh
     if (kind == HP_FACTOR_F64) {
       hipLaunchKernelGGL(k_dd_to_f64, dim3(g1(n)), dim3(256), 0, s, rh, rlo, d64, n);
       HP_LAUNCH_CHECK();
-      if ((rc = fwd_exact<ArF64>(Lm, d64, nullptr, n, s)) || (rc = bwd_fast<ArF64>(Lm, d64, nullptr, n, s))) return rc;
+      HP_HIP(hipMemcpyAsync(w64, d64, n * sizeof(double), hipMemcpyDeviceToDevice, s));
+      if ((rc = solve_inv<ArF64>(Lm, ivh, nullptr, w64, nullptr, d64, nullptr, n, s))) return rc;
       hipLaunchKernelGGL(k_dd_add_f64, dim3(g1(n)), dim3(256), 0, s, xh, xl, d64, n);
       HP_LAUNCH_CHECK();
       dn = absmax_dev(d64, n, scr, s, &rc);
     } else {
-      HP_HIP(hipMemcpyAsync(dh, rh, n * sizeof(double), hipMemcpyDeviceToDevice, s));
-      HP_HIP(hipMemcpyAsync(dl, rlo, n * sizeof(double), hipMemcpyDeviceToDevice, s));
-      if ((rc = fwd_exact<ArDD>(Lm, dh, dl, n, s)) || (rc = bwd_fast<ArDD>(Lm, dh, dl, n, s))) return rc;
+      HP_HIP(hipMemcpyAsync(wh, rh, n * sizeof(double), hipMemcpyDeviceToDevice, s));
+      HP_HIP(hipMemcpyAsync(wl, rlo, n * sizeof(double), hipMemcpyDeviceToDevice, s));
+      if ((rc = solve_inv<ArDD>(Lm, ivh, ivl, wh, wl, dh, dl, n, s))) return rc;
       hipLaunchKernelGGL(k_dd_add, dim3(g1(n)), dim3(256), 0, s, xh, xl, dh, dl, n);
       HP_LAUNCH_CHECK();
       dn = absmax_dev(dh, n, scr, s, &rc);

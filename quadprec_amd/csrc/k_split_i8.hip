@@ -248,7 +248,8 @@ __device__ __forceinline__ v4i load16_guarded(const int8_t* p, long cnt) {
 template <int GK>
 __global__ __launch_bounds__(512) void k_gemm_i8_mod(const int8_t* __restrict__ A, const int8_t* __restrict__ B,
                                                      uint8_t* __restrict__ C, long M, long N, long K,
-                                                     const CrtTable* __restrict__ tab) {
+                                                     const CrtTable* __restrict__ tab, int lower) {
+  if (lower && blockIdx.x > blockIdx.y) return;  // symmetric product: tiles on / below the diagonal
   constexpr int GKP = GK + 16;   // row pitch: keeps the 16-lane groups of ds_read_b128 on distinct banks
   constexpr int NV = GK / 16;    // 16-byte vectors per row per stage
   extern __shared__ __align__(16) int8_t smem[];
@@ -360,7 +361,7 @@ __device__ __forceinline__ dd u64_to_dd(uint64_t x) {  // exact
 __global__ __launch_bounds__(256) void k_crt(const uint8_t* __restrict__ Cr, long M, long N,
                                              const int* __restrict__ er, const int* __restrict__ fc, int W,
                                              const CrtTable* __restrict__ tab, double* __restrict__ chi,
-                                             double* __restrict__ clo, long ldc) {
+                                             double* __restrict__ clo, long ldc, int lower, int subtract) {
   __shared__ uint32_t Mr[CRT_MAXR][CRT_LIMBS];
   __shared__ uint32_t Ml[CRT_LIMBS];
   __shared__ uint32_t mods[CRT_MAXR], ys[CRT_MAXR];
@@ -376,6 +377,7 @@ __global__ __launch_bounds__(256) void k_crt(const uint8_t* __restrict__ Cr, lon
   __syncthreads();
   const size_t plane = (size_t)M * N;
   for (size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x; idx < plane; idx += (size_t)gridDim.x * blockDim.x) {
+    if (lower && (long)(idx % N) > (long)(idx / N)) continue;
     uint64_t acc[CRT_LIMBS] = {0, 0, 0, 0, 0, 0, 0, 0};
     double t = 0.0;
     for (int r = 0; r < R; ++r) {
@@ -423,6 +425,7 @@ __global__ __launch_bounds__(256) void k_crt(const uint8_t* __restrict__ Cr, lon
       if (neg) res = dd_neg(res);
     }
     const size_t co = (size_t)(idx / N) * ldc + idx % N;
+    if (subtract) res = dd_sub(mkdd(chi[co], clo[co]), res);
     chi[co] = res.hi;
     clo[co] = res.lo;
   }
@@ -475,7 +478,11 @@ int crt_bits_W(int R, size_t k) {
 }
 
 int split_gemm_general(const SplitOperand& A, const SplitOperand& B, double* chi, double* clo, size_t ldc,
-                       size_t m, size_t n, size_t k, int R, hipStream_t s) {
+                       size_t m, size_t n, size_t k, int R, hipStream_t s, const SplitOpts& opt) {
+  if ((opt.lower_only || opt.same_operand) && m != n) {
+    set_error("split gemm: symmetric options need m == n");
+    return HP_EINVAL;
+  }
   if (m == 0 || n == 0) return HP_OK;
   if (R <= 0) R = 31;
   if (R < 8 || R > CRT_MAXR) {
@@ -545,25 +552,30 @@ int split_gemm_general(const SplitOperand& A, const SplitOperand& B, double* chi
     return HP_OK;
   };
   if (int rc = scales_and_residues(A, (long)m, eA, rA)) return rc;
-  if (int rc = scales_and_residues(B, (long)n, fB, rB)) return rc;
+  if (opt.same_operand) {
+    rB = rA;
+    fB = eA;
+  } else if (int rc = scales_and_residues(B, (long)n, fB, rB)) {
+    return rc;
+  }
   tm.mark(s);
   dim3 gg((unsigned)((n + GN - 1) / GN), (unsigned)((m + GM - 1) / GM), (unsigned)R);
   auto launch = [&](int variant) -> int {
     if (variant == 128)
       hipLaunchKernelGGL(k_gemm_i8_mod<128>, gg, dim3(512), gemm_i8_lds<128>(), s, rA, rB, rC, (long)m, (long)n,
-                         (long)k, g_work.tab);
+                         (long)k, g_work.tab, opt.lower_only ? 1 : 0);
     else if (variant == 32)
       hipLaunchKernelGGL(k_gemm_i8_mod<32>, gg, dim3(512), gemm_i8_lds<32>(), s, rA, rB, rC, (long)m, (long)n,
-                         (long)k, g_work.tab);
+                         (long)k, g_work.tab, opt.lower_only ? 1 : 0);
     else
       hipLaunchKernelGGL(k_gemm_i8_mod<64>, gg, dim3(512), gemm_i8_lds<64>(), s, rA, rB, rC, (long)m, (long)n,
-                         (long)k, g_work.tab);
+                         (long)k, g_work.tab, opt.lower_only ? 1 : 0);
     HP_LAUNCH_CHECK();
     return HP_OK;
   };
   // autotuner: the first call for a shape class times every K-stage depth on the real
   // operands (HIP events on this stream) and keeps the fastest
-  const uint64_t key = shape_key(m, n, k, R);  // R already defaulted
+  const uint64_t key = shape_key(m, n, k, R) | (opt.lower_only ? (1ull << 40) : 0);  // R already defaulted
   int variant = 0;
   {
     auto it = g_tuned.find(key);
@@ -599,7 +611,7 @@ int split_gemm_general(const SplitOperand& A, const SplitOperand& B, double* chi
   size_t total = m * n;
   unsigned gc = (unsigned)std::min<size_t>((total + 255) / 256, 1u << 16);
   hipLaunchKernelGGL(k_crt, dim3(gc), dim3(256), 0, s, rC, (long)m, (long)n, eA, fB, W, g_work.tab, chi, clo,
-                     (long)ldc);
+                     (long)ldc, opt.lower_only ? 1 : 0, opt.subtract ? 1 : 0);
   HP_LAUNCH_CHECK();
   tm.mark(s);
   // the workspace is reused by the next call: keep it owned until this one has run
@@ -612,7 +624,7 @@ int launch_gemm_split_i8(int trans_a, const double* ahi, const double* alo, cons
                          double* chi, double* clo, size_t m, size_t n, size_t k, int R, hipStream_t s) {
   SplitOperand A{ahi, alo, trans_a ? (long)m : (long)k, !trans_a};
   SplitOperand B{bhi, blo, (long)n, false};
-  return split_gemm_general(A, B, chi, clo, n, m, n, k, R, s);
+  return split_gemm_general(A, B, chi, clo, n, m, n, k, R, s, SplitOpts{});
 }
 
 }  // namespace hp
