@@ -165,6 +165,30 @@ int hp_solve_spd_refined(int factor_kind, const double* s_dd, const double* b_dd
 /* Panel width of HP_FACTOR_DD_SPLIT (64, 128, 256 or 512; process-wide, default 128). */
 int hp_set_split_factor_panel(int nb);
 
+/* ------------------------------------------------------------------ batched factor + solve */
+
+/* Many dd SPD systems of uneven size in one persistent launch: each instance is copied to
+ * L and factored with cholesky_blocked(nb = 64) (bit-identical to hp_cholesky_blocked),
+ * then L L^T X = B is solved for `nrhs` (1..8) right-hand sides (forward in reference
+ * order, backward right-looking blocked).  Instances are taken largest first from an
+ * atomic queue, one workgroup each.
+ * Layout (device): matrices packed in instance order, instance i at sum_{j<i} n_j^2 in each
+ * of the hi / lo planes (s_*, l_*); right-hand sides packed likewise, instance i's r-th
+ * vector at sum_{j<i} nrhs*n_j + r*n_i (b_*, x_*).  status[i] = 0, or p+1 when instance i
+ * broke down at pivot p. */
+int hp_dev_batch_cholesky_solve(const int* sizes, size_t count, int nrhs, const double* s_hi, const double* s_lo,
+                                double* l_hi, double* l_lo, const double* b_hi, const double* b_lo, double* x_hi,
+                                double* x_lo, int* status, void* stream);
+
+/* The same batch captured once in a hipGraph (queue reset by a kernel node + the
+ * persistent kernel) and replayed by hp_batch_graph_launch on any stream. */
+typedef struct hp_batch hp_batch;
+hp_batch* hp_batch_graph_create(const int* sizes, size_t count, int nrhs, const double* s_hi, const double* s_lo,
+                                double* l_hi, double* l_lo, const double* b_hi, const double* b_lo, double* x_hi,
+                                double* x_lo, int* status, void* stream);
+int hp_batch_graph_launch(hp_batch* batch, void* stream);
+void hp_batch_graph_destroy(hp_batch* batch);
+
 /* ------------------------------------------------------------------ overlap build */
 
 /* Normalised s-type Gaussian overlap in dd: S_uv = (4 a_u a_v/(a_u+a_v)^2)^{3/4}

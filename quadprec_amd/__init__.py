@@ -104,6 +104,13 @@ def lib():
     L.hp_solve_spd_refined.argtypes = [C.c_int, _dp, _dp, _sz, C.c_double, C.c_int, _dp, C.POINTER(RefineStats)]
     L.hp_set_split_factor_panel.argtypes = [C.c_int]
     L.hp_set_split_factor_panel.restype = C.c_int
+    _ip = np.ctypeslib.ndpointer(dtype=np.int32, flags="C_CONTIGUOUS")
+    L.hp_dev_batch_cholesky_solve.argtypes = [_ip, _sz, C.c_int] + [_vp] * 9 + [_vp]
+    L.hp_batch_graph_create.argtypes = [_ip, _sz, C.c_int] + [_vp] * 9 + [_vp]
+    L.hp_batch_graph_create.restype = _vp
+    L.hp_batch_graph_launch.argtypes = [_vp, _vp]
+    L.hp_batch_graph_destroy.argtypes = [_vp]
+    L.hp_batch_graph_destroy.restype = None
     L.hp_overlap_s.argtypes = [_dp, _dp, _sz, _dp]
     L.hp_dev_overlap_s.argtypes = [_vp, _vp, _sz, _vp, _vp, _vp]
     L.hp_gen_overlap_basis.argtypes = [C.c_uint64, _sz, C.c_double, C.c_double, C.c_double, C.c_double, _dp, _dp]

@@ -65,6 +65,42 @@ def solve_spd_refined(s, b, x, n, factor_kind, tol=1e-30, max_iter=20):
     return st
 
 
+def batch_cholesky_solve(sizes, nrhs, s, l, b, x, status, graph=False):
+    """Batched factor + solve on packed planes (see hp_dev_batch_cholesky_solve).
+    s, l: (2, sum n_i^2); b, x: (2, nrhs * sum n_i); status: int32 (count,).  With graph=True
+    returns a BatchGraph whose launch() replays the captured batch."""
+    import torch
+    sizes = np.ascontiguousarray(sizes, dtype=np.int32)
+    tot = int((sizes.astype(np.int64) ** 2).sum())
+    tr = int(sizes.astype(np.int64).sum()) * nrhs
+    for t, num, name in ((s, 2 * tot, "s"), (l, 2 * tot, "l"), (b, 2 * tr, "b"), (x, 2 * tr, "x")):
+        _need(t, num, name)
+    if status.dtype != torch.int32 or status.numel() != sizes.size:
+        raise TypeError("status: int32 tensor with one entry per instance")
+    args = (sizes, sizes.size, nrhs, _ptr(s[0]), _ptr(s[1]), _ptr(l[0]), _ptr(l[1]), _ptr(b[0]), _ptr(b[1]),
+            _ptr(x[0]), _ptr(x[1]), _ptr(status), _stream())
+    if not graph:
+        _check(lib().hp_dev_batch_cholesky_solve(*args), "dev.batch_cholesky_solve")
+        return None
+    h = lib().hp_batch_graph_create(*args)
+    if not h:
+        _check(4, "dev.batch_cholesky_solve(graph)")
+    return BatchGraph(h)
+
+
+class BatchGraph:
+    def __init__(self, h):
+        self.h = h
+
+    def launch(self):
+        _check(lib().hp_batch_graph_launch(self.h, _stream()), "batch graph launch")
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib().hp_batch_graph_destroy(self.h)
+            self.h = None
+
+
 def overlap_s(centers, alpha, s, n):
     """dd overlap matrix into planes s (2, n*n); centers (n*3,), alpha (n,) device tensors."""
     _check(lib().hp_dev_overlap_s(_ptr(centers), _ptr(alpha), n, _ptr(s[0]), _ptr(s[1]), _stream()), "dev.overlap_s")
