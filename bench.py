@@ -143,6 +143,98 @@ def bench_batch(qp, torch, dev, count=512, nrhs=4, seed=3):
             "breakdowns": failed, "matrix_bytes": 2 * 16 * tot}
 
 
+MF4_N = 1024
+# mean FP64 operations of one MF4 MAC s = s + a*b along the accumulation chains of these
+# inputs, from the oracle's op-counting restatement (re-derived by tests/test_oracle_mf4.py)
+MF4_OPS_PER_MAC_MEAN = 531
+
+
+def bench_mf4(qp, torch, dev, steps, cpu):
+    """Config 1, second part: MultiFloat<4> mf_gemm C = A*B at 1024^3 (limbs generated as
+    the dd operands, seed 1), the reference's per-element order (bit-identical)."""
+    n = MF4_N
+    a = qp.gen_mf4_normal(1, n * n)
+    b = qp.gen_mf4_normal(1 + 0x5EED, n * n)
+    da, db = qp.dev.to_planes(a, 4, dev), qp.dev.to_planes(b, 4, dev)
+    dc = torch.empty((4, n * n), dtype=torch.float64, device=dev)
+    stream = torch.cuda.current_stream()
+    ms, kern = time_steps(lambda: qp.dev.mf_gemm(da, db, dc, n, n, n), max(2, steps), 1, stream, torch)
+    macs = float(n) ** 3
+    ops = MF4_OPS_PER_MAC_MEAN * macs / (kern * 1e-3) / 1e12
+    out = {"workload": "MF4 mf_gemm C=A*B, m=n=k=1024, seed 1", "ms_per_step": round(ms, 3),
+           "value": round(2.0 * macs / (ms * 1e-3) / 1e9, 3), "unit": "MF4 GFLOP/s (2 per MAC)",
+           "roofline": {"bound": "valu", "kernel": "k_mf4_gemm", "kernel_ms": round(kern, 3),
+                        "ops_per_mac": MF4_OPS_PER_MAC_MEAN, "achieved": round(2.0 * ops, 3),
+                        "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s", "frac": round(2.0 * ops / FP64_PEAK_TFLOPS, 4),
+                        "note": "FP64 VALU ops per MAC from the op-counting restatement on these inputs (mean), "
+                                "FMA-equivalent (2 flops each)"}}
+    if cpu:
+        sys.path.insert(0, os.path.join(ROOT, "tests"))
+        import oracle_lib as O
+        threads = O.lib().hpo_num_threads()
+        rows = threads * 2
+        t0 = time.perf_counter()
+        O.mf4_gemm_rows(a, b, n, n, n, 0, rows)
+        t = time.perf_counter() - t0
+        out["cpu_baseline"] = {"value": round(2.0 * rows * n * n / t / 1e9, 4), "unit": "MF4 GFLOP/s",
+                               "cores": threads, "kind": "port",
+                               "sample": f"{rows} of {n} rows ({t:.1f} s), oracle C++ restatement, OpenMP rows"}
+    return out
+
+
+ADJ_N, ADJ_SEED, ADJ_B_SEED = 2048, 4, 44
+
+
+def bench_adjoint(qp, torch, dev, cpu):
+    """Config 4: gradients of log det S and b^T S^-1 b over 3N centre coordinates and N
+    exponents, N = 2048 Gaussians (lattice, seed 4, exponents in [0.01, 10]), dd.  Forward =
+    overlap build + factor + refined solve + log det; backward = L^-1, S^-1, contraction.
+    Accuracy against f256 central differences: tests/test_gpu_adjoint.py."""
+    n = ADJ_N
+    c, a = qp.gen_overlap_basis(ADJ_SEED, n, 0.01, 10.0)
+    b = qp.gen_dd_normal(ADJ_B_SEED, n)
+    dc = torch.from_numpy(c.ravel()).to(dev)
+    da = torch.from_numpy(a).to(dev)
+    db = qp.dev.to_planes(b, 2, dev)
+    out = {"workload": "d/d(R, alpha) of log det S and b^T S^-1 b, N=2048 Gaussians, seed 4, dd",
+           "parameters": 4 * n}
+    qp.set_phase_timing(True)
+    for label, fk, gm in (("dd_factor_plain_gemm", qp.HP_FACTOR_DD, qp.HP_GEMM_PLAIN),
+                          ("split_factor_split_gemm", qp.HP_FACTOR_DD_SPLIT, qp.HP_GEMM_SPLIT_I8)):
+        best = None
+        for _ in range(3):  # the first pass also warms allocations and the split autotuner
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            st = qp.dev.OverlapAdjoint(dc, da, db, fk)
+            torch.cuda.synchronize()
+            t1 = time.perf_counter()
+            st.gradients(gm)
+            torch.cuda.synchronize()
+            t2 = time.perf_counter()
+            inf = st.refresh_info()
+            rec = ((t2 - t0) * 1e3, (t1 - t0) * 1e3, (t2 - t1) * 1e3, inf.inverse_ms, inf.gram_ms, inf.contract_ms)
+            st.close()
+            if best is None or rec[0] < best[0]:
+                best = rec
+        bw_macs = n ** 3 / 3.0  # algorithmic: L^-1 (n^3/6 dd MACs) + lower triangle of L^-T L^-1 (n^3/6)
+        out[label] = {"ms": round(best[0], 2), "forward_ms": round(best[1], 2), "backward_ms": round(best[2], 2),
+                      "inverse_ms": round(best[3], 3), "gram_ms": round(best[4], 3),
+                      "contract_ms": round(best[5], 3),
+                      "backward_dd_gflops": round(2.0 * bw_macs / (best[2] * 1e-3) / 1e9, 1)}
+    qp.set_phase_timing(False)
+    if cpu:
+        sys.path.insert(0, os.path.join(ROOT, "tests"))
+        import oracle_lib as O
+        t0 = time.perf_counter()
+        O.overlap_adjoint_dd(c, a, b)
+        t = time.perf_counter() - t0
+        out["cpu_baseline"] = {"value": round(t * 1e3, 1), "unit": "ms", "cores": O.lib().hpo_num_threads(),
+                               "kind": "port", "sample": "the full N=2048 gradient pair: dd S build, "
+                               "cholesky_blocked, S^-1 by solve_cholesky per column, contraction (oracle C++, "
+                               "OpenMP over columns / rows)"}
+    return out
+
+
 def time_steps(step, steps, warmup, stream, torch):
     for _ in range(warmup):
         step()
@@ -166,6 +258,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-solve", action="store_true", help="skip the N=4096 SPD solve timings")
     ap.add_argument("--no-batch", action="store_true", help="skip the batched uneven solves")
+    ap.add_argument("--no-mf4", action="store_true", help="skip the MF4 1024^3 gemm")
+    ap.add_argument("--no-adjoint", action="store_true", help="skip the N=2048 adjoint")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); "
@@ -238,6 +332,10 @@ def main():
         out["spd_solve_n4096"] = bench_solves(qp, torch, dev)
     if not args.no_batch:
         out["batched_solves"] = bench_batch(qp, torch, dev)
+    if not args.no_mf4:
+        out["mf4_gemm_1024"] = bench_mf4(qp, torch, dev, args.steps, not args.no_cpu)
+    if not args.no_adjoint:
+        out["adjoint_n2048"] = bench_adjoint(qp, torch, dev, not args.no_cpu)
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(a, b)
     print(json.dumps(out), flush=True)

@@ -198,6 +198,50 @@ void hp_batch_graph_destroy(hp_batch* batch);
 int hp_overlap_s(const double* centers, const double* alpha, size_t n, double* s_dd);
 int hp_dev_overlap_s(const double* centers, const double* alpha, size_t n, double* s_hi, double* s_lo, void* stream);
 
+/* ------------------------------------------------------------------ adjoints */
+
+/* Gradients of F1 = log det S and F2 = b^T S^-1 b (S as hp_overlap_s, b dd) with respect to
+ * the 4n parameters: the 3n centre coordinates (Gaussian-major) then the n exponents.
+ * No reference counterpart: the reference has no differentiation.  The functions
+ * differentiated are its overlap entry (crates/overlap/src/integral.rs:29-56, closed form
+ * crates/demo/src/main.rs:104-124) and log det / solve of its cholesky_blocked factor
+ * (crates/quad/src/linalg.rs:238-293, 123-126).  Forward/backward follow
+ * torch.autograd.Function: the forward saves the factor L and x = S^-1 b in an opaque
+ * context; the backward forms a vector-Jacobian product from them, recomputing S^-1 and the
+ * overlap entries instead of storing them. */
+typedef struct hp_adjoint hp_adjoint;
+typedef struct {
+  double logdet[2];        /* dd log det S = 2 sum ln L_ii */
+  double quad[2];          /* dd b^T S^-1 b */
+  int refine_iters;        /* quad-double refinement steps of x = S^-1 b */
+  double last_correction;  /* max|dx| / max|x| of the last step */
+  size_t pivot;            /* failing pivot when the forward returns HP_ENOTSPD */
+  double forward_ms;       /* host wall time of the forward (build, factor, solve) */
+  double inverse_ms;       /* last backward, when phase timing is on: L^-1 */
+  double gram_ms;          /* S^-1 = L^-T L^-1 */
+  double contract_ms;      /* per-row contraction with recomputed overlap entries */
+} hp_adjoint_info;
+/* Forward on device memory: centers (3n), alpha (n), b planes (n each); factor_kind is
+ * HP_FACTOR_DD (reference-order factor) or HP_FACTOR_DD_SPLIT.  *ctx receives the saved
+ * state (free with hp_adjoint_destroy); info receives F1, F2.  HP_ENOTSPD if S is not
+ * positive definite (info->pivot set). */
+int hp_dev_overlap_forward(const double* centers, const double* alpha, const double* b_hi, const double* b_lo,
+                           size_t n, int factor_kind, hp_adjoint** ctx, hp_adjoint_info* info, void* stream);
+/* Backward: cot = host dd cotangents {c1.hi, c1.lo, c2.hi, c2.lo} of (F1, F2); writes
+ * c1 dF1 + c2 dF2 as dd planes: centres gc (3n), exponents ga (n), and b: gb = 2 c2 x
+ * (gb may be NULL).  gemm_mode HP_GEMM_PLAIN or HP_GEMM_SPLIT_I8 for the S^-1 products;
+ * with c1 = 0 no S^-1 is formed. */
+int hp_dev_overlap_backward(hp_adjoint* ctx, const double* cot, int gemm_mode, double* gc_hi, double* gc_lo,
+                            double* ga_hi, double* ga_lo, double* gb_hi, double* gb_lo, void* stream);
+/* Both full gradients (4n dd planes each; gld may be NULL to skip F1). */
+int hp_dev_overlap_gradients(hp_adjoint* ctx, int gemm_mode, double* gld_hi, double* gld_lo, double* gq_hi,
+                             double* gq_lo, void* stream);
+int hp_adjoint_get_info(const hp_adjoint* ctx, hp_adjoint_info* info);
+void hp_adjoint_destroy(hp_adjoint* ctx);
+/* Host convenience: forward and both gradients; g_ld_dd, g_q_dd: 4n dd (AoS). */
+int hp_overlap_adjoint(const double* centers, const double* alpha, const double* b_dd, size_t n, int factor_kind,
+                       int gemm_mode, hp_adjoint_info* info, double* g_ld_dd, double* g_q_dd);
+
 /* ------------------------------------------------------------------ measurement */
 
 /* Per-phase HIP-event timing of the multi-kernel entry points (no reference

@@ -605,6 +605,192 @@ void hpo_overlap_s_dd(const double* centers, const double* alpha, size_t n, doub
   }
 }
 
+// ---- adjoints of the overlap build: F1 = log det S, F2 = b^T S^-1 b ----
+// S_uu = 1 for every parameter value, and S_uv = S_vu depends on Gaussian u's parameters
+// only through row / column u, so with G = dF/dS (symmetric)
+//   dF/dtheta_u = 2 sum_{v != u} G_uv dS_uv/dtheta_u,
+//   dS_uv/dR_uc = -2 p (R_uc - R_vc) S_uv,                         p = a_u a_v / g, g = a_u + a_v
+//   dS_uv/da_u  = S_uv (3/4 (1/a_u - 2/g) - |R_u - R_v|^2 a_v^2 / g^2)
+// and G = S^-1 for F1, G = -x x^T (x = S^-1 b) for F2.  Everything below is built from the
+// reference's dd operations (crates/quad/src/dd.rs) and linalg (cholesky_blocked,
+// solve_cholesky: crates/quad/src/linalg.rs:99-126, 238-293).
+
+// the per-pair factors: t[c] = p (R_uc - R_vc) S_uv (c < 3), t[3] = dS_uv/da_u.
+// `mistake` plants a known error for the tests that check the checkers (0: none; 1: a_u for
+// a_v in the |R|^2 term; 2: the 3/4 term dropped; 3: the centre factor's sign flipped).
+static void adj_pair(double au, const double* cu, double av, const double* cv, D s, D t[4], int mistake) {
+  D a = dd_from(au), b = dd_from(av);
+  D g = add(a, b);
+  D p = div(mul(a, b), g);
+  D d[3];
+  for (int c = 0; c < 3; ++c) two_sum(cu[c], -cv[c], d[c].hi, d[c].lo);
+  D r2 = add(add(mul(d[0], d[0]), mul(d[1], d[1])), mul(d[2], d[2]));
+  D ps = mul(p, s);
+  if (mistake == 3) ps = neg(ps);
+  for (int c = 0; c < 3; ++c) t[c] = mul(ps, d[c]);
+  D bg = div(mistake == 1 ? a : b, g);
+  D t34 = mul_f(sub(recip(a), div(dd_from(2.0), g)), 0.75);
+  if (mistake == 2) t34 = dd_from(0.0);
+  D fa = sub(t34, mul(r2, mul(bg, bg)));
+  t[3] = mul(s, fa);
+}
+
+int hpo_overlap_adjoint_dd_planted(const double* centers, const double* alpha, const double* b, size_t n,
+                                   double* logdet, double* quad, double* g_ld, double* g_q, double* x_out,
+                                   int mistake) {
+  std::vector<double> s(2 * n * n), l;
+  hpo_overlap_s_dd(centers, alpha, n, s.data());
+  l = s;
+  size_t piv;
+  double val_;
+  if (hpo_cholesky_blocked(l.data(), n, 64, &piv, &val_)) return 1;
+  std::vector<double> x(2 * n), w(2 * n * n);
+  hpo_solve_cholesky(l.data(), b, x.data(), n);
+#pragma omp parallel for schedule(dynamic, 4)
+  for (long jj = 0; jj < (long)n; ++jj) {  // column j of S^-1 = solve_cholesky(L, e_j)
+    size_t j = (size_t)jj;
+    std::vector<double> e(2 * n, 0.0), col(2 * n);
+    e[2 * j] = 1.0;
+    hpo_solve_cholesky(l.data(), e.data(), col.data(), n);
+    for (size_t i = 0; i < n; ++i) { w[2 * (i * n + j)] = col[2 * i]; w[2 * (i * n + j) + 1] = col[2 * i + 1]; }
+  }
+  D ldv = dd_from(0.0), qv = dd_from(0.0);
+  for (size_t i = 0; i < n; ++i) ldv = add(ldv, mul_f(dln(ld(l.data(), i * n + i)), 2.0));
+  for (size_t i = 0; i < n; ++i) qv = add(qv, mul(ld(b, i), ld(x.data(), i)));
+  st(logdet, 0, ldv);
+  st(quad, 0, qv);
+#pragma omp parallel for schedule(dynamic, 8)
+  for (long uu = 0; uu < (long)n; ++uu) {
+    size_t u = (size_t)uu;
+    D a1[4], a2[4], t[4];
+    for (int c = 0; c < 4; ++c) a1[c] = a2[c] = dd_from(0.0);
+    for (size_t v = 0; v < n; ++v) {
+      if (v == u) continue;
+      adj_pair(alpha[u], centers + 3 * u, alpha[v], centers + 3 * v, ld(s.data(), u * n + v), t, mistake);
+      D wv = ld(w.data(), u * n + v), xv = ld(x.data(), v);
+      for (int c = 0; c < 4; ++c) {
+        a1[c] = add(a1[c], mul(wv, t[c]));
+        a2[c] = add(a2[c], mul(xv, t[c]));
+      }
+    }
+    D xu = ld(x.data(), u);
+    for (int c = 0; c < 3; ++c) {
+      st(g_ld, 3 * u + c, mul_f(a1[c], -4.0));
+      st(g_q, 3 * u + c, mul(mul_f(a2[c], 4.0), xu));
+    }
+    st(g_ld, 3 * n + u, mul_f(a1[3], 2.0));
+    st(g_q, 3 * n + u, mul(mul_f(a2[3], -2.0), xu));
+  }
+  if (x_out) std::memcpy(x_out, x.data(), sizeof(double) * 2 * n);
+  return 0;
+}
+int hpo_overlap_adjoint_dd(const double* centers, const double* alpha, const double* b, size_t n, double* logdet,
+                           double* quad, double* g_ld, double* g_q, double* x_out) {
+  return hpo_overlap_adjoint_dd_planted(centers, alpha, b, n, logdet, quad, g_ld, g_q, x_out, 0);
+}
+
+// f256 overlap entry from MultiFloat<4> parameters (the dd entry's formula in the
+// reference's MF4 ops, crates/quad/src/multifloat); arguments below -708 give 0.
+static M4 mf_overlap_entry(const M4& a, const M4* cu, const M4& b, const M4* cv) {
+  M4 g = mf_add(a, b), ab = mf_mul(a, b);
+  M4 p = mf_div(ab, g);
+  M4 ratio = mf_div(mf_mul_f(ab, 4.0), mf_mul(g, g));
+  M4 t = mf_sqrt(ratio);
+  M4 pref = mf_mul(t, mf_sqrt(t));
+  M4 r2 = mf_zero<double>();
+  for (int c = 0; c < 3; ++c) {
+    M4 d = mf_sub(cu[c], cv[c]);
+    r2 = mf_add(r2, mf_mul(d, d));
+  }
+  M4 arg = mf_neg(mf_mul(p, r2));
+  if (arg.l[0] < -708.0) return mf_zero<double>();
+  return mf_mul(pref, mf_exp(arg));
+}
+static M4 mf_log1p_small(const M4& z) {  // log(1+z) = 2 atanh(z/(2+z)), |z| small
+  M4 y = mf_div(z, mf_add(mf_from(2.0), z));
+  M4 y2 = mf_mul(y, y), term = y, sum = y;
+  for (int k = 1; k < 60; ++k) {
+    term = mf_mul(term, y2);
+    M4 q = mf_div(term, mf_from((double)(2 * k + 1)));
+    sum = mf_add(sum, q);
+    if (std::fabs(q.l[0]) <= 1e-70 * std::fabs(y.l[0])) break;
+  }
+  return mf_mul_f(sum, 2.0);
+}
+
+int hpo_overlap_fd_f256(const double* centers, const double* alpha, const double* b, size_t n, const long* coords,
+                        size_t ncoords, double h, double* fd_ld, double* fd_q) {
+  std::vector<M4> A(n), C(3 * n);
+  for (size_t i = 0; i < n; ++i) {
+    A[i] = mf_from(alpha[i]);
+    for (int c = 0; c < 3; ++c) C[3 * i + c] = mf_from(centers[3 * i + c]);
+  }
+  std::vector<double> S(4 * n * n);
+#pragma omp parallel for schedule(dynamic, 8)
+  for (long ii = 0; ii < (long)n; ++ii) {
+    size_t i = (size_t)ii;
+    stm(S.data(), i * n + i, mf_from(1.0));
+    for (size_t j = 0; j < i; ++j) {
+      M4 v = mf_overlap_entry(A[i], &C[3 * i], A[j], &C[3 * j]);
+      stm(S.data(), i * n + j, v);
+      stm(S.data(), j * n + i, v);
+    }
+  }
+  if (hpo_mf4_cholesky(S.data(), n)) return 1;
+  const double* L = S.data();
+  std::vector<double> b4(4 * n, 0.0), x4(4 * n);
+  for (size_t i = 0; i < n; ++i) { b4[4 * i] = b[2 * i]; b4[4 * i + 1] = b[2 * i + 1]; }
+  hpo_mf4_cholesky_solve(L, b4.data(), x4.data(), n);
+#pragma omp parallel for schedule(dynamic, 1)
+  for (long ci = 0; ci < (long)ncoords; ++ci) {
+    long k = coords[ci];
+    size_t u = (size_t)(k < 3 * (long)n ? k / 3 : k - 3 * (long)n);
+    int comp = k < 3 * (long)n ? (int)(k % 3) : 3;
+    std::vector<double> e(4 * n, 0.0), a(4 * n), uv(4 * n), cv(4 * n);
+    e[4 * u] = 1.0;
+    hpo_mf4_cholesky_solve(L, e.data(), a.data(), n);  // a = S^-1 e_u
+    std::vector<M4> base(n);
+    for (size_t v = 0; v < n; ++v)
+      base[v] = v == u ? mf_zero<double>() : mf_overlap_entry(A[u], &C[3 * u], A[v], &C[3 * v]);
+    M4 au = ldm(a.data(), u), xu = ldm(x4.data(), u);
+    for (int step = 0; step < 2; ++step) {
+      double hh = step == 0 ? h : 2.0 * h;
+      M4 dl[2], dq[2];
+      for (int sg = 0; sg < 2; ++sg) {
+        M4 ap = A[u], cp[3] = {C[3 * u], C[3 * u + 1], C[3 * u + 2]};
+        M4 dlt = mf_from(sg == 0 ? hh : -hh);
+        if (comp == 3) ap = mf_add(ap, dlt);
+        else cp[comp] = mf_add(cp[comp], dlt);
+        for (size_t v = 0; v < n; ++v) {
+          M4 d = v == u ? mf_zero<double>() : mf_sub(mf_overlap_entry(ap, cp, A[v], &C[3 * v]), base[v]);
+          stm(uv.data(), v, d);
+        }
+        // S + e_u u^T + u e_u^T = S + U V^T, U = [e_u, u], V = [u, e_u]:
+        // det(I + V^T S^-1 U) = (1 + c_u)^2 - (u^T c) a_u with c = S^-1 u, a = S^-1 e_u
+        hpo_mf4_cholesky_solve(L, uv.data(), cv.data(), n);
+        M4 cu = ldm(cv.data(), u), utc = mf_zero<double>(), q = mf_zero<double>();
+        for (size_t v = 0; v < n; ++v) {
+          utc = mf_add(utc, mf_mul(ldm(uv.data(), v), ldm(cv.data(), v)));
+          q = mf_add(q, mf_mul(ldm(x4.data(), v), ldm(uv.data(), v)));
+        }
+        M4 z = mf_sub(mf_add(mf_mul_f(cu, 2.0), mf_mul(cu, cu)), mf_mul(utc, au));
+        dl[sg] = mf_log1p_small(z);
+        // Woodbury: b^T (S + U V^T)^-1 b - b^T S^-1 b = -[p q] M^-1 [q p]^T, p = x_u, q = x^T u,
+        // M = [[1 + c_u, u^T c], [a_u, 1 + c_u]]
+        M4 m11 = mf_add(mf_from(1.0), cu);
+        M4 num = mf_sub(mf_sub(mf_mul(mf_mul(xu, q), mf_mul_f(m11, 2.0)), mf_mul(utc, mf_mul(xu, xu))),
+                        mf_mul(au, mf_mul(q, q)));
+        dq[sg] = mf_neg(mf_div(num, mf_add(mf_from(1.0), z)));
+      }
+      M4 gl = mf_div(mf_sub(dl[0], dl[1]), mf_from(2.0 * hh));
+      M4 gq = mf_div(mf_sub(dq[0], dq[1]), mf_from(2.0 * hh));
+      stm(fd_ld, (size_t)ci * 2 + step, gl);
+      stm(fd_q, (size_t)ci * 2 + step, gq);
+    }
+  }
+  return 0;
+}
+
 }  // extern "C"
 
 // ------------------------------------------------------------------ op counting

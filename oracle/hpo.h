@@ -115,6 +115,24 @@ void hpo_assemble_overlap_s(const double* centers, const double* alpha, const do
  *   S = (4 a b / (a+b)^2)^{3/4} exp(-(a b/(a+b)) |Ri-Rj|^2), Ri-Rj by TwoSum.  Full n x n, AoS dd. */
 hpo_dd hpo_overlap_s_dd_entry(double ai, const double* ci, double aj, const double* cj);
 void hpo_overlap_s_dd(const double* centers, const double* alpha, size_t n, double* s_dd);
+/* gradients of log det S and b^T S^-1 b over the 4n parameters (3n centre coordinates,
+ * Gaussian-major, then n exponents), from the reference's dd ops: S^-1 by solve_cholesky
+ * against unit vectors, x = S^-1 b.  logdet, quad: 1 dd; g_ld, g_q: 4n dd; x_out: n dd
+ * (may be NULL).  Returns 1 if S is not positive definite. */
+int hpo_overlap_adjoint_dd(const double* centers, const double* alpha, const double* b_dd, size_t n, double* logdet,
+                           double* quad, double* g_ld, double* g_q, double* x_out);
+/* the same with a planted mistake, for tests of the checks themselves (1: a_u in place of
+ * a_v in the |R|^2 a_v^2/g^2 term; 2: the 3/4 (1/a_u - 2/g) term dropped; 3: the sign of
+ * dS/dR flipped) */
+int hpo_overlap_adjoint_dd_planted(const double* centers, const double* alpha, const double* b_dd, size_t n,
+                                   double* logdet, double* quad, double* g_ld, double* g_q, double* x_out,
+                                   int mistake);
+/* f256 central finite differences of log det S and b^T S^-1 b at the given coordinates
+ * (indices into the 4n parameters), steps h and 2h, each evaluated exactly through a
+ * rank-2 update of the f256 factor (determinant lemma, Woodbury).  fd_ld, fd_q:
+ * ncoords x 2 (step) MF4 values.  Returns 1 if S is not positive definite. */
+int hpo_overlap_fd_f256(const double* centers, const double* alpha, const double* b_dd, size_t n, const long* coords,
+                        size_t ncoords, double h, double* fd_ld, double* fd_q);
 
 /* ---- counting: FP64 ops per unit of work, from an instrumented restatement ---- */
 /* returns the number of binary64 operations (+,-,*,/,fma,sqrt) one dd MAC s += a*b costs (29) */

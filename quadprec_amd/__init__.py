@@ -50,6 +50,13 @@ class RefineStats(C.Structure):
 HP_FACTOR_F64, HP_FACTOR_DD, HP_FACTOR_DD_SPLIT = 0, 1, 2
 
 
+class AdjointInfo(C.Structure):
+    """hp_adjoint_info: dd log det S and b^T S^-1 b of the forward, refinement and timings."""
+    _fields_ = [("logdet", C.c_double * 2), ("quad", C.c_double * 2), ("refine_iters", C.c_int),
+                ("last_correction", C.c_double), ("pivot", C.c_size_t), ("forward_ms", C.c_double),
+                ("inverse_ms", C.c_double), ("gram_ms", C.c_double), ("contract_ms", C.c_double)]
+
+
 class Strategy:
     """crates/solver/src/lib.rs:25-30"""
     Float64, MixedPrecision, Float128 = 0, 1, 2
@@ -113,6 +120,14 @@ def lib():
     L.hp_batch_graph_destroy.restype = None
     L.hp_overlap_s.argtypes = [_dp, _dp, _sz, _dp]
     L.hp_dev_overlap_s.argtypes = [_vp, _vp, _sz, _vp, _vp, _vp]
+    L.hp_dev_overlap_forward.argtypes = [_vp, _vp, _vp, _vp, _sz, C.c_int, C.POINTER(_vp), C.POINTER(AdjointInfo),
+                                         _vp]
+    L.hp_dev_overlap_backward.argtypes = [_vp, _dp, C.c_int] + [_vp] * 6 + [_vp]
+    L.hp_dev_overlap_gradients.argtypes = [_vp, C.c_int] + [_vp] * 4 + [_vp]
+    L.hp_adjoint_get_info.argtypes = [_vp, C.POINTER(AdjointInfo)]
+    L.hp_adjoint_destroy.argtypes = [_vp]
+    L.hp_adjoint_destroy.restype = None
+    L.hp_overlap_adjoint.argtypes = [_dp, _dp, _dp, _sz, C.c_int, C.c_int, C.POINTER(AdjointInfo), _dp, _dp]
     L.hp_gen_overlap_basis.argtypes = [C.c_uint64, _sz, C.c_double, C.c_double, C.c_double, C.c_double, _dp, _dp]
     L.hp_gen_overlap_basis.restype = None
     L.hp_split_gemm_variant.argtypes = [_sz, _sz, _sz, C.c_int]
@@ -292,6 +307,25 @@ def overlap_s(centers, alpha):
     s = np.zeros((n * n, 2))
     _check(lib().hp_overlap_s(centers, alpha, n, s), "overlap_s")
     return s
+
+
+def overlap_adjoint(centers, alpha, b, factor_kind=HP_FACTOR_DD, gemm_mode=HP_GEMM_PLAIN):
+    """Gradients of log det S and b^T S^-1 b (S = overlap_s(centers, alpha), b dd (n, 2))
+    over the 4n parameters (3n centre coordinates, Gaussian-major, then n exponents).
+    Returns (g_logdet (4n, 2), g_quad (4n, 2), AdjointInfo); raises SolveError if S is not
+    positive definite."""
+    alpha = np.ascontiguousarray(alpha, dtype=np.float64)
+    n = alpha.size
+    centers = _arr(centers, 3, n, "centers")
+    b = _arr(b, 2, n, "b")
+    gl = np.zeros((4 * n, 2))
+    gq = np.zeros((4 * n, 2))
+    info = AdjointInfo()
+    rc = lib().hp_overlap_adjoint(centers, alpha, b, n, factor_kind, gemm_mode, C.byref(info), gl, gq)
+    if rc == HP_ENOTSPD:
+        raise SolveError(info.pivot)
+    _check(rc, "overlap_adjoint")
+    return gl, gq, info
 
 
 def gen_overlap_basis(seed, n, alpha_lo=0.01, alpha_hi=10.0, spacing=1.4, jitter=0.05):

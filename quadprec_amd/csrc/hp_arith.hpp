@@ -111,6 +111,16 @@ HP_HD dd dd_exp(dd a) {
   double scale = __builtin_bit_cast(double, bits);
   return dd_mul(sum, mkdd(scale));
 }
+// ln(a): binary64 seed log(a.hi), then three Newton steps x += a exp(-x) - 1 in dd
+// (dd.rs:124-137).  NaN for a.hi <= 0, exactly 0 for a == 1.  The seed comes from the
+// platform's log; after the Newton steps the result is fixed by the dd operations.
+HP_HD dd dd_ln(dd a) {
+  if (a.hi <= 0.0) return mkdd(__builtin_nan(""), __builtin_nan(""));
+  if (a.hi == 1.0 && a.lo == 0.0) return mkdd(0.0, 0.0);
+  dd x = mkdd(log(a.hi));
+  for (int i = 0; i < 3; ++i) x = dd_add(x, dd_sub(dd_mul(a, dd_exp(dd_neg(x))), mkdd(1.0)));
+  return x;
+}
 
 // ------------------------------------------------------------------ MultiFloat<4>
 HP_HD mf4 mkmf(double a, double b = 0.0, double c = 0.0, double d = 0.0) {

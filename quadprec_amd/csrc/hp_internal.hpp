@@ -77,6 +77,10 @@ int download_planes(const double* dev_planes, size_t count, int limbs, double* h
 int launch_gemm_dd_plain(int trans_a, const double* ahi, const double* alo, const double* bhi,
                          const double* blo, double* chi, double* clo, size_t m, size_t n, size_t k,
                          hipStream_t s);
+// strided / negated / batched / triangular form (see k_gemm_dd.hip)
+int launch_gemm_dd_plain_ex(int trans_a, const double* ahi, const double* alo, long lda, const double* bhi,
+                            const double* blo, long ldb, double* chi, double* clo, long ldc, size_t m, size_t n,
+                            size_t k, bool neg, bool tri, int batch, long sa, long sb, long sc, hipStream_t s);
 int launch_gemm_split_i8(int trans_a, const double* ahi, const double* alo, const double* bhi, const double* blo,
                          double* chi, double* clo, size_t m, size_t n, size_t k, int R, hipStream_t s);
 // operand of the split GEMM: logical element (row, p) at hi[row*ld + p] when contig, else

@@ -17,24 +17,9 @@
 // of a 16x16 tile of the lower triangle; each entry is mirrored into the upper triangle.
 #include "hp_arith.hpp"
 #include "hp_internal.hpp"
+#include "hp_overlap.hpp"
 
 namespace hp {
-
-__host__ __device__ __forceinline__ dd overlap_s_entry(double au, const double* cu, double av, const double* cv) {
-  dd a = mkdd(au), b = mkdd(av);
-  dd g = dd_add(a, b);
-  dd ab = dd_mul(a, b);
-  dd p = dd_div(ab, g);
-  dd ratio = dd_div(dd_mul_f(ab, 4.0), dd_mul(g, g));
-  dd t = dd_sqrt(ratio);
-  dd pref = dd_mul(t, dd_sqrt(t));
-  dd d[3];
-  for (int c = 0; c < 3; ++c) two_sum(cu[c], -cv[c], d[c].hi, d[c].lo);
-  dd r2 = dd_add(dd_add(dd_mul(d[0], d[0]), dd_mul(d[1], d[1])), dd_mul(d[2], d[2]));
-  dd arg = dd_neg(dd_mul(p, r2));
-  if (arg.hi < -708.0) return mkdd(0.0, 0.0);
-  return dd_mul(pref, dd_exp(arg));
-}
 
 namespace {
 __global__ __launch_bounds__(256) void k_overlap_s(const double* __restrict__ centers, const double* __restrict__ alpha,

@@ -413,55 +413,15 @@ int chol_split(Mat a, long n, long nb, Mat S, ErrRec* err, double* inv_h, double
 
 int g_split_panel = 128;  // panel width of the split-dd factorisation
 
-int solve_refined(int kind, const double* sh, const double* sl, const double* bh, const double* bl, long n, double tol,
-                  int max_iter, double* xh, double* xl, hp_refine_stats* st, hipStream_t s) {
-  using clk = std::chrono::steady_clock;
-  std::memset(st, 0, sizeof(*st));
-  st->factor_kind = kind;
-  if (n == 0) return HP_OK;
-  const size_t nn = (size_t)n * n;
-  DevBuf L, Sw, inv, err, vec, sc;
+// x = S^-1 b by corrections through a given factor L (f64 or dd by kind) and the inverses of
+// its 64 x 64 diagonal blocks: r = b - S x in quad-double, x += L^-T L^-1 r, until the
+// correction's max-norm falls to tol relative to x (or max_iter steps).
+int refine_loop(int kind, Mat Lm, const double* ivh, const double* ivl, const double* sh, const double* sl,
+                const double* bh, const double* bl, long n, double tol, int max_iter, double* xh, double* xl,
+                int* iters, double* last_out, hipStream_t s) {
   int rc;
-  const int limbs = (kind == HP_FACTOR_F64) ? 1 : 2;
-  DevBuf binv;
-  const long nblk = (n + BW - 1) / BW;
-  if ((rc = L.alloc(nn * limbs * sizeof(double))) || (rc = Sw.alloc(nn * limbs * sizeof(double))) ||
-      (rc = inv.alloc(2 * 64 * sizeof(double))) || (rc = err.alloc(sizeof(ErrRec))) ||
-      (rc = vec.alloc(8 * n * sizeof(double))) || (rc = sc.alloc(64)) ||
-      (rc = binv.alloc((size_t)nblk * BW * BW * limbs * sizeof(double))))
-    return rc;
-  double* ivh = binv.d();
-  double* ivl = limbs == 2 ? binv.d() + (size_t)nblk * BW * BW : nullptr;
-  HP_HIP(hipMemsetAsync(err.p, 0, sizeof(ErrRec), s));
-  HP_HIP(hipStreamSynchronize(s));
-  auto t0 = clk::now();
-  Mat Lm{L.d(), limbs == 2 ? L.d() + nn : nullptr, n};
-  Mat Sm{Sw.d(), limbs == 2 ? Sw.d() + nn : nullptr, n};
-  HP_HIP(hipMemcpyAsync(Lm.hi, sh, nn * sizeof(double), hipMemcpyDeviceToDevice, s));
-  if (limbs == 2) HP_HIP(hipMemcpyAsync(Lm.lo, sl, nn * sizeof(double), hipMemcpyDeviceToDevice, s));
-  ErrRec* er = (ErrRec*)err.p;
-  switch (kind) {
-    case HP_FACTOR_F64: rc = chol_exact<ArF64>(Lm, n, 64, false, Sm, er, inv.d(), inv.d() + 64, s); break;
-    case HP_FACTOR_DD: rc = chol_exact<ArDD>(Lm, n, 64, false, Sm, er, inv.d(), inv.d() + 64, s); break;
-    case HP_FACTOR_DD_SPLIT: rc = chol_split(Lm, n, g_split_panel, Sm, er, inv.d(), inv.d() + 64, s); break;
-    default: set_error("solve_refined: unknown factor kind %d", kind); return HP_EINVAL;
-  }
-  if (rc) return rc;
-  ErrRec e;
-  HP_HIP(hipMemcpyAsync(&e, er, sizeof(e), hipMemcpyDeviceToHost, s));
-  HP_HIP(hipStreamSynchronize(s));
-  if (!e.flag) {
-    rc = (kind == HP_FACTOR_F64) ? diag_inverses<ArF64>(Lm, n, ivh, ivl, s) : diag_inverses<ArDD>(Lm, n, ivh, ivl, s);
-    if (rc) return rc;
-    HP_HIP(hipStreamSynchronize(s));
-  }
-  auto t1 = clk::now();
-  st->factor_ms = std::chrono::duration<double, std::milli>(t1 - t0).count();
-  if (e.flag) {
-    set_error("Cholesky failed at pivot %lld (value=%.4e)", e.pivot, e.value);
-    st->pivot = (size_t)e.pivot;
-    return HP_ENOTSPD;
-  }
+  DevBuf vec, sc;
+  if ((rc = vec.alloc(8 * n * sizeof(double))) || (rc = sc.alloc(64))) return rc;
   double* rh = vec.d();
   double* rlo = rh + n;
   double* dh = rh + 2 * n;
@@ -506,6 +466,78 @@ int solve_refined(int kind, const double* sh, const double* sl, const double* bh
     last = xn > 0 ? dn / xn : dn;
     if (last <= tol) { ++it; break; }
   }
+  *iters = it;
+  *last_out = last;
+  return HP_OK;
+}
+
+// dd solve with a given dd factor (reference-order or split), refined as above
+int solve_with_dd_factor(Mat Lm, const double* sh, const double* sl, const double* bh, const double* bl, long n,
+                         double tol, int max_iter, double* xh, double* xl, int* iters, double* last, hipStream_t s) {
+  const long nblk = (n + BW - 1) / BW;
+  DevBuf binv;
+  int rc;
+  if ((rc = binv.alloc((size_t)nblk * BW * BW * 2 * sizeof(double)))) return rc;
+  double* ivh = binv.d();
+  double* ivl = binv.d() + (size_t)nblk * BW * BW;
+  if ((rc = diag_inverses<ArDD>(Lm, n, ivh, ivl, s))) return rc;
+  rc = refine_loop(HP_FACTOR_DD, Lm, ivh, ivl, sh, sl, bh, bl, n, tol, max_iter, xh, xl, iters, last, s);
+  if (rc == HP_OK) HP_HIP(hipStreamSynchronize(s));  // binv is freed on return
+  return rc;
+}
+
+
+int solve_refined(int kind, const double* sh, const double* sl, const double* bh, const double* bl, long n, double tol,
+                  int max_iter, double* xh, double* xl, hp_refine_stats* st, hipStream_t s) {
+  using clk = std::chrono::steady_clock;
+  std::memset(st, 0, sizeof(*st));
+  st->factor_kind = kind;
+  if (n == 0) return HP_OK;
+  const size_t nn = (size_t)n * n;
+  DevBuf L, Sw, inv, err;
+  int rc;
+  const int limbs = (kind == HP_FACTOR_F64) ? 1 : 2;
+  DevBuf binv;
+  const long nblk = (n + BW - 1) / BW;
+  if ((rc = L.alloc(nn * limbs * sizeof(double))) || (rc = Sw.alloc(nn * limbs * sizeof(double))) ||
+      (rc = inv.alloc(2 * 64 * sizeof(double))) || (rc = err.alloc(sizeof(ErrRec))) ||
+      (rc = binv.alloc((size_t)nblk * BW * BW * limbs * sizeof(double))))
+    return rc;
+  double* ivh = binv.d();
+  double* ivl = limbs == 2 ? binv.d() + (size_t)nblk * BW * BW : nullptr;
+  HP_HIP(hipMemsetAsync(err.p, 0, sizeof(ErrRec), s));
+  HP_HIP(hipStreamSynchronize(s));
+  auto t0 = clk::now();
+  Mat Lm{L.d(), limbs == 2 ? L.d() + nn : nullptr, n};
+  Mat Sm{Sw.d(), limbs == 2 ? Sw.d() + nn : nullptr, n};
+  HP_HIP(hipMemcpyAsync(Lm.hi, sh, nn * sizeof(double), hipMemcpyDeviceToDevice, s));
+  if (limbs == 2) HP_HIP(hipMemcpyAsync(Lm.lo, sl, nn * sizeof(double), hipMemcpyDeviceToDevice, s));
+  ErrRec* er = (ErrRec*)err.p;
+  switch (kind) {
+    case HP_FACTOR_F64: rc = chol_exact<ArF64>(Lm, n, 64, false, Sm, er, inv.d(), inv.d() + 64, s); break;
+    case HP_FACTOR_DD: rc = chol_exact<ArDD>(Lm, n, 64, false, Sm, er, inv.d(), inv.d() + 64, s); break;
+    case HP_FACTOR_DD_SPLIT: rc = chol_split(Lm, n, g_split_panel, Sm, er, inv.d(), inv.d() + 64, s); break;
+    default: set_error("solve_refined: unknown factor kind %d", kind); return HP_EINVAL;
+  }
+  if (rc) return rc;
+  ErrRec e;
+  HP_HIP(hipMemcpyAsync(&e, er, sizeof(e), hipMemcpyDeviceToHost, s));
+  HP_HIP(hipStreamSynchronize(s));
+  if (!e.flag) {
+    rc = (kind == HP_FACTOR_F64) ? diag_inverses<ArF64>(Lm, n, ivh, ivl, s) : diag_inverses<ArDD>(Lm, n, ivh, ivl, s);
+    if (rc) return rc;
+    HP_HIP(hipStreamSynchronize(s));
+  }
+  auto t1 = clk::now();
+  st->factor_ms = std::chrono::duration<double, std::milli>(t1 - t0).count();
+  if (e.flag) {
+    set_error("Cholesky failed at pivot %lld (value=%.4e)", e.pivot, e.value);
+    st->pivot = (size_t)e.pivot;
+    return HP_ENOTSPD;
+  }
+  int it = 0;
+  double last = 1.0;
+  if ((rc = refine_loop(kind, Lm, ivh, ivl, sh, sl, bh, bl, n, tol, max_iter, xh, xl, &it, &last, s))) return rc;
   auto t2 = clk::now();
   st->iterations = it;
   st->last_correction = last;
@@ -513,4 +545,11 @@ int solve_refined(int kind, const double* sh, const double* sl, const double* bh
   return HP_OK;
 }
 
+}  // namespace hp
+
+namespace hp {
+// inverses of the 64 x 64 diagonal blocks of a dd factor, stored [block][i][c]
+int diag_block_inverses_dd(Mat l, long n, double* ih, double* il, hipStream_t s) {
+  return diag_inverses<ArDD>(l, n, ih, il, s);
+}
 }  // namespace hp

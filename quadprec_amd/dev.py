@@ -114,3 +114,120 @@ def mf_gemm(a, b, c, m, n, k, trans_a=False):
     _need(c, 4 * m * n, "c")
     _check(lib().hp_dev_mf_gemm(int(trans_a), _ptr(a), _ptr(b), _ptr(c), m, n, k, _stream()), "dev.mf_gemm")
     return c
+
+
+# ---------------------------------------------------------------- adjoints
+class OverlapAdjoint:
+    """Saved state of one forward evaluation of F1 = log det S and F2 = b^T S^-1 b for the
+    overlap matrix S(centers, alpha) (hp_dev_overlap_forward): the factor L and x = S^-1 b.
+    centers (n, 3) / (3n,), alpha (n,) float64 device tensors; b dd planes (2, n)."""
+
+    def __init__(self, centers, alpha, b, factor_kind=None):
+        import ctypes as C
+        import torch
+        from . import AdjointInfo, HP_FACTOR_DD, HP_ENOTSPD, SolveError
+        n = alpha.numel()
+        _need(centers, 3 * n, "centers")
+        _need(alpha, n, "alpha")
+        _need(b, 2 * n, "b")
+        self.n = n
+        self.h = None
+        self.info = AdjointInfo()
+        h = C.c_void_p()
+        kind = HP_FACTOR_DD if factor_kind is None else factor_kind
+        rc = lib().hp_dev_overlap_forward(_ptr(centers), _ptr(alpha), _ptr(b[0]), _ptr(b[1]), n, kind, C.byref(h),
+                                          C.byref(self.info), _stream())
+        if rc == HP_ENOTSPD:
+            raise SolveError(self.info.pivot)
+        _check(rc, "dev.overlap_forward")
+        self.h = h.value
+        self.device = alpha.device
+
+    @property
+    def logdet(self):
+        return np.array(self.info.logdet[:])
+
+    @property
+    def quad(self):
+        return np.array(self.info.quad[:])
+
+    def gradients(self, gemm_mode=HP_GEMM_PLAIN, logdet=True):
+        """Both gradients as dd planes (2, 4n): (g_logdet or None, g_quad)."""
+        import torch
+        gq = torch.empty((2, 4 * self.n), dtype=torch.float64, device=self.device)
+        gl = torch.empty_like(gq) if logdet else None
+        rc = lib().hp_dev_overlap_gradients(self.h, gemm_mode, _ptr(gl[0]) if logdet else None,
+                                            _ptr(gl[1]) if logdet else None, _ptr(gq[0]), _ptr(gq[1]), _stream())
+        _check(rc, "dev.overlap_gradients")
+        return gl, gq
+
+    def backward(self, cot_logdet, cot_quad, gemm_mode=HP_GEMM_PLAIN, grad_b=True):
+        """Vector-Jacobian product with dd cotangents (hi, lo) of (F1, F2): returns dd planes
+        (grad_centers (2, 3n), grad_alpha (2, n), grad_b (2, n) or None)."""
+        import torch
+        cot = np.array([cot_logdet[0], cot_logdet[1], cot_quad[0], cot_quad[1]], dtype=np.float64)
+        gc = torch.empty((2, 3 * self.n), dtype=torch.float64, device=self.device)
+        ga = torch.empty((2, self.n), dtype=torch.float64, device=self.device)
+        gb = torch.empty((2, self.n), dtype=torch.float64, device=self.device) if grad_b else None
+        rc = lib().hp_dev_overlap_backward(self.h, cot, gemm_mode, _ptr(gc[0]), _ptr(gc[1]), _ptr(ga[0]), _ptr(ga[1]),
+                                           _ptr(gb[0]) if grad_b else None, _ptr(gb[1]) if grad_b else None,
+                                           _stream())
+        _check(rc, "dev.overlap_backward")
+        return gc, ga, gb
+
+    def refresh_info(self):
+        import ctypes as C
+        _check(lib().hp_adjoint_get_info(self.h, C.byref(self.info)), "adjoint info")
+        return self.info
+
+    def close(self):
+        if self.h:
+            lib().hp_adjoint_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+
+def _overlap_function():
+    import torch
+
+    class OverlapLogdetQuad(torch.autograd.Function):
+        """(log det S, b^T S^-1 b) of the overlap matrix as a torch.autograd.Function on
+        binary64 leaves: centers (n, 3), alpha (n,), b (n,) (b enters as an exact dd with a
+        zero low part).  The values are returned rounded to binary64; the dd values stay on
+        ctx.state.  backward saves nothing but the factor and x (inside ctx.state) and returns
+        the dd vector-Jacobian product rounded to binary64."""
+
+        @staticmethod
+        def forward(ctx, centers, alpha, b, gemm_mode=HP_GEMM_PLAIN):
+            bp = torch.stack([b.detach().contiguous(), torch.zeros_like(b)])
+            st = OverlapAdjoint(centers.detach().contiguous().reshape(-1), alpha.detach().contiguous(), bp)
+            ctx.state = st
+            ctx.gemm_mode = gemm_mode
+            ld, q = st.logdet, st.quad
+            return (torch.tensor(ld[0] + ld[1], dtype=torch.float64, device=alpha.device),
+                    torch.tensor(q[0] + q[1], dtype=torch.float64, device=alpha.device))
+
+        @staticmethod
+        def backward(ctx, g_ld, g_q):
+            c1 = (float(g_ld), 0.0) if g_ld is not None else (0.0, 0.0)
+            c2 = (float(g_q), 0.0) if g_q is not None else (0.0, 0.0)
+            gc, ga, gb = ctx.state.backward(c1, c2, ctx.gemm_mode, grad_b=ctx.needs_input_grad[2])
+            gcen = (gc[0] + gc[1]).reshape(-1, 3)
+            galp = ga[0] + ga[1]
+            gbv = (gb[0] + gb[1]) if gb is not None else None
+            return gcen, galp, gbv, None
+
+    return OverlapLogdetQuad
+
+
+_OVERLAP_FN = None
+
+
+def overlap_logdet_quad(centers, alpha, b, gemm_mode=HP_GEMM_PLAIN):
+    """Differentiable (log det S, b^T S^-1 b) for torch float64 tensors on the device."""
+    global _OVERLAP_FN
+    if _OVERLAP_FN is None:
+        _OVERLAP_FN = _overlap_function()
+    return _OVERLAP_FN.apply(centers, alpha, b, gemm_mode)

@@ -87,6 +87,14 @@ def lib():
         L.hpo_overlap_s_dd_entry.argtypes = [C.c_double, _dp, C.c_double, _dp]
         L.hpo_overlap_s_dd_entry.restype = DD
         L.hpo_overlap_s_dd.argtypes = [_dp, _dp, _sz, _dp]
+        L.hpo_overlap_adjoint_dd.argtypes = [_dp, _dp, _dp, _sz, _dp, _dp, _dp, _dp, C.c_void_p]
+        L.hpo_overlap_adjoint_dd.restype = C.c_int
+        L.hpo_overlap_adjoint_dd_planted.argtypes = [_dp, _dp, _dp, _sz, _dp, _dp, _dp, _dp, C.c_void_p, C.c_int]
+        L.hpo_overlap_adjoint_dd_planted.restype = C.c_int
+        L.hpo_overlap_fd_f256.argtypes = [_dp, _dp, _dp, _sz,
+                                          np.ctypeslib.ndpointer(dtype=np.int64, flags="C_CONTIGUOUS"),
+                                          _sz, C.c_double, _dp, _dp]
+        L.hpo_overlap_fd_f256.restype = C.c_int
         L.hpo_count_dd_mac.restype = C.c_long
         L.hpo_count_mf4_mac.argtypes = [_dp, _dp, _dp]
         L.hpo_count_mf4_mac.restype = C.c_long
@@ -301,6 +309,34 @@ def assemble_overlap_s(centers, alpha, coeff, params=(0.1, 500.0, 1e-14, 12.0)):
     counts = np.zeros(3, dtype=np.uintp)
     lib().hpo_assemble_overlap_s(_c(centers), _c(alpha), _c(coeff), n, _c(params), s, counts)
     return s, counts
+
+
+def overlap_adjoint_dd(centers, alpha, b, mistake=0):
+    """Reference gradient of (log det S, b^T S^-1 b) from dd ops: returns
+    (logdet (2,), quad (2,), g_ld (4n, 2), g_q (4n, 2), x (n, 2)); parameters are the 3n
+    centre coordinates (Gaussian-major) then the n exponents.  mistake > 0 plants a known
+    error (see hpo.h) for tests of the checks."""
+    n = len(alpha)
+    ld, q = np.zeros(2), np.zeros(2)
+    gl, gq, x = np.zeros((4 * n, 2)), np.zeros((4 * n, 2)), np.zeros((n, 2))
+    rc = lib().hpo_overlap_adjoint_dd_planted(_c(centers), _c(alpha), _c(b), n, ld, q, gl, gq, x.ctypes.data,
+                                              mistake)
+    if rc:
+        raise CholeskyError(-1, 0.0)
+    return ld, q, gl, gq, x
+
+
+def overlap_fd_f256(centers, alpha, b, coords, h=1e-15):
+    """f256 central differences at steps h and 2h: arrays (ncoords, 2, 4) for log det S and
+    for b^T S^-1 b."""
+    n = len(alpha)
+    coords = np.ascontiguousarray(coords, dtype=np.int64)
+    fl = np.zeros((coords.size, 2, 4))
+    fq = np.zeros((coords.size, 2, 4))
+    rc = lib().hpo_overlap_fd_f256(_c(centers), _c(alpha), _c(b), n, coords, coords.size, h, fl, fq)
+    if rc:
+        raise CholeskyError(-1, 0.0)
+    return fl, fq
 
 
 def count_dd_mac():

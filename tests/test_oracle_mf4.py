@@ -134,3 +134,26 @@ def test_exp_vs_mpmath():
     for i, c in enumerate(G["exp"]):
         k = abs(round(X[i, 0] / 0.6931471805599453))
         assert rel_err(out[i], c["exp"]) <= (400 + k) * 2.0 ** -208, (i, rel_err(out[i], c["exp"]))
+
+
+def test_mf4_mac_op_count_matches_the_bench_constant():
+    """bench.py prices the MF4 1024^3 gemm at MF4_OPS_PER_MAC_MEAN FP64 operations per MAC:
+    re-derive that mean from the op-counting restatement on the same inputs (4 accumulation
+    chains of 1024 MACs)."""
+    import re
+    import os
+    import quadprec_amd as qp
+    src = open(os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "bench.py")).read()
+    const = int(re.search(r"MF4_OPS_PER_MAC_MEAN = (\d+)", src).group(1))
+    n = 1024
+    a = qp.gen_mf4_normal(1, n * n)
+    b = qp.gen_mf4_normal(1 + 0x5EED, n * n)
+    tot = cnt = 0
+    for i, j in ((0, 0), (17, 900), (511, 3), (1023, 1023)):
+        s = np.zeros(4)
+        for p in range(n):
+            ai, bj = a[i * n + p], b[p * n + j]
+            tot += O.count_mf4_mac(s, ai, bj)
+            cnt += 1
+            s = O.mf4_binop("add", s.reshape(1, 4), O.mf4_binop("mul", ai.reshape(1, 4), bj.reshape(1, 4)))[0]
+    assert abs(tot / cnt - const) <= 0.01 * const, tot / cnt
