@@ -42,6 +42,15 @@ struct ArDD {
   __device__ static T readlane(T v, int lane) {
     return mkdd(__shfl(v.hi, lane, 64), __shfl(v.lo, lane, 64));
   }
+  // sqrt(d) and 1/sqrt(d) by one dd Newton step on the binary64 reciprocal square root:
+  // y = y0 + y0 (1 - d y0^2) / 2, sqrt = d y.  One division instead of the reference's
+  // three (sqrt then recip, dd.rs:61-83); used where reference order is not required.
+  __device__ static void sqrt_recip_fast(T d, T& root, T& inv) {
+    const dd y0 = mkdd(1.0 / __builtin_sqrt(d.hi));
+    const dd e = dd_sub(mkdd(1.0), dd_mul(d, dd_mul(y0, y0)));
+    inv = dd_add(y0, dd_mul_f(dd_mul(y0, e), 0.5));
+    root = dd_mul(d, inv);
+  }
 };
 
 struct ArF64 {
@@ -62,6 +71,10 @@ struct ArF64 {
   __device__ static bool nonpos(T d) { return d <= 0.0; }         // lib.rs:177
   __device__ static double f64(T d) { return d; }
   __device__ static T readlane(T v, int lane) { return __shfl(v, lane, 64); }
+  __device__ static void sqrt_recip_fast(T d, T& root, T& inv) {
+    root = __builtin_sqrt(d);
+    inv = 1.0 / root;
+  }
 };
 
 }  // namespace hp

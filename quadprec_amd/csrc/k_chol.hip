@@ -58,7 +58,17 @@ __device__ __forceinline__ void lset(Lds2<A>& s, int i, int j, typename A::T v) 
 // partial sums start from S (unblocked emulation across panels) instead of zero.  On
 // success the lower triangle is written back and inv[j] = recip(L[j][j]); on a
 // breakdown the block is left as it was and err records the pivot (linalg.rs:54-57).
-template <class A, bool S_INIT>
+// One barrier per column.  Between barriers j and j+1, wavefront 0 finishes column j+1
+// alone (lane i holds row i): it adds column j's product to S(i, j+1), takes the pivot on
+// lane j+1, broadcasts recip(L[j+1][j+1]) with v_readlane and scales the column; the
+// other three wavefronts add column j's products to the columns beyond j+1 (walked
+// through a packed-triangle index table, so every lane has work).  Every element still
+// receives its products in increasing k, one dd add each, and is scaled exactly as in
+// the unblocked algorithm.  The reciprocals stay in LDS until the block is done, so no
+// global store is pending at a barrier.  FAST (split-dd factorisation only) takes each
+// pivot's root and reciprocal by a dd reciprocal square root instead of the reference's
+// sqrt then recip.
+template <class A, bool S_INIT, bool FAST = false>
 __global__ __launch_bounds__(256) void k_diag_factor(Mat a, Mat S, long r0, int jb, ErrRec* err, double* inv_hi,
                                                      double* inv_lo) {
   using T = typename A::T;
@@ -66,8 +76,11 @@ __global__ __launch_bounds__(256) void k_diag_factor(Mat a, Mat S, long r0, int 
   Lds2<A>& L = *reinterpret_cast<Lds2<A>*>(smem);
   Lds2<A>& Sa = *reinterpret_cast<Lds2<A>*>(smem + sizeof(Lds2<A>));
   __shared__ int fail;
-  __shared__ double invs[2];
-  const int tid = threadIdx.x;
+  __shared__ double invs_h[DB], invs_l[DB];
+  // packed triangle: entry e = q(q+1)/2 + p, 0 <= p <= q, is the element (jb-1-p, jb-1-q);
+  // the columns beyond j+1 form the prefix q <= jb-3-j
+  __shared__ unsigned short tri[DB * (DB - 1) / 2];
+  const int tid = threadIdx.x, lane = tid & 63;
   if (err->flag) return;
   for (int idx = tid; idx < jb * jb; idx += blockDim.x) {
     int i = idx / jb, c = idx % jb;
@@ -76,45 +89,83 @@ __global__ __launch_bounds__(256) void k_diag_factor(Mat a, Mat S, long r0, int 
       lset<A>(Sa, i, c, S_INIT ? A::ld(S, r0 + i, r0 + c) : A::zero());
     }
   }
+  for (int e = tid; e < DB * (DB - 1) / 2; e += blockDim.x) {
+    int q = (int)((__builtin_sqrtf(8.0f * e + 1.0f) - 1.0f) * 0.5f);
+    while (q * (q + 1) / 2 > e) --q;
+    while ((q + 1) * (q + 2) / 2 <= e) ++q;
+    tri[e] = (unsigned short)((e - q * (q + 1) / 2) | (q << 8));
+  }
   if (tid == 0) fail = 0;
   __syncthreads();
-  for (int j = 0; j < jb; ++j) {
-    if (tid == 0) {
-      T diag = A::sub(lget<A>(L, j, j), lget<A>(Sa, j, j));
+  // wavefront 0: finish column c whose partial sums (lane i: row i) are in sc
+  T lcol = A::zero();
+  auto finish_column = [&](int c, T sc) {
+    T inv = A::zero();
+    int bad = 0;
+    if (lane == c) {
+      T diag = A::sub(lget<A>(L, c, c), sc);
       if (A::nonpos(diag)) {
         err->flag = 1;
-        err->pivot = r0 + j;
+        err->pivot = r0 + c;
         err->value = A::f64(diag);
         fail = 1;
+        bad = 1;
       } else {
-        T ljj = A::sqrt(diag);
-        lset<A>(L, j, j, ljj);
-        T inv = A::recip(ljj);
-        if constexpr (A::LIMBS == 2) { invs[0] = inv.hi; invs[1] = inv.lo; inv_hi[j] = inv.hi; inv_lo[j] = inv.lo; }
-        else { invs[0] = inv; inv_hi[j] = inv; }
+        T ljj;
+        if constexpr (FAST) A::sqrt_recip_fast(diag, ljj, inv);
+        else { ljj = A::sqrt(diag); inv = A::recip(ljj); }
+        lset<A>(L, c, c, ljj);
+        if constexpr (A::LIMBS == 2) { invs_h[c] = inv.hi; invs_l[c] = inv.lo; }
+        else invs_h[c] = inv;
       }
     }
-    __syncthreads();
+    if (__builtin_amdgcn_readlane(bad, c)) return;
+    inv = rl(inv, c);
+    if (lane > c && lane < jb) {
+      lcol = A::mul(A::sub(lget<A>(L, lane, c), sc), inv);
+      lset<A>(L, lane, c, lcol);
+    }
+  };
+  if (tid < 64) finish_column(0, lane < jb ? lget<A>(Sa, lane, 0) : A::zero());
+  __syncthreads();
+  for (int j = 0; j + 1 < jb; ++j) {
     if (fail) return;
-    T inv;
-    if constexpr (A::LIMBS == 2) inv = mkdd(invs[0], invs[1]);
-    else inv = invs[0];
-    for (int i = j + 1 + tid; i < jb; i += blockDim.x)
-      lset<A>(L, i, j, A::mul(A::sub(lget<A>(L, i, j), lget<A>(Sa, i, j)), inv));
-    __syncthreads();
-    const int m = jb - j - 1;
-    for (int idx = tid; idx < m * m; idx += blockDim.x) {
-      int ii = idx / m, cc = idx % m;
-      if (cc <= ii) {
-        int i = j + 1 + ii, c = j + 1 + cc;
-        lset<A>(Sa, i, c, A::add(lget<A>(Sa, i, c), A::mul(lget<A>(L, i, j), lget<A>(L, c, j))));
+    if (tid < 64) {
+      const int c = j + 1;
+      T sc = A::zero();
+      const T ljc = rl(lcol, c);  // L(j+1, j)
+      if (lane >= c && lane < jb) sc = A::add(lget<A>(Sa, lane, c), A::mul(lcol, ljc));
+      finish_column(c, sc);
+    } else {
+      const int m = jb - j - 2;  // columns j+2 .. jb-1
+      const int cnt = m * (m + 1) / 2;
+      for (int e0 = tid - 64; e0 < cnt; e0 += 2 * 192) {
+        T sv[2], li[2], lc[2];
+        int ii[2], cc[2];
+#pragma unroll
+        for (int u = 0; u < 2; ++u) {  // loads of both elements first (index clamped in range)
+          const unsigned short pq = tri[min(e0 + u * 192, cnt - 1)];
+          ii[u] = jb - 1 - (pq & 255);
+          cc[u] = jb - 1 - (pq >> 8);
+          sv[u] = lget<A>(Sa, ii[u], cc[u]);
+          li[u] = lget<A>(L, ii[u], j);
+          lc[u] = lget<A>(L, cc[u], j);
+        }
+#pragma unroll
+        for (int u = 0; u < 2; ++u)
+          if (e0 + u * 192 < cnt) lset<A>(Sa, ii[u], cc[u], A::add(sv[u], A::mul(li[u], lc[u])));
       }
     }
     __syncthreads();
   }
+  if (fail) return;
   for (int idx = tid; idx < jb * jb; idx += blockDim.x) {
     int i = idx / jb, c = idx % jb;
     if (c <= i) A::st(a, r0 + i, r0 + c, lget<A>(L, i, c));
+  }
+  if (tid < jb) {
+    inv_hi[tid] = invs_h[tid];
+    if constexpr (A::LIMBS == 2) inv_lo[tid] = invs_l[tid];
   }
 }
 
@@ -169,22 +220,26 @@ __global__ __launch_bounds__(256) void k_row_recur(Mat a, Mat S, long c0, int w,
 enum { RU_SUMSUB = 0, RU_ACC = 1, RU_SUBINC = 2 };
 constexpr int TB = 64, TK = 16;
 
-template <class A, int MODE, bool TRI>
+// RB x RB outputs per thread on a 16 x 16 thread grid: tiles of TB = 16 RB.  Skinny
+// updates (the next panel's columns, the sub-panel updates) use RB = 1 or 2 so that
+// enough workgroups exist; each element's chain is the same either way.
+template <class A, int MODE, bool TRI, int RB>
 __global__ __launch_bounds__(256) void k_rank_update(Mat a, Mat S, long ri0, long M, long cj0, long N, long bj0,
                                                      long ka0, long K, const ErrRec* err) {
   using T = typename A::T;
+  constexpr int TBr = 16 * RB;
   const int ti = blockIdx.y, tc = blockIdx.x;
   if (TRI && tc > ti) return;
   if (err->flag) return;
-  __shared__ double As[2][TK][TB], Bs[2][TK][TB];
+  __shared__ double As[2][TK][TBr], Bs[2][TK][TBr];
   const int t = threadIdx.x, tx = t & 15, ty = t >> 4;
-  const long i0 = (long)ti * TB, j0 = (long)tc * TB;  // tile origin relative to ri0 / cj0
-  T acc[4][4];
+  const long i0 = (long)ti * TBr, j0 = (long)tc * TBr;  // tile origin relative to ri0 / cj0
+  T acc[RB][RB];
 #pragma unroll
-  for (int r = 0; r < 4; ++r)
+  for (int r = 0; r < RB; ++r)
 #pragma unroll
-    for (int c = 0; c < 4; ++c) {
-      long li = i0 + ty * 4 + r, lj = j0 + tx * 4 + c;
+    for (int c = 0; c < RB; ++c) {
+      long li = i0 + ty * RB + r, lj = j0 + tx * RB + c;
       bool in = li < M && lj < N && (!TRI || ri0 + li >= cj0 + lj);
       T v = A::zero();
       if (in && MODE == RU_ACC) v = A::ld(S, ri0 + li, cj0 + lj);
@@ -192,10 +247,11 @@ __global__ __launch_bounds__(256) void k_rank_update(Mat a, Mat S, long ri0, lon
       acc[r][c] = v;
     }
   for (long k0 = 0; k0 < K; k0 += TK) {
-    {  // thread t stages tile row t/4, k (t%4)*4..+3 of both operands
-      int r = t >> 2, kc = (t & 3) * 4;
+    {  // thread t stages tile row t / (16/RB), k (t % (16/RB)) * RB .. + RB-1 of both operands
+      constexpr int TPR = 16 / RB;
+      int r = t / TPR, kc = (t % TPR) * RB;
 #pragma unroll
-      for (int q = 0; q < 4; ++q) {
+      for (int q = 0; q < RB; ++q) {
         long kk = k0 + kc + q;
         T va = (i0 + r < M && kk < K) ? A::ld(a, ri0 + i0 + r, ka0 + kk) : A::zero();
         T vb = (j0 + r < N && kk < K) ? A::ld(a, bj0 + j0 + r, ka0 + kk) : A::zero();
@@ -210,21 +266,21 @@ __global__ __launch_bounds__(256) void k_rank_update(Mat a, Mat S, long ri0, lon
     __syncthreads();
     const int kmax = (int)min((long)TK, K - k0);
     for (int kk = 0; kk < kmax; ++kk) {
-      T av[4], bv[4];
+      T av[RB], bv[RB];
 #pragma unroll
-      for (int r = 0; r < 4; ++r) {
+      for (int r = 0; r < RB; ++r) {
         if constexpr (A::LIMBS == 2) {
-          av[r] = mkdd(As[0][kk][ty * 4 + r], As[1][kk][ty * 4 + r]);
-          bv[r] = mkdd(Bs[0][kk][tx * 4 + r], Bs[1][kk][tx * 4 + r]);
+          av[r] = mkdd(As[0][kk][ty * RB + r], As[1][kk][ty * RB + r]);
+          bv[r] = mkdd(Bs[0][kk][tx * RB + r], Bs[1][kk][tx * RB + r]);
         } else {
-          av[r] = As[0][kk][ty * 4 + r];
-          bv[r] = Bs[0][kk][tx * 4 + r];
+          av[r] = As[0][kk][ty * RB + r];
+          bv[r] = Bs[0][kk][tx * RB + r];
         }
       }
 #pragma unroll
-      for (int r = 0; r < 4; ++r)
+      for (int r = 0; r < RB; ++r)
 #pragma unroll
-        for (int c = 0; c < 4; ++c) {
+        for (int c = 0; c < RB; ++c) {
           T p = A::mul(av[r], bv[c]);
           acc[r][c] = (MODE == RU_SUBINC) ? A::sub(acc[r][c], p) : A::add(acc[r][c], p);
         }
@@ -232,10 +288,10 @@ __global__ __launch_bounds__(256) void k_rank_update(Mat a, Mat S, long ri0, lon
     __syncthreads();
   }
 #pragma unroll
-  for (int r = 0; r < 4; ++r)
+  for (int r = 0; r < RB; ++r)
 #pragma unroll
-    for (int c = 0; c < 4; ++c) {
-      long li = i0 + ty * 4 + r, lj = j0 + tx * 4 + c;
+    for (int c = 0; c < RB; ++c) {
+      long li = i0 + ty * RB + r, lj = j0 + tx * RB + c;
       if (li < M && lj < N && (!TRI || ri0 + li >= cj0 + lj)) {
         long gi = ri0 + li, gj = cj0 + lj;
         if (MODE == RU_ACC) A::st(S, gi, gj, acc[r][c]);
@@ -404,6 +460,10 @@ static int set_lds_attrs() {
   const int one = (int)sizeof(Lds2<A>);
   HP_HIP(hipFuncSetAttribute((const void*)k_diag_factor<A, true>, hipFuncAttributeMaxDynamicSharedMemorySize, big));
   HP_HIP(hipFuncSetAttribute((const void*)k_diag_factor<A, false>, hipFuncAttributeMaxDynamicSharedMemorySize, big));
+  HP_HIP(hipFuncSetAttribute((const void*)k_diag_factor<A, true, true>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                             big));
+  HP_HIP(hipFuncSetAttribute((const void*)k_diag_factor<A, false, true>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                             big));
   HP_HIP(hipFuncSetAttribute((const void*)k_row_recur<A, true>, hipFuncAttributeMaxDynamicSharedMemorySize, one));
   HP_HIP(hipFuncSetAttribute((const void*)k_row_recur<A, false>, hipFuncAttributeMaxDynamicSharedMemorySize, one));
   HP_HIP(hipFuncSetAttribute((const void*)k_fwd_diag<A>, hipFuncAttributeMaxDynamicSharedMemorySize, one));
@@ -415,15 +475,28 @@ static int ensure_attrs() {
   return rc;
 }
 
+template <class A, int MODE, bool TRI, int RB>
+static int rank_update_rb(Mat a, Mat S, long ri0, long M, long cj0, long N, long bj0, long ka0, long K, ErrRec* err,
+                          hipStream_t s) {
+  constexpr long TBr = 16 * RB;
+  dim3 grid((unsigned)((N + TBr - 1) / TBr), (unsigned)((M + TBr - 1) / TBr));
+  hipLaunchKernelGGL((k_rank_update<A, MODE, TRI, RB>), grid, dim3(256), 0, s, a, S, ri0, M, cj0, N, bj0, ka0, K,
+                     (const ErrRec*)err);
+  HP_LAUNCH_CHECK();
+  return HP_OK;
+}
+// the register block is chosen so that the launch has enough workgroups to fill the chip
 template <class A, int MODE, bool TRI>
 static int rank_update(Mat a, Mat S, long ri0, long M, long cj0, long N, long bj0, long ka0, long K, ErrRec* err,
                        hipStream_t s) {
   if (M <= 0 || N <= 0) return HP_OK;
-  dim3 grid((unsigned)((N + TB - 1) / TB), (unsigned)((M + TB - 1) / TB));
-  hipLaunchKernelGGL((k_rank_update<A, MODE, TRI>), grid, dim3(256), 0, s, a, S, ri0, M, cj0, N, bj0, ka0, K,
-                     (const ErrRec*)err);
-  HP_LAUNCH_CHECK();
-  return HP_OK;
+  auto tiles = [&](long tb) {
+    long tm = (M + tb - 1) / tb, tn = (N + tb - 1) / tb;
+    return TRI ? tm * (tn + 1) / 2 : tm * tn;
+  };
+  if (tiles(64) >= 1024) return rank_update_rb<A, MODE, TRI, 4>(a, S, ri0, M, cj0, N, bj0, ka0, K, err, s);
+  if (tiles(32) >= 1024) return rank_update_rb<A, MODE, TRI, 2>(a, S, ri0, M, cj0, N, bj0, ka0, K, err, s);
+  return rank_update_rb<A, MODE, TRI, 1>(a, S, ri0, M, cj0, N, bj0, ka0, K, err, s);
 }
 
 template <class A, bool ACC>
@@ -438,7 +511,7 @@ static int row_recur(Mat a, Mat S, long c0, int w, long rb, long re, double* inv
 }
 
 // Unblocked Cholesky (linalg.rs:42-96) of a[r0:r0+len, r0:r0+len], S = partial sums.
-template <class A>
+template <class A, bool FAST = false>
 static int factor_unblocked_region(Mat a, Mat S, long r0, long len, ErrRec* err, double* inv_h, double* inv_l,
                                    hipStream_t s) {
   {
@@ -449,8 +522,8 @@ static int factor_unblocked_region(Mat a, Mat S, long r0, long len, ErrRec* err,
   }
   for (long c0 = r0; c0 < r0 + len; c0 += DB) {
     int w = (int)std::min<long>(DB, r0 + len - c0);
-    hipLaunchKernelGGL((k_diag_factor<A, true>), dim3(1), dim3(256), 2 * sizeof(Lds2<A>), s, a, S, c0, w, err, inv_h,
-                       inv_l);
+    hipLaunchKernelGGL((k_diag_factor<A, true, FAST>), dim3(1), dim3(256), 2 * sizeof(Lds2<A>), s, a, S, c0, w, err,
+                       inv_h, inv_l);
     HP_LAUNCH_CHECK();
     long rb = c0 + w, re = r0 + len;
     int rc;
@@ -462,16 +535,16 @@ static int factor_unblocked_region(Mat a, Mat S, long r0, long len, ErrRec* err,
 
 // One block column of cholesky_blocked (linalg.rs:247-261): the diagonal block
 // [c0, c0+jb) by the unblocked algorithm, then trsm_right_lt on rows [c0+jb, n).
-template <class A>
+template <class A, bool FAST>
 int chol_panel_exact(Mat a, Mat S, long c0, long jb, long n, ErrRec* err, double* inv_h, double* inv_l,
                      hipStream_t s) {
   int rc = ensure_attrs<A>();
   if (rc) return rc;
   if (jb <= DB) {
-    hipLaunchKernelGGL((k_diag_factor<A, false>), dim3(1), dim3(256), 2 * sizeof(Lds2<A>), s, a, S, c0, (int)jb, err,
-                       inv_h, inv_l);
+    hipLaunchKernelGGL((k_diag_factor<A, false, FAST>), dim3(1), dim3(256), 2 * sizeof(Lds2<A>), s, a, S, c0, (int)jb,
+                       err, inv_h, inv_l);
     HP_LAUNCH_CHECK();
-  } else if ((rc = factor_unblocked_region<A>(a, S, c0, jb, err, inv_h, inv_l, s))) {
+  } else if ((rc = factor_unblocked_region<A, FAST>(a, S, c0, jb, err, inv_h, inv_l, s))) {
     return rc;
   }
   long rem = n - c0 - jb;
@@ -530,7 +603,7 @@ int chol_exact(Mat a, long n, long nb, bool unblocked, Mat S, ErrRec* err, doubl
   if ((rc = la.init())) return rc;
   for (long jb0 = 0; jb0 < n; jb0 += nb) {
     long jb = std::min(nb, n - jb0);
-    if ((rc = chol_panel_exact<A>(a, S, jb0, jb, n, err, inv_h, inv_l, s))) return rc;
+    if ((rc = chol_panel_exact<A, false>(a, S, jb0, jb, n, err, inv_h, inv_l, s))) return rc;
     long rem = n - jb0 - jb;
     if (rem <= 0) continue;
     long t0 = jb0 + jb, w2 = std::min(nb, rem);
@@ -591,8 +664,9 @@ int zero_upper_f64(double* l, long n, hipStream_t s) {
 }
 
 template int chol_exact<ArDD>(Mat, long, long, bool, Mat, ErrRec*, double*, double*, hipStream_t);
-template int chol_panel_exact<ArDD>(Mat, Mat, long, long, long, ErrRec*, double*, double*, hipStream_t);
-template int chol_panel_exact<ArF64>(Mat, Mat, long, long, long, ErrRec*, double*, double*, hipStream_t);
+template int chol_panel_exact<ArDD, false>(Mat, Mat, long, long, long, ErrRec*, double*, double*, hipStream_t);
+template int chol_panel_exact<ArDD, true>(Mat, Mat, long, long, long, ErrRec*, double*, double*, hipStream_t);
+template int chol_panel_exact<ArF64, false>(Mat, Mat, long, long, long, ErrRec*, double*, double*, hipStream_t);
 template int rank_update_sumsub<ArDD>(Mat, Mat, long, long, long, long, long, long, ErrRec*, hipStream_t);
 template int chol_exact<ArF64>(Mat, long, long, bool, Mat, ErrRec*, double*, double*, hipStream_t);
 template int fwd_exact<ArDD>(Mat, double*, double*, long, hipStream_t);

@@ -24,7 +24,7 @@ namespace hp {
 
 template <class A>
 int chol_exact(Mat a, long n, long nb, bool unblocked, Mat S, ErrRec* err, double* inv_h, double* inv_l, hipStream_t s);
-template <class A>
+template <class A, bool FAST>
 int chol_panel_exact(Mat a, Mat S, long c0, long jb, long n, ErrRec* err, double* inv_h, double* inv_l, hipStream_t s);
 template <class A>
 int fwd_exact(Mat l, double* yh, double* yl, long n, hipStream_t s);
@@ -107,7 +107,8 @@ __device__ __forceinline__ typename A::T Vg(double (*Vh)[BW + 1], double (*Vl)[B
 // Correction solves with precomputed inverses of the 64 x 64 diagonal blocks of L: each
 // block step is one launch in which every workgroup forms x_b = inv(L_bb) y_b (a 64-long
 // dot per lane, no sequential chain) and then subtracts the block's contributions from its
-// own rows.  x is written to a separate vector so no workgroup reads a value another one
+// own rows.  Both 64-term sums run as four interleaved dd chains (the order is free here:
+// these solves only feed the refinement).  x is written to a separate vector so no workgroup reads a value another one
 // is overwriting.
 template <class A>
 __global__ __launch_bounds__(64) void k_diag_block_inverse(Mat l, long n, double* ih, double* il) {
@@ -162,22 +163,40 @@ __global__ __launch_bounds__(256) void k_inv_step(Mat l, const double* ih, const
                                                   double* xoh, double* xol, long n, long b0, int bw) {
   using T = typename A::T;
   __shared__ double Yh[BW], Yl[BW], Xh[BW], Xl[BW];
+  __shared__ double Ih[BW][BW + 1], Il[A::LIMBS == 2 ? BW : 1][BW + 1];
   const int t = threadIdx.x;
   const size_t base = (size_t)(b0 / BW) * BW * BW;
+  for (int e = t; e < BW * BW; e += 256) {  // the block inverse, one coalesced sweep
+    Ih[e >> 6][e & 63] = ih[base + e];
+    if constexpr (A::LIMBS == 2) Il[e >> 6][e & 63] = il[base + e];
+  }
   if (t < bw) {
     Yh[t] = yh[b0 + t];
     if constexpr (A::LIMBS == 2) Yl[t] = yl[b0 + t];
   }
   __syncthreads();
   if (t < bw) {  // x_t = sum_k inv[t][k] y_k (FWD) or sum_k inv[k][t] y_k (transposed)
-    T s = A::zero();
-    for (int k = 0; k < bw; ++k) {
-      size_t o = FWD ? base + (size_t)t * BW + k : base + (size_t)k * BW + t;
-      T yv;
-      if constexpr (A::LIMBS == 2) yv = mkdd(Yh[k], Yl[k]);
-      else yv = Yh[k];
-      s = A::add(s, A::mul(A::ldv(ih, il, (long)o), yv));
+    T p[4] = {A::zero(), A::zero(), A::zero(), A::zero()};  // four independent chains
+    auto term = [&](int k) {
+      const int r = FWD ? t : k, c = FWD ? k : t;
+      T iv, yv;
+      if constexpr (A::LIMBS == 2) { iv = mkdd(Ih[r][c], Il[r][c]); yv = mkdd(Yh[k], Yl[k]); }
+      else { iv = Ih[r][c]; yv = Yh[k]; }
+      return A::mul(iv, yv);
+    };
+    if (bw == BW) {
+#pragma unroll 4
+      for (int k = 0; k < BW; k += 4) {
+        T t0 = term(k), t1 = term(k + 1), t2 = term(k + 2), t3 = term(k + 3);
+        p[0] = A::add(p[0], t0);
+        p[1] = A::add(p[1], t1);
+        p[2] = A::add(p[2], t2);
+        p[3] = A::add(p[3], t3);
+      }
+    } else {
+      for (int k = 0; k < bw; ++k) p[0] = A::add(p[0], term(k));
     }
+    T s = A::add(A::add(p[0], p[1]), A::add(p[2], p[3]));
     if constexpr (A::LIMBS == 2) { Xh[t] = s.hi; Xl[t] = s.lo; }
     else Xh[t] = s;
     if (blockIdx.x == 0) A::stv(xoh, xol, b0 + t, s);
@@ -186,15 +205,24 @@ __global__ __launch_bounds__(256) void k_inv_step(Mat l, const double* ih, const
   long i = (long)blockIdx.x * 256 + t;
   if (FWD) i += b0 + bw;
   if (FWD ? (i >= n) : (i >= b0)) return;
-  T y = A::ldv(yh, yl, i);
-  for (int k = 0; k < bw; ++k) {
-    T xv;
-    if constexpr (A::LIMBS == 2) xv = mkdd(Xh[k], Xl[k]);
-    else xv = Xh[k];
-    T lv = FWD ? A::ld(l, i, b0 + k) : A::ld(l, b0 + k, i);
-    y = A::sub(y, A::mul(lv, xv));
+  T q[4] = {A::zero(), A::zero(), A::zero(), A::zero()};
+  auto xk = [&](int k) {
+    if constexpr (A::LIMBS == 2) return mkdd(Xh[k], Xl[k]);
+    else return Xh[k];
+  };
+  if (bw == BW) {
+#pragma unroll
+    for (int k0 = 0; k0 < BW; k0 += 16) {  // 16 loads in flight, then their products
+      T lv[16];
+#pragma unroll
+      for (int u = 0; u < 16; ++u) lv[u] = FWD ? A::ld(l, i, b0 + k0 + u) : A::ld(l, b0 + k0 + u, i);
+#pragma unroll
+      for (int u = 0; u < 16; ++u) q[u & 3] = A::add(q[u & 3], A::mul(lv[u], xk(k0 + u)));
+    }
+  } else {
+    for (int k = 0; k < bw; ++k) q[0] = A::add(q[0], A::mul(FWD ? A::ld(l, i, b0 + k) : A::ld(l, b0 + k, i), xk(k)));
   }
-  A::stv(yh, yl, i, y);
+  A::stv(yh, yl, i, A::sub(A::ldv(yh, yl, i), A::add(A::add(q[0], q[1]), A::add(q[2], q[3]))));
 }
 
 // ---------------------------------------------------------------- qd residual
@@ -362,7 +390,8 @@ double absmax_dev(const double* v, long n, unsigned long long* scratch, hipStrea
 
 }  // namespace
 
-// dd blocked Cholesky with the trailing update P P^T on the INT8 matrix cores.  Per panel:
+// dd blocked Cholesky with the trailing update P P^T on the INT8 matrix cores (pivots by
+// dd reciprocal square root, see ArDD::sqrt_recip_fast).  Per panel:
 // the panel (diagonal block + trsm) on FP64 VALU, the next panel's columns updated on
 // VALU, then the rest of the trailing matrix updated by one split SYRK (lower tiles only,
 // the subtraction fused into the CRT) on a second stream while the next panel is factored.
@@ -382,7 +411,7 @@ int chol_split(Mat a, long n, long nb, Mat S, ErrRec* err, double* inv_h, double
   } guard{s2, evA, evB};
   HP_HIP(hipEventCreateWithFlags(&evA, hipEventDisableTiming));
   HP_HIP(hipEventCreateWithFlags(&evB, hipEventDisableTiming));
-  int rc = chol_panel_exact<ArDD>(a, S, 0, std::min(nb, n), n, err, inv_h, inv_l, s);
+  int rc = chol_panel_exact<ArDD, true>(a, S, 0, std::min(nb, n), n, err, inv_h, inv_l, s);
   if (rc) return rc;
   for (long c0 = 0; c0 < n; c0 += nb) {
     long jb = std::min(nb, n - c0);
@@ -393,7 +422,7 @@ int chol_split(Mat a, long n, long nb, Mat S, ErrRec* err, double* inv_h, double
     if ((rc = rank_update_sumsub<ArDD>(a, S, t0, rem, t0, w2, c0, jb, err, s))) return rc;
     HP_HIP(hipEventRecord(evA, s));
     // next panel on s, concurrently with the rest update below
-    if ((rc = chol_panel_exact<ArDD>(a, S, t0, w2, n, err, inv_h, inv_l, s))) return rc;
+    if ((rc = chol_panel_exact<ArDD, true>(a, S, t0, w2, n, err, inv_h, inv_l, s))) return rc;
     if (rem > w2) {
       HP_HIP(hipStreamWaitEvent(s2, evA, 0));
       long r0 = t0 + w2, m = n - r0;
@@ -411,7 +440,7 @@ int chol_split(Mat a, long n, long nb, Mat S, ErrRec* err, double* inv_h, double
   return HP_OK;
 }
 
-int g_split_panel = 128;  // panel width of the split-dd factorisation
+int g_split_panel = 256;  // panel width of the split-dd factorisation
 
 // x = S^-1 b by corrections through a given factor L (f64 or dd by kind) and the inverses of
 // its 64 x 64 diagonal blocks: r = b - S x in quad-double, x += L^-T L^-1 r, until the

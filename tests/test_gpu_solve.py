@@ -89,3 +89,24 @@ def test_split_cholesky_factor_matches_plain_to_dd_accuracy(qp):
     x1, st1 = qp.solve_spd_refined(s, b, n, qp.HP_FACTOR_DD, tol=0.0, max_iter=1)
     x2, st2 = qp.solve_spd_refined(s, b, n, qp.HP_FACTOR_DD_SPLIT, tol=0.0, max_iter=1)
     assert rel_err(x2, x1) < 1e-20
+
+
+@pytest.mark.parametrize("nb", [64, 128, 256, 512])
+@pytest.mark.parametrize("n", [300, 700])
+def test_split_factor_panel_widths(qp, nb, n):
+    """The split-dd factorisation at every panel width (three-stream schedule, ragged last
+    panel): one refinement-free solve agrees with the reference-order dd factor's to dd
+    accuracy at kappa ~ 1e3..1e6, and the refined solve still meets 1e-30."""
+    c, a = qp.gen_overlap_basis(11, n, 0.3, 10.0)
+    s = qp.overlap_s(c, a)
+    b = qp.gen_dd_normal(12, n)
+    x1, _ = qp.solve_spd_refined(s, b, n, qp.HP_FACTOR_DD, tol=0.0, max_iter=1)
+    qp.set_split_factor_panel(nb)
+    try:
+        x2, _ = qp.solve_spd_refined(s, b, n, qp.HP_FACTOR_DD_SPLIT, tol=0.0, max_iter=1)
+        x3, st = qp.solve_spd_refined(s, b, n, qp.HP_FACTOR_DD_SPLIT, tol=1e-32, max_iter=10)
+        x4, _ = qp.solve_spd_refined(s, b, n, qp.HP_FACTOR_DD, tol=1e-32, max_iter=10)
+    finally:
+        qp.set_split_factor_panel(256)
+    assert rel_err(x2, x1) < 1e-22, rel_err(x2, x1)
+    assert rel_err(x3, x4) < 1e-30, rel_err(x3, x4)
