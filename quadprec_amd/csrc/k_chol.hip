@@ -562,6 +562,57 @@ int chol_panel_exact(Mat a, Mat S, long c0, long jb, long n, ErrRec* err, double
   return HP_OK;
 }
 
+// The split-dd factorisation's panel: as chol_panel_exact with fast pivots, but the rows
+// below the panel are solved on a side stream, sub-panel q as soon as its diagonal block
+// is factored, overlapping the factor of the next diagonal sub-block.  The reciprocals of
+// sub-panel q live in slot q of inv (2 x 64 doubles per slot).  evq: one event per
+// sub-panel; evdone is recorded on `side` once the panel is complete and `s` waits for it.
+template <class A>
+int chol_panel_split(Mat a, Mat S, long c0, long jb, long n, ErrRec* err, double* inv, hipStream_t s,
+                     hipStream_t side, hipEvent_t* evq, hipEvent_t evdone) {
+  int rc = ensure_attrs<A>();
+  if (rc) return rc;
+  const bool region = jb > DB;
+  if (region) {
+    long total = jb * jb;
+    unsigned g = (unsigned)std::min<long>((total + 255) / 256, 65536);
+    hipLaunchKernelGGL(k_zero_lower<A>, dim3(g ? g : 1), dim3(256), 0, s, S, c0, jb);
+    HP_LAUNCH_CHECK();
+  }
+  const long rem = n - c0 - jb;
+  for (long q0 = 0; q0 < jb; q0 += DB) {
+    const int w = (int)std::min<long>(DB, jb - q0);
+    const int q = (int)(q0 / DB);
+    double* ih = inv + (size_t)q * 2 * DB;
+    double* il = ih + DB;
+    if (region)
+      hipLaunchKernelGGL((k_diag_factor<A, true, true>), dim3(1), dim3(256), 2 * sizeof(Lds2<A>), s, a, S, c0 + q0, w,
+                         err, ih, il);
+    else
+      hipLaunchKernelGGL((k_diag_factor<A, false, true>), dim3(1), dim3(256), 2 * sizeof(Lds2<A>), s, a, S, c0 + q0,
+                         w, err, ih, il);
+    HP_LAUNCH_CHECK();
+    HP_HIP(hipEventRecord(evq[q], s));
+    const long rb = c0 + q0 + w, re = c0 + jb;
+    if (re > rb) {
+      if ((rc = row_recur<A, true>(a, S, c0 + q0, w, rb, re, ih, il, err, s))) return rc;
+      if ((rc = rank_update<A, RU_ACC, true>(a, S, rb, re - rb, rb, re - rb, rb, c0 + q0, w, err, s))) return rc;
+    }
+    if (rem > 0) {
+      HP_HIP(hipStreamWaitEvent(side, evq[q], 0));
+      if (q0 > 0 &&
+          (rc = rank_update<A, RU_SUBINC, false>(a, S, c0 + jb, rem, c0 + q0, w, c0 + q0, c0, q0, err, side)))
+        return rc;
+      if ((rc = row_recur<A, false>(a, S, c0 + q0, w, c0 + jb, n, ih, il, err, side))) return rc;
+    }
+  }
+  HP_HIP(hipEventRecord(evdone, side));
+  HP_HIP(hipStreamWaitEvent(s, evdone, 0));
+  return HP_OK;
+}
+template int chol_panel_split<ArDD>(Mat, Mat, long, long, long, ErrRec*, double*, hipStream_t, hipStream_t,
+                                    hipEvent_t*, hipEvent_t);
+
 // Second stream and events for the factorisations' lookahead (default priority).
 struct Lookahead {
   hipStream_t s2 = nullptr;
@@ -584,6 +635,12 @@ template <class A>
 int rank_update_sumsub(Mat a, Mat S, long ri0, long M, long cj0, long N, long ka0, long K, ErrRec* err,
                        hipStream_t s) {
   return rank_update<A, RU_SUMSUB, true>(a, S, ri0, M, cj0, N, cj0, ka0, K, err, s);
+}
+// the same update on a block strictly below the diagonal (rows ri0.., columns cj0..)
+template <class A>
+int rank_update_sumsub_below(Mat a, Mat S, long ri0, long M, long cj0, long N, long ka0, long K, ErrRec* err,
+                             hipStream_t s) {
+  return rank_update<A, RU_SUMSUB, false>(a, S, ri0, M, cj0, N, cj0, ka0, K, err, s);
 }
 
 // cholesky_blocked with lookahead: after panel k, the next panel's columns receive panel
@@ -668,6 +725,7 @@ template int chol_panel_exact<ArDD, false>(Mat, Mat, long, long, long, ErrRec*, 
 template int chol_panel_exact<ArDD, true>(Mat, Mat, long, long, long, ErrRec*, double*, double*, hipStream_t);
 template int chol_panel_exact<ArF64, false>(Mat, Mat, long, long, long, ErrRec*, double*, double*, hipStream_t);
 template int rank_update_sumsub<ArDD>(Mat, Mat, long, long, long, long, long, long, ErrRec*, hipStream_t);
+template int rank_update_sumsub_below<ArDD>(Mat, Mat, long, long, long, long, long, long, ErrRec*, hipStream_t);
 template int chol_exact<ArF64>(Mat, long, long, bool, Mat, ErrRec*, double*, double*, hipStream_t);
 template int fwd_exact<ArDD>(Mat, double*, double*, long, hipStream_t);
 template int fwd_exact<ArF64>(Mat, double*, double*, long, hipStream_t);

@@ -27,9 +27,15 @@ int chol_exact(Mat a, long n, long nb, bool unblocked, Mat S, ErrRec* err, doubl
 template <class A, bool FAST>
 int chol_panel_exact(Mat a, Mat S, long c0, long jb, long n, ErrRec* err, double* inv_h, double* inv_l, hipStream_t s);
 template <class A>
+int chol_panel_split(Mat a, Mat S, long c0, long jb, long n, ErrRec* err, double* inv, hipStream_t s,
+                     hipStream_t side, hipEvent_t* evq, hipEvent_t evdone);
+template <class A>
 int fwd_exact(Mat l, double* yh, double* yl, long n, hipStream_t s);
 template <class A>
 int rank_update_sumsub(Mat a, Mat S, long ri0, long M, long cj0, long N, long ka0, long K, ErrRec* err, hipStream_t s);
+template <class A>
+int rank_update_sumsub_below(Mat a, Mat S, long ri0, long M, long cj0, long N, long ka0, long K, ErrRec* err,
+                             hipStream_t s);
 
 namespace {
 
@@ -202,27 +208,41 @@ __global__ __launch_bounds__(256) void k_inv_step(Mat l, const double* ih, const
     if (blockIdx.x == 0) A::stv(xoh, xol, b0 + t, s);
   }
   __syncthreads();
-  long i = (long)blockIdx.x * 256 + t;
+  // 64 rows per workgroup, four lanes per row, each lane a quarter of the 64 terms; the
+  // quarters are combined with lane shuffles
+  const int part = t & 3;
+  long i = (long)blockIdx.x * 64 + (t >> 2);
   if (FWD) i += b0 + bw;
-  if (FWD ? (i >= n) : (i >= b0)) return;
-  T q[4] = {A::zero(), A::zero(), A::zero(), A::zero()};
+  const bool live = FWD ? (i < n) : (i < b0);
+  T q0 = A::zero(), q1 = A::zero();
   auto xk = [&](int k) {
     if constexpr (A::LIMBS == 2) return mkdd(Xh[k], Xl[k]);
     else return Xh[k];
   };
-  if (bw == BW) {
-#pragma unroll
-    for (int k0 = 0; k0 < BW; k0 += 16) {  // 16 loads in flight, then their products
+  if (live) {
+    if (bw == BW) {
       T lv[16];
+      const int kb = part * 16;
 #pragma unroll
-      for (int u = 0; u < 16; ++u) lv[u] = FWD ? A::ld(l, i, b0 + k0 + u) : A::ld(l, b0 + k0 + u, i);
+      for (int u = 0; u < 16; ++u) lv[u] = FWD ? A::ld(l, i, b0 + kb + u) : A::ld(l, b0 + kb + u, i);
 #pragma unroll
-      for (int u = 0; u < 16; ++u) q[u & 3] = A::add(q[u & 3], A::mul(lv[u], xk(k0 + u)));
+      for (int u = 0; u < 16; u += 2) {
+        q0 = A::add(q0, A::mul(lv[u], xk(kb + u)));
+        q1 = A::add(q1, A::mul(lv[u + 1], xk(kb + u + 1)));
+      }
+    } else {
+      for (int k = part; k < bw; k += 4) q0 = A::add(q0, A::mul(FWD ? A::ld(l, i, b0 + k) : A::ld(l, b0 + k, i), xk(k)));
     }
-  } else {
-    for (int k = 0; k < bw; ++k) q[0] = A::add(q[0], A::mul(FWD ? A::ld(l, i, b0 + k) : A::ld(l, b0 + k, i), xk(k)));
   }
-  A::stv(yh, yl, i, A::sub(A::ldv(yh, yl, i), A::add(A::add(q[0], q[1]), A::add(q[2], q[3]))));
+  T q = A::add(q0, q1);
+#pragma unroll
+  for (int off = 1; off < 4; off <<= 1) {  // lanes 4r..4r+3 of one wavefront hold row r
+    T o;
+    if constexpr (A::LIMBS == 2) o = mkdd(__shfl_xor(q.hi, off, 64), __shfl_xor(q.lo, off, 64));
+    else o = __shfl_xor(q, off, 64);
+    q = A::add(q, o);
+  }
+  if (live && part == 0) A::stv(yh, yl, i, A::sub(A::ldv(yh, yl, i), q));
 }
 
 // ---------------------------------------------------------------- qd residual
@@ -356,7 +376,7 @@ int solve_inv(Mat l, const double* ih, const double* il, double* yh, double* yl,
     long b0 = b * BW;
     int bw = (int)std::min<long>(BW, n - b0);
     long rows = n - b0 - bw;
-    unsigned g = (unsigned)std::max<long>(1, (rows + 255) / 256);
+    unsigned g = (unsigned)std::max<long>(1, (rows + 63) / 64);
     hipLaunchKernelGGL((k_inv_step<A, true>), dim3(g), dim3(256), 0, s, l, ih, il, yh, yl, xh, xl, n, b0, bw);
     HP_LAUNCH_CHECK();
   }
@@ -367,7 +387,7 @@ int solve_inv(Mat l, const double* ih, const double* il, double* yh, double* yl,
       HP_HIP(hipMemcpyAsync(yh, xh, n * sizeof(double), hipMemcpyDeviceToDevice, s));
       if (A::LIMBS == 2) HP_HIP(hipMemcpyAsync(yl, xl, n * sizeof(double), hipMemcpyDeviceToDevice, s));
     }
-    unsigned g = (unsigned)std::max<long>(1, (b0 + 255) / 256);
+    unsigned g = (unsigned)std::max<long>(1, (b0 + 63) / 64);
     hipLaunchKernelGGL((k_inv_step<A, false>), dim3(g), dim3(256), 0, s, l, ih, il, yh, yl, xh, xl, n, b0, bw);
     HP_LAUNCH_CHECK();
   }
@@ -390,39 +410,61 @@ double absmax_dev(const double* v, long n, unsigned long long* scratch, hipStrea
 
 }  // namespace
 
-// dd blocked Cholesky with the trailing update P P^T on the INT8 matrix cores (pivots by
-// dd reciprocal square root, see ArDD::sqrt_recip_fast).  Per panel:
-// the panel (diagonal block + trsm) on FP64 VALU, the next panel's columns updated on
-// VALU, then the rest of the trailing matrix updated by one split SYRK (lower tiles only,
-// the subtraction fused into the CRT) on a second stream while the next panel is factored.
-int chol_split(Mat a, long n, long nb, Mat S, ErrRec* err, double* inv_h, double* inv_l, hipStream_t s) {
+// dd blocked Cholesky with the trailing update P P^T on the INT8 matrix cores.  Per panel:
+// the panel on FP64 VALU (diagonal sub-blocks with dd reciprocal-square-root pivots, the
+// rows below solved on a side stream as each sub-block is ready), the next panel's
+// columns updated on VALU (its diagonal block first, the rows below on the side stream),
+// then the rest of the trailing matrix updated by one split SYRK
+// (lower tiles only, the subtraction fused into the CRT) on a second stream while the next
+// panel is factored.  Every element receives the panels' updates in panel order.
+int chol_split(Mat a, long n, long nb, Mat S, ErrRec* err, double*, double*, hipStream_t s) {
   if (nb <= 0) nb = 256;
-  hipStream_t s2 = nullptr;
-  hipEvent_t evA = nullptr, evB = nullptr;
+  nb = std::max<long>(64, (nb / 64) * 64);
+  const int Q = (int)(nb / 64);
+  if (Q > 8) {
+    set_error("split factorisation: panel width %ld above 512", nb);
+    return HP_EINVAL;
+  }
+  DevBuf invb;
+  int rc;
+  if ((rc = invb.alloc((size_t)Q * 2 * 64 * sizeof(double)))) return rc;
+  hipStream_t s2 = nullptr, s3 = nullptr;
+  hipEvent_t evA = nullptr, evB = nullptr, evP = nullptr, evq[8] = {};
   bool evB_live = false;
-  HP_HIP(hipStreamCreateWithFlags(&s2, hipStreamNonBlocking));
   struct Guard {
-    hipStream_t& s2; hipEvent_t& a; hipEvent_t& b;
+    hipStream_t *s2, *s3;
+    hipEvent_t *a, *b, *p, *q;
     ~Guard() {
-      if (s2) { (void)hipStreamSynchronize(s2); (void)hipStreamDestroy(s2); }
-      if (a) (void)hipEventDestroy(a);
-      if (b) (void)hipEventDestroy(b);
+      for (hipStream_t* x : {s2, s3})
+        if (*x) { (void)hipStreamSynchronize(*x); (void)hipStreamDestroy(*x); }
+      for (hipEvent_t* e : {a, b, p})
+        if (*e) (void)hipEventDestroy(*e);
+      for (int i = 0; i < 8; ++i)
+        if (q[i]) (void)hipEventDestroy(q[i]);
     }
-  } guard{s2, evA, evB};
-  HP_HIP(hipEventCreateWithFlags(&evA, hipEventDisableTiming));
-  HP_HIP(hipEventCreateWithFlags(&evB, hipEventDisableTiming));
-  int rc = chol_panel_exact<ArDD, true>(a, S, 0, std::min(nb, n), n, err, inv_h, inv_l, s);
-  if (rc) return rc;
+  } guard{&s2, &s3, &evA, &evB, &evP, evq};
+  HP_HIP(hipStreamCreateWithFlags(&s2, hipStreamNonBlocking));
+  HP_HIP(hipStreamCreateWithFlags(&s3, hipStreamNonBlocking));
+  for (hipEvent_t* e : {&evA, &evB, &evP}) HP_HIP(hipEventCreateWithFlags(e, hipEventDisableTiming));
+  for (int q = 0; q < Q; ++q) HP_HIP(hipEventCreateWithFlags(&evq[q], hipEventDisableTiming));
+  if ((rc = chol_panel_split<ArDD>(a, S, 0, std::min(nb, n), n, err, invb.d(), s, s3, evq, evP))) return rc;
   for (long c0 = 0; c0 < n; c0 += nb) {
     long jb = std::min(nb, n - c0);
     long rem = n - c0 - jb;
     if (rem <= 0) break;
     long t0 = c0 + jb, w2 = std::min(nb, rem);
-    if (evB_live) HP_HIP(hipStreamWaitEvent(s, evB, 0));
-    if ((rc = rank_update_sumsub<ArDD>(a, S, t0, rem, t0, w2, c0, jb, err, s))) return rc;
+    if (evB_live) {
+      HP_HIP(hipStreamWaitEvent(s, evB, 0));
+      HP_HIP(hipStreamWaitEvent(s3, evB, 0));
+    }
+    // next panel's columns take this panel's update: its diagonal block on s, the rows
+    // below on s3 (where the next panel's rows below are solved)
+    if ((rc = rank_update_sumsub<ArDD>(a, S, t0, w2, t0, w2, c0, jb, err, s))) return rc;
+    if (rem > w2 && (rc = rank_update_sumsub_below<ArDD>(a, S, t0 + w2, rem - w2, t0, w2, c0, jb, err, s3)))
+      return rc;
     HP_HIP(hipEventRecord(evA, s));
-    // next panel on s, concurrently with the rest update below
-    if ((rc = chol_panel_exact<ArDD, true>(a, S, t0, w2, n, err, inv_h, inv_l, s))) return rc;
+    // next panel on s (and s3), concurrently with the rest update below
+    if ((rc = chol_panel_split<ArDD>(a, S, t0, w2, n, err, invb.d(), s, s3, evq, evP))) return rc;
     if (rem > w2) {
       HP_HIP(hipStreamWaitEvent(s2, evA, 0));
       long r0 = t0 + w2, m = n - r0;
@@ -437,6 +479,7 @@ int chol_split(Mat a, long n, long nb, Mat S, ErrRec* err, double* inv_h, double
     }
   }
   if (evB_live) HP_HIP(hipStreamWaitEvent(s, evB, 0));
+  HP_HIP(hipStreamSynchronize(s));  // invb is freed on return
   return HP_OK;
 }
 
@@ -444,7 +487,8 @@ int g_split_panel = 256;  // panel width of the split-dd factorisation
 
 // x = S^-1 b by corrections through a given factor L (f64 or dd by kind) and the inverses of
 // its 64 x 64 diagonal blocks: r = b - S x in quad-double, x += L^-T L^-1 r, until the
-// correction's max-norm falls to tol relative to x (or max_iter steps).
+// correction's max-norm falls to tol relative to x, or the next one is predicted to be
+// ten times below tol (or max_iter steps).
 int refine_loop(int kind, Mat Lm, const double* ivh, const double* ivl, const double* sh, const double* sl,
                 const double* bh, const double* bl, long n, double tol, int max_iter, double* xh, double* xl,
                 int* iters, double* last_out, hipStream_t s) {
@@ -492,8 +536,12 @@ int refine_loop(int kind, Mat Lm, const double* ivh, const double* ivl, const do
     if (rc) return rc;
     double xn = absmax_dev(xh, n, scr, s, &rc);
     if (rc) return rc;
+    const double prev = last;
     last = xn > 0 ? dn / xn : dn;
     if (last <= tol) { ++it; break; }
+    // the corrections shrink geometrically by the factor's contraction rho ~ kappa u_f:
+    // stop once the predicted next one, last * (last / prev), is ten times below tol
+    if (it >= 1 && prev > 0 && last * (last / prev) <= 0.1 * tol) { ++it; break; }
   }
   *iters = it;
   *last_out = last;
