@@ -28,6 +28,9 @@ SLICES = 31                      # CRT moduli = int8 GEMMs per dd GEMM
 FP64_PEAK_TFLOPS = 78.6          # MI355X FP64 vector (= FP64 matrix) dense peak
 INT8_PEAK_TOPS = 5000.0          # MI355X dense INT8 matrix peak
 DD_MAC_OPS = 29                  # FP64 ops per reference dd MAC (dd add 20 + dd mul 9)
+# HBM bytes one k_gemm_i8_mod launch at 4096^3 moves (read 4.68e9 + write 0.52e9), from
+# rocprofv3 FETCH_SIZE / WRITE_SIZE counters scaled by a device copy of known size
+GEMM_I8_TRAFFIC_BYTES = 5.20e9
 
 
 def cpu_baseline(a, b, target_s=12.0):
@@ -260,6 +263,9 @@ def main():
     ap.add_argument("--no-batch", action="store_true", help="skip the batched uneven solves")
     ap.add_argument("--no-mf4", action="store_true", help="skip the MF4 1024^3 gemm")
     ap.add_argument("--no-adjoint", action="store_true", help="skip the N=2048 adjoint")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
+                    help="only copy MIB MiB device-to-device three times (a plain copy kernel of known size, "
+                         "to calibrate rocprofv3's FETCH_SIZE / WRITE_SIZE counters) and exit")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); "
@@ -270,6 +276,16 @@ def main():
     import quadprec_amd as qp
 
     dev = torch.device("cuda:0")
+    if args.copy_calibration:
+        count = args.copy_calibration * (1 << 20) // 8
+        src = torch.ones(count, dtype=torch.float64, device=dev)
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_calibration_bytes_read_per_copy": count * 8, "bytes_written_per_copy": count * 8,
+                          "copies": 3}), flush=True)
+        return
     a = qp.gen_dd_normal(1, M * K)
     b = qp.gen_dd_normal(1 + 0x5EED, K * N)
     da, db = qp.dev.to_planes(a, 2, dev), qp.dev.to_planes(b, 2, dev)
@@ -317,7 +333,10 @@ def main():
                    "parallelism": "1 GPU"},
         "roofline": {"bound": "mfma", "kernel": "k_gemm_i8_mod", "achieved": round(achieved, 2),
                      "peak": INT8_PEAK_TOPS, "unit": "TOP/s (int8)", "frac": round(achieved / INT8_PEAK_TOPS, 4),
-                     "traffic": None, "kernel_ms": round(gemm_ms, 4),
+                     "traffic": GEMM_I8_TRAFFIC_BYTES, "traffic_algorithmic": 3 * SLICES * M * N,
+                     "traffic_source": "rocprofv3 --pmc FETCH_SIZE (x2.000) + WRITE_SIZE (x1.000) per launch, "
+                                       "calibrated on a 256 MiB device copy; profiles/k_gemm_i8_mod_4096.md",
+                     "kernel_ms": round(gemm_ms, 4),
                      "k_stage": qp.split_gemm_variant(M, N, K, SLICES),
                      "phases_ms": {"scale_and_residues": round(ph[0], 4), "int8_gemms": round(ph[1], 4),
                                    "crt": round(ph[2], 4)}},
