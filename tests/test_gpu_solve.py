@@ -49,13 +49,14 @@ def rel_err(x, xs):
     return float(np.max(np.abs(d[:, 0] + d[:, 1])) / np.max(np.abs(xs[:, 0])))
 
 
-@pytest.mark.parametrize("lo,kinds,bar", [
-    (0.5, ("f64", "dd", "split"), 1e-30),   # kappa ~ 1e8: every factor converges
-    (0.01, ("dd", "split"), 1e-28),          # kappa ~ 1e20 (config 0's exponent range)
+@pytest.mark.parametrize("lo,spacing,kinds,bar", [
+    (0.5, 1.4, ("f64", "dd", "split"), 1e-30),   # kappa ~ 1e2: every factor converges
+    (0.01, 1.4, ("dd", "split"), 1e-28),          # config 0's exponent range (kappa ~ 2e9 here)
+    (0.001, 0.7, ("dd", "split"), 1e-28),         # tight lattice, diffuse exponents: kappa far beyond 1e16
 ])
-def test_refined_solve_vs_cpu_reference(qp, lo, kinds, bar):
+def test_refined_solve_vs_cpu_reference(qp, lo, spacing, kinds, bar):
     n = 512
-    c, a = qp.gen_overlap_basis(0, n, lo, 10.0)
+    c, a = qp.gen_overlap_basis(0, n, lo, 10.0, spacing)
     s = qp.overlap_s(c, a)
     b = qp.gen_dd_normal(77, n)
     xs = cpu_reference(s, b, n)
@@ -63,7 +64,8 @@ def test_refined_solve_vs_cpu_reference(qp, lo, kinds, bar):
     for kind in kinds:
         x, st = qp.solve_spd_refined(s, b, n, kind_id[kind], tol=bar * 1e-2, max_iter=30)
         e = rel_err(x, xs)
-        print(f"alpha_lo={lo} factor={kind}: iters={st.iterations} err={e:.3e} last={st.last_correction:.2e} "
+        print(f"alpha_lo={lo} spacing={spacing} factor={kind}: iters={st.iterations} err={e:.3e} "
+              f"last={st.last_correction:.2e} "
               f"factor {st.factor_ms:.1f} ms refine {st.refine_ms:.1f} ms")
         assert e <= bar, (kind, e)
 
