@@ -249,10 +249,12 @@ int hp_overlap_adjoint(const double* centers, const double* alpha, const double*
  * stream at its phase boundaries (scaling + residues | int8 GEMMs | CRT) and
  * hp_get_phase_times copies the elapsed milliseconds, returning the phase count. */
 void hp_set_phase_timing(int on);
-/* K-stage depth the split GEMM's autotuner chose for this shape class (0 = not tuned yet).
+/* Int8 kernel the split GEMM's autotuner chose for this shape class (0 = not tuned yet).
  * The first split GEMM of a shape class times each variant on its real operands. */
 int hp_split_gemm_variant(size_t m, size_t n, size_t k, int slices);
-/* Pins the K-stage depth (32, 64 or 128) for a shape class; 0 clears it (re-tune). */
+/* Pins the int8 kernel for a shape class: 32, 64 or 128 (K stages of that depth staged
+ * through registers) or 3064 (operand tiles streamed straight into a 4-deep LDS ring of
+ * 64-deep stages); 0 clears it (re-tune). */
 int hp_split_gemm_set_variant(size_t m, size_t n, size_t k, int slices, int variant);
 int hp_get_phase_times(double* ms, int max);
 

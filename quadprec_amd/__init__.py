@@ -343,12 +343,13 @@ def set_phase_timing(on):
 
 
 def split_gemm_variant(m, n, k, slices=0):
-    """K-stage depth the split GEMM autotuner chose for this shape class (0 = untuned)."""
+    """Int8 kernel the split GEMM autotuner chose for this shape class (0 = untuned)."""
     return lib().hp_split_gemm_variant(m, n, k, slices)
 
 
 def split_gemm_set_variant(m, n, k, variant, slices=0):
-    """Pin the split GEMM's K-stage depth (32, 64, 128) for a shape class; 0 re-tunes."""
+    """Pin the split GEMM's int8 kernel for a shape class: 32, 64 or 128 (K stages of that
+    depth staged through registers) or 3064 (direct-to-LDS ring of 64-deep stages); 0 re-tunes."""
     _check(lib().hp_split_gemm_set_variant(m, n, k, slices, variant), "split_gemm_set_variant")
 
 

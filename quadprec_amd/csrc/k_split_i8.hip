@@ -9,15 +9,25 @@
 //      int8 values in [-128, 127]; C'_r = (A'_r B'_r mod m_r) is an int8 x int8 -> int32
 //      GEMM on v_mfma_i32_32x32x32_i8 -- exact, |partial sums| <= k*2^14 < 2^31;
 //   3. C' = A'B' (exactly, |C'| < 2^(2W+13) < M/16, M = prod m_r) is rebuilt from its
-//      residues by the Chinese remainder theorem in 256-bit integer arithmetic and turned
-//      into a dd value times 2^(e_i + f_j - 2W).
+//      residues by the Chinese remainder theorem and turned into a dd value times
+//      2^(e_i + f_j - 2W).
 // The product of the integer matrices is exact, so the only errors are the W-bit
 // truncation of the inputs and the final rounding to dd: |C - C*|_ij <=
 // 2^(1-W) (max_p|A_ip| sum_p|B_pj| + sum_p|A_ip| max_p|B_pj|) + 2^-106 |C*_ij|.
 // Cost: R int8 GEMMs (2R int ops per dd MAC) instead of 29 FP64 VALU ops per dd MAC.
 //
-// Layouts: residue planes are [r][row][k] (k contiguous) for both operands, so A (m x k)
-// and B^T (n x k) are loaded identically; C residues are [r][i][j] bytes.
+// All integer work outside the MFMAs is done in FP64 on exact integers: an integer below
+// 2^53 is a double, and the FMA unit multiplies and adds such integers without rounding.
+//   * Residues: T(h 2^s) + T(l 2^s) = t2 2^80 + t1 2^40 + t0 (signed limbs, |t| < 2^41),
+//     v = t0 + t1 (2^40 mod m) + t2 (2^80 mod m) with the two coefficients centred
+//     (|v| < 2^48.2), and the centred residue is v - m floor(v/m + 1/2).
+//   * CRT: the GEMM epilogue emits the digit u_r = (C'_r y_r) mod m_r, y_r = (M/m_r)^-1
+//     mod m_r; then S = sum_r u_r (M/m_r) is accumulated in six 40-bit limbs, each sum
+//     below 32 * 255 * 2^40 < 2^53, and C' = S - round(S/M) M.
+//
+// Layouts: residue planes are [r][row][p] with the row pitch Kp = K rounded up to 64
+// (zero-filled), so A (m x k) and B^T (n x k) are loaded identically and every K stage
+// is whole; the digits of C are [r][i][j] bytes with the row pitch Np = N rounded up to 4.
 #include <cmath>
 #include <cstdint>
 #include <cstring>
@@ -31,7 +41,11 @@
 namespace hp {
 
 constexpr int CRT_MAXR = 32;
-constexpr int CRT_LIMBS = 8;  // 256-bit integers as 8 x 32-bit limbs
+constexpr int CRT_LIMBS = 8;  // host big integers: 256 bits as 8 x 32-bit limbs
+constexpr int CRT_L40 = 6;    // device: 240 bits as 6 x 40-bit limbs held in doubles (M < 2^239.4)
+
+typedef int v4i_t __attribute__((ext_vector_type(4)));
+typedef int v16i_t __attribute__((ext_vector_type(16)));
 
 // The moduli: 2^8 and the largest pairwise-coprime values below it.
 static const uint32_t kModuli[CRT_MAXR] = {256, 255, 253, 251, 247, 241, 239, 233, 229, 227, 223,
@@ -42,11 +56,12 @@ struct CrtTable {
   int R;                                   // number of moduli in use
   int pad;
   uint32_t mod[CRT_MAXR];
-  uint32_t yinv[CRT_MAXR];                 // (M / m_r)^-1 mod m_r
   double inv_m[CRT_MAXR];                  // 1 / m_r
-  uint32_t Mr[CRT_MAXR][CRT_LIMBS];        // M / m_r, little-endian limbs
-  uint32_t M[CRT_LIMBS];                   // M
-  double c24[CRT_MAXR][5];                 // 2^(24q) mod m_r, q < 5
+  double y[CRT_MAXR];                      // (M / m_r)^-1 mod m_r
+  double split[CRT_MAXR][4];               // centred 2^40 mod m_r, centred 2^80 mod m_r, m_r, 1 / m_r
+  double mr40[CRT_MAXR][CRT_L40];          // M / m_r in 40-bit limbs
+  double m40[CRT_L40];                     // M in 40-bit limbs
+  double inv_M;                            // 1 / M, rounded
 };
 
 namespace {
@@ -75,27 +90,41 @@ uint32_t inv_mod(uint32_t a, uint32_t m) {  // a^-1 mod m (gcd(a, m) = 1)
   if (t < 0) t += m;
   return (uint32_t)t;
 }
+void limbs40(const uint32_t* x, double* out) {  // 32-bit limbs -> 40-bit limbs as exact doubles
+  for (int q = 0; q < CRT_L40; ++q) {
+    uint64_t v = 0;
+    for (int b = 0; b < 40; ++b) {
+      const int bit = 40 * q + b;
+      if (bit < 32 * CRT_LIMBS && ((x[bit / 32] >> (bit % 32)) & 1u)) v |= 1ull << b;
+    }
+    out[q] = (double)v;
+  }
+}
+double centred(uint64_t r, uint32_t m) { return r > (m - 1) / 2 ? (double)r - (double)m : (double)r; }
 
 void build_table(int R, CrtTable& T) {
   std::memset(&T, 0, sizeof(T));
   T.R = R;
   uint32_t M[CRT_LIMBS] = {1, 0, 0, 0, 0, 0, 0, 0};
   for (int r = 0; r < R; ++r) big_mul_small(M, kModuli[r]);
-  std::memcpy(T.M, M, sizeof(M));
+  limbs40(M, T.m40);
+  double md = 0.0;
+  for (int q = CRT_L40 - 1; q >= 0; --q) md = md * 0x1p40 + T.m40[q];
+  T.inv_M = 1.0 / md;
   for (int r = 0; r < R; ++r) {
-    uint32_t m = kModuli[r];
+    const uint32_t m = kModuli[r];
     T.mod[r] = m;
     T.inv_m[r] = 1.0 / (double)m;
     uint32_t Mr[CRT_LIMBS] = {1, 0, 0, 0, 0, 0, 0, 0};
     for (int q = 0; q < R; ++q)
       if (q != r) big_mul_small(Mr, kModuli[q]);
-    std::memcpy(T.Mr[r], Mr, sizeof(Mr));
-    T.yinv[r] = inv_mod(big_mod_small(Mr, m), m);
-    uint64_t p = 1 % m;
-    for (int q = 0; q < 5; ++q) {
-      T.c24[r][q] = (double)p;
-      p = (p << 24) % m;
-    }
+    limbs40(Mr, T.mr40[r]);
+    T.y[r] = (double)inv_mod(big_mod_small(Mr, m), m);
+    const uint64_t p40 = (1ull << 40) % m, p80 = (p40 * p40) % m;
+    T.split[r][0] = centred(p40, m);
+    T.split[r][1] = centred(p80, m);
+    T.split[r][2] = (double)m;
+    T.split[r][3] = T.inv_m[r];
   }
 }
 
@@ -106,148 +135,224 @@ double log2_M(int R) {
 }
 
 // ------------------------------------------------------------------ device helpers
-__device__ __forceinline__ uint32_t mod32(uint32_t x, uint32_t m, double inv_m) {
-  uint32_t q = (uint32_t)((double)x * inv_m);
-  int32_t r = (int32_t)(x - q * m);
-  if (r < 0) r += m;
-  if (r >= (int32_t)m) r -= m;
-  return (uint32_t)r;
-}
-// sign(v) * floor(|v|) as a 128-bit integer; |v| < 2^120
-__device__ __forceinline__ __int128 trunc128(double v) {
-  uint64_t b = __builtin_bit_cast(uint64_t, v);
-  int ex = (int)((b >> 52) & 0x7ff);
-  if (ex == 0) return 0;  // zero / subnormal: |v| < 1
-  uint64_t mn = (b & 0xfffffffffffffull) | (1ull << 52);
-  int e2 = ex - 1075;     // |v| = mn * 2^e2
-  unsigned __int128 u;
-  if (e2 >= 0) u = (unsigned __int128)mn << e2;
-  else if (e2 > -53) u = mn >> (-e2);
-  else u = 0;
-  __int128 x = (__int128)u;
-  return (b >> 63) ? -x : x;
-}
-__device__ __forceinline__ int8_t center(uint32_t r, uint32_t m) {
-  return (int8_t)((r > (m - 1) / 2) ? (int)r - (int)m : (int)r);
-}
-__device__ __forceinline__ int frexp_exp(double v) {  // |v| < 2^e, 0 for 0
+__device__ __forceinline__ int frexp_exp(double v) {  // |v| < 2^e, -100000 for 0
   if (v == 0.0) return -100000;
   int e;
   (void)__builtin_frexp(v, &e);
   return e;
 }
+// T(h 2^sc) + T(l 2^sc) = t2 2^80 + t1 2^40 + t0 exactly, T truncating toward zero; the
+// limbs are signed integers with |t0|, |t1| < 2^41 and |t2| < 2^38 (|h 2^sc| < 2^117).
+// Each split keeps the bits of an integer double below a power of two (same sign), which
+// a double holds exactly, so the FMAs do not round.
+__device__ __forceinline__ void int_limbs(double h, double l, int sc, double& t2, double& t1, double& t0) {
+  const double H = __builtin_trunc(__builtin_ldexp(h, sc)), L = __builtin_trunc(__builtin_ldexp(l, sc));
+  const double h2 = __builtin_trunc(H * 0x1p-80), l2 = __builtin_trunc(L * 0x1p-80);
+  const double hr = __builtin_fma(h2, -0x1p80, H), lr = __builtin_fma(l2, -0x1p80, L);
+  const double h1 = __builtin_trunc(hr * 0x1p-40), l1 = __builtin_trunc(lr * 0x1p-40);
+  t0 = __builtin_fma(h1, -0x1p40, hr) + __builtin_fma(l1, -0x1p40, lr);
+  t1 = h1 + l1;
+  t2 = h2 + l2;
+}
+// the centred residue (r > (m-1)/2 ? r - m : r, r = X mod m) of X = t2 2^80 + t1 2^40 + t0,
+// c1 / c2 the centred residues of 2^40 / 2^80.  v is exact (|v| < 2^48.2); fma(v, 1/m, 1/2)
+// is within 2^-10.7 of v/m + 1/2, whose distance to the next integer is at least 1/(2m)
+// unless it is one (m = 256, where every step is exact), so the floor is exact.
+__device__ __forceinline__ int residue8(double t2, double t1, double t0, double c1, double c2, double m,
+                                        double inv_m) {
+  const double v = __builtin_fma(t2, c2, __builtin_fma(t1, c1, t0));
+  return (int)__builtin_fma(__builtin_floor(__builtin_fma(v, inv_m, 0.5)), -m, v);
+}
+__device__ __forceinline__ uint32_t pack4(int a, int b, int c, int d) {
+  return (uint32_t)(a & 255) | ((uint32_t)(b & 255) << 8) | ((uint32_t)(c & 255) << 16) | ((uint32_t)d << 24);
+}
+// CRT digit (v y) mod m in [0, m) of an exact int32 GEMM sum v: |v y| < 2^39, so v y is
+// exact and floor(v y / m) is off by at most one (at exact multiples), which the
+// corrections absorb.
+__device__ __forceinline__ uint8_t crt_digit(int v, double m, double inv_m, double y) {
+  const double x = (double)v * y;
+  double r = __builtin_fma(__builtin_floor(x * inv_m), -m, x);
+  r = r < 0.0 ? r + m : r;
+  r = r >= m ? r - m : r;
+  return (uint8_t)(int)r;
+}
 
 // ------------------------------------------------------------------ scale exponents
 // row scales: element (row, p) at base[row*ld + p] (contiguous p); one wavefront per row
 __global__ void k_scale_rows(const double* __restrict__ hi, long rows, long K, long ld, int* __restrict__ e) {
-  long row = (long)blockIdx.x * 4 + (threadIdx.x >> 6);
-  int lane = threadIdx.x & 63;
-  if (row >= rows) return;
-  int mx = -100000;
-  for (long p = lane; p < K; p += 64) mx = max(mx, frexp_exp(hi[row * ld + p]));
-  for (int o = 32; o > 0; o >>= 1) mx = max(mx, __shfl_xor(mx, o, 64));
-  if (lane == 0) e[row] = mx;
+  for (long row = (long)blockIdx.x * 4 + (threadIdx.x >> 6); row < rows; row += (long)gridDim.x * 4) {
+    const int lane = threadIdx.x & 63;
+    int mx = -100000;
+    for (long p = lane; p < K; p += 64) mx = max(mx, frexp_exp(hi[row * ld + p]));
+    for (int o = 32; o > 0; o >>= 1) mx = max(mx, __shfl_xor(mx, o, 64));
+    if (lane == 0) e[row] = mx;
+  }
 }
-// column scales: element (col, p) at base[p*ld + col]; a thread per column and 256-row
-// chunk, combined with atomicMax (e[] preset to INT_MIN by k_fill_int)
 __global__ void k_fill_int(int* p, long n, int v) {
   long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) p[i] = v;
 }
-__global__ void k_scale_cols(const double* __restrict__ hi, long cols, long K, long ld, int* __restrict__ e) {
-  long col = (long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (col >= cols) return;
-  long p0 = (long)blockIdx.y * 256, p1 = min(K, p0 + 256);
+// column scales: element (col, p) at base[p*ld + col]; a block takes 64 columns x 256 p
+// (wavefront w the 64 p from 64w, lanes along the columns), reduces through LDS and
+// combines with one atomicMax per column (e[] preset to -100000 by k_fill_int)
+__global__ __launch_bounds__(256) void k_scale_cols(const double* __restrict__ hi, long cols, long K, long ld,
+                                                    int* __restrict__ e) {
+  __shared__ int red[4][64];
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const long col = (long)blockIdx.x * 64 + lane;
+  const long pa = (long)blockIdx.y * 256 + 64 * w, pb = min(K, pa + 64);
   int mx = -100000;
-  for (long p = p0; p < p1; ++p) mx = max(mx, frexp_exp(hi[p * ld + col]));
-  atomicMax(&e[col], mx);
+  if (col < cols)
+    for (long p = pa; p < pb; ++p) mx = max(mx, frexp_exp(hi[p * ld + col]));
+  red[w][lane] = mx;
+  __syncthreads();
+  if (w == 0 && col < cols) atomicMax(&e[col], max(max(red[0][lane], red[1][lane]), max(red[2][lane], red[3][lane])));
 }
 
 // ------------------------------------------------------------------ residues
-// Element (row, p) of the logical rows x K operand -> out[r][row][p] (int8, centred).
-// CONTIG: the source element is at base[row*ld + p]; else at base[p*ld + row] (staged
-// through an LDS transpose so both the dd reads and the byte writes are coalesced).
-template <bool CONTIG>
-__global__ __launch_bounds__(256) void k_residues(const double* __restrict__ hi, const double* __restrict__ lo,
-                                                  long rows, long K, long ld, const int* __restrict__ escale, int W,
-                                                  const CrtTable* __restrict__ tab, int8_t* __restrict__ out) {
-  __shared__ double c24s[CRT_MAXR][5];
-  __shared__ uint32_t mods[CRT_MAXR];
-  __shared__ double mods_d[CRT_MAXR], invs[CRT_MAXR];
-  __shared__ double th[64][65], tl[64][65];
+// Element (row, p) of the logical rows x K operand -> out[r][row][p] (int8, centred),
+// row pitch Kp (p in [K, Kp) written as 0).
+//
+// Rows contiguous (element at base[row*ld + p]): a wavefront takes 512 p of one row, each
+// lane p = p0 + 4 lane + e and p0 + 256 + 4 lane + e (e < 4), and stores one dword of
+// four residues per modulus.
+__global__ __launch_bounds__(256) void k_residues_rows(const double* __restrict__ hi, const double* __restrict__ lo,
+                                                       long rows, long K, long Kp, long ld,
+                                                       const int* __restrict__ escale, int W,
+                                                       const CrtTable* __restrict__ tab, int8_t* __restrict__ out) {
+  __shared__ __align__(16) double cs[CRT_MAXR][4];
   const int R = tab->R;
-  for (int i = threadIdx.x; i < R * 5; i += 256) c24s[i / 5][i % 5] = tab->c24[i / 5][i % 5];
-  if (threadIdx.x < R) {
-    mods[threadIdx.x] = tab->mod[threadIdx.x];
-    mods_d[threadIdx.x] = (double)tab->mod[threadIdx.x];
-    invs[threadIdx.x] = tab->inv_m[threadIdx.x];
-  }
-  const long row0 = (long)blockIdx.y * 64, p0 = (long)blockIdx.x * 64;
-  if (!CONTIG) {  // stage the 64 x 64 tile: source rows p (contiguous in `row`)
-    for (int idx = threadIdx.x; idx < 64 * 64; idx += 256) {
-      int pp = idx / 64, rr = idx % 64;
-      long gr = row0 + rr, gp = p0 + pp;
-      bool ok = gr < rows && gp < K;
-      th[rr][pp] = ok ? hi[gp * ld + gr] : 0.0;
-      tl[rr][pp] = ok ? lo[gp * ld + gr] : 0.0;
-    }
-  }
+  if ((int)threadIdx.x < 4 * R) cs[threadIdx.x >> 2][threadIdx.x & 3] = tab->split[threadIdx.x >> 2][threadIdx.x & 3];
   __syncthreads();
-  const size_t plane = (size_t)rows * K;
-  for (int idx = threadIdx.x; idx < 64 * 64; idx += 256) {
-    int rr = idx / 64, pp = idx % 64;
-    long gr = row0 + rr, gp = p0 + pp;
-    if (gr >= rows || gp >= K) continue;
-    double h, l;
-    if (CONTIG) { h = hi[gr * ld + gp]; l = lo[gr * ld + gp]; }
-    else { h = th[rr][pp]; l = tl[rr][pp]; }
-    int sc = W - max(escale[gr], -1100);
-    __int128 X = trunc128(__builtin_ldexp(h, sc)) + trunc128(__builtin_ldexp(l, sc));
-    const bool neg = X < 0;
-    unsigned __int128 U = neg ? (unsigned __int128)(-X) : (unsigned __int128)X;
-    double L[5];
+  const int lane = threadIdx.x & 63;
+  const size_t plane = (size_t)rows * Kp;
+  const long p0 = (long)blockIdx.x * 512 + 4 * lane;
+  for (long row = (long)blockIdx.y * 4 + (threadIdx.x >> 6); row < rows; row += (long)gridDim.y * 4) {
+    const int sc = W - max(escale[row], -1100);
+    double t[8][3];
 #pragma unroll
-    for (int q = 0; q < 5; ++q) L[q] = (double)(uint32_t)((U >> (24 * q)) & 0xffffffu);
-    size_t o = (size_t)gr * K + gp;
+    for (int g = 0; g < 2; ++g)
+#pragma unroll
+      for (int e = 0; e < 4; ++e) {
+        const long p = p0 + 256 * g + e;
+        double h = 0.0, l = 0.0;
+        if (p < K) {
+          h = hi[row * ld + p];
+          l = lo[row * ld + p];
+        }
+        int_limbs(h, l, sc, t[4 * g + e][2], t[4 * g + e][1], t[4 * g + e][0]);
+      }
+    int8_t* o = out + (size_t)row * Kp + p0;
     for (int r = 0; r < R; ++r) {
-      const double m = mods_d[r];
-      double v = L[0] * c24s[r][0] + L[1] * c24s[r][1] + L[2] * c24s[r][2] + L[3] * c24s[r][3] + L[4] * c24s[r][4];
-      double rr = v - m * __builtin_floor(v * invs[r]);  // exact: v < 2^35
-      if (rr < 0) rr += m;
-      if (rr >= m) rr -= m;
-      uint32_t res = (uint32_t)rr;
-      uint32_t mi = mods[r];
-      if (neg && res) res = mi - res;
-      out[(size_t)r * plane + o] = center(res, mi);
+      const double c1 = cs[r][0], c2 = cs[r][1], m = cs[r][2], im = cs[r][3];
+#pragma unroll
+      for (int g = 0; g < 2; ++g) {
+        if (p0 + 256 * g >= Kp) continue;  // Kp % 4 == 0: the whole dword is in the row
+        int b[4];
+#pragma unroll
+        for (int e = 0; e < 4; ++e) b[e] = residue8(t[4 * g + e][2], t[4 * g + e][1], t[4 * g + e][0], c1, c2, m, im);
+        *reinterpret_cast<uint32_t*>(o + (size_t)r * plane + 256 * g) = pack4(b[0], b[1], b[2], b[3]);
+      }
     }
+  }
+}
+
+// Rows strided (element at base[p*ld + row], the columns of a row-major B): a block takes
+// 64 rows x 64 p, lane = row (coalesced dd loads), wavefront w the 16 p from 16w.  The
+// residues of one modulus are transposed through LDS (two buffers, alternating by
+// modulus, so one barrier per modulus) and leave as 16-byte stores, 64 bytes per row.
+__global__ __launch_bounds__(256) void k_residues_cols(const double* __restrict__ hi, const double* __restrict__ lo,
+                                                       long rows, long K, long Kp, long ld,
+                                                       const int* __restrict__ escale, int W,
+                                                       const CrtTable* __restrict__ tab, int8_t* __restrict__ out) {
+  __shared__ __align__(16) double cs[CRT_MAXR][4];
+  __shared__ __align__(16) uint8_t tb[2][64][80];
+  const int R = tab->R;
+  if ((int)threadIdx.x < 4 * R) cs[threadIdx.x >> 2][threadIdx.x & 3] = tab->split[threadIdx.x >> 2][threadIdx.x & 3];
+  __syncthreads();
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const int srow = threadIdx.x >> 2, sch = threadIdx.x & 3;  // store: 16 bytes of row srow
+  const size_t plane = (size_t)rows * Kp;
+  const long p0 = (long)blockIdx.x * 64;
+  for (long row0 = (long)blockIdx.y * 64; row0 < rows; row0 += (long)gridDim.y * 64) {
+    const long row = row0 + lane;
+    const bool rok = row < rows;
+    const int sc = rok ? W - max(escale[row], -1100) : 0;
+    double t[16][3];
+#pragma unroll
+    for (int e = 0; e < 16; ++e) {
+      const long p = p0 + 16 * w + e;
+      double h = 0.0, l = 0.0;
+      if (rok && p < K) {
+        h = hi[p * ld + row];
+        l = lo[p * ld + row];
+      }
+      int_limbs(h, l, sc, t[e][2], t[e][1], t[e][0]);
+    }
+    const bool sok = row0 + srow < rows;
+    int8_t* so = out + (size_t)(row0 + srow) * Kp + p0 + 16 * sch;
+    for (int r = 0; r < R; ++r) {
+      const double c1 = cs[r][0], c2 = cs[r][1], m = cs[r][2], im = cs[r][3];
+      v4i_t pk;
+#pragma unroll
+      for (int d = 0; d < 4; ++d) {
+        int b[4];
+#pragma unroll
+        for (int e = 0; e < 4; ++e) b[e] = residue8(t[4 * d + e][2], t[4 * d + e][1], t[4 * d + e][0], c1, c2, m, im);
+        pk[d] = (int)pack4(b[0], b[1], b[2], b[3]);
+      }
+      *reinterpret_cast<v4i_t*>(&tb[r & 1][lane][16 * w]) = pk;
+      __syncthreads();  // buffer r & 1 complete; buffer (r + 1) & 1 was read before this point
+      if (sok) *reinterpret_cast<v4i_t*>(so + (size_t)r * plane) = *reinterpret_cast<const v4i_t*>(&tb[r & 1][srow][16 * sch]);
+    }
+    __syncthreads();  // the next row block's first modulus reuses buffer 0
   }
 }
 
 // ------------------------------------------------------------------ int8 GEMM mod m
-// Cr[i][j] = (sum_p Ar[i][p] * Br[j][p]) mod m_r for every modulus r = blockIdx.z.
-// 256 x 256 block tile, 4 wavefronts of 128 x 128 (4 x 4 tiles of v_mfma_i32_32x32x32_i8,
-// accumulators in AGPRs); K advances in GK-deep stages double-buffered through LDS, the
-// next stage fetched from HBM into registers while the current one feeds the MFMAs.
-// Operand k-slot layout is the same for A and B, so only the output layout matters:
-// lane l holds column l%32, register q holds row 8*(q/4) + 4*(l/32) + q%4.
-typedef int v4i __attribute__((ext_vector_type(4)));
-typedef int v16i __attribute__((ext_vector_type(16)));
+// Cr[i][j] = (y_r sum_p Ar[i][p] Br[j][p]) mod m_r for every modulus r = blockIdx.z: the
+// CRT digit of the exact product.  256 x 256 block tile, 8 wavefronts of 64 x 128 (2 x 4
+// tiles of v_mfma_i32_32x32x32_i8, accumulators in AGPRs).  The operands' k-slot layout is
+// the same for A and B, so only the output layout matters: lane l holds column l%32,
+// register q holds row 8*(q/4) + 4*(l/32) + q%4.
 constexpr int GM = 256, GN = 256;
+constexpr long KCH = 65536;  // partial sums are reduced mod m before |sum| could reach 2^31
 
-// 16 bytes at p, bytes at index >= cnt replaced by 0 (static indexing only)
-__device__ __forceinline__ v4i load16_guarded(const int8_t* p, long cnt) {
-  v4i x = {0, 0, 0, 0};
+template <int NA>
+__device__ __forceinline__ void reduce_acc(v16i_t (&acc)[NA][4], uint32_t m, double inv_m) {
 #pragma unroll
-  for (int q = 0; q < 16; ++q) {
-    uint32_t b = (q < cnt) ? (uint8_t)p[q] : 0u;
-    x[q >> 2] |= (int)(b << (8 * (q & 3)));
-  }
-  return x;
+  for (int a = 0; a < NA; ++a)
+#pragma unroll
+    for (int b = 0; b < 4; ++b)
+#pragma unroll
+      for (int q = 0; q < 16; ++q) {
+        const int v = acc[a][b][q];
+        acc[a][b][q] = v - (int32_t)m * (int32_t)__builtin_floor((double)v * inv_m);
+      }
+}
+__device__ __forceinline__ void store_digits(const v16i_t (&acc)[2][4], uint8_t* __restrict__ C, long M, long N,
+                                             long ldc, long i0, long j0, int wm, int wn, int lane, double m,
+                                             double inv_m, double y) {
+#pragma unroll
+  for (int a = 0; a < 2; ++a)
+#pragma unroll
+    for (int b = 0; b < 4; ++b) {
+      const long gj = j0 + wn * 128 + b * 32 + (lane & 31);
+      const long gib = i0 + wm * 64 + a * 32 + 4 * (lane >> 5);
+#pragma unroll
+      for (int q = 0; q < 16; ++q) {
+        const long gi = gib + 8 * (q >> 2) + (q & 3);
+        const uint8_t u = crt_digit(acc[a][b][q], m, inv_m, y);
+        if (gi < M && gj < N) C[gi * ldc + gj] = u;
+      }
+    }
 }
 
+// K advances in GK-deep stages double-buffered through LDS, the next stage fetched from
+// HBM into registers while the current one feeds the MFMAs.  K % 16 == 0.
 template <int GK>
 __global__ __launch_bounds__(512) void k_gemm_i8_mod(const int8_t* __restrict__ A, const int8_t* __restrict__ B,
-                                                     uint8_t* __restrict__ C, long M, long N, long K,
+                                                     uint8_t* __restrict__ C, long M, long N, long K, long ldc,
                                                      const CrtTable* __restrict__ tab, int lower) {
   if (lower && blockIdx.x > blockIdx.y) return;  // symmetric product: tiles on / below the diagonal
   constexpr int GKP = GK + 16;   // row pitch: keeps the 16-lane groups of ds_read_b128 on distinct banks
@@ -260,12 +365,12 @@ __global__ __launch_bounds__(512) void k_gemm_i8_mod(const int8_t* __restrict__ 
   const double inv_m = tab->inv_m[r];
   A += (size_t)r * M * K;
   B += (size_t)r * N * K;
-  C += (size_t)r * M * N;
+  C += (size_t)r * M * ldc;
   const int t = threadIdx.x, lane = t & 63, w = t >> 6;
   const int wm = w >> 1, wn = w & 1;  // 8 wavefronts: 4 x 2, each 64 rows x 128 columns
   const long i0 = (long)blockIdx.y * GM, j0 = (long)blockIdx.x * GN;
 
-  v16i acc[2][4];
+  v16i_t acc[2][4];
 #pragma unroll
   for (int a = 0; a < 2; ++a)
 #pragma unroll
@@ -273,161 +378,226 @@ __global__ __launch_bounds__(512) void k_gemm_i8_mod(const int8_t* __restrict__ 
 #pragma unroll
       for (int q = 0; q < 16; ++q) acc[a][b][q] = 0;
 
-  const bool kvec = (K % 16) == 0;
   // threads 0..255 stage row t of the A tile, threads 256..511 row t-256 of the B tile
   const bool isA = t < 256;
   const long srow = (isA ? i0 : j0) + (t & 255);
   const long slim = isA ? M : N;
   const int8_t* ps = (isA ? A : B) + srow * K;
   int8_t (*Ss)[GM][GKP] = isA ? As : Bs;
-  v4i rs[NV];
+  v4i_t rs[NV];
   auto fetch = [&](long k0) {
 #pragma unroll
     for (int v = 0; v < NV; ++v) {
-      long kk = k0 + 16 * v;
-      if (srow < slim && kvec && kk + 16 <= K) rs[v] = *reinterpret_cast<const v4i*>(ps + kk);
-      else rs[v] = (srow < slim && kk < K) ? load16_guarded(ps + kk, K - kk) : v4i{0, 0, 0, 0};
+      const long kk = k0 + 16 * v;
+      rs[v] = (srow < slim && kk < K) ? *reinterpret_cast<const v4i_t*>(ps + kk) : v4i_t{0, 0, 0, 0};
     }
   };
   const long nst = (K + GK - 1) / GK;
   fetch(0);
-  const long KCH = 65536;  // partial sums are reduced mod m before |sum| could reach 2^31
   for (long st = 0; st < nst; ++st) {
     const int buf = st & 1;
 #pragma unroll
-    for (int v = 0; v < NV; ++v) *reinterpret_cast<v4i*>(&Ss[buf][t & 255][16 * v]) = rs[v];
+    for (int v = 0; v < NV; ++v) *reinterpret_cast<v4i_t*>(&Ss[buf][t & 255][16 * v]) = rs[v];
     __syncthreads();
     if (st + 1 < nst) fetch((st + 1) * GK);
 #pragma unroll
     for (int ks = 0; ks < GK; ks += 32) {
       const int kq = ks + 16 * (lane >> 5);
-      v4i af[2], bf[4];
+      v4i_t af[2], bf[4];
 #pragma unroll
-      for (int a = 0; a < 2; ++a) af[a] = *reinterpret_cast<const v4i*>(&As[buf][wm * 64 + a * 32 + (lane & 31)][kq]);
+      for (int a = 0; a < 2; ++a) af[a] = *reinterpret_cast<const v4i_t*>(&As[buf][wm * 64 + a * 32 + (lane & 31)][kq]);
 #pragma unroll
-      for (int b = 0; b < 4; ++b) bf[b] = *reinterpret_cast<const v4i*>(&Bs[buf][wn * 128 + b * 32 + (lane & 31)][kq]);
+      for (int b = 0; b < 4; ++b) bf[b] = *reinterpret_cast<const v4i_t*>(&Bs[buf][wn * 128 + b * 32 + (lane & 31)][kq]);
 #pragma unroll
       for (int a = 0; a < 2; ++a)
 #pragma unroll
         for (int b = 0; b < 4; ++b) acc[a][b] = __builtin_amdgcn_mfma_i32_32x32x32_i8(af[a], bf[b], acc[a][b], 0, 0, 0);
     }
-    if (((st + 1) * GK) % KCH == 0 && st + 1 < nst) {
-#pragma unroll
-      for (int a = 0; a < 2; ++a)
-#pragma unroll
-        for (int b = 0; b < 4; ++b)
-#pragma unroll
-          for (int q = 0; q < 16; ++q) {
-            int v = acc[a][b][q];
-            acc[a][b][q] = v - (int32_t)m * (int32_t)__builtin_floor((double)v * inv_m);
-          }
-    }
+    if (((st + 1) * GK) % KCH == 0 && st + 1 < nst) reduce_acc(acc, m, inv_m);
     __syncthreads();  // the other buffer is overwritten next stage
   }
-#pragma unroll
-  for (int a = 0; a < 2; ++a)
-#pragma unroll
-    for (int b = 0; b < 4; ++b) {
-      const long gj = j0 + wn * 128 + b * 32 + (lane & 31);
-      const long gib = i0 + wm * 64 + a * 32 + 4 * (lane >> 5);
-#pragma unroll
-      for (int q = 0; q < 16; ++q) {
-        long gi = gib + 8 * (q >> 2) + (q & 3);
-        int v = acc[a][b][q];
-        int32_t rr = v - (int32_t)m * (int32_t)__builtin_floor((double)v * inv_m);
-        if (rr < 0) rr += m;
-        if (rr >= (int32_t)m) rr -= m;
-        if (gi < M && gj < N) C[gi * N + gj] = (uint8_t)rr;
-      }
-    }
+  store_digits(acc, C, M, N, ldc, i0, j0, wm, wn, lane, (double)m, inv_m, tab->y[r]);
 }
 
 template <int GK>
 constexpr size_t gemm_i8_lds() { return (size_t)2 * (GM + GN) * (GK + 16); }
 
-// ------------------------------------------------------------------ CRT reconstruction
-// C'_ij = sum_r u_r (M/m_r) - q M, u_r = c_r y_r mod m_r, q = round(sum_r u_r / m_r);
-// then C_ij = C'_ij 2^(e_i + f_j - 2W) rounded to dd.
-__device__ __forceinline__ dd u64_to_dd(uint64_t x) {  // exact
-  double h = (double)x;  // round to nearest; may equal 2^64
-  uint64_t hb = (h >= 18446744073709551616.0) ? 0 : (uint64_t)h;
-  int64_t rem = (h >= 18446744073709551616.0) ? -(int64_t)(0 - x) : (int64_t)(x - hb);
-  double l = (double)rem;
-  dd r;
-  quick_two_sum(h, l, r.hi, r.lo);
-  return r;
+// The same product with the operand tiles streamed from HBM straight into LDS
+// (global_load_lds_dwordx4: each lane's 16 bytes land at the instruction's LDS base +
+// 16 * lane), a 4-deep ring of 64-byte K stages with three in flight and no staging
+// registers.  In LDS a tile row is 64 bytes; its 16-byte chunk c sits at position
+// c ^ ((row >> 2) & 3), so the 16 lanes of each ds_read_b128 phase hit distinct banks
+// (each load lane fetches the source chunk that its slot holds).  One barrier per stage:
+// the ring slot refilled at stage s was last read at stage s-1, before that barrier.
+// Needs K % 64 == 0; rows beyond M / N read row M-1 / N-1 (in range) and their products
+// are never stored.
+constexpr int DL_GK = 64, DL_RING = 4;
+constexpr size_t DL_SLOT = 2 * 256 * DL_GK;  // A and B tiles of one stage
+constexpr size_t gemm_i8_dl_lds() { return DL_RING * DL_SLOT; }
+
+__global__ __launch_bounds__(512) void k_gemm_i8_mod_dl(const int8_t* __restrict__ A, const int8_t* __restrict__ B,
+                                                        uint8_t* __restrict__ C, long M, long N, long K, long ldc,
+                                                        const CrtTable* __restrict__ tab, int lower) {
+  if (lower && blockIdx.x > blockIdx.y) return;
+  extern __shared__ __align__(16) int8_t smem[];
+  const int r = blockIdx.z;
+  const uint32_t m = tab->mod[r];
+  const double inv_m = tab->inv_m[r];
+  A += (size_t)r * M * K;
+  B += (size_t)r * N * K;
+  C += (size_t)r * M * ldc;
+  const int t = threadIdx.x, lane = t & 63, w = t >> 6;
+  const int wm = w >> 1, wn = w & 1;
+  const long i0 = (long)blockIdx.y * GM, j0 = (long)blockIdx.x * GN;
+
+  // this lane's part of the 32 load instructions of a stage (wavefront w issues 4w..4w+3):
+  // instruction q covers 16 rows of operand q >> 4 starting at row 16 (q & 15)
+  const int8_t* src[4];
+  int dst[4];
+#pragma unroll
+  for (int j = 0; j < 4; ++j) {
+    const int q = 4 * w + j, op = q >> 4, rb = (q & 15) * 16;
+    const int row = rb + (lane >> 2);
+    const int chunk = (lane & 3) ^ ((row >> 2) & 3);
+    const long grow = op == 0 ? min(i0 + row, M - 1) : min(j0 + row, N - 1);
+    src[j] = (op == 0 ? A : B) + grow * K + 16 * chunk;
+    dst[j] = op * (256 * DL_GK) + rb * DL_GK;
+  }
+  auto issue = [&](long st) {
+    int8_t* slot = smem + (size_t)(st % DL_RING) * DL_SLOT;
+#pragma unroll
+    for (int j = 0; j < 4; ++j)
+      __builtin_amdgcn_global_load_lds(src[j] + st * DL_GK, (__attribute__((address_space(3))) void*)(slot + dst[j]),
+                                       16, 0, 0);
+  };
+  auto frag = [&](const int8_t* tile, int row, int c) {
+    return *reinterpret_cast<const v4i_t*>(tile + row * DL_GK + ((c ^ ((row >> 2) & 3)) << 4));
+  };
+
+  v16i_t acc[2][4];
+#pragma unroll
+  for (int a = 0; a < 2; ++a)
+#pragma unroll
+    for (int b = 0; b < 4; ++b)
+#pragma unroll
+      for (int q = 0; q < 16; ++q) acc[a][b][q] = 0;
+
+  const long nst = K / DL_GK;
+  for (long st = 0; st < 3 && st < nst; ++st) issue(st);
+  for (long st = 0; st < nst; ++st) {
+    // this wavefront's loads of stage st done; those of up to two later stages may fly
+    const long ahead = min(nst - 1 - st, 2L);
+    if (ahead >= 2) __builtin_amdgcn_s_waitcnt(0x0F78);       // vmcnt(8)
+    else if (ahead == 1) __builtin_amdgcn_s_waitcnt(0x0F74);  // vmcnt(4)
+    else __builtin_amdgcn_s_waitcnt(0x0F70);                  // vmcnt(0)
+    // a barrier without the full fence of __syncthreads(), which would also wait for the
+    // later stages' loads: LDS reads of the previous stage are complete, memory is not reordered
+    asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory");
+    if (st + 3 < nst) issue(st + 3);
+    const int8_t* at = smem + (size_t)(st % DL_RING) * DL_SLOT;
+    const int8_t* bt = at + 256 * DL_GK;
+#pragma unroll
+    for (int ks = 0; ks < DL_GK; ks += 32) {
+      const int c = (ks >> 4) + (lane >> 5);
+      v4i_t af[2], bf[4];
+#pragma unroll
+      for (int a = 0; a < 2; ++a) af[a] = frag(at, wm * 64 + a * 32 + (lane & 31), c);
+#pragma unroll
+      for (int b = 0; b < 4; ++b) bf[b] = frag(bt, wn * 128 + b * 32 + (lane & 31), c);
+#pragma unroll
+      for (int a = 0; a < 2; ++a)
+#pragma unroll
+        for (int b = 0; b < 4; ++b) acc[a][b] = __builtin_amdgcn_mfma_i32_32x32x32_i8(af[a], bf[b], acc[a][b], 0, 0, 0);
+    }
+    if (((st + 1) * DL_GK) % KCH == 0 && st + 1 < nst) reduce_acc(acc, m, inv_m);
+  }
+  store_digits(acc, C, M, N, ldc, i0, j0, wm, wn, lane, (double)m, inv_m, tab->y[r]);
 }
 
-__global__ __launch_bounds__(256) void k_crt(const uint8_t* __restrict__ Cr, long M, long N,
+// ------------------------------------------------------------------ CRT reconstruction
+// S = sum_r u_r (M/m_r) in limbs s[0..5] (weights 2^(40q), each an integer below 2^53)
+// -> C' = S - round(S/M) M (|C'| < M/16, so S/M is within 1/16 of an integer and the
+// rounding is robust) -> dd value of C' 2^sh.  Every step is exact up to the final sum of
+// the six limbs into a dd.
+__device__ __forceinline__ dd crt_finish(double (&s)[CRT_L40], const double* mt, int sh) {
+#pragma unroll
+  for (int q = 0; q < CRT_L40 - 1; ++q) {
+    const double c = __builtin_floor(s[q] * 0x1p-40);
+    s[q] = __builtin_fma(c, -0x1p40, s[q]);
+    s[q + 1] += c;
+  }
+  double sv = s[CRT_L40 - 1];
+#pragma unroll
+  for (int q = CRT_L40 - 2; q >= 0; --q) sv = __builtin_fma(sv, 0x1p40, s[q]);
+  const double qn = __builtin_rint(sv * mt[CRT_L40]);
+#pragma unroll
+  for (int q = 0; q < CRT_L40; ++q) s[q] = __builtin_fma(-qn, mt[q], s[q]);
+#pragma unroll
+  for (int q = 0; q < CRT_L40 - 1; ++q) {  // s[0..4] into [0, 2^40), the sign gathers in s[5]
+    const double c = __builtin_floor(s[q] * 0x1p-40);
+    s[q] = __builtin_fma(c, -0x1p40, s[q]);
+    s[q + 1] += c;
+  }
+  const bool neg = s[CRT_L40 - 1] < 0.0;
+  if (neg) {  // magnitude
+#pragma unroll
+    for (int q = 0; q < CRT_L40; ++q) s[q] = -s[q];
+#pragma unroll
+    for (int q = 0; q < CRT_L40 - 1; ++q) {
+      const double c = __builtin_floor(s[q] * 0x1p-40);
+      s[q] = __builtin_fma(c, -0x1p40, s[q]);
+      s[q + 1] += c;
+    }
+  }
+  dd res = mkdd(0.0);
+#pragma unroll
+  for (int q = CRT_L40 - 1; q >= 0; --q) res = dd_add(res, mkdd(__builtin_ldexp(s[q], 40 * q + sh)));
+  return neg ? dd_neg(res) : res;
+}
+
+// a thread takes 4 consecutive columns of one row (one dword of digits per modulus)
+__global__ __launch_bounds__(256) void k_crt(const uint8_t* __restrict__ Cr, long M, long N, long Np,
                                              const int* __restrict__ er, const int* __restrict__ fc, int W,
                                              const CrtTable* __restrict__ tab, double* __restrict__ chi,
                                              double* __restrict__ clo, long ldc, int lower, int subtract) {
-  __shared__ uint32_t Mr[CRT_MAXR][CRT_LIMBS];
-  __shared__ uint32_t Ml[CRT_LIMBS];
-  __shared__ uint32_t mods[CRT_MAXR], ys[CRT_MAXR];
-  __shared__ double invs[CRT_MAXR];
+  __shared__ __align__(16) double mr[CRT_MAXR][CRT_L40];
+  __shared__ double mt[CRT_L40 + 1];
   const int R = tab->R;
-  for (int i = threadIdx.x; i < R * CRT_LIMBS; i += blockDim.x) Mr[i / CRT_LIMBS][i % CRT_LIMBS] = tab->Mr[i / CRT_LIMBS][i % CRT_LIMBS];
-  if (threadIdx.x < CRT_LIMBS) Ml[threadIdx.x] = tab->M[threadIdx.x];
-  if (threadIdx.x < R) {
-    mods[threadIdx.x] = tab->mod[threadIdx.x];
-    ys[threadIdx.x] = tab->yinv[threadIdx.x];
-    invs[threadIdx.x] = tab->inv_m[threadIdx.x];
-  }
+  for (int x = threadIdx.x; x < R * CRT_L40; x += blockDim.x) mr[x / CRT_L40][x % CRT_L40] = tab->mr40[x / CRT_L40][x % CRT_L40];
+  if (threadIdx.x < CRT_L40) mt[threadIdx.x] = tab->m40[threadIdx.x];
+  if (threadIdx.x == CRT_L40) mt[CRT_L40] = tab->inv_M;
   __syncthreads();
-  const size_t plane = (size_t)M * N;
-  for (size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x; idx < plane; idx += (size_t)gridDim.x * blockDim.x) {
-    if (lower && (long)(idx % N) > (long)(idx / N)) continue;
-    uint64_t acc[CRT_LIMBS] = {0, 0, 0, 0, 0, 0, 0, 0};
-    double t = 0.0;
+  const size_t plane = (size_t)M * Np;
+  const long j0 = ((long)blockIdx.x * 256 + threadIdx.x) * 4;
+  for (long i = blockIdx.y; i < M; i += gridDim.y) {
+    if (j0 >= N || (lower && j0 > i)) continue;
+    const uint8_t* src = Cr + (size_t)i * Np + j0;
+    double s[4][CRT_L40];
+#pragma unroll
+    for (int e = 0; e < 4; ++e)
+#pragma unroll
+      for (int q = 0; q < CRT_L40; ++q) s[e][q] = 0.0;
     for (int r = 0; r < R; ++r) {
-      uint32_t m = mods[r];
-      uint32_t u = mod32((uint32_t)Cr[(size_t)r * plane + idx] * ys[r], m, invs[r]);
-      t += (double)u * invs[r];
+      const uint32_t d = *reinterpret_cast<const uint32_t*>(src + (size_t)r * plane);
+      const double u[4] = {(double)(d & 255u), (double)((d >> 8) & 255u), (double)((d >> 16) & 255u), (double)(d >> 24)};
 #pragma unroll
-      for (int q = 0; q < CRT_LIMBS; ++q) acc[q] += (uint64_t)u * Mr[r][q];
-    }
-    uint64_t qn = (uint64_t)__builtin_rint(t);
-    // S - qn*M with carries, 256-bit two's complement
-    uint32_t v[CRT_LIMBS];
-    int64_t carry = 0;
+      for (int q = 0; q < CRT_L40; ++q) {
+        const double mq = mr[r][q];
 #pragma unroll
-    for (int q = 0; q < CRT_LIMBS; ++q) {
-      int64_t s = (int64_t)(acc[q] & 0xffffffffull) - (int64_t)(qn * Ml[q]) + carry;
-      // acc[q] >> 32 contributes to the next limb
-      v[q] = (uint32_t)s;
-      carry = (s >> 32) + (int64_t)(acc[q] >> 32);
-    }
-    bool neg = (v[CRT_LIMBS - 1] >> 31) != 0;
-    if (neg) {  // magnitude
-      uint64_t c = 1;
-#pragma unroll
-      for (int q = 0; q < CRT_LIMBS; ++q) {
-        uint64_t s = (uint64_t)(uint32_t)~v[q] + c;
-        v[q] = (uint32_t)s;
-        c = s >> 32;
+        for (int e = 0; e < 4; ++e) s[e][q] = __builtin_fma(u[e], mq, s[e][q]);
       }
     }
-    int top = CRT_LIMBS - 1;
-    while (top > 0 && v[top] == 0) --top;
-    dd res = mkdd(0.0);
-    if (top > 0 || v[0] != 0) {
-      // sum of the top six 32-bit limbs (192 bits) as two 64-bit words and two 32-bit words
-      const int base = er[idx / N] + fc[idx % N] - 2 * W;
-      for (int q = top; q >= 0 && q >= top - 5; q -= 2) {
-        uint64_t wv = (q >= 1) ? (((uint64_t)v[q] << 32) | v[q - 1]) : (uint64_t)v[q];
-        int sh = 32 * ((q >= 1) ? q - 1 : q) + base;
-        dd part = u64_to_dd(wv);
-        part.hi = __builtin_ldexp(part.hi, sh);
-        part.lo = __builtin_ldexp(part.lo, sh);
-        res = dd_add(res, part);
-      }
-      if (neg) res = dd_neg(res);
+    const int base = er[i] - 2 * W;
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+      const long j = j0 + e;
+      if (j >= N || (lower && j > i)) continue;
+      dd res = crt_finish(s[e], mt, base + fc[j]);
+      const size_t co = (size_t)i * ldc + j;
+      if (subtract) res = dd_sub(mkdd(chi[co], clo[co]), res);
+      chi[co] = res.hi;
+      clo[co] = res.lo;
     }
-    const size_t co = (size_t)(idx / N) * ldc + idx % N;
-    if (subtract) res = dd_sub(mkdd(chi[co], clo[co]), res);
-    chi[co] = res.hi;
-    clo[co] = res.lo;
   }
 }
 
@@ -440,7 +610,7 @@ struct SplitWork {
   int tabR = -1;
 };
 SplitWork g_work;
-std::map<uint64_t, int> g_tuned;  // autotuned K-stage depth per shape class (under g_work.mu)
+std::map<uint64_t, int> g_tuned;  // autotuned GEMM kernel per shape class (under g_work.mu)
 
 uint64_t shape_key(size_t m, size_t n, size_t k, int R) {
   auto bucket = [](size_t v) {  // next power of two
@@ -454,8 +624,9 @@ uint64_t shape_key(size_t m, size_t n, size_t k, int R) {
 }  // namespace
 
 int set_split_gemm_variant(size_t m, size_t n, size_t k, int R, int variant) {
-  if (variant != 0 && variant != 32 && variant != 64 && variant != 128) {
-    set_error("split gemm variant must be 0 (autotune), 32, 64 or 128");
+  if (variant != 0 && variant != 32 && variant != 64 && variant != 128 && variant != 3064) {
+    set_error("split gemm variant must be 0 (autotune), 32, 64, 128 (K stages of that depth staged through "
+              "registers) or 3064 (direct-to-LDS ring of 64-deep stages)");
     return HP_EINVAL;
   }
   std::lock_guard<std::mutex> lock(g_work.mu);
@@ -495,11 +666,12 @@ int split_gemm_general(const SplitOperand& A, const SplitOperand& B, double* chi
     return HP_OK;
   }
   const int W = crt_bits_W(R, k);
-  if (W < 20 || W > 120) {
+  if (W < 20 || W > 117) {
     set_error("split int8 gemm: k = %zu too large for %d moduli", k, R);
     return HP_EINVAL;
   }
-  const size_t resA = (size_t)R * m * k, resB = (size_t)R * n * k, resC = (size_t)R * m * n;
+  const size_t Kp = (k + DL_GK - 1) / DL_GK * DL_GK, Np = (n + 3) / 4 * 4;
+  const size_t resA = (size_t)R * m * Kp, resB = (size_t)R * n * Kp, resC = (size_t)R * m * Np;
   const size_t need = resA + resB + resC + (m + n) * sizeof(int) + 1024;
   std::lock_guard<std::mutex> lock(g_work.mu);
   if (g_work.bytes < need) {
@@ -517,6 +689,8 @@ int split_gemm_general(const SplitOperand& A, const SplitOperand& B, double* chi
                                (int)gemm_i8_lds<64>()));
     HP_HIP(hipFuncSetAttribute((const void*)k_gemm_i8_mod<32>, hipFuncAttributeMaxDynamicSharedMemorySize,
                                (int)gemm_i8_lds<32>()));
+    HP_HIP(hipFuncSetAttribute((const void*)k_gemm_i8_mod_dl, hipFuncAttributeMaxDynamicSharedMemorySize,
+                               (int)gemm_i8_dl_lds()));
   }
   if (g_work.tabR != R) {
     static CrtTable host;
@@ -535,19 +709,20 @@ int split_gemm_general(const SplitOperand& A, const SplitOperand& B, double* chi
   // scales: rows of A (contiguous unless gemm_atb), columns of B
   hipLaunchKernelGGL(k_fill_int, dim3((unsigned)((m + n + 255) / 256)), dim3(256), 0, s, eA, (long)(m + n), -100000);
   HP_LAUNCH_CHECK();
-  const unsigned kch = (unsigned)((k + 255) / 256);
+  auto gy = [](size_t blocks) { return (unsigned)std::min<size_t>(blocks, 65535); };
   auto scales_and_residues = [&](const SplitOperand& X, long rows, int* e, int8_t* out) -> int {
     if (X.contig)
-      hipLaunchKernelGGL(k_scale_rows, dim3((unsigned)((rows + 3) / 4)), dim3(256), 0, s, X.hi, rows, (long)k, X.ld, e);
+      hipLaunchKernelGGL(k_scale_rows, dim3(gy((rows + 3) / 4)), dim3(256), 0, s, X.hi, rows, (long)k, X.ld, e);
     else
-      hipLaunchKernelGGL(k_scale_cols, dim3((unsigned)((rows + 255) / 256), kch), dim3(256), 0, s, X.hi, rows, (long)k,
-                         X.ld, e);
+      hipLaunchKernelGGL(k_scale_cols, dim3((unsigned)((rows + 63) / 64), (unsigned)((k + 255) / 256)), dim3(256), 0,
+                         s, X.hi, rows, (long)k, X.ld, e);
     HP_LAUNCH_CHECK();
-    dim3 g((unsigned)((k + 63) / 64), (unsigned)((rows + 63) / 64));
     if (X.contig)
-      hipLaunchKernelGGL(k_residues<true>, g, dim3(256), 0, s, X.hi, X.lo, rows, (long)k, X.ld, e, W, g_work.tab, out);
+      hipLaunchKernelGGL(k_residues_rows, dim3((unsigned)((Kp + 511) / 512), gy((rows + 3) / 4)), dim3(256), 0, s,
+                         X.hi, X.lo, rows, (long)k, (long)Kp, X.ld, e, W, g_work.tab, out);
     else
-      hipLaunchKernelGGL(k_residues<false>, g, dim3(256), 0, s, X.hi, X.lo, rows, (long)k, X.ld, e, W, g_work.tab, out);
+      hipLaunchKernelGGL(k_residues_cols, dim3((unsigned)(Kp / 64), gy((rows + 63) / 64)), dim3(256), 0, s, X.hi,
+                         X.lo, rows, (long)k, (long)Kp, X.ld, e, W, g_work.tab, out);
     HP_LAUNCH_CHECK();
     return HP_OK;
   };
@@ -560,21 +735,25 @@ int split_gemm_general(const SplitOperand& A, const SplitOperand& B, double* chi
   }
   tm.mark(s);
   dim3 gg((unsigned)((n + GN - 1) / GN), (unsigned)((m + GM - 1) / GM), (unsigned)R);
+  const int low = opt.lower_only ? 1 : 0;
   auto launch = [&](int variant) -> int {
-    if (variant == 128)
+    if (variant == 3064)
+      hipLaunchKernelGGL(k_gemm_i8_mod_dl, gg, dim3(512), gemm_i8_dl_lds(), s, rA, rB, rC, (long)m, (long)n, (long)Kp,
+                         (long)Np, g_work.tab, low);
+    else if (variant == 128)
       hipLaunchKernelGGL(k_gemm_i8_mod<128>, gg, dim3(512), gemm_i8_lds<128>(), s, rA, rB, rC, (long)m, (long)n,
-                         (long)k, g_work.tab, opt.lower_only ? 1 : 0);
+                         (long)Kp, (long)Np, g_work.tab, low);
     else if (variant == 32)
       hipLaunchKernelGGL(k_gemm_i8_mod<32>, gg, dim3(512), gemm_i8_lds<32>(), s, rA, rB, rC, (long)m, (long)n,
-                         (long)k, g_work.tab, opt.lower_only ? 1 : 0);
+                         (long)Kp, (long)Np, g_work.tab, low);
     else
       hipLaunchKernelGGL(k_gemm_i8_mod<64>, gg, dim3(512), gemm_i8_lds<64>(), s, rA, rB, rC, (long)m, (long)n,
-                         (long)k, g_work.tab, opt.lower_only ? 1 : 0);
+                         (long)Kp, (long)Np, g_work.tab, low);
     HP_LAUNCH_CHECK();
     return HP_OK;
   };
-  // autotuner: the first call for a shape class times every K-stage depth on the real
-  // operands (HIP events on this stream) and keeps the fastest
+  // autotuner: the first call for a shape class times each kernel on the real operands
+  // (HIP events on this stream) and keeps the fastest
   const uint64_t key = shape_key(m, n, k, R) | (opt.lower_only ? (1ull << 40) : 0);  // R already defaulted
   int variant = 0;
   {
@@ -582,13 +761,13 @@ int split_gemm_general(const SplitOperand& A, const SplitOperand& B, double* chi
     if (it != g_tuned.end()) variant = it->second;
   }
   if (!variant) {
-    const int cand[3] = {32, 64, 128};
+    const int cand[3] = {64, 128, 3064};
     float best = 1e30f;
     hipEvent_t e0, e1;
     HP_HIP(hipEventCreate(&e0));
     HP_HIP(hipEventCreate(&e1));
     for (int c : cand) {
-      if (c > 32 && (long)k < c) continue;
+      if (c == 128 && Kp < 128) continue;
       int rc = launch(c);  // warm
       if (rc) return rc;
       HP_HIP(hipEventRecord(e0, s));
@@ -608,10 +787,8 @@ int split_gemm_general(const SplitOperand& A, const SplitOperand& B, double* chi
   }
   if (int rc = launch(variant)) return rc;
   tm.mark(s);
-  size_t total = m * n;
-  unsigned gc = (unsigned)std::min<size_t>((total + 255) / 256, 1u << 16);
-  hipLaunchKernelGGL(k_crt, dim3(gc), dim3(256), 0, s, rC, (long)m, (long)n, eA, fB, W, g_work.tab, chi, clo,
-                     (long)ldc, opt.lower_only ? 1 : 0, opt.subtract ? 1 : 0);
+  hipLaunchKernelGGL(k_crt, dim3((unsigned)((n + 1023) / 1024), gy(m)), dim3(256), 0, s, rC, (long)m, (long)n,
+                     (long)Np, eA, fB, W, g_work.tab, chi, clo, (long)ldc, low, opt.subtract ? 1 : 0);
   HP_LAUNCH_CHECK();
   tm.mark(s);
   // the workspace is reused by the next call: keep it owned until this one has run

@@ -65,7 +65,7 @@ def test_split_i8_vs_f256(qp, m, n, k):
     assert e_split <= max(4 * e_plain, 2.0 ** -104)
 
 
-@pytest.mark.parametrize("variant", [32, 64, 128])
+@pytest.mark.parametrize("variant", [32, 64, 128, 3064])
 def test_split_i8_every_k_stage_variant(qp, variant):
     m, n, k = 300, 270, 640
     a = qp.gen_dd_normal(41, m * k)
@@ -106,6 +106,25 @@ def test_split_i8_exact_on_small_integers(qp):
     a[:, 0] = rng.integers(-1000, 1000, m * k)
     b[:, 0] = rng.integers(-1000, 1000, k * n)
     c = dev_gemm(qp, a, b, m, n, k, qp.HP_GEMM_SPLIT_I8)
+    ex = (a[:, 0].reshape(m, k).astype(np.int64) @ b[:, 0].reshape(k, n).astype(np.int64)).ravel()
+    assert np.array_equal(c[:, 0], ex.astype(np.float64)) and not c[:, 1].any()
+
+
+@pytest.mark.parametrize("variant", [128, 3064])
+def test_split_i8_exact_beyond_one_reduction_chunk(qp, variant):
+    """k = 70000: the int32 sums are reduced mod m once after 65536 products; K is padded to
+    70016 with zero residues and N = 47 leaves a partial dword of CRT digits per row."""
+    m, n, k = 33, 47, 70000
+    rng = np.random.default_rng(2)
+    a = np.zeros((m * k, 2))
+    b = np.zeros((k * n, 2))
+    a[:, 0] = rng.integers(-1000, 1000, m * k)
+    b[:, 0] = rng.integers(-1000, 1000, k * n)
+    qp.split_gemm_set_variant(m, n, k, variant)
+    try:
+        c = dev_gemm(qp, a, b, m, n, k, qp.HP_GEMM_SPLIT_I8)
+    finally:
+        qp.split_gemm_set_variant(m, n, k, 0)
     ex = (a[:, 0].reshape(m, k).astype(np.int64) @ b[:, 0].reshape(k, n).astype(np.int64)).ravel()
     assert np.array_equal(c[:, 0], ex.astype(np.float64)) and not c[:, 1].any()
 
