@@ -330,15 +330,17 @@ __device__ __forceinline__ void reduce_acc(v16i_t (&acc)[NA][4], uint32_t m, dou
         acc[a][b][q] = v - (int32_t)m * (int32_t)__builtin_floor((double)v * inv_m);
       }
 }
-__device__ __forceinline__ void store_digits(const v16i_t (&acc)[2][4], uint8_t* __restrict__ C, long M, long N,
+// wavefront (wm, wn) holds rows wm*32*NA + 32a and columns wn*128 + 32b of the block tile
+template <int NA>
+__device__ __forceinline__ void store_digits(const v16i_t (&acc)[NA][4], uint8_t* __restrict__ C, long M, long N,
                                              long ldc, long i0, long j0, int wm, int wn, int lane, double m,
                                              double inv_m, double y) {
 #pragma unroll
-  for (int a = 0; a < 2; ++a)
+  for (int a = 0; a < NA; ++a)
 #pragma unroll
     for (int b = 0; b < 4; ++b) {
       const long gj = j0 + wn * 128 + b * 32 + (lane & 31);
-      const long gib = i0 + wm * 64 + a * 32 + 4 * (lane >> 5);
+      const long gib = i0 + wm * 32 * NA + a * 32 + 4 * (lane >> 5);
 #pragma unroll
       for (int q = 0; q < 16; ++q) {
         const long gi = gib + 8 * (q >> 2) + (q & 3);
@@ -514,6 +516,296 @@ __global__ __launch_bounds__(512) void k_gemm_i8_mod_dl(const int8_t* __restrict
   store_digits(acc, C, M, N, ldc, i0, j0, wm, wn, lane, (double)m, inv_m, tab->y[r]);
 }
 
+// XCD-aware tile order.  Workgroups are dispatched to the 8 XCDs round-robin by linear id,
+// and each XCD has its own L2.  Linear id b is renumbered so that XCD c takes the contiguous
+// logical range [c B/8, (c+1) B/8); within a modulus the logical order walks 4 x 8 patches of
+// 256 x 256 tiles, so the 32 blocks resident on an XCD share 4 A row tiles and 8 B column
+// tiles (12 tiles streamed through its L2 per K stage instead of 18 in launch order).
+__device__ __forceinline__ void xcd_tile(long& tx, long& ty, int& tz) {
+  const unsigned gx = gridDim.x, gy = gridDim.y, T = gx * gy, nb = T * gridDim.z;
+  unsigned b = blockIdx.x + gx * (blockIdx.y + gy * blockIdx.z);
+  if (nb % 8 == 0) b = (b % 8) * (nb / 8) + b / 8;
+  tz = (int)(b / T);
+  const unsigned u = b % T;
+  if (gx % 8 == 0 && gy % 4 == 0) {
+    const unsigned p = u / 32, v = u % 32, pw = gx / 8;
+    ty = (long)((p / pw) * 4 + v / 8);
+    tx = (long)((p % pw) * 8 + v % 8);
+  } else {
+    ty = (long)(u / gx);
+    tx = (long)(u % gx);
+  }
+}
+
+// s_waitcnt immediate that waits until at most n vector-memory operations are outstanding
+// (vmcnt is split: bits 3:0 and 15:14; expcnt and lgkmcnt left at their no-wait maxima)
+constexpr unsigned waitcnt_vm(int n) { return 0x0F70u | (unsigned)(n & 15) | ((unsigned)(n >> 4) << 14); }
+
+// Software-pipelined form of the direct-to-LDS product (same ring, swizzle and loads).
+// NA = 2: 8 wavefronts of 64 x 128; NA = 4: 4 wavefronts of 128 x 128, one per SIMD, which
+// read a third fewer LDS bytes per MFMA.  Per stage st the fragments of its second 32-deep
+// half are read while the MFMAs of its first half run; the barrier that admits stage st+1
+// sits between the two MFMA halves, so the MFMAs already queued cover it; the first
+// fragments of stage st+1 are then read while the second half's MFMAs run.  That barrier
+// also waits for every wavefront's LDS reads of stage st, so its ring slot takes stage st+4.
+template <int NA>
+__device__ __forceinline__ void gemm_dlp_body(const int8_t* __restrict__ A, const int8_t* __restrict__ B,
+                                              uint8_t* __restrict__ C, long M, long N, long K, long ldc,
+                                              const CrtTable* __restrict__ tab, int lower) {
+  constexpr int NW = 16 / NA;   // wavefronts
+  constexpr int LPW = 32 / NW;  // load instructions per wavefront and stage
+  extern __shared__ __align__(16) int8_t smem[];
+  long tx, ty;
+  int r;
+  xcd_tile(tx, ty, r);
+  if (lower && tx > ty) return;  // symmetric product: tiles on / below the diagonal
+  const uint32_t m = tab->mod[r];
+  const double inv_m = tab->inv_m[r];
+  A += (size_t)r * M * K;
+  B += (size_t)r * N * K;
+  C += (size_t)r * M * ldc;
+  const int t = threadIdx.x, lane = t & 63, w = t >> 6;
+  const int wm = w >> 1, wn = w & 1;
+  const long i0 = ty * GM, j0 = tx * GN;
+
+  const int8_t* src[LPW];
+  int dst[LPW];
+#pragma unroll
+  for (int j = 0; j < LPW; ++j) {
+    const int q = LPW * w + j, op = q >> 4, rb = (q & 15) * 16;
+    const int row = rb + (lane >> 2);
+    const int chunk = (lane & 3) ^ ((row >> 2) & 3);
+    const long grow = op == 0 ? min(i0 + row, M - 1) : min(j0 + row, N - 1);
+    src[j] = (op == 0 ? A : B) + grow * K + 16 * chunk;
+    dst[j] = op * (256 * DL_GK) + rb * DL_GK;
+  }
+  auto issue = [&](long st) {
+    int8_t* slot = smem + (size_t)(st % DL_RING) * DL_SLOT;
+#pragma unroll
+    for (int j = 0; j < LPW; ++j)
+      __builtin_amdgcn_global_load_lds(src[j] + st * DL_GK, (__attribute__((address_space(3))) void*)(slot + dst[j]),
+                                       16, 0, 0);
+  };
+  // this wavefront's loads of the next stage done (at most `ahead` later stages in flight),
+  // then the bare barrier (LDS reads complete, no wait for the later stages' loads)
+  auto admit = [&](long ahead) {
+    if (ahead >= 2) __builtin_amdgcn_s_waitcnt(waitcnt_vm(2 * LPW));
+    else if (ahead == 1) __builtin_amdgcn_s_waitcnt(waitcnt_vm(LPW));
+    else __builtin_amdgcn_s_waitcnt(waitcnt_vm(0));
+    asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory");
+  };
+  auto read_frags = [&](long st, int ks, v4i_t (&af)[NA], v4i_t (&bf)[4]) {
+    const int8_t* at = smem + (size_t)(st % DL_RING) * DL_SLOT;
+    const int8_t* bt = at + 256 * DL_GK;
+    const int c = (ks >> 4) + (lane >> 5);
+#pragma unroll
+    for (int a = 0; a < NA; ++a) {
+      const int row = wm * 32 * NA + a * 32 + (lane & 31);
+      af[a] = *reinterpret_cast<const v4i_t*>(at + row * DL_GK + ((c ^ ((row >> 2) & 3)) << 4));
+    }
+#pragma unroll
+    for (int b = 0; b < 4; ++b) {
+      const int row = wn * 128 + b * 32 + (lane & 31);
+      bf[b] = *reinterpret_cast<const v4i_t*>(bt + row * DL_GK + ((c ^ ((row >> 2) & 3)) << 4));
+    }
+  };
+
+  v16i_t acc[NA][4];
+#pragma unroll
+  for (int a = 0; a < NA; ++a)
+#pragma unroll
+    for (int b = 0; b < 4; ++b)
+#pragma unroll
+      for (int q = 0; q < 16; ++q) acc[a][b][q] = 0;
+  auto mfmas = [&](const v4i_t (&af)[NA], const v4i_t (&bf)[4]) {
+#pragma unroll
+    for (int a = 0; a < NA; ++a)
+#pragma unroll
+      for (int b = 0; b < 4; ++b) acc[a][b] = __builtin_amdgcn_mfma_i32_32x32x32_i8(af[a], bf[b], acc[a][b], 0, 0, 0);
+  };
+
+  const long nst = K / DL_GK;
+  for (long st = 0; st < 3 && st < nst; ++st) issue(st);
+  admit(min(nst - 1, 2L));
+  if (nst > 3) issue(3);
+  v4i_t a0[NA], b0[4], a1[NA], b1[4];
+  read_frags(0, 0, a0, b0);
+  // 256 accumulators (NA = 4) leave no registers for an in-loop reduction: K <= 65536 there
+  long st = 0;
+  // steady state without branches between the LDS reads and the MFMAs that consume them
+  // (a join there would make the compiler wait for every outstanding LDS read): stages
+  // st+2, st+3 in flight, stage st+4 issued
+  // The issue order is pinned: each LDS read (and each LDS-DMA load) goes out right after
+  // an MFMA, so the reads overlap the MFMAs instead of gathering at the end of the body.
+  constexpr int NR = NA + 4;  // fragment reads per 32-deep half
+  for (; st + 4 < nst; ++st) {
+    read_frags(st, 32, a1, b1);
+    mfmas(a0, b0);
+#pragma unroll
+    for (int i = 0; i < NR; ++i) {
+      __builtin_amdgcn_sched_group_barrier(0x008, 1, 0);  // one MFMA
+      __builtin_amdgcn_sched_group_barrier(0x100, 1, 0);  // one LDS read
+    }
+    __builtin_amdgcn_sched_group_barrier(0x008, 4 * NA - NR, 0);
+    __builtin_amdgcn_sched_barrier(0);
+    admit(2);
+    __builtin_amdgcn_sched_barrier(0);
+    issue(st + 4);
+    read_frags(st + 1, 0, a0, b0);
+    mfmas(a1, b1);
+#pragma unroll
+    for (int i = 0; i < NR; ++i) {
+      __builtin_amdgcn_sched_group_barrier(0x008, 1, 0);
+      __builtin_amdgcn_sched_group_barrier(0x100, 1, 0);
+    }
+#pragma unroll
+    for (int i = 0; i < 4 * NA - NR; ++i) {
+      __builtin_amdgcn_sched_group_barrier(0x008, 1, 0);
+      __builtin_amdgcn_sched_group_barrier(0x020, 1, 0);  // one global -> LDS load
+    }
+    __builtin_amdgcn_sched_barrier(0);
+    if (NA == 2 && ((st + 1) * DL_GK) % KCH == 0) reduce_acc(acc, m, inv_m);
+  }
+  for (; st < nst; ++st) {  // the last (up to) four stages: everything is issued
+    read_frags(st, 32, a1, b1);
+    mfmas(a0, b0);
+    if (st + 1 < nst) {
+      admit(nst - 2 - st);
+      read_frags(st + 1, 0, a0, b0);
+    }
+    mfmas(a1, b1);
+    if (NA == 2 && ((st + 1) * DL_GK) % KCH == 0 && st + 1 < nst) reduce_acc(acc, m, inv_m);
+  }
+  store_digits(acc, C, M, N, ldc, i0, j0, wm, wn, lane, (double)m, inv_m, tab->y[r]);
+}
+
+// Register-staged product (8 wavefronts of 64 x 128): plain global loads (about twice the
+// throughput of direct-to-LDS loads on this access pattern) fetch each stage three stages
+// ahead into three register sets (stage s in set s % 3; the loop is unrolled by three so
+// each set has a fixed name, and K % 192 == 0 so there is no ragged tail) and write it to a
+// double-buffered LDS tile with the chunk swizzle one stage before use.  A single fragment
+// set keeps the registers within two wavefronts per SIMD; the other wavefront of the SIMD
+// covers the LDS read latency.  Per stage st: two 32-deep halves (6 fragment reads, 8
+// MFMAs each), stage st+1 written from its set, barrier (the slot written held stage st-1,
+// read before the previous barrier), stage st+4 fetched into the freed set.
+__global__ __launch_bounds__(512) void k_gemm_i8_mod_rs(const int8_t* __restrict__ A, const int8_t* __restrict__ B,
+                                                        uint8_t* __restrict__ C, long M, long N, long K, long ldc,
+                                                        const CrtTable* __restrict__ tab, int lower) {
+  constexpr int NA = 2;
+  extern __shared__ __align__(16) int8_t smem[];
+  long tx, ty;
+  int r;
+  xcd_tile(tx, ty, r);
+  if (lower && tx > ty) return;
+  const uint32_t m = tab->mod[r];
+  const double inv_m = tab->inv_m[r];
+  A += (size_t)r * M * K;
+  B += (size_t)r * N * K;
+  C += (size_t)r * M * ldc;
+  const int t = threadIdx.x, lane = t & 63, w = t >> 6;
+  const int wm = w >> 1, wn = w & 1;
+  const long i0 = ty * GM, j0 = tx * GN;
+  const long nst = K / DL_GK;
+
+  // piece j (< 4) of this lane: chunk lane & 3 of row 16 (4w + j mod 16) + lane / 4 of operand
+  // (4w + j) / 16
+  const int8_t* src[4];
+  int dst[4];
+#pragma unroll
+  for (int j = 0; j < 4; ++j) {
+    const int q = 4 * w + j, op = q >> 4, rb = (q & 15) * 16;
+    const int row = rb + (lane >> 2), c = lane & 3;
+    src[j] = (op == 0 ? A + min(i0 + row, M - 1) * K : B + min(j0 + row, N - 1) * K) + 16 * c;
+    dst[j] = op * (256 * DL_GK) + row * DL_GK + ((c ^ ((row >> 2) & 3)) << 4);
+  }
+  auto load = [&](long st, v4i_t (&R)[4]) {  // stages past the end re-read the last one (in range)
+    const long s = min(st, nst - 1) * DL_GK;
+#pragma unroll
+    for (int j = 0; j < 4; ++j) R[j] = *reinterpret_cast<const v4i_t*>(src[j] + s);
+  };
+  auto store = [&](long st, const v4i_t (&R)[4]) {
+    int8_t* slot = smem + (size_t)(st & 1) * DL_SLOT;
+#pragma unroll
+    for (int j = 0; j < 4; ++j) *reinterpret_cast<v4i_t*>(slot + dst[j]) = R[j];
+  };
+  v16i_t acc[NA][4];
+#pragma unroll
+  for (int a = 0; a < NA; ++a)
+#pragma unroll
+    for (int b = 0; b < 4; ++b)
+#pragma unroll
+      for (int q = 0; q < 16; ++q) acc[a][b][q] = 0;
+  auto compute = [&](long st) {
+    const int8_t* at = smem + (size_t)(st & 1) * DL_SLOT;
+    const int8_t* bt = at + 256 * DL_GK;
+#pragma unroll
+    for (int ks = 0; ks < DL_GK; ks += 32) {
+      const int c = (ks >> 4) + (lane >> 5);
+      v4i_t af[NA], bf[4];
+#pragma unroll
+      for (int a = 0; a < NA; ++a) {
+        const int row = wm * 64 + a * 32 + (lane & 31);
+        af[a] = *reinterpret_cast<const v4i_t*>(at + row * DL_GK + ((c ^ ((row >> 2) & 3)) << 4));
+      }
+#pragma unroll
+      for (int b = 0; b < 4; ++b) {
+        const int row = wn * 128 + b * 32 + (lane & 31);
+        bf[b] = *reinterpret_cast<const v4i_t*>(bt + row * DL_GK + ((c ^ ((row >> 2) & 3)) << 4));
+      }
+#pragma unroll
+      for (int a = 0; a < NA; ++a)
+#pragma unroll
+        for (int b = 0; b < 4; ++b) acc[a][b] = __builtin_amdgcn_mfma_i32_32x32x32_i8(af[a], bf[b], acc[a][b], 0, 0, 0);
+    }
+  };
+  auto barrier = [&]() { asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory"); };
+  // stage st with stage st+1's data in Rw (refilled with stage st+4)
+  // (the scheduler would otherwise sink the loads into the next stage to save registers,
+  // which shortens the lookahead)
+  auto phase = [&](long st, v4i_t (&Rw)[4]) __attribute__((always_inline)) {
+    compute(st);
+    store(st + 1, Rw);
+    barrier();
+    load(st + 4, Rw);
+    __builtin_amdgcn_sched_barrier(0);
+  };
+  v4i_t R0[4], R1[4], R2[4];
+  load(0, R0);
+  load(1, R1);
+  load(2, R2);
+  store(0, R0);
+  barrier();
+  load(3, R0);
+  long st = 0;
+  // chunks of at most 1020 stages between reductions mod m (|partial sums| < 1023 * 64 *
+  // 2^14 < 2^31 with the final trip), outside the unrolled loop
+  while (st + 3 < nst) {
+    const long stop = st + 1020;
+    for (; st + 3 < nst && st < stop; st += 3) {
+      phase(st, R1);
+      phase(st + 1, R2);
+      phase(st + 2, R0);
+    }
+    if (st + 3 < nst) reduce_acc(acc, m, inv_m);
+  }
+  phase(st, R1);
+  phase(st + 1, R2);
+  compute(st + 2);
+  store_digits(acc, C, M, N, ldc, i0, j0, wm, wn, lane, (double)m, inv_m, tab->y[r]);
+}
+constexpr size_t gemm_i8_rs_lds() { return 2 * DL_SLOT; }
+
+__global__ __launch_bounds__(512) void k_gemm_i8_mod_dlp8(const int8_t* __restrict__ A, const int8_t* __restrict__ B,
+                                                          uint8_t* __restrict__ C, long M, long N, long K, long ldc,
+                                                          const CrtTable* __restrict__ tab, int lower) {
+  gemm_dlp_body<2>(A, B, C, M, N, K, ldc, tab, lower);
+}
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 1))) void k_gemm_i8_mod_dlp4(
+    const int8_t* __restrict__ A, const int8_t* __restrict__ B, uint8_t* __restrict__ C, long M, long N, long K,
+    long ldc, const CrtTable* __restrict__ tab, int lower) {
+  gemm_dlp_body<4>(A, B, C, M, N, K, ldc, tab, lower);
+}
+
 // ------------------------------------------------------------------ CRT reconstruction
 // S = sum_r u_r (M/m_r) in limbs s[0..5] (weights 2^(40q), each an integer below 2^53)
 // -> C' = S - round(S/M) M (|C'| < M/16, so S/M is within 1/16 of an integer and the
@@ -624,9 +916,12 @@ uint64_t shape_key(size_t m, size_t n, size_t k, int R) {
 }  // namespace
 
 int set_split_gemm_variant(size_t m, size_t n, size_t k, int R, int variant) {
-  if (variant != 0 && variant != 32 && variant != 64 && variant != 128 && variant != 3064) {
+  if (variant != 0 && variant != 32 && variant != 64 && variant != 128 && variant != 3064 && variant != 4064 &&
+      variant != 5064 && variant != 6064) {
     set_error("split gemm variant must be 0 (autotune), 32, 64, 128 (K stages of that depth staged through "
-              "registers) or 3064 (direct-to-LDS ring of 64-deep stages)");
+              "registers), 3064 (direct-to-LDS ring of 64-deep stages), 4064 / 5064 (the same ring, "
+              "software-pipelined, 8 wavefronts of 64 x 128 / 4 wavefronts of 128 x 128, K <= 65536) or 6064 "
+              "(register-staged, three stages ahead, K % 192 == 0)");
     return HP_EINVAL;
   }
   std::lock_guard<std::mutex> lock(g_work.mu);
@@ -691,6 +986,12 @@ int split_gemm_general(const SplitOperand& A, const SplitOperand& B, double* chi
                                (int)gemm_i8_lds<32>()));
     HP_HIP(hipFuncSetAttribute((const void*)k_gemm_i8_mod_dl, hipFuncAttributeMaxDynamicSharedMemorySize,
                                (int)gemm_i8_dl_lds()));
+    HP_HIP(hipFuncSetAttribute((const void*)k_gemm_i8_mod_dlp8, hipFuncAttributeMaxDynamicSharedMemorySize,
+                               (int)gemm_i8_dl_lds()));
+    HP_HIP(hipFuncSetAttribute((const void*)k_gemm_i8_mod_dlp4, hipFuncAttributeMaxDynamicSharedMemorySize,
+                               (int)gemm_i8_dl_lds()));
+    HP_HIP(hipFuncSetAttribute((const void*)k_gemm_i8_mod_rs, hipFuncAttributeMaxDynamicSharedMemorySize,
+                               (int)gemm_i8_rs_lds()));
   }
   if (g_work.tabR != R) {
     static CrtTable host;
@@ -737,7 +1038,18 @@ int split_gemm_general(const SplitOperand& A, const SplitOperand& B, double* chi
   dim3 gg((unsigned)((n + GN - 1) / GN), (unsigned)((m + GM - 1) / GM), (unsigned)R);
   const int low = opt.lower_only ? 1 : 0;
   auto launch = [&](int variant) -> int {
-    if (variant == 3064)
+    if (variant == 5064 && Kp > (size_t)KCH) variant = 4064;  // no intermediate reduction in that kernel
+    if (variant == 6064 && Kp % (3 * DL_GK) != 0) variant = 4064;  // whole three-stage trips only
+    if (variant == 6064)
+      hipLaunchKernelGGL(k_gemm_i8_mod_rs, gg, dim3(512), gemm_i8_rs_lds(), s, rA, rB, rC, (long)m, (long)n, (long)Kp,
+                         (long)Np, g_work.tab, low);
+    else if (variant == 5064)
+      hipLaunchKernelGGL(k_gemm_i8_mod_dlp4, gg, dim3(256), gemm_i8_dl_lds(), s, rA, rB, rC, (long)m, (long)n,
+                         (long)Kp, (long)Np, g_work.tab, low);
+    else if (variant == 4064)
+      hipLaunchKernelGGL(k_gemm_i8_mod_dlp8, gg, dim3(512), gemm_i8_dl_lds(), s, rA, rB, rC, (long)m, (long)n,
+                         (long)Kp, (long)Np, g_work.tab, low);
+    else if (variant == 3064)
       hipLaunchKernelGGL(k_gemm_i8_mod_dl, gg, dim3(512), gemm_i8_dl_lds(), s, rA, rB, rC, (long)m, (long)n, (long)Kp,
                          (long)Np, g_work.tab, low);
     else if (variant == 128)
@@ -761,13 +1073,13 @@ int split_gemm_general(const SplitOperand& A, const SplitOperand& B, double* chi
     if (it != g_tuned.end()) variant = it->second;
   }
   if (!variant) {
-    const int cand[3] = {64, 128, 3064};
+    const int cand[4] = {64, 128, 4064, 6064};
     float best = 1e30f;
     hipEvent_t e0, e1;
     HP_HIP(hipEventCreate(&e0));
     HP_HIP(hipEventCreate(&e1));
     for (int c : cand) {
-      if (c == 128 && Kp < 128) continue;
+      if ((c == 128 && Kp < 128) || (c == 6064 && Kp % (3 * DL_GK) != 0)) continue;
       int rc = launch(c);  // warm
       if (rc) return rc;
       HP_HIP(hipEventRecord(e0, s));

@@ -65,9 +65,9 @@ def test_split_i8_vs_f256(qp, m, n, k):
     assert e_split <= max(4 * e_plain, 2.0 ** -104)
 
 
-@pytest.mark.parametrize("variant", [32, 64, 128, 3064])
+@pytest.mark.parametrize("variant", [32, 64, 128, 3064, 4064, 5064, 6064])
 def test_split_i8_every_k_stage_variant(qp, variant):
-    m, n, k = 300, 270, 640
+    m, n, k = 300, 270, 576  # 9 stages of 64: whole three-stage trips for 6064
     a = qp.gen_dd_normal(41, m * k)
     b = qp.gen_dd_normal(42, k * n)
     qp.split_gemm_set_variant(m, n, k, variant)
@@ -110,11 +110,13 @@ def test_split_i8_exact_on_small_integers(qp):
     assert np.array_equal(c[:, 0], ex.astype(np.float64)) and not c[:, 1].any()
 
 
-@pytest.mark.parametrize("variant", [128, 3064])
-def test_split_i8_exact_beyond_one_reduction_chunk(qp, variant):
+@pytest.mark.parametrize("variant,k", [(128, 70000), (3064, 70000), (4064, 70000), (5064, 70000), (6064, 69888)])
+def test_split_i8_exact_beyond_one_reduction_chunk(qp, variant, k):
     """k = 70000: the int32 sums are reduced mod m once after 65536 products; K is padded to
-    70016 with zero residues and N = 47 leaves a partial dword of CRT digits per row."""
-    m, n, k = 33, 47, 70000
+    70016 with zero residues and N = 47 leaves a partial dword of CRT digits per row
+    (5064 has no reduction and runs 4064 there; 6064 takes whole three-stage trips,
+    k = 69888 = 364 * 192)."""
+    m, n = 33, 47
     rng = np.random.default_rng(2)
     a = np.zeros((m * k, 2))
     b = np.zeros((k * n, 2))
