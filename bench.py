@@ -28,9 +28,13 @@ SLICES = 31                      # CRT moduli = int8 GEMMs per dd GEMM
 FP64_PEAK_TFLOPS = 78.6          # MI355X FP64 vector (= FP64 matrix) dense peak
 INT8_PEAK_TOPS = 5000.0          # MI355X dense INT8 matrix peak
 DD_MAC_OPS = 29                  # FP64 ops per reference dd MAC (dd add 20 + dd mul 9)
-# HBM bytes one k_gemm_i8_mod launch at 4096^3 moves (read 4.68e9 + write 0.52e9), from
-# rocprofv3 FETCH_SIZE / WRITE_SIZE counters scaled by a device copy of known size
-GEMM_I8_TRAFFIC_BYTES = 5.20e9
+# int8 GEMM kernel per autotuner variant code
+I8_KERNELS = {32: "k_gemm_i8_mod<32>", 64: "k_gemm_i8_mod<64>", 128: "k_gemm_i8_mod<128>", 3064: "k_gemm_i8_mod_dl",
+              4064: "k_gemm_i8_mod_dlp8", 5064: "k_gemm_i8_mod_dlp4", 6064: "k_gemm_i8_mod_rs"}
+# HBM bytes one launch at 4096^3 x 31 moduli moves, from rocprofv3 FETCH_SIZE / WRITE_SIZE
+# counters scaled by a device copy of known size (k_gemm_i8_mod_dlp8: read 3.126e9 + write
+# 0.520e9); null for a kernel these were not measured on
+I8_TRAFFIC_BYTES = {4064: 3.646e9}
 
 
 def cpu_baseline(a, b, target_s=12.0):
@@ -306,6 +310,7 @@ def main():
     timed = phases[args.warmup:]
     ph = [sum(p[i] for p in timed) / len(timed) for i in range(len(timed[0]))]  # prep, int8 gemms, crt
     gemm_ms = ph[1]
+    variant = qp.split_gemm_variant(M, N, K, SLICES)
     achieved = 2.0 * SLICES * macs / (gemm_ms * 1e-3) / 1e12
 
     # plain path (reference op order) on the same operands
@@ -331,13 +336,13 @@ def main():
         "data": "synthetic: seeded N(0,1) dd operands, generated on the host and resident in HBM",
         "config": {"workload": "dd gemm C=A*B, m=n=k=4096, row-major, seed 1; split = 31 int8 GEMMs + CRT",
                    "parallelism": "1 GPU"},
-        "roofline": {"bound": "mfma", "kernel": "k_gemm_i8_mod", "achieved": round(achieved, 2),
+        "roofline": {"bound": "mfma", "kernel": I8_KERNELS.get(variant, str(variant)), "achieved": round(achieved, 2),
                      "peak": INT8_PEAK_TOPS, "unit": "TOP/s (int8)", "frac": round(achieved / INT8_PEAK_TOPS, 4),
-                     "traffic": GEMM_I8_TRAFFIC_BYTES, "traffic_algorithmic": 3 * SLICES * M * N,
+                     "traffic": I8_TRAFFIC_BYTES.get(variant), "traffic_algorithmic": 3 * SLICES * M * N,
                      "traffic_source": "rocprofv3 --pmc FETCH_SIZE (x2.000) + WRITE_SIZE (x1.000) per launch, "
-                                       "calibrated on a 256 MiB device copy; profiles/k_gemm_i8_mod_4096.md",
+                                       "calibrated on a 256 MiB device copy; profiles/k_gemm_i8_mod_dlp8_4096.md",
                      "kernel_ms": round(gemm_ms, 4),
-                     "k_stage": qp.split_gemm_variant(M, N, K, SLICES),
+                     "variant": variant,
                      "phases_ms": {"scale_and_residues": round(ph[0], 4), "int8_gemms": round(ph[1], 4),
                                    "crt": round(ph[2], 4)}},
         "plain": {"value": round(2.0 * macs / (ms_plain * 1e-3) / 1e9, 3), "unit": "GFLOP/s",
