@@ -294,6 +294,217 @@ double hpo_cond_estimate(const double* l, size_t n) {
   return r * r;
 }
 
+// ------------------------------------------------------------------ eigensolvers
+// dd ordering of the reference (dd.rs:349-356): by hi, then by lo
+static inline bool dd_lt(D a, D b) { return a.hi < b.hi || (a.hi == b.hi && a.lo < b.lo); }
+static const D ONE{1.0, 0.0};
+
+// jacobi_eigen: crates/quad/src/linalg.rs:299-362 (cyclic sweeps, stop when the dd
+// Frobenius norm of the strict upper triangle is below 1e-30, checked before each sweep).
+// mistake (tests of the checks): 1 = eigenvector update with the sine's sign flipped,
+// 2 = tau from (a_pp - a_qq).
+int hpo_jacobi_eigen_planted(const double* a, size_t n, size_t max_iter, double* evals, double* evecs, int mistake) {
+  std::vector<D> s(n * n), v(n * n, D{0.0, 0.0});
+  for (size_t i = 0; i < n * n; ++i) s[i] = ld(a, i);
+  for (size_t i = 0; i < n; ++i) v[i * n + i] = ONE;
+  const D threshold{1e-30, 0.0};
+  for (size_t it = 0; it < max_iter; ++it) {
+    D off{0.0, 0.0};
+    for (size_t i = 0; i < n; ++i)
+      for (size_t j = i + 1; j < n; ++j) {
+        D val = s[i * n + j];
+        off = add(off, mul(val, val));
+      }
+    off = dsqrt(off);
+    if (dd_lt(off, threshold)) {
+      for (size_t i = 0; i < n; ++i) st(evals, i, s[i * n + i]);
+      for (size_t i = 0; i < n; ++i)
+        for (size_t j = 0; j < n; ++j) st(evecs, i * n + j, v[j * n + i]);
+      return 0;
+    }
+    for (size_t p = 0; p < n; ++p)
+      for (size_t q = p + 1; q < n; ++q) {
+        D apq = s[p * n + q];
+        if (dd_lt(dabs(apq), D{1e-60, 0.0})) continue;
+        D app = s[p * n + p], aqq = s[q * n + q];
+        D tau = div(mistake == 2 ? sub(app, aqq) : sub(aqq, app), mul(D{2.0, 0.0}, apq));
+        D tau_abs = dabs(tau);
+        D t_denom = add(tau_abs, dsqrt(add(ONE, mul(tau, tau))));
+        D t = tau.hi >= 0.0 ? div(ONE, t_denom) : div(neg(ONE), t_denom);
+        D c = div(ONE, dsqrt(add(ONE, mul(t, t))));
+        D s_rot = mul(t, c);
+        s[p * n + p] = sub(app, mul(t, apq));
+        s[q * n + q] = add(aqq, mul(t, apq));
+        s[p * n + q] = D{0.0, 0.0};
+        s[q * n + p] = D{0.0, 0.0};
+        for (size_t r = 0; r < n; ++r) {
+          if (r == p || r == q) continue;
+          D srp = s[r * n + p], srq = s[r * n + q];
+          s[r * n + p] = sub(mul(c, srp), mul(s_rot, srq));
+          s[p * n + r] = sub(mul(c, srp), mul(s_rot, srq));
+          s[r * n + q] = add(mul(s_rot, srp), mul(c, srq));
+          s[q * n + r] = add(mul(s_rot, srp), mul(c, srq));
+        }
+        D sv = mistake == 1 ? neg(s_rot) : s_rot;
+        for (size_t r = 0; r < n; ++r) {
+          D vrp = v[r * n + p], vrq = v[r * n + q];
+          v[r * n + p] = sub(mul(c, vrp), mul(sv, vrq));
+          v[r * n + q] = add(mul(sv, vrp), mul(c, vrq));
+        }
+      }
+  }
+  return 1;  // "Jacobi eigenvalue solver did not converge within max iterations"
+}
+int hpo_jacobi_eigen(const double* a, size_t n, size_t max_iter, double* evals, double* evecs) {
+  return hpo_jacobi_eigen_planted(a, n, max_iter, evals, evecs, 0);
+}
+
+// jacobi_eigen of the solver crate: crates/solver/src/ortho.rs:61-152 (classic Jacobi on
+// the largest off-diagonal |a_pq| as f64, at most 100 n^2 rotations, stop below 1e-60).
+// a is destroyed; eigenvectors are the columns of v (row-major n x n).
+static void jacobi_maxpivot(std::vector<D>& a, size_t n, std::vector<D>& evals, std::vector<D>& v) {
+  v.assign(n * n, D{0.0, 0.0});
+  for (size_t i = 0; i < n; ++i) v[i * n + i] = ONE;
+  const size_t max_iter = 100 * n * n;
+  for (size_t it = 0; it < max_iter; ++it) {
+    double max_val = 0.0;
+    size_t p = 0, q = 1;
+    for (size_t i = 0; i < n; ++i)
+      for (size_t j = i + 1; j < n; ++j) {
+        double val = std::fabs(a[i * n + j].hi + a[i * n + j].lo);
+        if (val > max_val) {
+          max_val = val;
+          p = i;
+          q = j;
+        }
+      }
+    if (max_val < 1e-60) break;
+    D app = a[p * n + p], aqq = a[q * n + q], apq = a[p * n + q];
+    D diff = sub(aqq, app);
+    D c = ONE, s = D{0.0, 0.0};
+    if (!(std::fabs(apq.hi + apq.lo) < 1e-70)) {
+      D tau = div(diff, add(apq, apq));
+      D tau_abs = dabs(tau);
+      D t = tau.hi >= 0.0 ? div(ONE, add(tau_abs, dsqrt(add(ONE, mul(tau, tau)))))
+                          : div(neg(ONE), add(tau_abs, dsqrt(add(ONE, mul(tau, tau)))));
+      c = div(ONE, dsqrt(add(ONE, mul(t, t))));
+      s = mul(t, c);
+    }
+    for (size_t j = 0; j < n; ++j) {
+      if (j == p || j == q) continue;
+      D ajp = a[j * n + p], ajq = a[j * n + q];
+      D new_jp = sub(mul(c, ajp), mul(s, ajq));
+      D new_jq = add(mul(s, ajp), mul(c, ajq));
+      a[j * n + p] = new_jp;
+      a[p * n + j] = new_jp;
+      a[j * n + q] = new_jq;
+      a[q * n + j] = new_jq;
+    }
+    const D two{2.0, 0.0};
+    D cs2apq = mul(mul(mul(two, c), s), apq);
+    D new_pp = add(sub(mul(mul(c, c), app), cs2apq), mul(mul(s, s), aqq));
+    D new_qq = add(add(mul(mul(s, s), app), cs2apq), mul(mul(c, c), aqq));
+    a[p * n + p] = new_pp;
+    a[q * n + q] = new_qq;
+    a[p * n + q] = D{0.0, 0.0};
+    a[q * n + p] = D{0.0, 0.0};
+    for (size_t i = 0; i < n; ++i) {
+      D vip = v[i * n + p], viq = v[i * n + q];
+      v[i * n + p] = sub(mul(c, vip), mul(s, viq));
+      v[i * n + q] = add(mul(s, vip), mul(c, viq));
+    }
+  }
+  evals.resize(n);
+  for (size_t i = 0; i < n; ++i) evals[i] = a[i * n + i];
+}
+void hpo_jacobi_eigen_maxpivot(const double* a, size_t n, double* evals, double* evecs) {
+  std::vector<D> w(n * n), ev, v;
+  for (size_t i = 0; i < n * n; ++i) w[i] = ld(a, i);
+  jacobi_maxpivot(w, n, ev, v);
+  for (size_t i = 0; i < n; ++i) st(evals, i, ev[i]);
+  for (size_t i = 0; i < n * n; ++i) st(evecs, i, v[i]);
+}
+
+// ortho.rs:24-56 gemm_atb / gemm_rect (zero entries of A skipped, sums in k order)
+static std::vector<D> ortho_gemm_atb(const std::vector<D>& a, const std::vector<D>& b, size_t m, size_t n, size_t p) {
+  std::vector<D> c(n * p, D{0.0, 0.0});
+  for (size_t k = 0; k < m; ++k)
+    for (size_t i = 0; i < n; ++i) {
+      D aki = a[k * n + i];
+      if (aki.hi == 0.0 && aki.lo == 0.0) continue;
+      for (size_t j = 0; j < p; ++j) c[i * p + j] = add(c[i * p + j], mul(aki, b[k * p + j]));
+    }
+  return c;
+}
+static std::vector<D> ortho_gemm_rect(const std::vector<D>& a, const std::vector<D>& b, size_t m, size_t n,
+                                      size_t p) {
+  std::vector<D> c(m * p, D{0.0, 0.0});
+  for (size_t i = 0; i < m; ++i)
+    for (size_t k = 0; k < n; ++k) {
+      D aik = a[i * n + k];
+      if (aik.hi == 0.0 && aik.lo == 0.0) continue;
+      for (size_t j = 0; j < p; ++j) c[i * p + j] = add(c[i * p + j], mul(aik, b[k * p + j]));
+    }
+  return c;
+}
+
+// canonical_orthogonalize: ortho.rs:158-193.  x: n x n_orth (row-major), returns n_orth.
+static size_t canonical_ortho(const std::vector<D>& s, size_t n, double threshold, std::vector<D>& x) {
+  std::vector<D> w = s, evals, evecs;
+  jacobi_maxpivot(w, n, evals, evecs);
+  std::vector<std::pair<size_t, D>> surv;
+  for (size_t i = 0; i < n; ++i)
+    if (evals[i].hi + evals[i].lo > threshold) surv.push_back({i, evals[i]});
+  std::stable_sort(surv.begin(), surv.end(), [](const std::pair<size_t, D>& a, const std::pair<size_t, D>& b) {
+    return (a.second.hi + a.second.lo) < (b.second.hi + b.second.lo);
+  });
+  const size_t m = surv.size();
+  x.assign(n * m, D{0.0, 0.0});
+  for (size_t k = 0; k < m; ++k) {
+    D inv_sqrt = recip(dsqrt(surv[k].second));
+    for (size_t i = 0; i < n; ++i) x[i * m + k] = mul(evecs[i * n + surv[k].first], inv_sqrt);
+  }
+  return m;
+}
+size_t hpo_canonical_orthogonalize(const double* s, size_t n, double threshold, double* x) {
+  std::vector<D> sv(n * n), xv;
+  for (size_t i = 0; i < n * n; ++i) sv[i] = ld(s, i);
+  size_t m = canonical_ortho(sv, n, threshold, xv);
+  for (size_t i = 0; i < n * m; ++i) st(x, i, xv[i]);
+  return m;
+}
+
+// solve_gen_eigen: ortho.rs:201-248.  evals: n_basis dd (ascending), c: n x n_basis.
+size_t hpo_solve_gen_eigen(const double* f, const double* s, size_t n, double threshold, double* evals, double* c) {
+  std::vector<D> fv(n * n), sv(n * n), x;
+  for (size_t i = 0; i < n * n; ++i) {
+    fv[i] = ld(f, i);
+    sv[i] = ld(s, i);
+  }
+  const size_t m = canonical_ortho(sv, n, threshold, x);
+  std::vector<D> tmp = ortho_gemm_rect(fv, x, n, n, m);
+  std::vector<D> fp = ortho_gemm_atb(x, tmp, n, m, m);
+  for (size_t i = 0; i < m; ++i)
+    for (size_t j = i + 1; j < m; ++j) {
+      D avg = mul(add(fp[i * m + j], fp[j * m + i]), D{0.5, 0.0});
+      fp[i * m + j] = avg;
+      fp[j * m + i] = avg;
+    }
+  std::vector<D> ev, evp;
+  jacobi_maxpivot(fp, m, ev, evp);
+  std::vector<size_t> order(m);
+  for (size_t i = 0; i < m; ++i) order[i] = i;
+  std::stable_sort(order.begin(), order.end(),
+                   [&](size_t a, size_t b) { return (ev[a].hi + ev[a].lo) < (ev[b].hi + ev[b].lo); });
+  std::vector<D> sorted(m * m);
+  for (size_t nk = 0; nk < m; ++nk)
+    for (size_t i = 0; i < m; ++i) sorted[i * m + nk] = evp[i * m + order[nk]];
+  std::vector<D> cv = ortho_gemm_rect(x, sorted, n, m, m);
+  for (size_t k = 0; k < m; ++k) st(evals, k, ev[order[k]]);
+  for (size_t i = 0; i < n * m; ++i) st(c, i, cv[i]);
+  return m;
+}
+
 // ------------------------------------------------------------------ MF4 linalg
 // mf_gemm: multifloat/linalg.rs:39-61
 void hpo_mf4_gemm_rows(const double* a, const double* b, double* c, size_t m, size_t n, size_t k,

@@ -134,6 +134,24 @@ int hpo_overlap_adjoint_dd_planted(const double* centers, const double* alpha, c
 int hpo_overlap_fd_f256(const double* centers, const double* alpha, const double* b_dd, size_t n, const long* coords,
                         size_t ncoords, double h, double* fd_ld, double* fd_q);
 
+/* ---- eigensolvers ---- */
+/* jacobi_eigen (crates/quad/src/linalg.rs:299-362): cyclic Jacobi, max_iter sweeps.
+ * evals: n dd in diagonal order; evecs: n x n dd, row i = eigenvector i.  Returns 1 if
+ * the off-diagonal norm did not fall below 1e-30 within max_iter sweeps. */
+int hpo_jacobi_eigen(const double* a, size_t n, size_t max_iter, double* evals, double* evecs);
+/* the same with a planted mistake, for tests of the checks themselves (1: the sine's
+ * sign flipped in the eigenvector update; 2: tau from a_pp - a_qq) */
+int hpo_jacobi_eigen_planted(const double* a, size_t n, size_t max_iter, double* evals, double* evecs, int mistake);
+/* the solver crate's max-pivot Jacobi (crates/solver/src/ortho.rs:61-152): evals in
+ * diagonal order, evecs n x n with column j = eigenvector j */
+void hpo_jacobi_eigen_maxpivot(const double* a, size_t n, double* evals, double* evecs);
+/* canonical_orthogonalize (ortho.rs:158-193): x receives n x n_orth dd (capacity n x n);
+ * returns n_orth */
+size_t hpo_canonical_orthogonalize(const double* s, size_t n, double threshold, double* x);
+/* solve_gen_eigen (ortho.rs:201-248): evals n_basis dd ascending, c n x n_basis dd
+ * (capacities n and n x n); returns n_basis */
+size_t hpo_solve_gen_eigen(const double* f, const double* s, size_t n, double threshold, double* evals, double* c);
+
 /* ---- counting: FP64 ops per unit of work, from an instrumented restatement ---- */
 /* returns the number of binary64 operations (+,-,*,/,fma,sqrt) one dd MAC s += a*b costs (29) */
 long hpo_count_dd_mac(void);

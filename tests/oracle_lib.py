@@ -101,6 +101,15 @@ def lib():
         L.hpo_count_overlap_entry.argtypes = [C.c_double, _dp, C.c_double, _dp]
         L.hpo_count_overlap_entry.restype = C.c_long
         L.hpo_num_threads.restype = C.c_int
+        L.hpo_jacobi_eigen.argtypes = [_dp, _sz, _sz, _dp, _dp]
+        L.hpo_jacobi_eigen.restype = C.c_int
+        L.hpo_jacobi_eigen_planted.argtypes = [_dp, _sz, _sz, _dp, _dp, C.c_int]
+        L.hpo_jacobi_eigen_planted.restype = C.c_int
+        L.hpo_jacobi_eigen_maxpivot.argtypes = [_dp, _sz, _dp, _dp]
+        L.hpo_canonical_orthogonalize.argtypes = [_dp, _sz, C.c_double, _dp]
+        L.hpo_canonical_orthogonalize.restype = C.c_size_t
+        L.hpo_solve_gen_eigen.argtypes = [_dp, _dp, _sz, C.c_double, _dp, _dp]
+        L.hpo_solve_gen_eigen.restype = C.c_size_t
         _lib = L
     return _lib
 
@@ -349,3 +358,40 @@ def count_mf4_mac(s, a, b):
 
 def count_overlap_entry(ai, ci, aj, cj):
     return lib().hpo_count_overlap_entry(ai, _c(ci), aj, _c(cj))
+
+
+# ---------------------------------------------------------------- eigensolvers
+class NotConverged(Exception):
+    pass
+
+
+def jacobi_eigen(a, n, max_iter, mistake=0):
+    """linalg.rs:299-362: (evals (n, 2) in diagonal order, evecs (n*n, 2), row i = eigenvector i)."""
+    ev = np.zeros((n, 2))
+    vec = np.zeros((n * n, 2))
+    if lib().hpo_jacobi_eigen_planted(_c(a), n, max_iter, ev, vec, mistake):
+        raise NotConverged("Jacobi eigenvalue solver did not converge within max iterations")
+    return ev, vec
+
+
+def jacobi_eigen_maxpivot(a, n):
+    """ortho.rs:61-152: (evals (n, 2) in diagonal order, evecs (n*n, 2), column j = eigenvector j)."""
+    ev = np.zeros((max(n, 1), 2))
+    vec = np.zeros((max(n * n, 1), 2))
+    lib().hpo_jacobi_eigen_maxpivot(_c(a), n, ev, vec)
+    return ev[:n], vec[:n * n]
+
+
+def canonical_orthogonalize(s, n, threshold):
+    """ortho.rs:158-193: X (n * n_orth, 2) row-major, and n_orth."""
+    x = np.zeros((max(n * n, 1), 2))
+    m = lib().hpo_canonical_orthogonalize(_c(s), n, threshold, x)
+    return x[:n * m], m
+
+
+def solve_gen_eigen(f, s, n, threshold):
+    """ortho.rs:201-248: (eigenvalues (m, 2) ascending, C (n * m, 2), m)."""
+    ev = np.zeros((max(n, 1), 2))
+    c = np.zeros((max(n * n, 1), 2))
+    m = lib().hpo_solve_gen_eigen(_c(f), _c(s), n, threshold, ev, c)
+    return ev[:m], c[:n * m], m
