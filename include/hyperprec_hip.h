@@ -43,7 +43,8 @@ enum hp_status {
   HP_ENOTSPD = 2,  /* Cholesky breakdown (reference: CholeskyError / SolveError) */
   HP_EDEVICE = 3,  /* HIP runtime error or no usable gfx950 device */
   HP_EINVAL = 4,   /* invalid argument (null pointer, unknown mode) */
-  HP_ENOMEM = 5    /* device allocation failed */
+  HP_ENOMEM = 5,   /* device allocation failed */
+  HP_ENOCONV = 6   /* iteration limit reached (reference: jacobi_eigen's Err, linalg.rs:360) */
 };
 
 /* Message describing the last failure on the calling thread ("" if none). */
@@ -197,6 +198,36 @@ void hp_batch_graph_destroy(hp_batch* batch);
  * in dd without screening.  centers: n x 3 (x, y, z), alpha: n, s_dd: n x n dd (AoS). */
 int hp_overlap_s(const double* centers, const double* alpha, size_t n, double* s_dd);
 int hp_dev_overlap_s(const double* centers, const double* alpha, size_t n, double* s_hi, double* s_lo, void* stream);
+
+/* ------------------------------------------------------------------ eigensolvers */
+
+/* Symmetric dd eigendecomposition, replaces jacobi_eigen (crates/quad/src/linalg.rs:299-362):
+ * a n x n dd (AoS, symmetric); evals n dd in diagonal order of the original indices;
+ * evecs n x n dd, row i = eigenvector i.  Stops when the dd Frobenius norm of the strict
+ * upper triangle is below 1e-30, tested before each of at most max_iter sweeps;
+ * HP_ENOCONV (the reference's Err) if that never happens.  The sweeps rotate n/2 disjoint
+ * pairs at a time (round-robin order), so the results agree with the reference's cyclic
+ * order to dd rounding, not bit for bit. */
+int hp_jacobi_eigen(const double* a, size_t n, size_t max_iter, double* evals, double* evecs);
+/* the same on device planes; stop_rule 0: the Frobenius test above (HP_ENOCONV on
+ * failure), 1: max |a_pq| < 1e-60 (the solver crate's test, ortho.rs:84), at most
+ * max_sweeps sweeps, no error; vec_rows 1: row i = eigenvector i, 0: column i.  sweeps
+ * (optional) receives the number of sweeps run. */
+int hp_dev_jacobi_eigen(const double* a_hi, const double* a_lo, size_t n, size_t max_sweeps, int stop_rule,
+                        double* evals_hi, double* evals_lo, double* vecs_hi, double* vecs_lo, int vec_rows,
+                        size_t* sweeps, void* stream);
+/* canonical_orthogonalize (crates/solver/src/ortho.rs:158-193): diagonalise S (stop rule 1),
+ * keep eigenvalues above threshold in ascending order, X = U diag(1/sqrt(lambda)).
+ * x_dd: capacity n x n dd, receives n x n_orth (row-major). */
+int hp_canonical_orthogonalize(const double* s_dd, size_t n, double threshold, double* x_dd, size_t* n_orth);
+/* solve_gen_eigen (ortho.rs:201-248): F C = S C eps.  X from S, F' = X^T F X (symmetrised),
+ * F' = C' diag(eps) C'^T, C = X C'.  evals_dd: capacity n, receives n_basis ascending;
+ * c_dd: capacity n x n, receives n x n_basis (column k = eigenvector k). */
+int hp_solve_gen_eigen(const double* f_dd, const double* s_dd, size_t n, double threshold, double* evals_dd,
+                       double* c_dd, size_t* n_basis);
+int hp_dev_solve_gen_eigen(const double* f_hi, const double* f_lo, const double* s_hi, const double* s_lo, size_t n,
+                           double threshold, double* evals_hi, double* evals_lo, double* c_hi, double* c_lo,
+                           size_t* n_basis, size_t* sweeps, void* stream);
 
 /* ------------------------------------------------------------------ adjoints */
 

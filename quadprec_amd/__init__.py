@@ -21,7 +21,7 @@ import numpy as np
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "libhyperprec_hip.so")
 
-HP_OK, HP_ESHAPE, HP_ENOTSPD, HP_EDEVICE, HP_EINVAL, HP_ENOMEM = range(6)
+HP_OK, HP_ESHAPE, HP_ENOTSPD, HP_EDEVICE, HP_EINVAL, HP_ENOMEM, HP_ENOCONV = range(7)
 HP_GEMM_PLAIN = 0
 HP_GEMM_SPLIT_I8 = 2
 
@@ -118,6 +118,13 @@ def lib():
     L.hp_batch_graph_launch.argtypes = [_vp, _vp]
     L.hp_batch_graph_destroy.argtypes = [_vp]
     L.hp_batch_graph_destroy.restype = None
+    L.hp_jacobi_eigen.argtypes = [_dp, _sz, _sz, _dp, _dp]
+    L.hp_dev_jacobi_eigen.argtypes = [_vp, _vp, _sz, _sz, C.c_int, _vp, _vp, _vp, _vp, C.c_int,
+                                      C.POINTER(C.c_size_t), _vp]
+    L.hp_canonical_orthogonalize.argtypes = [_dp, _sz, C.c_double, _dp, C.POINTER(C.c_size_t)]
+    L.hp_solve_gen_eigen.argtypes = [_dp, _dp, _sz, C.c_double, _dp, _dp, C.POINTER(C.c_size_t)]
+    L.hp_dev_solve_gen_eigen.argtypes = [_vp, _vp, _vp, _vp, _sz, C.c_double, _vp, _vp, _vp, _vp,
+                                         C.POINTER(C.c_size_t), C.POINTER(C.c_size_t), _vp]
     L.hp_overlap_s.argtypes = [_dp, _dp, _sz, _dp]
     L.hp_dev_overlap_s.argtypes = [_vp, _vp, _sz, _vp, _vp, _vp]
     L.hp_dev_overlap_forward.argtypes = [_vp, _vp, _vp, _vp, _sz, C.c_int, C.POINTER(_vp), C.POINTER(AdjointInfo),
@@ -299,6 +306,45 @@ def set_split_factor_panel(nb):
 
 
 # ---------------------------------------------------------------- overlap
+# ---------------------------------------------------------------- eigensolvers
+class NotConverged(HyperprecError):
+    """jacobi_eigen's Err("Jacobi eigenvalue solver did not converge ...") (linalg.rs:360)."""
+
+
+def jacobi_eigen(a, n, max_iter):
+    """Symmetric dd eigendecomposition (crates/quad/src/linalg.rs:299-362): (eigenvalues (n, 2)
+    in diagonal order, eigenvectors (n*n, 2) with row i = eigenvector i)."""
+    a = _arr(a, 2, n * n, "a")
+    ev = np.zeros((max(n, 1), 2))
+    vec = np.zeros((max(n * n, 1), 2))
+    rc = lib().hp_jacobi_eigen(a, n, max_iter, ev, vec)
+    if rc == HP_ENOCONV:
+        raise NotConverged(lib().hp_last_error().decode(errors="replace"))
+    _check(rc, "jacobi_eigen")
+    return ev[:n], vec[:n * n]
+
+
+def canonical_orthogonalize(s, n, threshold):
+    """Canonical orthogonalization (crates/solver/src/ortho.rs:158-193): (X (n*n_orth, 2), n_orth)."""
+    s = _arr(s, 2, n * n, "s")
+    x = np.zeros((max(n * n, 1), 2))
+    m = C.c_size_t(0)
+    _check(lib().hp_canonical_orthogonalize(s, n, threshold, x, C.byref(m)), "canonical_orthogonalize")
+    return x[:n * m.value], m.value
+
+
+def solve_gen_eigen(f, s, n, threshold):
+    """F C = S C eps (crates/solver/src/ortho.rs:201-248): (eigenvalues (m, 2) ascending,
+    C (n*m, 2) with column k = eigenvector k, m)."""
+    f = _arr(f, 2, n * n, "f")
+    s = _arr(s, 2, n * n, "s")
+    ev = np.zeros((max(n, 1), 2))
+    c = np.zeros((max(n * n, 1), 2))
+    m = C.c_size_t(0)
+    _check(lib().hp_solve_gen_eigen(f, s, n, threshold, ev, c, C.byref(m)), "solve_gen_eigen")
+    return ev[:m.value], c[:n * m.value], m.value
+
+
 def overlap_s(centers, alpha):
     """Normalised s-type Gaussian overlap matrix in dd, (n*n, 2)."""
     alpha = np.ascontiguousarray(alpha, dtype=np.float64)

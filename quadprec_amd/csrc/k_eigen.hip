@@ -1,0 +1,559 @@
+// k_eigen.hip -- symmetric dd eigensolver, canonical orthogonalization and the
+// generalized eigenproblem F C = S C eps on the GPU (the consumer of the overlap matrix:
+// crates/quad/src/linalg.rs:299-362, crates/solver/src/ortho.rs:61-248).
+//
+// The reference rotates one (p, q) pair at a time (cyclic sweeps in linalg.rs, the
+// largest |a_pq| in ortho.rs), a serial chain of O(n) updates.  Here every step of a sweep
+// applies n/2 disjoint rotations at once (round-robin order, every pair once per sweep):
+//   * k_jac_rot: per pair, the reference's rotation (tau = (a_qq - a_pp) / 2 a_pq,
+//     t = sign(tau) / (|tau| + sqrt(1 + tau^2)), c = 1 / sqrt(1 + t^2), s = t c) and the
+//     new diagonal a_pp - t a_pq, a_qq + t a_pq (linalg.rs:323-333);
+//   * k_jac_apply: every 2 x 2 block (pairs I <= J) becomes R_I^T X R_J (the diagonal
+//     blocks take the new diagonal and exact zeros) and is written with its mirror, so A
+//     stays exactly symmetric; the eigenvector matrix takes V R.
+// The matrix is kept in "pair layout": the two members of pair i sit at positions 2i and
+// 2i+1, so each step reads adjacent columns, and the apply kernel writes its results
+// straight into the next step's layout (the circle rotation sigma).  Odd n is padded by a
+// zero row and column, whose rotations are the identity.
+//
+// Rotations differ in order from the reference, so the results agree to dd rounding, not
+// bit for bit: eigenvalues to a few units of 2^-104 |A|, eigenvectors up to sign (and up
+// to rotations inside a degenerate eigenspace).
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <numeric>
+#include <vector>
+
+#include "hp_arith.hpp"
+#include "hp_internal.hpp"
+
+namespace hp {
+namespace {
+
+// position of an element after one round-robin step (circle method: the member at
+// position 0 stays, the others move one place around the circle t1 .. t_{m-1}, b_{m-1} .. b0)
+__host__ __device__ __forceinline__ long rr_sigma(long pos, long m) {
+  if (m == 1 || pos == 0) return pos;
+  const long i = pos >> 1;
+  if ((pos & 1) == 0) return i <= m - 2 ? 2 * i + 2 : 2 * m - 1;
+  return i == 0 ? 2 : 2 * i - 1;
+}
+
+struct Rot {
+  double ch, cl, sh, sl;  // c, s
+  double ph, pl, qh, ql;  // new a_pp, a_qq
+  int active;             // 0: identity (a_pq is zero or negligible), block left as is
+  int pad;
+};
+
+__global__ void k_jac_rot(const double* __restrict__ ah, const double* __restrict__ al, long N2, Rot* __restrict__ rot) {
+  const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  const long m = N2 / 2;
+  if (i >= m) return;
+  const long p = 2 * i, q = 2 * i + 1;
+  const dd app = mkdd(ah[p * N2 + p], al[p * N2 + p]);
+  const dd aqq = mkdd(ah[q * N2 + q], al[q * N2 + q]);
+  const dd apq = mkdd(ah[p * N2 + q], al[p * N2 + q]);
+  Rot r;
+  r.active = 0;
+  r.pad = 0;
+  dd c = mkdd(1.0), s = mkdd(0.0), np = app, nq = aqq;
+  if (apq.hi != 0.0) {
+    const dd one = mkdd(1.0);
+    const dd tau = dd_div(dd_sub(aqq, app), dd_mul(mkdd(2.0), apq));
+    // |tau| > 1e150: a_pq is below 1e-150 of the diagonal gap (tau^2 would overflow); such a
+    // rotation is the identity to dd precision, so the pair is left alone
+    if (__builtin_fabs(tau.hi) <= 1e150) {
+      const dd tau_abs = dd_abs(tau);
+      const dd t_denom = dd_add(tau_abs, dd_sqrt(dd_add(one, dd_mul(tau, tau))));
+      const dd t = tau.hi >= 0.0 ? dd_div(one, t_denom) : dd_div(dd_neg(one), t_denom);
+      c = dd_div(one, dd_sqrt(dd_add(one, dd_mul(t, t))));
+      s = dd_mul(t, c);
+      np = dd_sub(app, dd_mul(t, apq));
+      nq = dd_add(aqq, dd_mul(t, apq));
+      r.active = 1;
+    }
+  }
+  r.ch = c.hi; r.cl = c.lo; r.sh = s.hi; r.sl = s.lo;
+  r.ph = np.hi; r.pl = np.lo; r.qh = nq.hi; r.ql = nq.lo;
+  rot[i] = r;
+}
+
+// z = 0: A (pair layout) -> B (next layout), one thread per 2 x 2 block (I <= J);
+// z = 1: V -> W, one thread per row r and pair i
+__global__ __launch_bounds__(64) void k_jac_apply(const double* __restrict__ ah, const double* __restrict__ al,
+                                                  double* __restrict__ bh, double* __restrict__ bl,
+                                                  const double* __restrict__ vh, const double* __restrict__ vl,
+                                                  double* __restrict__ wh, double* __restrict__ wl, long N2,
+                                                  const Rot* __restrict__ rot) {
+  const long m = N2 / 2;
+  const long x = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (blockIdx.z == 1) {
+    const long r = blockIdx.y, i = x;
+    if (r >= N2 || i >= m) return;
+    const Rot R = rot[i];
+    const dd c = mkdd(R.ch, R.cl), s = mkdd(R.sh, R.sl);
+    const dd v0 = mkdd(vh[r * N2 + 2 * i], vl[r * N2 + 2 * i]);
+    const dd v1 = mkdd(vh[r * N2 + 2 * i + 1], vl[r * N2 + 2 * i + 1]);
+    const dd w0 = dd_sub(dd_mul(c, v0), dd_mul(s, v1));
+    const dd w1 = dd_add(dd_mul(s, v0), dd_mul(c, v1));
+    const long o0 = r * N2 + rr_sigma(2 * i, m), o1 = r * N2 + rr_sigma(2 * i + 1, m);
+    wh[o0] = w0.hi; wl[o0] = w0.lo;
+    wh[o1] = w1.hi; wl[o1] = w1.lo;
+    return;
+  }
+  const long I = blockIdx.y, J = x;
+  if (I >= m || J >= m || J < I) return;
+  const long p0 = 2 * I, p1 = p0 + 1, q0 = 2 * J, q1 = q0 + 1;
+  dd x00 = mkdd(ah[p0 * N2 + q0], al[p0 * N2 + q0]), x01 = mkdd(ah[p0 * N2 + q1], al[p0 * N2 + q1]);
+  dd x10 = mkdd(ah[p1 * N2 + q0], al[p1 * N2 + q0]), x11 = mkdd(ah[p1 * N2 + q1], al[p1 * N2 + q1]);
+  const Rot RI = rot[I];
+  const long np0 = rr_sigma(p0, m), np1 = rr_sigma(p1, m), nq0 = rr_sigma(q0, m), nq1 = rr_sigma(q1, m);
+  if (I == J) {
+    if (RI.active) {
+      x00 = mkdd(RI.ph, RI.pl);
+      x11 = mkdd(RI.qh, RI.ql);
+      x01 = mkdd(0.0);
+      x10 = mkdd(0.0);
+    }
+    bh[np0 * N2 + np0] = x00.hi; bl[np0 * N2 + np0] = x00.lo;
+    bh[np0 * N2 + np1] = x01.hi; bl[np0 * N2 + np1] = x01.lo;
+    bh[np1 * N2 + np0] = x10.hi; bl[np1 * N2 + np0] = x10.lo;
+    bh[np1 * N2 + np1] = x11.hi; bl[np1 * N2 + np1] = x11.lo;
+    return;
+  }
+  const Rot RJ = rot[J];
+  const dd cI = mkdd(RI.ch, RI.cl), sI = mkdd(RI.sh, RI.sl), cJ = mkdd(RJ.ch, RJ.cl), sJ = mkdd(RJ.sh, RJ.sl);
+  // columns q0, q1 by rotation J, then rows p0, p1 by rotation I
+  const dd c00 = dd_sub(dd_mul(cJ, x00), dd_mul(sJ, x01)), c01 = dd_add(dd_mul(sJ, x00), dd_mul(cJ, x01));
+  const dd c10 = dd_sub(dd_mul(cJ, x10), dd_mul(sJ, x11)), c11 = dd_add(dd_mul(sJ, x10), dd_mul(cJ, x11));
+  const dd y00 = dd_sub(dd_mul(cI, c00), dd_mul(sI, c10)), y10 = dd_add(dd_mul(sI, c00), dd_mul(cI, c10));
+  const dd y01 = dd_sub(dd_mul(cI, c01), dd_mul(sI, c11)), y11 = dd_add(dd_mul(sI, c01), dd_mul(cI, c11));
+  bh[np0 * N2 + nq0] = y00.hi; bl[np0 * N2 + nq0] = y00.lo;
+  bh[np0 * N2 + nq1] = y01.hi; bl[np0 * N2 + nq1] = y01.lo;
+  bh[np1 * N2 + nq0] = y10.hi; bl[np1 * N2 + nq0] = y10.lo;
+  bh[np1 * N2 + nq1] = y11.hi; bl[np1 * N2 + nq1] = y11.lo;
+  bh[nq0 * N2 + np0] = y00.hi; bl[nq0 * N2 + np0] = y00.lo;
+  bh[nq1 * N2 + np0] = y01.hi; bl[nq1 * N2 + np0] = y01.lo;
+  bh[nq0 * N2 + np1] = y10.hi; bl[nq0 * N2 + np1] = y10.lo;
+  bh[nq1 * N2 + np1] = y11.hi; bl[nq1 * N2 + np1] = y11.lo;
+}
+
+// off-diagonal size of A: sum of hi^2 over a < b (for the Frobenius stop test) and
+// max |hi + lo| (for the largest-element stop test)
+__global__ __launch_bounds__(256) void k_jac_off(const double* __restrict__ ah, const double* __restrict__ al, long N2,
+                                                 double* __restrict__ sum, unsigned long long* __restrict__ mx) {
+  __shared__ double ss[4];
+  __shared__ double sm[4];
+  const long a = blockIdx.y;
+  const long b = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  double v2 = 0.0, va = 0.0;
+  if (b > a && b < N2) {
+    const double h = ah[a * N2 + b];
+    v2 = h * h;
+    va = __builtin_fabs(h + al[a * N2 + b]);
+  }
+  for (int o = 32; o > 0; o >>= 1) {
+    v2 += __shfl_xor(v2, o, 64);
+    va = fmax(va, __shfl_xor(va, o, 64));
+  }
+  const int w = threadIdx.x >> 6;
+  if ((threadIdx.x & 63) == 0) {
+    ss[w] = v2;
+    sm[w] = va;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    const double t = ss[0] + ss[1] + ss[2] + ss[3];
+    const double u = fmax(fmax(sm[0], sm[1]), fmax(sm[2], sm[3]));
+    if (t > 0.0) atomicAdd(sum, t);
+    if (u > 0.0) atomicMax(mx, (unsigned long long)__double_as_longlong(u));  // >= 0: ordered as integers
+  }
+}
+
+// B (N2 x N2, zero padded) <- A (n x n); V <- identity
+__global__ void k_jac_init(const double* __restrict__ ah, const double* __restrict__ al, long n, long N2,
+                           double* __restrict__ bh, double* __restrict__ bl, double* __restrict__ vh,
+                           double* __restrict__ vl) {
+  const long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= N2 * N2) return;
+  const long r = idx / N2, c = idx % N2;
+  const bool in = r < n && c < n;
+  bh[idx] = in ? ah[r * n + c] : 0.0;
+  bl[idx] = in ? al[r * n + c] : 0.0;
+  vh[idx] = r == c ? 1.0 : 0.0;
+  vl[idx] = 0.0;
+}
+
+// outputs from the pair layout: evals[o] = A[pos(o)][pos(o)]; vec_rows: evecs[o][r] =
+// V[r][pos(o)] (row o = eigenvector o), else evecs[r][o] (column o)
+__global__ void k_jac_extract(const double* __restrict__ ah, const double* __restrict__ al,
+                              const double* __restrict__ vh, const double* __restrict__ vl, long n, long N2,
+                              const long* __restrict__ pos, double* __restrict__ eh, double* __restrict__ el,
+                              double* __restrict__ xh, double* __restrict__ xl, int vec_rows) {
+  const long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= n * n) return;
+  const long o = idx / n, r = idx % n;
+  const long po = pos[o];
+  const long out = vec_rows ? o * n + r : r * n + o;
+  xh[out] = vh[r * N2 + po];
+  xl[out] = vl[r * N2 + po];
+  if (r == 0) {
+    eh[o] = ah[po * N2 + po];
+    el[o] = al[po * N2 + po];
+  }
+}
+
+unsigned blocks(long n, int t = 256) { return (unsigned)((n + t - 1) / t); }
+
+// X[i][k] = V[i][sel[k]] * recip(sqrt(lambda_k)) (ortho.rs:185-190), V's columns the eigenvectors
+__global__ void k_ortho_x(const double* __restrict__ vh, const double* __restrict__ vl, long n,
+                          const long* __restrict__ sel, const double* __restrict__ lam, long m, double* __restrict__ xh,
+                          double* __restrict__ xl) {
+  const long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= n * m) return;
+  const long i = idx / m, k = idx % m;
+  const dd inv_sqrt = dd_recip(dd_sqrt(mkdd(lam[2 * k], lam[2 * k + 1])));
+  const dd v = mkdd(vh[i * n + sel[k]], vl[i * n + sel[k]]);
+  const dd x = dd_mul(v, inv_sqrt);
+  xh[idx] = x.hi;
+  xl[idx] = x.lo;
+}
+// F'[i][j] = F'[j][i] = (F'[i][j] + F'[j][i]) * 0.5 for i < j (ortho.rs:218-225)
+__global__ void k_symmetrize(double* __restrict__ h, double* __restrict__ l, long m) {
+  const long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= m * m) return;
+  const long i = idx / m, j = idx % m;
+  if (j <= i) return;
+  const dd avg = dd_mul(dd_add(mkdd(h[i * m + j], l[i * m + j]), mkdd(h[j * m + i], l[j * m + i])), mkdd(0.5));
+  h[i * m + j] = avg.hi; l[i * m + j] = avg.lo;
+  h[j * m + i] = avg.hi; l[j * m + i] = avg.lo;
+}
+// out[i][nk] = in[i][order[nk]] (m x m)
+__global__ void k_gather_cols(const double* __restrict__ ih, const double* __restrict__ il, long m,
+                              const long* __restrict__ order, double* __restrict__ oh, double* __restrict__ ol) {
+  const long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= m * m) return;
+  const long i = idx / m, k = idx % m;
+  oh[idx] = ih[i * m + order[k]];
+  ol[idx] = il[i * m + order[k]];
+}
+
+}  // namespace
+
+// Symmetric eigendecomposition of the n x n dd matrix (device planes, row-major).
+//   stop_frobenius: stop when sqrt(sum_{a<b} a_ab^2) < 1e-30, tested before each sweep,
+//                   with at most max_sweeps - 1 sweeps run (linalg.rs's loop); HP_ENOCONV
+//                   otherwise;
+//   else:           sweep until max |a_ab| < 1e-60 (ortho.rs's test), at most max_sweeps.
+// evals: n dd (diagonal order of the original indices); evecs: n x n dd, row o =
+// eigenvector o when vec_rows, else column o.
+int jacobi_eigen_dev(const double* a_h, const double* a_l, long n, bool stop_frobenius, long max_sweeps,
+                     double* e_h, double* e_l, double* x_h, double* x_l, bool vec_rows, long* sweeps_run,
+                     hipStream_t s) {
+  if (sweeps_run) *sweeps_run = 0;
+  if (n <= 0) return HP_OK;
+  const long N2 = n + (n & 1), m = N2 / 2;
+  if (N2 > 65534) {
+    set_error("jacobi_eigen: n = %ld too large (at most 65534)", n);
+    return HP_EINVAL;
+  }
+  const size_t nn = (size_t)N2 * N2;
+  DevBuf A0, A1, V0, V1, rot, red, posb;
+  int rc;
+  if ((rc = A0.alloc(2 * nn * sizeof(double))) || (rc = A1.alloc(2 * nn * sizeof(double))) ||
+      (rc = V0.alloc(2 * nn * sizeof(double))) || (rc = V1.alloc(2 * nn * sizeof(double))) ||
+      (rc = rot.alloc(m * sizeof(Rot))) || (rc = red.alloc(16)) || (rc = posb.alloc(N2 * sizeof(long))))
+    return rc;
+  double *ah = A0.d(), *al = ah + nn, *bh = A1.d(), *bl = bh + nn;
+  double *vh = V0.d(), *vl = vh + nn, *wh = V1.d(), *wl = wh + nn;
+  hipLaunchKernelGGL(k_jac_init, dim3(blocks((long)nn)), dim3(256), 0, s, a_h, a_l, n, N2, ah, al, vh, vl);
+  HP_LAUNCH_CHECK();
+  std::vector<long> layout(N2);  // position -> original index
+  std::iota(layout.begin(), layout.end(), 0L);
+  std::vector<long> next(N2);
+  auto off = [&](double* fro, double* mxv) -> int {
+    HP_HIP(hipMemsetAsync(red.p, 0, 16, s));
+    double* sum = red.d();
+    unsigned long long* mx = reinterpret_cast<unsigned long long*>(red.d() + 1);
+    hipLaunchKernelGGL(k_jac_off, dim3(blocks(N2), (unsigned)N2), dim3(256), 0, s, ah, al, N2, sum, mx);
+    HP_LAUNCH_CHECK();
+    double host[2];
+    HP_HIP(hipMemcpyAsync(host, red.p, 16, hipMemcpyDeviceToHost, s));
+    HP_HIP(hipStreamSynchronize(s));
+    *fro = std::sqrt(host[0]);
+    unsigned long long bits;
+    std::memcpy(&bits, &host[1], 8);
+    double mv;
+    std::memcpy(&mv, &bits, 8);
+    *mxv = mv;
+    return HP_OK;
+  };
+  auto sweep = [&]() -> int {
+    for (long step = 0; step < N2 - 1; ++step) {
+      hipLaunchKernelGGL(k_jac_rot, dim3(blocks(m, 64)), dim3(64), 0, s, ah, al, N2, static_cast<Rot*>(rot.p));
+      HP_LAUNCH_CHECK();
+      hipLaunchKernelGGL(k_jac_apply, dim3(blocks(m, 64), (unsigned)N2, 2), dim3(64), 0, s, ah, al, bh, bl, vh, vl,
+                         wh, wl, N2, static_cast<const Rot*>(rot.p));
+      HP_LAUNCH_CHECK();
+      std::swap(ah, bh);
+      std::swap(al, bl);
+      std::swap(vh, wh);
+      std::swap(vl, wl);
+      for (long p = 0; p < N2; ++p) next[rr_sigma(p, m)] = layout[p];
+      layout.swap(next);
+    }
+    return HP_OK;
+  };
+  bool converged = false;
+  long run = 0;
+  for (long it = 0; it < max_sweeps; ++it) {
+    double fro = 0.0, mxv = 0.0;
+    if ((rc = off(&fro, &mxv))) return rc;
+    if (stop_frobenius ? fro < 1e-30 : mxv < 1e-60) {
+      converged = true;
+      break;
+    }
+    if ((rc = sweep())) return rc;
+    ++run;
+  }
+  if (!converged && !stop_frobenius) {
+    double fro = 0.0, mxv = 0.0;
+    if ((rc = off(&fro, &mxv))) return rc;
+    converged = mxv < 1e-60;
+  }
+  if (sweeps_run) *sweeps_run = run;
+  if (!converged && stop_frobenius) {
+    set_error("Jacobi eigenvalue solver did not converge within max iterations");
+    return HP_ENOCONV;
+  }
+  std::vector<long> pos(N2);
+  for (long p = 0; p < N2; ++p) pos[layout[p]] = p;
+  HP_HIP(hipMemcpyAsync(posb.p, pos.data(), N2 * sizeof(long), hipMemcpyHostToDevice, s));
+  hipLaunchKernelGGL(k_jac_extract, dim3(blocks(n * n)), dim3(256), 0, s, ah, al, vh, vl, n, N2,
+                     static_cast<const long*>(posb.p), e_h, e_l, x_h, x_l, vec_rows ? 1 : 0);
+  HP_LAUNCH_CHECK();
+  HP_HIP(hipStreamSynchronize(s));
+  return HP_OK;
+}
+
+
+// The solver crate's eigensolver stops below max |a_pq| = 1e-60 or after 100 n^2 rotations
+// (ortho.rs:73-85); the parallel sweeps stop at the same test, capped at 200 sweeps (each
+// sweep is n(n-1)/2 rotations).
+constexpr long ORTHO_MAX_SWEEPS = 200;
+
+// canonical_orthogonalize (ortho.rs:158-193) of the n x n dd matrix (device planes):
+// X (n x n_orth) is allocated in xbuf (planes hi | lo).  eval_host (optional) receives the
+// n eigenvalues (dd, diagonal order).
+int canonical_ortho_dev(const double* s_h, const double* s_l, long n, double threshold, DevBuf& xbuf, long* m_out,
+                        long* sweeps, hipStream_t s) {
+  DevBuf E, V, selb, lamb;
+  int rc;
+  if ((rc = E.alloc(2 * n * sizeof(double))) || (rc = V.alloc(2 * (size_t)n * n * sizeof(double)))) return rc;
+  const size_t nn = (size_t)n * n;
+  if ((rc = jacobi_eigen_dev(s_h, s_l, n, false, ORTHO_MAX_SWEEPS, E.d(), E.d() + n, V.d(), V.d() + nn, false, sweeps,
+                             s)))
+    return rc;
+  std::vector<double> ev(2 * n);
+  HP_HIP(hipMemcpyAsync(ev.data(), E.p, 2 * n * sizeof(double), hipMemcpyDeviceToHost, s));
+  HP_HIP(hipStreamSynchronize(s));
+  std::vector<long> sel;
+  for (long i = 0; i < n; ++i)
+    if (ev[i] + ev[n + i] > threshold) sel.push_back(i);
+  std::stable_sort(sel.begin(), sel.end(), [&](long a, long b) { return ev[a] + ev[n + a] < ev[b] + ev[n + b]; });
+  const long m = (long)sel.size();
+  *m_out = m;
+  if ((rc = xbuf.alloc(2 * (size_t)n * std::max<long>(m, 1) * sizeof(double)))) return rc;
+  if (m == 0) return HP_OK;
+  std::vector<double> lam(2 * m);
+  for (long k = 0; k < m; ++k) {
+    lam[2 * k] = ev[sel[k]];
+    lam[2 * k + 1] = ev[n + sel[k]];
+  }
+  if ((rc = selb.alloc(m * sizeof(long))) || (rc = lamb.alloc(2 * m * sizeof(double)))) return rc;
+  HP_HIP(hipMemcpyAsync(selb.p, sel.data(), m * sizeof(long), hipMemcpyHostToDevice, s));
+  HP_HIP(hipMemcpyAsync(lamb.p, lam.data(), 2 * m * sizeof(double), hipMemcpyHostToDevice, s));
+  hipLaunchKernelGGL(k_ortho_x, dim3(blocks(n * m)), dim3(256), 0, s, V.d(), V.d() + nn, n,
+                     static_cast<const long*>(selb.p), lamb.d(), m, xbuf.d(), xbuf.d() + (size_t)n * m);
+  HP_LAUNCH_CHECK();
+  HP_HIP(hipStreamSynchronize(s));
+  return HP_OK;
+}
+
+// solve_gen_eigen (ortho.rs:201-248) on device planes: eigenvalues ascending (m dd into
+// ev_h / ev_l), C = X C' (n x m into c_h / c_l; capacity n x n)
+int solve_gen_eigen_dev(const double* f_h, const double* f_l, const double* s_h, const double* s_l, long n,
+                        double threshold, double* ev_h, double* ev_l, double* c_h, double* c_l, long* m_out,
+                        long* sweeps, hipStream_t s) {
+  DevBuf X;
+  long m = 0;
+  long sw[2] = {0, 0};
+  int rc;
+  if ((rc = canonical_ortho_dev(s_h, s_l, n, threshold, X, &m, &sw[0], s))) return rc;
+  *m_out = m;
+  if (m == 0) return HP_OK;
+  const size_t nm = (size_t)n * m, mm = (size_t)m * m;
+  DevBuf T, Fp, E, Vp, Cp, ordb;
+  if ((rc = T.alloc(2 * nm * sizeof(double))) || (rc = Fp.alloc(2 * mm * sizeof(double))) ||
+      (rc = E.alloc(2 * m * sizeof(double))) || (rc = Vp.alloc(2 * mm * sizeof(double))) ||
+      (rc = Cp.alloc(2 * mm * sizeof(double))) || (rc = ordb.alloc(m * sizeof(long))))
+    return rc;
+  const double *xh = X.d(), *xl = X.d() + nm;
+  // tmp = F X (n x m), F' = X^T tmp (m x m): sums in k order as the reference's gemm_rect /
+  // gemm_atb (whose skipped zero terms would add exact zeros)
+  if ((rc = launch_gemm_dd_plain(0, f_h, f_l, xh, xl, T.d(), T.d() + nm, (size_t)n, (size_t)m, (size_t)n, s)))
+    return rc;
+  if ((rc = launch_gemm_dd_plain(1, xh, xl, T.d(), T.d() + nm, Fp.d(), Fp.d() + mm, (size_t)m, (size_t)m, (size_t)n,
+                                 s)))
+    return rc;
+  hipLaunchKernelGGL(k_symmetrize, dim3(blocks((long)mm)), dim3(256), 0, s, Fp.d(), Fp.d() + mm, m);
+  HP_LAUNCH_CHECK();
+  if ((rc = jacobi_eigen_dev(Fp.d(), Fp.d() + mm, m, false, ORTHO_MAX_SWEEPS, E.d(), E.d() + m, Vp.d(), Vp.d() + mm,
+                             false, &sw[1], s)))
+    return rc;
+  if (sweeps) *sweeps = sw[0] + sw[1];
+  std::vector<double> ev(2 * m);
+  HP_HIP(hipMemcpyAsync(ev.data(), E.p, 2 * m * sizeof(double), hipMemcpyDeviceToHost, s));
+  HP_HIP(hipStreamSynchronize(s));
+  std::vector<long> order(m);
+  std::iota(order.begin(), order.end(), 0L);
+  std::stable_sort(order.begin(), order.end(), [&](long a, long b) { return ev[a] + ev[m + a] < ev[b] + ev[m + b]; });
+  std::vector<double> sorted(2 * m);
+  for (long k = 0; k < m; ++k) {
+    sorted[k] = ev[order[k]];
+    sorted[m + k] = ev[m + order[k]];
+  }
+  HP_HIP(hipMemcpyAsync(ev_h, sorted.data(), m * sizeof(double), hipMemcpyHostToDevice, s));
+  HP_HIP(hipMemcpyAsync(ev_l, sorted.data() + m, m * sizeof(double), hipMemcpyHostToDevice, s));
+  HP_HIP(hipMemcpyAsync(ordb.p, order.data(), m * sizeof(long), hipMemcpyHostToDevice, s));
+  hipLaunchKernelGGL(k_gather_cols, dim3(blocks((long)mm)), dim3(256), 0, s, Vp.d(), Vp.d() + mm, m,
+                     static_cast<const long*>(ordb.p), Cp.d(), Cp.d() + mm);
+  HP_LAUNCH_CHECK();
+  if ((rc = launch_gemm_dd_plain(0, xh, xl, Cp.d(), Cp.d() + mm, c_h, c_l, (size_t)n, (size_t)m, (size_t)m, s)))
+    return rc;
+  HP_HIP(hipStreamSynchronize(s));
+  return HP_OK;
+}
+
+}  // namespace hp
+
+using namespace hp;
+
+extern "C" {
+
+int hp_jacobi_eigen(const double* a, size_t n, size_t max_iter, double* evals, double* evecs) {
+  if (n == 0) return HP_OK;
+  if (!a || !evals || !evecs) {
+    set_error("null pointer argument");
+    return HP_EINVAL;
+  }
+  hipStream_t s = hipStreamPerThread;
+  const size_t nn = n * n;
+  DevBuf A, E, V;
+  int rc;
+  if ((rc = A.alloc(2 * nn * sizeof(double))) || (rc = E.alloc(2 * n * sizeof(double))) ||
+      (rc = V.alloc(2 * nn * sizeof(double))))
+    return rc;
+  if ((rc = upload_planes(a, nn, 2, A.d(), s))) return rc;
+  if ((rc = jacobi_eigen_dev(A.d(), A.d() + nn, (long)n, true, (long)max_iter, E.d(), E.d() + n, V.d(), V.d() + nn,
+                             true, nullptr, s)))
+    return rc;
+  if ((rc = download_planes(E.d(), n, 2, evals, s))) return rc;
+  return download_planes(V.d(), nn, 2, evecs, s);
+}
+
+int hp_dev_jacobi_eigen(const double* a_hi, const double* a_lo, size_t n, size_t max_sweeps, int stop_rule,
+                        double* evals_hi, double* evals_lo, double* vecs_hi, double* vecs_lo, int vec_rows,
+                        size_t* sweeps, void* stream) {
+  long run = 0;
+  int rc = jacobi_eigen_dev(a_hi, a_lo, (long)n, stop_rule == 0, (long)max_sweeps, evals_hi, evals_lo, vecs_hi, vecs_lo,
+                            vec_rows != 0, &run, (hipStream_t)stream);
+  if (sweeps) *sweeps = (size_t)run;
+  return rc;
+}
+
+int hp_canonical_orthogonalize(const double* s_dd, size_t n, double threshold, double* x_dd, size_t* n_orth) {
+  if (!n_orth) {
+    set_error("null pointer argument");
+    return HP_EINVAL;
+  }
+  *n_orth = 0;
+  if (n == 0) return HP_OK;
+  if (!s_dd || !x_dd) {
+    set_error("null pointer argument");
+    return HP_EINVAL;
+  }
+  hipStream_t s = hipStreamPerThread;
+  const size_t nn = n * n;
+  DevBuf S, X;
+  int rc;
+  if ((rc = S.alloc(2 * nn * sizeof(double)))) return rc;
+  if ((rc = upload_planes(s_dd, nn, 2, S.d(), s))) return rc;
+  long m = 0;
+  if ((rc = canonical_ortho_dev(S.d(), S.d() + nn, (long)n, threshold, X, &m, nullptr, s))) return rc;
+  *n_orth = (size_t)m;
+  if (m == 0) return HP_OK;
+  return download_planes(X.d(), n * (size_t)m, 2, x_dd, s);
+}
+
+int hp_solve_gen_eigen(const double* f_dd, const double* s_dd, size_t n, double threshold, double* evals_dd,
+                       double* c_dd, size_t* n_basis) {
+  if (!n_basis) {
+    set_error("null pointer argument");
+    return HP_EINVAL;
+  }
+  *n_basis = 0;
+  if (n == 0) return HP_OK;
+  if (!f_dd || !s_dd || !evals_dd || !c_dd) {
+    set_error("null pointer argument");
+    return HP_EINVAL;
+  }
+  hipStream_t s = hipStreamPerThread;
+  const size_t nn = n * n;
+  DevBuf F, S, E, C;
+  int rc;
+  if ((rc = F.alloc(2 * nn * sizeof(double))) || (rc = S.alloc(2 * nn * sizeof(double))) ||
+      (rc = E.alloc(2 * n * sizeof(double))) || (rc = C.alloc(2 * nn * sizeof(double))))
+    return rc;
+  if ((rc = upload_planes(f_dd, nn, 2, F.d(), s)) || (rc = upload_planes(s_dd, nn, 2, S.d(), s))) return rc;
+  long m = 0;
+  if ((rc = solve_gen_eigen_dev(F.d(), F.d() + nn, S.d(), S.d() + nn, (long)n, threshold, E.d(), E.d() + n, C.d(),
+                                C.d() + nn, &m, nullptr, s)))
+    return rc;
+  *n_basis = (size_t)m;
+  if (m == 0) return HP_OK;
+  // planes of m and n x m values packed at the front of each buffer
+  std::vector<double> eh(2 * m);
+  HP_HIP(hipMemcpyAsync(eh.data(), E.d(), m * sizeof(double), hipMemcpyDeviceToHost, s));
+  HP_HIP(hipMemcpyAsync(eh.data() + m, E.d() + n, m * sizeof(double), hipMemcpyDeviceToHost, s));
+  std::vector<double> ch(2 * n * (size_t)m);
+  HP_HIP(hipMemcpyAsync(ch.data(), C.d(), n * (size_t)m * sizeof(double), hipMemcpyDeviceToHost, s));
+  HP_HIP(hipMemcpyAsync(ch.data() + n * (size_t)m, C.d() + nn, n * (size_t)m * sizeof(double), hipMemcpyDeviceToHost,
+                        s));
+  HP_HIP(hipStreamSynchronize(s));
+  for (long k = 0; k < m; ++k) {
+    evals_dd[2 * k] = eh[k];
+    evals_dd[2 * k + 1] = eh[m + k];
+  }
+  const size_t nm = n * (size_t)m;
+  for (size_t i = 0; i < nm; ++i) {
+    c_dd[2 * i] = ch[i];
+    c_dd[2 * i + 1] = ch[nm + i];
+  }
+  return HP_OK;
+}
+
+int hp_dev_solve_gen_eigen(const double* f_hi, const double* f_lo, const double* s_hi, const double* s_lo, size_t n,
+                           double threshold, double* evals_hi, double* evals_lo, double* c_hi, double* c_lo,
+                           size_t* n_basis, size_t* sweeps, void* stream) {
+  long m = 0, sw = 0;
+  int rc = solve_gen_eigen_dev(f_hi, f_lo, s_hi, s_lo, (long)n, threshold, evals_hi, evals_lo, c_hi, c_lo, &m, &sw,
+                               (hipStream_t)stream);
+  if (n_basis) *n_basis = (size_t)m;
+  if (sweeps) *sweeps = (size_t)sw;
+  return rc;
+}
+
+}  // extern "C"
