@@ -189,6 +189,73 @@ def bench_mf4(qp, torch, dev, steps, cpu):
     return out
 
 
+def bench_eigen(qp, torch, dev, cpu):
+    """The consumer of S: solve_gen_eigen F C = S C eps (crates/solver/src/ortho.rs:201-248)
+    with S the config-0 overlap matrix (lattice, seed 0, exponents in [0.01, 10]) and F a
+    seeded symmetric dd matrix, threshold 1e-8; both Jacobi eigenproblems on the GPU
+    (parallel round-robin sweeps to max |a_pq| < 1e-60).  Accuracy against the oracle:
+    tests/test_gpu_eigen.py."""
+    import ctypes as C
+    import numpy as np
+    L = qp.lib()
+
+    def inputs(n):
+        c, a = qp.gen_overlap_basis(0, n, 0.01, 10.0)
+        s_dd = qp.overlap_s(c, a)
+        rng = np.random.default_rng(1)
+        b = rng.standard_normal((n, n)) / np.sqrt(n)
+        f_dd = np.zeros((n * n, 2))
+        f_dd[:, 0] = ((b + b.T) / 2).ravel()
+        return f_dd, s_dd
+
+    def gpu(n, reps=2):
+        f_dd, s_dd = inputs(n)
+        fd, sd = qp.dev.to_planes(f_dd, 2, dev), qp.dev.to_planes(s_dd, 2, dev)
+        ev = torch.zeros((2, n), dtype=torch.float64, device=dev)
+        cc = torch.zeros((2, n * n), dtype=torch.float64, device=dev)
+        m, sw = C.c_size_t(0), C.c_size_t(0)
+        best = None
+        for _ in range(reps + 1):  # the first run also builds the sweep graphs' allocations
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            rc = L.hp_dev_solve_gen_eigen(fd[0].data_ptr(), fd[1].data_ptr(), sd[0].data_ptr(), sd[1].data_ptr(), n,
+                                          1e-8, ev[0].data_ptr(), ev[1].data_ptr(), cc[0].data_ptr(),
+                                          cc[1].data_ptr(), C.byref(m), C.byref(sw), None)
+            torch.cuda.synchronize()
+            dt = (time.perf_counter() - t0) * 1e3
+            if rc:
+                raise RuntimeError(f"hp_dev_solve_gen_eigen: status {rc}")
+            best = dt if best is None else min(best, dt)
+        return best, m.value, sw.value, (f_dd, s_dd)
+
+    n = EIG_N
+    ms, m, sweeps, _ = gpu(n)
+    n2 = n + (n & 1)
+    steps = sweeps * (n2 - 1)
+    out = {"workload": f"solve_gen_eigen, S = overlap N={n} (lattice, seed 0, alpha in [0.01, 10]), F seeded "
+                       "symmetric dd, threshold 1e-8",
+           "ms": round(ms, 2), "n_basis": m, "jacobi_sweeps": sweeps,
+           "roofline": {"bound": "hbm", "kernel": "k_jac_apply",
+                        "achieved": round(64.0 * n2 * n2 * steps / (ms * 1e-3) / 1e9, 1), "peak": 8000.0,
+                        "unit": "GB/s", "frac": round(64.0 * n2 * n2 * steps / (ms * 1e-3) / 8e12, 4),
+                        "note": "64 N^2 bytes per round-robin step (A and V, read and written, dd) over the whole "
+                                "call's time; the steps are latency bound (scattered mirror writes, a dependent "
+                                "rotation kernel per step)"}}
+    if cpu:
+        sys.path.insert(0, os.path.join(ROOT, "tests"))
+        import oracle_lib as O
+        nc = EIG_CPU_N
+        ms_g, _, _, (f_dd, s_dd) = gpu(nc)
+        t0 = time.perf_counter()
+        O.solve_gen_eigen(f_dd, s_dd, nc, 1e-8)
+        t = time.perf_counter() - t0
+        out["cpu_baseline"] = {"value": round(t * 1e3, 1), "unit": "ms", "cores": 1, "kind": "port",
+                               "sample": f"the full solve at N={nc} (max-pivot Jacobi as ortho.rs:61-152, "
+                                         f"oracle C++), against {ms_g:.1f} ms on the GPU at the same N"}
+    return out
+
+
+EIG_N, EIG_CPU_N = 512, 192
 ADJ_N, ADJ_SEED, ADJ_B_SEED = 2048, 4, 44
 
 
@@ -267,6 +334,7 @@ def main():
     ap.add_argument("--no-batch", action="store_true", help="skip the batched uneven solves")
     ap.add_argument("--no-mf4", action="store_true", help="skip the MF4 1024^3 gemm")
     ap.add_argument("--no-adjoint", action="store_true", help="skip the N=2048 adjoint")
+    ap.add_argument("--no-eigen", action="store_true", help="skip the N=512 generalized eigensolve")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only copy MIB MiB device-to-device three times (a plain copy kernel of known size, "
                          "to calibrate rocprofv3's FETCH_SIZE / WRITE_SIZE counters) and exit")
@@ -360,6 +428,8 @@ def main():
         out["mf4_gemm_1024"] = bench_mf4(qp, torch, dev, args.steps, not args.no_cpu)
     if not args.no_adjoint:
         out["adjoint_n2048"] = bench_adjoint(qp, torch, dev, not args.no_cpu)
+    if not args.no_eigen:
+        out["gen_eigen_n512"] = bench_eigen(qp, torch, dev, not args.no_cpu)
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(a, b)
     print(json.dumps(out), flush=True)

@@ -290,17 +290,49 @@ int jacobi_eigen_dev(const double* a_h, const double* a_l, long n, bool stop_fro
     *mxv = mv;
     return HP_OK;
   };
-  auto sweep = [&]() -> int {
+  // A sweep is 2 (N2 - 1) launches; each is captured once into a graph per buffer parity
+  // (N2 - 1 is odd, so consecutive sweeps start from opposite buffers) and replayed.
+  struct Graphs {
+    hipStream_t cs = nullptr;
+    hipGraphExec_t exec[2] = {nullptr, nullptr};
+    ~Graphs() {
+      for (auto& e : exec)
+        if (e) (void)hipGraphExecDestroy(e);
+      if (cs) (void)hipStreamDestroy(cs);
+    }
+  } gr;
+  auto capture = [&](int parity) -> int {
+    if (!gr.cs) HP_HIP(hipStreamCreateWithFlags(&gr.cs, hipStreamNonBlocking));
+    double *xh = ah, *xl = al, *yh = bh, *yl = bl, *uh = vh, *ul = vl, *zh = wh, *zl = wl;
+    HP_HIP(hipStreamBeginCapture(gr.cs, hipStreamCaptureModeThreadLocal));
     for (long step = 0; step < N2 - 1; ++step) {
-      hipLaunchKernelGGL(k_jac_rot, dim3(blocks(m, 64)), dim3(64), 0, s, ah, al, N2, static_cast<Rot*>(rot.p));
-      HP_LAUNCH_CHECK();
-      hipLaunchKernelGGL(k_jac_apply, dim3(blocks(m, 64), (unsigned)N2, 2), dim3(64), 0, s, ah, al, bh, bl, vh, vl,
-                         wh, wl, N2, static_cast<const Rot*>(rot.p));
-      HP_LAUNCH_CHECK();
-      std::swap(ah, bh);
-      std::swap(al, bl);
-      std::swap(vh, wh);
-      std::swap(vl, wl);
+      hipLaunchKernelGGL(k_jac_rot, dim3(blocks(m, 64)), dim3(64), 0, gr.cs, xh, xl, N2, static_cast<Rot*>(rot.p));
+      hipLaunchKernelGGL(k_jac_apply, dim3(blocks(m, 64), (unsigned)N2, 2), dim3(64), 0, gr.cs, xh, xl, yh, yl, uh,
+                         ul, zh, zl, N2, static_cast<const Rot*>(rot.p));
+      std::swap(xh, yh);
+      std::swap(xl, yl);
+      std::swap(uh, zh);
+      std::swap(ul, zl);
+    }
+    hipGraph_t g = nullptr;
+    HP_HIP(hipStreamEndCapture(gr.cs, &g));
+    hipError_t e = hipGraphInstantiate(&gr.exec[parity], g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    HP_HIP(e);
+    return HP_OK;
+  };
+  int parity = 0;
+  auto sweep = [&]() -> int {
+    if (!gr.exec[parity]) {
+      if (int rc2 = capture(parity)) return rc2;
+    }
+    HP_HIP(hipGraphLaunch(gr.exec[parity], s));
+    std::swap(ah, bh);  // N2 - 1 (odd) swaps
+    std::swap(al, bl);
+    std::swap(vh, wh);
+    std::swap(vl, wl);
+    parity ^= 1;
+    for (long step = 0; step < N2 - 1; ++step) {
       for (long p = 0; p < N2; ++p) next[rr_sigma(p, m)] = layout[p];
       layout.swap(next);
     }
