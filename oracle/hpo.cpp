@@ -785,6 +785,209 @@ void hpo_assemble_overlap_s(const double* centers, const double* alpha, const do
     }
 }
 
+// ---- full assemble_overlap_mixed: contracted s/p/d/f shells (crates/overlap/src) ----
+// A basis is flattened: shell s has centre centers[3s..3s+2], angular momentum l[s] in
+// 0..3 and primitives prim_off[s] .. prim_off[s+1]-1 (alpha[], coeff[]).
+struct ShellView {
+  const double* c;
+  int l;
+  long p0, p1;
+};
+static double shell_alpha_min(const ShellView& s, const double* alpha) {  // basis.rs:40-43
+  double m = INFINITY;
+  for (long p = s.p0; p < s.p1; ++p) m = std::fmin(m, alpha[p]);
+  return m;
+}
+static double shell_alpha_max(const ShellView& s, const double* alpha) {  // basis.rs:45-48
+  double m = 0.0;
+  for (long p = s.p0; p < s.p1; ++p) m = std::fmax(m, alpha[p]);
+  return m;
+}
+// screen_shell_pair: screen.rs:57-101 (0 F64, 1 F128, 2 Skip)
+static int screen_pair(const ShellView& a, const ShellView& b, const double* alpha, const double* prm) {
+  double dx = a.c[0] - b.c[0], dy = a.c[1] - b.c[1], dz = a.c[2] - b.c[2];
+  double r2 = dx * dx + dy * dy + dz * dz;
+  double amin_a = shell_alpha_min(a, alpha), amin_b = shell_alpha_min(b, alpha);
+  double alpha_sum = amin_a + amin_b;
+  double gamma = amin_a * amin_b / alpha_sum;
+  double exponent = gamma * r2;
+  if (exponent > -std::log(prm[2])) return 2;
+  bool ad = amin_a < prm[0], bd = amin_b < prm[0];
+  if (!ad && !bd) return 0;
+  double amax_a = shell_alpha_max(a, alpha), amax_b = shell_alpha_max(b, alpha);
+  double ratio = std::fmax(amax_a / amin_b, amax_b / amin_a);
+  double r = std::sqrt(r2);
+  if (ratio > prm[1] && r < prm[3]) return 1;
+  if (ad && bd && r < prm[3]) return 1;
+  return 0;
+}
+// cart_components: integral.rs:220-231
+static int cart_components(int l, int (*out)[3]) {
+  int n = 0;
+  for (int lx = l; lx >= 0; --lx)
+    for (int ly = l - lx; ly >= 0; --ly) {
+      out[n][0] = lx;
+      out[n][1] = ly;
+      out[n][2] = l - lx - ly;
+      ++n;
+    }
+  return n;
+}
+// obara_saika_1d_f64: integral.rs:158-185 (s[i][j], nb = lb + 1)
+static void os1d_f64(int la, int lb, double pa, double pb, double gamma, double* s) {
+  const int nb = lb + 1;
+  const double oo2g = 0.5 / gamma;
+  for (int k = 0; k < (la + 1) * nb; ++k) s[k] = 0.0;
+  s[0] = 1.0;
+  for (int i = 0; i < la; ++i) s[(i + 1) * nb] = pa * s[i * nb] + oo2g * (double)i * (i > 0 ? s[(i - 1) * nb] : 0.0);
+  for (int j = 0; j < lb; ++j)
+    for (int i = 0; i <= la; ++i) {
+      double tim1 = oo2g * (double)i * (i > 0 ? s[(i - 1) * nb + j] : 0.0);
+      double tjm1 = oo2g * (double)j * (j > 0 ? s[i * nb + j - 1] : 0.0);
+      s[i * nb + j + 1] = pb * s[i * nb + j] + tim1 + tjm1;
+    }
+}
+// obara_saika_1d_f128: integral.rs:187-216
+static void os1d_dd(int la, int lb, D pa, D pb, D gamma, D* s) {
+  const int nb = lb + 1;
+  const D oo2g = div(D{0.5, 0.0}, gamma);
+  for (int k = 0; k < (la + 1) * nb; ++k) s[k] = D{0.0, 0.0};
+  s[0] = D{1.0, 0.0};
+  for (int i = 0; i < la; ++i) {
+    D fi = dd_from((double)i);
+    D prev = i > 0 ? s[(i - 1) * nb] : D{0.0, 0.0};
+    s[(i + 1) * nb] = add(mul(pa, s[i * nb]), mul(mul(oo2g, fi), prev));
+  }
+  for (int j = 0; j < lb; ++j) {
+    D fj = dd_from((double)j);
+    for (int i = 0; i <= la; ++i) {
+      D fi = dd_from((double)i);
+      D im1 = i > 0 ? s[(i - 1) * nb + j] : D{0.0, 0.0};
+      D jm1 = j > 0 ? s[i * nb + j - 1] : D{0.0, 0.0};
+      s[i * nb + j + 1] = add(add(mul(pb, s[i * nb + j]), mul(mul(oo2g, fi), im1)), mul(mul(oo2g, fj), jm1));
+    }
+  }
+}
+// overlap_shell_pair: integral.rs:65-142 and 233-346; block row-major ni x nj (f64)
+static void shell_pair_block(const ShellView& A, const ShellView& B, const double* alpha, const double* coeff,
+                             int tier, double* block) {
+  int ca[10][3], cb[10][3];
+  const int ni = cart_components(A.l, ca), nj = cart_components(B.l, cb);
+  for (int k = 0; k < ni * nj; ++k) block[k] = 0.0;
+  if (tier == 2) return;
+  const int nb = B.l + 1;
+  if (tier == 0) {
+    if (A.l == 0 && B.l == 0) {  // integral.rs:99-110
+      double sum = 0.0;
+      for (long p = A.p0; p < A.p1; ++p)
+        for (long q = B.p0; q < B.p1; ++q)
+          sum += coeff[p] * coeff[q] * hpo_overlap_primitive_f64(alpha[p], A.c, alpha[q], B.c);
+      block[0] = sum;
+      return;
+    }
+    for (long p = A.p0; p < A.p1; ++p)  // integral.rs:233-282
+      for (long q = B.p0; q < B.p1; ++q) {
+        double ai = alpha[p], aj = alpha[q];
+        double gamma = ai + aj;
+        double mu = ai * aj / gamma;
+        double dx = A.c[0] - B.c[0], dy = A.c[1] - B.c[1], dz = A.c[2] - B.c[2];
+        double r2 = dx * dx + dy * dy + dz * dz;
+        double sq = std::sqrt(PI_F64 / gamma);
+        double prefactor = (sq * sq) * sq * std::exp(-mu * r2);
+        double pa[3], pb[3];
+        for (int k = 0; k < 3; ++k) {
+          double pk = (ai * A.c[k] + aj * B.c[k]) / gamma;
+          pa[k] = pk - A.c[k];
+          pb[k] = pk - B.c[k];
+        }
+        double sx[16], sy[16], sz[16];
+        os1d_f64(A.l, B.l, pa[0], pb[0], gamma, sx);
+        os1d_f64(A.l, B.l, pa[1], pb[1], gamma, sy);
+        os1d_f64(A.l, B.l, pa[2], pb[2], gamma, sz);
+        for (int u = 0; u < ni; ++u)
+          for (int v = 0; v < nj; ++v) {
+            double val = sx[ca[u][0] * nb + cb[v][0]] * sy[ca[u][1] * nb + cb[v][1]] * sz[ca[u][2] * nb + cb[v][2]] *
+                         prefactor * coeff[p] * coeff[q];
+            block[u * nj + v] += val;
+          }
+      }
+    return;
+  }
+  if (A.l == 0 && B.l == 0) {  // integral.rs:119-136
+    D sum{0.0, 0.0};
+    for (long p = A.p0; p < A.p1; ++p)
+      for (long q = B.p0; q < B.p1; ++q) {
+        D sp = from_c(hpo_overlap_primitive_f128(alpha[p], A.c, alpha[q], B.c));
+        sum = add(sum, mul(mul(dd_from(coeff[p]), dd_from(coeff[q])), sp));
+      }
+    block[0] = sum.hi + sum.lo;
+    return;
+  }
+  D acc[100];
+  for (int k = 0; k < ni * nj; ++k) acc[k] = D{0.0, 0.0};
+  for (long p = A.p0; p < A.p1; ++p)  // integral.rs:284-346
+    for (long q = B.p0; q < B.p1; ++q) {
+      D ai = dd_from(alpha[p]), aj = dd_from(alpha[q]);
+      D gamma = add(ai, aj);
+      D mu = div(mul(ai, aj), gamma);
+      D dx = dd_from(A.c[0] - B.c[0]), dy = dd_from(A.c[1] - B.c[1]), dz = dd_from(A.c[2] - B.c[2]);
+      D r2 = add(add(mul(dx, dx), mul(dy, dy)), mul(dz, dz));
+      D ratio = div(dd_from(PI_F64), gamma);
+      D prefactor = mul(ratio, dsqrt(ratio));
+      D nmr2 = mul(sub(D{0.0, 0.0}, mu), r2);
+      D exp_val = mul(dd_from(std::exp(nmr2.hi)), add(D{1.0, 0.0}, dd_from(nmr2.lo)));
+      D total_pre = mul(prefactor, exp_val);
+      D pa[3], pb[3];
+      for (int k = 0; k < 3; ++k) {
+        D cik = dd_from(A.c[k]), cjk = dd_from(B.c[k]);
+        D pk = div(add(mul(ai, cik), mul(aj, cjk)), gamma);
+        pa[k] = sub(pk, cik);
+        pb[k] = sub(pk, cjk);
+      }
+      D sx[16], sy[16], sz[16];
+      os1d_dd(A.l, B.l, pa[0], pb[0], gamma, sx);
+      os1d_dd(A.l, B.l, pa[1], pb[1], gamma, sy);
+      os1d_dd(A.l, B.l, pa[2], pb[2], gamma, sz);
+      D cf = mul(mul(dd_from(coeff[p]), dd_from(coeff[q])), total_pre);
+      for (int u = 0; u < ni; ++u)
+        for (int v = 0; v < nj; ++v) {
+          D val = mul(mul(mul(sx[ca[u][0] * nb + cb[v][0]], sy[ca[u][1] * nb + cb[v][1]]), sz[ca[u][2] * nb + cb[v][2]]),
+                      cf);
+          acc[u * nj + v] = add(acc[u * nj + v], val);
+        }
+    }
+  for (int k = 0; k < ni * nj; ++k) block[k] = acc[k].hi + acc[k].lo;
+}
+// assemble_overlap_mixed: assemble.rs:35-85.  Returns n_ao; s: n_ao x n_ao (capacity given
+// by the caller: sum of (l+1)(l+2)/2); counts: f64, f128, skipped shell pairs.
+size_t hpo_assemble_overlap_mixed(const double* centers, const int* l, const long* prim_off, const double* alpha,
+                                  const double* coeff, size_t n_shells, const double* params4, double* s,
+                                  size_t* counts) {
+  std::vector<long> off(n_shells + 1, 0);
+  for (size_t i = 0; i < n_shells; ++i) off[i + 1] = off[i] + (l[i] + 1) * (l[i] + 2) / 2;
+  const size_t n_ao = (size_t)off[n_shells];
+  std::memset(s, 0, sizeof(double) * n_ao * n_ao);
+  counts[0] = counts[1] = counts[2] = 0;
+  double block[100];
+  for (size_t i = 0; i < n_shells; ++i)
+    for (size_t j = 0; j <= i; ++j) {
+      ShellView A{centers + 3 * i, l[i], prim_off[i], prim_off[i + 1]};
+      ShellView B{centers + 3 * j, l[j], prim_off[j], prim_off[j + 1]};
+      int tier = screen_pair(A, B, alpha, params4);
+      shell_pair_block(A, B, alpha, coeff, tier, block);
+      counts[tier] += 1;
+      if (tier == 2) continue;
+      const int ni = (l[i] + 1) * (l[i] + 2) / 2, nj = (l[j] + 1) * (l[j] + 2) / 2;
+      for (int u = 0; u < ni; ++u)
+        for (int v = 0; v < nj; ++v) {
+          double val = block[u * nj + v];
+          s[(off[i] + u) * n_ao + (off[j] + v)] = val;
+          if (i != j) s[(off[j] + v) * n_ao + (off[i] + u)] = val;
+        }
+    }
+  return n_ao;
+}
+
 // Normalised s-type overlap built from the reference's dd operations.  The reference's
 // dd exp (dd.rs:119) wraps the exponent field for k < -1022, so for arguments below
 // -708 the entry is returned as 0 (the true value is below 1e-307, under a dd ulp of

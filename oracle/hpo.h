@@ -134,6 +134,15 @@ int hpo_overlap_adjoint_dd_planted(const double* centers, const double* alpha, c
 int hpo_overlap_fd_f256(const double* centers, const double* alpha, const double* b_dd, size_t n, const long* coords,
                         size_t ncoords, double h, double* fd_ld, double* fd_q);
 
+/* assemble_overlap_mixed (crates/overlap/src/assemble.rs:35-85) over contracted s/p/d/f
+ * shells: shell s has centre centers[3s..3s+2], l[s] in 0..3 and primitives
+ * prim_off[s] .. prim_off[s+1]-1 of alpha / coeff; params4 = ScreenParams {diffuse,
+ * ratio, eps_skip, safe_distance}.  s: n_ao x n_ao f64; counts: F64, F128, Skip shell
+ * pairs.  Returns n_ao = sum (l+1)(l+2)/2. */
+size_t hpo_assemble_overlap_mixed(const double* centers, const int* l, const long* prim_off, const double* alpha,
+                                  const double* coeff, size_t n_shells, const double* params4, double* s,
+                                  size_t* counts);
+
 /* ---- eigensolvers ---- */
 /* jacobi_eigen (crates/quad/src/linalg.rs:299-362): cyclic Jacobi, max_iter sweeps.
  * evals: n dd in diagonal order; evecs: n x n dd, row i = eigenvector i.  Returns 1 if

@@ -101,6 +101,10 @@ def lib():
         L.hpo_count_overlap_entry.argtypes = [C.c_double, _dp, C.c_double, _dp]
         L.hpo_count_overlap_entry.restype = C.c_long
         L.hpo_num_threads.restype = C.c_int
+        L.hpo_assemble_overlap_mixed.argtypes = [_dp, np.ctypeslib.ndpointer(dtype=np.int32, flags="C_CONTIGUOUS"),
+                                                 np.ctypeslib.ndpointer(dtype=np.int64, flags="C_CONTIGUOUS"),
+                                                 _dp, _dp, _sz, _dp, _dp, C.POINTER(C.c_size_t)]
+        L.hpo_assemble_overlap_mixed.restype = C.c_size_t
         L.hpo_jacobi_eigen.argtypes = [_dp, _sz, _sz, _dp, _dp]
         L.hpo_jacobi_eigen.restype = C.c_int
         L.hpo_jacobi_eigen_planted.argtypes = [_dp, _sz, _sz, _dp, _dp, C.c_int]
@@ -395,3 +399,29 @@ def solve_gen_eigen(f, s, n, threshold):
     c = np.zeros((max(n * n, 1), 2))
     m = lib().hpo_solve_gen_eigen(_c(f), _c(s), n, threshold, ev, c)
     return ev[:m], c[:n * m], m
+
+
+# ---------------------------------------------------------------- contracted overlap
+DEFAULT_SCREEN = (0.1, 500.0, 1e-14, 12.0)  # ScreenParams::default() (screen.rs:38-46)
+
+
+def flatten_basis(shells):
+    """shells: list of (center (3,), l, [(alpha, coeff), ...]) -> flat arrays."""
+    centers = np.array([c for c, _, _ in shells], dtype=np.float64).ravel()
+    ls = np.array([l for _, l, _ in shells], dtype=np.int32)
+    off = np.zeros(len(shells) + 1, dtype=np.int64)
+    for i, (_, _, prims) in enumerate(shells):
+        off[i + 1] = off[i] + len(prims)
+    alpha = np.array([a for _, _, prims in shells for a, _ in prims], dtype=np.float64)
+    coeff = np.array([c for _, _, prims in shells for _, c in prims], dtype=np.float64)
+    return centers, ls, off, alpha, coeff
+
+
+def assemble_overlap_mixed(shells, params=DEFAULT_SCREEN):
+    """assemble.rs:35-85: (S (n_ao, n_ao) f64, (f64_pairs, f128_pairs, skipped_pairs))."""
+    centers, ls, off, alpha, coeff = flatten_basis(shells)
+    n_ao = int(sum((l + 1) * (l + 2) // 2 for l in ls))
+    s = np.zeros(max(n_ao * n_ao, 1))
+    counts = (C.c_size_t * 3)()
+    lib().hpo_assemble_overlap_mixed(centers, ls, off, alpha, coeff, len(shells), _c(params), s, counts)
+    return s[:n_ao * n_ao].reshape(n_ao, n_ao), tuple(counts)
