@@ -199,6 +199,19 @@ void hp_batch_graph_destroy(hp_batch* batch);
 int hp_overlap_s(const double* centers, const double* alpha, size_t n, double* s_dd);
 int hp_dev_overlap_s(const double* centers, const double* alpha, size_t n, double* s_hi, double* s_lo, void* stream);
 
+/* assemble_overlap_mixed (crates/overlap/src/assemble.rs:35-85) over contracted s/p/d/f
+ * shells: shell i has centre centers[3i..3i+2], angular momentum l[i] (0..3 = S, P, D, F,
+ * Cartesian components in cart_components order, integral.rs:220-231) and primitives
+ * prim_off[i] .. prim_off[i+1]-1 of alpha / coeff (prim_off: n_shells + 1 entries).
+ * params4 = ScreenParams {diffuse_threshold, ratio_threshold, eps_skip, safe_distance}
+ * (screen.rs:18-46).  s: n_ao x n_ao f64 (row-major), n_ao = sum (l+1)(l+2)/2 (HP_ESHAPE
+ * otherwise); counts: TierCounts {f64_pairs, f128_pairs, skipped_pairs}.  Each shell pair
+ * is screened and its block formed in f64 or dd (integral.rs:65-346) exactly as the
+ * reference; only libm exp may differ in its last bit. */
+int hp_assemble_overlap_mixed(const double* centers, const int* l, const long* prim_off, const double* alpha,
+                              const double* coeff, size_t n_shells, const double* params4, double* s, size_t n_ao,
+                              size_t* counts);
+
 /* ------------------------------------------------------------------ eigensolvers */
 
 /* Symmetric dd eigendecomposition, replaces jacobi_eigen (crates/quad/src/linalg.rs:299-362):
@@ -228,6 +241,30 @@ int hp_solve_gen_eigen(const double* f_dd, const double* s_dd, size_t n, double 
 int hp_dev_solve_gen_eigen(const double* f_hi, const double* f_lo, const double* s_hi, const double* s_lo, size_t n,
                            double threshold, double* evals_hi, double* evals_lo, double* c_hi, double* c_lo,
                            size_t* n_basis, size_t* sweeps, void* stream);
+
+/* ------------------------------------------------------------------ reductions */
+
+/* dd dot product, replaces dot (crates/quad/src/linalg.rs:7-18): below 1024 elements the
+ * serial sum from ZERO (linalg.rs:20-27); else the serial sums of 1024-element chunks folded
+ * left to right from ZERO, one of the association orders of the reference's rayon reduce.
+ * a, b: n dd (AoS); out_dd: 1 dd. */
+int hp_dot(const double* a, const double* b, size_t n, double* out_dd);
+int hp_dev_dot(const double* a_hi, const double* a_lo, const double* b_hi, const double* b_lo, size_t n,
+               double* out_hi, double* out_lo, void* stream);
+/* y = A x with A row-major rows x cols, replaces gemv (linalg.rs:29-40): y_i is the serial
+ * dot of row i with x. */
+int hp_gemv(const double* a, size_t rows, size_t cols, const double* x, double* y);
+int hp_dev_gemv(const double* a_hi, const double* a_lo, size_t rows, size_t cols, const double* x_hi,
+                const double* x_lo, double* y_hi, double* y_lo, void* stream);
+/* Neumaier compensated sums, replace neumaier_sum_f64 / neumaier_sum / par_neumaier_sum_f64 /
+ * par_neumaier_sum (crates/compensated-sum/src/lib.rs:6-66).  The par forms sum 4096-element
+ * chunks and then the partials, in order (the serial sum below 4096 elements); the plain
+ * forms run the whole sum in order on one GPU thread.  v: n f64 or n dd (AoS); out: one
+ * f64 or one dd. */
+int hp_neumaier_sum_f64(const double* v, size_t n, double* out);
+int hp_neumaier_sum(const double* v, size_t n, double* out_dd);
+int hp_par_neumaier_sum_f64(const double* v, size_t n, double* out);
+int hp_par_neumaier_sum(const double* v, size_t n, double* out_dd);
 
 /* ------------------------------------------------------------------ adjoints */
 

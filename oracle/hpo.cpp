@@ -294,6 +294,73 @@ double hpo_cond_estimate(const double* l, size_t n) {
   return r * r;
 }
 
+// ------------------------------------------------------------------ reductions
+// dot_serial: crates/quad/src/linalg.rs:20-27 (sum += a[i] * b[i] from ZERO)
+static D dot_serial(const double* a, const double* b, size_t n) {
+  D s{0.0, 0.0};
+  for (size_t i = 0; i < n; ++i) s = add(s, mul(ld(a, i), ld(b, i)));
+  return s;
+}
+// dot: linalg.rs:7-18.  Below 1024 elements the serial sum; else serial sums of 1024-element
+// chunks combined by rayon's reduce from ZERO, whose association tree is unspecified: fixed
+// here as the left fold ((ZERO + p0) + p1) + ..., one of the trees rayon can produce.
+hpo_dd hpo_dot(const double* a, const double* b, size_t n) {
+  const size_t CHUNK = 1024;
+  if (n < CHUNK) return to_c(dot_serial(a, b, n));
+  D acc{0.0, 0.0};
+  for (size_t c0 = 0; c0 < n; c0 += CHUNK)
+    acc = add(acc, dot_serial(a + 2 * c0, b + 2 * c0, std::min(CHUNK, n - c0)));
+  return to_c(acc);
+}
+// gemv: linalg.rs:29-40 (y_i = dot_serial(row_i, x))
+void hpo_gemv(const double* a, size_t rows, size_t cols, const double* x, double* y) {
+#pragma omp parallel for schedule(static)
+  for (long i = 0; i < (long)rows; ++i) st(y, (size_t)i, dot_serial(a + 2 * (size_t)i * cols, x, cols));
+}
+// neumaier_sum_f64: crates/compensated-sum/src/lib.rs:6-21
+double hpo_neumaier_sum_f64(const double* v, size_t n) {
+  double sum = 0.0, comp = 0.0;
+  for (size_t i = 0; i < n; ++i) {
+    double t = sum + v[i];
+    if (std::fabs(sum) >= std::fabs(v[i])) comp += (sum - t) + v[i];
+    else comp += (v[i] - t) + sum;
+    sum = t;
+  }
+  return sum + comp;
+}
+// neumaier_sum: lib.rs:24-39 (dd)
+hpo_dd hpo_neumaier_sum(const double* v, size_t n) {
+  D sum{0.0, 0.0}, comp{0.0, 0.0};
+  for (size_t i = 0; i < n; ++i) {
+    D x = ld(v, i);
+    D t = add(sum, x);
+    if (dabs(sum).hi >= dabs(x).hi) comp = add(comp, add(sub(sum, t), x));
+    else comp = add(comp, add(sub(x, t), sum));
+    sum = t;
+  }
+  return to_c(add(sum, comp));
+}
+// par_neumaier_sum_f64 / par_neumaier_sum: lib.rs:43-66 (4096-element chunk sums, in order,
+// then the Neumaier sum of the partials)
+double hpo_par_neumaier_sum_f64(const double* v, size_t n) {
+  const size_t CHUNK = 4096;
+  if (n < CHUNK) return hpo_neumaier_sum_f64(v, n);
+  std::vector<double> part;
+  for (size_t c0 = 0; c0 < n; c0 += CHUNK) part.push_back(hpo_neumaier_sum_f64(v + c0, std::min(CHUNK, n - c0)));
+  return hpo_neumaier_sum_f64(part.data(), part.size());
+}
+hpo_dd hpo_par_neumaier_sum(const double* v, size_t n) {
+  const size_t CHUNK = 4096;
+  if (n < CHUNK) return hpo_neumaier_sum(v, n);
+  std::vector<double> part;
+  for (size_t c0 = 0; c0 < n; c0 += CHUNK) {
+    hpo_dd p = hpo_neumaier_sum(v + 2 * c0, std::min(CHUNK, n - c0));
+    part.push_back(p.hi);
+    part.push_back(p.lo);
+  }
+  return hpo_neumaier_sum(part.data(), part.size() / 2);
+}
+
 // ------------------------------------------------------------------ eigensolvers
 // dd ordering of the reference (dd.rs:349-356): by hi, then by lo
 static inline bool dd_lt(D a, D b) { return a.hi < b.hi || (a.hi == b.hi && a.lo < b.lo); }

@@ -143,6 +143,18 @@ size_t hpo_assemble_overlap_mixed(const double* centers, const int* l, const lon
                                   const double* coeff, size_t n_shells, const double* params4, double* s,
                                   size_t* counts);
 
+/* ---- reductions ---- */
+/* dot (crates/quad/src/linalg.rs:7-27): serial below 1024 elements, else 1024-element
+ * chunk sums folded left to right (one of rayon's association orders) */
+hpo_dd hpo_dot(const double* a, const double* b, size_t n);
+/* gemv (linalg.rs:29-40): y_i = dot_serial(row_i, x) */
+void hpo_gemv(const double* a, size_t rows, size_t cols, const double* x, double* y);
+/* Neumaier compensated sums (crates/compensated-sum/src/lib.rs:6-66) */
+double hpo_neumaier_sum_f64(const double* v, size_t n);
+hpo_dd hpo_neumaier_sum(const double* v, size_t n);
+double hpo_par_neumaier_sum_f64(const double* v, size_t n);
+hpo_dd hpo_par_neumaier_sum(const double* v, size_t n);
+
 /* ---- eigensolvers ---- */
 /* jacobi_eigen (crates/quad/src/linalg.rs:299-362): cyclic Jacobi, max_iter sweeps.
  * evals: n dd in diagonal order; evecs: n x n dd, row i = eigenvector i.  Returns 1 if

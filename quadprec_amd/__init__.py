@@ -118,6 +118,9 @@ def lib():
     L.hp_batch_graph_launch.argtypes = [_vp, _vp]
     L.hp_batch_graph_destroy.argtypes = [_vp]
     L.hp_batch_graph_destroy.restype = None
+    L.hp_assemble_overlap_mixed.argtypes = [_dp, np.ctypeslib.ndpointer(dtype=np.int32, flags="C_CONTIGUOUS"),
+                                            np.ctypeslib.ndpointer(dtype=np.int64, flags="C_CONTIGUOUS"),
+                                            _dp, _dp, _sz, _dp, _dp, _sz, C.POINTER(C.c_size_t)]
     L.hp_jacobi_eigen.argtypes = [_dp, _sz, _sz, _dp, _dp]
     L.hp_dev_jacobi_eigen.argtypes = [_vp, _vp, _sz, _sz, C.c_int, _vp, _vp, _vp, _vp, C.c_int,
                                       C.POINTER(C.c_size_t), _vp]
@@ -125,6 +128,12 @@ def lib():
     L.hp_solve_gen_eigen.argtypes = [_dp, _dp, _sz, C.c_double, _dp, _dp, C.POINTER(C.c_size_t)]
     L.hp_dev_solve_gen_eigen.argtypes = [_vp, _vp, _vp, _vp, _sz, C.c_double, _vp, _vp, _vp, _vp,
                                          C.POINTER(C.c_size_t), C.POINTER(C.c_size_t), _vp]
+    L.hp_dot.argtypes = [_dp, _dp, _sz, _dp]
+    L.hp_dev_dot.argtypes = [_vp] * 4 + [_sz, _vp, _vp, _vp]
+    L.hp_gemv.argtypes = [_dp, _sz, _sz, _dp, _dp]
+    L.hp_dev_gemv.argtypes = [_vp, _vp, _sz, _sz, _vp, _vp, _vp, _vp, _vp]
+    for name in ("hp_neumaier_sum_f64", "hp_neumaier_sum", "hp_par_neumaier_sum_f64", "hp_par_neumaier_sum"):
+        getattr(L, name).argtypes = [_dp, _sz, _dp]
     L.hp_overlap_s.argtypes = [_dp, _dp, _sz, _dp]
     L.hp_dev_overlap_s.argtypes = [_vp, _vp, _sz, _vp, _vp, _vp]
     L.hp_dev_overlap_forward.argtypes = [_vp, _vp, _vp, _vp, _sz, C.c_int, C.POINTER(_vp), C.POINTER(AdjointInfo),
@@ -306,6 +315,38 @@ def set_split_factor_panel(nb):
 
 
 # ---------------------------------------------------------------- overlap
+# ---------------------------------------------------------------- contracted overlap
+SCREEN_DEFAULT = (0.1, 500.0, 1e-14, 12.0)  # ScreenParams::default() (crates/overlap/src/screen.rs:38-46)
+
+
+class TierCounts(tuple):
+    """TierCounts {f64_pairs, f128_pairs, skipped_pairs} (crates/overlap/src/assemble.rs:17-22)."""
+
+    f64_pairs = property(lambda self: self[0])
+    f128_pairs = property(lambda self: self[1])
+    skipped_pairs = property(lambda self: self[2])
+
+
+def assemble_overlap_mixed(shells, params=SCREEN_DEFAULT):
+    """assemble_overlap_mixed (crates/overlap/src/assemble.rs:35-85) on the GPU.  shells: list
+    of (center (3,), l in 0..3, [(alpha, coeff), ...]).  Returns (S (n_ao, n_ao) f64, TierCounts)."""
+    centers = np.ascontiguousarray([c for c, _, _ in shells], dtype=np.float64).ravel()
+    ls = np.ascontiguousarray([l for _, l, _ in shells], dtype=np.int32)
+    off = np.zeros(len(shells) + 1, dtype=np.int64)
+    for i, (_, _, prims) in enumerate(shells):
+        off[i + 1] = off[i] + len(prims)
+    alpha = np.ascontiguousarray([a for _, _, prims in shells for a, _ in prims], dtype=np.float64)
+    coeff = np.ascontiguousarray([c for _, _, prims in shells for _, c in prims], dtype=np.float64)
+    n_ao = int(sum((l + 1) * (l + 2) // 2 for l in ls))
+    s = np.zeros(max(n_ao * n_ao, 1))
+    counts = (C.c_size_t * 3)()
+    _check(lib().hp_assemble_overlap_mixed(centers, ls, off, alpha if alpha.size else np.zeros(1),
+                                           coeff if coeff.size else np.zeros(1), len(shells),
+                                           np.ascontiguousarray(params, dtype=np.float64), s, n_ao, counts),
+           "assemble_overlap_mixed")
+    return s[:n_ao * n_ao].reshape(n_ao, n_ao), TierCounts(tuple(counts))
+
+
 # ---------------------------------------------------------------- eigensolvers
 class NotConverged(HyperprecError):
     """jacobi_eigen's Err("Jacobi eigenvalue solver did not converge ...") (linalg.rs:360)."""
@@ -343,6 +384,45 @@ def solve_gen_eigen(f, s, n, threshold):
     m = C.c_size_t(0)
     _check(lib().hp_solve_gen_eigen(f, s, n, threshold, ev, c, C.byref(m)), "solve_gen_eigen")
     return ev[:m.value], c[:n * m.value], m.value
+
+
+# ---------------------------------------------------------------- reductions
+def dot(a, b):
+    """dd dot product (crates/quad/src/linalg.rs:7-18), as a (2,) array."""
+    a = np.ascontiguousarray(a, dtype=np.float64)
+    n = a.size // 2
+    b = _arr(b, 2, n, "b")
+    out = np.zeros(2)
+    _check(lib().hp_dot(a if n else np.zeros(2), b if n else np.zeros(2), n, out), "dot")
+    return out
+
+
+def gemv(a, rows, cols, x):
+    """y = A x, A row-major rows x cols (linalg.rs:29-40), as (rows, 2)."""
+    a = _arr(a, 2, rows * cols, "a")
+    x = _arr(x, 2, cols, "x")
+    y = np.zeros((max(rows, 1), 2))
+    _check(lib().hp_gemv(a if a.size else np.zeros(2), rows, cols, x if x.size else np.zeros(2), y), "gemv")
+    return y[:rows]
+
+
+def neumaier_sum_f64(values, parallel=False):
+    """Neumaier sum of f64 values (crates/compensated-sum/src/lib.rs:6-21; parallel: 43-53)."""
+    v = np.ascontiguousarray(values, dtype=np.float64).ravel()
+    out = np.zeros(1)
+    fn = lib().hp_par_neumaier_sum_f64 if parallel else lib().hp_neumaier_sum_f64
+    _check(fn(v if v.size else np.zeros(1), v.size, out), "neumaier_sum_f64")
+    return float(out[0])
+
+
+def neumaier_sum(values, parallel=False):
+    """Neumaier sum of dd values (lib.rs:24-39; parallel: 56-66), as a (2,) array."""
+    v = np.ascontiguousarray(values, dtype=np.float64)
+    n = v.size // 2
+    out = np.zeros(2)
+    fn = lib().hp_par_neumaier_sum if parallel else lib().hp_neumaier_sum
+    _check(fn(v if n else np.zeros(2), n, out), "neumaier_sum")
+    return out
 
 
 def overlap_s(centers, alpha):

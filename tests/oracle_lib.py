@@ -105,6 +105,15 @@ def lib():
                                                  np.ctypeslib.ndpointer(dtype=np.int64, flags="C_CONTIGUOUS"),
                                                  _dp, _dp, _sz, _dp, _dp, C.POINTER(C.c_size_t)]
         L.hpo_assemble_overlap_mixed.restype = C.c_size_t
+        L.hpo_dot.argtypes = [_dp, _dp, _sz]
+        L.hpo_dot.restype = DD
+        L.hpo_gemv.argtypes = [_dp, _sz, _sz, _dp, _dp]
+        for name in ("hpo_neumaier_sum_f64", "hpo_par_neumaier_sum_f64"):
+            getattr(L, name).argtypes = [_dp, _sz]
+            getattr(L, name).restype = C.c_double
+        for name in ("hpo_neumaier_sum", "hpo_par_neumaier_sum"):
+            getattr(L, name).argtypes = [_dp, _sz]
+            getattr(L, name).restype = DD
         L.hpo_jacobi_eigen.argtypes = [_dp, _sz, _sz, _dp, _dp]
         L.hpo_jacobi_eigen.restype = C.c_int
         L.hpo_jacobi_eigen_planted.argtypes = [_dp, _sz, _sz, _dp, _dp, C.c_int]
@@ -425,3 +434,26 @@ def assemble_overlap_mixed(shells, params=DEFAULT_SCREEN):
     counts = (C.c_size_t * 3)()
     lib().hpo_assemble_overlap_mixed(centers, ls, off, alpha, coeff, len(shells), _c(params), s, counts)
     return s[:n_ao * n_ao].reshape(n_ao, n_ao), tuple(counts)
+
+
+# ---------------------------------------------------------------- reductions
+def dot(a, b):
+    r = lib().hpo_dot(_c(a), _c(b), len(a))
+    return np.array([r.hi, r.lo])
+
+
+def gemv(a, rows, cols, x):
+    y = np.zeros((rows, 2))
+    lib().hpo_gemv(_c(a), rows, cols, _c(x), y)
+    return y
+
+
+def neumaier_sum_f64(v, parallel=False):
+    v = _c(v)
+    return (lib().hpo_par_neumaier_sum_f64 if parallel else lib().hpo_neumaier_sum_f64)(v, v.size)
+
+
+def neumaier_sum(v, parallel=False):
+    v = _c(v)
+    r = (lib().hpo_par_neumaier_sum if parallel else lib().hpo_neumaier_sum)(v, v.size // 2)
+    return np.array([r.hi, r.lo])
