@@ -266,6 +266,52 @@ int hp_neumaier_sum(const double* v, size_t n, double* out_dd);
 int hp_par_neumaier_sum_f64(const double* v, size_t n, double* out);
 int hp_par_neumaier_sum(const double* v, size_t n, double* out_dd);
 
+/* ------------------------------------------------------------------ text wire formats */
+
+/* Text arrays are Arrow-style: n strings concatenated (no separators, no NUL) in `text`,
+ * string i at text[offsets[i] .. offsets[i+1]); offsets has n + 1 entries.
+ *
+ * Formatting: values are AoS (1, 2 or 4 doubles per element).  offsets is always written;
+ * when text is NULL or cap < offsets[n] the call returns HP_ESHAPE without writing text
+ * (call again with cap >= offsets[n]).  Maximum lengths per element: f64 with precision P:
+ * P + 8 bytes; dd Display 40; MF4 Display 72; dd serde 53; MF4 serde 91.
+ *
+ * Parsing: out receives 1, 2 or 4 doubles per element (zeros for a rejected element);
+ * status (may be NULL) receives per element 0 ok, 1 rejected literal (the reference's
+ * Err), 2 the literal has more than 40 significant digits and lies so close to a halfway
+ * point that they do not settle the rounding.  Returns HP_EINVAL, naming the first bad
+ * element, when any status is nonzero. */
+
+/* Rust's format!("{:.P$e}") (force_sign = 0) or ("{:+.P$e}") of f64 values, 0 <= P <= 64:
+ * exact decimal, ties to even (the building block of the formats below). */
+int hp_format_f64(const double* v, size_t n, int precision, int force_sign, char* text, size_t cap, size_t* offsets);
+/* Display of f128, "{:.32e}" of hi + lo (crates/quad/src/dd.rs:337-341) */
+int hp_dd_to_string(const double* v_dd, size_t n, char* text, size_t cap, size_t* offsets);
+/* Display of MultiFloat<4>, "{:.64e}" of limb0 + limb1 (crates/quad/src/multifloat/fmt.rs:7-17) */
+int hp_mf4_to_string(const double* v_mf4, size_t n, char* text, size_t cap, size_t* offsets);
+/* human-readable serde of f128, "{:+.18e}+{:+.18e}" of hi, lo (dd.rs:446-450) */
+int hp_dd_serialize_str(const double* v_dd, size_t n, char* text, size_t cap, size_t* offsets);
+/* human-readable serde of MultiFloat<4>, "{:+.14e}" limbs joined by '+'
+ * (crates/quad/src/multifloat/traits.rs:14-24) */
+int hp_mf4_serialize_str(const double* v_mf4, size_t n, char* text, size_t cap, size_t* offsets);
+/* Rust's str::parse::<f64> (the building block of the serde parsers) */
+int hp_parse_f64(const char* text, const size_t* offsets, size_t n, double* out, uint8_t* status);
+/* FromStr of f128 (dd.rs:371-435) and of MultiFloat<4> (multifloat/fmt.rs:32-114) */
+int hp_dd_from_str(const char* text, const size_t* offsets, size_t n, double* out_dd, uint8_t* status);
+int hp_mf4_from_str(const char* text, const size_t* offsets, size_t n, double* out_mf4, uint8_t* status);
+/* serde's visit_str of f128 (dd.rs:475-479, parse_dd_string 496-511) and of MultiFloat<4>
+ * (traits.rs:53-57, parse_limbs_string 104-148) */
+int hp_dd_deserialize_str(const char* text, const size_t* offsets, size_t n, double* out_dd, uint8_t* status);
+int hp_mf4_deserialize_str(const char* text, const size_t* offsets, size_t n, double* out_mf4, uint8_t* status);
+/* Device-resident forms.  kind (format): 0 f64 "{:.Pe}", 1 dd Display, 2 MF4 Display, 3 dd
+ * serde, 4 MF4 serde; planes[k] = limb plane k (n doubles each); text / offsets are device
+ * memory, *need = offsets[n].  kind (parse): 0 f64, 1 dd FromStr, 2 MF4 FromStr, 3 dd serde,
+ * 4 MF4 serde; status is device memory (n bytes), *n_bad counts nonzero statuses. */
+int hp_dev_format(int kind, const double* const* planes, size_t n, int precision, int force_sign, char* text,
+                  size_t cap, size_t* offsets, size_t* need, void* stream);
+int hp_dev_parse(int kind, const char* text, const size_t* offsets, size_t n, double* const* planes, uint8_t* status,
+                 size_t* n_bad, void* stream);
+
 /* ------------------------------------------------------------------ adjoints */
 
 /* Gradients of F1 = log det S and F2 = b^T S^-1 b (S as hp_overlap_s, b dd) with respect to

@@ -10,7 +10,12 @@
 #include <algorithm>
 #include <cfloat>
 #include <cmath>
+#include <cctype>
+#include <cstdint>
+#include <cstdio>
+#include <cstdlib>
 #include <cstring>
+#include <string>
 #include <vector>
 #ifdef _OPENMP
 #include <omp.h>
@@ -1325,4 +1330,263 @@ long hpo_count_overlap_entry(double ai, const double* ci, double aj, const doubl
   (void)r;
   return g_ops;
 }
+}
+
+// ------------------------------------------------------------------ text wire formats
+// Rust's `{:.Pe}`: glibc's printf("%.*e") is exact with ties to even (the default rounding
+// mode), as Rust's exact-mode formatter is; only the exponent spelling differs ("e-7", "e12"
+// in Rust against "e-07", "e+12"), and Rust prints NaN as "NaN" without a sign.
+int hpo_format_exp(double x, int prec, int plus, char* out) {
+  if (std::isnan(x)) {
+    std::memcpy(out, "NaN", 3);
+    return 3;
+  }
+  int n = 0;
+  if (std::signbit(x)) out[n++] = '-';
+  else if (plus) out[n++] = '+';
+  if (std::isinf(x)) {
+    std::memcpy(out + n, "inf", 3);
+    return n + 3;
+  }
+  char buf[128];
+  std::snprintf(buf, sizeof buf, "%.*e", prec, std::fabs(x));
+  const char* e = std::strchr(buf, 'e');
+  const size_t mlen = (size_t)(e - buf);
+  std::memcpy(out + n, buf, mlen);
+  n += (int)mlen;
+  n += std::sprintf(out + n, "e%d", std::atoi(e + 1));
+  return n;
+}
+int hpo_dd_to_string(const double* v, char* out) { return hpo_format_exp(v[0] + v[1], 32, 0, out); }
+int hpo_mf4_to_string(const double* v, char* out) { return hpo_format_exp(v[0] + v[1], 64, 0, out); }
+int hpo_dd_serialize_str(const double* v, char* out) {
+  int n = hpo_format_exp(v[0], 18, 1, out);
+  out[n++] = '+';
+  return n + hpo_format_exp(v[1], 18, 1, out + n);
+}
+int hpo_mf4_serialize_str(const double* v, char* out) {
+  int n = 0;
+  for (int i = 0; i < 4; ++i) {
+    if (i) out[n++] = '+';
+    n += hpo_format_exp(v[i], 14, 1, out + n);
+  }
+  return n;
+}
+
+// Rust's f64 grammar: [+-] then inf | infinity | nan (any case), or digits [. digits]
+// [(e|E) [+-] digits] with at least one mantissa digit.  A literal that passes goes to
+// glibc's strtod, which rounds correctly for any number of digits.
+int hpo_parse_f64(const char* s, size_t len, double* out) {
+  size_t i = 0;
+  bool neg = false;
+  if (len && (s[0] == '+' || s[0] == '-')) {
+    neg = s[0] == '-';
+    i = 1;
+  }
+  if (i >= len) return 1;
+  std::string rest(s + i, len - i);
+  std::string low = rest;
+  for (char& c : low) c = (char)std::tolower((unsigned char)c);
+  if (low == "inf" || low == "infinity") {
+    *out = neg ? -INFINITY : INFINITY;
+    return 0;
+  }
+  if (low == "nan") {
+    *out = neg ? -NAN : NAN;
+    return 0;
+  }
+  size_t j = 0, nd = 0;
+  while (j < rest.size() && std::isdigit((unsigned char)rest[j])) ++j, ++nd;
+  if (j < rest.size() && rest[j] == '.') {
+    ++j;
+    while (j < rest.size() && std::isdigit((unsigned char)rest[j])) ++j, ++nd;
+  }
+  if (nd == 0) return 1;
+  if (j < rest.size() && (rest[j] == 'e' || rest[j] == 'E')) {
+    ++j;
+    if (j < rest.size() && (rest[j] == '+' || rest[j] == '-')) ++j;
+    size_t ne = 0;
+    while (j < rest.size() && std::isdigit((unsigned char)rest[j])) ++j, ++ne;
+    if (ne == 0) return 1;
+  }
+  if (j != rest.size()) return 1;
+  std::string lit = (neg ? "-" : "") + rest;
+  *out = std::strtod(lit.c_str(), nullptr);
+  return 0;
+}
+
+// str::trim: strip Unicode White_Space code points from both ends (UTF-8 input)
+static bool white_space(uint32_t c) {
+  return (c >= 0x09 && c <= 0x0D) || c == 0x20 || c == 0x85 || c == 0xA0 || c == 0x1680 ||
+         (c >= 0x2000 && c <= 0x200A) || c == 0x2028 || c == 0x2029 || c == 0x202F || c == 0x205F || c == 0x3000;
+}
+static std::vector<uint32_t> utf8_decode(const char* s, size_t len, std::vector<size_t>& start) {
+  std::vector<uint32_t> cp;
+  for (size_t i = 0; i < len;) {
+    const unsigned char c = (unsigned char)s[i];
+    const int k = c < 0x80 ? 1 : c < 0xE0 ? 2 : c < 0xF0 ? 3 : 4;
+    uint32_t v = k == 1 ? c : k == 2 ? (c & 0x1F) : k == 3 ? (c & 0x0F) : (c & 0x07);
+    for (int t = 1; t < k && i + t < len; ++t) v = (v << 6) | ((unsigned char)s[i + t] & 0x3F);
+    cp.push_back(v);
+    start.push_back(i);
+    i += k;
+  }
+  start.push_back(len);
+  return cp;
+}
+static void rust_trim(const char* s, size_t len, size_t& b, size_t& e) {
+  std::vector<size_t> st;
+  std::vector<uint32_t> cp = utf8_decode(s, len, st);
+  size_t lo = 0, hi = cp.size();
+  while (lo < hi && white_space(cp[lo])) ++lo;
+  while (hi > lo && white_space(cp[hi - 1])) --hi;
+  b = st[lo];
+  e = st[hi];
+}
+// i32::from_str
+static bool parse_i32(const std::string& t, int& out) {
+  size_t i = 0;
+  bool neg = false;
+  if (!t.empty() && (t[0] == '+' || t[0] == '-')) {
+    neg = t[0] == '-';
+    i = 1;
+  }
+  if (i >= t.size()) return false;
+  long long v = 0;
+  for (; i < t.size(); ++i) {
+    if (!std::isdigit((unsigned char)t[i])) return false;
+    v = v * 10 + (t[i] - '0');
+    if (v > 2147483648LL) return false;
+  }
+  if (!neg && v > 2147483647LL) return false;
+  out = (int)(neg ? -v : v);
+  return true;
+}
+// FromStr of dd.rs:371-435 and multifloat/fmt.rs:32-114, generic over the value type.
+// The exponent loop runs |exp| steps; i32::MIN runs none, as `0..(-exp)` does when -exp
+// wraps in a release build.
+template <class T, class Ops>
+static int from_str_impl(const char* s0, size_t len0, T& out) {
+  size_t b, e;
+  rust_trim(s0, len0, b, e);
+  const char* s = s0 + b;
+  size_t len = e - b;
+  if (len == 0) return 1;
+  bool negative = false;
+  if (s[0] == '-' || s[0] == '+') {
+    negative = s[0] == '-';
+    ++s;
+    --len;
+  }
+  if (len == 0) return 1;
+  size_t split = len;
+  for (size_t i = 0; i < len; ++i)
+    if (s[i] == 'e' || s[i] == 'E') {
+      split = i;
+      break;
+    }
+  if (split == 0) return 1;
+  const T ten = Ops::from(10.0);
+  T acc = Ops::zero(), pow10_neg = Ops::from(1.0);
+  bool saw_dot = false, any_digit = false;
+  for (size_t i = 0; i < split; ++i) {
+    const char c = s[i];
+    if (c == '.') {
+      if (saw_dot) return 1;
+      saw_dot = true;
+      continue;
+    }
+    if (c < '0' || c > '9') return 1;
+    any_digit = true;
+    const T digit = Ops::from((double)(c - '0'));
+    if (!saw_dot) {
+      acc = Ops::add(Ops::mul(acc, ten), digit);
+    } else {
+      pow10_neg = Ops::div(pow10_neg, ten);
+      acc = Ops::add(acc, Ops::mul(digit, pow10_neg));
+    }
+  }
+  if (!any_digit) return 1;
+  if (split < len) {
+    int ex;
+    if (!parse_i32(std::string(s + split + 1, len - split - 1), ex)) return 1;
+    if (ex > 0) {
+      for (int k = 0; k < ex; ++k) acc = Ops::mul(acc, ten);
+    } else if (ex < 0 && ex != INT32_MIN) {
+      for (int k = 0; k < -ex; ++k) acc = Ops::div(acc, ten);
+    }
+  }
+  out = negative ? Ops::neg(acc) : acc;
+  return 0;
+}
+struct DdTextOps {
+  static D zero() { return D{0.0, 0.0}; }
+  static D from(double x) { return dd_from(x); }
+  static D add(D a, D b) { return hpo::add(a, b); }
+  static D mul(D a, D b) { return hpo::mul(a, b); }
+  static D div(D a, D b) { return hpo::div(a, b); }
+  static D neg(D a) { return hpo::neg(a); }
+};
+struct Mf4TextOps {
+  static M4 zero() { return mf_zero<double>(); }
+  static M4 from(double x) { return mf_from(x); }
+  static M4 add(M4 a, M4 b) { return mf_add(a, b); }
+  static M4 mul(M4 a, M4 b) { return mf_mul(a, b); }
+  static M4 div(M4 a, M4 b) { return mf_div(a, b); }
+  static M4 neg(M4 a) { return mf_neg(a); }
+};
+int hpo_dd_from_str(const char* s, size_t len, double* out) {
+  D v{0.0, 0.0};
+  const int rc = from_str_impl<D, DdTextOps>(s, len, v);
+  out[0] = rc ? 0.0 : v.hi;
+  out[1] = rc ? 0.0 : v.lo;
+  return rc;
+}
+int hpo_mf4_from_str(const char* s, size_t len, double* out) {
+  M4 v = mf_zero<double>();
+  const int rc = from_str_impl<M4, Mf4TextOps>(s, len, v);
+  for (int k = 0; k < 4; ++k) out[k] = rc ? 0.0 : v.l[k];
+  return rc;
+}
+// dd.rs:496-511: split at the first '+' (index 1..len-2) that is followed by a sign and
+// not preceded by 'e'/'E'; both halves must parse as f64
+int hpo_dd_deserialize_str(const char* s, size_t len, double* out) {
+  out[0] = out[1] = 0.0;
+  for (size_t i = 1; i + 1 < len; ++i) {
+    if (s[i] == '+' && (s[i + 1] == '+' || s[i + 1] == '-')) {
+      if (s[i - 1] == 'e' || s[i - 1] == 'E') continue;
+      double hi, lo;
+      if (hpo_parse_f64(s, i, &hi) || hpo_parse_f64(s + i + 1, len - i - 1, &lo)) return 1;
+      out[0] = hi;
+      out[1] = lo;
+      return 0;
+    }
+  }
+  return 1;
+}
+// traits.rs:104-148: limb boundaries are '+' followed by a sign, not preceded by 'e'/'E';
+// exactly four limbs
+int hpo_mf4_deserialize_str(const char* s, size_t len, double* out) {
+  double limbs[4] = {0.0, 0.0, 0.0, 0.0};
+  size_t count = 0, start = 0, i = 1;
+  for (int k = 0; k < 4; ++k) out[k] = 0.0;
+  while (i < len) {
+    const bool boundary = s[i] == '+' && i + 1 < len && (s[i + 1] == '+' || s[i + 1] == '-') && s[i - 1] != 'e' &&
+                          s[i - 1] != 'E';
+    if (boundary) {
+      if (count >= 4) return 1;
+      if (hpo_parse_f64(s + start, i - start, &limbs[count])) return 1;
+      ++count;
+      start = i + 1;
+      i = start + 1;
+    } else {
+      ++i;
+    }
+  }
+  if (count >= 4) return 1;
+  if (hpo_parse_f64(s + start, len - start, &limbs[count])) return 1;
+  ++count;
+  if (count != 4) return 1;
+  for (int k = 0; k < 4; ++k) out[k] = limbs[k];
+  return 0;
 }

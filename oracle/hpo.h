@@ -173,6 +173,23 @@ size_t hpo_canonical_orthogonalize(const double* s, size_t n, double threshold, 
  * (capacities n and n x n); returns n_basis */
 size_t hpo_solve_gen_eigen(const double* f, const double* s, size_t n, double threshold, double* evals, double* c);
 
+/* ---- text wire formats (crates/quad/src/dd.rs:331-435,441-514; multifloat/fmt.rs;
+ * multifloat/traits.rs:7-150).  Formatting returns the length written to out (no NUL;
+ * out holds >= prec + 16 bytes per f64); parsing returns 0 on success, 1 on a rejected
+ * literal. ---- */
+/* Rust's format!("{:.prec$e}") / ("{:+.prec$e}") of one f64 */
+int hpo_format_exp(double x, int prec, int plus, char* out);
+int hpo_dd_to_string(const double* v2, char* out);      /* Display, dd.rs:337-341 */
+int hpo_mf4_to_string(const double* v4, char* out);     /* Display, multifloat/fmt.rs:7-17 */
+int hpo_dd_serialize_str(const double* v2, char* out);  /* dd.rs:446-450 */
+int hpo_mf4_serialize_str(const double* v4, char* out); /* traits.rs:14-24 */
+/* Rust's str::parse::<f64> */
+int hpo_parse_f64(const char* s, size_t len, double* out);
+int hpo_dd_from_str(const char* s, size_t len, double* out2);        /* dd.rs:371-435 */
+int hpo_mf4_from_str(const char* s, size_t len, double* out4);       /* multifloat/fmt.rs:32-114 */
+int hpo_dd_deserialize_str(const char* s, size_t len, double* out2);  /* dd.rs:496-511 */
+int hpo_mf4_deserialize_str(const char* s, size_t len, double* out4); /* traits.rs:104-148 */
+
 /* ---- counting: FP64 ops per unit of work, from an instrumented restatement ---- */
 /* returns the number of binary64 operations (+,-,*,/,fma,sqrt) one dd MAC s += a*b costs (29) */
 long hpo_count_dd_mac(void);

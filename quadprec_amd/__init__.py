@@ -134,6 +134,16 @@ def lib():
     L.hp_dev_gemv.argtypes = [_vp, _vp, _sz, _sz, _vp, _vp, _vp, _vp, _vp]
     for name in ("hp_neumaier_sum_f64", "hp_neumaier_sum", "hp_par_neumaier_sum_f64", "hp_par_neumaier_sum"):
         getattr(L, name).argtypes = [_dp, _sz, _dp]
+    _op = np.ctypeslib.ndpointer(dtype=np.uint64, flags="C_CONTIGUOUS")
+    _u8 = np.ctypeslib.ndpointer(dtype=np.uint8, flags="C_CONTIGUOUS")
+    L.hp_format_f64.argtypes = [_dp, _sz, C.c_int, C.c_int, C.c_char_p, _sz, _op]
+    for name in ("hp_dd_to_string", "hp_mf4_to_string", "hp_dd_serialize_str", "hp_mf4_serialize_str"):
+        getattr(L, name).argtypes = [_dp, _sz, C.c_char_p, _sz, _op]
+    for name in ("hp_parse_f64", "hp_dd_from_str", "hp_mf4_from_str", "hp_dd_deserialize_str",
+                 "hp_mf4_deserialize_str"):
+        getattr(L, name).argtypes = [C.c_char_p, _op, _sz, _dp, _u8]
+    L.hp_dev_format.argtypes = [C.c_int, _vp, _sz, C.c_int, C.c_int, _vp, _sz, _vp, C.POINTER(C.c_size_t), _vp]
+    L.hp_dev_parse.argtypes = [C.c_int, _vp, _vp, _sz, _vp, _vp, C.POINTER(C.c_size_t), _vp]
     L.hp_overlap_s.argtypes = [_dp, _dp, _sz, _dp]
     L.hp_dev_overlap_s.argtypes = [_vp, _vp, _sz, _vp, _vp, _vp]
     L.hp_dev_overlap_forward.argtypes = [_vp, _vp, _vp, _vp, _sz, C.c_int, C.POINTER(_vp), C.POINTER(AdjointInfo),
@@ -423,6 +433,105 @@ def neumaier_sum(values, parallel=False):
     fn = lib().hp_par_neumaier_sum if parallel else lib().hp_neumaier_sum
     _check(fn(v if n else np.zeros(2), n, out), "neumaier_sum")
     return out
+
+
+# ---------------------------------------------------------------- text wire formats
+class ParseError(HyperprecError):
+    """A literal the reference rejects (ParseF128Error / ParseMultiFloatError / serde's
+    invalid_value), or one whose rounding needs more than 40 significant digits."""
+
+
+def pack_strings(strings):
+    """list of str/bytes -> (bytes, offsets uint64 (n+1)), the Arrow layout of the text entry points."""
+    bs = [x.encode() if isinstance(x, str) else bytes(x) for x in strings]
+    off = np.zeros(len(bs) + 1, dtype=np.uint64)
+    off[1:] = np.cumsum([len(b) for b in bs]) if bs else []
+    return b"".join(bs), off
+
+
+def unpack_strings(text, offsets):
+    return [text[int(offsets[i]):int(offsets[i + 1])].decode() for i in range(len(offsets) - 1)]
+
+
+def _format(fn, what, v, limbs, max_len, *extra, raw=False):
+    v = np.ascontiguousarray(v, dtype=np.float64)
+    n = v.size // limbs
+    cap = max(n * max_len, 1)
+    buf = C.create_string_buffer(cap)
+    off = np.zeros(n + 1, dtype=np.uint64)
+    _check(fn(v if n else np.zeros(limbs), n, *extra, buf, cap, off), what)
+    text = buf.raw[:int(off[n])]
+    return (text, off) if raw else unpack_strings(text, off)
+
+
+def format_f64(values, precision, force_sign=False, raw=False):
+    """Rust's format!("{:.P$e}") / ("{:+.P$e}") of each value: list of str (raw: (bytes, offsets))."""
+    if not 0 <= precision <= 64:
+        raise ValueError("precision must be in 0..64")
+    return _format(lib().hp_format_f64, "format_f64", values, 1, precision + 8, int(precision), int(force_sign),
+                   raw=raw)
+
+
+def dd_to_string(v, raw=False):
+    """Display of f128 values (dd.rs:337-341): "{:.32e}" of hi + lo."""
+    return _format(lib().hp_dd_to_string, "dd_to_string", v, 2, 40, raw=raw)
+
+
+def mf4_to_string(v, raw=False):
+    """Display of MultiFloat<4> values (multifloat/fmt.rs:7-17): "{:.64e}" of limb0 + limb1."""
+    return _format(lib().hp_mf4_to_string, "mf4_to_string", v, 4, 72, raw=raw)
+
+
+def dd_serialize_str(v, raw=False):
+    """Human-readable serde of f128 values (dd.rs:446-450): "{:+.18e}+{:+.18e}"."""
+    return _format(lib().hp_dd_serialize_str, "dd_serialize_str", v, 2, 53, raw=raw)
+
+
+def mf4_serialize_str(v, raw=False):
+    """Human-readable serde of MultiFloat<4> values (traits.rs:14-24): '+'-joined "{:+.14e}" limbs."""
+    return _format(lib().hp_mf4_serialize_str, "mf4_serialize_str", v, 4, 91, raw=raw)
+
+
+def _parse(fn, what, strings, limbs, strict):
+    text, off = strings if isinstance(strings, tuple) else pack_strings(strings)
+    n = len(off) - 1
+    out = np.zeros((max(n, 1), limbs))
+    status = np.zeros(max(n, 1), dtype=np.uint8)
+    rc = fn(text if text else b"\0", off, n, out, status)
+    if rc == HP_EINVAL and not strict:
+        pass
+    elif rc == HP_EINVAL:
+        raise ParseError(f"{what}: " + lib().hp_last_error().decode(errors="replace"))
+    else:
+        _check(rc, what)
+    out = out[:n, 0] if limbs == 1 else out[:n]
+    return (out, status[:n]) if not strict else out
+
+
+def parse_f64(strings, strict=True):
+    """Rust's str::parse::<f64> of each string.  strict: raise ParseError on any rejected
+    element; else return (values, status) with status 0 ok / 1 rejected / 2 undecided."""
+    return _parse(lib().hp_parse_f64, "parse_f64", strings, 1, strict)
+
+
+def dd_from_str(strings, strict=True):
+    """FromStr of f128 (dd.rs:371-435): (n, 2)."""
+    return _parse(lib().hp_dd_from_str, "dd_from_str", strings, 2, strict)
+
+
+def mf4_from_str(strings, strict=True):
+    """FromStr of MultiFloat<4> (multifloat/fmt.rs:32-114): (n, 4)."""
+    return _parse(lib().hp_mf4_from_str, "mf4_from_str", strings, 4, strict)
+
+
+def dd_deserialize_str(strings, strict=True):
+    """serde string form of f128 (dd.rs:496-511): (n, 2)."""
+    return _parse(lib().hp_dd_deserialize_str, "dd_deserialize_str", strings, 2, strict)
+
+
+def mf4_deserialize_str(strings, strict=True):
+    """serde string form of MultiFloat<4> (traits.rs:104-148): (n, 4)."""
+    return _parse(lib().hp_mf4_deserialize_str, "mf4_deserialize_str", strings, 4, strict)
 
 
 def overlap_s(centers, alpha):

@@ -123,6 +123,12 @@ def lib():
         L.hpo_canonical_orthogonalize.restype = C.c_size_t
         L.hpo_solve_gen_eigen.argtypes = [_dp, _dp, _sz, C.c_double, _dp, _dp]
         L.hpo_solve_gen_eigen.restype = C.c_size_t
+        L.hpo_format_exp.argtypes = [C.c_double, C.c_int, C.c_int, C.c_char_p]
+        for name in ("hpo_dd_to_string", "hpo_mf4_to_string", "hpo_dd_serialize_str", "hpo_mf4_serialize_str"):
+            getattr(L, name).argtypes = [_dp, C.c_char_p]
+        L.hpo_parse_f64.argtypes = [C.c_char_p, _sz, C.POINTER(C.c_double)]
+        for name in ("hpo_dd_from_str", "hpo_mf4_from_str", "hpo_dd_deserialize_str", "hpo_mf4_deserialize_str"):
+            getattr(L, name).argtypes = [C.c_char_p, _sz, _dp]
         _lib = L
     return _lib
 
@@ -457,3 +463,66 @@ def neumaier_sum(v, parallel=False):
     v = _c(v)
     r = (lib().hpo_par_neumaier_sum if parallel else lib().hpo_neumaier_sum)(v, v.size // 2)
     return np.array([r.hi, r.lo])
+
+
+# ---------------------------------------------------------------- text wire formats
+def _bytes(t):
+    return t.encode() if isinstance(t, str) else bytes(t)
+
+
+def format_exp(x, prec, plus=False):
+    """Rust's format!("{:.prec$e}") (or "{:+.prec$e}") of one f64."""
+    buf = C.create_string_buffer(prec + 32)
+    n = lib().hpo_format_exp(float(x), prec, int(plus), buf)
+    return buf.raw[:n].decode()
+
+
+def _fmt_value(name, v, limbs):
+    buf = C.create_string_buffer(512)
+    n = getattr(lib(), name)(np.ascontiguousarray(v, dtype=np.float64).reshape(limbs), buf)
+    return buf.raw[:n].decode()
+
+
+def dd_to_string(v):
+    return _fmt_value("hpo_dd_to_string", v, 2)
+
+
+def mf4_to_string(v):
+    return _fmt_value("hpo_mf4_to_string", v, 4)
+
+
+def dd_serialize_str(v):
+    return _fmt_value("hpo_dd_serialize_str", v, 2)
+
+
+def mf4_serialize_str(v):
+    return _fmt_value("hpo_mf4_serialize_str", v, 4)
+
+
+def parse_f64(text):
+    """Rust's str::parse::<f64>: the value, or None when the literal is rejected."""
+    b = _bytes(text)
+    out = C.c_double(0.0)
+    return None if lib().hpo_parse_f64(b, len(b), C.byref(out)) else out.value
+
+
+def _parse_value(name, text, limbs):
+    b = _bytes(text)
+    out = np.zeros(limbs)
+    return None if getattr(lib(), name)(b, len(b), out) else out
+
+
+def dd_from_str(text):
+    return _parse_value("hpo_dd_from_str", text, 2)
+
+
+def mf4_from_str(text):
+    return _parse_value("hpo_mf4_from_str", text, 4)
+
+
+def dd_deserialize_str(text):
+    return _parse_value("hpo_dd_deserialize_str", text, 2)
+
+
+def mf4_deserialize_str(text):
+    return _parse_value("hpo_mf4_deserialize_str", text, 4)
