@@ -255,6 +255,81 @@ def bench_eigen(qp, torch, dev, cpu):
     return out
 
 
+def bench_text(qp, torch, dev, cpu, reps=5):
+    """The serde string form of f128 (dd.rs:446-450 and 496-511) over TEXT_N dd values
+    resident in HBM: format to Arrow-style text + offsets in device memory, and parse that
+    text back (bit-identical round trip; parity in tests/test_gpu_text.py).  Bytes per value:
+    16 in, 53 of text + 8 of offset out (format); the reverse for parse."""
+    import ctypes as C
+    import numpy as np
+    L = qp.lib()
+    n = TEXT_N
+    rng = np.random.default_rng(2)
+    hi = rng.standard_normal(n) * 10.0 ** rng.integers(-300, 300, n)
+    lo = hi * 2.0 ** -53 * rng.uniform(-0.5, 0.5, n)
+    dhi, dlo = torch.tensor(hi, device=dev), torch.tensor(lo, device=dev)
+    planes = (C.c_void_p * 4)(dhi.data_ptr(), dlo.data_ptr(), 0, 0)
+    text = torch.zeros(53 * n, dtype=torch.uint8, device=dev)
+    off = torch.zeros(n + 1, dtype=torch.int64, device=dev)
+    ohi, olo = torch.empty_like(dhi), torch.empty_like(dlo)
+    oplanes = (C.c_void_p * 4)(ohi.data_ptr(), olo.data_ptr(), 0, 0)
+    status = torch.empty(n, dtype=torch.uint8, device=dev)
+    need, bad = C.c_size_t(0), C.c_size_t(0)
+
+    def fmt():
+        rc = L.hp_dev_format(3, C.cast(planes, C.c_void_p), n, 18, 1, C.c_void_p(text.data_ptr()), 53 * n,
+                             C.c_void_p(off.data_ptr()), C.byref(need), None)
+        if rc:
+            raise RuntimeError(f"hp_dev_format: status {rc}")
+
+    def parse():
+        rc = L.hp_dev_parse(3, C.c_void_p(text.data_ptr()), C.c_void_p(off.data_ptr()), n,
+                            C.cast(oplanes, C.c_void_p), C.c_void_p(status.data_ptr()), C.byref(bad), None)
+        if rc or bad.value:
+            raise RuntimeError(f"hp_dev_parse: status {rc}, {bad.value} rejected")
+
+    def best(fn):
+        fn()
+        ts = []
+        for _ in range(reps):
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            fn()
+            torch.cuda.synchronize()
+            ts.append(time.perf_counter() - t0)
+        return min(ts)
+
+    tf, tp = best(fmt), best(parse)
+    ok = torch.equal(ohi, dhi) and torch.equal(olo, dlo)
+    nbytes = int(need.value)
+    out = {"workload": f"serde string form of f128, {n} dd values (N(0,1) x 10^U[-300,300), seed 2) in HBM",
+           "text_bytes": nbytes, "round_trip_bit_exact": bool(ok),
+           "format": {"ms": round(tf * 1e3, 3), "Mvalues_per_s": round(n / tf / 1e6, 1),
+                      "GB_per_s": round((16 * n + nbytes + 8 * n) / tf / 1e9, 1)},
+           "parse": {"ms": round(tp * 1e3, 3), "Mvalues_per_s": round(n / tp / 1e6, 1),
+                     "GB_per_s": round((16 * n + nbytes + 8 * n) / tp / 1e9, 1)},
+           "roofline": {"bound": "valu", "note": "exact decimal conversion with a 1024-bit integer per value "
+                                                 "(multiplies / divides by 5^13, shifts): integer VALU bound; "
+                                                 "bytes moved are ~77 per value, < 1% of HBM time"}}
+    if cpu:
+        sys.path.insert(0, os.path.join(ROOT, "tests"))
+        import oracle_lib as O
+        m = TEXT_CPU_N
+        v = np.stack([hi[:m], lo[:m]], axis=1)
+        t0 = time.perf_counter()
+        txt, offs = O.dd_serialize_n(v)
+        t1 = time.perf_counter()
+        back, nbad = O.dd_deserialize_n(txt, offs)
+        t2 = time.perf_counter()
+        assert nbad == 0 and np.array_equal(back, v)
+        out["cpu_baseline"] = {"value": round(m / (t1 - t0) / 1e6, 3), "unit": "Mvalues/s (format)",
+                               "parse_Mvalues_per_s": round(m / (t2 - t1) / 1e6, 3), "cores": 1, "kind": "port",
+                               "sample": f"the first {m} values, oracle C++ (glibc printf / strtod behind the "
+                                         "reference's grammar and split rules), one thread"}
+    return out
+
+
+TEXT_N, TEXT_CPU_N = 1 << 22, 1 << 19
 EIG_N, EIG_CPU_N = 512, 192
 ADJ_N, ADJ_SEED, ADJ_B_SEED = 2048, 4, 44
 
@@ -335,6 +410,7 @@ def main():
     ap.add_argument("--no-mf4", action="store_true", help="skip the MF4 1024^3 gemm")
     ap.add_argument("--no-adjoint", action="store_true", help="skip the N=2048 adjoint")
     ap.add_argument("--no-eigen", action="store_true", help="skip the N=512 generalized eigensolve")
+    ap.add_argument("--no-text", action="store_true", help="skip the dd serde text round trip")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only copy MIB MiB device-to-device three times (a plain copy kernel of known size, "
                          "to calibrate rocprofv3's FETCH_SIZE / WRITE_SIZE counters) and exit")
@@ -430,6 +506,8 @@ def main():
         out["adjoint_n2048"] = bench_adjoint(qp, torch, dev, not args.no_cpu)
     if not args.no_eigen:
         out["gen_eigen_n512"] = bench_eigen(qp, torch, dev, not args.no_cpu)
+    if not args.no_text:
+        out["dd_serde_text_4m"] = bench_text(qp, torch, dev, not args.no_cpu)
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(a, b)
     print(json.dumps(out), flush=True)

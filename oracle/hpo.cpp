@@ -1590,3 +1590,17 @@ int hpo_mf4_deserialize_str(const char* s, size_t len, double* out) {
   for (int k = 0; k < 4; ++k) out[k] = limbs[k];
   return 0;
 }
+size_t hpo_dd_serialize_n(const double* v, size_t n, char* text, size_t* offsets) {
+  size_t o = 0;
+  offsets[0] = 0;
+  for (size_t i = 0; i < n; ++i) {
+    o += (size_t)hpo_dd_serialize_str(v + 2 * i, text + o);
+    offsets[i + 1] = o;
+  }
+  return o;
+}
+size_t hpo_dd_deserialize_n(const char* text, const size_t* offsets, size_t n, double* out) {
+  size_t bad = 0;
+  for (size_t i = 0; i < n; ++i) bad += hpo_dd_deserialize_str(text + offsets[i], offsets[i + 1] - offsets[i], out + 2 * i) != 0;
+  return bad;
+}

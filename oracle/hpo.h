@@ -189,6 +189,10 @@ int hpo_dd_from_str(const char* s, size_t len, double* out2);        /* dd.rs:37
 int hpo_mf4_from_str(const char* s, size_t len, double* out4);       /* multifloat/fmt.rs:32-114 */
 int hpo_dd_deserialize_str(const char* s, size_t len, double* out2);  /* dd.rs:496-511 */
 int hpo_mf4_deserialize_str(const char* s, size_t len, double* out4); /* traits.rs:104-148 */
+/* batches over n dd values for the CPU-baseline timing: text + n + 1 offsets (Arrow
+ * layout), single thread; deserialize returns the number of rejected elements */
+size_t hpo_dd_serialize_n(const double* v, size_t n, char* text, size_t* offsets);
+size_t hpo_dd_deserialize_n(const char* text, const size_t* offsets, size_t n, double* out);
 
 /* ---- counting: FP64 ops per unit of work, from an instrumented restatement ---- */
 /* returns the number of binary64 operations (+,-,*,/,fma,sqrt) one dd MAC s += a*b costs (29) */

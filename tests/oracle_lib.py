@@ -129,6 +129,11 @@ def lib():
         L.hpo_parse_f64.argtypes = [C.c_char_p, _sz, C.POINTER(C.c_double)]
         for name in ("hpo_dd_from_str", "hpo_mf4_from_str", "hpo_dd_deserialize_str", "hpo_mf4_deserialize_str"):
             getattr(L, name).argtypes = [C.c_char_p, _sz, _dp]
+        _op = np.ctypeslib.ndpointer(dtype=np.uint64, flags="C_CONTIGUOUS")
+        L.hpo_dd_serialize_n.argtypes = [_dp, _sz, C.c_char_p, _op]
+        L.hpo_dd_serialize_n.restype = _sz
+        L.hpo_dd_deserialize_n.argtypes = [C.c_char_p, _op, _sz, _dp]
+        L.hpo_dd_deserialize_n.restype = _sz
         _lib = L
     return _lib
 
@@ -526,3 +531,20 @@ def dd_deserialize_str(text):
 
 def mf4_deserialize_str(text):
     return _parse_value("hpo_mf4_deserialize_str", text, 4)
+
+
+def dd_serialize_n(v):
+    """serde strings of n dd values in one C loop: (bytes, offsets)."""
+    v = _c(v)
+    n = v.size // 2
+    buf = C.create_string_buffer(max(53 * n, 1))
+    off = np.zeros(n + 1, dtype=np.uint64)
+    total = lib().hpo_dd_serialize_n(v, n, buf, off)
+    return buf.raw[:total], off
+
+
+def dd_deserialize_n(text, off):
+    n = len(off) - 1
+    out = np.zeros((max(n, 1), 2))
+    bad = lib().hpo_dd_deserialize_n(text, np.ascontiguousarray(off, dtype=np.uint64), n, out)
+    return out[:n], bad
