@@ -5,12 +5,13 @@
 // The reference rotates one (p, q) pair at a time (cyclic sweeps in linalg.rs, the
 // largest |a_pq| in ortho.rs), a serial chain of O(n) updates.  Here every step of a sweep
 // applies n/2 disjoint rotations at once (round-robin order, every pair once per sweep):
-//   * k_jac_rot: per pair, the reference's rotation (tau = (a_qq - a_pp) / 2 a_pq,
+//   * per pair, the reference's rotation (tau = (a_qq - a_pp) / 2 a_pq,
 //     t = sign(tau) / (|tau| + sqrt(1 + tau^2)), c = 1 / sqrt(1 + t^2), s = t c) and the
 //     new diagonal a_pp - t a_pq, a_qq + t a_pq (linalg.rs:323-333);
-//   * k_jac_apply: every 2 x 2 block (pairs I <= J) becomes R_I^T X R_J (the diagonal
-//     blocks take the new diagonal and exact zeros) and is written with its mirror, so A
-//     stays exactly symmetric; the eigenvector matrix takes V R.
+//   * k_jac_step: every 2 x 2 block (I, J) becomes R_I^T X R_J (the diagonal blocks take
+//     the new diagonal and exact zeros), the lower blocks in the mirrored order of
+//     operations so that A stays exactly symmetric; the eigenvector matrix takes V R.  The
+//     rotations are formed inside the step by the threads that use them, one launch per step.
 // The matrix is kept in "pair layout": the two members of pair i sit at positions 2i and
 // 2i+1, so each step reads adjacent columns, and the apply kernel writes its results
 // straight into the next step's layout (the circle rotation sigma).  Odd n is padded by a
@@ -47,10 +48,9 @@ struct Rot {
   int pad;
 };
 
-__global__ void k_jac_rot(const double* __restrict__ ah, const double* __restrict__ al, long N2, Rot* __restrict__ rot) {
-  const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
-  const long m = N2 / 2;
-  if (i >= m) return;
+// the rotation of pair i (positions 2i, 2i+1) from A's diagonal 2 x 2 block
+__device__ __forceinline__ Rot pair_rot(const double* __restrict__ ah, const double* __restrict__ al, long N2,
+                                        long i) {
   const long p = 2 * i, q = 2 * i + 1;
   const dd app = mkdd(ah[p * N2 + p], al[p * N2 + p]);
   const dd aqq = mkdd(ah[q * N2 + q], al[q * N2 + q]);
@@ -77,67 +77,103 @@ __global__ void k_jac_rot(const double* __restrict__ ah, const double* __restric
   }
   r.ch = c.hi; r.cl = c.lo; r.sh = s.hi; r.sl = s.lo;
   r.ph = np.hi; r.pl = np.lo; r.qh = nq.hi; r.ql = nq.lo;
-  rot[i] = r;
+  return r;
 }
 
-// z = 0: A (pair layout) -> B (next layout), one thread per 2 x 2 block (I <= J);
-// z = 1: V -> W, one thread per row r and pair i
-__global__ __launch_bounds__(64) void k_jac_apply(const double* __restrict__ ah, const double* __restrict__ al,
+// One round-robin step, one launch.  z = 0: A (pair layout) -> B (next layout); a block
+// takes the 2 x 2 blocks (I, J) of 4 WI pairs I by 64 pairs J (lane = J, each wavefront WI
+// of the I).  z = 1: V -> W; a block takes 64 pairs by 4 WR rows (lane = pair, each
+// wavefront WR rows).
+// The rotations a block uses are formed once, one per lane in parallel, and shared through
+// LDS.  An upper block (I < J) takes its columns by R_J and then its rows by R_I; a lower
+// block takes its rows by R_I and then its columns by R_J, which is the upper computation
+// transposed operation for operation, so A stays exactly symmetric without mirror writes.
+template <int WI, int WR>  // pairs I per wavefront (z = 0), rows per wavefront (z = 1)
+__global__ __launch_bounds__(256) void k_jac_step(const double* __restrict__ ah, const double* __restrict__ al,
                                                   double* __restrict__ bh, double* __restrict__ bl,
                                                   const double* __restrict__ vh, const double* __restrict__ vl,
-                                                  double* __restrict__ wh, double* __restrict__ wl, long N2,
-                                                  const Rot* __restrict__ rot) {
+                                                  double* __restrict__ wh, double* __restrict__ wl, long N2) {
+  constexpr int ST_I = 4 * WI, ST_R = 4 * WR;
+  __shared__ Rot rs[64 + ST_I];
   const long m = N2 / 2;
-  const long x = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  const int t = threadIdx.x, lane = t & 63, w = t >> 6;
   if (blockIdx.z == 1) {
-    const long r = blockIdx.y, i = x;
-    if (r >= N2 || i >= m) return;
-    const Rot R = rot[i];
+    const long i0 = (long)blockIdx.x * 64, r0 = (long)blockIdx.y * ST_R;
+    if (r0 >= N2) return;  // the whole block
+    if (t < 64 && i0 + t < m) rs[t] = pair_rot(ah, al, N2, i0 + t);
+    __syncthreads();
+    const long i = i0 + lane;
+    if (i >= m) return;
+    const Rot R = rs[lane];
     const dd c = mkdd(R.ch, R.cl), s = mkdd(R.sh, R.sl);
-    const dd v0 = mkdd(vh[r * N2 + 2 * i], vl[r * N2 + 2 * i]);
-    const dd v1 = mkdd(vh[r * N2 + 2 * i + 1], vl[r * N2 + 2 * i + 1]);
-    const dd w0 = dd_sub(dd_mul(c, v0), dd_mul(s, v1));
-    const dd w1 = dd_add(dd_mul(s, v0), dd_mul(c, v1));
-    const long o0 = r * N2 + rr_sigma(2 * i, m), o1 = r * N2 + rr_sigma(2 * i + 1, m);
-    wh[o0] = w0.hi; wl[o0] = w0.lo;
-    wh[o1] = w1.hi; wl[o1] = w1.lo;
-    return;
-  }
-  const long I = blockIdx.y, J = x;
-  if (I >= m || J >= m || J < I) return;
-  const long p0 = 2 * I, p1 = p0 + 1, q0 = 2 * J, q1 = q0 + 1;
-  dd x00 = mkdd(ah[p0 * N2 + q0], al[p0 * N2 + q0]), x01 = mkdd(ah[p0 * N2 + q1], al[p0 * N2 + q1]);
-  dd x10 = mkdd(ah[p1 * N2 + q0], al[p1 * N2 + q0]), x11 = mkdd(ah[p1 * N2 + q1], al[p1 * N2 + q1]);
-  const Rot RI = rot[I];
-  const long np0 = rr_sigma(p0, m), np1 = rr_sigma(p1, m), nq0 = rr_sigma(q0, m), nq1 = rr_sigma(q1, m);
-  if (I == J) {
-    if (RI.active) {
-      x00 = mkdd(RI.ph, RI.pl);
-      x11 = mkdd(RI.qh, RI.ql);
-      x01 = mkdd(0.0);
-      x10 = mkdd(0.0);
+    const long o0 = rr_sigma(2 * i, m), o1 = rr_sigma(2 * i + 1, m);
+#pragma unroll
+    for (int k = 0; k < WR; ++k) {
+      const long r = r0 + w * WR + k;
+      if (r >= N2) continue;
+      const dd v0 = mkdd(vh[r * N2 + 2 * i], vl[r * N2 + 2 * i]);
+      const dd v1 = mkdd(vh[r * N2 + 2 * i + 1], vl[r * N2 + 2 * i + 1]);
+      const dd w0 = dd_sub(dd_mul(c, v0), dd_mul(s, v1));
+      const dd w1 = dd_add(dd_mul(s, v0), dd_mul(c, v1));
+      wh[r * N2 + o0] = w0.hi; wl[r * N2 + o0] = w0.lo;
+      wh[r * N2 + o1] = w1.hi; wl[r * N2 + o1] = w1.lo;
     }
-    bh[np0 * N2 + np0] = x00.hi; bl[np0 * N2 + np0] = x00.lo;
-    bh[np0 * N2 + np1] = x01.hi; bl[np0 * N2 + np1] = x01.lo;
-    bh[np1 * N2 + np0] = x10.hi; bl[np1 * N2 + np0] = x10.lo;
-    bh[np1 * N2 + np1] = x11.hi; bl[np1 * N2 + np1] = x11.lo;
     return;
   }
-  const Rot RJ = rot[J];
-  const dd cI = mkdd(RI.ch, RI.cl), sI = mkdd(RI.sh, RI.sl), cJ = mkdd(RJ.ch, RJ.cl), sJ = mkdd(RJ.sh, RJ.sl);
-  // columns q0, q1 by rotation J, then rows p0, p1 by rotation I
-  const dd c00 = dd_sub(dd_mul(cJ, x00), dd_mul(sJ, x01)), c01 = dd_add(dd_mul(sJ, x00), dd_mul(cJ, x01));
-  const dd c10 = dd_sub(dd_mul(cJ, x10), dd_mul(sJ, x11)), c11 = dd_add(dd_mul(sJ, x10), dd_mul(cJ, x11));
-  const dd y00 = dd_sub(dd_mul(cI, c00), dd_mul(sI, c10)), y10 = dd_add(dd_mul(sI, c00), dd_mul(cI, c10));
-  const dd y01 = dd_sub(dd_mul(cI, c01), dd_mul(sI, c11)), y11 = dd_add(dd_mul(sI, c01), dd_mul(cI, c11));
-  bh[np0 * N2 + nq0] = y00.hi; bl[np0 * N2 + nq0] = y00.lo;
-  bh[np0 * N2 + nq1] = y01.hi; bl[np0 * N2 + nq1] = y01.lo;
-  bh[np1 * N2 + nq0] = y10.hi; bl[np1 * N2 + nq0] = y10.lo;
-  bh[np1 * N2 + nq1] = y11.hi; bl[np1 * N2 + nq1] = y11.lo;
-  bh[nq0 * N2 + np0] = y00.hi; bl[nq0 * N2 + np0] = y00.lo;
-  bh[nq1 * N2 + np0] = y01.hi; bl[nq1 * N2 + np0] = y01.lo;
-  bh[nq0 * N2 + np1] = y10.hi; bl[nq0 * N2 + np1] = y10.lo;
-  bh[nq1 * N2 + np1] = y11.hi; bl[nq1 * N2 + np1] = y11.lo;
+  const long J0 = (long)blockIdx.x * 64, I0 = (long)blockIdx.y * ST_I;
+  if (I0 >= m) return;  // the whole block
+  if (t < 64) {
+    if (J0 + t < m) rs[t] = pair_rot(ah, al, N2, J0 + t);
+  } else if (t < 64 + ST_I) {
+    if (I0 + t - 64 < m) rs[t] = pair_rot(ah, al, N2, I0 + t - 64);
+  }
+  __syncthreads();
+  const long J = J0 + lane;
+  if (J >= m) return;
+  const Rot RJ = rs[lane];
+  const dd cJ = mkdd(RJ.ch, RJ.cl), sJ = mkdd(RJ.sh, RJ.sl);
+  const long q0 = 2 * J, q1 = q0 + 1, nq0 = rr_sigma(q0, m), nq1 = rr_sigma(q1, m);
+#pragma unroll
+  for (int k = 0; k < WI; ++k) {
+    const long I = I0 + w * WI + k;
+    if (I >= m) continue;
+    const Rot RI = rs[64 + w * WI + k];
+    const long p0 = 2 * I, p1 = p0 + 1;
+    dd x00 = mkdd(ah[p0 * N2 + q0], al[p0 * N2 + q0]), x01 = mkdd(ah[p0 * N2 + q1], al[p0 * N2 + q1]);
+    dd x10 = mkdd(ah[p1 * N2 + q0], al[p1 * N2 + q0]), x11 = mkdd(ah[p1 * N2 + q1], al[p1 * N2 + q1]);
+    const long np0 = rr_sigma(p0, m), np1 = rr_sigma(p1, m);
+    dd y00, y01, y10, y11;
+    if (I == J) {
+      if (RI.active) {
+        x00 = mkdd(RI.ph, RI.pl);
+        x11 = mkdd(RI.qh, RI.ql);
+        x01 = mkdd(0.0);
+        x10 = mkdd(0.0);
+      }
+      y00 = x00; y01 = x01; y10 = x10; y11 = x11;
+    } else {
+      const dd cI = mkdd(RI.ch, RI.cl), sI = mkdd(RI.sh, RI.sl);
+      if (I < J) {  // columns q0, q1 by rotation J, then rows p0, p1 by rotation I
+        const dd c00 = dd_sub(dd_mul(cJ, x00), dd_mul(sJ, x01)), c01 = dd_add(dd_mul(sJ, x00), dd_mul(cJ, x01));
+        const dd c10 = dd_sub(dd_mul(cJ, x10), dd_mul(sJ, x11)), c11 = dd_add(dd_mul(sJ, x10), dd_mul(cJ, x11));
+        y00 = dd_sub(dd_mul(cI, c00), dd_mul(sI, c10));
+        y10 = dd_add(dd_mul(sI, c00), dd_mul(cI, c10));
+        y01 = dd_sub(dd_mul(cI, c01), dd_mul(sI, c11));
+        y11 = dd_add(dd_mul(sI, c01), dd_mul(cI, c11));
+      } else {  // rows p0, p1 by rotation I, then columns q0, q1 by rotation J
+        const dd r00 = dd_sub(dd_mul(cI, x00), dd_mul(sI, x10)), r10 = dd_add(dd_mul(sI, x00), dd_mul(cI, x10));
+        const dd r01 = dd_sub(dd_mul(cI, x01), dd_mul(sI, x11)), r11 = dd_add(dd_mul(sI, x01), dd_mul(cI, x11));
+        y00 = dd_sub(dd_mul(cJ, r00), dd_mul(sJ, r01));
+        y01 = dd_add(dd_mul(sJ, r00), dd_mul(cJ, r01));
+        y10 = dd_sub(dd_mul(cJ, r10), dd_mul(sJ, r11));
+        y11 = dd_add(dd_mul(sJ, r10), dd_mul(cJ, r11));
+      }
+    }
+    bh[np0 * N2 + nq0] = y00.hi; bl[np0 * N2 + nq0] = y00.lo;
+    bh[np0 * N2 + nq1] = y01.hi; bl[np0 * N2 + nq1] = y01.lo;
+    bh[np1 * N2 + nq0] = y10.hi; bl[np1 * N2 + nq0] = y10.lo;
+    bh[np1 * N2 + nq1] = y11.hi; bl[np1 * N2 + nq1] = y11.lo;
+  }
 }
 
 // off-diagonal size of A: sum of hi^2 over a < b (for the Frobenius stop test) and
@@ -260,11 +296,11 @@ int jacobi_eigen_dev(const double* a_h, const double* a_l, long n, bool stop_fro
     return HP_EINVAL;
   }
   const size_t nn = (size_t)N2 * N2;
-  DevBuf A0, A1, V0, V1, rot, red, posb;
+  DevBuf A0, A1, V0, V1, red, posb;
   int rc;
   if ((rc = A0.alloc(2 * nn * sizeof(double))) || (rc = A1.alloc(2 * nn * sizeof(double))) ||
       (rc = V0.alloc(2 * nn * sizeof(double))) || (rc = V1.alloc(2 * nn * sizeof(double))) ||
-      (rc = rot.alloc(m * sizeof(Rot))) || (rc = red.alloc(16)) || (rc = posb.alloc(N2 * sizeof(long))))
+      (rc = red.alloc(16)) || (rc = posb.alloc(N2 * sizeof(long))))
     return rc;
   double *ah = A0.d(), *al = ah + nn, *bh = A1.d(), *bl = bh + nn;
   double *vh = V0.d(), *vl = vh + nn, *wh = V1.d(), *wl = wh + nn;
@@ -290,7 +326,7 @@ int jacobi_eigen_dev(const double* a_h, const double* a_l, long n, bool stop_fro
     *mxv = mv;
     return HP_OK;
   };
-  // A sweep is 2 (N2 - 1) launches; each is captured once into a graph per buffer parity
+  // A sweep is N2 - 1 launches; each is captured once into a graph per buffer parity
   // (N2 - 1 is odd, so consecutive sweeps start from opposite buffers) and replayed.
   struct Graphs {
     hipStream_t cs = nullptr;
@@ -306,9 +342,14 @@ int jacobi_eigen_dev(const double* a_h, const double* a_l, long n, bool stop_fro
     double *xh = ah, *xl = al, *yh = bh, *yl = bl, *uh = vh, *ul = vl, *zh = wh, *zl = wl;
     HP_HIP(hipStreamBeginCapture(gr.cs, hipStreamCaptureModeThreadLocal));
     for (long step = 0; step < N2 - 1; ++step) {
-      hipLaunchKernelGGL(k_jac_rot, dim3(blocks(m, 64)), dim3(64), 0, gr.cs, xh, xl, N2, static_cast<Rot*>(rot.p));
-      hipLaunchKernelGGL(k_jac_apply, dim3(blocks(m, 64), (unsigned)N2, 2), dim3(64), 0, gr.cs, xh, xl, yh, yl, uh,
-                         ul, zh, zl, N2, static_cast<const Rot*>(rot.p));
+      // small matrices: one I per wavefront for enough blocks; larger ones: four, which
+      // forms fewer copies of each rotation
+      if (m <= 512)
+        hipLaunchKernelGGL((k_jac_step<1, 4>), dim3(blocks(m, 64), std::max(blocks(m, 4), blocks(N2, 16)), 2),
+                           dim3(256), 0, gr.cs, xh, xl, yh, yl, uh, ul, zh, zl, N2);
+      else
+        hipLaunchKernelGGL((k_jac_step<4, 16>), dim3(blocks(m, 64), std::max(blocks(m, 16), blocks(N2, 64)), 2),
+                           dim3(256), 0, gr.cs, xh, xl, yh, yl, uh, ul, zh, zl, N2);
       std::swap(xh, yh);
       std::swap(xl, yl);
       std::swap(uh, zh);
