@@ -331,6 +331,145 @@ def bench_text(qp, torch, dev, cpu, reps=5):
 
 
 TEXT_N, TEXT_CPU_N = 1 << 22, 1 << 19
+
+
+def molecule(seed, n_atoms, diffuse=True):
+    """a jittered planar lattice of atoms (1.6 bohr), each with s, s, p, d, f shells of 1-3
+    normalised primitives, exponents 10^U(-1.6 or -0.5, 1.8)"""
+    import numpy as np
+    rng = np.random.default_rng(seed)
+    side = int(np.ceil(np.sqrt(n_atoms)))
+    shells = []
+    for a in range(n_atoms):
+        c = (1.6 * (a % side) + rng.uniform(-0.2, 0.2), 1.6 * (a // side) + rng.uniform(-0.2, 0.2),
+             rng.uniform(-0.3, 0.3))
+        for l in (0, 0, 1, 2, 3):
+            npr = int(rng.integers(1, 4))
+            alphas = np.sort(10 ** rng.uniform(-1.6 if diffuse else -0.5, 1.8, npr))[::-1]
+            shells.append((c, l, [(float(x), float(rng.uniform(0.1, 1.0) * (2 * x / np.pi) ** 0.75))
+                                  for x in alphas]))
+    return shells
+
+
+def bench_overlap_mixed(qp, torch, dev, cpu, reps=3):
+    """assemble_overlap_mixed (crates/overlap/src/assemble.rs:35-85) over OVL_ATOMS atoms of
+    s, s, p, d, f contracted shells (n_ao = 21 per atom), the default screening; S written
+    to HBM (hp_dev_assemble_overlap_mixed).  Parity: tests/test_gpu_overlap_shells.py."""
+    import ctypes as C
+    import numpy as np
+    L = qp.lib()
+
+    def flat(shells):
+        centers = np.ascontiguousarray([c for c, _, _ in shells], dtype=np.float64).ravel()
+        ls = np.ascontiguousarray([l for _, l, _ in shells], dtype=np.int32)
+        off = np.zeros(len(shells) + 1, dtype=np.int64)
+        for i, (_, _, prims) in enumerate(shells):
+            off[i + 1] = off[i] + len(prims)
+        alpha = np.ascontiguousarray([a for _, _, pr in shells for a, _ in pr], dtype=np.float64)
+        coeff = np.ascontiguousarray([c for _, _, pr in shells for _, c in pr], dtype=np.float64)
+        return centers, ls, off, alpha, coeff, int(sum((l + 1) * (l + 2) // 2 for l in ls))
+
+    shells = molecule(5, OVL_ATOMS)
+    centers, ls, off, alpha, coeff, n_ao = flat(shells)
+    prm = np.ascontiguousarray(qp.SCREEN_DEFAULT, dtype=np.float64)
+    s = torch.empty(n_ao * n_ao, dtype=torch.float64, device=dev)
+    counts = (C.c_size_t * 3)()
+    ts = []
+    for _ in range(reps + 1):
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        rc = L.hp_dev_assemble_overlap_mixed(centers, ls, off, alpha, coeff, len(shells), prm, s.data_ptr(), n_ao,
+                                             counts, None)
+        torch.cuda.synchronize()
+        ts.append(time.perf_counter() - t0)
+        if rc:
+            raise RuntimeError(f"hp_dev_assemble_overlap_mixed: status {rc}")
+    t = min(ts[1:])
+    elems = n_ao * (n_ao + 1) / 2  # the computed half; the mirror is a second store
+    out = {"workload": f"assemble_overlap_mixed, {OVL_ATOMS} atoms x (s, s, p, d, f) shells of 1-3 primitives "
+                       f"(seed 5), n_ao {n_ao}, default screening",
+           "ms": round(t * 1e3, 3), "shells": len(shells), "n_ao": n_ao,
+           "tiers": {"f64_pairs": counts[0], "f128_pairs": counts[1], "skipped_pairs": counts[2]},
+           "Melements_per_s": round(elems / t / 1e6, 1),
+           "roofline": {"bound": "valu", "hbm_write_GB_per_s": round(8.0 * n_ao * n_ao / t / 1e9, 1),
+                        "note": "FP64 VALU: the contracted Obara-Saika element (1-9 primitive pairs, f64 or dd) "
+                                "per thread; the S write is 8 bytes per element"}}
+    if cpu:
+        sys.path.insert(0, os.path.join(ROOT, "tests"))
+        import oracle_lib as O
+        small = molecule(5, OVL_CPU_ATOMS)
+        n_small = sum((l + 1) * (l + 2) // 2 for _, l, _ in small)
+        t0 = time.perf_counter()
+        O.assemble_overlap_mixed(small, O.DEFAULT_SCREEN)
+        tc = time.perf_counter() - t0
+        out["cpu_baseline"] = {"value": round(n_small * (n_small + 1) / 2 / tc / 1e6, 3), "unit": "Melements/s",
+                               "cores": 1, "kind": "port",
+                               "sample": f"{OVL_CPU_ATOMS} atoms (n_ao {n_small}) of the same kind, oracle C++, "
+                                         f"one thread ({tc:.2f} s)"}
+    return out
+
+
+def bench_reduce(qp, torch, dev, cpu, reps=5):
+    """dot (linalg.rs:7-18) of RED_N dd values and gemv (linalg.rs:29-40) of a RED_ROWS^2 dd
+    matrix, both resident in HBM; the reference's summation order (bit-identical to the
+    oracle, tests/test_gpu_reduce.py)."""
+    import ctypes as C
+    import numpy as np
+    L = qp.lib()
+    n, r = RED_N, RED_ROWS
+    a = qp.dev.to_planes(qp.gen_dd_normal(7, n), 2, dev)
+    b = qp.dev.to_planes(qp.gen_dd_normal(8, n), 2, dev)
+    o = torch.zeros(2, dtype=torch.float64, device=dev)
+    A = qp.dev.to_planes(qp.gen_dd_normal(9, r * r), 2, dev)
+    x = qp.dev.to_planes(qp.gen_dd_normal(10, r), 2, dev)
+    y = torch.zeros((2, r), dtype=torch.float64, device=dev)
+
+    def best(fn):
+        fn()
+        ts = []
+        for _ in range(reps):
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            fn()
+            torch.cuda.synchronize()
+            ts.append(time.perf_counter() - t0)
+        return min(ts)
+
+    def dot():
+        if L.hp_dev_dot(a[0].data_ptr(), a[1].data_ptr(), b[0].data_ptr(), b[1].data_ptr(), n, o.data_ptr(),
+                        o.data_ptr() + 8, None):
+            raise RuntimeError("hp_dev_dot")
+
+    def gemv():
+        if L.hp_dev_gemv(A[0].data_ptr(), A[1].data_ptr(), r, r, x[0].data_ptr(), x[1].data_ptr(), y[0].data_ptr(),
+                         y[1].data_ptr(), None):
+            raise RuntimeError("hp_dev_gemv")
+
+    td, tg = best(dot), best(gemv)
+    out = {"workload": f"dd dot of {n} values, dd gemv {r} x {r} (gen_dd_normal seeds 7-10), in HBM",
+           "dot": {"ms": round(td * 1e3, 3), "GB_per_s": round(32.0 * n / td / 1e9, 1),
+                   "note": "1024-element chunks, one thread each (the reference's order), then a left fold: "
+                           "bound by the per-thread dependent dd chain, not HBM"},
+           "gemv": {"ms": round(tg * 1e3, 3), "GB_per_s": round(16.0 * r * r / tg / 1e9, 1),
+                    "roofline_frac_hbm": round(16.0 * r * r / tg / 8e12, 4),
+                    "note": "one thread per row through 64 x 64 LDS tiles; 16 bytes of A per dd MAC"}}
+    if cpu:
+        sys.path.insert(0, os.path.join(ROOT, "tests"))
+        import oracle_lib as O
+        ah = np.stack([a[0].cpu().numpy(), a[1].cpu().numpy()], axis=1)
+        bh = np.stack([b[0].cpu().numpy(), b[1].cpu().numpy()], axis=1)
+        t0 = time.perf_counter()
+        ref = O.dot(ah, bh)
+        t1 = time.perf_counter()
+        assert np.array_equal(ref, o.cpu().numpy())
+        out["cpu_baseline"] = {"value": round(32.0 * n / (t1 - t0) / 1e9, 2), "unit": "GB/s (dot)", "cores": 1,
+                               "kind": "port", "sample": f"the same {n}-value dot, oracle C++, one thread "
+                                                         f"({t1 - t0:.2f} s), result bit-identical to the GPU's"}
+    return out
+
+
+OVL_ATOMS, OVL_CPU_ATOMS = 400, 400
+RED_N, RED_ROWS = 1 << 24, 8192
 EIG_N, EIG_CPU_N = 512, 192
 ADJ_N, ADJ_SEED, ADJ_B_SEED = 2048, 4, 44
 
@@ -412,6 +551,8 @@ def main():
     ap.add_argument("--no-adjoint", action="store_true", help="skip the N=2048 adjoint")
     ap.add_argument("--no-eigen", action="store_true", help="skip the N=512 generalized eigensolve")
     ap.add_argument("--no-text", action="store_true", help="skip the dd serde text round trip")
+    ap.add_argument("--no-overlap", action="store_true", help="skip the mixed-tier overlap assembly")
+    ap.add_argument("--no-reduce", action="store_true", help="skip dot / gemv")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only copy MIB MiB device-to-device three times (a plain copy kernel of known size, "
                          "to calibrate rocprofv3's FETCH_SIZE / WRITE_SIZE counters) and exit")
@@ -509,6 +650,10 @@ def main():
         out["gen_eigen_n512"] = bench_eigen(qp, torch, dev, not args.no_cpu)
     if not args.no_text:
         out["dd_serde_text_4m"] = bench_text(qp, torch, dev, not args.no_cpu)
+    if not args.no_overlap:
+        out["overlap_mixed_400_atoms"] = bench_overlap_mixed(qp, torch, dev, not args.no_cpu)
+    if not args.no_reduce:
+        out["dot_gemv"] = bench_reduce(qp, torch, dev, not args.no_cpu)
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(a, b)
     print(json.dumps(out), flush=True)

@@ -211,6 +211,11 @@ int hp_dev_overlap_s(const double* centers, const double* alpha, size_t n, doubl
 int hp_assemble_overlap_mixed(const double* centers, const int* l, const long* prim_off, const double* alpha,
                               const double* coeff, size_t n_shells, const double* params4, double* s, size_t n_ao,
                               size_t* counts);
+/* the same with S left in device memory (s_dev: n_ao x n_ao f64), for device-resident
+ * consumers such as the factorizations; shell data and counts are host memory */
+int hp_dev_assemble_overlap_mixed(const double* centers, const int* l, const long* prim_off, const double* alpha,
+                                  const double* coeff, size_t n_shells, const double* params4, double* s_dev,
+                                  size_t n_ao, size_t* counts, void* stream);
 
 /* ------------------------------------------------------------------ eigensolvers */
 

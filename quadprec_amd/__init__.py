@@ -121,6 +121,9 @@ def lib():
     L.hp_assemble_overlap_mixed.argtypes = [_dp, np.ctypeslib.ndpointer(dtype=np.int32, flags="C_CONTIGUOUS"),
                                             np.ctypeslib.ndpointer(dtype=np.int64, flags="C_CONTIGUOUS"),
                                             _dp, _dp, _sz, _dp, _dp, _sz, C.POINTER(C.c_size_t)]
+    L.hp_dev_assemble_overlap_mixed.argtypes = [_dp, np.ctypeslib.ndpointer(dtype=np.int32, flags="C_CONTIGUOUS"),
+                                                np.ctypeslib.ndpointer(dtype=np.int64, flags="C_CONTIGUOUS"),
+                                                _dp, _dp, _sz, _dp, _vp, _sz, C.POINTER(C.c_size_t), _vp]
     L.hp_jacobi_eigen.argtypes = [_dp, _sz, _sz, _dp, _dp]
     L.hp_dev_jacobi_eigen.argtypes = [_vp, _vp, _sz, _sz, C.c_int, _vp, _vp, _vp, _vp, C.c_int,
                                       C.POINTER(C.c_size_t), _vp]

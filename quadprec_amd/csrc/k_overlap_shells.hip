@@ -1,14 +1,14 @@
 // k_overlap_shells.hip -- assemble_overlap_mixed over contracted s/p/d/f shells
 // (crates/overlap/src/assemble.rs:35-85, screen.rs:57-101, integral.rs:65-346).
 //
-// One thread per element (a, b) of the n_ao x n_ao matrix whose shell pair (s_a, s_b) has
-// s_a >= s_b; it screens the pair (every thread of a pair reaches the same tier), forms its
-// Cartesian component of the contracted block in the tier's arithmetic and writes it with
-// its mirror.  The first element of each pair counts the tier.  Per element the operations
-// are the reference's, in its order: the primitive-pair sums run p outer, q inner; the 1D
-// Obara-Saika tables are built as integral.rs builds them (statically unrolled 4 x 4 so they
-// stay in registers).  libm exp (glibc on the host, the device library here) is the one
-// operation whose last bit may differ.
+// Three launches: every shell pair (i >= j) is screened once and listed by (tier, l_i, l_j);
+// one thread per (listed pair, row u of its block) forms the row of v components in the
+// tier's arithmetic, templated on l_i, l_j and the tier so that no wavefront diverges; the
+// upper triangle of distinct-shell blocks is then copied from the lower through LDS tiles.
+// Per component the operations are the reference's, in its order: primitive pairs p outer,
+// q inner, the reference's set-up and 1D Obara-Saika recurrences, the same expression for
+// each term.  libm exp (glibc on the host, the device library here) is the one operation
+// whose last bit may differ.
 #include <cmath>
 #include <vector>
 
@@ -66,75 +66,6 @@ __device__ __forceinline__ void cart(int l, int u, int& lx, int& ly, int& lz) {
     }
 }
 
-// entry (ei, ej) of obara_saika_1d_f64 (integral.rs:158-185)
-__device__ double os1d_f64(int la, int lb, double pa, double pb, double gamma, int ei, int ej) {
-  double s[4][4];
-  const double oo2g = 0.5 / gamma;
-#pragma unroll
-  for (int i = 0; i < 4; ++i)
-#pragma unroll
-    for (int j = 0; j < 4; ++j) s[i][j] = 0.0;
-  s[0][0] = 1.0;
-#pragma unroll
-  for (int i = 0; i < 3; ++i)
-    if (i < la) s[i + 1][0] = pa * s[i][0] + oo2g * (double)i * (i > 0 ? s[i > 0 ? i - 1 : 0][0] : 0.0);
-#pragma unroll
-  for (int j = 0; j < 3; ++j) {
-    if (j >= lb) continue;
-#pragma unroll
-    for (int i = 0; i < 4; ++i) {
-      if (i > la) continue;
-      const double tim1 = oo2g * (double)i * (i > 0 ? s[i > 0 ? i - 1 : 0][j] : 0.0);
-      const double tjm1 = oo2g * (double)j * (j > 0 ? s[i][j > 0 ? j - 1 : 0] : 0.0);
-      s[i][j + 1] = pb * s[i][j] + tim1 + tjm1;
-    }
-  }
-  double r = 0.0;
-#pragma unroll
-  for (int i = 0; i < 4; ++i)
-#pragma unroll
-    for (int j = 0; j < 4; ++j)
-      if (i == ei && j == ej) r = s[i][j];
-  return r;
-}
-
-// entry (ei, ej) of obara_saika_1d_f128 (integral.rs:187-216)
-__device__ dd os1d_dd(int la, int lb, dd pa, dd pb, dd gamma, int ei, int ej) {
-  dd s[4][4];
-  const dd oo2g = dd_div(mkdd(0.5), gamma);
-#pragma unroll
-  for (int i = 0; i < 4; ++i)
-#pragma unroll
-    for (int j = 0; j < 4; ++j) s[i][j] = mkdd(0.0);
-  s[0][0] = mkdd(1.0);
-#pragma unroll
-  for (int i = 0; i < 3; ++i) {
-    if (i >= la) continue;
-    const dd prev = i > 0 ? s[i > 0 ? i - 1 : 0][0] : mkdd(0.0);
-    s[i + 1][0] = dd_add(dd_mul(pa, s[i][0]), dd_mul(dd_mul(oo2g, mkdd((double)i)), prev));
-  }
-#pragma unroll
-  for (int j = 0; j < 3; ++j) {
-    if (j >= lb) continue;
-    const dd fj = mkdd((double)j);
-#pragma unroll
-    for (int i = 0; i < 4; ++i) {
-      if (i > la) continue;
-      const dd fi = mkdd((double)i);
-      const dd im1 = i > 0 ? s[i > 0 ? i - 1 : 0][j] : mkdd(0.0);
-      const dd jm1 = j > 0 ? s[i][j > 0 ? j - 1 : 0] : mkdd(0.0);
-      s[i][j + 1] = dd_add(dd_add(dd_mul(pb, s[i][j]), dd_mul(dd_mul(oo2g, fi), im1)), dd_mul(dd_mul(oo2g, fj), jm1));
-    }
-  }
-  dd r = mkdd(0.0);
-#pragma unroll
-  for (int i = 0; i < 4; ++i)
-#pragma unroll
-    for (int j = 0; j < 4; ++j)
-      if (i == ei && j == ej) r = s[i][j];
-  return r;
-}
-
 // overlap_primitive_f64 (integral.rs:15-25)
 __device__ double prim_f64(double ai, const double* ci, double aj, const double* cj) {
   const double gamma = ai + aj;
@@ -158,20 +89,169 @@ __device__ dd prim_dd(double ai, const double* ci, double aj, const double* cj) 
   return dd_mul(prefactor, exp_val);
 }
 
-// component (u, v) of the contracted block of shell pair (A, B) in the given tier
-__device__ double element(const ShellRec& A, const ShellRec& B, int u, int v, int tier,
-                          const double* __restrict__ alpha, const double* __restrict__ coeff) {
-  if (tier == 0) {
-    if (A.l == 0 && B.l == 0) {  // integral.rs:99-110
+// ---- assembly in three launches:
+//   k_classify: every shell pair (i >= j) screened once (screen.rs:57-101), counted by tier
+//     and listed by (tier, l_i, l_j), so that the next launches run without divergence;
+//   k_rows<LA, LB, TIER>: one thread per (pair, row u of the block): per primitive pair the
+//     reference's set-up and its three 1D Obara-Saika tables, then the whole row of v
+//     components (integral.rs:65-346); every component accumulates over the primitive
+//     pairs p outer, q inner, with the reference's expression;
+//   k_mirror: the upper triangle of distinct-shell blocks copied from the lower through
+//     64 x 64 LDS tiles (assemble.rs:74-78 writes the mirror; a shell's own block is
+//     computed whole, as in the reference).
+constexpr int NBUCKET = 32;  // tier (0 f64, 1 dd) x l of the row shell x l of the column shell
+
+__device__ __forceinline__ void decode_pair(long k, long& i, long& j) {
+  i = (long)((sqrt(8.0 * (double)k + 1.0) - 1.0) * 0.5);
+  while (i * (i + 1) / 2 > k) --i;
+  while ((i + 1) * (i + 2) / 2 <= k) ++i;
+  j = k - i * (i + 1) / 2;
+}
+
+template <bool FILL>
+__global__ __launch_bounds__(256) void k_classify(const ShellRec* __restrict__ sh, long npairs,
+                                                  const double* __restrict__ prm,
+                                                  unsigned long long* __restrict__ tiers,
+                                                  unsigned int* __restrict__ bcount,
+                                                  const unsigned int* __restrict__ bstart, int2* __restrict__ list) {
+  const long k = (long)blockIdx.x * 256 + threadIdx.x;
+  const int lane = threadIdx.x & 63;
+  int tier = 3, bucket = -1;
+  long i = 0, j = 0;
+  if (k < npairs) {
+    decode_pair(k, i, j);
+    const ShellRec A = sh[i], B = sh[j];
+    tier = screen(A, B, prm);
+    if (tier < 2) bucket = tier * 16 + A.l * 4 + B.l;
+  }
+  if (!FILL) {
+#pragma unroll
+    for (int t = 0; t < 3; ++t) {
+      const unsigned long long m = __ballot(tier == t);
+      if (lane == 0 && m) atomicAdd(&tiers[t], (unsigned long long)__popcll(m));
+    }
+  }
+  unsigned long long active = __ballot(bucket >= 0);
+  while (active) {  // one atomic per distinct bucket of the wavefront
+    const int leader = __ffsll((long long)active) - 1;
+    const int bk = __shfl(bucket, leader);
+    const unsigned long long same = __ballot(bucket == bk);
+    unsigned int base = 0;
+    if (lane == leader) base = atomicAdd(&bcount[bk], (unsigned int)__popcll(same));
+    base = __shfl(base, leader);
+    if (FILL && bucket == bk) {
+      const unsigned int rank = (unsigned int)__popcll(same & ((1ull << lane) - 1ull));
+      list[bstart[bk] + base + rank] = make_int2((int)i, (int)j);
+    }
+    active &= ~same;
+  }
+}
+
+// Row ei (runtime, <= LA) of the 1D Obara-Saika table s[i][j], i <= LA, j <= LB, of
+// obara_saika_1d_f64 / _f128 (integral.rs:158-216), with their expressions.  Entry (i, j)
+// uses only (i, j - 1), (i - 1, j) and, in column 0, (i - 1, 0), (i - 2, 0), so building
+// the table row by row gives every entry the value the reference's column order gives;
+// only the previous row is kept, and the wanted row is picked as a whole.
+template <int LA, int LB>
+__device__ __forceinline__ void row_f64(double pa, double pb, double gamma, int ei, double (&out)[LB + 1]) {
+  const double oo2g = 0.5 / gamma;
+  double pm[LB + 1], cur[LB + 1], pm2_0 = 0.0;
+#pragma unroll
+  for (int j = 0; j <= LB; ++j) pm[j] = 0.0;
+#pragma unroll
+  for (int i = 0; i <= LA; ++i) {
+    cur[0] = i == 0 ? 1.0 : pa * pm[0] + oo2g * (double)(i - 1) * (i - 1 > 0 ? pm2_0 : 0.0);
+#pragma unroll
+    for (int j = 0; j < LB; ++j) {
+      const double tim1 = oo2g * (double)i * (i > 0 ? pm[j] : 0.0);
+      const double tjm1 = oo2g * (double)j * (j > 0 ? cur[j > 0 ? j - 1 : 0] : 0.0);
+      cur[j + 1] = pb * cur[j] + tim1 + tjm1;
+    }
+#pragma unroll
+    for (int j = 0; j <= LB; ++j) out[j] = (i == 0 || ei == i) ? cur[j] : out[j];
+    pm2_0 = pm[0];
+#pragma unroll
+    for (int j = 0; j <= LB; ++j) pm[j] = cur[j];
+  }
+}
+template <int LA, int LB>
+__device__ __forceinline__ void row_dd(dd pa, dd pb, dd gamma, int ei, dd (&out)[LB + 1]) {
+  const dd oo2g = dd_div(mkdd(0.5), gamma);
+  dd pm[LB + 1], cur[LB + 1], pm2_0 = mkdd(0.0);
+#pragma unroll
+  for (int j = 0; j <= LB; ++j) pm[j] = mkdd(0.0);
+#pragma unroll
+  for (int i = 0; i <= LA; ++i) {
+    if (i == 0) {
+      cur[0] = mkdd(1.0);
+    } else {
+      const dd prev = i - 1 > 0 ? pm2_0 : mkdd(0.0);
+      cur[0] = dd_add(dd_mul(pa, pm[0]), dd_mul(dd_mul(oo2g, mkdd((double)(i - 1))), prev));
+    }
+    const dd fi = mkdd((double)i);
+#pragma unroll
+    for (int j = 0; j < LB; ++j) {
+      const dd fj = mkdd((double)j);
+      const dd im1 = i > 0 ? pm[j] : mkdd(0.0);
+      const dd jm1 = j > 0 ? cur[j > 0 ? j - 1 : 0] : mkdd(0.0);
+      cur[j + 1] = dd_add(dd_add(dd_mul(pb, cur[j]), dd_mul(dd_mul(oo2g, fi), im1)), dd_mul(dd_mul(oo2g, fj), jm1));
+    }
+    const bool take = i == 0 || ei == i;
+#pragma unroll
+    for (int j = 0; j <= LB; ++j) {
+      out[j].hi = take ? cur[j].hi : out[j].hi;
+      out[j].lo = take ? cur[j].lo : out[j].lo;
+    }
+    pm2_0 = pm[0];
+#pragma unroll
+    for (int j = 0; j <= LB; ++j) pm[j] = cur[j];
+  }
+}
+// the lx of component v of angular momentum L, in cart_components order (compile time)
+__host__ __device__ constexpr int cart_x(int L, int v) {
+  int n = 0;
+  for (int x = L; x >= 0; --x)
+    for (int y = L - x; y >= 0; --y) {
+      if (n == v) return x;
+      ++n;
+    }
+  return 0;
+}
+__host__ __device__ constexpr int cart_y(int L, int v) {
+  int n = 0;
+  for (int x = L; x >= 0; --x)
+    for (int y = L - x; y >= 0; --y) {
+      if (n == v) return y;
+      ++n;
+    }
+  return 0;
+}
+
+template <int LA, int LB, int TIER>
+__global__ __launch_bounds__(128) void k_rows(const ShellRec* __restrict__ sh, const int2* __restrict__ list, long cnt,
+                                              const double* __restrict__ alpha, const double* __restrict__ coeff,
+                                              long n_ao, double* __restrict__ s) {
+  constexpr int NU = (LA + 1) * (LA + 2) / 2, NV = (LB + 1) * (LB + 2) / 2;
+  const long t = (long)blockIdx.x * 128 + threadIdx.x;
+  if (t >= cnt * NU) return;
+  const long k = t / NU;
+  const int u = (int)(t - k * NU);
+  const int2 pr = list[k];
+  const ShellRec A = sh[pr.x], B = sh[pr.y];
+  double* row = s + (long)(A.off + u) * n_ao + B.off;
+  if (TIER == 0) {
+    if (LA == 0 && LB == 0) {  // integral.rs:99-110
       double sum = 0.0;
       for (long p = A.p0; p < A.p1; ++p)
         for (long q = B.p0; q < B.p1; ++q) sum += coeff[p] * coeff[q] * prim_f64(alpha[p], A.c, alpha[q], B.c);
-      return sum;
+      row[0] = sum;
+      return;
     }
-    int ux, uy, uz, vx, vy, vz;
-    cart(A.l, u, ux, uy, uz);
-    cart(B.l, v, vx, vy, vz);
-    double acc = 0.0;
+    int ux, uy, uz;
+    cart(LA, u, ux, uy, uz);
+    double acc[NV];
+#pragma unroll
+    for (int v = 0; v < NV; ++v) acc[v] = 0.0;
     for (long p = A.p0; p < A.p1; ++p)  // integral.rs:233-282
       for (long q = B.p0; q < B.p1; ++q) {
         const double ai = alpha[p], aj = alpha[q];
@@ -182,29 +262,39 @@ __device__ double element(const ShellRec& A, const ShellRec& B, int u, int v, in
         const double prefactor = (sq * sq) * sq * exp(-mu * r2);
         double pa[3], pb[3];
 #pragma unroll
-        for (int k = 0; k < 3; ++k) {
-          const double pk = (ai * A.c[k] + aj * B.c[k]) / gamma;
-          pa[k] = pk - A.c[k];
-          pb[k] = pk - B.c[k];
+        for (int d = 0; d < 3; ++d) {
+          const double pk = (ai * A.c[d] + aj * B.c[d]) / gamma;
+          pa[d] = pk - A.c[d];
+          pb[d] = pk - B.c[d];
         }
-        const double sx = os1d_f64(A.l, B.l, pa[0], pb[0], gamma, ux, vx);
-        const double sy = os1d_f64(A.l, B.l, pa[1], pb[1], gamma, uy, vy);
-        const double sz = os1d_f64(A.l, B.l, pa[2], pb[2], gamma, uz, vz);
-        acc += sx * sy * sz * prefactor * coeff[p] * coeff[q];
+        double rx[LB + 1], ry[LB + 1], rz[LB + 1];
+        row_f64<LA, LB>(pa[0], pb[0], gamma, ux, rx);
+        row_f64<LA, LB>(pa[1], pb[1], gamma, uy, ry);
+        row_f64<LA, LB>(pa[2], pb[2], gamma, uz, rz);
+        const double cp = coeff[p], cq = coeff[q];
+#pragma unroll
+        for (int v = 0; v < NV; ++v) {
+          const int vx = cart_x(LB, v), vy = cart_y(LB, v), vz = LB - vx - vy;
+          acc[v] += rx[vx] * ry[vy] * rz[vz] * prefactor * cp * cq;
+        }
       }
-    return acc;
+#pragma unroll
+    for (int v = 0; v < NV; ++v) row[v] = acc[v];
+    return;
   }
-  if (A.l == 0 && B.l == 0) {  // integral.rs:119-136
+  if (LA == 0 && LB == 0) {  // integral.rs:119-136
     dd sum = mkdd(0.0);
     for (long p = A.p0; p < A.p1; ++p)
       for (long q = B.p0; q < B.p1; ++q)
         sum = dd_add(sum, dd_mul(dd_mul(mkdd(coeff[p]), mkdd(coeff[q])), prim_dd(alpha[p], A.c, alpha[q], B.c)));
-    return sum.hi + sum.lo;
+    row[0] = sum.hi + sum.lo;
+    return;
   }
-  int ux, uy, uz, vx, vy, vz;
-  cart(A.l, u, ux, uy, uz);
-  cart(B.l, v, vx, vy, vz);
-  dd acc = mkdd(0.0);
+  int ux, uy, uz;
+  cart(LA, u, ux, uy, uz);
+  dd acc[NV];
+#pragma unroll
+  for (int v = 0; v < NV; ++v) acc[v] = mkdd(0.0);
   for (long p = A.p0; p < A.p1; ++p)  // integral.rs:284-346
     for (long q = B.p0; q < B.p1; ++q) {
       const dd ai = mkdd(alpha[p]), aj = mkdd(alpha[q]);
@@ -219,38 +309,74 @@ __device__ double element(const ShellRec& A, const ShellRec& B, int u, int v, in
       const dd total_pre = dd_mul(prefactor, exp_val);
       dd pa[3], pb[3];
 #pragma unroll
-      for (int k = 0; k < 3; ++k) {
-        const dd cik = mkdd(A.c[k]), cjk = mkdd(B.c[k]);
+      for (int d = 0; d < 3; ++d) {
+        const dd cik = mkdd(A.c[d]), cjk = mkdd(B.c[d]);
         const dd pk = dd_div(dd_add(dd_mul(ai, cik), dd_mul(aj, cjk)), gamma);
-        pa[k] = dd_sub(pk, cik);
-        pb[k] = dd_sub(pk, cjk);
+        pa[d] = dd_sub(pk, cik);
+        pb[d] = dd_sub(pk, cjk);
       }
-      const dd sx = os1d_dd(A.l, B.l, pa[0], pb[0], gamma, ux, vx);
-      const dd sy = os1d_dd(A.l, B.l, pa[1], pb[1], gamma, uy, vy);
-      const dd sz = os1d_dd(A.l, B.l, pa[2], pb[2], gamma, uz, vz);
+      dd rx[LB + 1], ry[LB + 1], rz[LB + 1];
+      row_dd<LA, LB>(pa[0], pb[0], gamma, ux, rx);
+      row_dd<LA, LB>(pa[1], pb[1], gamma, uy, ry);
+      row_dd<LA, LB>(pa[2], pb[2], gamma, uz, rz);
       const dd cf = dd_mul(dd_mul(mkdd(coeff[p]), mkdd(coeff[q])), total_pre);
-      acc = dd_add(acc, dd_mul(dd_mul(dd_mul(sx, sy), sz), cf));
+#pragma unroll
+      for (int v = 0; v < NV; ++v) {
+        const int vx = cart_x(LB, v), vy = cart_y(LB, v), vz = LB - vx - vy;
+        acc[v] = dd_add(acc[v], dd_mul(dd_mul(dd_mul(rx[vx], ry[vy]), rz[vz]), cf));
+      }
     }
-  return acc.hi + acc.lo;
+#pragma unroll
+  for (int v = 0; v < NV; ++v) row[v] = acc[v].hi + acc[v].lo;
 }
 
-__global__ __launch_bounds__(128) void k_overlap_shells(const ShellRec* __restrict__ sh, const int* __restrict__ ao_shell,
-                                                        const double* __restrict__ alpha,
-                                                        const double* __restrict__ coeff, long n_ao, const double* prm,
-                                                        double* __restrict__ s, unsigned long long* __restrict__ counts) {
-  const long b = (long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (b >= n_ao) return;
-  for (long a = blockIdx.y; a < n_ao; a += gridDim.y) {
-    const int sa = ao_shell[a], sb = ao_shell[b];
-    if (sa < sb) continue;
-    const ShellRec A = sh[sa], B = sh[sb];
-    const int u = (int)(a - A.off), v = (int)(b - B.off);
-    const int tier = screen(A, B, prm);
-    if (u == 0 && v == 0) atomicAdd(&counts[tier], 1ull);
-    if (tier == 2) continue;
-    const double val = element(A, B, u, v, tier, alpha, coeff);
-    s[a * n_ao + b] = val;
-    if (sa != sb) s[b * n_ao + a] = val;
+// S[a][b] = S[b][a] for a < b in distinct shells; block (ta, tb), tb >= ta, reads the lower
+// tile (tb, ta) into LDS and writes the upper tile (ta, tb): each tile is read or written
+// by one block only
+__global__ __launch_bounds__(256) void k_mirror(double* __restrict__ s, const int* __restrict__ ao_shell, long n) {
+  __shared__ double tile[64][65];
+  const long ta = blockIdx.y, tb = blockIdx.x;
+  if (tb < ta) return;
+  for (int idx = threadIdx.x; idx < 64 * 64; idx += 256) {
+    const int r = idx >> 6, c = idx & 63;
+    const long row = tb * 64 + r, col = ta * 64 + c;
+    tile[r][c] = (row < n && col < n) ? s[row * n + col] : 0.0;
+  }
+  __syncthreads();
+  for (int idx = threadIdx.x; idx < 64 * 64; idx += 256) {
+    const int r = idx >> 6, c = idx & 63;
+    const long a = ta * 64 + r, b = tb * 64 + c;
+    if (a < n && b < n && a < b && ao_shell[a] != ao_shell[b]) s[a * n + b] = tile[c][r];
+  }
+}
+
+using RowsFn = void (*)(const ShellRec*, const int2*, long, const double*, const double*, long, double*);
+template <int TIER, int LA, int LB>
+void launch_rows(const ShellRec* sh, const int2* list, long cnt, const double* al, const double* co, long n_ao,
+                 double* s, hipStream_t st) {
+  constexpr int NU = (LA + 1) * (LA + 2) / 2;
+  const long threads = cnt * NU;
+  hipLaunchKernelGGL((k_rows<LA, LB, TIER>), dim3((unsigned)((threads + 127) / 128)), dim3(128), 0, st, sh, list, cnt,
+                     al, co, n_ao, s);
+}
+template <int TIER, int LA>
+void launch_rows_lb(int lb, const ShellRec* sh, const int2* list, long cnt, const double* al, const double* co,
+                    long n_ao, double* s, hipStream_t st) {
+  switch (lb) {
+    case 0: launch_rows<TIER, LA, 0>(sh, list, cnt, al, co, n_ao, s, st); break;
+    case 1: launch_rows<TIER, LA, 1>(sh, list, cnt, al, co, n_ao, s, st); break;
+    case 2: launch_rows<TIER, LA, 2>(sh, list, cnt, al, co, n_ao, s, st); break;
+    default: launch_rows<TIER, LA, 3>(sh, list, cnt, al, co, n_ao, s, st); break;
+  }
+}
+template <int TIER>
+void launch_rows_la(int la, int lb, const ShellRec* sh, const int2* list, long cnt, const double* al,
+                    const double* co, long n_ao, double* s, hipStream_t st) {
+  switch (la) {
+    case 0: launch_rows_lb<TIER, 0>(lb, sh, list, cnt, al, co, n_ao, s, st); break;
+    case 1: launch_rows_lb<TIER, 1>(lb, sh, list, cnt, al, co, n_ao, s, st); break;
+    case 2: launch_rows_lb<TIER, 2>(lb, sh, list, cnt, al, co, n_ao, s, st); break;
+    default: launch_rows_lb<TIER, 3>(lb, sh, list, cnt, al, co, n_ao, s, st); break;
   }
 }
 
@@ -259,18 +385,19 @@ __global__ __launch_bounds__(128) void k_overlap_shells(const ShellRec* __restri
 
 using namespace hp;
 
-extern "C" {
-
-int hp_assemble_overlap_mixed(const double* centers, const int* l, const long* prim_off, const double* alpha,
-                              const double* coeff, size_t n_shells, const double* params4, double* s, size_t n_ao,
-                              size_t* counts) {
+namespace {
+// s_dev: the caller's device matrix (n_ao x n_ao), or null to assemble into a scratch
+// buffer and copy the result to s_host
+int assemble_mixed(const double* centers, const int* l, const long* prim_off, const double* alpha,
+                   const double* coeff, size_t n_shells, const double* params4, double* s_host, double* s_dev,
+                   size_t n_ao, size_t* counts, hipStream_t st) {
   if (!counts || !params4) {
     set_error("null pointer argument");
     return HP_EINVAL;
   }
   counts[0] = counts[1] = counts[2] = 0;
   if (n_shells == 0) return n_ao == 0 ? HP_OK : HP_ESHAPE;
-  if (!centers || !l || !prim_off || !alpha || !coeff || !s) {
+  if (!centers || !l || !prim_off || !alpha || !coeff || (!s_host && !s_dev)) {
     set_error("null pointer argument");
     return HP_EINVAL;
   }
@@ -304,13 +431,16 @@ int hp_assemble_overlap_mixed(const double* centers, const int* l, const long* p
   }
   const long np = prim_off[n_shells];
   const double prm[4] = {params4[0], params4[1], -std::log(params4[2]), params4[3]};
-  hipStream_t st = hipStreamPerThread;
   DevBuf dsh, dao, dal, dco, dprm, ds, dcnt;
   int rc;
   if ((rc = dsh.alloc(n_shells * sizeof(ShellRec))) || (rc = dao.alloc(n_ao * sizeof(int))) ||
       (rc = dal.alloc(np * sizeof(double))) || (rc = dco.alloc(np * sizeof(double))) || (rc = dprm.alloc(32)) ||
-      (rc = ds.alloc(n_ao * n_ao * sizeof(double))) || (rc = dcnt.alloc(3 * sizeof(unsigned long long))))
+      (rc = dcnt.alloc(3 * sizeof(unsigned long long))))
     return rc;
+  if (!s_dev) {
+    if ((rc = ds.alloc(n_ao * n_ao * sizeof(double)))) return rc;
+    s_dev = ds.d();
+  }
   HP_HIP(hipMemcpyAsync(dsh.p, rec.data(), n_shells * sizeof(ShellRec), hipMemcpyHostToDevice, st));
   HP_HIP(hipMemcpyAsync(dao.p, ao_shell.data(), n_ao * sizeof(int), hipMemcpyHostToDevice, st));
   if (np > 0) {
@@ -318,19 +448,78 @@ int hp_assemble_overlap_mixed(const double* centers, const int* l, const long* p
     HP_HIP(hipMemcpyAsync(dco.p, coeff, np * sizeof(double), hipMemcpyHostToDevice, st));
   }
   HP_HIP(hipMemcpyAsync(dprm.p, prm, sizeof(prm), hipMemcpyHostToDevice, st));
-  HP_HIP(hipMemsetAsync(ds.p, 0, n_ao * n_ao * sizeof(double), st));
-  HP_HIP(hipMemsetAsync(dcnt.p, 0, 3 * sizeof(unsigned long long), st));
-  const unsigned gx = (unsigned)((n_ao + 127) / 128), gy = (unsigned)std::min<size_t>(n_ao, 65535);
-  hipLaunchKernelGGL(k_overlap_shells, dim3(gx, gy), dim3(128), 0, st, static_cast<const ShellRec*>(dsh.p),
-                     static_cast<const int*>(dao.p), dal.d(), dco.d(), (long)n_ao, dprm.d(), ds.d(),
-                     static_cast<unsigned long long*>(dcnt.p));
+  HP_HIP(hipMemsetAsync(s_dev, 0, n_ao * n_ao * sizeof(double), st));  // skipped pairs stay zero
+  const long npairs = (long)n_shells * ((long)n_shells + 1) / 2;
+  DevBuf dbk, dlist;
+  if ((rc = dbk.alloc(2 * NBUCKET * sizeof(unsigned int))) || (rc = dlist.alloc(npairs * sizeof(int2)))) return rc;
+  unsigned int* bcount = static_cast<unsigned int*>(dbk.p);
+  unsigned int* bstart = bcount + NBUCKET;
+  auto* tiers = static_cast<unsigned long long*>(dcnt.p);
+  HP_HIP(hipMemsetAsync(tiers, 0, 3 * sizeof(unsigned long long), st));
+  HP_HIP(hipMemsetAsync(bcount, 0, NBUCKET * sizeof(unsigned int), st));
+  const unsigned cgrid = (unsigned)((npairs + 255) / 256);
+  const ShellRec* dshp = static_cast<const ShellRec*>(dsh.p);
+  int2* list = static_cast<int2*>(dlist.p);
+  hipLaunchKernelGGL(k_classify<false>, dim3(cgrid), dim3(256), 0, st, dshp, npairs, dprm.d(), tiers, bcount,
+                     (const unsigned int*)nullptr, list);
+  HP_LAUNCH_CHECK();
+  unsigned int hcount[NBUCKET], hstart[NBUCKET];
+  HP_HIP(hipMemcpyAsync(hcount, bcount, sizeof(hcount), hipMemcpyDeviceToHost, st));
+  HP_HIP(hipStreamSynchronize(st));
+  unsigned int acc = 0;
+  for (int b = 0; b < NBUCKET; ++b) {
+    hstart[b] = acc;
+    acc += hcount[b];
+  }
+  HP_HIP(hipMemcpyAsync(bstart, hstart, sizeof(hstart), hipMemcpyHostToDevice, st));
+  HP_HIP(hipMemsetAsync(bcount, 0, NBUCKET * sizeof(unsigned int), st));  // cursors of the fill pass
+  hipLaunchKernelGGL(k_classify<true>, dim3(cgrid), dim3(256), 0, st, dshp, npairs, dprm.d(),
+                     (unsigned long long*)nullptr, bcount, (const unsigned int*)bstart, list);
+  HP_LAUNCH_CHECK();
+  for (int b = 0; b < NBUCKET; ++b) {
+    if (!hcount[b]) continue;
+    const int tier = b >> 4, la = (b >> 2) & 3, lb = b & 3;
+    if (tier == 0)
+      launch_rows_la<0>(la, lb, dshp, list + hstart[b], hcount[b], dal.d(), dco.d(), (long)n_ao, s_dev, st);
+    else
+      launch_rows_la<1>(la, lb, dshp, list + hstart[b], hcount[b], dal.d(), dco.d(), (long)n_ao, s_dev, st);
+    HP_LAUNCH_CHECK();
+  }
+  const unsigned tiles = (unsigned)((n_ao + 63) / 64);
+  hipLaunchKernelGGL(k_mirror, dim3(tiles, tiles), dim3(256), 0, st, s_dev, static_cast<const int*>(dao.p),
+                     (long)n_ao);
   HP_LAUNCH_CHECK();
   unsigned long long hc[3];
-  HP_HIP(hipMemcpyAsync(s, ds.p, n_ao * n_ao * sizeof(double), hipMemcpyDeviceToHost, st));
-  HP_HIP(hipMemcpyAsync(hc, dcnt.p, sizeof(hc), hipMemcpyDeviceToHost, st));
+  if (s_host) HP_HIP(hipMemcpyAsync(s_host, s_dev, n_ao * n_ao * sizeof(double), hipMemcpyDeviceToHost, st));
+  HP_HIP(hipMemcpyAsync(hc, tiers, sizeof(hc), hipMemcpyDeviceToHost, st));
   HP_HIP(hipStreamSynchronize(st));
   for (int k = 0; k < 3; ++k) counts[k] = (size_t)hc[k];
   return HP_OK;
+}
+}  // namespace
+
+extern "C" {
+
+int hp_assemble_overlap_mixed(const double* centers, const int* l, const long* prim_off, const double* alpha,
+                              const double* coeff, size_t n_shells, const double* params4, double* s, size_t n_ao,
+                              size_t* counts) {
+  if (n_shells && !s) {
+    set_error("null pointer argument");
+    return HP_EINVAL;
+  }
+  return assemble_mixed(centers, l, prim_off, alpha, coeff, n_shells, params4, s, nullptr, n_ao, counts,
+                        hipStreamPerThread);
+}
+
+int hp_dev_assemble_overlap_mixed(const double* centers, const int* l, const long* prim_off, const double* alpha,
+                                  const double* coeff, size_t n_shells, const double* params4, double* s_dev,
+                                  size_t n_ao, size_t* counts, void* stream) {
+  if (n_shells && !s_dev) {
+    set_error("null pointer argument");
+    return HP_EINVAL;
+  }
+  return assemble_mixed(centers, l, prim_off, alpha, coeff, n_shells, params4, nullptr, s_dev, n_ao, counts,
+                        (hipStream_t)stream);
 }
 
 }  // extern "C"
