@@ -448,11 +448,14 @@ def bench_reduce(qp, torch, dev, cpu, reps=5):
     td, tg = best(dot), best(gemv)
     out = {"workload": f"dd dot of {n} values, dd gemv {r} x {r} (gen_dd_normal seeds 7-10), in HBM",
            "dot": {"ms": round(td * 1e3, 3), "GB_per_s": round(32.0 * n / td / 1e9, 1),
-                   "note": "1024-element chunks, one thread each (the reference's order), then a left fold: "
-                           "bound by the per-thread dependent dd chain, not HBM"},
+                   "note": "1024-element chunks, one thread each (the reference's order), then the left "
+                           "fold of the 16384 chunk sums on one wavefront: bound by the fold's dependent dd "
+                           "additions, not HBM"},
            "gemv": {"ms": round(tg * 1e3, 3), "GB_per_s": round(16.0 * r * r / tg / 1e9, 1),
                     "roofline_frac_hbm": round(16.0 * r * r / tg / 8e12, 4),
-                    "note": "one thread per row through 64 x 64 LDS tiles; 16 bytes of A per dd MAC"}}
+                    "note": "one thread per row (the reference's order); 64 x 32 tiles through double-buffered "
+                            "LDS, staged by three loader wavefronts: bound by the 8192-step dependent dd chain "
+                            "of each row (~480 cycles a step), not HBM"}}
     if cpu:
         sys.path.insert(0, os.path.join(ROOT, "tests"))
         import oracle_lib as O

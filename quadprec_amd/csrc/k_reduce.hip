@@ -4,11 +4,11 @@
 //   * dot: below 1024 elements the serial dd sum; else one thread per 1024-element chunk
 //     (serial inside the chunk, as dot_serial) and a left fold of the chunk sums from ZERO,
 //     one of the association orders rayon's reduce may produce;
-//   * gemv: y_i = dot_serial(row_i, x), one thread per row, the rows staged through LDS in
-//     64 x 64 tiles so the loads are coalesced and each thread reads its row across banks;
+//   * gemv: y_i = dot_serial(row_i, x), one thread per row; three loader wavefronts stage
+//     64 x 32 tiles through double-buffered LDS while the fourth runs the row chains;
 //   * par_neumaier_sum(_f64): one thread per 4096-element chunk, then the Neumaier sum of the
-//     partials in order; neumaier_sum(_f64): the whole sum on one thread (the reference's
-//     serial order; use the par form for long inputs).
+//     partials in order; neumaier_sum(_f64): the whole sum in order.  The in-order sums (and
+//     dot's fold) run on one wavefront that loads 64 values at a time ahead of the chain.
 #include "hp_arith.hpp"
 #include "hp_internal.hpp"
 
@@ -17,58 +17,115 @@ namespace {
 
 constexpr long DOT_CHUNK = 1024, NEU_CHUNK = 4096;
 
-__global__ void k_dot_chunks(const double* __restrict__ ah, const double* __restrict__ al,
-                             const double* __restrict__ bh, const double* __restrict__ bl, long n,
-                             double* __restrict__ ph, double* __restrict__ pl) {
+// one thread per 1024-element chunk, its serial dd sum; the loads of the next 8 elements
+// are issued before their multiply-adds, so the dependent chain does not wait on memory
+__global__ __launch_bounds__(64) void k_dot_chunks(const double* __restrict__ ah, const double* __restrict__ al,
+                                                   const double* __restrict__ bh, const double* __restrict__ bl,
+                                                   long n, double* __restrict__ ph, double* __restrict__ pl) {
   const long c = (long)blockIdx.x * blockDim.x + threadIdx.x;
   const long c0 = c * DOT_CHUNK;
   if (c0 >= n) return;
   const long c1 = min(n, c0 + DOT_CHUNK);
   dd s = mkdd(0.0);
-  for (long i = c0; i < c1; ++i) s = dd_add(s, dd_mul(mkdd(ah[i], al[i]), mkdd(bh[i], bl[i])));
+  for (long i = c0; i < c1; i += 8) {
+    double x0[8], x1[8], y0[8], y1[8];
+#pragma unroll
+    for (int e = 0; e < 8; ++e) {
+      const bool ok = i + e < c1;
+      x0[e] = ok ? ah[i + e] : 0.0;
+      x1[e] = ok ? al[i + e] : 0.0;
+      y0[e] = ok ? bh[i + e] : 0.0;
+      y1[e] = ok ? bl[i + e] : 0.0;
+    }
+#pragma unroll
+    for (int e = 0; e < 8; ++e)
+      if (i + e < c1) s = dd_add(s, dd_mul(mkdd(x0[e], x1[e]), mkdd(y0[e], y1[e])));
+  }
   ph[c] = s.hi;
   pl[c] = s.lo;
 }
-// n >= 1024: out = ((ZERO + p0) + p1) + ...; else out = p0, the serial dot (ZERO for n = 0)
-__global__ void k_fold(const double* __restrict__ ph, const double* __restrict__ pl, long np, long n,
-                       double* out_h, double* out_l) {
-  if (threadIdx.x != 0 || blockIdx.x != 0) return;
-  const bool chunked = n >= DOT_CHUNK;
-  dd acc = n ? mkdd(ph[0], pl[0]) : mkdd(0.0);
-  if (chunked) {
-    acc = mkdd(0.0);
-    for (long c = 0; c < np; ++c) acc = dd_add(acc, mkdd(ph[c], pl[c]));
-  }
-  out_h[0] = acc.hi;
-  out_l[0] = acc.lo;
+
+__device__ __forceinline__ double lane_value(double v, int k) {  // v of lane k (k uniform)
+  const long long b = __double_as_longlong(v);
+  const unsigned lo = (unsigned)__builtin_amdgcn_readlane((int)b, k);
+  const unsigned hi = (unsigned)__builtin_amdgcn_readlane((int)(b >> 32), k);
+  return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
 }
 
-// y_i = sum_j a_ij x_j in j order; 64 rows per block, 64 x 64 tiles through LDS
-__global__ __launch_bounds__(64) void k_gemv(const double* __restrict__ ah, const double* __restrict__ al, long rows,
-                                             long cols, const double* __restrict__ xh, const double* __restrict__ xl,
-                                             double* __restrict__ yh, double* __restrict__ yl) {
-  __shared__ double th[64][65], tl[64][65];
-  __shared__ double sxh[64], sxl[64];
-  const int t = threadIdx.x;
-  const long r0 = (long)blockIdx.x * 64;
-  dd s = mkdd(0.0);
-  for (long c0 = 0; c0 < cols; c0 += 64) {
-    const int w = (int)min(64L, cols - c0);
-    for (int r = 0; r < 64; ++r) {  // row r of the tile: its w contiguous entries
-      const long gr = r0 + r;
-      const bool ok = gr < rows && t < w;
-      th[r][t] = ok ? ah[gr * cols + c0 + t] : 0.0;
-      tl[r][t] = ok ? al[gr * cols + c0 + t] : 0.0;
+// n >= 1024: out = ((ZERO + p0) + p1) + ...; else out = p0, the serial dot (ZERO for n = 0).
+// One wavefront: 64 partials are loaded at a time (the next 64 while these are summed) and
+// the fold runs, the same in every lane, on values read across lanes, never on memory.
+__global__ __launch_bounds__(64) void k_fold(const double* __restrict__ ph, const double* __restrict__ pl, long np,
+                                             long n, double* out_h, double* out_l) {
+  const int lane = threadIdx.x;
+  dd acc = mkdd(0.0);
+  if (n < DOT_CHUNK) {
+    if (n) acc = mkdd(ph[0], pl[0]);
+  } else {
+    double nh = lane < np ? ph[lane] : 0.0, nl = lane < np ? pl[lane] : 0.0;
+    for (long c0 = 0; c0 < np; c0 += 64) {
+      const double h = nh, l = nl;
+      const long nx = c0 + 64 + lane;
+      nh = nx < np ? ph[nx] : 0.0;
+      nl = nx < np ? pl[nx] : 0.0;
+      const long cnt = np - c0 < 64 ? np - c0 : 64;
+#pragma unroll
+      for (int k = 0; k < 64; ++k)
+        if (k < cnt) acc = dd_add(acc, mkdd(lane_value(h, k), lane_value(l, k)));
     }
-    sxh[t] = t < w ? xh[c0 + t] : 0.0;
-    sxl[t] = t < w ? xl[c0 + t] : 0.0;
-    __syncthreads();
-    for (int k = 0; k < w; ++k) s = dd_add(s, dd_mul(mkdd(th[t][k], tl[t][k]), mkdd(sxh[k], sxl[k])));
-    __syncthreads();
   }
-  if (r0 + t < rows) {
-    yh[r0 + t] = s.hi;
-    yl[r0 + t] = s.lo;
+  if (lane == 0) {
+    out_h[0] = acc.hi;
+    out_l[0] = acc.lo;
+  }
+}
+
+// y_i = sum_j a_ij x_j in j order.  A block takes 64 rows: wavefronts 1-3 stage 64 x 32
+// tiles of A (and x) into double-buffered LDS with coalesced loads, one tile ahead, while
+// wavefront 0 runs the 64 serial row chains over the current tile.
+constexpr int GV_T = 32;
+__global__ __launch_bounds__(256) void k_gemv(const double* __restrict__ ah, const double* __restrict__ al, long rows,
+                                              long cols, const double* __restrict__ xh, const double* __restrict__ xl,
+                                              double* __restrict__ yh, double* __restrict__ yl) {
+  __shared__ double th[2][64][GV_T + 1], tl[2][64][GV_T + 1];
+  __shared__ double sxh[2][GV_T], sxl[2][GV_T];
+  const int t = threadIdx.x, w = t >> 6, lane = t & 63;
+  const long r0 = (long)blockIdx.x * 64;
+  const long ntile = (cols + GV_T - 1) / GV_T;
+  auto load = [&](long tile, int buf) {  // wavefronts 1-3
+    const long c0 = tile * GV_T;
+    for (int e = t - 64; e < 64 * GV_T; e += 192) {
+      const int r = e / GV_T, c = e % GV_T;
+      const long gr = r0 + r, gc = c0 + c;
+      const bool ok = gr < rows && gc < cols;
+      th[buf][r][c] = ok ? ah[gr * cols + gc] : 0.0;
+      tl[buf][r][c] = ok ? al[gr * cols + gc] : 0.0;
+    }
+    for (int e = t - 64; e < GV_T; e += 192) {
+      const long gc = c0 + e;
+      sxh[buf][e] = gc < cols ? xh[gc] : 0.0;
+      sxl[buf][e] = gc < cols ? xl[gc] : 0.0;
+    }
+  };
+  if (w > 0 && ntile > 0) load(0, 0);
+  __syncthreads();
+  dd s = mkdd(0.0);
+  for (long tile = 0; tile < ntile; ++tile) {
+    const int buf = (int)(tile & 1);
+    if (w > 0) {
+      if (tile + 1 < ntile) load(tile + 1, buf ^ 1);
+    } else {
+      const long rem = cols - tile * GV_T;
+      const int wc = rem < GV_T ? (int)rem : GV_T;
+#pragma unroll
+      for (int k = 0; k < GV_T; ++k)  // unrolled: the LDS reads go ahead of the chain
+        if (k < wc) s = dd_add(s, dd_mul(mkdd(th[buf][lane][k], tl[buf][lane][k]), mkdd(sxh[buf][k], sxl[buf][k])));
+    }
+    __syncthreads();  // tile read by wavefront 0 and the next one written before either is reused
+  }
+  if (w == 0 && r0 + lane < rows) {
+    yh[r0 + lane] = s.hi;
+    yl[r0 + lane] = s.lo;
   }
 }
 
@@ -85,27 +142,83 @@ __device__ __forceinline__ void neu_dd(dd& sum, dd& comp, dd v) {
   else comp = dd_add(comp, dd_add(dd_sub(v, t), sum));
   sum = t;
 }
-// chunk c of `chunk` values -> its Neumaier sum (chunk = n: the serial sum on one thread)
-__global__ void k_neu_f64(const double* __restrict__ v, long n, long chunk, double* __restrict__ part) {
+// chunk c of NEU_CHUNK values -> its Neumaier sum, one thread per chunk, 8 loads ahead
+__global__ __launch_bounds__(64) void k_neu_f64(const double* __restrict__ v, long n, double* __restrict__ part) {
   const long c = (long)blockIdx.x * blockDim.x + threadIdx.x;
-  const long c0 = c * chunk;
+  const long c0 = c * NEU_CHUNK;
   if (c0 >= n) return;
-  const long c1 = min(n, c0 + chunk);
+  const long c1 = min(n, c0 + NEU_CHUNK);
   double sum = 0.0, comp = 0.0;
-  for (long i = c0; i < c1; ++i) neu_f64(sum, comp, v[i]);
+  for (long i = c0; i < c1; i += 8) {
+    double x[8];
+#pragma unroll
+    for (int e = 0; e < 8; ++e) x[e] = i + e < c1 ? v[i + e] : 0.0;
+#pragma unroll
+    for (int e = 0; e < 8; ++e)
+      if (i + e < c1) neu_f64(sum, comp, x[e]);
+  }
   part[c] = sum + comp;
 }
-__global__ void k_neu_dd(const double* __restrict__ vh, const double* __restrict__ vl, long n, long chunk,
-                         double* __restrict__ ph, double* __restrict__ pl) {
+__global__ __launch_bounds__(64) void k_neu_dd(const double* __restrict__ vh, const double* __restrict__ vl, long n,
+                                               double* __restrict__ ph, double* __restrict__ pl) {
   const long c = (long)blockIdx.x * blockDim.x + threadIdx.x;
-  const long c0 = c * chunk;
+  const long c0 = c * NEU_CHUNK;
   if (c0 >= n) return;
-  const long c1 = min(n, c0 + chunk);
+  const long c1 = min(n, c0 + NEU_CHUNK);
   dd sum = mkdd(0.0), comp = mkdd(0.0);
-  for (long i = c0; i < c1; ++i) neu_dd(sum, comp, mkdd(vh[i], vl[i]));
+  for (long i = c0; i < c1; i += 8) {
+    double x0[8], x1[8];
+#pragma unroll
+    for (int e = 0; e < 8; ++e) {
+      x0[e] = i + e < c1 ? vh[i + e] : 0.0;
+      x1[e] = i + e < c1 ? vl[i + e] : 0.0;
+    }
+#pragma unroll
+    for (int e = 0; e < 8; ++e)
+      if (i + e < c1) neu_dd(sum, comp, mkdd(x0[e], x1[e]));
+  }
   const dd r = dd_add(sum, comp);
   ph[c] = r.hi;
   pl[c] = r.lo;
+}
+// the whole Neumaier sum of n values in order (the serial forms, and the sum of the
+// partials): one wavefront loads 64 values at a time, the next 64 while these are summed,
+// and runs the sum, the same in every lane, on values read across lanes
+__global__ __launch_bounds__(64) void k_neu_f64_wave(const double* __restrict__ v, long n, double* __restrict__ out) {
+  const int lane = threadIdx.x;
+  double sum = 0.0, comp = 0.0;
+  double nv = lane < n ? v[lane] : 0.0;
+  for (long c0 = 0; c0 < n; c0 += 64) {
+    const double x = nv;
+    const long nx = c0 + 64 + lane;
+    nv = nx < n ? v[nx] : 0.0;
+    const long cnt = n - c0 < 64 ? n - c0 : 64;
+#pragma unroll
+    for (int k = 0; k < 64; ++k)
+      if (k < cnt) neu_f64(sum, comp, lane_value(x, k));
+  }
+  if (lane == 0) out[0] = sum + comp;
+}
+__global__ __launch_bounds__(64) void k_neu_dd_wave(const double* __restrict__ vh, const double* __restrict__ vl,
+                                                    long n, double* __restrict__ oh, double* __restrict__ ol) {
+  const int lane = threadIdx.x;
+  dd sum = mkdd(0.0), comp = mkdd(0.0);
+  double nh = lane < n ? vh[lane] : 0.0, nl = lane < n ? vl[lane] : 0.0;
+  for (long c0 = 0; c0 < n; c0 += 64) {
+    const double h = nh, l = nl;
+    const long nx = c0 + 64 + lane;
+    nh = nx < n ? vh[nx] : 0.0;
+    nl = nx < n ? vl[nx] : 0.0;
+    const long cnt = n - c0 < 64 ? n - c0 : 64;
+#pragma unroll
+    for (int k = 0; k < 64; ++k)
+      if (k < cnt) neu_dd(sum, comp, mkdd(lane_value(h, k), lane_value(l, k)));
+  }
+  if (lane == 0) {
+    const dd r = dd_add(sum, comp);
+    oh[0] = r.hi;
+    ol[0] = r.lo;
+  }
 }
 
 unsigned nblk(long n, int t) { return (unsigned)((n + t - 1) / t); }
@@ -131,7 +244,7 @@ int dot_dev(const double* ah, const double* al, const double* bh, const double* 
 int gemv_dev(const double* ah, const double* al, long rows, long cols, const double* xh, const double* xl, double* yh,
              double* yl, hipStream_t s) {
   if (rows == 0) return HP_OK;
-  hipLaunchKernelGGL(k_gemv, dim3(nblk(rows, 64)), dim3(64), 0, s, ah, al, rows, cols, xh, xl, yh, yl);
+  hipLaunchKernelGGL(k_gemv, dim3(nblk(rows, 64)), dim3(256), 0, s, ah, al, rows, cols, xh, xl, yh, yl);
   HP_LAUNCH_CHECK();
   return HP_OK;
 }
@@ -151,35 +264,33 @@ int neumaier(const double* v, size_t n, int limbs, bool par, double* out) {
   if (n == 0) return HP_OK;
   hipStream_t s = hipStreamPerThread;
   const bool chunked = par && (long)n >= NEU_CHUNK;  // lib.rs:43-66; else the serial sum
-  const long chunk = chunked ? NEU_CHUNK : (long)n;
-  const long np = ((long)n + chunk - 1) / chunk;
+  const long np = chunked ? ((long)n + NEU_CHUNK - 1) / NEU_CHUNK : 0;
   DevBuf dv, part, fin;
   int rc;
-  if ((rc = dv.alloc(limbs * n * sizeof(double))) || (rc = part.alloc(limbs * np * sizeof(double))) ||
+  if ((rc = dv.alloc(limbs * n * sizeof(double))) || (rc = part.alloc(limbs * std::max(np, 1L) * sizeof(double))) ||
       (rc = fin.alloc(limbs * sizeof(double))))
     return rc;
   if ((rc = upload_planes(v, n, limbs, dv.d(), s))) return rc;
   if (limbs == 1) {
-    hipLaunchKernelGGL(k_neu_f64, dim3(nblk(np, 64)), dim3(64), 0, s, dv.d(), (long)n, chunk, part.d());
-    HP_LAUNCH_CHECK();
-    if (chunked) {  // the Neumaier sum of the partials, in order
-      hipLaunchKernelGGL(k_neu_f64, dim3(1), dim3(64), 0, s, part.d(), np, np, fin.d());
+    if (chunked) {  // the chunk sums, then the Neumaier sum of the partials in order
+      hipLaunchKernelGGL(k_neu_f64, dim3(nblk(np, 64)), dim3(64), 0, s, dv.d(), (long)n, part.d());
       HP_LAUNCH_CHECK();
+      hipLaunchKernelGGL(k_neu_f64_wave, dim3(1), dim3(64), 0, s, part.d(), np, fin.d());
     } else {
-      HP_HIP(hipMemcpyAsync(fin.p, part.p, sizeof(double), hipMemcpyDeviceToDevice, s));
+      hipLaunchKernelGGL(k_neu_f64_wave, dim3(1), dim3(64), 0, s, dv.d(), (long)n, fin.d());
     }
+    HP_LAUNCH_CHECK();
     HP_HIP(hipMemcpyAsync(out, fin.p, sizeof(double), hipMemcpyDeviceToHost, s));
   } else {
-    hipLaunchKernelGGL(k_neu_dd, dim3(nblk(np, 64)), dim3(64), 0, s, dv.d(), dv.d() + n, (long)n, chunk, part.d(),
-                       part.d() + np);
-    HP_LAUNCH_CHECK();
     if (chunked) {
-      hipLaunchKernelGGL(k_neu_dd, dim3(1), dim3(64), 0, s, part.d(), part.d() + np, np, np, fin.d(), fin.d() + 1);
+      hipLaunchKernelGGL(k_neu_dd, dim3(nblk(np, 64)), dim3(64), 0, s, dv.d(), dv.d() + n, (long)n, part.d(),
+                         part.d() + np);
       HP_LAUNCH_CHECK();
+      hipLaunchKernelGGL(k_neu_dd_wave, dim3(1), dim3(64), 0, s, part.d(), part.d() + np, np, fin.d(), fin.d() + 1);
     } else {
-      HP_HIP(hipMemcpyAsync(fin.p, part.p, sizeof(double), hipMemcpyDeviceToDevice, s));
-      HP_HIP(hipMemcpyAsync(fin.d() + 1, part.d() + np, sizeof(double), hipMemcpyDeviceToDevice, s));
+      hipLaunchKernelGGL(k_neu_dd_wave, dim3(1), dim3(64), 0, s, dv.d(), dv.d() + n, (long)n, fin.d(), fin.d() + 1);
     }
+    HP_LAUNCH_CHECK();
     HP_HIP(hipMemcpyAsync(out, fin.p, 2 * sizeof(double), hipMemcpyDeviceToHost, s));
   }
   HP_HIP(hipStreamSynchronize(s));

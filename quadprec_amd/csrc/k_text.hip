@@ -323,15 +323,30 @@ __global__ __launch_bounds__(256) void k_scan_blocks(const uint32_t* __restrict_
   }
   if (threadIdx.x == 0) bsum[blockIdx.x] = red[0];
 }
-__global__ void k_scan_sums(uint64_t* __restrict__ bsum, long nb) {
-  if (threadIdx.x != 0 || blockIdx.x != 0) return;
-  uint64_t acc = 0;
-  for (long i = 0; i < nb; ++i) {
-    const uint64_t v = bsum[i];
-    bsum[i] = acc;
-    acc += v;
+// exclusive scan of the nb block sums in place (bsum[nb] = total): one block of 1024
+// threads, each a contiguous run of the sums, the run totals scanned through LDS
+__global__ __launch_bounds__(1024) void k_scan_sums(uint64_t* __restrict__ bsum, long nb) {
+  __shared__ uint64_t part[1024];
+  const int t = threadIdx.x;
+  const long per = (nb + 1023) / 1024;
+  const long b0 = t * per, b1 = b0 + per < nb ? b0 + per : nb;
+  uint64_t own = 0;
+  for (long i = b0; i < b1; ++i) own += bsum[i];
+  part[t] = own;
+  __syncthreads();
+  for (int d = 1; d < 1024; d <<= 1) {  // inclusive Hillis-Steele scan
+    const uint64_t add = t >= d ? part[t - d] : 0;
+    __syncthreads();
+    part[t] += add;
+    __syncthreads();
   }
-  bsum[nb] = acc;
+  uint64_t run = part[t] - own;
+  for (long i = b0; i < b1; ++i) {
+    const uint64_t v = bsum[i];
+    bsum[i] = run;
+    run += v;
+  }
+  if (t == 1023) bsum[nb] = part[1023];
 }
 __global__ __launch_bounds__(256) void k_scan_final(const uint32_t* __restrict__ len, long n,
                                                     const uint64_t* __restrict__ bsum, long nb,
@@ -884,7 +899,7 @@ int format_dev(int kind, const double* const* planes, long n, int prec, int plus
   uint64_t* bs = static_cast<uint64_t*>(bsum.p);
   hipLaunchKernelGGL(k_scan_blocks, dim3((unsigned)nb), dim3(256), 0, s, ln, n, bs);
   HP_LAUNCH_CHECK();
-  hipLaunchKernelGGL(k_scan_sums, dim3(1), dim3(64), 0, s, bs, nb);
+  hipLaunchKernelGGL(k_scan_sums, dim3(1), dim3(1024), 0, s, bs, nb);
   HP_LAUNCH_CHECK();
   hipLaunchKernelGGL(k_scan_final, dim3((unsigned)nb), dim3(256), 0, s, ln, n, bs, nb, off);
   HP_LAUNCH_CHECK();
