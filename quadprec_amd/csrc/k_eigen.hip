@@ -88,7 +88,9 @@ __device__ __forceinline__ Rot pair_rot(const double* __restrict__ ah, const dou
 // LDS.  An upper block (I < J) takes its columns by R_J and then its rows by R_I; a lower
 // block takes its rows by R_I and then its columns by R_J, which is the upper computation
 // transposed operation for operation, so A stays exactly symmetric without mirror writes.
-template <int WI, int WR>  // pairs I per wavefront (z = 0), rows per wavefront (z = 1)
+// WI pairs I per wavefront (z = 0), WR rows per wavefront (z = 1); HOIST: load the blocks
+// before forming the rotations (hides the load latency when few blocks run; costs registers)
+template <int WI, int WR, bool HOIST>
 __global__ __launch_bounds__(256) void k_jac_step(const double* __restrict__ ah, const double* __restrict__ al,
                                                   double* __restrict__ bh, double* __restrict__ bl,
                                                   const double* __restrict__ vh, const double* __restrict__ vl,
@@ -100,9 +102,19 @@ __global__ __launch_bounds__(256) void k_jac_step(const double* __restrict__ ah,
   if (blockIdx.z == 1) {
     const long i0 = (long)blockIdx.x * 64, r0 = (long)blockIdx.y * ST_R;
     if (r0 >= N2) return;  // the whole block
+    const long i = i0 + lane;
+    dd v0[HOIST ? WR : 1], v1[HOIST ? WR : 1];  // the rotations do not depend on them
+    if constexpr (HOIST) {
+#pragma unroll
+      for (int k = 0; k < WR; ++k) {
+        const long r = r0 + w * WR + k;
+        const bool ok = i < m && r < N2;
+        v0[k] = ok ? mkdd(vh[r * N2 + 2 * i], vl[r * N2 + 2 * i]) : mkdd(0.0);
+        v1[k] = ok ? mkdd(vh[r * N2 + 2 * i + 1], vl[r * N2 + 2 * i + 1]) : mkdd(0.0);
+      }
+    }
     if (t < 64 && i0 + t < m) rs[t] = pair_rot(ah, al, N2, i0 + t);
     __syncthreads();
-    const long i = i0 + lane;
     if (i >= m) return;
     const Rot R = rs[lane];
     const dd c = mkdd(R.ch, R.cl), s = mkdd(R.sh, R.sl);
@@ -111,10 +123,10 @@ __global__ __launch_bounds__(256) void k_jac_step(const double* __restrict__ ah,
     for (int k = 0; k < WR; ++k) {
       const long r = r0 + w * WR + k;
       if (r >= N2) continue;
-      const dd v0 = mkdd(vh[r * N2 + 2 * i], vl[r * N2 + 2 * i]);
-      const dd v1 = mkdd(vh[r * N2 + 2 * i + 1], vl[r * N2 + 2 * i + 1]);
-      const dd w0 = dd_sub(dd_mul(c, v0), dd_mul(s, v1));
-      const dd w1 = dd_add(dd_mul(s, v0), dd_mul(c, v1));
+      const dd a0 = HOIST ? v0[HOIST ? k : 0] : mkdd(vh[r * N2 + 2 * i], vl[r * N2 + 2 * i]);
+      const dd a1 = HOIST ? v1[HOIST ? k : 0] : mkdd(vh[r * N2 + 2 * i + 1], vl[r * N2 + 2 * i + 1]);
+      const dd w0 = dd_sub(dd_mul(c, a0), dd_mul(s, a1));
+      const dd w1 = dd_add(dd_mul(s, a0), dd_mul(c, a1));
       wh[r * N2 + o0] = w0.hi; wl[r * N2 + o0] = w0.lo;
       wh[r * N2 + o1] = w1.hi; wl[r * N2 + o1] = w1.lo;
     }
@@ -122,25 +134,46 @@ __global__ __launch_bounds__(256) void k_jac_step(const double* __restrict__ ah,
   }
   const long J0 = (long)blockIdx.x * 64, I0 = (long)blockIdx.y * ST_I;
   if (I0 >= m) return;  // the whole block
+  const long J = J0 + lane;
+  const long q0 = 2 * J, q1 = q0 + 1;
+  dd xs[HOIST ? WI : 1][4];  // the rotations do not depend on them
+  if constexpr (HOIST) {
+#pragma unroll
+    for (int k = 0; k < WI; ++k) {
+      const long I = I0 + w * WI + k;
+      const bool ok = I < m && J < m;
+      const long p0 = 2 * I, p1 = p0 + 1;
+      xs[k][0] = ok ? mkdd(ah[p0 * N2 + q0], al[p0 * N2 + q0]) : mkdd(0.0);
+      xs[k][1] = ok ? mkdd(ah[p0 * N2 + q1], al[p0 * N2 + q1]) : mkdd(0.0);
+      xs[k][2] = ok ? mkdd(ah[p1 * N2 + q0], al[p1 * N2 + q0]) : mkdd(0.0);
+      xs[k][3] = ok ? mkdd(ah[p1 * N2 + q1], al[p1 * N2 + q1]) : mkdd(0.0);
+    }
+  }
   if (t < 64) {
     if (J0 + t < m) rs[t] = pair_rot(ah, al, N2, J0 + t);
   } else if (t < 64 + ST_I) {
     if (I0 + t - 64 < m) rs[t] = pair_rot(ah, al, N2, I0 + t - 64);
   }
   __syncthreads();
-  const long J = J0 + lane;
   if (J >= m) return;
   const Rot RJ = rs[lane];
   const dd cJ = mkdd(RJ.ch, RJ.cl), sJ = mkdd(RJ.sh, RJ.sl);
-  const long q0 = 2 * J, q1 = q0 + 1, nq0 = rr_sigma(q0, m), nq1 = rr_sigma(q1, m);
+  const long nq0 = rr_sigma(q0, m), nq1 = rr_sigma(q1, m);
 #pragma unroll
   for (int k = 0; k < WI; ++k) {
     const long I = I0 + w * WI + k;
     if (I >= m) continue;
     const Rot RI = rs[64 + w * WI + k];
     const long p0 = 2 * I, p1 = p0 + 1;
-    dd x00 = mkdd(ah[p0 * N2 + q0], al[p0 * N2 + q0]), x01 = mkdd(ah[p0 * N2 + q1], al[p0 * N2 + q1]);
-    dd x10 = mkdd(ah[p1 * N2 + q0], al[p1 * N2 + q0]), x11 = mkdd(ah[p1 * N2 + q1], al[p1 * N2 + q1]);
+    dd x00, x01, x10, x11;
+    if constexpr (HOIST) {
+      x00 = xs[k][0]; x01 = xs[k][1]; x10 = xs[k][2]; x11 = xs[k][3];
+    } else {
+      x00 = mkdd(ah[p0 * N2 + q0], al[p0 * N2 + q0]);
+      x01 = mkdd(ah[p0 * N2 + q1], al[p0 * N2 + q1]);
+      x10 = mkdd(ah[p1 * N2 + q0], al[p1 * N2 + q0]);
+      x11 = mkdd(ah[p1 * N2 + q1], al[p1 * N2 + q1]);
+    }
     const long np0 = rr_sigma(p0, m), np1 = rr_sigma(p1, m);
     dd y00, y01, y10, y11;
     if (I == J) {
@@ -342,14 +375,18 @@ int jacobi_eigen_dev(const double* a_h, const double* a_l, long n, bool stop_fro
     double *xh = ah, *xl = al, *yh = bh, *yl = bl, *uh = vh, *ul = vl, *zh = wh, *zl = wl;
     HP_HIP(hipStreamBeginCapture(gr.cs, hipStreamCaptureModeThreadLocal));
     for (long step = 0; step < N2 - 1; ++step) {
-      // small matrices: one I per wavefront for enough blocks; larger ones: four, which
-      // forms fewer copies of each rotation
-      if (m <= 512)
-        hipLaunchKernelGGL((k_jac_step<1, 4>), dim3(blocks(m, 64), std::max(blocks(m, 4), blocks(N2, 16)), 2),
+      // small matrices: one I per wavefront for enough blocks, loads ahead of the rotations;
+      // larger ones: four, which forms fewer copies of each rotation
+      if (m <= 256)
+        hipLaunchKernelGGL((k_jac_step<1, 4, true>), dim3(blocks(m, 64), std::max(blocks(m, 4), blocks(N2, 16)), 2),
+                           dim3(256), 0, gr.cs, xh, xl, yh, yl, uh, ul, zh, zl, N2);
+      else if (m <= 512)
+        hipLaunchKernelGGL((k_jac_step<1, 4, false>), dim3(blocks(m, 64), std::max(blocks(m, 4), blocks(N2, 16)), 2),
                            dim3(256), 0, gr.cs, xh, xl, yh, yl, uh, ul, zh, zl, N2);
       else
-        hipLaunchKernelGGL((k_jac_step<4, 16>), dim3(blocks(m, 64), std::max(blocks(m, 16), blocks(N2, 64)), 2),
-                           dim3(256), 0, gr.cs, xh, xl, yh, yl, uh, ul, zh, zl, N2);
+        hipLaunchKernelGGL((k_jac_step<4, 16, false>),
+                           dim3(blocks(m, 64), std::max(blocks(m, 16), blocks(N2, 64)), 2), dim3(256), 0, gr.cs, xh,
+                           xl, yh, yl, uh, ul, zh, zl, N2);
       std::swap(xh, yh);
       std::swap(xl, yl);
       std::swap(uh, zh);
