@@ -91,7 +91,8 @@ __device__ dd prim_dd(double ai, const double* ci, double aj, const double* cj) 
 
 // ---- assembly in three launches:
 //   k_classify: every shell pair (i >= j) screened once (screen.rs:57-101), counted by tier
-//     and listed by (tier, l_i, l_j), so that the next launches run without divergence;
+//     and listed by (tier, l_i, l_j) (counts gathered in LDS per block of 4096 pairs), so
+//     that the next launches run without divergence;
 //   k_rows<LA, LB, TIER>: one thread per (pair, row u of the block): per primitive pair the
 //     reference's set-up and its three 1D Obara-Saika tables, then the whole row of v
 //     components (integral.rs:65-346); every component accumulates over the primitive
@@ -108,43 +109,55 @@ __device__ __forceinline__ void decode_pair(long k, long& i, long& j) {
   j = k - i * (i + 1) / 2;
 }
 
+// A block takes 4096 pairs (4 per thread): counts per tier and per bucket gather in LDS,
+// then one global atomic per nonempty bucket (and tier) per block; in the fill pass each
+// pair is listed at its bucket's start + the block's base + its rank within the block.
+constexpr int CL_PER = 4, CL_THREADS = 1024;
 template <bool FILL>
-__global__ __launch_bounds__(256) void k_classify(const ShellRec* __restrict__ sh, long npairs,
-                                                  const double* __restrict__ prm,
-                                                  unsigned long long* __restrict__ tiers,
-                                                  unsigned int* __restrict__ bcount,
-                                                  const unsigned int* __restrict__ bstart, int2* __restrict__ list) {
-  const long k = (long)blockIdx.x * 256 + threadIdx.x;
-  const int lane = threadIdx.x & 63;
-  int tier = 3, bucket = -1;
-  long i = 0, j = 0;
-  if (k < npairs) {
-    decode_pair(k, i, j);
-    const ShellRec A = sh[i], B = sh[j];
-    tier = screen(A, B, prm);
-    if (tier < 2) bucket = tier * 16 + A.l * 4 + B.l;
-  }
-  if (!FILL) {
+__global__ __launch_bounds__(CL_THREADS) void k_classify(const ShellRec* __restrict__ sh, long npairs,
+                                                         const double* __restrict__ prm,
+                                                         unsigned long long* __restrict__ tiers,
+                                                         unsigned int* __restrict__ bcount,
+                                                         const unsigned int* __restrict__ bstart,
+                                                         int2* __restrict__ list) {
+  __shared__ unsigned int cnt[NBUCKET], base[NBUCKET];
+  __shared__ unsigned int tcnt[3];
+  const int t = threadIdx.x;
+  if (t < NBUCKET) cnt[t] = 0;
+  if (t < 3) tcnt[t] = 0;
+  __syncthreads();
+  const long k0 = (long)blockIdx.x * CL_PER * CL_THREADS;
+  int bk[CL_PER];
+  unsigned int rank[CL_PER];
+  int pi[CL_PER], pj[CL_PER];
 #pragma unroll
-    for (int t = 0; t < 3; ++t) {
-      const unsigned long long m = __ballot(tier == t);
-      if (lane == 0 && m) atomicAdd(&tiers[t], (unsigned long long)__popcll(m));
+  for (int e = 0; e < CL_PER; ++e) {
+    bk[e] = -1;
+    rank[e] = 0;
+    pi[e] = pj[e] = 0;
+    const long k = k0 + (long)e * CL_THREADS + t;
+    if (k < npairs) {
+      long i, j;
+      decode_pair(k, i, j);
+      const ShellRec A = sh[i], B = sh[j];
+      const int tier = screen(A, B, prm);
+      if (!FILL) atomicAdd(&tcnt[tier], 1u);
+      if (tier < 2) {
+        bk[e] = tier * 16 + A.l * 4 + B.l;
+        rank[e] = atomicAdd(&cnt[bk[e]], 1u);
+        pi[e] = (int)i;
+        pj[e] = (int)j;
+      }
     }
   }
-  unsigned long long active = __ballot(bucket >= 0);
-  while (active) {  // one atomic per distinct bucket of the wavefront
-    const int leader = __ffsll((long long)active) - 1;
-    const int bk = __shfl(bucket, leader);
-    const unsigned long long same = __ballot(bucket == bk);
-    unsigned int base = 0;
-    if (lane == leader) base = atomicAdd(&bcount[bk], (unsigned int)__popcll(same));
-    base = __shfl(base, leader);
-    if (FILL && bucket == bk) {
-      const unsigned int rank = (unsigned int)__popcll(same & ((1ull << lane) - 1ull));
-      list[bstart[bk] + base + rank] = make_int2((int)i, (int)j);
-    }
-    active &= ~same;
-  }
+  __syncthreads();
+  if (t < NBUCKET && cnt[t]) base[t] = atomicAdd(&bcount[t], cnt[t]);
+  if (!FILL && t < 3 && tcnt[t]) atomicAdd(&tiers[t], (unsigned long long)tcnt[t]);
+  if (!FILL) return;
+  __syncthreads();
+#pragma unroll
+  for (int e = 0; e < CL_PER; ++e)
+    if (bk[e] >= 0) list[bstart[bk[e]] + base[bk[e]] + rank[e]] = make_int2(pi[e], pj[e]);
 }
 
 // Row ei (runtime, <= LA) of the 1D Obara-Saika table s[i][j], i <= LA, j <= LB, of
@@ -457,10 +470,10 @@ int assemble_mixed(const double* centers, const int* l, const long* prim_off, co
   auto* tiers = static_cast<unsigned long long*>(dcnt.p);
   HP_HIP(hipMemsetAsync(tiers, 0, 3 * sizeof(unsigned long long), st));
   HP_HIP(hipMemsetAsync(bcount, 0, NBUCKET * sizeof(unsigned int), st));
-  const unsigned cgrid = (unsigned)((npairs + 255) / 256);
+  const unsigned cgrid = (unsigned)((npairs + CL_PER * CL_THREADS - 1) / (CL_PER * CL_THREADS));
   const ShellRec* dshp = static_cast<const ShellRec*>(dsh.p);
   int2* list = static_cast<int2*>(dlist.p);
-  hipLaunchKernelGGL(k_classify<false>, dim3(cgrid), dim3(256), 0, st, dshp, npairs, dprm.d(), tiers, bcount,
+  hipLaunchKernelGGL(k_classify<false>, dim3(cgrid), dim3(CL_THREADS), 0, st, dshp, npairs, dprm.d(), tiers, bcount,
                      (const unsigned int*)nullptr, list);
   HP_LAUNCH_CHECK();
   unsigned int hcount[NBUCKET], hstart[NBUCKET];
@@ -473,7 +486,7 @@ int assemble_mixed(const double* centers, const int* l, const long* prim_off, co
   }
   HP_HIP(hipMemcpyAsync(bstart, hstart, sizeof(hstart), hipMemcpyHostToDevice, st));
   HP_HIP(hipMemsetAsync(bcount, 0, NBUCKET * sizeof(unsigned int), st));  // cursors of the fill pass
-  hipLaunchKernelGGL(k_classify<true>, dim3(cgrid), dim3(256), 0, st, dshp, npairs, dprm.d(),
+  hipLaunchKernelGGL(k_classify<true>, dim3(cgrid), dim3(CL_THREADS), 0, st, dshp, npairs, dprm.d(),
                      (unsigned long long*)nullptr, bcount, (const unsigned int*)bstart, list);
   HP_LAUNCH_CHECK();
   for (int b = 0; b < NBUCKET; ++b) {
