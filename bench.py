@@ -309,9 +309,10 @@ def bench_text(qp, torch, dev, cpu, reps=5):
                       "GB_per_s": round((16 * n + nbytes + 8 * n) / tf / 1e9, 1)},
            "parse": {"ms": round(tp * 1e3, 3), "Mvalues_per_s": round(n / tp / 1e6, 1),
                      "GB_per_s": round((16 * n + nbytes + 8 * n) / tp / 1e9, 1)},
-           "roofline": {"bound": "valu", "note": "exact decimal conversion with a 1024-bit integer per value "
-                                                 "(multiplies / divides by 5^13, shifts): integer VALU bound; "
-                                                 "bytes moved are ~77 per value, < 1% of HBM time"}}
+           "roofline": {"bound": "valu", "note": "per f64 one 64 x 128-bit multiply against a table of 10^s "
+                                                 "(the exact 1024-bit path only near rounding boundaries), byte-wise "
+                                                 "string work per lane: integer VALU bound; ~77 algorithmic bytes "
+                                                 "per value (+128 through the format slots)"}}
     if cpu:
         sys.path.insert(0, os.path.join(ROOT, "tests"))
         import oracle_lib as O

@@ -27,6 +27,13 @@
 //           status 2 instead of a guess.
 // Sizes: the largest intermediate is M 5^388 < 2^954 (format, P = 64, subnormal x) and
 // w 2^A < 2^920 (parse), inside the 1024 bits.
+//
+// Fast paths.  Each call first builds 10^s as truncated 128-bit mantissas F 2^g for
+// -365 <= s <= 345 (exactly, with the integer above).  Formatting with P <= 18 and parsing
+// up to 19 significant digits then take one 64 x 128 multiply: the exact value lies between
+// the products with F and with F + 1, and when both round alike that is the answer; only
+// values within about 2^-120 of a rounding boundary (exact ties among them) take the exact
+// path, so every result is the exact path's.
 #include <algorithm>
 #include <cstdint>
 #include <vector>
@@ -171,11 +178,131 @@ __device__ __forceinline__ uint64_t big_bits64(const Big& b, int pos, bool& belo
   return s ? (lo >> s) | ((uint64_t)x2 << (64 - s)) : lo;
 }
 
+// ------------------------------------------------------------------ powers of ten, 128 bits
+// 10^s = F 2^g (1 + d), F in [2^127, 2^128) truncated (0 <= d < 2^-127), for TMIN <= s <= TMAX:
+// the fast paths below multiply by F and bracket the exact product between F and F + 1.
+constexpr int TMIN = -365, TMAX = 345, TCOUNT = TMAX - TMIN + 1;
+struct P10 {
+  uint64_t hi, lo;
+  int g, pad;
+};
+// one thread per s, exact with the 1024-bit integer: 5^s (s >= 0) or floor(2^K / 5^-s)
+__global__ void k_pow10_table(P10* __restrict__ tab) {
+  const int idx = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+  if (idx >= TCOUNT) return;
+  const int s = TMIN + idx;
+  Big b;
+  bool sticky = false;
+  int g;
+  if (s >= 0) {
+    big_set(b, 1);
+    big_mul_pow5(b, s);
+    const int L = big_bitlen(b);
+    if (L > 128) big_shr(b, L - 128, sticky);
+    else big_shl(b, 128 - L);
+    g = s + L - 128;
+  } else {
+    Big f;
+    big_set(f, 1);
+    big_mul_pow5(f, -s);
+    const int K = 128 + big_bitlen(f);
+    big_set(b, 1);
+    big_shl(b, K);
+    big_div_pow5(b, -s, sticky);
+    const int L = big_bitlen(b);  // 128 or 129
+    if (L > 128) big_shr(b, L - 128, sticky);
+    g = -K + s + (L - 128);
+  }
+  P10 r;
+  r.hi = ((uint64_t)b.w[3] << 32) | b.w[2];
+  r.lo = ((uint64_t)b.w[1] << 32) | b.w[0];
+  r.g = g;
+  r.pad = 0;
+  tab[idx] = r;
+}
+// p2:p1:p0 = a * (fh:fl)
+__device__ __forceinline__ void mul_64x128(uint64_t a, uint64_t fh, uint64_t fl, uint64_t& p2, uint64_t& p1,
+                                           uint64_t& p0) {
+  p0 = a * fl;
+  const uint64_t h0 = __umul64hi(a, fl), l1 = a * fh, h1 = __umul64hi(a, fh);
+  p1 = l1 + h0;
+  p2 = h1 + (p1 < l1 ? 1u : 0u);
+}
+__device__ __forceinline__ void add_64(uint64_t& p2, uint64_t& p1, uint64_t& p0, uint64_t a) {
+  p0 += a;
+  const uint64_t c0 = p0 < a ? 1u : 0u;
+  p1 += c0;
+  p2 += (c0 && p1 == 0) ? 1u : 0u;
+}
+// bits [a, a + 64) of p2:p1:p0 (0 <= a < 192; bits above 191 read as zero)
+__device__ __forceinline__ uint64_t bits3(uint64_t p2, uint64_t p1, uint64_t p0, int a) {
+  const int k = a >> 6, r = a & 63;
+  const uint64_t x0 = k == 0 ? p0 : (k == 1 ? p1 : p2);
+  const uint64_t x1 = k == 0 ? p1 : (k == 1 ? p2 : 0u);
+  return r ? (x0 >> r) | (x1 << (64 - r)) : x0;
+}
+// any of bits [0, b) set (0 <= b <= 192)
+__device__ __forceinline__ bool low3(uint64_t p2, uint64_t p1, uint64_t p0, int b) {
+  auto part = [](uint64_t x, int n) { return n >= 64 ? x != 0 : (n <= 0 ? false : (x & ((1ull << n) - 1)) != 0); };
+  return part(p0, b) || part(p1, b - 64) || part(p2, b - 128);
+}
+__device__ __forceinline__ int bitlen3(uint64_t p2, uint64_t p1, uint64_t p0) {
+  return p2 ? 192 - __clzll(p2) : (p1 ? 128 - __clzll(p1) : (p0 ? 64 - __clzll(p0) : 0));
+}
+
 // ------------------------------------------------------------------ formatting
 // Rust's `{:.Pe}` / `{:+.Pe}` of x into out; returns the length.  Finite nonzero values
 // get P+1 significant digits rounded half to even; the exponent is printed without '+'
 // or padding; NaN is "NaN" whatever its sign, infinities "inf", zero "0.000e0".
-__device__ int fmt_exp(double x, int P, bool plus, char* out) {
+__constant__ uint64_t POW10_U64[20] = {1ull, 10ull, 100ull, 1000ull, 10000ull, 100000ull, 1000000ull, 10000000ull,
+                                       100000000ull, 1000000000ull, 10000000000ull, 100000000000ull,
+                                       1000000000000ull, 10000000000000ull, 100000000000000ull,
+                                       1000000000000000ull, 10000000000000000ull, 100000000000000000ull,
+                                       1000000000000000000ull, 10000000000000000000ull};
+// P <= 18: the P+1 digits D of M 2^E 10^s, s = P - e10, from one 64 x 128 multiply.  The
+// exact product lies in [M F, M F + M) 2^(E + g); when the integer part and the side of 1/2
+// are the same at both ends, the rounding (half to even; a tie cannot lie strictly inside)
+// is decided.  e10 may be one low: then the integer part has P+2 digits and s drops by one.
+__device__ bool fmt_fast(uint64_t M, int E, int P, const P10* __restrict__ tab, int& e10, uint64_t& D) {
+  int ee = e10;
+  for (int pass = 0; pass < 2; ++pass) {
+    const int s = P - ee;
+    if (s < TMIN || s > TMAX) return false;
+    const P10 F = tab[s - TMIN];
+    const int sh = -(E + F.g);  // integer part = product >> sh
+    if (sh < 2 || sh > 190) return false;
+    uint64_t p2, p1, p0;
+    mul_64x128(M, F.hi, F.lo, p2, p1, p0);
+    const bool big = sh + 64 < 192 && bits3(p2, p1, p0, sh + 64) != 0;  // integer part >= 2^64
+    const uint64_t N = bits3(p2, p1, p0, sh);
+    if (big || N >= POW10_U64[P + 1]) {  // e10 was one low
+      if (pass) return false;
+      ++ee;
+      continue;
+    }
+    const bool hb = (bits3(p2, p1, p0, sh - 1) & 1u) != 0, rest = low3(p2, p1, p0, sh - 1);
+    uint64_t q2 = p2, q1 = p1, q0 = p0;
+    add_64(q2, q1, q0, M);
+    const uint64_t N2 = bits3(q2, q1, q0, sh);
+    const bool hb2 = (bits3(q2, q1, q0, sh - 1) & 1u) != 0, rest2 = low3(q2, q1, q0, sh - 1);
+    if (N2 == N && (!hb2 || !rest2)) {  // the top end is at most N + 1/2: the value lies below
+      D = N;
+    } else if (hb && rest && (N2 == N || (N2 == N + 1 && !hb2 && !rest2))) {
+      D = N + 1;
+    } else {
+      return false;
+    }
+    if (D == POW10_U64[P + 1]) {  // 9.99..9 rounded up to 10.00..0
+      D = POW10_U64[P];
+      ++ee;
+    }
+    e10 = ee;
+    return true;
+  }
+  return false;
+}
+
+__device__ int fmt_exp(double x, int P, bool plus, char* out, const P10* __restrict__ tab) {
   int n = 0;
   const uint64_t bits = (uint64_t)__double_as_longlong(x);
   const bool neg = bits >> 63;
@@ -209,6 +336,16 @@ __device__ int fmt_exp(double x, int P, bool plus, char* out) {
     }
     const int e2 = 63 - __clzll(mant) + E;
     e10 = (int)floor(e2 * 0.30102999566398119521);  // floor(log10|x|) or one less
+    uint64_t D;
+    if (P <= 18 && fmt_fast(mant, E, P, tab, e10, D)) {
+      const int total = P + 1;
+      n = d0 + (P > 0 ? total + 1 : 1);
+      if (P > 0) out[d0 + 1] = '.';
+      for (int k = total - 1; k >= 0; --k) {
+        out[k == 0 ? d0 : d0 + 1 + k] = (char)('0' + D % 10u);
+        D /= 10u;
+      }
+    } else {
     Big t, q, p10;
     big_pow10(p10, P + 1);
     bool sticky;
@@ -255,6 +392,7 @@ __device__ int fmt_exp(double x, int P, bool plus, char* out) {
         }
       }
     }
+    }
   }
   out[n++] = 'e';
   const unsigned ae = (unsigned)(e10 < 0 ? -e10 : e10);  // at most 324
@@ -268,19 +406,19 @@ __device__ int fmt_exp(double x, int P, bool plus, char* out) {
 enum FmtKind { F_F64 = 0, F_DD_DISPLAY = 1, F_MF4_DISPLAY = 2, F_DD_SERDE = 3, F_MF4_SERDE = 4 };
 
 template <int KIND>
-__device__ int format_elem(const double* const* pl, long e, int prec, bool plus, char* out) {
-  if (KIND == F_F64) return fmt_exp(pl[0][e], prec, plus, out);
-  if (KIND == F_DD_DISPLAY) return fmt_exp(pl[0][e] + pl[1][e], 32, false, out);
-  if (KIND == F_MF4_DISPLAY) return fmt_exp(pl[0][e] + pl[1][e], 64, false, out);
+__device__ int format_elem(const double* const* pl, long e, int prec, bool plus, char* out, const P10* tab) {
+  if (KIND == F_F64) return fmt_exp(pl[0][e], prec, plus, out, tab);
+  if (KIND == F_DD_DISPLAY) return fmt_exp(pl[0][e] + pl[1][e], 32, false, out, tab);
+  if (KIND == F_MF4_DISPLAY) return fmt_exp(pl[0][e] + pl[1][e], 64, false, out, tab);
   if (KIND == F_DD_SERDE) {
-    int n = fmt_exp(pl[0][e], 18, true, out);
+    int n = fmt_exp(pl[0][e], 18, true, out, tab);
     out[n++] = '+';
-    return n + fmt_exp(pl[1][e], 18, true, out + n);
+    return n + fmt_exp(pl[1][e], 18, true, out + n, tab);
   }
   int n = 0;  // F_MF4_SERDE
   for (int i = 0; i < 4; ++i) {
     if (i) out[n++] = '+';
-    n += fmt_exp(pl[i][e], 14, true, out + n);
+    n += fmt_exp(pl[i][e], 14, true, out + n, tab);
   }
   return n;
 }
@@ -293,10 +431,10 @@ struct Planes {
 // stored with coalesced 16-byte writes
 template <int KIND, int SLOT>
 __global__ __launch_bounds__(64) void k_format(Planes pl, long n, int prec, int plus, char* __restrict__ slots,
-                                               uint32_t* __restrict__ lens) {
+                                               uint32_t* __restrict__ lens, const P10* __restrict__ tab) {
   __shared__ __align__(16) char buf[64 * SLOT];
   const long e = (long)blockIdx.x * 64 + threadIdx.x;
-  if (e < n) lens[e] = (uint32_t)format_elem<KIND>(pl.p, e, prec, plus != 0, buf + threadIdx.x * SLOT);
+  if (e < n) lens[e] = (uint32_t)format_elem<KIND>(pl.p, e, prec, plus != 0, buf + threadIdx.x * SLOT, tab);
   __syncthreads();
   const long base = (long)blockIdx.x * 64 * SLOT;
   const long nb = (n - (long)blockIdx.x * 64 < 64 ? n - (long)blockIdx.x * 64 : 64) * SLOT;
@@ -410,19 +548,9 @@ __device__ bool eq_nocase(const char* s, long len, const char* word, int wl) {
   return true;
 }
 
-// round T 2^e2off (+ sticky, below T's last bit) to a double, half to even
-__device__ double big_to_double(const Big& t, int e2off, bool sticky, bool neg) {
-  const int L = big_bitlen(t);
-  uint64_t m64;
-  if (L <= 64) {
-    const uint64_t v = (uint64_t)t.w[0] | ((uint64_t)t.w[1] << 32);
-    m64 = v << (64 - L);
-  } else {
-    bool below;
-    m64 = big_bits64(t, L - 64, below);
-    sticky |= below;
-  }
-  const int E = L - 1 + e2off;  // value in [2^E, 2^(E+1))
+// the double nearest m64 2^(E - 63) (+ sticky below m64's last bit; bit 63 of m64 set),
+// half to even, with subnormals and overflow
+__device__ double round_m64(uint64_t m64, bool sticky, int E, bool neg) {
   uint64_t out;
   if (E > 1023) {
     out = 0x7FF0000000000000ull;
@@ -454,6 +582,49 @@ __device__ double big_to_double(const Big& t, int e2off, bool sticky, bool neg) 
   return __longlong_as_double((long long)out);
 }
 
+// round T 2^e2off (+ sticky, below T's last bit) to a double, half to even
+__device__ double big_to_double(const Big& t, int e2off, bool sticky, bool neg) {
+  const int L = big_bitlen(t);
+  uint64_t m64;
+  if (L <= 64) {
+    const uint64_t v = (uint64_t)t.w[0] | ((uint64_t)t.w[1] << 32);
+    m64 = v << (64 - L);
+  } else {
+    bool below;
+    m64 = big_bits64(t, L - 64, below);
+    sticky |= below;
+  }
+  return round_m64(m64, sticky, L - 1 + e2off, neg);
+}
+// p2:p1:p0 2^g (p nonzero) rounded to a double
+__device__ double p3_to_double(uint64_t p2, uint64_t p1, uint64_t p0, int g, bool neg) {
+  const int L = bitlen3(p2, p1, p0);
+  uint64_t m64;
+  bool sticky = false;
+  if (L <= 64) {
+    m64 = p0 << (64 - L);
+  } else {
+    m64 = bits3(p2, p1, p0, L - 64);
+    sticky = low3(p2, p1, p0, L - 64);
+  }
+  return round_m64(m64, sticky, L - 1 + g, neg);
+}
+// w 10^q for w < 2^64, q in the table: the roundings of w F 2^g and (w F + w) 2^g, which
+// bracket the exact value, agree on every value but those within 2^-120 or so of a halfway
+// point; then that is the answer (false: take the exact path)
+__device__ bool parse_fast(uint64_t w, int q, bool neg, const P10* __restrict__ tab, double& out) {
+  if (q < TMIN || q > TMAX) return false;
+  const P10 F = tab[q - TMIN];
+  uint64_t p2, p1, p0;
+  mul_64x128(w, F.hi, F.lo, p2, p1, p0);
+  const double lo = p3_to_double(p2, p1, p0, F.g, neg);
+  add_64(p2, p1, p0, w);
+  const double hi = p3_to_double(p2, p1, p0, F.g, neg);
+  if (__double_as_longlong(lo) != __double_as_longlong(hi)) return false;
+  out = lo;
+  return true;
+}
+
 // w 10^q rounded to a double (w nonzero, |q| within the cut-offs of parse_f64)
 __device__ double decimal_to_double(const Big& w, int q, bool neg) {
   Big t = w;
@@ -473,7 +644,7 @@ __device__ double decimal_to_double(const Big& w, int q, bool neg) {
 // Rust's str::parse::<f64>: [+-] (inf | infinity | nan, any case) or
 // [+-] digits [. digits] [(e|E) [+-] digits] with at least one mantissa digit, nothing else.
 // status: 0 ok, 1 syntax error, 2 more than 40 significant digits needed to round.
-__device__ double parse_f64(const char* s, long len, int& status) {
+__device__ double parse_f64(const char* s, long len, int& status, const P10* __restrict__ tab) {
   status = 0;
   long i = 0;
   bool neg = false;
@@ -493,6 +664,7 @@ __device__ double parse_f64(const char* s, long len, int& status) {
     return __longlong_as_double((long long)((neg ? 0xFFF8000000000000ull : 0x7FF8000000000000ull)));
   Big w;
   big_set(w, 0);
+  uint64_t w64 = 0;  // the same digits while there are at most 19
   int nsig = 0;
   long dexp = 0;
   bool trunc = false, any = false;
@@ -502,6 +674,7 @@ __device__ double parse_f64(const char* s, long len, int& status) {
     if (nsig < MAX_SIG) {
       if (nsig > 0 || d) {
         big_mul_add(w, 10u, d);
+        w64 = w64 * 10u + d;
         ++nsig;
       }
     } else {
@@ -516,6 +689,7 @@ __device__ double parse_f64(const char* s, long len, int& status) {
       if (nsig < MAX_SIG) {
         if (nsig > 0 || d) {
           big_mul_add(w, 10u, d);
+          w64 = w64 * 10u + d;
           ++nsig;
         }
         --dexp;
@@ -552,6 +726,10 @@ __device__ double parse_f64(const char* s, long len, int& status) {
   const long qeff = dexp + nsig - 1;  // decimal exponent of the leading digit
   if (qeff > 310) return neg ? -__longlong_as_double(0x7FF0000000000000ll) : __longlong_as_double(0x7FF0000000000000ll);
   if (qeff < -326) return zero;
+  if (!trunc && nsig <= 19) {
+    double r;
+    if (parse_fast(w64, (int)dexp, neg, tab, r)) return r;
+  }
   const double r1 = decimal_to_double(w, (int)dexp, neg);
   if (trunc) {
     Big w1 = w;
@@ -749,14 +927,15 @@ struct OutPlanes {
 
 template <int KIND>
 __global__ __launch_bounds__(64) void k_parse(const char* __restrict__ text, const uint64_t* __restrict__ off, long n,
-                                              OutPlanes o, uint8_t* __restrict__ status) {
+                                              OutPlanes o, uint8_t* __restrict__ status,
+                                              const P10* __restrict__ tab) {
   const long e = (long)blockIdx.x * 64 + threadIdx.x;
   if (e >= n) return;
   const char* s = text + off[e];
   const long len = (long)(off[e + 1] - off[e]);
   int st = 0;
   if (KIND == P_F64) {
-    o.p[0][e] = parse_f64(s, len, st);
+    o.p[0][e] = parse_f64(s, len, st, tab);
   } else if (KIND == P_DD_FROMSTR) {
     dd v = mkdd(0.0);
     if (!from_str<DdOps>(s, len, v)) st = 1;
@@ -779,8 +958,8 @@ __global__ __launch_bounds__(64) void k_parse(const char* __restrict__ text, con
       st = 1;
     } else {
       int s1, s2;
-      hi = parse_f64(s, split, s1);
-      lo = parse_f64(s + split + 1, len - split - 1, s2);
+      hi = parse_f64(s, split, s1, tab);
+      lo = parse_f64(s + split + 1, len - split - 1, s2, tab);
       st = s1 == 1 || s2 == 1 ? 1 : (s1 | s2);
     }
     o.p[0][e] = st == 1 ? 0.0 : hi;
@@ -798,7 +977,7 @@ __global__ __launch_bounds__(64) void k_parse(const char* __restrict__ text, con
           break;
         }
         int sl;
-        const double v = parse_f64(s + start, i - start, sl);
+        const double v = parse_f64(s + start, i - start, sl, tab);
         if (sl == 1) {
           st = 1;
           break;
@@ -818,7 +997,7 @@ __global__ __launch_bounds__(64) void k_parse(const char* __restrict__ text, con
         st = 1;
       } else {
         int sl;
-        const double v = parse_f64(s + start, len - start, sl);
+        const double v = parse_f64(s + start, len - start, sl, tab);
         if (sl == 1) st = 1;
         else st |= sl;
 #pragma unroll
@@ -856,8 +1035,9 @@ int limbs_of_fmt(int kind) { return kind == F_F64 ? 1 : (kind == F_MF4_DISPLAY |
 int limbs_of_parse(int kind) { return kind == P_F64 ? 1 : (kind == P_MF4_FROMSTR || kind == P_MF4_SERDE) ? 4 : 2; }
 
 template <int KIND, int SLOT>
-void launch_format(const Planes& pl, long n, int prec, int plus, char* slots, uint32_t* lens, hipStream_t s) {
-  hipLaunchKernelGGL((k_format<KIND, SLOT>), dim3(nblk64(n)), dim3(64), 0, s, pl, n, prec, plus, slots, lens);
+void launch_format(const Planes& pl, long n, int prec, int plus, char* slots, uint32_t* lens, const P10* tab,
+                   hipStream_t s) {
+  hipLaunchKernelGGL((k_format<KIND, SLOT>), dim3(nblk64(n)), dim3(64), 0, s, pl, n, prec, plus, slots, lens, tab);
 }
 
 }  // namespace
@@ -878,21 +1058,24 @@ int format_dev(int kind, const double* const* planes, long n, int prec, int plus
   }
   const int slot = slot_of(kind);
   const long nb = (n + 1023) / 1024;
-  DevBuf slots, lens, bsum;
+  DevBuf slots, lens, bsum, tabb;
   int rc;
   if ((rc = slots.alloc((size_t)nblk64(n) * 64 * slot)) || (rc = lens.alloc(n * sizeof(uint32_t))) ||
-      (rc = bsum.alloc((nb + 1) * sizeof(uint64_t))))
+      (rc = bsum.alloc((nb + 1) * sizeof(uint64_t))) || (rc = tabb.alloc(TCOUNT * sizeof(P10))))
     return rc;
+  const P10* ptab = static_cast<const P10*>(tabb.p);
+  hipLaunchKernelGGL(k_pow10_table, dim3((TCOUNT + 63) / 64), dim3(64), 0, s, static_cast<P10*>(tabb.p));
+  HP_LAUNCH_CHECK();
   Planes pl{};
   for (int k = 0; k < limbs_of_fmt(kind); ++k) pl.p[k] = planes[k];
   char* sl = static_cast<char*>(slots.p);
   uint32_t* ln = static_cast<uint32_t*>(lens.p);
   switch (kind) {
-    case F_F64: launch_format<F_F64, 80>(pl, n, prec, plus, sl, ln, s); break;
-    case F_DD_DISPLAY: launch_format<F_DD_DISPLAY, 48>(pl, n, prec, plus, sl, ln, s); break;
-    case F_MF4_DISPLAY: launch_format<F_MF4_DISPLAY, 80>(pl, n, prec, plus, sl, ln, s); break;
-    case F_DD_SERDE: launch_format<F_DD_SERDE, 64>(pl, n, prec, plus, sl, ln, s); break;
-    case F_MF4_SERDE: launch_format<F_MF4_SERDE, 96>(pl, n, prec, plus, sl, ln, s); break;
+    case F_F64: launch_format<F_F64, 80>(pl, n, prec, plus, sl, ln, ptab, s); break;
+    case F_DD_DISPLAY: launch_format<F_DD_DISPLAY, 48>(pl, n, prec, plus, sl, ln, ptab, s); break;
+    case F_MF4_DISPLAY: launch_format<F_MF4_DISPLAY, 80>(pl, n, prec, plus, sl, ln, ptab, s); break;
+    case F_DD_SERDE: launch_format<F_DD_SERDE, 64>(pl, n, prec, plus, sl, ln, ptab, s); break;
+    case F_MF4_SERDE: launch_format<F_MF4_SERDE, 96>(pl, n, prec, plus, sl, ln, ptab, s); break;
     default: set_error("unknown format kind %d", kind); return HP_EINVAL;
   }
   HP_LAUNCH_CHECK();
@@ -930,19 +1113,25 @@ int parse_dev(int kind, const char* text, const uint64_t* off, long n, double* c
   if (n == 0) return HP_OK;
   OutPlanes o{};
   for (int k = 0; k < limbs_of_parse(kind); ++k) o.p[k] = planes[k];
+  DevBuf tabb;
+  int rc0;
+  if ((rc0 = tabb.alloc(TCOUNT * sizeof(P10)))) return rc0;
+  const P10* ptab = static_cast<const P10*>(tabb.p);
+  hipLaunchKernelGGL(k_pow10_table, dim3((TCOUNT + 63) / 64), dim3(64), 0, s, static_cast<P10*>(tabb.p));
+  HP_LAUNCH_CHECK();
   switch (kind) {
-    case P_F64: hipLaunchKernelGGL(k_parse<P_F64>, dim3(nblk64(n)), dim3(64), 0, s, text, off, n, o, status); break;
+    case P_F64: hipLaunchKernelGGL(k_parse<P_F64>, dim3(nblk64(n)), dim3(64), 0, s, text, off, n, o, status, ptab); break;
     case P_DD_FROMSTR:
-      hipLaunchKernelGGL(k_parse<P_DD_FROMSTR>, dim3(nblk64(n)), dim3(64), 0, s, text, off, n, o, status);
+      hipLaunchKernelGGL(k_parse<P_DD_FROMSTR>, dim3(nblk64(n)), dim3(64), 0, s, text, off, n, o, status, ptab);
       break;
     case P_MF4_FROMSTR:
-      hipLaunchKernelGGL(k_parse<P_MF4_FROMSTR>, dim3(nblk64(n)), dim3(64), 0, s, text, off, n, o, status);
+      hipLaunchKernelGGL(k_parse<P_MF4_FROMSTR>, dim3(nblk64(n)), dim3(64), 0, s, text, off, n, o, status, ptab);
       break;
     case P_DD_SERDE:
-      hipLaunchKernelGGL(k_parse<P_DD_SERDE>, dim3(nblk64(n)), dim3(64), 0, s, text, off, n, o, status);
+      hipLaunchKernelGGL(k_parse<P_DD_SERDE>, dim3(nblk64(n)), dim3(64), 0, s, text, off, n, o, status, ptab);
       break;
     case P_MF4_SERDE:
-      hipLaunchKernelGGL(k_parse<P_MF4_SERDE>, dim3(nblk64(n)), dim3(64), 0, s, text, off, n, o, status);
+      hipLaunchKernelGGL(k_parse<P_MF4_SERDE>, dim3(nblk64(n)), dim3(64), 0, s, text, off, n, o, status, ptab);
       break;
     default: set_error("unknown parse kind %d", kind); return HP_EINVAL;
   }
