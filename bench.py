@@ -472,6 +472,56 @@ def bench_reduce(qp, torch, dev, cpu, reps=5):
     return out
 
 
+def bench_mf4_chol(qp, torch, dev, cpu, reps=3):
+    """mf_cholesky::<4> and mf_cholesky_solve::<4> (crates/quad/src/multifloat/linalg.rs:89-172)
+    at n = MFC_N on B B^T + n I (B seeded MF4 N(0,1)), host arrays in and out (the drop-in
+    entry points); bit-identical to the oracle (tests/test_gpu_mf4_linalg.py)."""
+    import numpy as np
+    n = MFC_N
+    rng = np.random.default_rng(6)
+    b = rng.standard_normal((n, n))
+    a = b @ b.T + n * np.eye(n)  # exact f64 entries, promoted to MF4
+    a4 = np.zeros((n * n, 4))
+    a4[:, 0] = a.ravel()
+    rhs = np.zeros((n, 4))
+    rhs[:, 0] = rng.standard_normal(n)
+
+    def best(fn):
+        fn()
+        ts = []
+        for _ in range(reps):
+            t0 = time.perf_counter()
+            fn()
+            ts.append(time.perf_counter() - t0)
+        return min(ts)
+
+    L = qp.mf_cholesky(a4, n)
+    tc = best(lambda: qp.mf_cholesky(a4, n))
+    ts = best(lambda: qp.mf_cholesky_solve(L, rhs, n))
+    macs = n ** 3 / 6.0
+    out = {"workload": f"mf_cholesky + mf_cholesky_solve, MF4, n = {n}, A = B B^T + n I (seed 6)",
+           "cholesky_ms": round(tc * 1e3, 2), "solve_ms": round(ts * 1e3, 2),
+           "cholesky_MF4_GMAC_per_s": round(macs / tc / 1e9, 3),
+           "note": "column-serial right-looking factor (one launch for the column's MF sqrt / recip chain, "
+                   "one for the trailing one-MAC update); the reference's per-element order; host arrays "
+                   "in and out"}
+    if cpu:
+        sys.path.insert(0, os.path.join(ROOT, "tests"))
+        import oracle_lib as O
+        t0 = time.perf_counter()
+        ref = O.mf4_cholesky(a4, n)
+        t1 = time.perf_counter()
+        O.mf4_cholesky_solve(ref, rhs, n)
+        t2 = time.perf_counter()
+        assert np.array_equal(ref, L)
+        out["cpu_baseline"] = {"value": round((t1 - t0) * 1e3, 1), "unit": "ms (cholesky)",
+                               "solve_ms": round((t2 - t1) * 1e3, 1), "cores": 1, "kind": "port",
+                               "sample": f"the same n = {n} factor and solve, oracle C++, one thread; factor "
+                                         "bit-identical to the GPU's"}
+    return out
+
+
+MFC_N = 256
 OVL_ATOMS, OVL_CPU_ATOMS = 400, 400
 RED_N, RED_ROWS = 1 << 24, 8192
 EIG_N, EIG_CPU_N = 512, 192
@@ -557,6 +607,7 @@ def main():
     ap.add_argument("--no-text", action="store_true", help="skip the dd serde text round trip")
     ap.add_argument("--no-overlap", action="store_true", help="skip the mixed-tier overlap assembly")
     ap.add_argument("--no-reduce", action="store_true", help="skip dot / gemv")
+    ap.add_argument("--no-mfchol", action="store_true", help="skip the MF4 Cholesky")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only copy MIB MiB device-to-device three times (a plain copy kernel of known size, "
                          "to calibrate rocprofv3's FETCH_SIZE / WRITE_SIZE counters) and exit")
@@ -658,6 +709,8 @@ def main():
         out["overlap_mixed_400_atoms"] = bench_overlap_mixed(qp, torch, dev, not args.no_cpu)
     if not args.no_reduce:
         out["dot_gemv"] = bench_reduce(qp, torch, dev, not args.no_cpu)
+    if not args.no_mfchol:
+        out["mf4_cholesky_256"] = bench_mf4_chol(qp, torch, dev, not args.no_cpu)
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(a, b)
     print(json.dumps(out), flush=True)

@@ -88,6 +88,22 @@ int hp_dev_gemm(int trans_a, const double* a_hi, const double* a_lo, const doubl
 int hp_dev_mf_gemm(int trans_a, const double* a, const double* b, double* c, size_t m, size_t n, size_t k,
                    void* stream);
 
+/* ------------------------------------------------------------------ MultiFloat<4> linear algebra */
+
+/* In-place MF4 Cholesky A = L L^T (n x n, row-major, AoS), replaces mf_cholesky::<4>
+ * (crates/quad/src/multifloat/linalg.rs:89-141): lower triangle written, upper keeps A,
+ * bit-identical.  On a pivot with limb 0 <= 0 returns HP_ENOTSPD (the reference's Err) with
+ * *pivot set and the columns before it factored, as the reference leaves them. */
+int hp_mf_cholesky(double* a, size_t n, size_t* pivot);
+/* L L^T x = b by forward then backward substitution, replaces mf_cholesky_solve::<4>
+ * (multifloat/linalg.rs:145-172); bit-identical. */
+int hp_mf_cholesky_solve(const double* l, const double* b, double* x, size_t n);
+/* y = A x, A n x m row-major, replaces mf_gemv::<4> (multifloat/linalg.rs:17-36). */
+int hp_mf_gemv(const double* a, const double* x, double* y, size_t n, size_t m);
+/* Elementwise MF4 ops over n values: op 0 recip(a) (multifloat/div.rs:14-41), 1 sqrt(a)
+ * (multifloat/math.rs:22-40), 2 a / b (div.rs:44-77); b is read only for op 2. */
+int hp_mf4_elementwise(int op, const double* a, const double* b, double* out, size_t n);
+
 /* ------------------------------------------------------------------ Cholesky and solves */
 
 /* In-place unblocked Cholesky A = L L^T of an n x n dd matrix (row-major, AoS).

@@ -147,6 +147,10 @@ def lib():
         getattr(L, name).argtypes = [C.c_char_p, _op, _sz, _dp, _u8]
     L.hp_dev_format.argtypes = [C.c_int, _vp, _sz, C.c_int, C.c_int, _vp, _sz, _vp, C.POINTER(C.c_size_t), _vp]
     L.hp_dev_parse.argtypes = [C.c_int, _vp, _vp, _sz, _vp, _vp, C.POINTER(C.c_size_t), _vp]
+    L.hp_mf_cholesky.argtypes = [_dp, _sz, C.POINTER(C.c_size_t)]
+    L.hp_mf_cholesky_solve.argtypes = [_dp, _dp, _dp, _sz]
+    L.hp_mf_gemv.argtypes = [_dp, _dp, _dp, _sz, _sz]
+    L.hp_mf4_elementwise.argtypes = [C.c_int, _dp, _dp, _dp, _sz]
     L.hp_overlap_s.argtypes = [_dp, _dp, _sz, _dp]
     L.hp_dev_overlap_s.argtypes = [_vp, _vp, _sz, _vp, _vp, _vp]
     L.hp_dev_overlap_forward.argtypes = [_vp, _vp, _vp, _vp, _sz, C.c_int, C.POINTER(_vp), C.POINTER(AdjointInfo),
@@ -361,6 +365,12 @@ def assemble_overlap_mixed(shells, params=SCREEN_DEFAULT):
 
 
 # ---------------------------------------------------------------- eigensolvers
+class NotPositiveDefinite(HyperprecError):
+    """mf_cholesky's Err("Cholesky failed: matrix is not positive definite")
+    (crates/quad/src/multifloat/linalg.rs:103-105); .pivot names the failing column."""
+    pivot = None
+
+
 class NotConverged(HyperprecError):
     """jacobi_eigen's Err("Jacobi eigenvalue solver did not converge ...") (linalg.rs:360)."""
 
@@ -397,6 +407,52 @@ def solve_gen_eigen(f, s, n, threshold):
     m = C.c_size_t(0)
     _check(lib().hp_solve_gen_eigen(f, s, n, threshold, ev, c, C.byref(m)), "solve_gen_eigen")
     return ev[:m.value], c[:n * m.value], m.value
+
+
+# ---------------------------------------------------------------- MultiFloat<4> linear algebra
+def mf_cholesky(a, n):
+    """mf_cholesky::<4> (crates/quad/src/multifloat/linalg.rs:89-141): the factored copy (n*n, 4).
+    Raises NotPositiveDefinite (with .pivot) like the reference's Err."""
+    a = _arr(a, 4, n * n, "a").copy().reshape(-1, 4)
+    piv = C.c_size_t(0)
+    rc = lib().hp_mf_cholesky(a if a.size else np.zeros(4), n, C.byref(piv))
+    if rc == HP_ENOTSPD:
+        e = NotPositiveDefinite(lib().hp_last_error().decode(errors="replace"))
+        e.pivot = piv.value
+        raise e
+    _check(rc, "mf_cholesky")
+    return a
+
+
+def mf_cholesky_solve(l, b, n):
+    """mf_cholesky_solve::<4> (multifloat/linalg.rs:145-172): x (n, 4)."""
+    l = _arr(l, 4, n * n, "l")
+    b = _arr(b, 4, n, "b")
+    x = np.zeros((max(n, 1), 4))
+    _check(lib().hp_mf_cholesky_solve(l if l.size else np.zeros(4), b if b.size else np.zeros(4), x, n),
+           "mf_cholesky_solve")
+    return x[:n]
+
+
+def mf_gemv(a, x, n, m):
+    """mf_gemv::<4> (multifloat/linalg.rs:17-36): y = A x, A n x m; (n, 4)."""
+    a = _arr(a, 4, n * m, "a")
+    x = _arr(x, 4, m, "x")
+    y = np.zeros((max(n, 1), 4))
+    _check(lib().hp_mf_gemv(a if a.size else np.zeros(4), x if x.size else np.zeros(4), y, n, m), "mf_gemv")
+    return y[:n]
+
+
+def mf4_elementwise(op, a, b=None):
+    """MF4 recip / sqrt / div over arrays (multifloat/div.rs:14-77, math.rs:22-40): op in
+    {"recip", "sqrt", "div"}; a (and b) (n, 4)."""
+    code = {"recip": 0, "sqrt": 1, "div": 2}[op]
+    a = np.ascontiguousarray(a, dtype=np.float64).reshape(-1, 4)
+    n = a.shape[0]
+    bb = np.ascontiguousarray(b, dtype=np.float64).reshape(-1, 4) if b is not None else np.zeros((1, 4))
+    out = np.zeros((max(n, 1), 4))
+    _check(lib().hp_mf4_elementwise(code, a if n else np.zeros(4), bb, out, n), "mf4_elementwise")
+    return out[:n]
 
 
 # ---------------------------------------------------------------- reductions

@@ -301,4 +301,39 @@ HP_HD mf4 mf_mul(mf4 a, mf4 b) {
   return mkmf(o[0], o[1], o[2], o[3]);
 }
 
+// MultiFloat<4> reciprocal (multifloat/div.rs:14-41): Newton from the f64 reciprocal,
+// ceil(log2 4) + 2 = 4 steps of x + x (1 - a x); 1/0 is the signed infinity, 1/non-finite 0
+HP_HD mf4 mf_recip(mf4 a) {
+  if (a.l0 == 0.0) return mkmf(signbit(a.l0) ? -(double)INFINITY : (double)INFINITY);
+  if (!isfinite(a.l0)) return mkmf(0.0);
+  mf4 x = mkmf(1.0 / a.l0);
+  const mf4 one = mkmf(1.0);
+  for (int s = 0; s < 4; ++s) {
+    const mf4 residual = mf_sub(one, mf_mul(a, x));
+    x = mf_add(x, mf_mul(x, residual));
+  }
+  return x;
+}
+// a / b (div.rs:44-77): the special cases, then a * recip(b)
+HP_HD mf4 mf_div(mf4 a, mf4 b) {
+  if (b.l0 == 0.0) {
+    const double sign = (!signbit(a.l0) == !signbit(b.l0)) ? 1.0 : -1.0;
+    return mkmf(a.l0 == 0.0 ? (double)NAN : sign * (double)INFINITY);
+  }
+  if (a.l0 == 0.0) return mkmf(0.0);
+  if (!isfinite(a.l0) || !isfinite(b.l0)) return mkmf(a.l0 / b.l0);
+  return mf_mul(a, mf_recip(b));
+}
+// sqrt (multifloat/math.rs:22-40): NaN below zero, 0 at 0, non-finite passed through, else
+// ceil(log2 4) + 1 = 3 steps of (x + a / x) / 2 from the f64 root
+HP_HD mf4 mf_sqrt(mf4 a) {
+  if (a.l0 < 0.0) return mkmf((double)NAN);
+  if (a.l0 == 0.0) return mkmf(0.0);
+  if (!isfinite(a.l0)) return mkmf(a.l0);
+  mf4 x = mkmf(sqrt(a.l0));
+  const mf4 half = mkmf(0.5);
+  for (int s = 0; s < 3; ++s) x = mf_mul(mf_add(x, mf_div(a, x)), half);
+  return x;
+}
+
 }  // namespace hp

@@ -807,20 +807,6 @@ struct DdOps {
   }
 };
 
-// MultiFloat<4> reciprocal and division (multifloat/div.rs:14-77): Newton from the f64
-// reciprocal, ceil(log2 4) + 2 = 4 steps
-__device__ mf4 mf_recip(mf4 a) {
-  if (a.l0 == 0.0) return mkmf(signbit(a.l0) ? -__longlong_as_double(0x7FF0000000000000ll)
-                                             : __longlong_as_double(0x7FF0000000000000ll));
-  if (!isfinite(a.l0)) return mkmf(0.0);
-  mf4 x = mkmf(1.0 / a.l0);
-  const mf4 one = mkmf(1.0);
-  for (int s = 0; s < 4; ++s) {
-    const mf4 residual = mf_sub(one, mf_mul(a, x));
-    x = mf_add(x, mf_mul(x, residual));
-  }
-  return x;
-}
 struct Mf4Ops {
   using T = mf4;
   __device__ static T zero() { return mkmf(0.0); }
