@@ -103,6 +103,10 @@ int hp_mf_gemv(const double* a, const double* x, double* y, size_t n, size_t m);
 /* Elementwise MF4 ops over n values: op 0 recip(a) (multifloat/div.rs:14-41), 1 sqrt(a)
  * (multifloat/math.rs:22-40), 2 a / b (div.rs:44-77); b is read only for op 2. */
 int hp_mf4_elementwise(int op, const double* a, const double* b, double* out, size_t n);
+/* MultiFloat<4>::from_f128 / to_f128 (multifloat/conv.rs:120-131) over n values: limbs
+ * (hi, lo, 0, 0), and (limb 0, limb 1) with limbs 2 and 3 dropped unrounded. */
+int hp_mf4_from_dd(const double* dd, double* mf4, size_t n);
+int hp_dd_from_mf4(const double* mf4, double* dd, size_t n);
 
 /* ------------------------------------------------------------------ Cholesky and solves */
 

@@ -151,6 +151,8 @@ def lib():
     L.hp_mf_cholesky_solve.argtypes = [_dp, _dp, _dp, _sz]
     L.hp_mf_gemv.argtypes = [_dp, _dp, _dp, _sz, _sz]
     L.hp_mf4_elementwise.argtypes = [C.c_int, _dp, _dp, _dp, _sz]
+    L.hp_mf4_from_dd.argtypes = [_dp, _dp, _sz]
+    L.hp_dd_from_mf4.argtypes = [_dp, _dp, _sz]
     L.hp_overlap_s.argtypes = [_dp, _dp, _sz, _dp]
     L.hp_dev_overlap_s.argtypes = [_vp, _vp, _sz, _vp, _vp, _vp]
     L.hp_dev_overlap_forward.argtypes = [_vp, _vp, _vp, _vp, _sz, C.c_int, C.POINTER(_vp), C.POINTER(AdjointInfo),
@@ -453,6 +455,22 @@ def mf4_elementwise(op, a, b=None):
     out = np.zeros((max(n, 1), 4))
     _check(lib().hp_mf4_elementwise(code, a if n else np.zeros(4), bb, out, n), "mf4_elementwise")
     return out[:n]
+
+
+def mf4_from_dd(v):
+    """MultiFloat::<4>::from_f128 (multifloat/conv.rs:120-125) of each dd: (n, 4)."""
+    v = np.ascontiguousarray(v, dtype=np.float64).reshape(-1, 2)
+    out = np.zeros((max(len(v), 1), 4))
+    _check(lib().hp_mf4_from_dd(v if len(v) else np.zeros(2), out, len(v)), "mf4_from_dd")
+    return out[:len(v)]
+
+
+def dd_from_mf4(v):
+    """MultiFloat::<4>::to_f128 (multifloat/conv.rs:127-129) of each MF4: (n, 2)."""
+    v = np.ascontiguousarray(v, dtype=np.float64).reshape(-1, 4)
+    out = np.zeros((max(len(v), 1), 2))
+    _check(lib().hp_dd_from_mf4(v if len(v) else np.zeros(4), out, len(v)), "dd_from_mf4")
+    return out[:len(v)]
 
 
 # ---------------------------------------------------------------- reductions

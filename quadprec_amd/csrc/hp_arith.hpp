@@ -229,7 +229,9 @@ HP_HD mf4 mf_add(mf4 a, mf4 b) {
     }
     if (moving) w[0] = v;
   }
-  // grow-expansion, most significant first (add.rs:60-95)
+  // grow-expansion, most significant first (add.rs:60-95).  Before term i the expansion
+  // holds at most i limbs (each term adds at most one), so the loops stop at i: the same
+  // operations as running them to 8 under the len guard, which skips the rest
   double ex[8] = {0, 0, 0, 0, 0, 0, 0, 0};
   int len = 0;
 #pragma unroll
@@ -239,7 +241,7 @@ HP_HD mf4 mf_add(mf4 a, mf4 b) {
     if (len == 0) { ex[0] = term; len = 1; continue; }
     double carry = term;
 #pragma unroll
-    for (int j = 7; j >= 0; --j) {
+    for (int j = i - 1; j >= 0; --j) {
       if (j < len) {
         double s, e;
         two_sum(ex[j], carry, s, e);
@@ -250,7 +252,7 @@ HP_HD mf4 mf_add(mf4 a, mf4 b) {
     if (carry != 0.0) {
       int nl = len + 1;
 #pragma unroll
-      for (int q = 7; q >= 1; --q)
+      for (int q = i; q >= 1; --q)
         if (q < nl) ex[q] = ex[q - 1];
       ex[0] = carry;
       len = nl;

@@ -194,6 +194,17 @@ __global__ void k_mf_elem(int op, const double* a, const double* b, double* out,
   stv(out, n, i, r);
 }
 
+// MultiFloat<4>::from_f128 / to_f128 (multifloat/conv.rs:120-131) over arrays: limbs
+// (hi, lo, 0, 0), and (limb 0, limb 1) with limbs 2 and 3 dropped; AoS in and out
+__global__ void k_mf4_from_dd(const double2* __restrict__ in, double4* __restrict__ out, long n) {
+  const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) out[i] = make_double4(in[i].x, in[i].y, 0.0, 0.0);
+}
+__global__ void k_dd_from_mf4(const double4* __restrict__ in, double2* __restrict__ out, long n) {
+  const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) out[i] = make_double2(in[i].x, in[i].y);
+}
+
 unsigned blocks_of(long n, int t) { return (unsigned)((n + t - 1) / t); }
 
 }  // namespace
@@ -278,6 +289,39 @@ int hp_mf_gemv(const double* a, const double* x, double* y, size_t n, size_t m) 
                      dy.d());
   HP_LAUNCH_CHECK();
   return download_planes(dy.d(), n, 4, y, s);
+}
+
+// dd <-> MF4 conversion of n values (from_f128 / to_f128), device or host AoS arrays
+// through the device
+int hp_mf4_from_dd(const double* dd, double* mf4, size_t n) {
+  if (n == 0) return HP_OK;
+  if (!dd || !mf4) return null_arg();
+  hipStream_t s = hipStreamPerThread;
+  DevBuf di, dout;
+  int rc;
+  if ((rc = di.alloc(2 * n * sizeof(double))) || (rc = dout.alloc(4 * n * sizeof(double)))) return rc;
+  HP_HIP(hipMemcpyAsync(di.p, dd, 2 * n * sizeof(double), hipMemcpyHostToDevice, s));
+  hipLaunchKernelGGL(k_mf4_from_dd, dim3(blocks_of((long)n, 256)), dim3(256), 0, s,
+                     static_cast<const double2*>(di.p), static_cast<double4*>(dout.p), (long)n);
+  HP_LAUNCH_CHECK();
+  HP_HIP(hipMemcpyAsync(mf4, dout.p, 4 * n * sizeof(double), hipMemcpyDeviceToHost, s));
+  HP_HIP(hipStreamSynchronize(s));
+  return HP_OK;
+}
+int hp_dd_from_mf4(const double* mf4, double* dd, size_t n) {
+  if (n == 0) return HP_OK;
+  if (!dd || !mf4) return null_arg();
+  hipStream_t s = hipStreamPerThread;
+  DevBuf di, dout;
+  int rc;
+  if ((rc = di.alloc(4 * n * sizeof(double))) || (rc = dout.alloc(2 * n * sizeof(double)))) return rc;
+  HP_HIP(hipMemcpyAsync(di.p, mf4, 4 * n * sizeof(double), hipMemcpyHostToDevice, s));
+  hipLaunchKernelGGL(k_dd_from_mf4, dim3(blocks_of((long)n, 256)), dim3(256), 0, s,
+                     static_cast<const double4*>(di.p), static_cast<double2*>(dout.p), (long)n);
+  HP_LAUNCH_CHECK();
+  HP_HIP(hipMemcpyAsync(dd, dout.p, 2 * n * sizeof(double), hipMemcpyDeviceToHost, s));
+  HP_HIP(hipStreamSynchronize(s));
+  return HP_OK;
 }
 
 // elementwise MF4 ops on n values (AoS, 4 doubles each): op 0 recip(a), 1 sqrt(a), 2 a / b

@@ -112,3 +112,12 @@ def test_mf_gemv_bit_exact(qp, n, m):
     y = qp.mf_gemv(a, x, n, m)
     ref = O.mf4_gemv(a, x, n, m) if m else np.zeros((n, 4))
     assert same(y, ref)
+
+
+def test_dd_mf4_conversions(qp):  # multifloat/conv.rs:120-131
+    rng = np.random.default_rng(3)
+    m = mf4_random(rng, 1000, 5)
+    d = qp.dd_from_mf4(m)
+    assert np.array_equal(d, m[:, :2])
+    back = qp.mf4_from_dd(d)
+    assert np.array_equal(back[:, :2], d) and not back[:, 2:].any()
