@@ -240,12 +240,12 @@ __host__ __device__ constexpr int cart_y(int L, int v) {
   return 0;
 }
 
+// thread t of bucket (LA, LB, TIER): row u = t mod NU of listed pair t / NU
 template <int LA, int LB, int TIER>
-__global__ __launch_bounds__(128) void k_rows(const ShellRec* __restrict__ sh, const int2* __restrict__ list, long cnt,
-                                              const double* __restrict__ alpha, const double* __restrict__ coeff,
-                                              long n_ao, double* __restrict__ s) {
+__device__ __forceinline__ void rows_body(const ShellRec* __restrict__ sh, const int2* __restrict__ list, long cnt,
+                                          const double* __restrict__ alpha, const double* __restrict__ coeff,
+                                          long n_ao, double* __restrict__ s, long t) {
   constexpr int NU = (LA + 1) * (LA + 2) / 2, NV = (LB + 1) * (LB + 2) / 2;
-  const long t = (long)blockIdx.x * 128 + threadIdx.x;
   if (t >= cnt * NU) return;
   const long k = t / NU;
   const int u = (int)(t - k * NU);
@@ -363,34 +363,67 @@ __global__ __launch_bounds__(256) void k_mirror(double* __restrict__ s, const in
   }
 }
 
-using RowsFn = void (*)(const ShellRec*, const int2*, long, const double*, const double*, long, double*);
-template <int TIER, int LA, int LB>
-void launch_rows(const ShellRec* sh, const int2* list, long cnt, const double* al, const double* co, long n_ao,
-                 double* s, hipStream_t st) {
-  constexpr int NU = (LA + 1) * (LA + 2) / 2;
-  const long threads = cnt * NU;
-  hipLaunchKernelGGL((k_rows<LA, LB, TIER>), dim3((unsigned)((threads + 127) / 128)), dim3(128), 0, st, sh, list, cnt,
-                     al, co, n_ao, s);
-}
-template <int TIER, int LA>
-void launch_rows_lb(int lb, const ShellRec* sh, const int2* list, long cnt, const double* al, const double* co,
-                    long n_ao, double* s, hipStream_t st) {
-  switch (lb) {
-    case 0: launch_rows<TIER, LA, 0>(sh, list, cnt, al, co, n_ao, s, st); break;
-    case 1: launch_rows<TIER, LA, 1>(sh, list, cnt, al, co, n_ao, s, st); break;
-    case 2: launch_rows<TIER, LA, 2>(sh, list, cnt, al, co, n_ao, s, st); break;
-    default: launch_rows<TIER, LA, 3>(sh, list, cnt, al, co, n_ao, s, st); break;
-  }
-}
+// One launch per tier covers all its (l_i, l_j) buckets: blocks [first[e], first[e + 1])
+// run bucket code[e] (4 l_i + l_j) over its count[e] pairs listed from start[e].  The host
+// orders the buckets by descending per-thread work so that the longest blocks start first.
+struct RowsTab {
+  unsigned int first[17], start[16], count[16];
+  int code[16], n;
+};
+constexpr int ROWS_THREADS = 128;
+
 template <int TIER>
-void launch_rows_la(int la, int lb, const ShellRec* sh, const int2* list, long cnt, const double* al,
-                    const double* co, long n_ao, double* s, hipStream_t st) {
-  switch (la) {
-    case 0: launch_rows_lb<TIER, 0>(lb, sh, list, cnt, al, co, n_ao, s, st); break;
-    case 1: launch_rows_lb<TIER, 1>(lb, sh, list, cnt, al, co, n_ao, s, st); break;
-    case 2: launch_rows_lb<TIER, 2>(lb, sh, list, cnt, al, co, n_ao, s, st); break;
-    default: launch_rows_lb<TIER, 3>(lb, sh, list, cnt, al, co, n_ao, s, st); break;
+__global__ __launch_bounds__(ROWS_THREADS) void k_rows(const ShellRec* __restrict__ sh, const int2* __restrict__ list,
+                                                       RowsTab tab, const double* __restrict__ alpha,
+                                                       const double* __restrict__ coeff, long n_ao,
+                                                       double* __restrict__ s) {
+  const unsigned int blk = blockIdx.x;
+  int e = 0;
+  while (e + 1 < tab.n && blk >= tab.first[e + 1]) ++e;
+  const long t = (long)(blk - tab.first[e]) * ROWS_THREADS + threadIdx.x;
+  const int2* lst = list + tab.start[e];
+  const long cnt = tab.count[e];
+#define HP_ROWS_CASE(LA, LB) \
+  case LA * 4 + LB: rows_body<LA, LB, TIER>(sh, lst, cnt, alpha, coeff, n_ao, s, t); break;
+  switch (tab.code[e]) {
+    HP_ROWS_CASE(0, 0) HP_ROWS_CASE(0, 1) HP_ROWS_CASE(0, 2) HP_ROWS_CASE(0, 3)
+    HP_ROWS_CASE(1, 0) HP_ROWS_CASE(1, 1) HP_ROWS_CASE(1, 2) HP_ROWS_CASE(1, 3)
+    HP_ROWS_CASE(2, 0) HP_ROWS_CASE(2, 1) HP_ROWS_CASE(2, 2) HP_ROWS_CASE(2, 3)
+    HP_ROWS_CASE(3, 0) HP_ROWS_CASE(3, 1) HP_ROWS_CASE(3, 2) HP_ROWS_CASE(3, 3)
+    default: break;
   }
+#undef HP_ROWS_CASE
+}
+
+// the nonempty buckets of one tier, heaviest per thread first (larger l_i + l_j, then l_i)
+RowsTab rows_table(int tier, const unsigned int* hcount, const unsigned int* hstart) {
+  RowsTab tab{};
+  int order[16];
+  int n = 0;
+  for (int c = 0; c < 16; ++c)
+    if (hcount[tier * 16 + c]) order[n++] = c;
+  for (int i = 1; i < n; ++i)
+    for (int j = i; j > 0; --j) {
+      const int x = order[j - 1], y = order[j];
+      const int kx = ((x >> 2) + (x & 3)) * 4 + (x >> 2), ky = ((y >> 2) + (y & 3)) * 4 + (y >> 2);
+      if (ky <= kx) break;
+      order[j - 1] = y;
+      order[j] = x;
+    }
+  unsigned int blocks = 0;
+  for (int e = 0; e < n; ++e) {
+    const int c = order[e], la = c >> 2;
+    const unsigned int cnt = hcount[tier * 16 + c];
+    const unsigned long long threads = (unsigned long long)cnt * ((la + 1) * (la + 2) / 2);
+    tab.first[e] = blocks;
+    tab.start[e] = hstart[tier * 16 + c];
+    tab.count[e] = cnt;
+    tab.code[e] = c;
+    blocks += (unsigned int)((threads + ROWS_THREADS - 1) / ROWS_THREADS);
+  }
+  tab.first[n] = blocks;
+  tab.n = n;
+  return tab;
 }
 
 }  // namespace
@@ -489,13 +522,15 @@ int assemble_mixed(const double* centers, const int* l, const long* prim_off, co
   hipLaunchKernelGGL(k_classify<true>, dim3(cgrid), dim3(CL_THREADS), 0, st, dshp, npairs, dprm.d(),
                      (unsigned long long*)nullptr, bcount, (const unsigned int*)bstart, list);
   HP_LAUNCH_CHECK();
-  for (int b = 0; b < NBUCKET; ++b) {
-    if (!hcount[b]) continue;
-    const int tier = b >> 4, la = (b >> 2) & 3, lb = b & 3;
-    if (tier == 0)
-      launch_rows_la<0>(la, lb, dshp, list + hstart[b], hcount[b], dal.d(), dco.d(), (long)n_ao, s_dev, st);
+  for (int tier = 1; tier >= 0; --tier) {
+    const RowsTab tab = rows_table(tier, hcount, hstart);
+    if (!tab.n) continue;
+    if (tier)
+      hipLaunchKernelGGL(k_rows<1>, dim3(tab.first[tab.n]), dim3(ROWS_THREADS), 0, st, dshp, list, tab, dal.d(),
+                         dco.d(), (long)n_ao, s_dev);
     else
-      launch_rows_la<1>(la, lb, dshp, list + hstart[b], hcount[b], dal.d(), dco.d(), (long)n_ao, s_dev, st);
+      hipLaunchKernelGGL(k_rows<0>, dim3(tab.first[tab.n]), dim3(ROWS_THREADS), 0, st, dshp, list, tab, dal.d(),
+                         dco.d(), (long)n_ao, s_dev);
     HP_LAUNCH_CHECK();
   }
   const unsigned tiles = (unsigned)((n_ao + 63) / 64);

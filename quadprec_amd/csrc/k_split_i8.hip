@@ -62,6 +62,10 @@ struct CrtTable {
   double mr40[CRT_MAXR][CRT_L40];          // M / m_r in 40-bit limbs
   double m40[CRT_L40];                     // M in 40-bit limbs
   double inv_M;                            // 1 / M, rounded
+  // k_crt's INT8 operand: entry [kb][rho][kk] is byte b(rho) (in [-128, 127]) of the balanced
+  // base-256 digits of M/m_r for k = 32 kb + kk = r < R, of Z for k = R, 0 above; rho -> b
+  // is crt_byte_of_row (bytes 0..29)
+  int8_t mrb[2][32][32];
 };
 
 namespace {
@@ -126,6 +130,99 @@ void build_table(int R, CrtTable& T) {
     T.split[r][2] = (double)m;
     T.split[r][3] = T.inv_m[r];
   }
+}
+
+// Balanced base-256 digits (each in [-128, 127]) of the two's-complement integer x of
+// CRT_LIMBS 32-bit limbs (negated first if neg); false if nbytes digits do not hold it.
+bool balanced_bytes(const uint32_t* x, bool neg, int8_t* out, int nbytes) {
+  constexpr int NL = CRT_LIMBS + 1;
+  uint32_t v[NL];
+  for (int i = 0; i < CRT_LIMBS; ++i) v[i] = neg ? ~x[i] : x[i];
+  v[CRT_LIMBS] = neg ? ~0u : 0u;
+  if (neg) {  // + 1
+    for (int i = 0; i < NL && ++v[i] == 0; ++i) {
+    }
+  }
+  for (int b = 0; b < nbytes; ++b) {
+    int d = (int)(v[0] & 255u);
+    if (d >= 128) d -= 256;
+    out[b] = (int8_t)d;
+    uint64_t carry = 0;  // v -= d
+    const uint32_t sub = (uint32_t)(-d), ext = -d < 0 ? ~0u : 0u;
+    for (int i = 0; i < NL; ++i) {
+      const uint64_t t = (uint64_t)v[i] + (i == 0 ? sub : ext) + carry;
+      v[i] = (uint32_t)t;
+      carry = t >> 32;
+    }
+    for (int i = 0; i < NL - 1; ++i) v[i] = (v[i] >> 8) | (v[i + 1] << 24);  // exact arithmetic shift
+    v[NL - 1] = (uint32_t)((int32_t)v[NL - 1] >> 8);
+  }
+  for (int i = 0; i < NL; ++i)
+    if (v[i]) return false;
+  return true;
+}
+bool big_ge(const uint32_t* a, const uint32_t* b) {
+  for (int i = CRT_LIMBS - 1; i >= 0; --i)
+    if (a[i] != b[i]) return a[i] > b[i];
+  return true;
+}
+void big_sub(uint32_t* a, const uint32_t* b) {  // a -= b, a >= b
+  int64_t borrow = 0;
+  for (int i = 0; i < CRT_LIMBS; ++i) {
+    const int64_t t = (int64_t)a[i] - b[i] - borrow;
+    a[i] = (uint32_t)t;
+    borrow = t < 0;
+  }
+}
+void big_add(uint32_t* a, const uint32_t* b) {
+  uint64_t carry = 0;
+  for (int i = 0; i < CRT_LIMBS; ++i) {
+    const uint64_t t = (uint64_t)a[i] + b[i] + carry;
+    a[i] = (uint32_t)t;
+    carry = t >> 32;
+  }
+}
+
+// k_crt's MFMA output row rho holds, in the lane half h = (rho >> 2) & 1, the entry q =
+// 4 (rho >> 3) + (rho & 3) of that half; entries q < 15 are byte 5 (2 (q / 5) + h) + q % 5,
+// i.e. the half h holds the 40-bit limbs 2g + h (g < 3) five bytes each; -1: unused row.
+int crt_byte_of_row(int rho) {
+  const int q = 4 * (rho >> 3) + (rho & 3), h = (rho >> 2) & 1;
+  return q < 15 ? 5 * (2 * (q / 5) + h) + q % 5 : -1;
+}
+
+// the balanced digits of M/m_r and of Z = 128 sum_r M/m_r reduced into [-M/2, M/2]
+bool build_crt_bytes(int R, CrtTable& T) {
+  int8_t bal[CRT_MAXR + 1][30];
+  uint32_t M[CRT_LIMBS] = {1, 0, 0, 0, 0, 0, 0, 0}, Z[CRT_LIMBS] = {0, 0, 0, 0, 0, 0, 0, 0};
+  for (int r = 0; r < R; ++r) big_mul_small(M, kModuli[r]);
+  for (int r = 0; r < R; ++r) {
+    uint32_t Mr[CRT_LIMBS] = {1, 0, 0, 0, 0, 0, 0, 0};
+    for (int q = 0; q < R; ++q)
+      if (q != r) big_mul_small(Mr, kModuli[q]);
+    if (!balanced_bytes(Mr, false, bal[r], 30)) return false;
+    big_add(Z, Mr);
+  }
+  big_mul_small(Z, 128);  // < 2^244
+  while (big_ge(Z, M)) big_sub(Z, M);
+  uint32_t twoZ[CRT_LIMBS];
+  std::memcpy(twoZ, Z, sizeof(twoZ));
+  big_add(twoZ, Z);
+  const bool neg = big_ge(twoZ, M) && std::memcmp(twoZ, M, sizeof(M)) != 0;  // Z > M/2: use Z - M
+  if (neg) {
+    uint32_t t[CRT_LIMBS];
+    std::memcpy(t, M, sizeof(t));
+    big_sub(t, Z);
+    std::memcpy(Z, t, sizeof(t));
+  }
+  if (!balanced_bytes(Z, neg, bal[R], 30)) return false;
+  for (int kb = 0; kb < 2; ++kb)
+    for (int rho = 0; rho < 32; ++rho)
+      for (int kk = 0; kk < 32; ++kk) {
+        const int k = 32 * kb + kk, b = crt_byte_of_row(rho);
+        T.mrb[kb][rho][kk] = (b >= 0 && k <= R) ? bal[k][b] : 0;
+      }
+  return true;
 }
 
 double log2_M(int R) {
@@ -807,17 +904,25 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 1))) voi
 }
 
 // ------------------------------------------------------------------ CRT reconstruction
-// S = sum_r u_r (M/m_r) in limbs s[0..5] (weights 2^(40q), each an integer below 2^53)
-// -> C' = S - round(S/M) M (|C'| < M/16, so S/M is within 1/16 of an integer and the
-// rounding is robust) -> dd value of C' 2^sh.  Every step is exact up to the final sum of
-// the six limbs into a dd.
+// dd_add(a, mkdd(b)) for a finite a.lo: the two_sum of a.lo and the zero low part of b
+// returns a.lo + 0.0 with the error +0.0, so only the constants are folded in; the same
+// operations otherwise, hence the same result.  (And dd_add(mkdd(0.0), mkdd(x)) is
+// mkdd(x) for a finite x.)
+__device__ __forceinline__ dd dd_add_d(dd a, double b) {
+  double s1, e1, s, e;
+  two_sum(a.hi, b, s1, e1);
+  quick_two_sum(s1, e1 + (a.lo + 0.0), s, e);
+  return renorm(s, e + 0.0);
+}
+
+// C' = S - round(S/M) M from limbs s[0..5] (weights 2^(40q), signed integers below 2^51.2
+// in magnitude) of an S congruent to sum_r u_r (M/m_r) mod M with |S| < 26 M.  |C'| < M/16,
+// so S/M is within 1/16 of an integer, and the Horner estimate (relative error ~2^-32 of M
+// here) rounds to the right quotient.  The subtraction and the carry pass are exact and
+// give the unique normalised limbs of C' (s[0..4] in [0, 2^40)), so the dd value of C' 2^sh
+// that follows depends on C' only; every step is exact up to the final sum of the six
+// limbs into a dd.
 __device__ __forceinline__ dd crt_finish(double (&s)[CRT_L40], const double* mt, int sh) {
-#pragma unroll
-  for (int q = 0; q < CRT_L40 - 1; ++q) {
-    const double c = __builtin_floor(s[q] * 0x1p-40);
-    s[q] = __builtin_fma(c, -0x1p40, s[q]);
-    s[q + 1] += c;
-  }
   double sv = s[CRT_L40 - 1];
 #pragma unroll
   for (int q = CRT_L40 - 2; q >= 0; --q) sv = __builtin_fma(sv, 0x1p40, s[q]);
@@ -841,55 +946,126 @@ __device__ __forceinline__ dd crt_finish(double (&s)[CRT_L40], const double* mt,
       s[q + 1] += c;
     }
   }
-  dd res = mkdd(0.0);
+  dd res = mkdd(__builtin_ldexp(s[CRT_L40 - 1], 40 * (CRT_L40 - 1) + sh));
 #pragma unroll
-  for (int q = CRT_L40 - 1; q >= 0; --q) res = dd_add(res, mkdd(__builtin_ldexp(s[q], 40 * q + sh)));
+  for (int q = CRT_L40 - 2; q >= 0; --q) res = dd_add_d(res, __builtin_ldexp(s[q], 40 * q + sh));
   return neg ? dd_neg(res) : res;
 }
 
-// a thread takes 4 consecutive columns of one row (one dword of digits per modulus)
+// CRT on the INT8 matrix cores.  With the digit byte read as s_r = u_r - 128 (its top bit
+// flipped, as int8) and Z = 128 sum_r M/m_r reduced into [-M/2, M/2], S = sum_r s_r M/m_r + Z
+// is congruent to sum_r u_r M/m_r mod M.  In balanced base-256 digits (bytes in [-128, 127])
+// of M/m_r and Z, S = sum_b P_b 256^b with P_b = sum_k T[b][k] D[k][j]: an int8 x int8 ->
+// int32 product over k = modulus (k = R carries the constant digit 1 that picks up Z), one
+// v_mfma_i32_32x32x32_i8 per 32 elements and 32 moduli.  |P_b| <= 33 * 2^14 < 2^19.1, so the
+// 40-bit limbs sum_{j<5} P_(5g+j) 2^(8j) are exact doubles below 2^51.1.
+//
+// A block takes 256 elements of a row per step.  Wavefront w loads plane quads w and w + 4
+// (and those of the second block of 32 moduli), lane l the dword of elements 4l..4l+3 of
+// each plane: 256 contiguous bytes per load, issued unconditionally (clamped addresses,
+// selected afterwards).  Each lane turns its 4 dwords into 4 dwords of 4 planes of one
+// element (byte transpose) and stores them to LDS rows of ROWD = 8 KB + 1 dwords (KB
+// blocks of 32 moduli), element e of the tile in row crt_row(e), a permutation within
+// groups of 4: the 32 lanes of every store and of every ds_read_b32 of the product's
+// column operand hit 32 distinct banks.  Product rows (crt_byte_of_row) put the limbs
+// 2g + h in lane half h, so one v_permlane32_swap per dword leaves each lane with the six
+// limbs of one element: lane l of wavefront w finishes element 64 w + l of the tile.
+constexpr int CRT_TILE = 256;
+__device__ __forceinline__ int crt_row(int e) { return (e & ~3) | (((e & 3) + (e >> 5)) & 3); }
+
+template <int KB>
 __global__ __launch_bounds__(256) void k_crt(const uint8_t* __restrict__ Cr, long M, long N, long Np,
                                              const int* __restrict__ er, const int* __restrict__ fc, int W,
                                              const CrtTable* __restrict__ tab, double* __restrict__ chi,
                                              double* __restrict__ clo, long ldc, int lower, int subtract) {
-  __shared__ __align__(16) double mr[CRT_MAXR][CRT_L40];
+  constexpr int ROWD = 8 * KB + 1;
+  __shared__ uint32_t dig[CRT_TILE * ROWD];
   __shared__ double mt[CRT_L40 + 1];
   const int R = tab->R;
-  for (int x = threadIdx.x; x < R * CRT_L40; x += blockDim.x) mr[x / CRT_L40][x % CRT_L40] = tab->mr40[x / CRT_L40][x % CRT_L40];
-  if (threadIdx.x < CRT_L40) mt[threadIdx.x] = tab->m40[threadIdx.x];
-  if (threadIdx.x == CRT_L40) mt[CRT_L40] = tab->inv_M;
+  const int t = threadIdx.x, lane = t & 63, w = __builtin_amdgcn_readfirstlane(t >> 6), h = lane >> 5;
+  if (t < CRT_L40) mt[t] = tab->m40[t];
+  if (t == CRT_L40) mt[CRT_L40] = tab->inv_M;
+  v4i_t tf[KB];
+#pragma unroll
+  for (int kb = 0; kb < KB; ++kb) tf[kb] = *reinterpret_cast<const v4i_t*>(&tab->mrb[kb][lane & 31][16 * h]);
   __syncthreads();
   const size_t plane = (size_t)M * Np;
-  const long j0 = ((long)blockIdx.x * 256 + threadIdx.x) * 4;
-  for (long i = blockIdx.y; i < M; i += gridDim.y) {
-    if (j0 >= N || (lower && j0 > i)) continue;
-    const uint8_t* src = Cr + (size_t)i * Np + j0;
-    double s[4][CRT_L40];
+  const long j0 = (long)blockIdx.x * CRT_TILE, j = j0 + 64 * w + lane;
+  const long jq = j0 + 4 * lane, jc = min(jq, Np - 4);  // Np >= 4, a multiple of 4
+  int wrow[4];
 #pragma unroll
-    for (int e = 0; e < 4; ++e)
+  for (int e = 0; e < 4; ++e) wrow[e] = crt_row(4 * lane + e) * ROWD;
+  const int rrow[2] = {crt_row(64 * w + (lane & 31)) * ROWD + 4 * h, crt_row(64 * w + 32 + (lane & 31)) * ROWD + 4 * h};
+  // rows i = blockIdx.y + n gridDim.y; the next row's digits are loaded before this row's
+  // finish.  In the lower mode, rows above the tile's first column are skipped.
+  const long gy = gridDim.y;
+  long i = blockIdx.y;
+  if (lower && i < j0) i += (j0 - i + gy - 1) / gy * gy;
+  uint32_t d[KB][2][4];
+  auto fetch = [&](long ii) {
+    const uint8_t* row = Cr + (size_t)ii * Np + jc;
 #pragma unroll
-      for (int q = 0; q < CRT_L40; ++q) s[e][q] = 0.0;
-    for (int r = 0; r < R; ++r) {
-      const uint32_t d = *reinterpret_cast<const uint32_t*>(src + (size_t)r * plane);
-      const double u[4] = {(double)(d & 255u), (double)((d >> 8) & 255u), (double)((d >> 16) & 255u), (double)(d >> 24)};
+    for (int kb = 0; kb < KB; ++kb)
 #pragma unroll
-      for (int q = 0; q < CRT_L40; ++q) {
-        const double mq = mr[r][q];
+      for (int p = 0; p < 2; ++p)
 #pragma unroll
-        for (int e = 0; e < 4; ++e) s[e][q] = __builtin_fma(u[e], mq, s[e][q]);
+        for (int k = 0; k < 4; ++k)
+          d[kb][p][k] = *reinterpret_cast<const uint32_t*>(row + (size_t)min(32 * kb + 4 * (w + 4 * p) + k, R - 1) * plane);
+  };
+  if (i < M) fetch(i);
+  for (; i < M; i += gy) {
+#pragma unroll
+    for (int kb = 0; kb < KB; ++kb)
+#pragma unroll
+      for (int p = 0; p < 2; ++p) {
+        const int rq = w + 4 * p;
+        uint32_t v[4];
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+          const int r = 32 * kb + 4 * rq + k;
+          v[k] = r < R ? (jq < Np ? d[kb][p][k] ^ 0x80808080u : 0u) : (r == R ? 0x01010101u : 0u);
+        }
+#pragma unroll
+        for (int e = 0; e < 4; ++e)
+          dig[wrow[e] + 8 * kb + rq] = ((v[0] >> (8 * e)) & 255u) | (((v[1] >> (8 * e)) & 255u) << 8) |
+                                      (((v[2] >> (8 * e)) & 255u) << 16) | ((v[3] >> (8 * e)) << 24);
+      }
+    __syncthreads();
+    double y[2][3];
+#pragma unroll
+    for (int c = 0; c < 2; ++c) {
+      v16i_t acc;
+#pragma unroll
+      for (int q = 0; q < 16; ++q) acc[q] = 0;
+#pragma unroll
+      for (int kb = 0; kb < KB; ++kb) {
+        const uint32_t* src = &dig[rrow[c] + 8 * kb];
+        const v4i_t bf = {(int)src[0], (int)src[1], (int)src[2], (int)src[3]};
+        acc = __builtin_amdgcn_mfma_i32_32x32x32_i8(tf[kb], bf, acc, 0, 0, 0);
+      }
+#pragma unroll
+      for (int g = 0; g < 3; ++g) {
+        const int a = acc[5 * g] + 256 * acc[5 * g + 1], b = acc[5 * g + 2] + 256 * acc[5 * g + 3];
+        y[c][g] = __builtin_fma((double)acc[5 * g + 4], 0x1p32, __builtin_fma((double)b, 0x1p16, (double)a));
       }
     }
-    const int base = er[i] - 2 * W;
+    __syncthreads();  // dig is restaged for the next row
+    if (i + gy < M) fetch(i + gy);
+    double sl[CRT_L40];
 #pragma unroll
-    for (int e = 0; e < 4; ++e) {
-      const long j = j0 + e;
-      if (j >= N || (lower && j > i)) continue;
-      dd res = crt_finish(s[e], mt, base + fc[j]);
-      const size_t co = (size_t)i * ldc + j;
-      if (subtract) res = dd_sub(mkdd(chi[co], clo[co]), res);
-      chi[co] = res.hi;
-      clo[co] = res.lo;
+    for (int g = 0; g < 3; ++g) {
+      const uint64_t e0 = __builtin_bit_cast(uint64_t, y[0][g]), o0 = __builtin_bit_cast(uint64_t, y[1][g]);
+      const auto lw = __builtin_amdgcn_permlane32_swap((uint32_t)e0, (uint32_t)o0, false, false);
+      const auto hw = __builtin_amdgcn_permlane32_swap((uint32_t)(e0 >> 32), (uint32_t)(o0 >> 32), false, false);
+      sl[2 * g] = __builtin_bit_cast(double, (uint64_t)lw[0] | ((uint64_t)hw[0] << 32));
+      sl[2 * g + 1] = __builtin_bit_cast(double, (uint64_t)lw[1] | ((uint64_t)hw[1] << 32));
     }
+    if (j >= N || (lower && j > i)) continue;
+    dd res = crt_finish(sl, mt, er[i] - 2 * W + fc[j]);
+    const size_t co = (size_t)i * ldc + j;
+    if (subtract) res = dd_sub(mkdd(chi[co], clo[co]), res);
+    chi[co] = res.hi;
+    clo[co] = res.lo;
   }
 }
 
@@ -996,6 +1172,10 @@ int split_gemm_general(const SplitOperand& A, const SplitOperand& B, double* chi
   if (g_work.tabR != R) {
     static CrtTable host;
     build_table(R, host);
+    if (!build_crt_bytes(R, host)) {
+      set_error("split int8 gemm: CRT digit table for %d moduli does not fit 30 bytes", R);
+      return HP_EINVAL;
+    }
     HP_HIP(hipMemcpy(g_work.tab, &host, sizeof(CrtTable), hipMemcpyHostToDevice));
     g_work.tabR = R;
   }
@@ -1099,8 +1279,17 @@ int split_gemm_general(const SplitOperand& A, const SplitOperand& B, double* chi
   }
   if (int rc = launch(variant)) return rc;
   tm.mark(s);
-  hipLaunchKernelGGL(k_crt, dim3((unsigned)((n + 1023) / 1024), gy(m)), dim3(256), 0, s, rC, (long)m, (long)n,
-                     (long)Np, eA, fB, W, g_work.tab, chi, clo, (long)ldc, low, opt.subtract ? 1 : 0);
+  // about 2048 blocks (8 per CU), each taking rows blockIdx.y + n gridDim.y
+  const unsigned crt_gx = (unsigned)((n + CRT_TILE - 1) / CRT_TILE);
+  const unsigned crt_gy = (unsigned)std::max<size_t>(1, std::min<size_t>(m, (2048 + crt_gx - 1) / crt_gx));
+  if (R < 32)
+    hipLaunchKernelGGL(k_crt<1>, dim3(crt_gx, crt_gy), dim3(256), 0, s, rC,
+                       (long)m, (long)n, (long)Np, eA, fB, W, g_work.tab, chi, clo, (long)ldc, low,
+                       opt.subtract ? 1 : 0);
+  else
+    hipLaunchKernelGGL(k_crt<2>, dim3(crt_gx, crt_gy), dim3(256), 0, s, rC,
+                       (long)m, (long)n, (long)Np, eA, fB, W, g_work.tab, chi, clo, (long)ldc, low,
+                       opt.subtract ? 1 : 0);
   HP_LAUNCH_CHECK();
   tm.mark(s);
   // the workspace is reused by the next call: keep it owned until this one has run
