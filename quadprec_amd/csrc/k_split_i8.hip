@@ -275,12 +275,21 @@ __device__ __forceinline__ uint8_t crt_digit(int v, double m, double inv_m, doub
 }
 
 // ------------------------------------------------------------------ scale exponents
-// row scales: element (row, p) at base[row*ld + p] (contiguous p); one wavefront per row
+// row scales: element (row, p) at base[row*ld + p] (contiguous p); one wavefront per row,
+// 8 loads in flight per lane
 __global__ void k_scale_rows(const double* __restrict__ hi, long rows, long K, long ld, int* __restrict__ e) {
   for (long row = (long)blockIdx.x * 4 + (threadIdx.x >> 6); row < rows; row += (long)gridDim.x * 4) {
     const int lane = threadIdx.x & 63;
     int mx = -100000;
-    for (long p = lane; p < K; p += 64) mx = max(mx, frexp_exp(hi[row * ld + p]));
+    long p = lane;
+    for (; p + 7 * 64 < K; p += 8 * 64) {
+      double v[8];
+#pragma unroll
+      for (int u = 0; u < 8; ++u) v[u] = hi[row * ld + p + 64 * u];
+#pragma unroll
+      for (int u = 0; u < 8; ++u) mx = max(mx, frexp_exp(v[u]));
+    }
+    for (; p < K; p += 64) mx = max(mx, frexp_exp(hi[row * ld + p]));
     for (int o = 32; o > 0; o >>= 1) mx = max(mx, __shfl_xor(mx, o, 64));
     if (lane == 0) e[row] = mx;
   }
@@ -290,8 +299,9 @@ __global__ void k_fill_int(int* p, long n, int v) {
   if (i < n) p[i] = v;
 }
 // column scales: element (col, p) at base[p*ld + col]; a block takes 64 columns x 256 p
-// (wavefront w the 64 p from 64w, lanes along the columns), reduces through LDS and
-// combines with one atomicMax per column (e[] preset to -100000 by k_fill_int)
+// (wavefront w the 64 p from 64w, lanes along the columns, 8 loads in flight per lane),
+// reduces through LDS and combines with one atomicMax per column (e[] preset to -100000
+// by k_fill_int)
 __global__ __launch_bounds__(256) void k_scale_cols(const double* __restrict__ hi, long cols, long K, long ld,
                                                     int* __restrict__ e) {
   __shared__ int red[4][64];
@@ -299,8 +309,17 @@ __global__ __launch_bounds__(256) void k_scale_cols(const double* __restrict__ h
   const long col = (long)blockIdx.x * 64 + lane;
   const long pa = (long)blockIdx.y * 256 + 64 * w, pb = min(K, pa + 64);
   int mx = -100000;
-  if (col < cols)
-    for (long p = pa; p < pb; ++p) mx = max(mx, frexp_exp(hi[p * ld + col]));
+  if (col < cols) {
+    long p = pa;
+    for (; p + 8 <= pb; p += 8) {
+      double v[8];
+#pragma unroll
+      for (int u = 0; u < 8; ++u) v[u] = hi[(p + u) * ld + col];
+#pragma unroll
+      for (int u = 0; u < 8; ++u) mx = max(mx, frexp_exp(v[u]));
+    }
+    for (; p < pb; ++p) mx = max(mx, frexp_exp(hi[p * ld + col]));
+  }
   red[w][lane] = mx;
   __syncthreads();
   if (w == 0 && col < cols) atomicMax(&e[col], max(max(red[0][lane], red[1][lane]), max(red[2][lane], red[3][lane])));
@@ -355,9 +374,12 @@ __global__ __launch_bounds__(256) void k_residues_rows(const double* __restrict_
 }
 
 // Rows strided (element at base[p*ld + row], the columns of a row-major B): a block takes
-// 64 rows x 64 p, lane = row (coalesced dd loads), wavefront w the 16 p from 16w.  The
-// residues of one modulus are transposed through LDS (two buffers, alternating by
-// modulus, so one barrier per modulus) and leave as 16-byte stores, 64 bytes per row.
+// 64 rows x 64 p per tile, lane = row (coalesced dd loads), wavefront w the 16 p from 16w,
+// and walks row tiles row0 = 64 blockIdx.y + n 64 gridDim.y; the next tile's dd inputs are
+// loaded while this tile's residues are formed (the blocks of a CU would otherwise load
+// and compute in phase).  The residues of one modulus are transposed through LDS (two
+// buffers, alternating by modulus, so one barrier per modulus) and leave as 16-byte
+// stores, 64 bytes per row.
 __global__ __launch_bounds__(256) void k_residues_cols(const double* __restrict__ hi, const double* __restrict__ lo,
                                                        long rows, long K, long Kp, long ld,
                                                        const int* __restrict__ escale, int W,
@@ -370,22 +392,27 @@ __global__ __launch_bounds__(256) void k_residues_cols(const double* __restrict_
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
   const int srow = threadIdx.x >> 2, sch = threadIdx.x & 3;  // store: 16 bytes of row srow
   const size_t plane = (size_t)rows * Kp;
-  const long p0 = (long)blockIdx.x * 64;
-  for (long row0 = (long)blockIdx.y * 64; row0 < rows; row0 += (long)gridDim.y * 64) {
-    const long row = row0 + lane;
-    const bool rok = row < rows;
-    const int sc = rok ? W - max(escale[row], -1100) : 0;
-    double t[16][3];
+  const long p0 = (long)blockIdx.x * 64, stride = (long)gridDim.y * 64;
+  double hv[16], lv[16];
+  auto fetch = [&](long r0) {
+    const long row = r0 + lane;
 #pragma unroll
     for (int e = 0; e < 16; ++e) {
       const long p = p0 + 16 * w + e;
-      double h = 0.0, l = 0.0;
-      if (rok && p < K) {
-        h = hi[p * ld + row];
-        l = lo[p * ld + row];
-      }
-      int_limbs(h, l, sc, t[e][2], t[e][1], t[e][0]);
+      const bool ok = row < rows && p < K;
+      hv[e] = ok ? hi[p * ld + row] : 0.0;
+      lv[e] = ok ? lo[p * ld + row] : 0.0;
     }
+  };
+  long row0 = (long)blockIdx.y * 64;
+  if (row0 < rows) fetch(row0);
+  for (; row0 < rows; row0 += stride) {
+    const long row = row0 + lane;
+    const int sc = row < rows ? W - max(escale[row], -1100) : 0;
+    double t[16][3];
+#pragma unroll
+    for (int e = 0; e < 16; ++e) int_limbs(hv[e], lv[e], sc, t[e][2], t[e][1], t[e][0]);
+    if (row0 + stride < rows) fetch(row0 + stride);
     const bool sok = row0 + srow < rows;
     int8_t* so = out + (size_t)(row0 + srow) * Kp + p0 + 16 * sch;
     for (int r = 0; r < R; ++r) {
@@ -402,7 +429,7 @@ __global__ __launch_bounds__(256) void k_residues_cols(const double* __restrict_
       __syncthreads();  // buffer r & 1 complete; buffer (r + 1) & 1 was read before this point
       if (sok) *reinterpret_cast<v4i_t*>(so + (size_t)r * plane) = *reinterpret_cast<const v4i_t*>(&tb[r & 1][srow][16 * sch]);
     }
-    __syncthreads();  // the next row block's first modulus reuses buffer 0
+    __syncthreads();  // the next row tile's first modulus reuses buffer 0
   }
 }
 
@@ -1191,6 +1218,11 @@ int split_gemm_general(const SplitOperand& A, const SplitOperand& B, double* chi
   hipLaunchKernelGGL(k_fill_int, dim3((unsigned)((m + n + 255) / 256)), dim3(256), 0, s, eA, (long)(m + n), -100000);
   HP_LAUNCH_CHECK();
   auto gy = [](size_t blocks) { return (unsigned)std::min<size_t>(blocks, 65535); };
+  // k_residues_cols: about 512 blocks (two per CU at its register use), each walking row tiles
+  auto cols_gy = [&](long rows) {
+    const size_t tiles = (size_t)(rows + 63) / 64, gx = Kp / 64;
+    return (unsigned)std::max<size_t>(1, std::min<size_t>(tiles, (512 + gx - 1) / gx));
+  };
   auto scales_and_residues = [&](const SplitOperand& X, long rows, int* e, int8_t* out) -> int {
     if (X.contig)
       hipLaunchKernelGGL(k_scale_rows, dim3(gy((rows + 3) / 4)), dim3(256), 0, s, X.hi, rows, (long)k, X.ld, e);
@@ -1202,7 +1234,7 @@ int split_gemm_general(const SplitOperand& A, const SplitOperand& B, double* chi
       hipLaunchKernelGGL(k_residues_rows, dim3((unsigned)((Kp + 511) / 512), gy((rows + 3) / 4)), dim3(256), 0, s,
                          X.hi, X.lo, rows, (long)k, (long)Kp, X.ld, e, W, g_work.tab, out);
     else
-      hipLaunchKernelGGL(k_residues_cols, dim3((unsigned)(Kp / 64), gy((rows + 63) / 64)), dim3(256), 0, s, X.hi,
+      hipLaunchKernelGGL(k_residues_cols, dim3((unsigned)(Kp / 64), cols_gy(rows)), dim3(256), 0, s, X.hi,
                          X.lo, rows, (long)k, (long)Kp, X.ld, e, W, g_work.tab, out);
     HP_LAUNCH_CHECK();
     return HP_OK;
