@@ -343,23 +343,57 @@ __device__ __forceinline__ void rows_body(const ShellRec* __restrict__ sh, const
   for (int v = 0; v < NV; ++v) row[v] = acc[v].hi + acc[v].lo;
 }
 
-// S[a][b] = S[b][a] for a < b in distinct shells; block (ta, tb), tb >= ta, reads the lower
-// tile (tb, ta) into LDS and writes the upper tile (ta, tb): each tile is read or written
-// by one block only
+// tile pair p of the T(T+1)/2 pairs (ta <= tb) of a T x T tiling, row by row of ta
+__device__ __forceinline__ void tile_pair(long p, long T, long& ta, long& tb) {
+  // row ta holds the T - ta pairs from index ta T - ta (ta - 1) / 2
+  long r = (long)((2.0 * T + 1.0 - sqrt((2.0 * T + 1.0) * (2.0 * T + 1.0) - 8.0 * (double)p)) * 0.5);
+  auto first = [&](long x) { return x * T - x * (x - 1) / 2; };
+  while (r > 0 && first(r) > p) --r;
+  while (r + 1 < T && first(r + 1) <= p) ++r;
+  ta = r;
+  tb = r + (p - first(r));
+}
+
+// S[a][b] = S[b][a] for a < b in distinct shells; block p takes the tile pair (ta, tb),
+// ta <= tb: it reads the lower tile (tb, ta) into LDS and writes the upper tile (ta, tb),
+// so each tile is read or written by one block only
 __global__ __launch_bounds__(256) void k_mirror(double* __restrict__ s, const int* __restrict__ ao_shell, long n) {
   __shared__ double tile[64][65];
-  const long ta = blockIdx.y, tb = blockIdx.x;
-  if (tb < ta) return;
+  __shared__ int sha[64], shb[64];  // shells of the upper tile's rows / columns
+  const long T = (n + 63) / 64;
+  long ta, tb;
+  tile_pair(blockIdx.x, T, ta, tb);
+  if (threadIdx.x < 128) {
+    const long g = (threadIdx.x < 64 ? ta : tb) * 64 + (threadIdx.x & 63);
+    (threadIdx.x < 64 ? sha : shb)[threadIdx.x & 63] = g < n ? ao_shell[g] : -1;
+  }
+#pragma unroll 4
   for (int idx = threadIdx.x; idx < 64 * 64; idx += 256) {
     const int r = idx >> 6, c = idx & 63;
     const long row = tb * 64 + r, col = ta * 64 + c;
     tile[r][c] = (row < n && col < n) ? s[row * n + col] : 0.0;
   }
   __syncthreads();
+#pragma unroll 4
   for (int idx = threadIdx.x; idx < 64 * 64; idx += 256) {
     const int r = idx >> 6, c = idx & 63;
     const long a = ta * 64 + r, b = tb * 64 + c;
-    if (a < n && b < n && a < b && ao_shell[a] != ao_shell[b]) s[a * n + b] = tile[c][r];
+    if (a < n && b < n && a < b && sha[r] != shb[c]) s[a * n + b] = tile[c][r];
+  }
+}
+
+// zeroes the tiles on and below the diagonal, where the pairs the screen skips must read
+// zero (k_mirror then writes every upper entry outside the shells' own blocks, which
+// k_rows computes whole): block p takes the tile pair (ta, tb), ta <= tb, and zeroes
+// tile (tb, ta)
+__global__ __launch_bounds__(256) void k_zero_lower(double* __restrict__ s, long n) {
+  const long T = (n + 63) / 64;
+  long ta, tb;
+  tile_pair(blockIdx.x, T, ta, tb);
+#pragma unroll 4
+  for (int idx = threadIdx.x; idx < 64 * 64; idx += 256) {
+    const long row = tb * 64 + (idx >> 6), col = ta * 64 + (idx & 63);
+    if (row < n && col < n) s[row * n + col] = 0.0;
   }
 }
 
@@ -494,7 +528,9 @@ int assemble_mixed(const double* centers, const int* l, const long* prim_off, co
     HP_HIP(hipMemcpyAsync(dco.p, coeff, np * sizeof(double), hipMemcpyHostToDevice, st));
   }
   HP_HIP(hipMemcpyAsync(dprm.p, prm, sizeof(prm), hipMemcpyHostToDevice, st));
-  HP_HIP(hipMemsetAsync(s_dev, 0, n_ao * n_ao * sizeof(double), st));  // skipped pairs stay zero
+  const long tiles = (long)((n_ao + 63) / 64), tpairs = tiles * (tiles + 1) / 2;
+  hipLaunchKernelGGL(k_zero_lower, dim3((unsigned)tpairs), dim3(256), 0, st, s_dev, (long)n_ao);
+  HP_LAUNCH_CHECK();
   const long npairs = (long)n_shells * ((long)n_shells + 1) / 2;
   DevBuf dbk, dlist;
   if ((rc = dbk.alloc(2 * NBUCKET * sizeof(unsigned int))) || (rc = dlist.alloc(npairs * sizeof(int2)))) return rc;
@@ -533,8 +569,7 @@ int assemble_mixed(const double* centers, const int* l, const long* prim_off, co
                          dco.d(), (long)n_ao, s_dev);
     HP_LAUNCH_CHECK();
   }
-  const unsigned tiles = (unsigned)((n_ao + 63) / 64);
-  hipLaunchKernelGGL(k_mirror, dim3(tiles, tiles), dim3(256), 0, st, s_dev, static_cast<const int*>(dao.p),
+  hipLaunchKernelGGL(k_mirror, dim3((unsigned)tpairs), dim3(256), 0, st, s_dev, static_cast<const int*>(dao.p),
                      (long)n_ao);
   HP_LAUNCH_CHECK();
   unsigned long long hc[3];
