@@ -165,7 +165,7 @@ def bench_mf4(qp, torch, dev, steps, cpu):
     da, db = qp.dev.to_planes(a, 4, dev), qp.dev.to_planes(b, 4, dev)
     dc = torch.empty((4, n * n), dtype=torch.float64, device=dev)
     stream = torch.cuda.current_stream()
-    ms, kern = time_steps(lambda: qp.dev.mf_gemm(da, db, dc, n, n, n), max(2, steps), 1, stream, torch)
+    ms, kern = time_steps(lambda: qp.dev.mf_gemm(da, db, dc, n, n, n), max(2, min(steps, 5)), 1, stream, torch)
     macs = float(n) ** 3
     ops = MF4_OPS_PER_MAC_MEAN * macs / (kern * 1e-3) / 1e12
     out = {"workload": "MF4 mf_gemm C=A*B, m=n=k=1024, seed 1", "ms_per_step": round(ms, 3),
@@ -596,8 +596,8 @@ def time_steps(step, steps, warmup, stream, torch):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=5)
-    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-solve", action="store_true", help="skip the N=4096 SPD solve timings")
     ap.add_argument("--no-batch", action="store_true", help="skip the batched uneven solves")
