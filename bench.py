@@ -638,18 +638,21 @@ def main():
     stream = torch.cuda.current_stream()
     macs = M * N * K
 
-    # split path (headline); per-phase HIP events on the launch stream
-    qp.set_phase_timing(True)
-    phases = []
+    # split path (headline); per-phase HIP events on the launch stream, accumulated over the
+    # timed steps without a host wait per call
+    qp.set_phase_timing(2)
 
     def split_step():
         qp.dev.gemm(da, db, dc, M, N, K, mode=qp.HP_GEMM_SPLIT_I8, slices=SLICES)
-        phases.append(qp.phase_times())
 
-    ms_split, kern_split = time_steps(split_step, args.steps, args.warmup, stream, torch)
+    for _ in range(args.warmup):
+        split_step()
+    torch.cuda.synchronize()
+    qp.collect_phase_times()  # the warmup calls' phases are dropped
+    ms_split, kern_split = time_steps(split_step, args.steps, 0, stream, torch)
+    sums, calls = qp.collect_phase_times()
     qp.set_phase_timing(False)
-    timed = phases[args.warmup:]
-    ph = [sum(p[i] for p in timed) / len(timed) for i in range(len(timed[0]))]  # prep, int8 gemms, crt
+    ph = [x / calls for x in sums]  # prep, int8 gemms, crt
     gemm_ms = ph[1]
     variant = qp.split_gemm_variant(M, N, K, SLICES)
     achieved = 2.0 * SLICES * macs / (gemm_ms * 1e-3) / 1e12

@@ -384,10 +384,14 @@ int hp_overlap_adjoint(const double* centers, const double* alpha, const double*
 /* ------------------------------------------------------------------ measurement */
 
 /* Per-phase HIP-event timing of the multi-kernel entry points (no reference
- * counterpart; for benchmarks).  When on, the next split GEMM records events on its
- * stream at its phase boundaries (scaling + residues | int8 GEMMs | CRT) and
- * hp_get_phase_times copies the elapsed milliseconds, returning the phase count. */
+ * counterpart; for benchmarks).  on = 1: each split GEMM records events on its stream at
+ * its phase boundaries (scaling + residues | int8 GEMMs | CRT), waits for them, and
+ * hp_get_phase_times copies the elapsed milliseconds of the last call, returning the phase
+ * count.  on = 2: the calls record their boundaries without waiting, and
+ * hp_collect_phase_times waits for the last one, sums each phase over the calls since the
+ * previous collect (their number in *calls) and returns the phase count.  0: off. */
 void hp_set_phase_timing(int on);
+int hp_collect_phase_times(double* ms_sum, int max, int* calls);
 /* Int8 kernel the split GEMM's autotuner chose for this shape class (0 = not tuned yet).
  * The first split GEMM of a shape class times each variant on its real operands. */
 int hp_split_gemm_variant(size_t m, size_t n, size_t k, int slices);

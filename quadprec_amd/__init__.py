@@ -173,6 +173,8 @@ def lib():
     L.hp_set_phase_timing.restype = None
     L.hp_get_phase_times.argtypes = [_dp, C.c_int]
     L.hp_get_phase_times.restype = C.c_int
+    L.hp_collect_phase_times.argtypes = [_dp, C.c_int, C.POINTER(C.c_int)]
+    L.hp_collect_phase_times.restype = C.c_int
     L.hp_gen_dd_normal.argtypes = [C.c_uint64, _sz, _dp]
     L.hp_gen_dd_normal.restype = None
     L.hp_gen_mf4_normal.argtypes = [C.c_uint64, _sz, _dp]
@@ -651,7 +653,9 @@ def gen_overlap_basis(seed, n, alpha_lo=0.01, alpha_hi=10.0, spacing=1.4, jitter
 
 # ---------------------------------------------------------------- measurement
 def set_phase_timing(on):
-    lib().hp_set_phase_timing(int(bool(on)))
+    """0 / False: off; 1 / True: each split GEMM waits for its phase events (phase_times);
+    2: calls accumulate without waiting (collect_phase_times)."""
+    lib().hp_set_phase_timing(2 if on == 2 else int(bool(on)))
 
 
 def split_gemm_variant(m, n, k, slices=0):
@@ -670,6 +674,16 @@ def phase_times():
     buf = np.zeros(8)
     n = lib().hp_get_phase_times(buf, 8)
     return list(buf[:n])
+
+
+def collect_phase_times():
+    """(per-phase milliseconds summed over the calls since the last collect, number of
+    calls) in accumulate mode (set_phase_timing(2)); waits for the last call's events."""
+    buf = np.zeros(8)
+    calls = C.c_int(0)
+    n = lib().hp_collect_phase_times(buf, 8, C.byref(calls))
+    _check(n if n < 0 else 0, "collect_phase_times")
+    return list(buf[:n]), calls.value
 
 
 # ---------------------------------------------------------------- generators

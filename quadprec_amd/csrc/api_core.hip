@@ -1,5 +1,6 @@
 // api_core.hip -- C ABI plumbing: errors, AoS<->SoA staging, GEMM entry points and the
 // deterministic input generators.
+#include <algorithm>
 #include <cmath>
 #include <cstdarg>
 #include <cstring>
@@ -69,11 +70,26 @@ static thread_local PhaseTimer g_timer;
 PhaseTimer& phase_timer() { return g_timer; }
 void PhaseTimer::mark(hipStream_t s) {
   if (!on || n > MAXP) return;
+  if (on == 2) {
+    if (n == 0) call_start = used;
+    if (used == pool.size()) {
+      hipEvent_t e = nullptr;
+      if (hipEventCreate(&e) != hipSuccess) return;
+      pool.push_back(e);
+    }
+    (void)hipEventRecord(pool[used++], s);
+    ++n;
+    return;
+  }
   if (!ev[n]) (void)hipEventCreate(&ev[n]);
   (void)hipEventRecord(ev[n], s);
   ++n;
 }
 void PhaseTimer::finish() {
+  if (on == 2) {
+    if (n > 1) calls.emplace_back(call_start, n);
+    return;
+  }
   if (!on) return;
   for (int i = 0; i + 1 < n; ++i) {
     float t = 0.f;
@@ -81,6 +97,26 @@ void PhaseTimer::finish() {
     (void)hipEventElapsedTime(&t, ev[i], ev[i + 1]);
     ms[i] = t;
   }
+}
+int PhaseTimer::collect(double* sum, int max, int* ncalls) {
+  int phases = 0;
+  for (int i = 0; i < max; ++i) sum[i] = 0.0;
+  if (!calls.empty()) {
+    const auto& last = calls.back();
+    (void)hipEventSynchronize(pool[last.first + last.second - 1]);
+    phases = MAXP;
+    for (const auto& c : calls) phases = std::min(phases, c.second - 1);
+    for (const auto& c : calls)
+      for (int i = 0; i < phases && i < max; ++i) {
+        float t = 0.f;
+        (void)hipEventElapsedTime(&t, pool[c.first + i], pool[c.first + i + 1]);
+        sum[i] += t;
+      }
+  }
+  if (ncalls) *ncalls = (int)calls.size();
+  calls.clear();
+  used = 0;
+  return std::min(phases, max);
 }
 
 }  // namespace hp
@@ -147,7 +183,19 @@ int hp_split_gemm_set_variant(size_t m, size_t n, size_t k, int slices, int vari
   return set_split_gemm_variant(m, n, k, slices, variant);
 }
 
-void hp_set_phase_timing(int on) { phase_timer().on = on != 0; }
+void hp_set_phase_timing(int on) {
+  PhaseTimer& t = phase_timer();
+  t.on = on == 2 ? 2 : (on != 0);
+  t.calls.clear();
+  t.used = 0;
+}
+int hp_collect_phase_times(double* ms_sum, int max, int* calls) {
+  if (!ms_sum || max < 0) {
+    set_error("hp_collect_phase_times: null pointer");
+    return HP_EINVAL;
+  }
+  return phase_timer().collect(ms_sum, max, calls);
+}
 int hp_get_phase_times(double* ms, int max) {
   PhaseTimer& t = phase_timer();
   int cnt = t.n > 0 ? t.n - 1 : 0;

@@ -2,6 +2,8 @@
 // thread-local error message, HIP error checks and device scratch buffers.
 #pragma once
 #include <hip/hip_runtime.h>
+#include <utility>
+#include <vector>
 
 #include <cstddef>
 #include <cstdio>
@@ -59,12 +61,17 @@ struct DevBuf {
 // Phase boundaries are recorded on the stream the kernels run on.
 struct PhaseTimer {
   static constexpr int MAXP = 8;
-  bool on = false;
-  int n = 0;
+  int on = 0;  // 0 off; 1: each call waits for its own events; 2: calls accumulate (collect)
+  int n = 0;   // boundaries recorded by the current call
   hipEvent_t ev[MAXP + 1] = {};
   double ms[MAXP] = {};
+  // mode 2: every call's boundaries in a growing pool of events, summed by collect()
+  std::vector<hipEvent_t> pool;
+  size_t used = 0, call_start = 0;
+  std::vector<std::pair<size_t, int>> calls;  // (first boundary in the pool, boundaries)
   void mark(hipStream_t s);  // record the next boundary (no-op when off)
-  void finish();             // after synchronisation: elapsed times between boundaries
+  void finish();             // mode 1: after synchronisation, elapsed times between boundaries
+  int collect(double* sum, int max, int* ncalls);  // mode 2: per-phase sums since the last collect
 };
 PhaseTimer& phase_timer();
 

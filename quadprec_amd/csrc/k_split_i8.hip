@@ -1097,12 +1097,19 @@ __global__ __launch_bounds__(256) void k_crt(const uint8_t* __restrict__ Cr, lon
 }
 
 // ------------------------------------------------------------------ workspace cache
+// The workspace (residue planes, CRT digits, scales) is shared by all calls.  A call
+// records `done` after its last kernel; the next call on the same stream is ordered by the
+// stream, one on another stream first waits for `done` (no host synchronisation), and the
+// buffer or the table is replaced only after `done` has completed.
 struct SplitWork {
   std::mutex mu;
   void* buf = nullptr;
   size_t bytes = 0;
   CrtTable* tab = nullptr;
   int tabR = -1;
+  hipEvent_t done = nullptr;
+  hipStream_t last = nullptr;
+  bool pending = false;  // done has been recorded
 };
 SplitWork g_work;
 std::map<uint64_t, int> g_tuned;  // autotuned GEMM kernel per shape class (under g_work.mu)
@@ -1172,6 +1179,10 @@ int split_gemm_general(const SplitOperand& A, const SplitOperand& B, double* chi
   const size_t resA = (size_t)R * m * Kp, resB = (size_t)R * n * Kp, resC = (size_t)R * m * Np;
   const size_t need = resA + resB + resC + (m + n) * sizeof(int) + 1024;
   std::lock_guard<std::mutex> lock(g_work.mu);
+  if (!g_work.done) HP_HIP(hipEventCreateWithFlags(&g_work.done, hipEventDisableTiming));
+  if (g_work.pending && g_work.last != s) HP_HIP(hipStreamWaitEvent(s, g_work.done, 0));
+  if ((g_work.bytes < need || g_work.tabR != R) && g_work.pending)
+    HP_HIP(hipEventSynchronize(g_work.done));  // the previous call's kernels are done
   if (g_work.bytes < need) {
     if (g_work.buf) (void)hipFree(g_work.buf);
     g_work.buf = nullptr;
@@ -1324,8 +1335,9 @@ int split_gemm_general(const SplitOperand& A, const SplitOperand& B, double* chi
                        opt.subtract ? 1 : 0);
   HP_LAUNCH_CHECK();
   tm.mark(s);
-  // the workspace is reused by the next call: keep it owned until this one has run
-  HP_HIP(hipStreamSynchronize(s));
+  HP_HIP(hipEventRecord(g_work.done, s));
+  g_work.last = s;
+  g_work.pending = true;
   tm.finish();
   return HP_OK;
 }
