@@ -283,24 +283,70 @@ HP_HD void mf_accumulate(double (&w)[8], double v) {
     }
   }
 }
-HP_HD mf4 mf_mul(mf4 a, mf4 b) {
+// mf_accumulate while the slots from F on are still +0.0 (no cascade has reached them)
+// and every value is finite: two_sum(+0.0, e) is (e + 0.0, +0.0) for a finite e, so a
+// cascade that reaches slot F stops there and the later levels are never taken; slot F
+// then holds 0.0 + e, which is also the +0.0 it keeps when the cascade stopped earlier
+// (e = +-0.0).  Slot IDX >= F takes 0.0 + v the same way.
+template <int IDX, int F>
+HP_HD void mf_accumulate_front(double (&w)[8], double v) {
+  if constexpr (IDX >= F) {
+    w[IDX] = 0.0 + v;
+  } else {
+    double s, e;
+    two_sum(w[IDX], v, s, e);
+    w[IDX] = s;
+#pragma unroll
+    for (int k = IDX + 1; k < F; ++k) {
+      if (e != 0.0) {
+        double s2, e2;
+        two_sum(w[k], e, s2, e2);
+        w[k] = s2;
+        e = e2;
+      }
+    }
+    w[F] = 0.0 + e;
+  }
+}
+// (i, j) pairs with i + j < 4 in the reference's loop order (mul.rs:212-224); FAST: every
+// limb is below 2^500 in magnitude, so no product, sum or error overflows and the first
+// four products' cascades stop at the first untouched slot (slots 0..7 are reached in
+// order: the frontier before their eight accumulations is 0, 1, ..., 7)
+template <bool FAST>
+HP_HD mf4 mf_mul_impl(mf4 a, mf4 b) {
   double w[8] = {0, 0, 0, 0, 0, 0, 0, 0};
   const double A[4] = {a.l0, a.l1, a.l2, a.l3};
   const double B[4] = {b.l0, b.l1, b.l2, b.l3};
   double p, e;
-  // (i, j) pairs with i + j < 4 in the reference's loop order (mul.rs:212-224)
 #define HP_MFMUL_TERM(I, J)            \
   two_prod(A[I], B[J], p, e);          \
   mf_accumulate<(I) + (J)>(w, p);      \
   mf_accumulate<(I) + (J) + 1>(w, e);
-  HP_MFMUL_TERM(0, 0) HP_MFMUL_TERM(0, 1) HP_MFMUL_TERM(0, 2) HP_MFMUL_TERM(0, 3)
+#define HP_MFMUL_FRONT(I, J, F)                        \
+  two_prod(A[I], B[J], p, e);                          \
+  mf_accumulate_front<(I) + (J), (F)>(w, p);           \
+  mf_accumulate_front<(I) + (J) + 1, (F) + 1>(w, e);
+  if constexpr (FAST) {
+    HP_MFMUL_FRONT(0, 0, 0) HP_MFMUL_FRONT(0, 1, 2) HP_MFMUL_FRONT(0, 2, 4) HP_MFMUL_FRONT(0, 3, 6)
+  } else {
+    HP_MFMUL_TERM(0, 0) HP_MFMUL_TERM(0, 1) HP_MFMUL_TERM(0, 2) HP_MFMUL_TERM(0, 3)
+  }
   HP_MFMUL_TERM(1, 0) HP_MFMUL_TERM(1, 1) HP_MFMUL_TERM(1, 2)
   HP_MFMUL_TERM(2, 0) HP_MFMUL_TERM(2, 1)
   HP_MFMUL_TERM(3, 0)
+#undef HP_MFMUL_FRONT
 #undef HP_MFMUL_TERM
   double o[4];
   mf_renormalize_from8(w, o);
   return mkmf(o[0], o[1], o[2], o[3]);
+}
+HP_HD mf4 mf_mul(mf4 a, mf4 b) {
+  constexpr double LIM = 0x1p500;
+  const bool fast = (__builtin_fabs(a.l0) < LIM) & (__builtin_fabs(a.l1) < LIM) & (__builtin_fabs(a.l2) < LIM) &
+                    (__builtin_fabs(a.l3) < LIM) & (__builtin_fabs(b.l0) < LIM) & (__builtin_fabs(b.l1) < LIM) &
+                    (__builtin_fabs(b.l2) < LIM) & (__builtin_fabs(b.l3) < LIM);
+  if (fast) return mf_mul_impl<true>(a, b);
+  return mf_mul_impl<false>(a, b);
 }
 
 // MultiFloat<4> reciprocal (multifloat/div.rs:14-41): Newton from the f64 reciprocal,

@@ -85,6 +85,29 @@ def test_mf_gemm_special_inputs(qp):
     assert np.array_equal(qp.mf_gemm(a, b, m, n, k).view(np.uint64), O.mf4_gemm(a, b, m, n, k).view(np.uint64))
 
 
+def test_mf_gemm_fast_and_general_multiply(qp):
+    """mf_mul takes a pruned cascade when every limb is below 2^500 (nothing can overflow)
+    and the reference's full cascade otherwise: limbs just below, at and above 2^500, an
+    overflowing product, inf and nan, mixed in one wavefront, all equal to the oracle (NaNs
+    compared as NaNs: the host's and the device's default NaN differ in sign)."""
+    m, n, k = 8, 8, 16
+    a = qp.gen_mf4_normal(31, m * k)
+    b = qp.gen_mf4_normal(32, k * n)
+    a[3] *= 2.0 ** 499.5
+    a[10] *= 2.0 ** 500
+    a[17] = [2.0 ** 500, 2.0 ** 440, 0, 0]
+    a[25] = [1e200, 1e183, 0, 0]
+    b[4] *= 2.0 ** 520
+    b[33] = [1e200, 0, 0, 0]
+    b[9] = [np.inf, 0, 0, 0]
+    b[20] = [np.nan, 0, 0, 0]
+    b[50] = [-0.0, 0, 0, 0]
+    got, ref = qp.mf_gemm(a, b, m, n, k), O.mf4_gemm(a, b, m, n, k)
+    same = (got.view(np.uint64) == ref.view(np.uint64)) | (np.isnan(got) & np.isnan(ref))
+    assert same.all(), np.argwhere(~same)[:4]
+    assert np.isfinite(got).sum() > 0 and np.isnan(got).sum() > 0
+
+
 def test_dev_gemm_full_size_rows(qp):
     """Full-size 4096^3 plain GEMM on device planes: a sample of rows must be
     bit-identical to the oracle (each element is independent of the others)."""
