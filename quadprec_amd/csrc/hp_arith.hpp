@@ -208,31 +208,13 @@ HP_HD void mf_renormalize_from8(const double (&in)[8], double (&out)[4]) {
   if (!done) HP_PUT4(out, k, s);
 }
 
-HP_HD mf4 mf_add(mf4 a, mf4 b) {
-  double w[8];
-  two_sum(a.l0, b.l0, w[0], w[1]);
-  two_sum(a.l1, b.l1, w[2], w[3]);
-  two_sum(a.l2, b.l2, w[4], w[5]);
-  two_sum(a.l3, b.l3, w[6], w[7]);
-  // stable insertion sort by descending magnitude (add.rs:42-54)
+// The general grow-expansion of the sorted terms w into ex (zeros past its length), as in
+// mf_add_t<false>.  Before term i the expansion holds at most i limbs (each term adds at
+// most one), so the loops stop at i: the same operations as running them to 8 under the
+// len guard, which skips the rest.
+HP_HD void mf_add_expand(const double (&w)[8], double (&ex)[8]) {
 #pragma unroll
-  for (int i = 1; i < 8; ++i) {
-    double v = w[i];
-    double mag = __builtin_fabs(v);
-    bool moving = true;
-#pragma unroll
-    for (int j = i; j >= 1; --j) {
-      bool c = moving && (__builtin_fabs(w[j - 1]) < mag);
-      double nw = c ? w[j - 1] : (moving ? v : w[j]);
-      moving = c;
-      w[j] = nw;
-    }
-    if (moving) w[0] = v;
-  }
-  // grow-expansion, most significant first (add.rs:60-95).  Before term i the expansion
-  // holds at most i limbs (each term adds at most one), so the loops stop at i: the same
-  // operations as running them to 8 under the len guard, which skips the rest
-  double ex[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+  for (int i = 0; i < 8; ++i) ex[i] = 0.0;
   int len = 0;
 #pragma unroll
   for (int i = 0; i < 8; ++i) {
@@ -258,14 +240,100 @@ HP_HD mf4 mf_add(mf4 a, mf4 b) {
       len = nl;
     }
   }
+}
+// FIXED: the expansion takes the fixed sequence below when its conditions hold (fewer
+// dependent operations per add: for serial chains), else the general loop only (fewer
+// registers: for throughput-bound kernels such as the GEMM); both give the same bits.
+template <bool FIXED>
+HP_HD mf4 mf_add_t(mf4 a, mf4 b) {
+  double w[8];
+  two_sum(a.l0, b.l0, w[0], w[1]);
+  two_sum(a.l1, b.l1, w[2], w[3]);
+  two_sum(a.l2, b.l2, w[4], w[5]);
+  two_sum(a.l3, b.l3, w[6], w[7]);
+  // stable insertion sort by descending magnitude (add.rs:42-54)
+#pragma unroll
+  for (int i = 1; i < 8; ++i) {
+    double v = w[i];
+    double mag = __builtin_fabs(v);
+    bool moving = true;
+#pragma unroll
+    for (int j = i; j >= 1; --j) {
+      bool c = moving && (__builtin_fabs(w[j - 1]) < mag);
+      double nw = c ? w[j - 1] : (moving ? v : w[j]);
+      moving = c;
+      w[j] = nw;
+    }
+    if (moving) w[0] = v;
+  }
+  // grow-expansion, most significant first (add.rs:60-95).  When no term is zero and no
+  // cascade's carry is zero (the usual case), term i meets exactly i limbs: the fixed
+  // sequence below performs the reference's operations without its length bookkeeping,
+  // and a lane where either condition fails redoes the expansion with the general loop
+  if constexpr (!FIXED) {
+    double ex[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    int len = 0;
+#pragma unroll
+    for (int i = 0; i < 8; ++i) {
+      double term = w[i];
+      if (term == 0.0) continue;
+      if (len == 0) { ex[0] = term; len = 1; continue; }
+      double carry = term;
+#pragma unroll
+      for (int j = i - 1; j >= 0; --j) {
+        if (j < len) {
+          double s, e;
+          two_sum(ex[j], carry, s, e);
+          ex[j] = e;
+          carry = s;
+        }
+      }
+      if (carry != 0.0) {
+        int nl = len + 1;
+#pragma unroll
+        for (int q = i; q >= 1; --q)
+          if (q < nl) ex[q] = ex[q - 1];
+        ex[0] = carry;
+        len = nl;
+      }
+    }
+    double l[4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i) l[i] = (i < len) ? ex[i] : 0.0;
+    mf_renormalize(l);
+    return mkmf(l[0], l[1], l[2], l[3]);
+  }
+  double ex[8];
+  bool irregular = w[0] == 0.0;
+  ex[0] = w[0];
+#pragma unroll
+  for (int i = 1; i < 8; ++i) {
+    double carry = w[i];
+    irregular |= carry == 0.0;
+#pragma unroll
+    for (int j = i - 1; j >= 0; --j) {
+      double s, e;
+      two_sum(ex[j], carry, s, e);
+      ex[j] = e;
+      carry = s;
+    }
+    irregular |= carry == 0.0;
+#pragma unroll
+    for (int q = i; q >= 1; --q) ex[q] = ex[q - 1];
+    ex[0] = carry;
+  }
+  if (irregular) mf_add_expand(w, ex);
   double l[4];
 #pragma unroll
-  for (int i = 0; i < 4; ++i) l[i] = (i < len) ? ex[i] : 0.0;
+  for (int i = 0; i < 4; ++i) l[i] = ex[i];
   mf_renormalize(l);
   return mkmf(l[0], l[1], l[2], l[3]);
 }
+HP_HD mf4 mf_add(mf4 a, mf4 b) { return mf_add_t<false>(a, b); }
 HP_HD mf4 mf_neg(mf4 a) { return mkmf(-a.l0, -a.l1, -a.l2, -a.l3); }
 HP_HD mf4 mf_sub(mf4 a, mf4 b) { return mf_add(a, mf_neg(b)); }
+// the same subtraction with the fixed expansion sequence, for serial chains
+HP_HD mf4 mf_sub_chain(mf4 a, mf4 b) { return mf_add_t<true>(a, mf_neg(b)); }
 
 // error-free accumulate of v into w[idx] with the cascade of mul.rs:31-42 (limit 8)
 template <int IDX>

@@ -117,7 +117,7 @@ __global__ __launch_bounds__(1024) void k_mf_fwd(MfMat l, double* y, const doubl
     if (threadIdx.x == 0) stv(y, n, j, mf_div_by(ldv(y, n, j), l.ld(j, j), ldv(rd, n, j)));
     __syncthreads();
     const mf4 yj = ldv(y, n, j);
-    for (long i = j + 1 + threadIdx.x; i < n; i += blockDim.x) stv(y, n, i, mf_sub(ldv(y, n, i), mf_mul(l.ld(i, j), yj)));
+    for (long i = j + 1 + threadIdx.x; i < n; i += blockDim.x) stv(y, n, i, mf_sub_chain(ldv(y, n, i), mf_mul(l.ld(i, j), yj)));
     __syncthreads();
   }
 }
@@ -140,7 +140,7 @@ __global__ __launch_bounds__(256) void k_mf_bwd(MfMat l, double* x, const double
       __syncthreads();
       if (threadIdx.x == 0) {
         const int cnt = (int)std::min(256L, n - jc);
-        for (int q = 0; q < cnt; ++q) s = mf_sub(s, mkmf(P[0][q], P[1][q], P[2][q], P[3][q]));
+        for (int q = 0; q < cnt; ++q) s = mf_sub_chain(s, mkmf(P[0][q], P[1][q], P[2][q], P[3][q]));
       }
       __syncthreads();
     }
