@@ -456,7 +456,7 @@ def bench_reduce(qp, torch, dev, cpu, reps=5):
                     "roofline_frac_hbm": round(16.0 * r * r / tg / 8e12, 4),
                     "note": "one thread per row (the reference's order); 64 x 32 tiles through double-buffered "
                             "LDS, staged by three loader wavefronts: bound by the 8192-step dependent dd chain "
-                            "of each row (~480 cycles a step), not HBM"}}
+                            "of each row (~200 cycles a step), not HBM"}}
     if cpu:
         sys.path.insert(0, os.path.join(ROOT, "tests"))
         import oracle_lib as O

@@ -27,19 +27,33 @@ __global__ __launch_bounds__(64) void k_dot_chunks(const double* __restrict__ ah
   if (c0 >= n) return;
   const long c1 = min(n, c0 + DOT_CHUNK);
   dd s = mkdd(0.0);
-  for (long i = c0; i < c1; i += 8) {
-    double x0[8], x1[8], y0[8], y1[8];
+  // the next 8 elements are loaded while the current 8 go through the chain (addresses
+  // clamped into the chunk; elements past its end are never used)
+  double x0[8], x1[8], y0[8], y1[8];
+  auto fetch = [&](long i) {
 #pragma unroll
     for (int e = 0; e < 8; ++e) {
-      const bool ok = i + e < c1;
-      x0[e] = ok ? ah[i + e] : 0.0;
-      x1[e] = ok ? al[i + e] : 0.0;
-      y0[e] = ok ? bh[i + e] : 0.0;
-      y1[e] = ok ? bl[i + e] : 0.0;
+      const long q = min(i + e, c1 - 1);
+      x0[e] = ah[q];
+      x1[e] = al[q];
+      y0[e] = bh[q];
+      y1[e] = bl[q];
     }
+  };
+  fetch(c0);
+  for (long i = c0; i < c1; i += 8) {
+    double u0[8], u1[8], v0[8], v1[8];
+#pragma unroll
+    for (int e = 0; e < 8; ++e) {
+      u0[e] = x0[e];
+      u1[e] = x1[e];
+      v0[e] = y0[e];
+      v1[e] = y1[e];
+    }
+    if (i + 8 < c1) fetch(i + 8);
 #pragma unroll
     for (int e = 0; e < 8; ++e)
-      if (i + e < c1) s = dd_add(s, dd_mul(mkdd(x0[e], x1[e]), mkdd(y0[e], y1[e])));
+      if (i + e < c1) s = dd_add(s, dd_mul(mkdd(u0[e], u1[e]), mkdd(v0[e], v1[e])));
   }
   ph[c] = s.hi;
   pl[c] = s.lo;
@@ -92,14 +106,25 @@ __global__ __launch_bounds__(256) void k_gemv(const double* __restrict__ ah, con
   const int t = threadIdx.x, w = t >> 6, lane = t & 63;
   const long r0 = (long)blockIdx.x * 64;
   const long ntile = (cols + GV_T - 1) / GV_T;
-  auto load = [&](long tile, int buf) {  // wavefronts 1-3
+  auto load = [&](long tile, int buf) {  // wavefronts 1-3: every load issued before the LDS stores
     const long c0 = tile * GV_T;
-    for (int e = t - 64; e < 64 * GV_T; e += 192) {
-      const int r = e / GV_T, c = e % GV_T;
-      const long gr = r0 + r, gc = c0 + c;
-      const bool ok = gr < rows && gc < cols;
-      th[buf][r][c] = ok ? ah[gr * cols + gc] : 0.0;
-      tl[buf][r][c] = ok ? al[gr * cols + gc] : 0.0;
+    constexpr int NL = (64 * GV_T + 191) / 192;
+    double vh[NL], vl[NL];
+#pragma unroll
+    for (int i = 0; i < NL; ++i) {
+      const int e = t - 64 + 192 * i;
+      const long gr = r0 + e / GV_T, gc = c0 + e % GV_T;
+      const bool ok = e < 64 * GV_T && gr < rows && gc < cols;
+      vh[i] = ok ? ah[gr * cols + gc] : 0.0;
+      vl[i] = ok ? al[gr * cols + gc] : 0.0;
+    }
+#pragma unroll
+    for (int i = 0; i < NL; ++i) {
+      const int e = t - 64 + 192 * i;
+      if (e < 64 * GV_T) {
+        th[buf][e / GV_T][e % GV_T] = vh[i];
+        tl[buf][e / GV_T][e % GV_T] = vl[i];
+      }
     }
     for (int e = t - 64; e < GV_T; e += 192) {
       const long gc = c0 + e;
