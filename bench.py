@@ -241,7 +241,7 @@ def bench_eigen(qp, torch, dev, cpu):
                         "note": "64 N^2 bytes per round-robin step (A and V, read and written, dd) over the whole "
                                 "call's time; the steps are latency bound (each step's rotations need the previous "
                                 "step's diagonal: loads, a dependent dd divide / sqrt chain, the update, one launch "
-                                "per step)"}}
+                                "per step); V takes the stored rotations of 128 steps at a time in LDS"}}
     if cpu:
         sys.path.insert(0, os.path.join(ROOT, "tests"))
         import oracle_lib as O

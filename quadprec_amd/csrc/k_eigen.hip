@@ -90,11 +90,14 @@ __device__ __forceinline__ Rot pair_rot(const double* __restrict__ ah, const dou
 // transposed operation for operation, so A stays exactly symmetric without mirror writes.
 // WI pairs I per wavefront (z = 0), WR rows per wavefront (z = 1); HOIST: load the blocks
 // before forming the rotations (hides the load latency when few blocks run; costs registers)
+// rot (optional): the z = 0 blocks of the first row of blocks store the step's rotations
+// (c.hi, c.lo, s.hi, s.lo per pair) for k_jac_vbatch
 template <int WI, int WR, bool HOIST>
 __global__ __launch_bounds__(256) void k_jac_step(const double* __restrict__ ah, const double* __restrict__ al,
                                                   double* __restrict__ bh, double* __restrict__ bl,
                                                   const double* __restrict__ vh, const double* __restrict__ vl,
-                                                  double* __restrict__ wh, double* __restrict__ wl, long N2) {
+                                                  double* __restrict__ wh, double* __restrict__ wl, long N2,
+                                                  double* __restrict__ rot) {
   constexpr int ST_I = 4 * WI, ST_R = 4 * WR;
   __shared__ Rot rs[64 + ST_I];
   const long m = N2 / 2;
@@ -150,7 +153,13 @@ __global__ __launch_bounds__(256) void k_jac_step(const double* __restrict__ ah,
     }
   }
   if (t < 64) {
-    if (J0 + t < m) rs[t] = pair_rot(ah, al, N2, J0 + t);
+    if (J0 + t < m) {
+      rs[t] = pair_rot(ah, al, N2, J0 + t);
+      if (rot && blockIdx.y == 0) {
+        double* q = rot + 4 * (J0 + t);
+        q[0] = rs[t].ch; q[1] = rs[t].cl; q[2] = rs[t].sh; q[3] = rs[t].sl;
+      }
+    }
   } else if (t < 64 + ST_I) {
     if (I0 + t - 64 < m) rs[t] = pair_rot(ah, al, N2, I0 + t - 64);
   }
@@ -206,6 +215,52 @@ __global__ __launch_bounds__(256) void k_jac_step(const double* __restrict__ ah,
     bh[np0 * N2 + nq1] = y01.hi; bl[np0 * N2 + nq1] = y01.lo;
     bh[np1 * N2 + nq0] = y10.hi; bl[np1 * N2 + nq0] = y10.lo;
     bh[np1 * N2 + nq1] = y11.hi; bl[np1 * N2 + nq1] = y11.lo;
+  }
+}
+
+// V <- V R_1 ... R_cnt for cnt consecutive steps' rotations (as stored by k_jac_step),
+// each step followed by its layout permutation: the operations of k_jac_step's V part,
+// step after step, on VR rows of V held in LDS (two buffers), so V is read and written
+// once per cnt steps instead of once per step.  Rows are updated in place.
+template <int VR>
+__global__ __launch_bounds__(256) void k_jac_vbatch(double* __restrict__ vh, double* __restrict__ vl,
+                                                    const double* __restrict__ rot, int cnt, long N2) {
+  extern __shared__ double vsm[];  // [2 buffers][hi, lo][VR][N2]
+  const long m = N2 / 2, r0 = (long)blockIdx.x * VR;
+  const int t = threadIdx.x;
+  auto at = [&](int buf, int part, int rr, long c) -> double& { return vsm[((size_t)(buf * 2 + part) * VR + rr) * N2 + c]; };
+  for (long idx = t; idx < (long)VR * N2; idx += 256) {
+    const int rr = (int)(idx / N2);
+    const long c = idx % N2, row = r0 + rr;
+    at(0, 0, rr, c) = row < N2 ? vh[row * N2 + c] : 0.0;
+    at(0, 1, rr, c) = row < N2 ? vl[row * N2 + c] : 0.0;
+  }
+  __syncthreads();
+  int cur = 0;
+  for (int st = 0; st < cnt; ++st) {
+    const double* rs = rot + (size_t)st * 4 * m;
+    for (long idx = t; idx < (long)VR * m; idx += 256) {
+      const int rr = (int)(idx / m);
+      const long i = idx % m;
+      const dd c = mkdd(rs[4 * i], rs[4 * i + 1]), s = mkdd(rs[4 * i + 2], rs[4 * i + 3]);
+      const dd a0 = mkdd(at(cur, 0, rr, 2 * i), at(cur, 1, rr, 2 * i));
+      const dd a1 = mkdd(at(cur, 0, rr, 2 * i + 1), at(cur, 1, rr, 2 * i + 1));
+      const dd w0 = dd_sub(dd_mul(c, a0), dd_mul(s, a1));
+      const dd w1 = dd_add(dd_mul(s, a0), dd_mul(c, a1));
+      const long o0 = rr_sigma(2 * i, m), o1 = rr_sigma(2 * i + 1, m);
+      at(cur ^ 1, 0, rr, o0) = w0.hi; at(cur ^ 1, 1, rr, o0) = w0.lo;
+      at(cur ^ 1, 0, rr, o1) = w1.hi; at(cur ^ 1, 1, rr, o1) = w1.lo;
+    }
+    __syncthreads();
+    cur ^= 1;
+  }
+  for (long idx = t; idx < (long)VR * N2; idx += 256) {
+    const int rr = (int)(idx / N2);
+    const long c = idx % N2, row = r0 + rr;
+    if (row < N2) {
+      vh[row * N2 + c] = at(cur, 0, rr, c);
+      vl[row * N2 + c] = at(cur, 1, rr, c);
+    }
   }
 }
 
@@ -370,27 +425,52 @@ int jacobi_eigen_dev(const double* a_h, const double* a_l, long n, bool stop_fro
       if (cs) (void)hipStreamDestroy(cs);
     }
   } gr;
+  // V batching (N2 <= 2048): the steps update A only and store their rotations, and every
+  // VB steps (and at the end of a sweep) k_jac_vbatch applies them to V in place: V is
+  // read and written once per VB steps instead of once per step.  (A forked stream for
+  // the batches, joined by graph edges, measured slower.)  Larger matrices update V inside
+  // each step.
+  constexpr int VB = 128;
+  const bool vbatch = N2 <= 2048;
+  const int vr = N2 <= 512 ? 2 : 1;  // rows of V per block: LDS of 32 vr N2 bytes <= 64 KiB
+  DevBuf rotb;
+  if (vbatch && (rc = rotb.alloc((size_t)VB * 4 * m * sizeof(double)))) return rc;
   auto capture = [&](int parity) -> int {
     if (!gr.cs) HP_HIP(hipStreamCreateWithFlags(&gr.cs, hipStreamNonBlocking));
     double *xh = ah, *xl = al, *yh = bh, *yl = bl, *uh = vh, *ul = vl, *zh = wh, *zl = wl;
     HP_HIP(hipStreamBeginCapture(gr.cs, hipStreamCaptureModeThreadLocal));
+    const unsigned gz = vbatch ? 1 : 2;
     for (long step = 0; step < N2 - 1; ++step) {
+      double* rp = vbatch ? rotb.d() + (size_t)(step % VB) * 4 * m : nullptr;
       // small matrices: one I per wavefront for enough blocks, loads ahead of the rotations;
       // larger ones: four, which forms fewer copies of each rotation
       if (m <= 256)
-        hipLaunchKernelGGL((k_jac_step<1, 4, true>), dim3(blocks(m, 64), std::max(blocks(m, 4), blocks(N2, 16)), 2),
-                           dim3(256), 0, gr.cs, xh, xl, yh, yl, uh, ul, zh, zl, N2);
+        hipLaunchKernelGGL((k_jac_step<1, 4, true>),
+                           dim3(blocks(m, 64), vbatch ? blocks(m, 4) : std::max(blocks(m, 4), blocks(N2, 16)), gz),
+                           dim3(256), 0, gr.cs, xh, xl, yh, yl, uh, ul, zh, zl, N2, rp);
       else if (m <= 512)
-        hipLaunchKernelGGL((k_jac_step<1, 4, false>), dim3(blocks(m, 64), std::max(blocks(m, 4), blocks(N2, 16)), 2),
-                           dim3(256), 0, gr.cs, xh, xl, yh, yl, uh, ul, zh, zl, N2);
+        hipLaunchKernelGGL((k_jac_step<1, 4, false>),
+                           dim3(blocks(m, 64), vbatch ? blocks(m, 4) : std::max(blocks(m, 4), blocks(N2, 16)), gz),
+                           dim3(256), 0, gr.cs, xh, xl, yh, yl, uh, ul, zh, zl, N2, rp);
       else
         hipLaunchKernelGGL((k_jac_step<4, 16, false>),
-                           dim3(blocks(m, 64), std::max(blocks(m, 16), blocks(N2, 64)), 2), dim3(256), 0, gr.cs, xh,
-                           xl, yh, yl, uh, ul, zh, zl, N2);
+                           dim3(blocks(m, 64), vbatch ? blocks(m, 16) : std::max(blocks(m, 16), blocks(N2, 64)), gz),
+                           dim3(256), 0, gr.cs, xh, xl, yh, yl, uh, ul, zh, zl, N2, rp);
       std::swap(xh, yh);
       std::swap(xl, yl);
-      std::swap(uh, zh);
-      std::swap(ul, zl);
+      if (!vbatch) {
+        std::swap(uh, zh);
+        std::swap(ul, zl);
+      } else if ((step + 1) % VB == 0 || step == N2 - 2) {
+        const int cnt = (int)(step % VB) + 1;
+        const size_t lds = (size_t)32 * vr * N2;
+        if (vr == 2)
+          hipLaunchKernelGGL(k_jac_vbatch<2>, dim3((unsigned)((N2 + 1) / 2)), dim3(256), lds, gr.cs, vh, vl,
+                             rotb.d(), cnt, N2);
+        else
+          hipLaunchKernelGGL(k_jac_vbatch<1>, dim3((unsigned)N2), dim3(256), lds, gr.cs, vh, vl, rotb.d(), cnt, N2);
+        HP_LAUNCH_CHECK();
+      }
     }
     hipGraph_t g = nullptr;
     HP_HIP(hipStreamEndCapture(gr.cs, &g));
@@ -407,8 +487,10 @@ int jacobi_eigen_dev(const double* a_h, const double* a_l, long n, bool stop_fro
     HP_HIP(hipGraphLaunch(gr.exec[parity], s));
     std::swap(ah, bh);  // N2 - 1 (odd) swaps
     std::swap(al, bl);
-    std::swap(vh, wh);
-    std::swap(vl, wl);
+    if (!vbatch) {  // batched V stays in place
+      std::swap(vh, wh);
+      std::swap(vl, wl);
+    }
     parity ^= 1;
     for (long step = 0; step < N2 - 1; ++step) {
       for (long p = 0; p < N2; ++p) next[rr_sigma(p, m)] = layout[p];
