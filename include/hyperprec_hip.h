@@ -183,7 +183,7 @@ int hp_dev_solve_spd_refined(int factor_kind, const double* s_hi, const double* 
                              hp_refine_stats* stats, void* stream);
 int hp_solve_spd_refined(int factor_kind, const double* s_dd, const double* b_dd, size_t n, double tol, int max_iter,
                          double* x_dd, hp_refine_stats* stats);
-/* Panel width of HP_FACTOR_DD_SPLIT (64, 128, 256 or 512; process-wide, default 128). */
+/* Panel width of HP_FACTOR_DD_SPLIT (64, 128, 256 or 512; process-wide, default 256). */
 int hp_set_split_factor_panel(int nb);
 
 /* ------------------------------------------------------------------ batched factor + solve */

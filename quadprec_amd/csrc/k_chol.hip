@@ -564,9 +564,13 @@ int chol_panel_exact(Mat a, Mat S, long c0, long jb, long n, ErrRec* err, double
 
 // The split-dd factorisation's panel: as chol_panel_exact with fast pivots, but the rows
 // below the panel are solved on a side stream, sub-panel q as soon as its diagonal block
-// is factored, overlapping the factor of the next diagonal sub-block.  The reciprocals of
-// sub-panel q live in slot q of inv (2 x 64 doubles per slot).  evq: one event per
-// sub-panel; evdone is recorded on `side` once the panel is complete and `s` waits for it.
+// is factored, overlapping the factor of the next diagonal sub-block.  Each solved
+// sub-panel then updates the panel's later columns of the rows below (right-looking, every
+// element still takes its products in k order), so after the last diagonal block only
+// its own row solve remains.  The reciprocals of sub-panel q live in slot q of inv
+// (2 x 64 doubles per slot).  evq: 16 events, [q] after sub-panel q's diagonal block,
+// [8 + q] after its rows within the panel; evdone is recorded on `side` once the panel is
+// complete and `s` waits for it.
 template <class A>
 int chol_panel_split(Mat a, Mat S, long c0, long jb, long n, ErrRec* err, double* inv, hipStream_t s,
                      hipStream_t side, hipEvent_t* evq, hipEvent_t evdone) {
@@ -596,14 +600,17 @@ int chol_panel_split(Mat a, Mat S, long c0, long jb, long n, ErrRec* err, double
     const long rb = c0 + q0 + w, re = c0 + jb;
     if (re > rb) {
       if ((rc = row_recur<A, true>(a, S, c0 + q0, w, rb, re, ih, il, err, s))) return rc;
+      HP_HIP(hipEventRecord(evq[8 + q], s));
       if ((rc = rank_update<A, RU_ACC, true>(a, S, rb, re - rb, rb, re - rb, rb, c0 + q0, w, err, s))) return rc;
     }
     if (rem > 0) {
       HP_HIP(hipStreamWaitEvent(side, evq[q], 0));
-      if (q0 > 0 &&
-          (rc = rank_update<A, RU_SUBINC, false>(a, S, c0 + jb, rem, c0 + q0, w, c0 + q0, c0, q0, err, side)))
-        return rc;
       if ((rc = row_recur<A, false>(a, S, c0 + q0, w, c0 + jb, n, ih, il, err, side))) return rc;
+      if (re > rb) {  // L[rb:re, sub-panel q] comes from the row solve on s
+        HP_HIP(hipStreamWaitEvent(side, evq[8 + q], 0));
+        if ((rc = rank_update<A, RU_SUBINC, false>(a, S, c0 + jb, rem, rb, re - rb, rb, c0 + q0, w, err, side)))
+          return rc;
+      }
     }
   }
   HP_HIP(hipEventRecord(evdone, side));

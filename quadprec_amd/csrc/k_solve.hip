@@ -429,7 +429,7 @@ int chol_split(Mat a, long n, long nb, Mat S, ErrRec* err, double*, double*, hip
   int rc;
   if ((rc = invb.alloc((size_t)Q * 2 * 64 * sizeof(double)))) return rc;
   hipStream_t s2 = nullptr, s3 = nullptr;
-  hipEvent_t evA = nullptr, evB = nullptr, evP = nullptr, evq[8] = {};
+  hipEvent_t evA = nullptr, evB = nullptr, evP = nullptr, evq[16] = {};
   bool evB_live = false;
   struct Guard {
     hipStream_t *s2, *s3;
@@ -439,14 +439,17 @@ int chol_split(Mat a, long n, long nb, Mat S, ErrRec* err, double*, double*, hip
         if (*x) { (void)hipStreamSynchronize(*x); (void)hipStreamDestroy(*x); }
       for (hipEvent_t* e : {a, b, p})
         if (*e) (void)hipEventDestroy(*e);
-      for (int i = 0; i < 8; ++i)
+      for (int i = 0; i < 16; ++i)
         if (q[i]) (void)hipEventDestroy(q[i]);
     }
   } guard{&s2, &s3, &evA, &evB, &evP, evq};
   HP_HIP(hipStreamCreateWithFlags(&s2, hipStreamNonBlocking));
   HP_HIP(hipStreamCreateWithFlags(&s3, hipStreamNonBlocking));
   for (hipEvent_t* e : {&evA, &evB, &evP}) HP_HIP(hipEventCreateWithFlags(e, hipEventDisableTiming));
-  for (int q = 0; q < Q; ++q) HP_HIP(hipEventCreateWithFlags(&evq[q], hipEventDisableTiming));
+  for (int q = 0; q < Q; ++q) {
+    HP_HIP(hipEventCreateWithFlags(&evq[q], hipEventDisableTiming));
+    HP_HIP(hipEventCreateWithFlags(&evq[8 + q], hipEventDisableTiming));
+  }
   if ((rc = chol_panel_split<ArDD>(a, S, 0, std::min(nb, n), n, err, invb.d(), s, s3, evq, evP))) return rc;
   for (long c0 = 0; c0 < n; c0 += nb) {
     long jb = std::min(nb, n - c0);
