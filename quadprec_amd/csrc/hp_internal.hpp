@@ -102,6 +102,8 @@ struct SplitOpts {
   bool lower_only = false;    // C symmetric: only j <= i is formed (m == n)
   bool subtract = false;      // C -= A B^T instead of C = A B^T
   bool same_operand = false;  // B is A (one set of residues)
+  int slot = 0;     // workspace 0 or 1: calls in flight on two streams at once use different slots
+  int variant = 0;  // int8 kernel (a variant number of set_split_gemm_variant); 0: the tuned one
 };
 int split_gemm_general(const SplitOperand& A, const SplitOperand& B, double* chi, double* clo, size_t ldc, size_t m,
                        size_t n, size_t k, int R, hipStream_t s, const SplitOpts& opt);

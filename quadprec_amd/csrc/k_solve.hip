@@ -460,11 +460,22 @@ int chol_split(Mat a, long n, long nb, Mat S, ErrRec* err, double*, double*, hip
       HP_HIP(hipStreamWaitEvent(s, evB, 0));
       HP_HIP(hipStreamWaitEvent(s3, evB, 0));
     }
-    // next panel's columns take this panel's update: its diagonal block on s, the rows
-    // below on s3 (where the next panel's rows below are solved)
+    // next panel's columns take this panel's update: its diagonal block on s (FP64 VALU),
+    // the rows below on s3 (where the next panel's rows below are solved) as a split GEMM
+    // in the second workspace slot, beside the trailing SYRK's
     if ((rc = rank_update_sumsub<ArDD>(a, S, t0, w2, t0, w2, c0, jb, err, s))) return rc;
-    if (rem > w2 && (rc = rank_update_sumsub_below<ArDD>(a, S, t0 + w2, rem - w2, t0, w2, c0, jb, err, s3)))
-      return rc;
+    if (rem > w2) {
+      const long r1 = t0 + w2;
+      SplitOperand P{a.hi + r1 * a.ld + c0, a.lo + r1 * a.ld + c0, a.ld, true};
+      SplitOperand Q{a.hi + t0 * a.ld + c0, a.lo + t0 * a.ld + c0, a.ld, true};
+      SplitOpts o;
+      o.subtract = true;
+      o.slot = 1;
+      o.variant = 4064;
+      if ((rc = split_gemm_general(P, Q, a.hi + r1 * a.ld + t0, a.lo + r1 * a.ld + t0, (size_t)a.ld,
+                                   (size_t)(rem - w2), (size_t)w2, (size_t)jb, 31, s3, o)))
+        return rc;
+    }
     HP_HIP(hipEventRecord(evA, s));
     // next panel on s (and s3), concurrently with the rest update below
     if ((rc = chol_panel_split<ArDD>(a, S, t0, w2, n, err, invb.d(), s, s3, evq, evP))) return rc;

@@ -1097,10 +1097,12 @@ __global__ __launch_bounds__(256) void k_crt(const uint8_t* __restrict__ Cr, lon
 }
 
 // ------------------------------------------------------------------ workspace cache
-// The workspace (residue planes, CRT digits, scales) is shared by all calls.  A call
-// records `done` after its last kernel; the next call on the same stream is ordered by the
-// stream, one on another stream first waits for `done` (no host synchronisation), and the
-// buffer or the table is replaced only after `done` has completed.
+// A workspace (residue planes, CRT digits, scales) is shared by all calls of its slot.  A
+// call records `done` after its last kernel; the next call of the slot on the same stream
+// is ordered by the stream, one on another stream first waits for `done` (no host
+// synchronisation), and the buffer or the table is replaced only after `done` has
+// completed.  Two slots let two streams run split GEMMs concurrently (the split-dd
+// factorisation's trailing SYRK and its next panel's rows).
 struct SplitWork {
   std::mutex mu;
   void* buf = nullptr;
@@ -1111,8 +1113,9 @@ struct SplitWork {
   hipStream_t last = nullptr;
   bool pending = false;  // done has been recorded
 };
-SplitWork g_work;
-std::map<uint64_t, int> g_tuned;  // autotuned GEMM kernel per shape class (under g_work.mu)
+SplitWork g_work[2];
+std::mutex g_tune_mu;
+std::map<uint64_t, int> g_tuned;  // autotuned GEMM kernel per shape class (under g_tune_mu)
 
 uint64_t shape_key(size_t m, size_t n, size_t k, int R) {
   auto bucket = [](size_t v) {  // next power of two
@@ -1134,7 +1137,7 @@ int set_split_gemm_variant(size_t m, size_t n, size_t k, int R, int variant) {
               "(register-staged, three stages ahead, K % 192 == 0)");
     return HP_EINVAL;
   }
-  std::lock_guard<std::mutex> lock(g_work.mu);
+  std::lock_guard<std::mutex> lock(g_tune_mu);
   uint64_t key = shape_key(m, n, k, R <= 0 ? 31 : R);
   if (variant == 0) g_tuned.erase(key);
   else g_tuned[key] = variant;
@@ -1142,7 +1145,7 @@ int set_split_gemm_variant(size_t m, size_t n, size_t k, int R, int variant) {
 }
 
 int split_gemm_variant(size_t m, size_t n, size_t k, int R) {
-  std::lock_guard<std::mutex> lock(g_work.mu);
+  std::lock_guard<std::mutex> lock(g_tune_mu);
   auto it = g_tuned.find(shape_key(m, n, k, R <= 0 ? 31 : R));
   return it == g_tuned.end() ? 0 : it->second;
 }
@@ -1165,6 +1168,15 @@ int split_gemm_general(const SplitOperand& A, const SplitOperand& B, double* chi
     set_error("split int8 gemm: moduli count %d outside [8, %d]", R, CRT_MAXR);
     return HP_EINVAL;
   }
+  if (opt.slot != 0 && opt.slot != 1) {
+    set_error("split gemm: workspace slot %d (0 or 1)", opt.slot);
+    return HP_EINVAL;
+  }
+  if (opt.variant != 0 && opt.variant != 32 && opt.variant != 64 && opt.variant != 128 && opt.variant != 3064 &&
+      opt.variant != 4064 && opt.variant != 5064 && opt.variant != 6064) {
+    set_error("split gemm: unknown kernel variant %d", opt.variant);
+    return HP_EINVAL;
+  }
   if (k == 0) {
     HP_HIP(hipMemset2DAsync(chi, ldc * sizeof(double), 0, n * sizeof(double), m, s));
     HP_HIP(hipMemset2DAsync(clo, ldc * sizeof(double), 0, n * sizeof(double), m, s));
@@ -1178,6 +1190,7 @@ int split_gemm_general(const SplitOperand& A, const SplitOperand& B, double* chi
   const size_t Kp = (k + DL_GK - 1) / DL_GK * DL_GK, Np = (n + 3) / 4 * 4;
   const size_t resA = (size_t)R * m * Kp, resB = (size_t)R * n * Kp, resC = (size_t)R * m * Np;
   const size_t need = resA + resB + resC + (m + n) * sizeof(int) + 1024;
+  SplitWork& g_work = hp::g_work[opt.slot];
   std::lock_guard<std::mutex> lock(g_work.mu);
   if (!g_work.done) HP_HIP(hipEventCreateWithFlags(&g_work.done, hipEventDisableTiming));
   if (g_work.pending && g_work.last != s) HP_HIP(hipStreamWaitEvent(s, g_work.done, 0));
@@ -1222,7 +1235,8 @@ int split_gemm_general(const SplitOperand& A, const SplitOperand& B, double* chi
   uint8_t* rC = reinterpret_cast<uint8_t*>(rB + resB);
   int* eA = reinterpret_cast<int*>(((uintptr_t)(rC + resC) + 15) & ~(uintptr_t)15);
   int* fB = eA + m;
-  PhaseTimer& tm = phase_timer();
+  PhaseTimer idle;  // phases are timed for slot 0 only
+  PhaseTimer& tm = opt.slot == 0 ? phase_timer() : idle;
   tm.n = 0;
   tm.mark(s);
   // scales: rows of A (contiguous unless gemm_atb), columns of B
@@ -1290,8 +1304,9 @@ int split_gemm_general(const SplitOperand& A, const SplitOperand& B, double* chi
   // autotuner: the first call for a shape class times each kernel on the real operands
   // (HIP events on this stream) and keeps the fastest
   const uint64_t key = shape_key(m, n, k, R) | (opt.lower_only ? (1ull << 40) : 0);  // R already defaulted
-  int variant = 0;
-  {
+  int variant = opt.variant;
+  if (!variant) {
+    std::lock_guard<std::mutex> tl(g_tune_mu);
     auto it = g_tuned.find(key);
     if (it != g_tuned.end()) variant = it->second;
   }
@@ -1315,7 +1330,10 @@ int split_gemm_general(const SplitOperand& A, const SplitOperand& B, double* chi
     }
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
-    g_tuned[key] = variant;
+    {
+      std::lock_guard<std::mutex> tl(g_tune_mu);
+      g_tuned[key] = variant;
+    }
     tm.n = 0;  // tuning runs are not part of the timed phases
     tm.mark(s);
     tm.mark(s);
