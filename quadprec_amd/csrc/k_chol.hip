@@ -169,6 +169,110 @@ __global__ __launch_bounds__(256) void k_diag_factor(Mat a, Mat S, long r0, int 
   }
 }
 
+// ---------------------------------------------------------------- diagonal block, split-dd
+// The split-dd factorisation's diagonal block (reference order not required).  a[r0:r0+jb,
+// r0:r0+jb] (jb <= 64) is reduced in place in LDS, column by column: every element takes
+// one dd multiply and one dd subtraction per column, T(i, c) -= L(i, j) L(c, j), and column
+// c is finished as L(i, c) = T(i, c) / L(c, c) with the pivot's root and reciprocal from a
+// dd reciprocal square root.  Per column the chain is one multiply-subtract, the
+// reciprocal square root, one broadcast and one multiply.  Wavefront 0 finishes column
+// j+1 while the other seven apply column j to the columns beyond j+1 (through the packed
+// triangle index table).  One 64x65 dd tile (66.5 KB).  On a breakdown the block is left
+// as it was and err records the pivot (linalg.rs:54-57).
+__global__ __launch_bounds__(512) void k_diag_factor_split(Mat a, long r0, int jb, ErrRec* err, double* inv_hi,
+                                                           double* inv_lo) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  Lds2<ArDD>& T = *reinterpret_cast<Lds2<ArDD>*>(smem);
+  __shared__ int fail;
+  __shared__ double invs_h[DB], invs_l[DB];
+  // packed triangle: entry e = q(q+1)/2 + p, 0 <= p <= q, is the element (jb-1-p, jb-1-q);
+  // the columns beyond j+1 form the prefix q <= jb-3-j
+  __shared__ unsigned short tri[DB * (DB - 1) / 2];
+  const int tid = threadIdx.x, lane = tid & 63;
+  if (err->flag) return;
+  for (int idx = tid; idx < jb * jb; idx += blockDim.x) {
+    int i = idx / jb, c = idx % jb;
+    if (c <= i) lset<ArDD>(T, i, c, ArDD::ld(a, r0 + i, r0 + c));
+  }
+  for (int e = tid; e < DB * (DB - 1) / 2; e += blockDim.x) {
+    int q = (int)((__builtin_sqrtf(8.0f * e + 1.0f) - 1.0f) * 0.5f);
+    while (q * (q + 1) / 2 > e) --q;
+    while ((q + 1) * (q + 2) / 2 <= e) ++q;
+    tri[e] = (unsigned short)((e - q * (q + 1) / 2) | (q << 8));
+  }
+  if (tid == 0) fail = 0;
+  __syncthreads();
+  // wavefront 0: finish column c from t = T(lane, c) after every earlier column
+  dd lcol = mkdd(0.0, 0.0);
+  auto finish_column = [&](int c, dd t) {
+    dd inv = mkdd(0.0, 0.0);
+    int bad = 0;
+    if (lane == c) {
+      if (ArDD::nonpos(t)) {
+        err->flag = 1;
+        err->pivot = r0 + c;
+        err->value = ArDD::f64(t);
+        fail = 1;
+        bad = 1;
+      } else {
+        dd ljj;
+        ArDD::sqrt_recip_fast(t, ljj, inv);
+        lset<ArDD>(T, c, c, ljj);
+        invs_h[c] = inv.hi;
+        invs_l[c] = inv.lo;
+      }
+    }
+    if (__builtin_amdgcn_readlane(bad, c)) return;
+    inv = rl(inv, c);
+    if (lane > c && lane < jb) {
+      lcol = dd_mul(t, inv);
+      lset<ArDD>(T, lane, c, lcol);
+    }
+  };
+  if (tid < 64) finish_column(0, lane < jb ? lget<ArDD>(T, lane, 0) : mkdd(0.0, 0.0));
+  __syncthreads();
+  constexpr int NT = 512 - 64;  // threads of the trailing wavefronts
+  for (int j = 0; j + 1 < jb; ++j) {
+    if (fail) return;
+    if (tid < 64) {
+      const int c = j + 1;
+      dd t = mkdd(0.0, 0.0);
+      const dd ljc = rl(lcol, c);  // L(j+1, j)
+      if (lane >= c && lane < jb) t = dd_sub(lget<ArDD>(T, lane, c), dd_mul(lcol, ljc));
+      finish_column(c, t);
+    } else {
+      const int m = jb - j - 2;  // columns j+2 .. jb-1
+      const int cnt = m * (m + 1) / 2;
+      for (int e0 = tid - 64; e0 < cnt; e0 += 2 * NT) {
+        dd sv[2], li[2], lc[2];
+        int ii[2], cc[2];
+#pragma unroll
+        for (int u = 0; u < 2; ++u) {  // loads of both elements first (index clamped in range)
+          const unsigned short pq = tri[min(e0 + u * NT, cnt - 1)];
+          ii[u] = jb - 1 - (pq & 255);
+          cc[u] = jb - 1 - (pq >> 8);
+          sv[u] = lget<ArDD>(T, ii[u], cc[u]);
+          li[u] = lget<ArDD>(T, ii[u], j);
+          lc[u] = lget<ArDD>(T, cc[u], j);
+        }
+#pragma unroll
+        for (int u = 0; u < 2; ++u)
+          if (e0 + u * NT < cnt) lset<ArDD>(T, ii[u], cc[u], dd_sub(sv[u], dd_mul(li[u], lc[u])));
+      }
+    }
+    __syncthreads();
+  }
+  if (fail) return;
+  for (int idx = tid; idx < jb * jb; idx += blockDim.x) {
+    int i = idx / jb, c = idx % jb;
+    if (c <= i) ArDD::st(a, r0 + i, r0 + c, lget<ArDD>(T, i, c));
+  }
+  if (tid < jb) {
+    inv_hi[tid] = invs_h[tid];
+    inv_lo[tid] = invs_l[tid];
+  }
+}
+
 // ---------------------------------------------------------------- rows below a panel
 // Rows [row_begin, row_end) x panel columns [c0, c0+w), w <= 64, L = the factored
 // diagonal block a[c0:c0+w, c0:c0+w], inv[k] = recip(L[k][k]).
@@ -464,6 +568,8 @@ static int set_lds_attrs() {
                              big));
   HP_HIP(hipFuncSetAttribute((const void*)k_diag_factor<A, false, true>, hipFuncAttributeMaxDynamicSharedMemorySize,
                              big));
+  if constexpr (A::LIMBS == 2)
+    HP_HIP(hipFuncSetAttribute((const void*)k_diag_factor_split, hipFuncAttributeMaxDynamicSharedMemorySize, one));
   HP_HIP(hipFuncSetAttribute((const void*)k_row_recur<A, true>, hipFuncAttributeMaxDynamicSharedMemorySize, one));
   HP_HIP(hipFuncSetAttribute((const void*)k_row_recur<A, false>, hipFuncAttributeMaxDynamicSharedMemorySize, one));
   HP_HIP(hipFuncSetAttribute((const void*)k_fwd_diag<A>, hipFuncAttributeMaxDynamicSharedMemorySize, one));
@@ -562,46 +668,36 @@ int chol_panel_exact(Mat a, Mat S, long c0, long jb, long n, ErrRec* err, double
   return HP_OK;
 }
 
-// The split-dd factorisation's panel: as chol_panel_exact with fast pivots, but the rows
+// The split-dd factorisation's panel (reference order not required): each 64-wide
+// diagonal sub-block by k_diag_factor_split, the panel's rows below it solved and updated
+// by subtraction (t = a; t -= l_ik l_jk; l_ij = t / l_jj), as in trsm_right_lt.  The rows
 // below the panel are solved on a side stream, sub-panel q as soon as its diagonal block
-// is factored, overlapping the factor of the next diagonal sub-block.  Each solved
-// sub-panel then updates the panel's later columns of the rows below (right-looking, every
-// element still takes its products in k order), so after the last diagonal block only
-// its own row solve remains.  The reciprocals of sub-panel q live in slot q of inv
-// (2 x 64 doubles per slot).  evq: 16 events, [q] after sub-panel q's diagonal block,
-// [8 + q] after its rows within the panel; evdone is recorded on `side` once the panel is
-// complete and `s` waits for it.
+// is factored, overlapping the factor of the next diagonal sub-block; each solved
+// sub-panel then updates the panel's later columns of those rows right away, so after the
+// last diagonal block only its own row solve remains.  The reciprocals of sub-panel q live
+// in slot q of inv (2 x 64 doubles per slot).  evq: 16 events, [q] after sub-panel q's
+// diagonal block, [8 + q] after its rows within the panel; evdone is recorded on `side`
+// once the panel is complete and `s` waits for it.
 template <class A>
 int chol_panel_split(Mat a, Mat S, long c0, long jb, long n, ErrRec* err, double* inv, hipStream_t s,
                      hipStream_t side, hipEvent_t* evq, hipEvent_t evdone) {
+  static_assert(A::LIMBS == 2, "the split-dd panel is double-double");
   int rc = ensure_attrs<A>();
   if (rc) return rc;
-  const bool region = jb > DB;
-  if (region) {
-    long total = jb * jb;
-    unsigned g = (unsigned)std::min<long>((total + 255) / 256, 65536);
-    hipLaunchKernelGGL(k_zero_lower<A>, dim3(g ? g : 1), dim3(256), 0, s, S, c0, jb);
-    HP_LAUNCH_CHECK();
-  }
   const long rem = n - c0 - jb;
   for (long q0 = 0; q0 < jb; q0 += DB) {
     const int w = (int)std::min<long>(DB, jb - q0);
     const int q = (int)(q0 / DB);
     double* ih = inv + (size_t)q * 2 * DB;
     double* il = ih + DB;
-    if (region)
-      hipLaunchKernelGGL((k_diag_factor<A, true, true>), dim3(1), dim3(256), 2 * sizeof(Lds2<A>), s, a, S, c0 + q0, w,
-                         err, ih, il);
-    else
-      hipLaunchKernelGGL((k_diag_factor<A, false, true>), dim3(1), dim3(256), 2 * sizeof(Lds2<A>), s, a, S, c0 + q0,
-                         w, err, ih, il);
+    hipLaunchKernelGGL(k_diag_factor_split, dim3(1), dim3(512), sizeof(Lds2<A>), s, a, c0 + q0, w, err, ih, il);
     HP_LAUNCH_CHECK();
     HP_HIP(hipEventRecord(evq[q], s));
     const long rb = c0 + q0 + w, re = c0 + jb;
     if (re > rb) {
-      if ((rc = row_recur<A, true>(a, S, c0 + q0, w, rb, re, ih, il, err, s))) return rc;
+      if ((rc = row_recur<A, false>(a, S, c0 + q0, w, rb, re, ih, il, err, s))) return rc;
       HP_HIP(hipEventRecord(evq[8 + q], s));
-      if ((rc = rank_update<A, RU_ACC, true>(a, S, rb, re - rb, rb, re - rb, rb, c0 + q0, w, err, s))) return rc;
+      if ((rc = rank_update<A, RU_SUBINC, true>(a, S, rb, re - rb, rb, re - rb, rb, c0 + q0, w, err, s))) return rc;
     }
     if (rem > 0) {
       HP_HIP(hipStreamWaitEvent(side, evq[q], 0));

@@ -416,7 +416,9 @@ double absmax_dev(const double* v, long n, unsigned long long* scratch, hipStrea
 // columns updated on VALU (its diagonal block first, the rows below on the side stream),
 // then the rest of the trailing matrix updated by one split SYRK
 // (lower tiles only, the subtraction fused into the CRT) on a second stream while the next
-// panel is factored.  Every element receives the panels' updates in panel order.
+// panel is factored.  Every element receives the panels' updates in panel order.  All
+// updates subtract in place: S (a partial-sum plane in the other factorisations) is not
+// read or written.
 int chol_split(Mat a, long n, long nb, Mat S, ErrRec* err, double*, double*, hipStream_t s) {
   if (nb <= 0) nb = 256;
   nb = std::max<long>(64, (nb / 64) * 64);
@@ -590,7 +592,9 @@ int solve_refined(int kind, const double* sh, const double* sl, const double* bh
   const int limbs = (kind == HP_FACTOR_F64) ? 1 : 2;
   DevBuf binv;
   const long nblk = (n + BW - 1) / BW;
-  if ((rc = L.alloc(nn * limbs * sizeof(double))) || (rc = Sw.alloc(nn * limbs * sizeof(double))) ||
+  // the split factorisation works by subtraction in place: no partial-sum plane
+  const size_t sw_elems = kind == HP_FACTOR_DD_SPLIT ? 0 : nn * limbs;
+  if ((rc = L.alloc(nn * limbs * sizeof(double))) || (rc = Sw.alloc(sw_elems * sizeof(double))) ||
       (rc = inv.alloc(2 * 64 * sizeof(double))) || (rc = err.alloc(sizeof(ErrRec))) ||
       (rc = binv.alloc((size_t)nblk * BW * BW * limbs * sizeof(double))))
     return rc;
@@ -600,7 +604,7 @@ int solve_refined(int kind, const double* sh, const double* sl, const double* bh
   HP_HIP(hipStreamSynchronize(s));
   auto t0 = clk::now();
   Mat Lm{L.d(), limbs == 2 ? L.d() + nn : nullptr, n};
-  Mat Sm{Sw.d(), limbs == 2 ? Sw.d() + nn : nullptr, n};
+  Mat Sm{Sw.d(), limbs == 2 && sw_elems ? Sw.d() + nn : nullptr, n};
   HP_HIP(hipMemcpyAsync(Lm.hi, sh, nn * sizeof(double), hipMemcpyDeviceToDevice, s));
   if (limbs == 2) HP_HIP(hipMemcpyAsync(Lm.lo, sl, nn * sizeof(double), hipMemcpyDeviceToDevice, s));
   ErrRec* er = (ErrRec*)err.p;

@@ -70,15 +70,17 @@ def test_refined_solve_vs_cpu_reference(qp, lo, spacing, kinds, bar):
         assert e <= bar, (kind, e)
 
 
-def test_refined_solve_f64_factor_fails_cleanly_on_indefinite(qp):
-    n = 64
+@pytest.mark.parametrize("n,piv", [(64, 10), (520, 300), (520, 517)])
+def test_refined_solve_split_factor_fails_cleanly_on_indefinite(qp, n, piv):
+    """A negative pivot in the first block, in a later panel's first sub-block, and in the
+    ragged last sub-block: HP_ENOTSPD with that pivot."""
     s = np.zeros((n * n, 2))
     for i in range(n):
         s[i * n + i, 0] = 1.0
-    s[10 * n + 10, 0] = -1.0
+    s[piv * n + piv, 0] = -1.0
     with pytest.raises(qp.SolveError) as e:
         qp.solve_spd_refined(s, qp.gen_dd_normal(1, n), n, qp.HP_FACTOR_DD_SPLIT)
-    assert e.value.pivot == 10
+    assert e.value.pivot == piv
 
 
 def test_split_cholesky_factor_matches_plain_to_dd_accuracy(qp):
