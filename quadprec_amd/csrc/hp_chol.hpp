@@ -42,11 +42,15 @@ struct ArDD {
   __device__ static T readlane(T v, int lane) {
     return mkdd(__shfl(v.hi, lane, 64), __shfl(v.lo, lane, 64));
   }
-  // sqrt(d) and 1/sqrt(d) by one dd Newton step on the binary64 reciprocal square root:
-  // y = y0 + y0 (1 - d y0^2) / 2, sqrt = d y.  One division instead of the reference's
-  // three (sqrt then recip, dd.rs:61-83); used where reference order is not required.
+  // sqrt(d) and 1/sqrt(d) by one dd Newton step, y = y0 + y0 (1 - d y0^2) / 2, sqrt = d y,
+  // from y0 = the hardware reciprocal square root of d.hi refined by one binary64 Newton
+  // step (relative error ~2^-52 before the dd step, ~2^-104 after).  No division (the
+  // reference takes sqrt, then recip: dd.rs:61-83); used where reference order is not
+  // required.
   __device__ static void sqrt_recip_fast(T d, T& root, T& inv) {
-    const dd y0 = mkdd(1.0 / __builtin_sqrt(d.hi));
+    double r = __builtin_amdgcn_rsq(d.hi);
+    r = r * __builtin_fma(-(0.5 * d.hi) * r, r, 1.5);
+    const dd y0 = mkdd(r);
     const dd e = dd_sub(mkdd(1.0), dd_mul(d, dd_mul(y0, y0)));
     inv = dd_add(y0, dd_mul_f(dd_mul(y0, e), 0.5));
     root = dd_mul(d, inv);
