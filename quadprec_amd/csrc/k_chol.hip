@@ -61,7 +61,7 @@ __device__ __forceinline__ void lset(Lds2<A>& s, int i, int j, typename A::T v) 
 // One barrier per column.  Between barriers j and j+1, wavefront 0 finishes column j+1
 // alone (lane i holds row i): it adds column j's product to S(i, j+1), takes the pivot on
 // lane j+1, broadcasts recip(L[j+1][j+1]) with v_readlane and scales the column; the
-// other three wavefronts add column j's products to the columns beyond j+1 (walked
+// other seven wavefronts add column j's products to the columns beyond j+1 (walked
 // through a packed-triangle index table, so every lane has work).  Every element still
 // receives its products in increasing k, one dd add each, and is scaled exactly as in
 // the unblocked algorithm.  The reciprocals stay in LDS until the block is done, so no
@@ -69,7 +69,7 @@ __device__ __forceinline__ void lset(Lds2<A>& s, int i, int j, typename A::T v) 
 // pivot's root and reciprocal by a dd reciprocal square root instead of the reference's
 // sqrt then recip.
 template <class A, bool S_INIT, bool FAST = false>
-__global__ __launch_bounds__(256) void k_diag_factor(Mat a, Mat S, long r0, int jb, ErrRec* err, double* inv_hi,
+__global__ __launch_bounds__(512) void k_diag_factor(Mat a, Mat S, long r0, int jb, ErrRec* err, double* inv_hi,
                                                      double* inv_lo) {
   using T = typename A::T;
   extern __shared__ __align__(16) unsigned char smem[];
@@ -137,14 +137,15 @@ __global__ __launch_bounds__(256) void k_diag_factor(Mat a, Mat S, long r0, int 
       if (lane >= c && lane < jb) sc = A::add(lget<A>(Sa, lane, c), A::mul(lcol, ljc));
       finish_column(c, sc);
     } else {
+      constexpr int NTR = 512 - 64;  // threads of the trailing wavefronts
       const int m = jb - j - 2;  // columns j+2 .. jb-1
       const int cnt = m * (m + 1) / 2;
-      for (int e0 = tid - 64; e0 < cnt; e0 += 2 * 192) {
+      for (int e0 = tid - 64; e0 < cnt; e0 += 2 * NTR) {
         T sv[2], li[2], lc[2];
         int ii[2], cc[2];
 #pragma unroll
         for (int u = 0; u < 2; ++u) {  // loads of both elements first (index clamped in range)
-          const unsigned short pq = tri[min(e0 + u * 192, cnt - 1)];
+          const unsigned short pq = tri[min(e0 + u * NTR, cnt - 1)];
           ii[u] = jb - 1 - (pq & 255);
           cc[u] = jb - 1 - (pq >> 8);
           sv[u] = lget<A>(Sa, ii[u], cc[u]);
@@ -153,7 +154,7 @@ __global__ __launch_bounds__(256) void k_diag_factor(Mat a, Mat S, long r0, int 
         }
 #pragma unroll
         for (int u = 0; u < 2; ++u)
-          if (e0 + u * 192 < cnt) lset<A>(Sa, ii[u], cc[u], A::add(sv[u], A::mul(li[u], lc[u])));
+          if (e0 + u * NTR < cnt) lset<A>(Sa, ii[u], cc[u], A::add(sv[u], A::mul(li[u], lc[u])));
       }
     }
     __syncthreads();
@@ -628,7 +629,7 @@ static int factor_unblocked_region(Mat a, Mat S, long r0, long len, ErrRec* err,
   }
   for (long c0 = r0; c0 < r0 + len; c0 += DB) {
     int w = (int)std::min<long>(DB, r0 + len - c0);
-    hipLaunchKernelGGL((k_diag_factor<A, true, FAST>), dim3(1), dim3(256), 2 * sizeof(Lds2<A>), s, a, S, c0, w, err,
+    hipLaunchKernelGGL((k_diag_factor<A, true, FAST>), dim3(1), dim3(512), 2 * sizeof(Lds2<A>), s, a, S, c0, w, err,
                        inv_h, inv_l);
     HP_LAUNCH_CHECK();
     long rb = c0 + w, re = r0 + len;
@@ -647,7 +648,7 @@ int chol_panel_exact(Mat a, Mat S, long c0, long jb, long n, ErrRec* err, double
   int rc = ensure_attrs<A>();
   if (rc) return rc;
   if (jb <= DB) {
-    hipLaunchKernelGGL((k_diag_factor<A, false, FAST>), dim3(1), dim3(256), 2 * sizeof(Lds2<A>), s, a, S, c0, (int)jb,
+    hipLaunchKernelGGL((k_diag_factor<A, false, FAST>), dim3(1), dim3(512), 2 * sizeof(Lds2<A>), s, a, S, c0, (int)jb,
                        err, inv_h, inv_l);
     HP_LAUNCH_CHECK();
   } else if ((rc = factor_unblocked_region<A, FAST>(a, S, c0, jb, err, inv_h, inv_l, s))) {
