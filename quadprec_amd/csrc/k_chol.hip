@@ -279,9 +279,12 @@ __global__ __launch_bounds__(512) void k_diag_factor_split(Mat a, long r0, int j
 // diagonal block a[c0:c0+w, c0:c0+w], inv[k] = recip(L[k][k]).
 //   SUB (trsm_right_lt, linalg.rs:224-235):  t = b_ij; t = t - b_ik*l_jk (k<j); b_ij = t*inv_j
 //   ACC (unblocked, linalg.rs:78-82):        t = S_ij; t = t + l_ik*l_jk (k<j); l_ij = (a_ij - t)*inv_j
-// One wavefront per row, lane j = panel column j.
+// One wavefront per row, lane j = panel column j; the rows of a workgroup share the L tile
+// (16 for binary64; 4 for dd, whose chain needs more registers than 16 rows leave).
+template <class A>
+constexpr int row_recur_rows() { return A::LIMBS == 2 ? 4 : 16; }
 template <class A, bool ACC>
-__global__ __launch_bounds__(256) void k_row_recur(Mat a, Mat S, long c0, int w, long row_begin, long row_end,
+__global__ __launch_bounds__(64 * row_recur_rows<A>()) void k_row_recur(Mat a, Mat S, long c0, int w, long row_begin, long row_end,
                                                    const double* inv_hi, const double* inv_lo, const ErrRec* err) {
   using T = typename A::T;
   extern __shared__ __align__(16) unsigned char smem[];
@@ -295,7 +298,8 @@ __global__ __launch_bounds__(256) void k_row_recur(Mat a, Mat S, long c0, int w,
   __syncthreads();
   T inv = A::zero();
   if (lane < w) inv = A::ldv(inv_hi, inv_lo, lane);
-  for (long row = row_begin + (long)blockIdx.x * 4 + wv; row < row_end; row += (long)gridDim.x * 4) {
+  constexpr int RPB = row_recur_rows<A>();
+  for (long row = row_begin + (long)blockIdx.x * RPB + wv; row < row_end; row += (long)gridDim.x * RPB) {
     T t = A::zero(), av = A::zero(), fin = A::zero();
     if (lane < w) {
       if (ACC) { t = A::ld(S, row, c0 + lane); av = A::ld(a, row, c0 + lane); }
@@ -610,8 +614,9 @@ template <class A, bool ACC>
 static int row_recur(Mat a, Mat S, long c0, int w, long rb, long re, double* inv_h, double* inv_l, ErrRec* err,
                      hipStream_t s) {
   if (re <= rb) return HP_OK;
-  unsigned g = (unsigned)std::min<long>((re - rb + 3) / 4, 2048);
-  hipLaunchKernelGGL((k_row_recur<A, ACC>), dim3(g), dim3(256), sizeof(Lds2<A>), s, a, S, c0, w, rb, re, inv_h,
+  constexpr int RPB = row_recur_rows<A>();
+  unsigned g = (unsigned)std::min<long>((re - rb + RPB - 1) / RPB, 8192 / RPB);
+  hipLaunchKernelGGL((k_row_recur<A, ACC>), dim3(g), dim3(64 * RPB), sizeof(Lds2<A>), s, a, S, c0, w, rb, re, inv_h,
                      inv_l, (const ErrRec*)err);
   HP_LAUNCH_CHECK();
   return HP_OK;
