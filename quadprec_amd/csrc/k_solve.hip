@@ -164,67 +164,66 @@ __global__ __launch_bounds__(64) void k_diag_block_inverse(Mat l, long n, double
 
 // one block step of L y = b (FWD) or L^T x = y (!FWD); rows handled: FWD rows >= b0+bw,
 // !FWD rows < b0.  x_b goes to xo; y is updated in place for the other rows.
+// Both products run on all four wavefronts with the same shape: 16 rows per wavefront,
+// four lanes per row, each lane 16 of the 64 terms (read straight from memory, the
+// inverse and the factor alike), the quarters combined with lane shuffles.  x_b passes
+// to the update through LDS.  All loads are issued before the first product.
+template <class A>
+__device__ __forceinline__ typename A::T quarter_sum(typename A::T q) {  // lanes 4r..4r+3
+#pragma unroll
+  for (int off = 1; off < 4; off <<= 1) {
+    typename A::T o;
+    if constexpr (A::LIMBS == 2) o = mkdd(__shfl_xor(q.hi, off, 64), __shfl_xor(q.lo, off, 64));
+    else o = __shfl_xor(q, off, 64);
+    q = A::add(q, o);
+  }
+  return q;
+}
 template <class A, bool FWD>
 __global__ __launch_bounds__(256) void k_inv_step(Mat l, const double* ih, const double* il, double* yh, double* yl,
                                                   double* xoh, double* xol, long n, long b0, int bw) {
   using T = typename A::T;
-  __shared__ double Yh[BW], Yl[BW], Xh[BW], Xl[BW];
-  __shared__ double Ih[BW][BW + 1], Il[A::LIMBS == 2 ? BW : 1][BW + 1];
-  const int t = threadIdx.x;
+  __shared__ double Xh[BW], Xl[BW];
+  const int t = threadIdx.x, part = t & 3, kb = part * 16;
+  const int r = t >> 2;  // 0..63: row of x_b, and row of this workgroup's update
   const size_t base = (size_t)(b0 / BW) * BW * BW;
-  for (int e = t; e < BW * BW; e += 256) {  // the block inverse, one coalesced sweep
-    Ih[e >> 6][e & 63] = ih[base + e];
-    if constexpr (A::LIMBS == 2) Il[e >> 6][e & 63] = il[base + e];
-  }
-  if (t < bw) {
-    Yh[t] = yh[b0 + t];
-    if constexpr (A::LIMBS == 2) Yl[t] = yl[b0 + t];
-  }
-  __syncthreads();
-  if (t < bw) {  // x_t = sum_k inv[t][k] y_k (FWD) or sum_k inv[k][t] y_k (transposed)
-    T p[4] = {A::zero(), A::zero(), A::zero(), A::zero()};  // four independent chains
-    auto term = [&](int k) {
-      const int r = FWD ? t : k, c = FWD ? k : t;
-      T iv, yv;
-      if constexpr (A::LIMBS == 2) { iv = mkdd(Ih[r][c], Il[r][c]); yv = mkdd(Yh[k], Yl[k]); }
-      else { iv = Ih[r][c]; yv = Yh[k]; }
-      return A::mul(iv, yv);
-    };
-    if (bw == BW) {
-#pragma unroll 4
-      for (int k = 0; k < BW; k += 4) {
-        T t0 = term(k), t1 = term(k + 1), t2 = term(k + 2), t3 = term(k + 3);
-        p[0] = A::add(p[0], t0);
-        p[1] = A::add(p[1], t1);
-        p[2] = A::add(p[2], t2);
-        p[3] = A::add(p[3], t3);
-      }
-    } else {
-      for (int k = 0; k < bw; ++k) p[0] = A::add(p[0], term(k));
-    }
-    T s = A::add(A::add(p[0], p[1]), A::add(p[2], p[3]));
-    if constexpr (A::LIMBS == 2) { Xh[t] = s.hi; Xl[t] = s.lo; }
-    else Xh[t] = s;
-    if (blockIdx.x == 0) A::stv(xoh, xol, b0 + t, s);
-  }
-  __syncthreads();
-  // 64 rows per workgroup, four lanes per row, each lane a quarter of the 64 terms; the
-  // quarters are combined with lane shuffles
-  const int part = t & 3;
-  long i = (long)blockIdx.x * 64 + (t >> 2);
+  long i = (long)blockIdx.x * 64 + r;
   if (FWD) i += b0 + bw;
   const bool live = FWD ? (i < n) : (i < b0);
-  T q0 = A::zero(), q1 = A::zero();
+  // x_r = sum_k inv[r][k] y_k (FWD) or sum_k inv[k][r] y_k (transposed); entries of the
+  // inverse outside the block are stored as zero
+  T iv[16], yv[16], lv[16];
+#pragma unroll
+  for (int u = 0; u < 16; ++u) {
+    const int k = kb + u;
+    const size_t e = base + (FWD ? (size_t)r * BW + k : (size_t)k * BW + r);
+    iv[u] = A::ldv(ih, il, (long)e);
+    yv[u] = k < bw ? A::ldv(yh, yl, b0 + k) : A::zero();
+  }
+  if (live && bw == BW) {
+#pragma unroll
+    for (int u = 0; u < 16; ++u) lv[u] = FWD ? A::ld(l, i, b0 + kb + u) : A::ld(l, b0 + kb + u, i);
+  }
+  T p0 = A::zero(), p1 = A::zero();
+#pragma unroll
+  for (int u = 0; u < 16; u += 2) {
+    p0 = A::add(p0, A::mul(iv[u], yv[u]));
+    p1 = A::add(p1, A::mul(iv[u + 1], yv[u + 1]));
+  }
+  const T xr = quarter_sum<A>(A::add(p0, p1));
+  if (part == 0 && r < bw) {
+    if constexpr (A::LIMBS == 2) { Xh[r] = xr.hi; Xl[r] = xr.lo; }
+    else Xh[r] = xr;
+    if (blockIdx.x == 0) A::stv(xoh, xol, b0 + r, xr);
+  }
+  __syncthreads();
   auto xk = [&](int k) {
     if constexpr (A::LIMBS == 2) return mkdd(Xh[k], Xl[k]);
     else return Xh[k];
   };
+  T q0 = A::zero(), q1 = A::zero();
   if (live) {
     if (bw == BW) {
-      T lv[16];
-      const int kb = part * 16;
-#pragma unroll
-      for (int u = 0; u < 16; ++u) lv[u] = FWD ? A::ld(l, i, b0 + kb + u) : A::ld(l, b0 + kb + u, i);
 #pragma unroll
       for (int u = 0; u < 16; u += 2) {
         q0 = A::add(q0, A::mul(lv[u], xk(kb + u)));
@@ -234,14 +233,7 @@ __global__ __launch_bounds__(256) void k_inv_step(Mat l, const double* ih, const
       for (int k = part; k < bw; k += 4) q0 = A::add(q0, A::mul(FWD ? A::ld(l, i, b0 + k) : A::ld(l, b0 + k, i), xk(k)));
     }
   }
-  T q = A::add(q0, q1);
-#pragma unroll
-  for (int off = 1; off < 4; off <<= 1) {  // lanes 4r..4r+3 of one wavefront hold row r
-    T o;
-    if constexpr (A::LIMBS == 2) o = mkdd(__shfl_xor(q.hi, off, 64), __shfl_xor(q.lo, off, 64));
-    else o = __shfl_xor(q, off, 64);
-    q = A::add(q, o);
-  }
+  const T q = quarter_sum<A>(A::add(q0, q1));
   if (live && part == 0) A::stv(yh, yl, i, A::sub(A::ldv(yh, yl, i), q));
 }
 
