@@ -75,10 +75,6 @@ struct ArF64 {
   __device__ static bool nonpos(T d) { return d <= 0.0; }         // lib.rs:177
   __device__ static double f64(T d) { return d; }
   __device__ static T readlane(T v, int lane) { return __shfl(v, lane, 64); }
-  __device__ static void sqrt_recip_fast(T d, T& root, T& inv) {
-    root = __builtin_sqrt(d);
-    inv = 1.0 / root;
-  }
 };
 
 }  // namespace hp

@@ -65,10 +65,8 @@ __device__ __forceinline__ void lset(Lds2<A>& s, int i, int j, typename A::T v) 
 // through a packed-triangle index table, so every lane has work).  Every element still
 // receives its products in increasing k, one dd add each, and is scaled exactly as in
 // the unblocked algorithm.  The reciprocals stay in LDS until the block is done, so no
-// global store is pending at a barrier.  FAST (split-dd factorisation only) takes each
-// pivot's root and reciprocal by a dd reciprocal square root instead of the reference's
-// sqrt then recip.
-template <class A, bool S_INIT, bool FAST = false>
+// global store is pending at a barrier.
+template <class A, bool S_INIT>
 __global__ __launch_bounds__(512) void k_diag_factor(Mat a, Mat S, long r0, int jb, ErrRec* err, double* inv_hi,
                                                      double* inv_lo) {
   using T = typename A::T;
@@ -111,9 +109,8 @@ __global__ __launch_bounds__(512) void k_diag_factor(Mat a, Mat S, long r0, int 
         fail = 1;
         bad = 1;
       } else {
-        T ljj;
-        if constexpr (FAST) A::sqrt_recip_fast(diag, ljj, inv);
-        else { ljj = A::sqrt(diag); inv = A::recip(ljj); }
+        const T ljj = A::sqrt(diag);
+        inv = A::recip(ljj);
         lset<A>(L, c, c, ljj);
         if constexpr (A::LIMBS == 2) { invs_h[c] = inv.hi; invs_l[c] = inv.lo; }
         else invs_h[c] = inv;
@@ -569,10 +566,6 @@ static int set_lds_attrs() {
   const int one = (int)sizeof(Lds2<A>);
   HP_HIP(hipFuncSetAttribute((const void*)k_diag_factor<A, true>, hipFuncAttributeMaxDynamicSharedMemorySize, big));
   HP_HIP(hipFuncSetAttribute((const void*)k_diag_factor<A, false>, hipFuncAttributeMaxDynamicSharedMemorySize, big));
-  HP_HIP(hipFuncSetAttribute((const void*)k_diag_factor<A, true, true>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                             big));
-  HP_HIP(hipFuncSetAttribute((const void*)k_diag_factor<A, false, true>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                             big));
   if constexpr (A::LIMBS == 2)
     HP_HIP(hipFuncSetAttribute((const void*)k_diag_factor_split, hipFuncAttributeMaxDynamicSharedMemorySize, one));
   HP_HIP(hipFuncSetAttribute((const void*)k_row_recur<A, true>, hipFuncAttributeMaxDynamicSharedMemorySize, one));
@@ -623,7 +616,7 @@ static int row_recur(Mat a, Mat S, long c0, int w, long rb, long re, double* inv
 }
 
 // Unblocked Cholesky (linalg.rs:42-96) of a[r0:r0+len, r0:r0+len], S = partial sums.
-template <class A, bool FAST = false>
+template <class A>
 static int factor_unblocked_region(Mat a, Mat S, long r0, long len, ErrRec* err, double* inv_h, double* inv_l,
                                    hipStream_t s) {
   {
@@ -634,7 +627,7 @@ static int factor_unblocked_region(Mat a, Mat S, long r0, long len, ErrRec* err,
   }
   for (long c0 = r0; c0 < r0 + len; c0 += DB) {
     int w = (int)std::min<long>(DB, r0 + len - c0);
-    hipLaunchKernelGGL((k_diag_factor<A, true, FAST>), dim3(1), dim3(512), 2 * sizeof(Lds2<A>), s, a, S, c0, w, err,
+    hipLaunchKernelGGL((k_diag_factor<A, true>), dim3(1), dim3(512), 2 * sizeof(Lds2<A>), s, a, S, c0, w, err,
                        inv_h, inv_l);
     HP_LAUNCH_CHECK();
     long rb = c0 + w, re = r0 + len;
@@ -647,16 +640,16 @@ static int factor_unblocked_region(Mat a, Mat S, long r0, long len, ErrRec* err,
 
 // One block column of cholesky_blocked (linalg.rs:247-261): the diagonal block
 // [c0, c0+jb) by the unblocked algorithm, then trsm_right_lt on rows [c0+jb, n).
-template <class A, bool FAST>
+template <class A>
 int chol_panel_exact(Mat a, Mat S, long c0, long jb, long n, ErrRec* err, double* inv_h, double* inv_l,
                      hipStream_t s) {
   int rc = ensure_attrs<A>();
   if (rc) return rc;
   if (jb <= DB) {
-    hipLaunchKernelGGL((k_diag_factor<A, false, FAST>), dim3(1), dim3(512), 2 * sizeof(Lds2<A>), s, a, S, c0, (int)jb,
+    hipLaunchKernelGGL((k_diag_factor<A, false>), dim3(1), dim3(512), 2 * sizeof(Lds2<A>), s, a, S, c0, (int)jb,
                        err, inv_h, inv_l);
     HP_LAUNCH_CHECK();
-  } else if ((rc = factor_unblocked_region<A, FAST>(a, S, c0, jb, err, inv_h, inv_l, s))) {
+  } else if ((rc = factor_unblocked_region<A>(a, S, c0, jb, err, inv_h, inv_l, s))) {
     return rc;
   }
   long rem = n - c0 - jb;
@@ -769,7 +762,7 @@ int chol_exact(Mat a, long n, long nb, bool unblocked, Mat S, ErrRec* err, doubl
   if ((rc = la.init())) return rc;
   for (long jb0 = 0; jb0 < n; jb0 += nb) {
     long jb = std::min(nb, n - jb0);
-    if ((rc = chol_panel_exact<A, false>(a, S, jb0, jb, n, err, inv_h, inv_l, s))) return rc;
+    if ((rc = chol_panel_exact<A>(a, S, jb0, jb, n, err, inv_h, inv_l, s))) return rc;
     long rem = n - jb0 - jb;
     if (rem <= 0) continue;
     long t0 = jb0 + jb, w2 = std::min(nb, rem);
@@ -830,9 +823,8 @@ int zero_upper_f64(double* l, long n, hipStream_t s) {
 }
 
 template int chol_exact<ArDD>(Mat, long, long, bool, Mat, ErrRec*, double*, double*, hipStream_t);
-template int chol_panel_exact<ArDD, false>(Mat, Mat, long, long, long, ErrRec*, double*, double*, hipStream_t);
-template int chol_panel_exact<ArDD, true>(Mat, Mat, long, long, long, ErrRec*, double*, double*, hipStream_t);
-template int chol_panel_exact<ArF64, false>(Mat, Mat, long, long, long, ErrRec*, double*, double*, hipStream_t);
+template int chol_panel_exact<ArDD>(Mat, Mat, long, long, long, ErrRec*, double*, double*, hipStream_t);
+template int chol_panel_exact<ArF64>(Mat, Mat, long, long, long, ErrRec*, double*, double*, hipStream_t);
 template int rank_update_sumsub<ArDD>(Mat, Mat, long, long, long, long, long, long, ErrRec*, hipStream_t);
 template int rank_update_sumsub_below<ArDD>(Mat, Mat, long, long, long, long, long, long, ErrRec*, hipStream_t);
 template int chol_exact<ArF64>(Mat, long, long, bool, Mat, ErrRec*, double*, double*, hipStream_t);

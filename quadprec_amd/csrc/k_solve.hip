@@ -24,8 +24,6 @@ namespace hp {
 
 template <class A>
 int chol_exact(Mat a, long n, long nb, bool unblocked, Mat S, ErrRec* err, double* inv_h, double* inv_l, hipStream_t s);
-template <class A, bool FAST>
-int chol_panel_exact(Mat a, Mat S, long c0, long jb, long n, ErrRec* err, double* inv_h, double* inv_l, hipStream_t s);
 template <class A>
 int chol_panel_split(Mat a, Mat S, long c0, long jb, long n, ErrRec* err, double* inv, hipStream_t s,
                      hipStream_t side, hipEvent_t* evq, hipEvent_t evdone);
