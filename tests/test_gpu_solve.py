@@ -70,6 +70,21 @@ def test_refined_solve_vs_cpu_reference(qp, lo, spacing, kinds, bar):
         assert e <= bar, (kind, e)
 
 
+@pytest.mark.parametrize("n", [300, 700])
+def test_refined_solve_ragged_vs_cpu_reference(qp, n):
+    """n not a multiple of 64: every factor's last diagonal block and the block-inverse
+    solves' last step are partial; each factor's refined solve against the oracle's x*."""
+    c, a = qp.gen_overlap_basis(2, n, 0.05, 10.0, 1.4)
+    s = qp.overlap_s(c, a)
+    b = qp.gen_dd_normal(78, n)
+    xs = cpu_reference(s, b, n)
+    for kind in (qp.HP_FACTOR_F64, qp.HP_FACTOR_DD, qp.HP_FACTOR_DD_SPLIT):
+        x, st = qp.solve_spd_refined(s, b, n, kind, tol=1e-30, max_iter=30)
+        e = rel_err(x, xs)
+        print(f"n={n} factor={kind}: iters={st.iterations} err={e:.3e}")
+        assert e <= 1e-28, (kind, e)
+
+
 @pytest.mark.parametrize("n,piv", [(64, 10), (520, 300), (520, 517)])
 def test_refined_solve_split_factor_fails_cleanly_on_indefinite(qp, n, piv):
     """A negative pivot in the first block, in a later panel's first sub-block, and in the
