@@ -168,8 +168,18 @@ int hp_solve_spd_f128(const double* a_dd, const double* b_dd, size_t n, hp_solut
  *   HP_FACTOR_DD        dd cholesky_blocked(nb = 64), reference order on FP64 VALU
  *   HP_FACTOR_DD_SPLIT  dd blocked Cholesky (nb = 256) with the trailing updates on the
  *                       INT8 matrix cores (split GEMM)
- * Iterates r = b - S x (qd), x += solve(L, r) until max|dx| <= tol * max|x| or max_iter. */
+ * Iterates r = b - S x (qd), x += solve(L, r) and stops on the first of:
+ *   - the measured correction max|dx| / max|x| <= tol                 (converged = 1);
+ *   - after two or more corrections, the next one predicted from the last two,
+ *     last * (last / prev), <= tol / 10 while the corrections shrink at least 4x per
+ *     step                                                              (converged = 2);
+ *   - after four or more corrections, a correction above half the smallest so far
+ *     (the reference's stagnation test, lib.rs:102-106)                (converged = 0);
+ *   - max_iter steps                                                   (converged = 0).
+ * The call returns HP_OK in every case, like the reference's iterative_refinement; the
+ * caller reads `converged`. */
 enum hp_factor_kind { HP_FACTOR_F64 = 0, HP_FACTOR_DD = 1, HP_FACTOR_DD_SPLIT = 2 };
+enum hp_converged { HP_NOT_CONVERGED = 0, HP_CONVERGED_MEASURED = 1, HP_CONVERGED_PREDICTED = 2 };
 typedef struct {
   int factor_kind;
   int iterations;
@@ -177,6 +187,7 @@ typedef struct {
   double refine_ms;        /* wall time of the refinement loop */
   double last_correction;  /* max|dx| / max|x| of the last step */
   size_t pivot;            /* failing pivot on HP_ENOTSPD */
+  int converged;           /* hp_converged: which stop rule ended the loop */
 } hp_refine_stats;
 int hp_dev_solve_spd_refined(int factor_kind, const double* s_hi, const double* s_lo, const double* b_hi,
                              const double* b_lo, size_t n, double tol, int max_iter, double* x_hi, double* x_lo,

@@ -43,11 +43,16 @@ class Solution(C.Structure):
 
 
 class RefineStats(C.Structure):
+    """hp_refine_stats; `converged` is HP_CONVERGED_MEASURED (the last measured correction is
+    below tol), HP_CONVERGED_PREDICTED (the next one is predicted ten times below tol) or
+    HP_NOT_CONVERGED (stagnation or max_iter)."""
     _fields_ = [("factor_kind", C.c_int), ("iterations", C.c_int), ("factor_ms", C.c_double),
-                ("refine_ms", C.c_double), ("last_correction", C.c_double), ("pivot", C.c_size_t)]
+                ("refine_ms", C.c_double), ("last_correction", C.c_double), ("pivot", C.c_size_t),
+                ("converged", C.c_int)]
 
 
 HP_FACTOR_F64, HP_FACTOR_DD, HP_FACTOR_DD_SPLIT = 0, 1, 2
+HP_NOT_CONVERGED, HP_CONVERGED_MEASURED, HP_CONVERGED_PREDICTED = 0, 1, 2
 
 
 class AdjointInfo(C.Structure):

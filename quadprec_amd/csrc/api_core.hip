@@ -202,7 +202,7 @@ int hp_get_phase_times(double* ms, int max) {
   for (int i = 0; i < cnt && i < max; ++i) ms[i] = t.ms[i];
   return cnt;
 }
-int hp_version(void) { return 100; }
+int hp_version(void) { return 200; }
 
 int hp_gemm(const double* a, const double* b, double* c, size_t m, size_t n, size_t k) {
   return gemm_host(0, 2, a, b, c, m, n, k);

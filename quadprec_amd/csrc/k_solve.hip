@@ -108,12 +108,8 @@ __device__ __forceinline__ typename A::T Vg(double (*Vh)[BW + 1], double (*Vl)[B
 }
 
 // ---------------------------------------------------------------- blocked inverse solves
-// Correction solves with precomputed inverses of the 64 x 64 diagonal blocks of L: each
-// block step is one launch in which every workgroup forms x_b = inv(L_bb) y_b (a 64-long
-// dot per lane, no sequential chain) and then subtracts the block's contributions from its
-// own rows.  Both 64-term sums run as four interleaved dd chains (the order is free here:
-// these solves only feed the refinement).  x is written to a separate vector so no workgroup reads a value another one
-// is overwriting.
+// The correction solves use precomputed inverses of the 64 x 64 diagonal blocks of L, so
+// no block step has a sequential substitution chain (k_inv_solve below).
 template <class A>
 __global__ __launch_bounds__(64) void k_diag_block_inverse(Mat l, long n, double* ih, double* il) {
   // inv(L_bb) for block b = blockIdx.x, lane c computes column c by forward substitution
@@ -160,12 +156,7 @@ __global__ __launch_bounds__(64) void k_diag_block_inverse(Mat l, long n, double
   }
 }
 
-// one block step of L y = b (FWD) or L^T x = y (!FWD); rows handled: FWD rows >= b0+bw,
-// !FWD rows < b0.  x_b goes to xo; y is updated in place for the other rows.
-// Both products run on all four wavefronts with the same shape: 16 rows per wavefront,
-// four lanes per row, each lane 16 of the 64 terms (read straight from memory, the
-// inverse and the factor alike), the quarters combined with lane shuffles.  x_b passes
-// to the update through LDS.  All loads are issued before the first product.
+// the sum over the four lanes 4r..4r+3 that share a row (each holds 16 of its 64 terms)
 template <class A>
 __device__ __forceinline__ typename A::T quarter_sum(typename A::T q) {  // lanes 4r..4r+3
 #pragma unroll
@@ -177,62 +168,175 @@ __device__ __forceinline__ typename A::T quarter_sum(typename A::T q) {  // lane
   }
   return q;
 }
-template <class A, bool FWD>
-__global__ __launch_bounds__(256) void k_inv_step(Mat l, const double* ih, const double* il, double* yh, double* yl,
-                                                  double* xoh, double* xol, long n, long b0, int bw) {
+
+// ---------------------------------------------------------------- persistent block solves
+// L L^T x = y in ONE launch of 2 nblk workgroups (256 threads), with the inverses of the
+// 64 x 64 diagonal blocks:
+//   workgroup b < nblk:        z_b = inv(L_bb)   (y_b - sum_{j<b} L_bj z_j)
+//   workgroup nblk + c:        x_b = inv(L_bb)^T (z_b - sum_{j>b} L_jb^T x_j),  b = nblk-1-c
+// Each block publishes its result with a flag (agent-scope release after its stores); a
+// consumer waits for a flag, then takes every block whose flag is already set, so a block
+// far down the chain keeps up with the front.  Every wait is on a lower-numbered
+// workgroup: under in-order dispatch the smallest-numbered resident workgroup never waits
+// on one that is not running, so the chain always advances, whatever the residency.  Each
+// wait is also bounded (flag_wait): past the limit it raises *fail and the workgroups
+// leave (the host reports a device error).  Summation order: four interleaved chains per
+// row, combined by lane shuffles (these solves only feed the refinement).
+constexpr int SPIN_LIMIT = 1 << 22;
+
+__device__ __forceinline__ int ld_acquire(const int* f) {
+  return __hip_atomic_load(f, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ bool flag_wait(const int* f, int epoch, int* fail) {
+  for (int spins = 0;; ++spins) {
+    if (ld_acquire(f) >= epoch) return true;
+    if ((spins & 255) == 255 && __hip_atomic_load(fail, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) return false;
+    if (spins > SPIN_LIMIT) {
+      __hip_atomic_store(fail, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      return false;
+    }
+    __builtin_amdgcn_s_sleep(1);
+  }
+}
+
+template <class A>
+__global__ __launch_bounds__(256) void k_inv_solve(Mat l, const double* __restrict__ ih, const double* __restrict__ il,
+                                                   const double* yh, const double* yl, double* zh, double* zl,
+                                                   double* xh, double* xl, long n, int nblk, int* flags, int epoch,
+                                                   int* fail, unsigned long long* red) {
   using T = typename A::T;
-  __shared__ double Xh[BW], Xl[BW];
-  const int t = threadIdx.x, part = t & 3, kb = part * 16;
-  const int r = t >> 2;  // 0..63: row of x_b, and row of this workgroup's update
-  const size_t base = (size_t)(b0 / BW) * BW * BW;
-  long i = (long)blockIdx.x * 64 + r;
-  if (FWD) i += b0 + bw;
-  const bool live = FWD ? (i < n) : (i < b0);
-  // x_r = sum_k inv[r][k] y_k (FWD) or sum_k inv[k][r] y_k (transposed); entries of the
-  // inverse outside the block are stored as zero
-  T iv[16], yv[16], lv[16];
+  __shared__ double Th[BW], Tl[BW];
+  __shared__ int s_ready[2], s_ok[2], s_zok;
+  const int t = threadIdx.x, part = t & 3, kb = part * 16, r = t >> 2;
+  const bool fwd = (int)blockIdx.x < nblk;
+  const int b = fwd ? (int)blockIdx.x : 2 * nblk - 1 - (int)blockIdx.x;
+  const long b0 = (long)b * BW;
+  const int bw = (int)min((long)BW, n - b0);
+  const bool live = r < bw;
+  if (blockIdx.x == 0 && t < 2) red[t] = 0;  // the max-reductions of the kernel that follows
+  // this lane's 16 terms of the inverse (fwd: row r, bwd: column r), loaded ahead of any wait
+  const size_t base = (size_t)b * BW * BW;
+  T iv[16];
 #pragma unroll
   for (int u = 0; u < 16; ++u) {
     const int k = kb + u;
-    const size_t e = base + (FWD ? (size_t)r * BW + k : (size_t)k * BW + r);
-    iv[u] = A::ldv(ih, il, (long)e);
-    yv[u] = k < bw ? A::ldv(yh, yl, b0 + k) : A::zero();
+    iv[u] = A::ldv(ih, il, (long)(base + (fwd ? (size_t)r * BW + k : (size_t)k * BW + r)));
   }
-  if (live && bw == BW) {
-#pragma unroll
-    for (int u = 0; u < 16; ++u) lv[u] = FWD ? A::ld(l, i, b0 + kb + u) : A::ld(l, b0 + kb + u, i);
-  }
+  const int* dep = flags + (fwd ? 0 : nblk);
   T p0 = A::zero(), p1 = A::zero();
+  int next = fwd ? 0 : nblk - 1;  // next dependency to take: fwd ascending below b, bwd descending above b
+  bool ok = true;
+  for (int round = 0; fwd ? next < b : next > b; ++round) {
+    if (t == 0) {
+      int k = next;
+      bool w = flag_wait(dep + k, epoch, fail);
+      if (w) {
+        if (fwd) { ++k; while (k < b && ld_acquire(dep + k) >= epoch) ++k; }
+        else { --k; while (k > b && ld_acquire(dep + k) >= epoch) --k; }
+      }
+      s_ready[round & 1] = k;
+      s_ok[round & 1] = w;
+    }
+    __syncthreads();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+    const int ready = s_ready[round & 1];
+    if (!s_ok[round & 1]) { ok = false; break; }
+    if (fwd) {
+      for (int j = next; j < ready; ++j) {  // full blocks (j < b)
+        T lv[16], zv[16];
 #pragma unroll
-  for (int u = 0; u < 16; u += 2) {
-    p0 = A::add(p0, A::mul(iv[u], yv[u]));
-    p1 = A::add(p1, A::mul(iv[u + 1], yv[u + 1]));
-  }
-  const T xr = quarter_sum<A>(A::add(p0, p1));
-  if (part == 0 && r < bw) {
-    if constexpr (A::LIMBS == 2) { Xh[r] = xr.hi; Xl[r] = xr.lo; }
-    else Xh[r] = xr;
-    if (blockIdx.x == 0) A::stv(xoh, xol, b0 + r, xr);
-  }
-  __syncthreads();
-  auto xk = [&](int k) {
-    if constexpr (A::LIMBS == 2) return mkdd(Xh[k], Xl[k]);
-    else return Xh[k];
-  };
-  T q0 = A::zero(), q1 = A::zero();
-  if (live) {
-    if (bw == BW) {
+        for (int u = 0; u < 16; ++u) {
+          zv[u] = A::ldv(zh, zl, (long)j * BW + kb + u);
+          lv[u] = live ? A::ld(l, b0 + r, (long)j * BW + kb + u) : A::zero();
+        }
 #pragma unroll
-      for (int u = 0; u < 16; u += 2) {
-        q0 = A::add(q0, A::mul(lv[u], xk(kb + u)));
-        q1 = A::add(q1, A::mul(lv[u + 1], xk(kb + u + 1)));
+        for (int u = 0; u < 16; u += 2) {
+          p0 = A::add(p0, A::mul(lv[u], zv[u]));
+          p1 = A::add(p1, A::mul(lv[u + 1], zv[u + 1]));
+        }
       }
     } else {
-      for (int k = part; k < bw; k += 4) q0 = A::add(q0, A::mul(FWD ? A::ld(l, i, b0 + k) : A::ld(l, b0 + k, i), xk(k)));
+      for (int j = next; j > ready; --j) {
+        const long j0 = (long)j * BW;
+        const int bwj = (int)min((long)BW, n - j0);
+        T lv[16], xv[16];
+#pragma unroll
+        for (int u = 0; u < 16; ++u) {
+          const int k = kb + u;
+          xv[u] = k < bwj ? A::ldv(xh, xl, j0 + k) : A::zero();
+          lv[u] = (live && k < bwj) ? A::ld(l, j0 + k, b0 + r) : A::zero();
+        }
+#pragma unroll
+        for (int u = 0; u < 16; u += 2) {
+          p0 = A::add(p0, A::mul(lv[u], xv[u]));
+          p1 = A::add(p1, A::mul(lv[u + 1], xv[u + 1]));
+        }
+      }
     }
+    next = ready;
   }
-  const T q = quarter_sum<A>(A::add(q0, q1));
-  if (live && part == 0) A::stv(yh, yl, i, A::sub(A::ldv(yh, yl, i), q));
+  if (!ok) return;
+  if (!fwd) {  // z_b of the forward pass
+    if (t == 0) s_zok = flag_wait(flags + b, epoch, fail);
+    __syncthreads();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+    if (!s_zok) return;
+  }
+  const T sum = quarter_sum<A>(A::add(p0, p1));
+  if (part == 0) {
+    T v = A::zero();
+    if (live) v = A::sub(fwd ? A::ldv(yh, yl, b0 + r) : A::ldv(zh, zl, b0 + r), sum);
+    if constexpr (A::LIMBS == 2) { Th[r] = v.hi; Tl[r] = v.lo; }
+    else Th[r] = v;
+  }
+  __syncthreads();
+  T q0 = A::zero(), q1 = A::zero();
+#pragma unroll
+  for (int u = 0; u < 16; u += 2) {
+    T t0, t1;
+    if constexpr (A::LIMBS == 2) { t0 = mkdd(Th[kb + u], Tl[kb + u]); t1 = mkdd(Th[kb + u + 1], Tl[kb + u + 1]); }
+    else { t0 = Th[kb + u]; t1 = Th[kb + u + 1]; }
+    q0 = A::add(q0, A::mul(iv[u], t0));
+    q1 = A::add(q1, A::mul(iv[u + 1], t1));
+  }
+  const T o = quarter_sum<A>(A::add(q0, q1));
+  if (part == 0 && live) {
+    if (fwd) A::stv(zh, zl, b0 + r, o);
+    else A::stv(xh, xl, b0 + r, o);
+  }
+  __threadfence();
+  __syncthreads();
+  if (t == 0) __hip_atomic_store(flags + (fwd ? 0 : nblk) + b, epoch, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// x += dx (dd += dd, or dd += f64 when dl is null), and red[0] = max|dx.hi|, red[1] =
+// max|x.hi| after the update (bit patterns of non-negative doubles order as integers);
+// red is cleared by the solve kernel that precedes this one on the stream
+__global__ __launch_bounds__(256) void k_add_absmax(double* xh, double* xl, const double* dh, const double* dl, long n,
+                                                    unsigned long long* red) {
+  __shared__ double m0[256], m1[256];
+  double a0 = 0.0, a1 = 0.0;
+  for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
+    dd r = dl ? dd_add(mkdd(xh[i], xl[i]), mkdd(dh[i], dl[i])) : dd_add_f(mkdd(xh[i], xl[i]), dh[i]);
+    xh[i] = r.hi;
+    xl[i] = r.lo;
+    a0 = fmax(a0, fabs(dh[i]));
+    a1 = fmax(a1, fabs(r.hi));
+  }
+  m0[threadIdx.x] = a0;
+  m1[threadIdx.x] = a1;
+  __syncthreads();
+  for (int s = 128; s > 0; s >>= 1) {
+    if ((int)threadIdx.x < s) {
+      m0[threadIdx.x] = fmax(m0[threadIdx.x], m0[threadIdx.x + s]);
+      m1[threadIdx.x] = fmax(m1[threadIdx.x], m1[threadIdx.x + s]);
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    atomicMax(red, __builtin_bit_cast(unsigned long long, m0[0]));
+    atomicMax(red + 1, __builtin_bit_cast(unsigned long long, m1[0]));
+  }
 }
 
 // ---------------------------------------------------------------- qd residual
@@ -289,36 +393,6 @@ __global__ void k_dd_to_f64(const double* h, const double* l, double* o, long n)
   long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) o[i] = h[i] + l[i];
 }
-__global__ void k_dd_add_f64(double* xh, double* xl, const double* d, long n) {  // x += d (dd += f64)
-  long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n) {
-    dd r = dd_add_f(mkdd(xh[i], xl[i]), d[i]);
-    xh[i] = r.hi;
-    xl[i] = r.lo;
-  }
-}
-__global__ void k_dd_add(double* xh, double* xl, const double* dh, const double* dl, long n) {
-  long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n) {
-    dd r = dd_add(mkdd(xh[i], xl[i]), mkdd(dh[i], dl[i]));
-    xh[i] = r.hi;
-    xl[i] = r.lo;
-  }
-}
-// max |v| over n doubles into out (as ordered bit patterns of non-negative doubles)
-__global__ void k_absmax(const double* v, long n, unsigned long long* out) {
-  __shared__ double red[256];
-  double m = 0.0;
-  for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x)
-    m = fmax(m, fabs(v[i]));
-  red[threadIdx.x] = m;
-  __syncthreads();
-  for (int s = 128; s > 0; s >>= 1) {
-    if (threadIdx.x < s) red[threadIdx.x] = fmax(red[threadIdx.x], red[threadIdx.x + s]);
-    __syncthreads();
-  }
-  if (threadIdx.x == 0) atomicMax(out, __builtin_bit_cast(unsigned long long, red[0]));
-}
 // a[r0+i][r0+j] -= c[i][j] (j <= i) -- the trailing update of the split factorisation
 __global__ void k_sub_lower(double* ah, double* al, long lda, long r0, const double* ch, const double* cl, long m) {
   long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -357,45 +431,17 @@ int diag_inverses(Mat l, long n, double* ih, double* il, hipStream_t s) {
   HP_LAUNCH_CHECK();
   return HP_OK;
 }
-// L L^T x = b with the block inverses; y (scratch, n) holds b on entry and is consumed
+// L L^T x = y through the persistent kernel; z (n) is scratch, flags (2 nblk ints, zero
+// before the first call) and fail belong to the caller, epoch grows by one per call
 template <class A>
-int solve_inv(Mat l, const double* ih, const double* il, double* yh, double* yl, double* xh, double* xl, long n,
-              hipStream_t s) {
-  long nblk = (n + BW - 1) / BW;
-  for (long b = 0; b < nblk; ++b) {  // forward: z = L^-1 y into x
-    long b0 = b * BW;
-    int bw = (int)std::min<long>(BW, n - b0);
-    long rows = n - b0 - bw;
-    unsigned g = (unsigned)std::max<long>(1, (rows + 63) / 64);
-    hipLaunchKernelGGL((k_inv_step<A, true>), dim3(g), dim3(256), 0, s, l, ih, il, yh, yl, xh, xl, n, b0, bw);
-    HP_LAUNCH_CHECK();
-  }
-  for (long b = nblk - 1; b >= 0; --b) {  // backward: x = L^-T z, z in x (copied to y as the working vector)
-    long b0 = b * BW;
-    int bw = (int)std::min<long>(BW, n - b0);
-    if (b == nblk - 1) {
-      HP_HIP(hipMemcpyAsync(yh, xh, n * sizeof(double), hipMemcpyDeviceToDevice, s));
-      if (A::LIMBS == 2) HP_HIP(hipMemcpyAsync(yl, xl, n * sizeof(double), hipMemcpyDeviceToDevice, s));
-    }
-    unsigned g = (unsigned)std::max<long>(1, (b0 + 63) / 64);
-    hipLaunchKernelGGL((k_inv_step<A, false>), dim3(g), dim3(256), 0, s, l, ih, il, yh, yl, xh, xl, n, b0, bw);
-    HP_LAUNCH_CHECK();
-  }
+int solve_inv_persistent(Mat l, const double* ih, const double* il, const double* yh, const double* yl, double* zh,
+                         double* zl, double* xh, double* xl, long n, int* flags, int epoch, int* fail,
+                         unsigned long long* red, hipStream_t s) {
+  const int nblk = (int)((n + BW - 1) / BW);
+  hipLaunchKernelGGL(k_inv_solve<A>, dim3((unsigned)(2 * nblk)), dim3(256), 0, s, l, ih, il, yh, yl, zh, zl, xh, xl, n,
+                     nblk, flags, epoch, fail, red);
+  HP_LAUNCH_CHECK();
   return HP_OK;
-}
-
-double absmax_dev(const double* v, long n, unsigned long long* scratch, hipStream_t s, int* rc) {
-  *rc = HP_OK;
-  if (hipMemsetAsync(scratch, 0, sizeof(unsigned long long), s) != hipSuccess) { *rc = HP_EDEVICE; return 0; }
-  hipLaunchKernelGGL(k_absmax, dim3(std::min<unsigned>(g1(n), 1024)), dim3(256), 0, s, v, n, scratch);
-  unsigned long long h = 0;
-  if (hipMemcpyAsync(&h, scratch, sizeof(h), hipMemcpyDeviceToHost, s) != hipSuccess ||
-      hipStreamSynchronize(s) != hipSuccess) {
-    *rc = HP_EDEVICE;
-    set_error("absmax: HIP error");
-    return 0;
-  }
-  return __builtin_bit_cast(double, h);
 }
 
 }  // namespace
@@ -493,61 +539,80 @@ int g_split_panel = 256;  // panel width of the split-dd factorisation
 
 // x = S^-1 b by corrections through a given factor L (f64 or dd by kind) and the inverses of
 // its 64 x 64 diagonal blocks: r = b - S x in quad-double, x += L^-T L^-1 r, until the
-// correction's max-norm falls to tol relative to x, or the next one is predicted to be
-// ten times below tol (or max_iter steps).
+// correction's max-norm falls to tol relative to x (converged = 1), or the next one,
+// predicted from the last two by the geometric contraction, is ten times below tol while
+// the corrections shrink at least 4x per step (converged = 2).  Stagnation (after four
+// corrections, one above half the smallest so far, as lib.rs:102-106) or max_iter steps
+// end the loop unconverged (converged = 0).
 int refine_loop(int kind, Mat Lm, const double* ivh, const double* ivl, const double* sh, const double* sl,
                 const double* bh, const double* bl, long n, double tol, int max_iter, double* xh, double* xl,
-                int* iters, double* last_out, hipStream_t s) {
+                int* iters, double* last_out, int* converged, hipStream_t s) {
   int rc;
+  const long nblk = (n + BW - 1) / BW;
   DevBuf vec, sc;
-  if ((rc = vec.alloc(8 * n * sizeof(double))) || (rc = sc.alloc(64))) return rc;
+  if ((rc = vec.alloc(6 * n * sizeof(double))) || (rc = sc.alloc(64 + (2 * nblk + 1) * sizeof(int)))) return rc;
   double* rh = vec.d();
   double* rlo = rh + n;
-  double* dh = rh + 2 * n;
+  double* dh = rh + 2 * n;  // the correction (dd, or f64 in dh for the f64 factor)
   double* dl = rh + 3 * n;
-  double* d64 = rh + 4 * n;
-  double* w64 = rh + 5 * n;
-  double* wh = rh + 6 * n;
-  double* wl = rh + 7 * n;
-  unsigned long long* scr = (unsigned long long*)sc.p;
+  double* zh = rh + 4 * n;  // the forward solve's result
+  double* zl = rh + 5 * n;
+  unsigned long long* red = (unsigned long long*)sc.p;  // max|dx|, max|x|
+  int* fail = (int*)((char*)sc.p + 16);
+  int* flags = (int*)((char*)sc.p + 64);
+  HP_HIP(hipMemsetAsync(sc.p, 0, 64 + (2 * nblk + 1) * sizeof(int), s));
   HP_HIP(hipMemsetAsync(xh, 0, n * sizeof(double), s));
   HP_HIP(hipMemsetAsync(xl, 0, n * sizeof(double), s));
-  double last = 1.0;
+  double last = 1.0, best = HUGE_VAL;
   int it = 0;
+  *converged = HP_NOT_CONVERGED;
   for (; it < max_iter; ++it) {
-    if (it == 0) {
-      HP_HIP(hipMemcpyAsync(rh, bh, n * sizeof(double), hipMemcpyDeviceToDevice, s));
-      HP_HIP(hipMemcpyAsync(rlo, bl, n * sizeof(double), hipMemcpyDeviceToDevice, s));
-    } else {
+    const double* yh_ = bh;  // the right-hand side of this correction: b, then r = b - S x
+    const double* yl_ = bl;
+    if (it > 0) {
       hipLaunchKernelGGL(k_resid_qd, dim3((unsigned)((n + 3) / 4)), dim3(256), 0, s, sh, sl, xh, xl, bh, bl, rh, rlo, n);
       HP_LAUNCH_CHECK();
+      yh_ = rh;
+      yl_ = rlo;
     }
-    double dn;
     if (kind == HP_FACTOR_F64) {
-      hipLaunchKernelGGL(k_dd_to_f64, dim3(g1(n)), dim3(256), 0, s, rh, rlo, d64, n);
+      hipLaunchKernelGGL(k_dd_to_f64, dim3(g1(n)), dim3(256), 0, s, yh_, yl_, dh, n);  // f64 right-hand side
       HP_LAUNCH_CHECK();
-      HP_HIP(hipMemcpyAsync(w64, d64, n * sizeof(double), hipMemcpyDeviceToDevice, s));
-      if ((rc = solve_inv<ArF64>(Lm, ivh, nullptr, w64, nullptr, d64, nullptr, n, s))) return rc;
-      hipLaunchKernelGGL(k_dd_add_f64, dim3(g1(n)), dim3(256), 0, s, xh, xl, d64, n);
-      HP_LAUNCH_CHECK();
-      dn = absmax_dev(d64, n, scr, s, &rc);
+      if ((rc = solve_inv_persistent<ArF64>(Lm, ivh, nullptr, dh, nullptr, zh, nullptr, dl, nullptr, n, flags,
+                                            it + 1, fail, red, s)))
+        return rc;
+      hipLaunchKernelGGL(k_add_absmax, dim3(std::min<unsigned>(g1(n), 1024)), dim3(256), 0, s, xh, xl, dl, nullptr, n,
+                         red);
     } else {
-      HP_HIP(hipMemcpyAsync(wh, rh, n * sizeof(double), hipMemcpyDeviceToDevice, s));
-      HP_HIP(hipMemcpyAsync(wl, rlo, n * sizeof(double), hipMemcpyDeviceToDevice, s));
-      if ((rc = solve_inv<ArDD>(Lm, ivh, ivl, wh, wl, dh, dl, n, s))) return rc;
-      hipLaunchKernelGGL(k_dd_add, dim3(g1(n)), dim3(256), 0, s, xh, xl, dh, dl, n);
-      HP_LAUNCH_CHECK();
-      dn = absmax_dev(dh, n, scr, s, &rc);
+      if ((rc = solve_inv_persistent<ArDD>(Lm, ivh, ivl, yh_, yl_, zh, zl, dh, dl, n, flags, it + 1, fail, red, s)))
+        return rc;
+      hipLaunchKernelGGL(k_add_absmax, dim3(std::min<unsigned>(g1(n), 1024)), dim3(256), 0, s, xh, xl, dh, dl, n, red);
     }
-    if (rc) return rc;
-    double xn = absmax_dev(xh, n, scr, s, &rc);
-    if (rc) return rc;
+    HP_LAUNCH_CHECK();
+    struct { unsigned long long m[2]; int fail; } h{};
+    HP_HIP(hipMemcpyAsync(&h, sc.p, sizeof(h), hipMemcpyDeviceToHost, s));
+    HP_HIP(hipStreamSynchronize(s));
+    if (h.fail) {
+      set_error("refinement: a block solve's wait exceeded its bound");
+      return HP_EDEVICE;
+    }
+    const double dn = __builtin_bit_cast(double, h.m[0]), xn = __builtin_bit_cast(double, h.m[1]);
     const double prev = last;
     last = xn > 0 ? dn / xn : dn;
-    if (last <= tol) { ++it; break; }
+    if (last <= tol) {
+      *converged = HP_CONVERGED_MEASURED;
+      ++it;
+      break;
+    }
     // the corrections shrink geometrically by the factor's contraction rho ~ kappa u_f:
     // stop once the predicted next one, last * (last / prev), is ten times below tol
-    if (it >= 1 && prev > 0 && last * (last / prev) <= 0.1 * tol) { ++it; break; }
+    if (it >= 1 && prev > 0 && last <= 0.25 * prev && last * (last / prev) <= 0.1 * tol) {
+      *converged = HP_CONVERGED_PREDICTED;
+      ++it;
+      break;
+    }
+    if (it >= 4 && last > 0.5 * best) { ++it; break; }  // stagnation
+    best = std::min(best, last);
   }
   *iters = it;
   *last_out = last;
@@ -564,7 +629,8 @@ int solve_with_dd_factor(Mat Lm, const double* sh, const double* sl, const doubl
   double* ivh = binv.d();
   double* ivl = binv.d() + (size_t)nblk * BW * BW;
   if ((rc = diag_inverses<ArDD>(Lm, n, ivh, ivl, s))) return rc;
-  rc = refine_loop(HP_FACTOR_DD, Lm, ivh, ivl, sh, sl, bh, bl, n, tol, max_iter, xh, xl, iters, last, s);
+  int conv = 0;
+  rc = refine_loop(HP_FACTOR_DD, Lm, ivh, ivl, sh, sl, bh, bl, n, tol, max_iter, xh, xl, iters, last, &conv, s);
   if (rc == HP_OK) HP_HIP(hipStreamSynchronize(s));  // binv is freed on return
   return rc;
 }
@@ -622,8 +688,11 @@ int solve_refined(int kind, const double* sh, const double* sl, const double* bh
   }
   int it = 0;
   double last = 1.0;
-  if ((rc = refine_loop(kind, Lm, ivh, ivl, sh, sl, bh, bl, n, tol, max_iter, xh, xl, &it, &last, s))) return rc;
+  int conv = 0;
+  if ((rc = refine_loop(kind, Lm, ivh, ivl, sh, sl, bh, bl, n, tol, max_iter, xh, xl, &it, &last, &conv, s)))
+    return rc;
   auto t2 = clk::now();
+  st->converged = conv;
   st->iterations = it;
   st->last_correction = last;
   st->refine_ms = std::chrono::duration<double, std::milli>(t2 - t1).count();
