@@ -75,14 +75,47 @@ int hp_mf_gemm_atb(const double* a, const double* b, double* c, size_t m, size_t
  *                     coprime moduli <= 256 and rebuilt by the Chinese remainder theorem.
  *                     Error <= 2^(1-W) (max_p|A_ip| sum_p|B_pj| + sum_p|A_ip| max_p|B_pj|)
  *                     + 2^-106 |C_ij|.  slices in [8, 32]; 0 selects 31.
- * The split modes keep a device workspace between calls (~3*slices*(m*k + k*n + m*n)/3
- * bytes) and synchronise `stream` before returning. */
+ *                     The bound is normwise per row of A and column of B, NOT
+ *                     componentwise: an entry far below that norm (graded or cancelling
+ *                     data) keeps only ~W bits of the norm, where the plain kernel keeps
+ *                     ~2^-102 k of (|A||B|)_ij.  It is not a componentwise drop-in for
+ *                     gemm on such data.  A row of A or a column of B holding an inf or a
+ *                     NaN yields NaN in C's row / column, as the reference's dd arithmetic
+ *                     does; results beyond the binary64 range may be inf where the
+ *                     reference gives NaN.
+ *   HP_GEMM_AUTO      `slices` is the accuracy asked for, in bits (0 selects 104): the
+ *                     result's error is at most 2^-bits of the split bound's norm.  Picks
+ *                     the fewest moduli whose W >= bits + 1 (hp_split_slices_for_bits) and,
+ *                     when the plain kernel's bound (k+1) 2^-102 (|A||B|)_ij meets bits too,
+ *                     the faster of the two, timed once per shape class on the first call's
+ *                     operands (hp_gemm_plan_get reports the choice).  HP_EINVAL above 112.
+ * The split modes keep a device workspace between calls, grown as needed
+ * (~slices*(m*k + k*n + m*n) bytes).  Calls on one stream are ordered by the stream; a call
+ * on another stream first waits (on the device) for the previous call's completion event.
+ * The call returns once its kernels are enqueued, except when the workspace must grow
+ * while a previous call is still running (it then waits for that call). */
 enum hp_gemm_mode {
   HP_GEMM_PLAIN = 0,
-  HP_GEMM_SPLIT_I8 = 2
+  HP_GEMM_SPLIT_I8 = 2,
+  HP_GEMM_AUTO = 3
 };
 int hp_dev_gemm(int trans_a, const double* a_hi, const double* a_lo, const double* b_hi, const double* b_lo,
                 double* c_hi, double* c_lo, size_t m, size_t n, size_t k, int mode, int slices, void* stream);
+/* The fewest split moduli whose integer width W reaches bits + 1 at inner dimension k
+ * (0 if none does).  No counterpart in the reference: the split form is this library's. */
+int hp_split_slices_for_bits(size_t k, int bits);
+/* The choice HP_GEMM_AUTO made for a shape class and accuracy: returns 1 and fills `plan`,
+ * or 0 when that class has not been tuned yet.  ms_* are the timings it chose on (0 when
+ * only one form met the accuracy). */
+typedef struct {
+  int mode;        /* HP_GEMM_PLAIN or HP_GEMM_SPLIT_I8 */
+  int slices;      /* moduli of the split form (0: none meets the accuracy) */
+  int w_bits;      /* its integer width W */
+  int variant;     /* its int8 kernel (hp_split_gemm_variant) */
+  float ms_plain;  /* event-timed run of each candidate */
+  float ms_split;
+} hp_gemm_plan;
+int hp_gemm_plan_get(int trans_a, size_t m, size_t n, size_t k, int bits, hp_gemm_plan* plan);
 
 /* Device MF4 GEMM on limb-major planes (see above). */
 int hp_dev_mf_gemm(int trans_a, const double* a, const double* b, double* c, size_t m, size_t n, size_t k,

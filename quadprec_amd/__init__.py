@@ -24,6 +24,7 @@ LIB_PATH = os.path.join(_HERE, "libhyperprec_hip.so")
 HP_OK, HP_ESHAPE, HP_ENOTSPD, HP_EDEVICE, HP_EINVAL, HP_ENOMEM, HP_ENOCONV = range(7)
 HP_GEMM_PLAIN = 0
 HP_GEMM_SPLIT_I8 = 2
+HP_GEMM_AUTO = 3     # slices = accuracy in bits (0: 104); see gemm_plan
 
 _lib = None
 _dp = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
@@ -53,6 +54,12 @@ class RefineStats(C.Structure):
 
 HP_FACTOR_F64, HP_FACTOR_DD, HP_FACTOR_DD_SPLIT = 0, 1, 2
 HP_NOT_CONVERGED, HP_CONVERGED_MEASURED, HP_CONVERGED_PREDICTED = 0, 1, 2
+
+
+class GemmPlan(C.Structure):
+    """hp_gemm_plan: the form HP_GEMM_AUTO chose for a shape class and accuracy."""
+    _fields_ = [("mode", C.c_int), ("slices", C.c_int), ("w_bits", C.c_int), ("variant", C.c_int),
+                ("ms_plain", C.c_float), ("ms_split", C.c_float)]
 
 
 class AdjointInfo(C.Structure):
@@ -174,6 +181,10 @@ def lib():
     L.hp_split_gemm_variant.restype = C.c_int
     L.hp_split_gemm_set_variant.argtypes = [_sz, _sz, _sz, C.c_int, C.c_int]
     L.hp_split_gemm_set_variant.restype = C.c_int
+    L.hp_split_slices_for_bits.argtypes = [_sz, C.c_int]
+    L.hp_split_slices_for_bits.restype = C.c_int
+    L.hp_gemm_plan_get.argtypes = [C.c_int, _sz, _sz, _sz, C.c_int, C.POINTER(GemmPlan)]
+    L.hp_gemm_plan_get.restype = C.c_int
     L.hp_set_phase_timing.argtypes = [C.c_int]
     L.hp_set_phase_timing.restype = None
     L.hp_get_phase_times.argtypes = [_dp, C.c_int]
@@ -666,6 +677,17 @@ def set_phase_timing(on):
 def split_gemm_variant(m, n, k, slices=0):
     """Int8 kernel the split GEMM autotuner chose for this shape class (0 = untuned)."""
     return lib().hp_split_gemm_variant(m, n, k, slices)
+
+
+def split_slices_for_bits(k, bits):
+    """The fewest split moduli whose integer width reaches bits + 1 at inner dimension k."""
+    return lib().hp_split_slices_for_bits(k, bits)
+
+
+def gemm_plan(m, n, k, bits=0, trans_a=False):
+    """The GemmPlan HP_GEMM_AUTO chose for this shape class and accuracy, or None if untuned."""
+    p = GemmPlan()
+    return p if lib().hp_gemm_plan_get(int(trans_a), m, n, k, bits, C.byref(p)) else None
 
 
 def split_gemm_set_variant(m, n, k, variant, slices=0):

@@ -179,6 +179,14 @@ extern "C" {
 const char* hp_last_error(void) { return hp::g_err; }
 
 int hp_split_gemm_variant(size_t m, size_t n, size_t k, int slices) { return split_gemm_variant(m, n, k, slices); }
+int hp_split_slices_for_bits(size_t k, int bits) { return split_slices_for_bits(bits, k); }
+int hp_gemm_plan_get(int trans_a, size_t m, size_t n, size_t k, int bits, hp_gemm_plan* plan) {
+  if (!plan) {
+    set_error("hp_gemm_plan_get: null plan");
+    return HP_EINVAL;
+  }
+  return gemm_plan_get(trans_a, m, n, k, bits, plan);
+}
 int hp_split_gemm_set_variant(size_t m, size_t n, size_t k, int slices, int variant) {
   return set_split_gemm_variant(m, n, k, slices, variant);
 }
@@ -225,6 +233,8 @@ int hp_dev_gemm(int trans_a, const double* a_hi, const double* a_lo, const doubl
       return launch_gemm_dd_plain(trans_a, a_hi, a_lo, b_hi, b_lo, c_hi, c_lo, m, n, k, s);
     case HP_GEMM_SPLIT_I8:
       return launch_gemm_split_i8(trans_a, a_hi, a_lo, b_hi, b_lo, c_hi, c_lo, m, n, k, slices, s);
+    case HP_GEMM_AUTO:
+      return gemm_auto(trans_a, a_hi, a_lo, b_hi, b_lo, c_hi, c_lo, m, n, k, slices, s);
     default:
       set_error("hp_dev_gemm: unknown mode %d", mode);
       return HP_EINVAL;

@@ -108,6 +108,11 @@ struct SplitOpts {
 int split_gemm_general(const SplitOperand& A, const SplitOperand& B, double* chi, double* clo, size_t ldc, size_t m,
                        size_t n, size_t k, int R, hipStream_t s, const SplitOpts& opt);
 int crt_bits_W(int R, size_t k);
+int split_slices_for_bits(int bits, size_t k);
+int plain_bits_for_k(size_t k);
+int gemm_plan_get(int trans_a, size_t m, size_t n, size_t k, int bits, hp_gemm_plan* plan);
+int gemm_auto(int trans_a, const double* ahi, const double* alo, const double* bhi, const double* blo, double* chi,
+              double* clo, size_t m, size_t n, size_t k, int bits, hipStream_t s);
 extern int g_split_panel;
 int solve_refined(int kind, const double* sh, const double* sl, const double* bh, const double* bl, long n, double tol,
                   int max_iter, double* xh, double* xl, hp_refine_stats* st, hipStream_t s);

@@ -156,40 +156,32 @@ __global__ __launch_bounds__(64) void k_diag_block_inverse(Mat l, long n, double
   }
 }
 
-// the sum over the four lanes 4r..4r+3 that share a row (each holds 16 of its 64 terms)
-template <class A>
-__device__ __forceinline__ typename A::T quarter_sum(typename A::T q) {  // lanes 4r..4r+3
-#pragma unroll
-  for (int off = 1; off < 4; off <<= 1) {
-    typename A::T o;
-    if constexpr (A::LIMBS == 2) o = mkdd(__shfl_xor(q.hi, off, 64), __shfl_xor(q.lo, off, 64));
-    else o = __shfl_xor(q, off, 64);
-    q = A::add(q, o);
-  }
-  return q;
-}
-
 // ---------------------------------------------------------------- persistent block solves
-// L L^T x = y in ONE launch of 2 nblk workgroups (256 threads), with the inverses of the
+// L L^T x = y in ONE launch of 2 nblk workgroups (512 threads), with the inverses of the
 // 64 x 64 diagonal blocks:
 //   workgroup b < nblk:        z_b = inv(L_bb)   (y_b - sum_{j<b} L_bj z_j)
 //   workgroup nblk + c:        x_b = inv(L_bb)^T (z_b - sum_{j>b} L_jb^T x_j),  b = nblk-1-c
-// Each block publishes its result with a flag (agent-scope release after its stores); a
-// consumer waits for a flag, then takes every block whose flag is already set, so a block
+// Each block publishes its result with a flag: wavefront 0 stores the block, takes one
+// agent-scope release fence and sets the flag; a consumer polls flags with relaxed loads,
+// takes one acquire fence, then every block whose flag is already set, so a block
 // far down the chain keeps up with the front.  Every wait is on a lower-numbered
 // workgroup: under in-order dispatch the smallest-numbered resident workgroup never waits
 // on one that is not running, so the chain always advances, whatever the residency.  Each
 // wait is also bounded (flag_wait): past the limit it raises *fail and the workgroups
-// leave (the host reports a device error).  Summation order: four interleaved chains per
-// row, combined by lane shuffles (these solves only feed the refinement).
+// leave (the host reports a device error).  Summation order: eight lanes per row, four
+// interleaved chains per lane, combined by lane shuffles (these solves only feed the
+// refinement).
 constexpr int SPIN_LIMIT = 1 << 22;
 
-__device__ __forceinline__ int ld_acquire(const int* f) {
-  return __hip_atomic_load(f, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
+// Flags are polled with relaxed agent-scope loads (coherent, no cache invalidation); the
+// consumer's workgroup takes one acquire fence after the barrier that follows a successful
+// wait.  (An acquire load per poll would invalidate the caches on every spin.)
+__device__ __forceinline__ int ld_flag(const int* f) {
+  return __hip_atomic_load(f, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 __device__ bool flag_wait(const int* f, int epoch, int* fail) {
   for (int spins = 0;; ++spins) {
-    if (ld_acquire(f) >= epoch) return true;
+    if (ld_flag(f) >= epoch) return true;
     if ((spins & 255) == 255 && __hip_atomic_load(fail, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) return false;
     if (spins > SPIN_LIMIT) {
       __hip_atomic_store(fail, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
@@ -199,90 +191,105 @@ __device__ bool flag_wait(const int* f, int epoch, int* fail) {
   }
 }
 
+// the sum over the eight lanes 8r..8r+7 that share a row (each holds 8 of its 64 terms)
 template <class A>
-__global__ __launch_bounds__(256) void k_inv_solve(Mat l, const double* __restrict__ ih, const double* __restrict__ il,
+__device__ __forceinline__ typename A::T octet_sum(typename A::T q) {
+#pragma unroll
+  for (int off = 1; off < 8; off <<= 1) {
+    typename A::T o;
+    if constexpr (A::LIMBS == 2) o = mkdd(__shfl_xor(q.hi, off, 64), __shfl_xor(q.lo, off, 64));
+    else o = __shfl_xor(q, off, 64);
+    q = A::add(q, o);
+  }
+  return q;
+}
+
+// 512 threads: row r = t / 8 of the block, part = t % 8 holds terms 8 part .. 8 part + 7 of
+// each 64-term sum, as four 2-term chains
+template <class A>
+__global__ __launch_bounds__(512) void k_inv_solve(Mat l, const double* __restrict__ ih, const double* __restrict__ il,
                                                    const double* yh, const double* yl, double* zh, double* zl,
                                                    double* xh, double* xl, long n, int nblk, int* flags, int epoch,
                                                    int* fail, unsigned long long* red) {
   using T = typename A::T;
-  __shared__ double Th[BW], Tl[BW];
+  constexpr int TPL = 8;  // terms per lane
+  __shared__ double Th[BW], Tl[BW], Oh[BW], Ol[BW];
   __shared__ int s_ready[2], s_ok[2], s_zok;
-  const int t = threadIdx.x, part = t & 3, kb = part * 16, r = t >> 2;
+  const int t = threadIdx.x, part = t & 7, kb = part * TPL, r = t >> 3;
   const bool fwd = (int)blockIdx.x < nblk;
   const int b = fwd ? (int)blockIdx.x : 2 * nblk - 1 - (int)blockIdx.x;
   const long b0 = (long)b * BW;
   const int bw = (int)min((long)BW, n - b0);
   const bool live = r < bw;
   if (blockIdx.x == 0 && t < 2) red[t] = 0;  // the max-reductions of the kernel that follows
-  // this lane's 16 terms of the inverse (fwd: row r, bwd: column r), loaded ahead of any wait
+  // this lane's terms of the inverse (fwd: row r, bwd: column r), loaded ahead of any wait
   const size_t base = (size_t)b * BW * BW;
-  T iv[16];
+  T iv[TPL];
 #pragma unroll
-  for (int u = 0; u < 16; ++u) {
+  for (int u = 0; u < TPL; ++u) {
     const int k = kb + u;
     iv[u] = A::ldv(ih, il, (long)(base + (fwd ? (size_t)r * BW + k : (size_t)k * BW + r)));
   }
   const int* dep = flags + (fwd ? 0 : nblk);
-  T p0 = A::zero(), p1 = A::zero();
+  T p[4] = {A::zero(), A::zero(), A::zero(), A::zero()};
+  // this lane's factor entries of dependency block j: fwd L[b0+r][j0+kb+u] (row r of L_bj),
+  // bwd L[j0+kb+u][b0+r] (column r of L_jb); zero outside the matrix
+  auto load_l = [&](int j, T (&lv)[TPL]) {
+    const long j0 = (long)j * BW;
+    const int bwj = (int)min((long)BW, n - j0);
+#pragma unroll
+    for (int u = 0; u < TPL; ++u) {
+      const int k = kb + u;
+      lv[u] = (live && k < bwj) ? (fwd ? A::ld(l, b0 + r, j0 + k) : A::ld(l, j0 + k, b0 + r)) : A::zero();
+    }
+  };
+  const double* vh = fwd ? zh : xh;  // the dependencies' results
+  const double* vl = fwd ? zl : xl;
+  auto take = [&](int j, const T (&lv)[TPL]) {
+    const long j0 = (long)j * BW;
+    const int bwj = (int)min((long)BW, n - j0);
+    T vv[TPL];
+#pragma unroll
+    for (int u = 0; u < TPL; ++u) vv[u] = kb + u < bwj ? A::ldv(vh, vl, j0 + kb + u) : A::zero();
+#pragma unroll
+    for (int u = 0; u < TPL; ++u) p[u & 3] = A::add(p[u & 3], A::mul(lv[u], vv[u]));
+  };
   int next = fwd ? 0 : nblk - 1;  // next dependency to take: fwd ascending below b, bwd descending above b
+  const int step = fwd ? 1 : -1;
   bool ok = true;
+  T lv[TPL];
   for (int round = 0; fwd ? next < b : next > b; ++round) {
+    load_l(next, lv);  // the factor does not depend on the flags: in flight during the wait
     if (t == 0) {
       int k = next;
       bool w = flag_wait(dep + k, epoch, fail);
       if (w) {
-        if (fwd) { ++k; while (k < b && ld_acquire(dep + k) >= epoch) ++k; }
-        else { --k; while (k > b && ld_acquire(dep + k) >= epoch) --k; }
+        k += step;
+        while ((fwd ? k < b : k > b) && ld_flag(dep + k) >= epoch) k += step;
       }
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");  // the flagged blocks' stores are visible
       s_ready[round & 1] = k;
       s_ok[round & 1] = w;
     }
     __syncthreads();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
     const int ready = s_ready[round & 1];
     if (!s_ok[round & 1]) { ok = false; break; }
-    if (fwd) {
-      for (int j = next; j < ready; ++j) {  // full blocks (j < b)
-        T lv[16], zv[16];
-#pragma unroll
-        for (int u = 0; u < 16; ++u) {
-          zv[u] = A::ldv(zh, zl, (long)j * BW + kb + u);
-          lv[u] = live ? A::ld(l, b0 + r, (long)j * BW + kb + u) : A::zero();
-        }
-#pragma unroll
-        for (int u = 0; u < 16; u += 2) {
-          p0 = A::add(p0, A::mul(lv[u], zv[u]));
-          p1 = A::add(p1, A::mul(lv[u + 1], zv[u + 1]));
-        }
-      }
-    } else {
-      for (int j = next; j > ready; --j) {
-        const long j0 = (long)j * BW;
-        const int bwj = (int)min((long)BW, n - j0);
-        T lv[16], xv[16];
-#pragma unroll
-        for (int u = 0; u < 16; ++u) {
-          const int k = kb + u;
-          xv[u] = k < bwj ? A::ldv(xh, xl, j0 + k) : A::zero();
-          lv[u] = (live && k < bwj) ? A::ld(l, j0 + k, b0 + r) : A::zero();
-        }
-#pragma unroll
-        for (int u = 0; u < 16; u += 2) {
-          p0 = A::add(p0, A::mul(lv[u], xv[u]));
-          p1 = A::add(p1, A::mul(lv[u + 1], xv[u + 1]));
-        }
-      }
+    for (int j = next; j != ready; j += step) {
+      if (j != next) load_l(j, lv);
+      take(j, lv);
     }
     next = ready;
   }
   if (!ok) return;
   if (!fwd) {  // z_b of the forward pass
-    if (t == 0) s_zok = flag_wait(flags + b, epoch, fail);
+    if (t == 0) {
+      s_zok = flag_wait(flags + b, epoch, fail);
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+    }
     __syncthreads();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
     if (!s_zok) return;
   }
-  const T sum = quarter_sum<A>(A::add(p0, p1));
+  const T sum = octet_sum<A>(A::add(A::add(p[0], p[1]), A::add(p[2], p[3])));
   if (part == 0) {
     T v = A::zero();
     if (live) v = A::sub(fwd ? A::ldv(yh, yl, b0 + r) : A::ldv(zh, zl, b0 + r), sum);
@@ -290,23 +297,31 @@ __global__ __launch_bounds__(256) void k_inv_solve(Mat l, const double* __restri
     else Th[r] = v;
   }
   __syncthreads();
-  T q0 = A::zero(), q1 = A::zero();
+  T q[4] = {A::zero(), A::zero(), A::zero(), A::zero()};
 #pragma unroll
-  for (int u = 0; u < 16; u += 2) {
-    T t0, t1;
-    if constexpr (A::LIMBS == 2) { t0 = mkdd(Th[kb + u], Tl[kb + u]); t1 = mkdd(Th[kb + u + 1], Tl[kb + u + 1]); }
-    else { t0 = Th[kb + u]; t1 = Th[kb + u + 1]; }
-    q0 = A::add(q0, A::mul(iv[u], t0));
-    q1 = A::add(q1, A::mul(iv[u + 1], t1));
+  for (int u = 0; u < TPL; ++u) {
+    T tv;
+    if constexpr (A::LIMBS == 2) tv = mkdd(Th[kb + u], Tl[kb + u]);
+    else tv = Th[kb + u];
+    q[u & 3] = A::add(q[u & 3], A::mul(iv[u], tv));
   }
-  const T o = quarter_sum<A>(A::add(q0, q1));
-  if (part == 0 && live) {
-    if (fwd) A::stv(zh, zl, b0 + r, o);
-    else A::stv(xh, xl, b0 + r, o);
+  const T o = octet_sum<A>(A::add(A::add(q[0], q[1]), A::add(q[2], q[3])));
+  if (part == 0) {
+    if constexpr (A::LIMBS == 2) { Oh[r] = o.hi; Ol[r] = o.lo; }
+    else Oh[r] = o;
   }
-  __threadfence();
   __syncthreads();
-  if (t == 0) __hip_atomic_store(flags + (fwd ? 0 : nblk) + b, epoch, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+  if (t < 64) {  // wavefront 0 stores the block and publishes it: one release for the block
+    if (t < bw) {
+      T v;
+      if constexpr (A::LIMBS == 2) v = mkdd(Oh[t], Ol[t]);
+      else v = Oh[t];
+      if (fwd) A::stv(zh, zl, b0 + t, v);
+      else A::stv(xh, xl, b0 + t, v);
+    }
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+    if (t == 0) __hip_atomic_store(flags + (fwd ? 0 : nblk) + b, epoch, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  }
 }
 
 // x += dx (dd += dd, or dd += f64 when dl is null), and red[0] = max|dx.hi|, red[1] =
@@ -438,7 +453,7 @@ int solve_inv_persistent(Mat l, const double* ih, const double* il, const double
                          double* zl, double* xh, double* xl, long n, int* flags, int epoch, int* fail,
                          unsigned long long* red, hipStream_t s) {
   const int nblk = (int)((n + BW - 1) / BW);
-  hipLaunchKernelGGL(k_inv_solve<A>, dim3((unsigned)(2 * nblk)), dim3(256), 0, s, l, ih, il, yh, yl, zh, zl, xh, xl, n,
+  hipLaunchKernelGGL(k_inv_solve<A>, dim3((unsigned)(2 * nblk)), dim3(512), 0, s, l, ih, il, yh, yl, zh, zl, xh, xl, n,
                      nblk, flags, epoch, fail, red);
   HP_LAUNCH_CHECK();
   return HP_OK;

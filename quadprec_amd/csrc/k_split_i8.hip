@@ -47,7 +47,8 @@ constexpr int CRT_L40 = 6;    // device: 240 bits as 6 x 40-bit limbs held in do
 typedef int v4i_t __attribute__((ext_vector_type(4)));
 typedef int v16i_t __attribute__((ext_vector_type(16)));
 
-// The moduli: 2^8 and the largest pairwise-coprime values below it.
+// The moduli: 2^8 and 31 pairwise-coprime values below it (primes and products of primes
+// that no other modulus shares), largest first: the first R of them serve R moduli.
 static const uint32_t kModuli[CRT_MAXR] = {256, 255, 253, 251, 247, 241, 239, 233, 229, 227, 223,
                                            211, 199, 197, 193, 191, 181, 179, 173, 167, 163, 157,
                                            151, 149, 139, 137, 131, 127, 113, 109, 107, 103};
@@ -232,8 +233,14 @@ double log2_M(int R) {
 }
 
 // ------------------------------------------------------------------ device helpers
+// scale exponent of a row / column holding an inf or a NaN: the max-reductions keep it, the
+// residues of such a row / column are meaningless (finite int8 values, no trap), and the
+// CRT writes NaN for every element it touches, as the reference's dd arithmetic yields
+// (two_prod of an inf gives a NaN error term, dd.rs:31-35, and NaN propagates)
+constexpr int E_NONFINITE = 1 << 28;
 __device__ __forceinline__ int frexp_exp(double v) {  // |v| < 2^e, -100000 for 0
   if (v == 0.0) return -100000;
+  if (!__builtin_isfinite(v)) return E_NONFINITE;
   int e;
   (void)__builtin_frexp(v, &e);
   return e;
@@ -1088,7 +1095,8 @@ __global__ __launch_bounds__(256) void k_crt(const uint8_t* __restrict__ Cr, lon
       sl[2 * g + 1] = __builtin_bit_cast(double, (uint64_t)lw[1] | ((uint64_t)hw[1] << 32));
     }
     if (j >= N || (lower && j > i)) continue;
-    dd res = crt_finish(sl, mt, er[i] - 2 * W + fc[j]);
+    dd res = (er[i] >= E_NONFINITE || fc[j] >= E_NONFINITE) ? mkdd(__builtin_nan(""), __builtin_nan(""))
+                                                              : crt_finish(sl, mt, er[i] - 2 * W + fc[j]);
     const size_t co = (size_t)i * ldc + j;
     if (subtract) res = dd_sub(mkdd(chi[co], clo[co]), res);
     chi[co] = res.hi;
@@ -1358,6 +1366,100 @@ int split_gemm_general(const SplitOperand& A, const SplitOperand& B, double* chi
   g_work.pending = true;
   tm.finish();
   return HP_OK;
+}
+
+// ------------------------------------------------------------------ accuracy-driven autotuner
+// The fewest moduli whose integer width W meets `bits`: the split error bound is
+// 2^(1-W) (max_p|A_ip| sum_p|B_pj| + sum_p|A_ip| max_p|B_pj|) (+ the final dd rounding), so
+// W >= bits + 1 gives 2^-bits of that norm; 0 if no moduli count reaches it.
+int split_slices_for_bits(int bits, size_t k) {
+  for (int R = 8; R <= CRT_MAXR; ++R) {
+    const int W = crt_bits_W(R, k);
+    if (W > 117) break;
+    if (W >= bits + 1) return R;
+  }
+  return 0;
+}
+// bits the plain kernel guarantees: each element is a k-term recursive dd sum whose every
+// product and addition errs by at most ~2^-102 relative, so |error| <= (k+1) 2^-102
+// (|A||B|)_ij, which the split form's norm bounds from above
+int plain_bits_for_k(size_t k) {
+  int lk = 0;
+  while ((1ull << lk) < (unsigned long long)(k + 1)) ++lk;
+  return 102 - lk;
+}
+
+namespace {
+std::map<uint64_t, hp_gemm_plan> g_plans;  // per (shape class, trans_a, bits), under g_tune_mu
+uint64_t plan_key(int trans_a, size_t m, size_t n, size_t k, int bits) {
+  return shape_key(m, n, k, 0) | ((uint64_t)(bits & 255) << 32) | ((uint64_t)(trans_a ? 1 : 0) << 41);
+}
+}  // namespace
+
+int gemm_plan_get(int trans_a, size_t m, size_t n, size_t k, int bits, hp_gemm_plan* plan) {
+  if (bits <= 0) bits = 104;
+  std::lock_guard<std::mutex> tl(g_tune_mu);
+  auto it = g_plans.find(plan_key(trans_a, m, n, k, bits));
+  if (it == g_plans.end()) return 0;
+  *plan = it->second;
+  return 1;
+}
+
+// HP_GEMM_AUTO: the cheaper of the plain kernel (when its bound meets `bits`) and the split
+// form with the fewest moduli meeting `bits`, timed once per shape class on the caller's
+// operands (each candidate run once to warm, once under events), then reused.
+int gemm_auto(int trans_a, const double* ahi, const double* alo, const double* bhi, const double* blo, double* chi,
+              double* clo, size_t m, size_t n, size_t k, int bits, hipStream_t s) {
+  if (bits <= 0) bits = 104;
+  if (bits > 112) {
+    set_error("hp_dev_gemm AUTO: %d bits requested; at most 112", bits);
+    return HP_EINVAL;
+  }
+  const int R = k ? split_slices_for_bits(bits, k) : 0;
+  const bool plain_ok = bits <= plain_bits_for_k(k);
+  if (!R && !plain_ok) {
+    set_error("hp_dev_gemm AUTO: no form meets %d bits at k = %zu", bits, k);
+    return HP_EINVAL;
+  }
+  hp_gemm_plan plan{};
+  if (!gemm_plan_get(trans_a, m, n, k, bits, &plan)) {
+    plan.mode = R ? HP_GEMM_SPLIT_I8 : HP_GEMM_PLAIN;
+    plan.slices = R;
+    plan.w_bits = R ? crt_bits_W(R, k) : 0;
+    if (R && plain_ok && m && n && k) {
+      hipEvent_t e0, e1, e2;
+      HP_HIP(hipEventCreate(&e0));
+      HP_HIP(hipEventCreate(&e1));
+      HP_HIP(hipEventCreate(&e2));
+      int rc;
+      if ((rc = launch_gemm_dd_plain(trans_a, ahi, alo, bhi, blo, chi, clo, m, n, k, s)) ||
+          (rc = launch_gemm_split_i8(trans_a, ahi, alo, bhi, blo, chi, clo, m, n, k, R, s)))
+        return rc;
+      HP_HIP(hipEventRecord(e0, s));
+      if ((rc = launch_gemm_dd_plain(trans_a, ahi, alo, bhi, blo, chi, clo, m, n, k, s))) return rc;
+      HP_HIP(hipEventRecord(e1, s));
+      if ((rc = launch_gemm_split_i8(trans_a, ahi, alo, bhi, blo, chi, clo, m, n, k, R, s))) return rc;
+      HP_HIP(hipEventRecord(e2, s));
+      HP_HIP(hipEventSynchronize(e2));
+      HP_HIP(hipEventElapsedTime(&plan.ms_plain, e0, e1));
+      HP_HIP(hipEventElapsedTime(&plan.ms_split, e1, e2));
+      (void)hipEventDestroy(e0);
+      (void)hipEventDestroy(e1);
+      (void)hipEventDestroy(e2);
+      if (plan.ms_plain <= plan.ms_split) plan.mode = HP_GEMM_PLAIN;
+    }
+    std::lock_guard<std::mutex> tl(g_tune_mu);
+    g_plans[plan_key(trans_a, m, n, k, bits)] = plan;
+  }
+  if (plan.mode == HP_GEMM_SPLIT_I8) {
+    int rc = launch_gemm_split_i8(trans_a, ahi, alo, bhi, blo, chi, clo, m, n, k, plan.slices, s);
+    if (rc) return rc;
+    plan.variant = split_gemm_variant(m, n, k, plan.slices);
+    std::lock_guard<std::mutex> tl(g_tune_mu);
+    g_plans[plan_key(trans_a, m, n, k, bits)].variant = plan.variant;
+    return HP_OK;
+  }
+  return launch_gemm_dd_plain(trans_a, ahi, alo, bhi, blo, chi, clo, m, n, k, s);
 }
 
 int launch_gemm_split_i8(int trans_a, const double* ahi, const double* alo, const double* bhi, const double* blo,

@@ -158,3 +158,98 @@ def test_split_i8_full_size_sampled(qp):
     got = np.ascontiguousarray(c[rows][:, cols]).reshape(-1, 2)
     ab = absprod(a_s, b_s, len(rows), len(cols), k)
     assert comp_err(got, ref, ab) <= TOL
+
+
+def split_norm_bound(a, b, m, n, k):
+    """max_p|A_ip| sum_p|B_pj| + sum_p|A_ip| max_p|B_pj| per element (the split form's norm)."""
+    A = np.abs(a[:, 0]).reshape(m, k)
+    B = np.abs(b[:, 0]).reshape(k, n)
+    return (np.outer(A.max(axis=1), B.sum(axis=0)) + np.outer(A.sum(axis=1), B.max(axis=0))).ravel()
+
+
+def abs_err(c, ref4):
+    return np.abs(((c[:, 0] - ref4[:, 0]) - ref4[:, 1] + c[:, 1]) - ref4[:, 2] - ref4[:, 3])
+
+
+@pytest.mark.parametrize("bits", [30, 53, 64, 80, 96, 104, 110])
+def test_auto_mode_meets_the_requested_bits(qp, bits):
+    """HP_GEMM_AUTO: the plan takes the fewest moduli whose W reaches bits + 1 (or the plain
+    kernel when its (k+1) 2^-102 bound meets bits and it is faster), and the result is within
+    2^-bits of the split norm against the f256 oracle."""
+    m, n, k = 130, 90, 1000
+    a = qp.gen_dd_normal(70 + bits, m * k)
+    b = qp.gen_dd_normal(71 + bits, k * n)
+    c = dev_gemm(qp, a, b, m, n, k, qp.HP_GEMM_AUTO, slices=bits)
+    plan = qp.gemm_plan(m, n, k, bits)
+    assert plan is not None
+    r = qp.split_slices_for_bits(k, bits)
+    assert r >= 8 and r <= 32
+    if plan.mode == qp.HP_GEMM_SPLIT_I8:
+        assert plan.slices == r and plan.w_bits >= bits + 1
+    else:
+        assert plan.mode == qp.HP_GEMM_PLAIN and bits <= 102 - 10
+    ref = O.mf4_gemm(to_mf4(a), to_mf4(b), m, n, k)
+    bound = 2.0 ** -bits * split_norm_bound(a, b, m, n, k) + 2.0 ** -105 * np.abs(ref[:, 0])
+    err = abs_err(c, ref)
+    print(f"bits={bits}: mode={plan.mode} slices={plan.slices} W={plan.w_bits} plain {plan.ms_plain:.3f} ms "
+          f"split {plan.ms_split:.3f} ms; worst error / bound {np.max(err / bound):.3e}")
+    assert np.all(err <= bound)
+
+
+def test_auto_mode_rejects_unreachable_accuracy(qp):
+    import torch
+    a = qp.dev.to_planes(qp.gen_dd_normal(1, 64), 2)
+    c = torch.empty((2, 64), dtype=torch.float64, device="cuda")
+    with pytest.raises(qp.HyperprecError):
+        qp.dev.gemm(a, a, c, 8, 8, 8, mode=qp.HP_GEMM_AUTO, slices=113)
+
+
+def test_split_nonfinite_rows_and_columns_give_nan(qp):
+    """inf / NaN in a row of A or a column of B: that row / column of C is NaN, exactly where
+    the reference's dd arithmetic (the bit-exact oracle) puts NaNs; the other entries stay
+    within the split bound."""
+    m, n, k = 70, 50, 130
+    a = qp.gen_dd_normal(81, m * k)
+    b = qp.gen_dd_normal(82, k * n)
+    a[3 * k + 5] = [np.inf, 0.0]
+    a[10 * k + 7] = [np.nan, np.nan]
+    b[8 * n + 20] = [-np.inf, 0.0]
+    b[2 * n + 33] = [np.nan, 0.0]
+    c = dev_gemm(qp, a, b, m, n, k, qp.HP_GEMM_SPLIT_I8)
+    ref = O.gemm(a, b, m, n, k)
+    assert np.array_equal(np.isnan(c[:, 0]), np.isnan(ref[:, 0]))
+    assert np.isnan(c[:, 0]).reshape(m, n)[[3, 10]].all() and np.isnan(c[:, 0]).reshape(m, n)[:, [20, 33]].all()
+    fin = ~np.isnan(ref[:, 0])
+    a_f, b_f = a.copy(), b.copy()
+    a_f[~np.isfinite(a_f)] = 0.0
+    b_f[~np.isfinite(b_f)] = 0.0
+    ref4 = O.mf4_gemm(to_mf4(a_f), to_mf4(b_f), m, n, k)
+    bound = 2.0 ** -100 * split_norm_bound(a_f, b_f, m, n, k)
+    assert np.all(abs_err(c, ref4)[fin] <= bound[fin])
+
+
+def test_split_bound_is_normwise_not_componentwise(qp):
+    """A row [1, e] times a column [e, 1] with e = 1e-20: C = 2e-20.  The split result meets its
+    normwise bound (2^(1-W) of ~2) but keeps only ~42 bits of C (e is truncated at 2^-108 of
+    its row's maximum); the plain kernel keeps dd accuracy componentwise.  This is the
+    documented contract of HP_GEMM_SPLIT_I8."""
+    m, n, k = 4, 4, 2
+    e = np.array([1e-20, 0.0])
+    a = np.zeros((m * k, 2))
+    b = np.zeros((k * n, 2))
+    for i in range(m):
+        a[i * k] = [1.0, 0.0]
+        a[i * k + 1] = e
+    for j in range(n):
+        b[j] = e
+        b[n + j] = [1.0, 0.0]
+    c_split = dev_gemm(qp, a, b, m, n, k, qp.HP_GEMM_SPLIT_I8)
+    c_plain = dev_gemm(qp, a, b, m, n, k, qp.HP_GEMM_PLAIN)
+    ref = O.mf4_gemm(to_mf4(a), to_mf4(b), m, n, k)
+    norm = split_norm_bound(a, b, m, n, k)  # 2^(1-W) of it is the bound; W > 107 at k = 2
+    es, ep = abs_err(c_split, ref), abs_err(c_plain, ref)
+    assert np.all(es <= 2.0 ** -106 * norm + 2.0 ** -106 * np.abs(ref[:, 0]))
+    rel_s, rel_p = np.max(es / np.abs(ref[:, 0])), np.max(ep / np.abs(ref[:, 0]))
+    print(f"graded: split componentwise {rel_s:.2e}, plain {rel_p:.2e}")
+    assert rel_p <= 2.0 ** -100
+    assert rel_s > 2.0 ** -60  # the componentwise loss the normwise bound allows
