@@ -24,7 +24,8 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 M = N = K = 4096
-SLICES = 31                      # CRT moduli = int8 GEMMs per dd GEMM
+DD_BITS = 104                    # accuracy the headline asks the autotuner for (dd)
+SLICES = 31                      # CRT moduli the autotuner picks for DD_BITS at k = 4096
 FP64_PEAK_TFLOPS = 78.6          # MI355X FP64 vector (= FP64 matrix) dense peak
 INT8_PEAK_TOPS = 5000.0          # MI355X dense INT8 matrix peak
 DD_MAC_OPS = 29                  # FP64 ops per reference dd MAC (dd add 20 + dd mul 9)
@@ -35,6 +36,39 @@ I8_KERNELS = {32: "k_gemm_i8_mod<32>", 64: "k_gemm_i8_mod<64>", 128: "k_gemm_i8_
 # counters scaled by a device copy of known size (k_gemm_i8_mod_dlp8: read 3.126e9 + write
 # 0.520e9); null for a kernel these were not measured on
 I8_TRAFFIC_BYTES = {4064: 3.646e9}
+
+
+def oracle():
+    """The oracle (tests/oracle_lib.py): only the cpu_baseline legs and the error checks of
+    the legs below use it, never inside a timed GPU region."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import oracle_lib as O
+    return O
+
+
+def host_cpu():
+    model = "unknown"
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    return {"model": model, "threads": oracle().lib().hpo_num_threads(), "nproc": len(os.sched_getaffinity(0))}
+
+
+def best_of(fn, reps, torch):
+    fn()
+    torch.cuda.synchronize()
+    ts = []
+    for _ in range(reps):
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        fn()
+        torch.cuda.synchronize()
+        ts.append(time.perf_counter() - t0)
+    return min(ts) * 1e3
 
 
 def cpu_baseline(a, b, target_s=12.0):
@@ -64,21 +98,83 @@ SOLVE_CASES = [  # (name, alpha_lo, spacing, target rel err, split-path factor, 
 ]
 
 
-def bench_solves(qp, torch, dev):
+def dd_rel_err(O, x, xs):
+    import numpy as np
+    d = O.dd_binop("sub", x, xs)
+    return float(np.max(np.abs(d[:, 0] + d[:, 1])) / np.max(np.abs(xs[:, 0])))
+
+
+def oracle_refined_solution(O, s, b, n, l, x0):
+    """x*: corrections of x0 through the oracle's factor l with MultiFloat<4> residuals b - S x
+    (crates/quad/src/multifloat/linalg.rs:17-36) until they stop at x's dd rounding level."""
+    import numpy as np
+
+    def mf4(v):
+        out = np.zeros((v.shape[0], 4))
+        out[:, :2] = v
+        return out
+    x, s4, b4, prev = x0, mf4(s), mf4(b), np.inf
+    for _ in range(12):
+        r4 = O.mf4_binop("sub", b4, O.mf4_gemv(s4, mf4(x), n, n))
+        dx = O.solve_cholesky(l, np.ascontiguousarray(r4[:, :2]), n)
+        x = O.dd_binop("add", x, dx)
+        rel = np.max(np.abs(dx[:, 0])) / np.max(np.abs(x[:, 0]))
+        if rel <= 1e-34 or (rel <= 1e-31 and rel > 0.25 * prev):
+            break
+        prev = rel
+    return x
+
+
+def oracle_iterative_refinement(O, s, b, n):
+    """The reference's CPU path for a dd system: cholesky_blocked(nb = 64)
+    (crates/quad/src/linalg.rs:238-293), solve_cholesky, then iterative_refinement's loop
+    (crates/solver/src/lib.rs:92-116): r = b - S x with the dd matvec, stop when
+    ||r|| < 1e-28 ||b|| or, after 3 steps, when ||r|| > 0.5 best; dx = solve_cholesky(L, r)."""
+    l = O.cholesky_blocked(s, n, 64)
+    x = O.solve_cholesky(l, b, n)
+    bn = O.norm_f128(b)
+    best, iters = float("inf"), 0
+    for it in range(30):
+        r = O.dd_binop("sub", b, O.matvec(s, x, n))
+        rn = O.norm_f128(r)
+        if rn < 1e-28 * bn or (it > 3 and rn > 0.5 * best):
+            break
+        best = min(best, rn)
+        x = O.dd_binop("add", x, O.solve_cholesky(l, r, n))
+        iters += 1
+    return l, x, iters
+
+
+def split_factor_work_ms(n, nb=256, slices=SLICES):
+    """Lower bound of the split-dd factor's time from its work at peak rates: the panels'
+    dd operations on FP64 VALU (the panel's own columns ~ n^2 nb / 4 MACs, the next panel's
+    diagonal blocks ~ n nb^2 / 2) at 29 ops per MAC, and the remaining n^3/6 MACs as split
+    products, 2 slices int8 ops each, on the INT8 matrix cores."""
+    valu_macs = n * n * nb / 4.0 + n * nb * nb / 2.0
+    i8_macs = n ** 3 / 6.0 - valu_macs
+    return (DD_MAC_OPS * valu_macs / (FP64_PEAK_TFLOPS / 2 * 1e12) + 2 * slices * i8_macs / (INT8_PEAK_TOPS * 1e12)) * 1e3
+
+
+def bench_solves(qp, torch, dev, cpu):
     """N=4096 overlap SPD solves to the target error: split path vs the plain dd factor,
-    both refined with quad-double residuals (time = factor + refinement, device-synced)."""
+    both refined with quad-double residuals (time = factor + refinement, device-synced).
+    Each solution's error is measured against x* (the oracle's dd factor corrected with
+    MultiFloat<4> residuals, computed on the host after the timed runs)."""
+    import numpy as np
     kinds = {"f64": qp.HP_FACTOR_F64, "dd": qp.HP_FACTOR_DD, "split": qp.HP_FACTOR_DD_SPLIT}
     out = {}
     n = SOLVE_N
+    O = oracle()
     for name, lo, sp, target, split_kind, seed in SOLVE_CASES:
         c, a = qp.gen_overlap_basis(seed, n, lo, 10.0, sp)
         dc = torch.from_numpy(c.ravel()).to(dev)
         da = torch.from_numpy(a).to(dev)
         S = torch.empty((2, n * n), dtype=torch.float64, device=dev)
         qp.dev.overlap_s(dc, da, S, n)
-        b = qp.dev.to_planes(qp.gen_dd_normal(seed + 100, n), 2, dev)
+        b_host = qp.gen_dd_normal(seed + 100, n)
+        b = qp.dev.to_planes(b_host, 2, dev)
         x = torch.empty((2, n), dtype=torch.float64, device=dev)
-        res = {}
+        res, xs_gpu = {}, {}
         for label, kind in (("split", split_kind), ("plain", "dd")):
             best = None
             for _ in range(2):  # the first run also warms the autotuner and allocations
@@ -88,19 +184,55 @@ def bench_solves(qp, torch, dev):
                 torch.cuda.synchronize()
                 t = (time.perf_counter() - t0) * 1e3
                 if best is None or t < best[0]:
-                    best = (t, st.factor_ms, st.refine_ms, st.iterations, st.last_correction)
+                    best = (t, st.factor_ms, st.refine_ms, st.iterations, st.last_correction, st.converged)
+            xs_gpu[label] = qp.dev.from_planes(x)
             res[label] = {"factor": kind, "ms": round(best[0], 2), "factor_ms": round(best[1], 2),
                           "refine_ms": round(best[2], 2), "iterations": best[3],
-                          "last_correction": float(f"{best[4]:.3e}")}
+                          "last_correction": float(f"{best[4]:.3e}"),
+                          "converged": {0: "no", 1: "measured", 2: "predicted"}[best[5]]}
+            if kind == "dd":
+                ach = 2.0 * DD_MAC_OPS * n ** 3 / 6.0 / (best[1] * 1e-3) / 1e12
+                res[label]["roofline"] = {
+                    "bound": "valu", "kernel": "reference-order factor (k_diag_factor, k_row_recur, k_rank_update)",
+                    "achieved": round(ach, 2), "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
+                    "frac": round(ach / FP64_PEAK_TFLOPS, 4),
+                    "note": "n^3/6 dd MACs x 29 FP64 ops (FMA-equivalent) over factor_ms"}
+            elif kind == "f64":
+                ach = 2.0 * n ** 3 / 6.0 / (best[1] * 1e-3) / 1e12
+                res[label]["roofline"] = {"bound": "valu", "achieved": round(ach, 3), "peak": FP64_PEAK_TFLOPS,
+                                          "unit": "TFLOP/s", "frac": round(ach / FP64_PEAK_TFLOPS, 4),
+                                          "note": "n^3/6 binary64 MACs (2 flops) over factor_ms: the 64-wide "
+                                                  "diagonal blocks' pivot chain bounds it, not the FLOPs"}
+            else:
+                wb = split_factor_work_ms(n)
+                res[label]["roofline"] = {
+                    "bound": "latency", "work_bound_ms": round(wb, 3), "frac": round(wb / best[1], 4),
+                    "note": "frac = the factor's work at the FP64 VALU and INT8 peaks over factor_ms; the "
+                            "factor is held by the serial chain of 64 diagonal 64x64 sub-block factorisations "
+                            "(k_diag_factor_split, ~58 us each)"}
+        # x* and the errors (host, after the timed runs)
+        s_host = O.overlap_s_dd(c, a)
+        t0 = time.perf_counter()
+        l_cpu, x_cpu, it_cpu = oracle_iterative_refinement(O, s_host, b_host, n)
+        t_cpu = time.perf_counter() - t0
+        xstar = oracle_refined_solution(O, s_host, b_host, n, l_cpu, x_cpu)
+        for label in ("split", "plain"):
+            res[label]["rel_err"] = float(f"{dd_rel_err(O, xs_gpu[label], xstar):.3e}")
         res["alpha"] = [lo, 10.0]
         res["spacing_bohr"] = sp
         res["target_rel_err"] = target
         res["split_over_plain"] = round(res["plain"]["ms"] / res["split"]["ms"], 2)
+        if cpu:
+            res["cpu_baseline"] = {
+                "value": round(t_cpu * 1e3, 1), "unit": "ms", "cores": O.lib().hpo_num_threads(), "kind": "port",
+                "rel_err": float(f"{dd_rel_err(O, x_cpu, xstar):.3e}"), "refinement_iters": it_cpu,
+                "sample": "the full case: oracle cholesky_blocked(nb=64) + solve_cholesky + iterative_refinement's "
+                          "dd-residual loop (lib.rs:92-116), OpenMP rows; its error is what dd residuals reach"}
         out[name] = res
     return out
 
 
-def bench_batch(qp, torch, dev, count=512, nrhs=4, seed=3):
+def bench_batch(qp, torch, dev, cpu, count=512, nrhs=4, seed=3):
     """Config 3: 512 overlap systems with n ~ U[64, 1024] (seeded), each from its own seed,
     exponent floor log-uniform in [1e-3, 0.5] and lattice spacing U[0.7, 1.4] bohr so the
     conditioning spans several decades; 4 dd N(0,1) right-hand sides each.  One persistent
@@ -143,11 +275,43 @@ def bench_batch(qp, torch, dev, count=512, nrhs=4, seed=3):
     torch.cuda.synchronize()
     ms = e0.elapsed_time(e1) / reps
     macs = float((sizes.astype(np.float64) ** 3).sum() / 6.0)
+    solve_macs = float(nrhs * (sizes.astype(np.float64) ** 2).sum())
     failed = int((st.cpu().numpy() != 0).sum())
-    return {"instances": count, "n_range": [int(sizes.min()), int(sizes.max())], "nrhs": nrhs,
-            "ms_graph_replay": round(ms, 2), "ms_eager_first_call": round(eager_ms, 2),
-            "factor_macs": macs, "dd_gflops": round(2.0 * macs / (ms * 1e-3) / 1e9, 1),
-            "breakdowns": failed, "matrix_bytes": 2 * 16 * tot}
+    ach = 2.0 * DD_MAC_OPS * (macs + solve_macs) / (ms * 1e-3) / 1e12
+    out = {"instances": count, "n_range": [int(sizes.min()), int(sizes.max())], "nrhs": nrhs,
+           "ms_graph_replay": round(ms, 2), "ms_eager_first_call": round(eager_ms, 2),
+           "factor_macs": macs, "dd_gflops": round(2.0 * macs / (ms * 1e-3) / 1e9, 1),
+           "breakdowns": failed, "matrix_bytes": 2 * 16 * tot,
+           "roofline": {"bound": "valu", "kernel": "k_batch_chol_solve", "achieved": round(ach, 3),
+                        "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s", "frac": round(ach / FP64_PEAK_TFLOPS, 4),
+                        "note": "(sum n^3/6 factor + nrhs sum n^2 solve) dd MACs x 29 FP64 ops (FMA-equivalent) "
+                                "over the replay; one workgroup per instance, so the largest instances' single-CU "
+                                "time bounds the replay"}}
+    if cpu:
+        O = oracle()
+        pick = list(range(0, count, count // 16))[:16]
+        mats = []
+        for i in pick:
+            c, a = qp.gen_overlap_basis(seed * 100000 + i, int(sizes[i]), float(lo[i]), 10.0, float(sp[i]))
+            mats.append(O.overlap_s_dd(c, a))
+        rhs = [qp.gen_dd_normal(99 + i, int(sizes[i])) for i in pick]
+        t0 = time.perf_counter()
+        for j, i in enumerate(pick):
+            n = int(sizes[i])
+            l = O.cholesky_blocked(mats[j], n, 64)
+            for _ in range(nrhs):
+                O.solve_cholesky(l, rhs[j], n)
+        t = time.perf_counter() - t0
+        smacs = float(sum(float(sizes[i]) ** 3 / 6.0 for i in pick))
+        rate = 2.0 * smacs / t / 1e9
+        out["cpu_baseline"] = {"value": round(rate, 3), "unit": "dd-GFLOP/s (factor MACs)",
+                               "est_ms_full_batch": round(2.0 * macs / (rate * 1e9) * 1e3, 1),
+                               "cores": O.lib().hpo_num_threads(), "kind": "port",
+                               "sample": f"16 of the {count} instances (every {count // 16}th, n "
+                                         f"{min(int(sizes[i]) for i in pick)}..{max(int(sizes[i]) for i in pick)}): "
+                                         f"oracle cholesky_blocked(nb=64) + {nrhs} solve_cholesky each ({t:.1f} s), "
+                                         "OpenMP rows"}
+    return out
 
 
 MF4_N = 1024
@@ -252,7 +416,8 @@ def bench_eigen(qp, torch, dev, cpu):
         t = time.perf_counter() - t0
         out["cpu_baseline"] = {"value": round(t * 1e3, 1), "unit": "ms", "cores": 1, "kind": "port",
                                "sample": f"the full solve at N={nc} (max-pivot Jacobi as ortho.rs:61-152, "
-                                         f"oracle C++), against {ms_g:.1f} ms on the GPU at the same N"}
+                                         f"oracle C++; one core: the reference's Jacobi sweeps are serial), "
+                                         f"against {ms_g:.1f} ms on the GPU at the same N"}
     return out
 
 
@@ -325,9 +490,10 @@ def bench_text(qp, torch, dev, cpu, reps=5):
         t2 = time.perf_counter()
         assert nbad == 0 and np.array_equal(back, v)
         out["cpu_baseline"] = {"value": round(m / (t1 - t0) / 1e6, 3), "unit": "Mvalues/s (format)",
-                               "parse_Mvalues_per_s": round(m / (t2 - t1) / 1e6, 3), "cores": 1, "kind": "port",
+                               "parse_Mvalues_per_s": round(m / (t2 - t1) / 1e6, 3),
+                               "cores": O.lib().hpo_num_threads(), "kind": "port",
                                "sample": f"the first {m} values, oracle C++ (glibc printf / strtod behind the "
-                                         "reference's grammar and split rules), one thread"}
+                                         "reference's grammar and split rules), OpenMP over values"}
     return out
 
 
@@ -404,9 +570,9 @@ def bench_overlap_mixed(qp, torch, dev, cpu, reps=3):
         O.assemble_overlap_mixed(small, O.DEFAULT_SCREEN)
         tc = time.perf_counter() - t0
         out["cpu_baseline"] = {"value": round(n_small * (n_small + 1) / 2 / tc / 1e6, 3), "unit": "Melements/s",
-                               "cores": 1, "kind": "port",
+                               "cores": O.lib().hpo_num_threads(), "kind": "port",
                                "sample": f"{OVL_CPU_ATOMS} atoms (n_ao {n_small}) of the same kind, oracle C++, "
-                                         f"one thread ({tc:.2f} s)"}
+                                         f"OpenMP over shell pairs as the reference's par_iter ({tc:.2f} s)"}
     return out
 
 
@@ -516,8 +682,9 @@ def bench_mf4_chol(qp, torch, dev, cpu, reps=3):
         assert np.array_equal(ref, L)
         out["cpu_baseline"] = {"value": round((t1 - t0) * 1e3, 1), "unit": "ms (cholesky)",
                                "solve_ms": round((t2 - t1) * 1e3, 1), "cores": 1, "kind": "port",
-                               "sample": f"the same n = {n} factor and solve, oracle C++, one thread; factor "
-                                         "bit-identical to the GPU's"}
+                               "sample": f"the same n = {n} factor and solve, oracle C++, one core (the reference's "
+                                         "mf_cholesky and its solves are serial loops); factor bit-identical to the "
+                                         "GPU's"}
     return out
 
 
@@ -564,6 +731,19 @@ def bench_adjoint(qp, torch, dev, cpu):
                       "inverse_ms": round(best[3], 3), "gram_ms": round(best[4], 3),
                       "contract_ms": round(best[5], 3),
                       "backward_dd_gflops": round(2.0 * bw_macs / (best[2] * 1e-3) / 1e9, 1)}
+        if gm == qp.HP_GEMM_PLAIN:
+            ach = 2.0 * DD_MAC_OPS * (n ** 3 / 6.0 + bw_macs) / (best[0] * 1e-3) / 1e12
+            out[label]["roofline"] = {"bound": "valu", "achieved": round(ach, 3), "peak": FP64_PEAK_TFLOPS,
+                                      "unit": "TFLOP/s", "frac": round(ach / FP64_PEAK_TFLOPS, 4),
+                                      "note": "factor n^3/6 + L^-1 n^3/6 + S^-1 n^3/6 dd MACs x 29 FP64 ops "
+                                              "(FMA-equivalent) over the whole forward + backward"}
+        else:
+            wb = split_factor_work_ms(n) + (DD_MAC_OPS * n ** 3 / 6.0 / (FP64_PEAK_TFLOPS / 2 * 1e12)
+                                            + 2 * SLICES * n ** 3 / 6.0 / (INT8_PEAK_TOPS * 1e12)) * 1e3
+            out[label]["roofline"] = {"bound": "latency", "work_bound_ms": round(wb, 3),
+                                      "frac": round(wb / best[0], 4),
+                                      "note": "frac = the work at peak rates (split factor as in spd_solve, L^-1 "
+                                              "on FP64 VALU, S^-1 on the INT8 matrix cores) over the total"}
     qp.set_phase_timing(False)
     if cpu:
         sys.path.insert(0, os.path.join(ROOT, "tests"))
@@ -575,6 +755,159 @@ def bench_adjoint(qp, torch, dev, cpu):
                                "kind": "port", "sample": "the full N=2048 gradient pair: dd S build, "
                                "cholesky_blocked, S^-1 by solve_cholesky per column, contraction (oracle C++, "
                                "OpenMP over columns / rows)"}
+    return out
+
+
+def bench_config0(qp, torch, cpu, reps=5):
+    """Config 0, the reference's CPU-runnable case, through the drop-in host entry points (host
+    arrays in and out, PCIe included): N=512 lattice Gaussians (seed 0, exponents in [0.01, 10])
+    -> dd overlap S (hp_overlap_s) -> cholesky_blocked(nb=64) -> solve_cholesky with b = ones;
+    the solution must be the oracle's bit for bit."""
+    import numpy as np
+    n = 512
+    c, a = qp.gen_overlap_basis(0, n, 0.01, 10.0, 1.4)
+    ones = np.zeros((n, 2))
+    ones[:, 0] = 1.0
+    st = {}
+
+    def run():
+        t0 = time.perf_counter()
+        s_ = qp.overlap_s(c, a)
+        t1 = time.perf_counter()
+        l_ = qp.cholesky_blocked(s_, n, 64)
+        t2 = time.perf_counter()
+        x_ = qp.solve_cholesky(l_, ones, n)
+        t3 = time.perf_counter()
+        return (t1 - t0, t2 - t1, t3 - t2), x_, l_
+
+    run()
+    best, x, l = None, None, None
+    for _ in range(reps):
+        tt, x, l = run()
+        best = tt if best is None or sum(tt) < sum(best) else best
+    kappa = qp.cond_estimate(l, n)
+    ach = 2.0 * DD_MAC_OPS * n ** 3 / 6.0 / best[1] / 1e12
+    out = {"workload": "config 0: N=512 s Gaussians (8^3 lattice, 1.4 bohr, seed 0, alpha in [0.01, 10]) -> dd S "
+                       "-> cholesky_blocked(nb=64) -> solve_cholesky(L, ones), host entry points",
+           "ms": round(sum(best) * 1e3, 3), "overlap_ms": round(best[0] * 1e3, 3),
+           "cholesky_blocked_ms": round(best[1] * 1e3, 3), "solve_cholesky_ms": round(best[2] * 1e3, 3),
+           "kappa_est": float(f"{kappa:.3e}"),
+           "roofline": {"bound": "latency", "achieved": round(ach, 3), "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
+                        "frac": round(ach / FP64_PEAK_TFLOPS, 4),
+                        "note": "the factor's n^3/6 dd MACs x 29 FP64 ops over cholesky_blocked_ms (PCIe staging "
+                                "of 4 MB included); at N=512 the 8 diagonal blocks' pivot chains dominate"}}
+    if cpu:
+        O = oracle()
+        t0 = time.perf_counter()
+        s_o = O.overlap_s_dd(c, a)
+        t1 = time.perf_counter()
+        l_o = O.cholesky_blocked(s_o, n, 64)
+        t2 = time.perf_counter()
+        x_o = O.solve_cholesky(l_o, ones, n)
+        t3 = time.perf_counter()
+        out["bit_identical_to_oracle"] = bool(np.array_equal(x.view(np.uint64), x_o.view(np.uint64)))
+        out["cpu_baseline"] = {"value": round((t3 - t0) * 1e3, 2), "unit": "ms",
+                               "overlap_ms": round((t1 - t0) * 1e3, 2), "cholesky_blocked_ms": round((t2 - t1) * 1e3, 2),
+                               "solve_cholesky_ms": round((t3 - t2) * 1e3, 2), "cores": O.lib().hpo_num_threads(),
+                               "kind": "port", "sample": "the whole config, oracle C++ (OpenMP rows)"}
+    return out
+
+
+OVL_S_NS = (512, 2048, 4096)
+
+
+def bench_overlap_s(qp, torch, dev, cpu, reps=5):
+    """The dd overlap build (k_overlap_s): S_uv = (4 a_u a_v / (a_u + a_v)^2)^{3/4}
+    exp(-a_u a_v / (a_u + a_v) |R_u - R_v|^2) for lattice Gaussians (seed 0, exponents in
+    [0.01, 10]) on centres / exponents resident in HBM; bit-identical to the oracle
+    (tests/test_gpu_linalg.py, tests/test_gpu_config_sizes.py)."""
+    import numpy as np
+    O = oracle()
+    out = {"workload": "dd Gaussian overlap S, N in (512, 2048, 4096), lattice 1.4 bohr, seed 0, alpha in [0.01, 10]"}
+    for n in OVL_S_NS:
+        c, a = qp.gen_overlap_basis(0, n, 0.01, 10.0, 1.4)
+        dc = torch.from_numpy(c.ravel()).to(dev)
+        da = torch.from_numpy(a).to(dev)
+        S = torch.empty((2, n * n), dtype=torch.float64, device=dev)
+        ms = best_of(lambda: qp.dev.overlap_s(dc, da, S, n), reps, torch)
+        out[f"n{n}"] = {"ms": round(ms, 4), "Mentries_per_s": round(n * (n + 1) / 2 / (ms * 1e-3) / 1e6, 1)}
+    # FP64 operations per entry from the oracle's op-counting restatement, mean over a sample
+    n = OVL_S_NS[-1]
+    c, a = qp.gen_overlap_basis(0, n, 0.01, 10.0, 1.4)
+    rng = np.random.default_rng(5)
+    ii, jj = rng.integers(0, n, 4000), rng.integers(0, n, 4000)
+    w = float(np.mean([O.count_overlap_entry(a[i], c[i], a[j], c[j]) for i, j in zip(ii, jj) if i != j]))
+    ms = out[f"n{n}"]["ms"]
+    ach = 2.0 * w * n * (n - 1) / 2 / (ms * 1e-3) / 1e12
+    out["ops_per_entry"] = round(w, 1)
+    out["roofline"] = {"bound": "valu", "kernel": "k_overlap_s", "achieved": round(ach, 3), "peak": FP64_PEAK_TFLOPS,
+                       "unit": "TFLOP/s", "frac": round(ach / FP64_PEAK_TFLOPS, 4),
+                       "note": f"N=4096: N(N-1)/2 off-diagonal entries x {w:.0f} FP64 ops (the op-counting "
+                               "restatement's mean; a dd division counts its IEEE divisions as one op each), "
+                               "FMA-equivalent; HBM writes 16 B per stored entry"}
+    if cpu:
+        t0 = time.perf_counter()
+        O.overlap_s_dd(c, a)
+        t = time.perf_counter() - t0
+        out["cpu_baseline"] = {"value": round(n * (n + 1) / 2 / t / 1e6, 2), "unit": "Mentries/s",
+                               "ms": round(t * 1e3, 1), "cores": O.lib().hpo_num_threads(), "kind": "port",
+                               "sample": "the full N=4096 build, oracle C++ (OpenMP rows)"}
+    return out
+
+
+def bench_gemm_atb(qp, torch, dev, steps, cpu):
+    """Config 1: gemm_atb C = A^T B at m=n=k=4096 (A stored k x m; hi N(0,1), seed 1),
+    plain (the reference's order, bit-identical) and split (HP_GEMM_AUTO at 104 bits)."""
+    m = n = k = 4096
+    a = qp.gen_dd_normal(1, k * m)
+    b = qp.gen_dd_normal(1 + 0x5EED, k * n)
+    da, db = qp.dev.to_planes(a, 2, dev), qp.dev.to_planes(b, 2, dev)
+    dc = torch.empty((2, m * n), dtype=torch.float64, device=dev)
+    stream = torch.cuda.current_stream()
+    macs = float(m) * n * k
+    ms_p, kern_p = time_steps(lambda: qp.dev.gemm(da, db, dc, m, n, k, trans_a=True, mode=qp.HP_GEMM_PLAIN), 2, 1,
+                              stream, torch)
+    ms_s, kern_s = time_steps(lambda: qp.dev.gemm(da, db, dc, m, n, k, trans_a=True, mode=qp.HP_GEMM_AUTO,
+                                                  slices=DD_BITS), max(3, min(steps, 10)), 2, stream, torch)
+    plan = qp.gemm_plan(m, n, k, DD_BITS, trans_a=True)
+    ach_p = 2.0 * DD_MAC_OPS * macs / (kern_p * 1e-3) / 1e12
+    out = {"workload": "dd gemm_atb C=A^T B, m=n=k=4096, A stored k x m, seed 1",
+           "plain": {"ms": round(ms_p, 3), "GFLOP_s": round(2.0 * macs / (ms_p * 1e-3) / 1e9, 1),
+                     "roofline": {"bound": "valu", "kernel": "k_gemm_dd_plain<true>", "achieved": round(ach_p, 3),
+                                  "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
+                                  "frac": round(ach_p / FP64_PEAK_TFLOPS, 4)}},
+           "split": {"ms": round(ms_s, 3), "GFLOP_s": round(2.0 * macs / (ms_s * 1e-3) / 1e9, 1),
+                     "bits": DD_BITS, "slices": plan.slices if plan else None, "w_bits": plan.w_bits if plan else None,
+                     "variant": plan.variant if plan else None,
+                     "note": "whole step (scales, residues, int8 GEMMs, CRT); the int8 kernel's roofline is the "
+                             "headline's"}}
+    out["split_over_plain"] = round(ms_p / ms_s, 2)
+    if cpu:
+        O = oracle()
+        threads = O.lib().hpo_num_threads()
+        rows = threads * 8
+        t0 = time.perf_counter()
+        O.gemm_atb_rows(a, b, m, n, k, 0, rows)
+        t = time.perf_counter() - t0
+        out["cpu_baseline"] = {"value": round(2.0 * rows * n * k / t / 1e9, 4), "unit": "GFLOP/s", "cores": threads,
+                               "kind": "port", "sample": f"{rows} of {m} rows ({t:.1f} s), oracle C++ restatement of "
+                                                         "gemm_atb (both operands strided), OpenMP rows"}
+    return out
+
+
+def bench_autotune(qp, torch, dev, da, db, dc, steps):
+    """Config 1, the autotuner: HP_GEMM_AUTO at 4096^3 for several requested accuracies --
+    the moduli (int8 GEMMs) it takes for each and the time of a whole step."""
+    stream = torch.cuda.current_stream()
+    out = {}
+    for bits in (53, 80, 96, 104):
+        ms, _ = time_steps(lambda: qp.dev.gemm(da, db, dc, M, N, K, mode=qp.HP_GEMM_AUTO, slices=bits),
+                           max(3, min(steps, 10)), 2, stream, torch)
+        plan = qp.gemm_plan(M, N, K, bits)
+        out[f"bits{bits}"] = {"mode": "split" if plan.mode == qp.HP_GEMM_SPLIT_I8 else "plain",
+                              "slices": plan.slices, "w_bits": plan.w_bits, "variant": plan.variant,
+                              "ms": round(ms, 3), "GFLOP_s": round(2.0 * M * N * K / (ms * 1e-3) / 1e9, 1),
+                              "bound_rel_to_norm": f"2^-{bits}"}
     return out
 
 
@@ -608,6 +941,10 @@ def main():
     ap.add_argument("--no-overlap", action="store_true", help="skip the mixed-tier overlap assembly")
     ap.add_argument("--no-reduce", action="store_true", help="skip dot / gemv")
     ap.add_argument("--no-mfchol", action="store_true", help="skip the MF4 Cholesky")
+    ap.add_argument("--no-config0", action="store_true", help="skip config 0 (N=512 overlap, factor, solve)")
+    ap.add_argument("--no-overlap-s", action="store_true", help="skip the dd overlap build")
+    ap.add_argument("--no-atb", action="store_true", help="skip gemm_atb at 4096^3")
+    ap.add_argument("--no-autotune", action="store_true", help="skip the autotuner's accuracy sweep")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only copy MIB MiB device-to-device three times (a plain copy kernel of known size, "
                          "to calibrate rocprofv3's FETCH_SIZE / WRITE_SIZE counters) and exit")
@@ -643,7 +980,7 @@ def main():
     qp.set_phase_timing(2)
 
     def split_step():
-        qp.dev.gemm(da, db, dc, M, N, K, mode=qp.HP_GEMM_SPLIT_I8, slices=SLICES)
+        qp.dev.gemm(da, db, dc, M, N, K, mode=qp.HP_GEMM_AUTO, slices=DD_BITS)
 
     for _ in range(args.warmup):
         split_step()
@@ -654,7 +991,9 @@ def main():
     qp.set_phase_timing(False)
     ph = [x / calls for x in sums]  # prep, int8 gemms, crt
     gemm_ms = ph[1]
-    variant = qp.split_gemm_variant(M, N, K, SLICES)
+    plan = qp.gemm_plan(M, N, K, DD_BITS)
+    assert plan is not None and plan.mode == qp.HP_GEMM_SPLIT_I8 and plan.slices == SLICES, "autotuner plan"
+    variant = plan.variant
     achieved = 2.0 * SLICES * macs / (gemm_ms * 1e-3) / 1e12
 
     # plain path (reference op order) on the same operands
@@ -678,7 +1017,8 @@ def main():
         "vs_baseline": None,
         "dtype": "int8 residues (31-modulus CRT) -> double-double",
         "data": "synthetic: seeded N(0,1) dd operands, generated on the host and resident in HBM",
-        "config": {"workload": "dd gemm C=A*B, m=n=k=4096, row-major, seed 1; split = 31 int8 GEMMs + CRT",
+        "config": {"workload": "dd gemm C=A*B, m=n=k=4096, row-major, seed 1; split at dd accuracy (104 bits, "
+                               "autotuned: 31 moduli = 31 int8 GEMMs + CRT)",
                    "parallelism": "1 GPU"},
         "roofline": {"bound": "mfma", "kernel": I8_KERNELS.get(variant, str(variant)), "achieved": round(achieved, 2),
                      "peak": INT8_PEAK_TOPS, "unit": "TOP/s (int8)", "frac": round(achieved / INT8_PEAK_TOPS, 4),
@@ -686,7 +1026,7 @@ def main():
                      "traffic_source": "rocprofv3 --pmc FETCH_SIZE (x2.000) + WRITE_SIZE (x1.000) per launch, "
                                        "calibrated on a 256 MiB device copy; profiles/k_gemm_i8_mod_dlp8_4096.md",
                      "kernel_ms": round(gemm_ms, 4),
-                     "variant": variant,
+                     "variant": variant, "slices": plan.slices, "w_bits": plan.w_bits,
                      "phases_ms": {"scale_and_residues": round(ph[0], 4), "int8_gemms": round(ph[1], 4),
                                    "crt": round(ph[2], 4)}},
         "plain": {"value": round(2.0 * macs / (ms_plain * 1e-3) / 1e9, 3), "unit": "GFLOP/s",
@@ -696,10 +1036,18 @@ def main():
                                "note": "29 FP64 VALU ops per dd MAC, FMA-equivalent (2 flops each)"}},
         "split_over_plain": round(ms_plain / ms_split, 2),
     }
+    if not args.no_autotune:
+        out["autotune_4096"] = bench_autotune(qp, torch, dev, da, db, dc, args.steps)
+    if not args.no_atb:
+        out["gemm_atb_4096"] = bench_gemm_atb(qp, torch, dev, args.steps, not args.no_cpu)
+    if not args.no_config0:
+        out["config0_n512"] = bench_config0(qp, torch, not args.no_cpu)
+    if not args.no_overlap_s:
+        out["overlap_s_dd"] = bench_overlap_s(qp, torch, dev, not args.no_cpu)
     if not args.no_solve:
-        out["spd_solve_n4096"] = bench_solves(qp, torch, dev)
+        out["spd_solve_n4096"] = bench_solves(qp, torch, dev, not args.no_cpu)
     if not args.no_batch:
-        out["batched_solves"] = bench_batch(qp, torch, dev)
+        out["batched_solves"] = bench_batch(qp, torch, dev, not args.no_cpu)
     if not args.no_mf4:
         out["mf4_gemm_1024"] = bench_mf4(qp, torch, dev, args.steps, not args.no_cpu)
     if not args.no_adjoint:
@@ -716,6 +1064,7 @@ def main():
         out["mf4_cholesky_256"] = bench_mf4_chol(qp, torch, dev, not args.no_cpu)
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(a, b)
+        out["host_cpu"] = host_cpu()
     print(json.dumps(out), flush=True)
 
 

@@ -1039,15 +1039,19 @@ size_t hpo_assemble_overlap_mixed(const double* centers, const int* l, const lon
   for (size_t i = 0; i < n_shells; ++i) off[i + 1] = off[i] + (l[i] + 1) * (l[i] + 2) / 2;
   const size_t n_ao = (size_t)off[n_shells];
   std::memset(s, 0, sizeof(double) * n_ao * n_ao);
-  counts[0] = counts[1] = counts[2] = 0;
-  double block[100];
-  for (size_t i = 0; i < n_shells; ++i)
+  size_t c0 = 0, c1 = 0, c2 = 0;
+  // parallel over the lower-triangle pairs as the reference's par_iter (assemble.rs:45-55);
+  // each pair writes its own two blocks, so the result does not depend on the schedule
+#pragma omp parallel for schedule(dynamic, 4) reduction(+ : c0, c1, c2)
+  for (long ii = 0; ii < (long)n_shells; ++ii) {
+    const size_t i = (size_t)ii;
+    double block[100];
     for (size_t j = 0; j <= i; ++j) {
       ShellView A{centers + 3 * i, l[i], prim_off[i], prim_off[i + 1]};
       ShellView B{centers + 3 * j, l[j], prim_off[j], prim_off[j + 1]};
       int tier = screen_pair(A, B, alpha, params4);
       shell_pair_block(A, B, alpha, coeff, tier, block);
-      counts[tier] += 1;
+      (tier == 0 ? c0 : tier == 1 ? c1 : c2) += 1;
       if (tier == 2) continue;
       const int ni = (l[i] + 1) * (l[i] + 2) / 2, nj = (l[j] + 1) * (l[j] + 2) / 2;
       for (int u = 0; u < ni; ++u)
@@ -1057,6 +1061,10 @@ size_t hpo_assemble_overlap_mixed(const double* centers, const int* l, const lon
           if (i != j) s[(off[j] + v) * n_ao + (off[i] + u)] = val;
         }
     }
+  }
+  counts[0] = c0;
+  counts[1] = c1;
+  counts[2] = c2;
   return n_ao;
 }
 
@@ -1590,17 +1598,23 @@ int hpo_mf4_deserialize_str(const char* s, size_t len, double* out) {
   for (int k = 0; k < 4; ++k) out[k] = limbs[k];
   return 0;
 }
+// Bulk forms: each value is formatted / parsed independently (OpenMP over values): the
+// strings go to 64-byte slots, then are packed behind their prefix-summed offsets.
 size_t hpo_dd_serialize_n(const double* v, size_t n, char* text, size_t* offsets) {
-  size_t o = 0;
+  std::vector<char> slots(n * 64);
+  std::vector<unsigned char> len(n);
+#pragma omp parallel for schedule(static)
+  for (long i = 0; i < (long)n; ++i) len[i] = (unsigned char)hpo_dd_serialize_str(v + 2 * i, slots.data() + 64 * i);
   offsets[0] = 0;
-  for (size_t i = 0; i < n; ++i) {
-    o += (size_t)hpo_dd_serialize_str(v + 2 * i, text + o);
-    offsets[i + 1] = o;
-  }
-  return o;
+  for (size_t i = 0; i < n; ++i) offsets[i + 1] = offsets[i] + len[i];
+#pragma omp parallel for schedule(static)
+  for (long i = 0; i < (long)n; ++i) std::memcpy(text + offsets[i], slots.data() + 64 * i, len[i]);
+  return offsets[n];
 }
 size_t hpo_dd_deserialize_n(const char* text, const size_t* offsets, size_t n, double* out) {
   size_t bad = 0;
-  for (size_t i = 0; i < n; ++i) bad += hpo_dd_deserialize_str(text + offsets[i], offsets[i + 1] - offsets[i], out + 2 * i) != 0;
+#pragma omp parallel for schedule(static) reduction(+ : bad)
+  for (long i = 0; i < (long)n; ++i)
+    bad += hpo_dd_deserialize_str(text + offsets[i], offsets[i + 1] - offsets[i], out + 2 * i) != 0;
   return bad;
 }
