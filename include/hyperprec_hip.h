@@ -25,7 +25,14 @@
  * CholeskyError{pivot, value} (linalg.rs:196-208), is HP_ENOTSPD with the pivot and the
  * f64 value of the failing diagonal written through the out-parameters.
  *
- * All functions are thread-safe; they keep no state between calls other than HIP's.
+ * All functions are thread-safe.  State kept between calls, all process-wide and guarded:
+ *   - the split GEMM's device workspaces (two slots, grown on demand, each ordered by a
+ *     completion event) and its CRT tables;
+ *   - the autotuner's choices (int8 kernel per shape class and moduli count, the
+ *     HP_GEMM_AUTO plan per shape class and accuracy), under a mutex;
+ *   - the split-dd factorisation's panel width (hp_set_split_factor_panel, atomic);
+ *   - the per-thread error message and the phase timer of hp_set_phase_timing.
+ * Everything else is allocated per call and freed before the call returns.
  */
 #ifndef HYPERPREC_HIP_H
 #define HYPERPREC_HIP_H
@@ -272,12 +279,12 @@ int hp_dev_overlap_s(const double* centers, const double* alpha, size_t n, doubl
  * otherwise); counts: TierCounts {f64_pairs, f128_pairs, skipped_pairs}.  Each shell pair
  * is screened and its block formed in f64 or dd (integral.rs:65-346) exactly as the
  * reference; only libm exp may differ in its last bit. */
-int hp_assemble_overlap_mixed(const double* centers, const int* l, const long* prim_off, const double* alpha,
+int hp_assemble_overlap_mixed(const double* centers, const int* l, const int64_t* prim_off, const double* alpha,
                               const double* coeff, size_t n_shells, const double* params4, double* s, size_t n_ao,
                               size_t* counts);
 /* the same with S left in device memory (s_dev: n_ao x n_ao f64), for device-resident
  * consumers such as the factorizations; shell data and counts are host memory */
-int hp_dev_assemble_overlap_mixed(const double* centers, const int* l, const long* prim_off, const double* alpha,
+int hp_dev_assemble_overlap_mixed(const double* centers, const int* l, const int64_t* prim_off, const double* alpha,
                                   const double* coeff, size_t n_shells, const double* params4, double* s_dev,
                                   size_t n_ao, size_t* counts, void* stream);
 

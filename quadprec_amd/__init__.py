@@ -215,12 +215,23 @@ def _arr(x, limbs, count, name):
     return a
 
 
+def _out(out, limbs, count, name="out"):
+    """A caller-supplied output: float64, C-contiguous, writable and exactly count values of
+    limbs doubles (the library writes that many), else ShapeError."""
+    if out is None:
+        return np.zeros((count, limbs))
+    if not isinstance(out, np.ndarray) or out.dtype != np.float64 or not out.flags.c_contiguous \
+            or not out.flags.writeable or out.size != limbs * count:
+        raise ShapeError(f"{name}: expected a writable C-contiguous float64 array of {limbs * count} doubles")
+    return out
+
+
 # ---------------------------------------------------------------- GEMM (linalg.rs:150-182)
 def gemm(a, b, m, n, k, out=None):
     """C = A*B in double-double (crates/quad/src/linalg.rs:150-165). Returns C (m*n, 2)."""
     a = _arr(a, 2, m * k, "a")
     b = _arr(b, 2, k * n, "b")
-    c = np.zeros((m * n, 2)) if out is None else out
+    c = _out(out, 2, m * n)
     _check(lib().hp_gemm(a, b, c, m, n, k), "gemm")
     return c
 
@@ -229,7 +240,7 @@ def gemm_atb(a, b, m, n, k, out=None):
     """C = A^T*B with A stored k x m (linalg.rs:168-182)."""
     a = _arr(a, 2, k * m, "a")
     b = _arr(b, 2, k * n, "b")
-    c = np.zeros((m * n, 2)) if out is None else out
+    c = _out(out, 2, m * n)
     _check(lib().hp_gemm_atb(a, b, c, m, n, k), "gemm_atb")
     return c
 
@@ -238,7 +249,7 @@ def mf_gemm(a, b, m, n, k, out=None):
     """MultiFloat<4> C = A*B (crates/quad/src/multifloat/linalg.rs:39-61)."""
     a = _arr(a, 4, m * k, "a")
     b = _arr(b, 4, k * n, "b")
-    c = np.zeros((m * n, 4)) if out is None else out
+    c = _out(out, 4, m * n)
     _check(lib().hp_mf_gemm(a, b, c, m, n, k), "mf_gemm")
     return c
 
@@ -247,7 +258,7 @@ def mf_gemm_atb(a, b, m, n, k, out=None):
     """MultiFloat<4> C = A^T*B, A stored k x m (multifloat/linalg.rs:64-85)."""
     a = _arr(a, 4, k * m, "a")
     b = _arr(b, 4, k * n, "b")
-    c = np.zeros((m * n, 4)) if out is None else out
+    c = _out(out, 4, m * n)
     _check(lib().hp_mf_gemm_atb(a, b, c, m, n, k), "mf_gemm_atb")
     return c
 

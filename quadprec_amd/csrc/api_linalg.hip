@@ -402,7 +402,7 @@ int hp_set_split_factor_panel(int nb) {
     set_error("split factor panel width must be 64, 128, 256 or 512");
     return HP_EINVAL;
   }
-  g_split_panel = nb;
+  g_split_panel.store(nb);
   return HP_OK;
 }
 

@@ -43,12 +43,14 @@ struct ArDD {
     return mkdd(__shfl(v.hi, lane, 64), __shfl(v.lo, lane, 64));
   }
   // sqrt(d) and 1/sqrt(d) by one dd Newton step, y = y0 + y0 (1 - d y0^2) / 2, sqrt = d y,
-  // from y0 = the hardware reciprocal square root of d.hi refined by one binary64 Newton
-  // step (relative error ~2^-52 before the dd step, ~2^-104 after).  No division (the
+  // from y0 = the hardware reciprocal square root of d.hi refined by two binary64 Newton
+  // steps: each at least doubles the correct bits, so any seed better than 2^-14 reaches
+  // binary64 rounding (~2^-52) before the dd step and ~2^-104 after.  No division (the
   // reference takes sqrt, then recip: dd.rs:61-83); used where reference order is not
   // required.
   __device__ static void sqrt_recip_fast(T d, T& root, T& inv) {
     double r = __builtin_amdgcn_rsq(d.hi);
+    r = r * __builtin_fma(-(0.5 * d.hi) * r, r, 1.5);
     r = r * __builtin_fma(-(0.5 * d.hi) * r, r, 1.5);
     const dd y0 = mkdd(r);
     const dd e = dd_sub(mkdd(1.0), dd_mul(d, dd_mul(y0, y0)));

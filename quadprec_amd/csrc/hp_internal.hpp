@@ -2,6 +2,7 @@
 // thread-local error message, HIP error checks and device scratch buffers.
 #pragma once
 #include <hip/hip_runtime.h>
+#include <atomic>
 #include <utility>
 #include <vector>
 
@@ -113,7 +114,7 @@ int plain_bits_for_k(size_t k);
 int gemm_plan_get(int trans_a, size_t m, size_t n, size_t k, int bits, hp_gemm_plan* plan);
 int gemm_auto(int trans_a, const double* ahi, const double* alo, const double* bhi, const double* blo, double* chi,
               double* clo, size_t m, size_t n, size_t k, int bits, hipStream_t s);
-extern int g_split_panel;
+extern std::atomic<int> g_split_panel;
 int solve_refined(int kind, const double* sh, const double* sl, const double* bh, const double* bl, long n, double tol,
                   int max_iter, double* xh, double* xl, hp_refine_stats* st, hipStream_t s);
 int split_gemm_variant(size_t m, size_t n, size_t k, int R);

@@ -380,7 +380,7 @@ int adjoint_forward(const double* centers, const double* alpha, const double* bh
   ADJ_HIP(hipMemsetAsync(err.p, 0, sizeof(ErrRec), s));
   ErrRec* er = (ErrRec*)err.p;
   rc = factor_kind == HP_FACTOR_DD ? chol_exact<ArDD>(Lm, n, 64, false, Am, er, inv.d(), inv.d() + 64, s)
-                                   : chol_split(Lm, n, g_split_panel, Am, er, inv.d(), inv.d() + 64, s);
+                                   : chol_split(Lm, n, g_split_panel.load(), Am, er, inv.d(), inv.d() + 64, s);
   if (rc) return fail(rc);
   ErrRec e;
   ADJ_HIP(hipMemcpyAsync(&e, er, sizeof(e), hipMemcpyDeviceToHost, s));

@@ -468,7 +468,7 @@ using namespace hp;
 namespace {
 // s_dev: the caller's device matrix (n_ao x n_ao), or null to assemble into a scratch
 // buffer and copy the result to s_host
-int assemble_mixed(const double* centers, const int* l, const long* prim_off, const double* alpha,
+int assemble_mixed(const double* centers, const int* l, const int64_t* prim_off, const double* alpha,
                    const double* coeff, size_t n_shells, const double* params4, double* s_host, double* s_dev,
                    size_t n_ao, size_t* counts, hipStream_t st) {
   if (!counts || !params4) {
@@ -493,7 +493,7 @@ int assemble_mixed(const double* centers, const int* l, const long* prim_off, co
     for (int k = 0; k < 3; ++k) r.c[k] = centers[3 * i + k];
     r.amin = INFINITY;
     r.amax = 0.0;
-    for (long p = prim_off[i]; p < prim_off[i + 1]; ++p) {
+    for (int64_t p = prim_off[i]; p < prim_off[i + 1]; ++p) {
       r.amin = std::fmin(r.amin, alpha[p]);
       r.amax = std::fmax(r.amax, alpha[p]);
     }
@@ -509,7 +509,7 @@ int assemble_mixed(const double* centers, const int* l, const long* prim_off, co
     set_error("assemble_overlap_mixed: n_ao = %zu, the shells give %ld", n_ao, off);
     return HP_ESHAPE;
   }
-  const long np = prim_off[n_shells];
+  const long np = (long)prim_off[n_shells];
   const double prm[4] = {params4[0], params4[1], -std::log(params4[2]), params4[3]};
   DevBuf dsh, dao, dal, dco, dprm, ds, dcnt;
   int rc;
@@ -583,7 +583,7 @@ int assemble_mixed(const double* centers, const int* l, const long* prim_off, co
 
 extern "C" {
 
-int hp_assemble_overlap_mixed(const double* centers, const int* l, const long* prim_off, const double* alpha,
+int hp_assemble_overlap_mixed(const double* centers, const int* l, const int64_t* prim_off, const double* alpha,
                               const double* coeff, size_t n_shells, const double* params4, double* s, size_t n_ao,
                               size_t* counts) {
   if (n_shells && !s) {
@@ -594,7 +594,7 @@ int hp_assemble_overlap_mixed(const double* centers, const int* l, const long* p
                         hipStreamPerThread);
 }
 
-int hp_dev_assemble_overlap_mixed(const double* centers, const int* l, const long* prim_off, const double* alpha,
+int hp_dev_assemble_overlap_mixed(const double* centers, const int* l, const int64_t* prim_off, const double* alpha,
                                   const double* coeff, size_t n_shells, const double* params4, double* s_dev,
                                   size_t n_ao, size_t* counts, void* stream) {
   if (n_shells && !s_dev) {

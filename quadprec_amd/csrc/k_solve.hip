@@ -550,7 +550,7 @@ int chol_split(Mat a, long n, long nb, Mat S, ErrRec* err, double*, double*, hip
   return HP_OK;
 }
 
-int g_split_panel = 256;  // panel width of the split-dd factorisation
+std::atomic<int> g_split_panel{256};  // panel width of the split-dd factorisation (process-wide)
 
 // x = S^-1 b by corrections through a given factor L (f64 or dd by kind) and the inverses of
 // its 64 x 64 diagonal blocks: r = b - S x in quad-double, x += L^-T L^-1 r, until the
@@ -682,7 +682,7 @@ int solve_refined(int kind, const double* sh, const double* sl, const double* bh
   switch (kind) {
     case HP_FACTOR_F64: rc = chol_exact<ArF64>(Lm, n, 64, false, Sm, er, inv.d(), inv.d() + 64, s); break;
     case HP_FACTOR_DD: rc = chol_exact<ArDD>(Lm, n, 64, false, Sm, er, inv.d(), inv.d() + 64, s); break;
-    case HP_FACTOR_DD_SPLIT: rc = chol_split(Lm, n, g_split_panel, Sm, er, inv.d(), inv.d() + 64, s); break;
+    case HP_FACTOR_DD_SPLIT: rc = chol_split(Lm, n, g_split_panel.load(), Sm, er, inv.d(), inv.d() + 64, s); break;
     default: set_error("solve_refined: unknown factor kind %d", kind); return HP_EINVAL;
   }
   if (rc) return rc;

@@ -32,6 +32,7 @@
 #include <cstdint>
 #include <cstring>
 #include <map>
+#include <memory>
 #include <mutex>
 #include <vector>
 
@@ -1229,7 +1230,8 @@ int split_gemm_general(const SplitOperand& A, const SplitOperand& B, double* chi
                                (int)gemm_i8_rs_lds()));
   }
   if (g_work.tabR != R) {
-    static CrtTable host;
+    std::unique_ptr<CrtTable> staged(new CrtTable());  // per call: slots rebuild tables independently
+    CrtTable& host = *staged;
     build_table(R, host);
     if (!build_crt_bytes(R, host)) {
       set_error("split int8 gemm: CRT digit table for %d moduli does not fit 30 bytes", R);

@@ -45,3 +45,16 @@ def test_generators_are_deterministic_and_normalised():
     for l in m:
         for i in range(3):
             assert l[i] == 0 or abs(l[i + 1]) <= math.ulp(l[i]) / 2
+
+
+@pytest.mark.skipif(not os.path.exists(LIB), reason="library not built")
+def test_gemm_rejects_a_wrong_out_buffer_before_any_call():
+    """out= is checked on the host (size, dtype, layout) before the library could write past it."""
+    import numpy as np
+    import quadprec_amd as qp
+    a = np.zeros((6, 2))
+    for bad in (np.zeros((3, 2)), np.zeros((4, 2), dtype=np.float32), np.zeros((4, 4))[:, ::2], np.zeros((5, 2))):
+        with pytest.raises(qp.ShapeError):
+            qp.gemm(a, a[:6], 2, 2, 3, out=bad)
+    with pytest.raises(qp.ShapeError):
+        qp.mf_gemm(np.zeros((6, 4)), np.zeros((6, 4)), 2, 2, 3, out=np.zeros((4, 2)))
