@@ -615,9 +615,9 @@ def bench_reduce(qp, torch, dev, cpu, reps=5):
     td, tg = best(dot), best(gemv)
     out = {"workload": f"dd dot of {n} values, dd gemv {r} x {r} (gen_dd_normal seeds 7-10), in HBM",
            "dot": {"ms": round(td * 1e3, 3), "GB_per_s": round(32.0 * n / td / 1e9, 1),
-                   "note": "1024-element chunks, one thread each (the reference's order), then the left "
-                           "fold of the 16384 chunk sums on one wavefront: bound by the fold's dependent dd "
-                           "additions, not HBM"},
+                   "note": "1024-element chunks, one thread each (the reference's order), then the 16384 "
+                           "chunk sums combined by rayon's halving tree, pairwise with DPP / ds_swizzle in each "
+                           "wavefront: bound by the chunks' serial 1024-step dd chains, not HBM"},
            "gemv": {"ms": round(tg * 1e3, 3), "GB_per_s": round(16.0 * r * r / tg / 1e9, 1),
                     "roofline_frac_hbm": round(16.0 * r * r / tg / 8e12, 4),
                     "note": "one thread per row (the reference's order); 64 x 32 tiles through double-buffered "
@@ -632,9 +632,11 @@ def bench_reduce(qp, torch, dev, cpu, reps=5):
         ref = O.dot(ah, bh)
         t1 = time.perf_counter()
         assert np.array_equal(ref, o.cpu().numpy())
-        out["cpu_baseline"] = {"value": round(32.0 * n / (t1 - t0) / 1e9, 2), "unit": "GB/s (dot)", "cores": 1,
-                               "kind": "port", "sample": f"the same {n}-value dot, oracle C++, one thread "
-                                                         f"({t1 - t0:.2f} s), result bit-identical to the GPU's"}
+        out["cpu_baseline"] = {"value": round(32.0 * n / (t1 - t0) / 1e9, 2), "unit": "GB/s (dot)",
+                               "cores": O.lib().hpo_num_threads(), "kind": "port",
+                               "sample": f"the same {n}-value dot, oracle C++ (chunk sums over OpenMP threads as "
+                                         f"rayon's par_chunks, then the tree) ({t1 - t0:.2f} s), result bit-identical "
+                                         "to the GPU's"}
     return out
 
 

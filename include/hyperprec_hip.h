@@ -321,8 +321,9 @@ int hp_dev_solve_gen_eigen(const double* f_hi, const double* f_lo, const double*
 /* ------------------------------------------------------------------ reductions */
 
 /* dd dot product, replaces dot (crates/quad/src/linalg.rs:7-18): below 1024 elements the
- * serial sum from ZERO (linalg.rs:20-27); else the serial sums of 1024-element chunks folded
- * left to right from ZERO, one of the association orders of the reference's rayon reduce.
+ * serial sum from ZERO (linalg.rs:20-27); else the serial sums p_i of 1024-element chunks
+ * combined as the reference's rayon reduce does when every range is split at len / 2:
+ * tree(lo, 1) = ZERO + p_lo, tree(lo, len) = tree(lo, len/2) + tree(lo + len/2, len - len/2).
  * a, b: n dd (AoS); out_dd: 1 dd. */
 int hp_dot(const double* a, const double* b, size_t n, double* out_dd);
 int hp_dev_dot(const double* a_hi, const double* a_lo, const double* b_hi, const double* b_lo, size_t n,

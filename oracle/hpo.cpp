@@ -306,16 +306,29 @@ static D dot_serial(const double* a, const double* b, size_t n) {
   for (size_t i = 0; i < n; ++i) s = add(s, mul(ld(a, i), ld(b, i)));
   return s;
 }
-// dot: linalg.rs:7-18.  Below 1024 elements the serial sum; else serial sums of 1024-element
-// chunks combined by rayon's reduce from ZERO, whose association tree is unspecified: fixed
-// here as the left fold ((ZERO + p0) + p1) + ..., one of the trees rayon can produce.
+// dot: linalg.rs:7-18.  Below 1024 elements the serial sum; else serial sums p_i of
+// 1024-element chunks combined by rayon's reduce(|| ZERO, |acc, x| acc + x).  Rayon's
+// indexed bridge splits a range of len items at len / 2 (left = the first half) and folds
+// an unsplit range from ZERO, joining halves as left + right; which ranges stay unsplit is
+// a scheduling choice.  Fixed here as the split carried down to single items:
+//   tree(lo, 1)   = ZERO + p_lo
+//   tree(lo, len) = tree(lo, len / 2) + tree(lo + len / 2, len - len / 2)
+// a tree rayon produces when every range is split (e.g. enough idle workers stealing).
+static D dot_tree(const std::vector<D>& p, size_t lo, size_t len) {
+  if (len == 1) return add(D{0.0, 0.0}, p[lo]);
+  const size_t half = len / 2;
+  const D l = dot_tree(p, lo, half);
+  return add(l, dot_tree(p, lo + half, len - half));
+}
 hpo_dd hpo_dot(const double* a, const double* b, size_t n) {
   const size_t CHUNK = 1024;
   if (n < CHUNK) return to_c(dot_serial(a, b, n));
-  D acc{0.0, 0.0};
-  for (size_t c0 = 0; c0 < n; c0 += CHUNK)
-    acc = add(acc, dot_serial(a + 2 * c0, b + 2 * c0, std::min(CHUNK, n - c0)));
-  return to_c(acc);
+  const size_t m = (n + CHUNK - 1) / CHUNK;
+  std::vector<D> p(m);
+#pragma omp parallel for schedule(static)
+  for (long c = 0; c < (long)m; ++c)
+    p[c] = dot_serial(a + 2 * c * CHUNK, b + 2 * c * CHUNK, std::min(CHUNK, n - c * CHUNK));
+  return to_c(dot_tree(p, 0, m));
 }
 // gemv: linalg.rs:29-40 (y_i = dot_serial(row_i, x))
 void hpo_gemv(const double* a, size_t rows, size_t cols, const double* x, double* y) {

@@ -2,13 +2,14 @@
 // compensated sums (crates/compensated-sum/src/lib.rs:6-66) on the GPU, in the reference's
 // summation orders:
 //   * dot: below 1024 elements the serial dd sum; else one thread per 1024-element chunk
-//     (serial inside the chunk, as dot_serial) and a left fold of the chunk sums from ZERO,
-//     one of the association orders rayon's reduce may produce;
+//     (serial inside the chunk, as dot_serial) and the chunk sums combined by rayon's
+//     halving tree (split at len / 2 down to single chunks), reduced pairwise with DPP /
+//     ds_swizzle across each wavefront;
 //   * gemv: y_i = dot_serial(row_i, x), one thread per row; three loader wavefronts stage
 //     64 x 32 tiles through double-buffered LDS while the fourth runs the row chains;
 //   * par_neumaier_sum(_f64): one thread per 4096-element chunk, then the Neumaier sum of the
-//     partials in order; neumaier_sum(_f64): the whole sum in order.  The in-order sums (and
-//     dot's fold) run on one wavefront that loads 64 values at a time ahead of the chain.
+//     partials in order; neumaier_sum(_f64): the whole sum in order.  The in-order sums run
+//     on one wavefront that loads 64 values at a time ahead of the chain.
 #include "hp_arith.hpp"
 #include "hp_internal.hpp"
 
@@ -66,31 +67,117 @@ __device__ __forceinline__ double lane_value(double v, int k) {  // v of lane k 
   return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
 }
 
-// n >= 1024: out = ((ZERO + p0) + p1) + ...; else out = p0, the serial dot (ZERO for n = 0).
-// One wavefront: 64 partials are loaded at a time (the next 64 while these are summed) and
-// the fold runs, the same in every lane, on values read across lanes, never on memory.
-__global__ __launch_bounds__(64) void k_fold(const double* __restrict__ ph, const double* __restrict__ pl, long np,
-                                             long n, double* out_h, double* out_l) {
-  const int lane = threadIdx.x;
-  dd acc = mkdd(0.0);
-  if (n < DOT_CHUNK) {
-    if (n) acc = mkdd(ph[0], pl[0]);
-  } else {
-    double nh = lane < np ? ph[lane] : 0.0, nl = lane < np ? pl[lane] : 0.0;
-    for (long c0 = 0; c0 < np; c0 += 64) {
-      const double h = nh, l = nl;
-      const long nx = c0 + 64 + lane;
-      nh = nx < np ? ph[nx] : 0.0;
-      nl = nx < np ? pl[nx] : 0.0;
-      const long cnt = np - c0 < 64 ? np - c0 : 64;
+// ---------------------------------------------------------------- dot: rayon's halving tree
+// The reference combines the chunk sums p_i with rayon's reduce(|| ZERO, |acc, x| acc + x);
+// the tree taken here splits every range at len / 2 down to single chunks:
+//   tree(lo, 1) = ZERO + p_lo,   tree(lo, len) = tree(lo, len/2) + tree(lo + len/2, len - len/2).
+// It is laid out as a complete binary tree of depth D = ceil(log2 m): leaf slot g descends
+// the D bits of g from (lo, len) = (0, m) (bit 1: lo += len/2, len -= len/2; bit 0: len =
+// len/2) and holds chunk lo when one item is left, else nothing.  Each node is then left
+// + right, an empty child passing the other through unchanged: the halving tree exactly.
+// The complete tree is reduced pairwise, left operand always the lower slot: four slots per
+// thread, the wavefront's lanes by DPP row shifts (distances 1, 2, 4, 8), ds_swizzle (16)
+// and a lane shuffle (32), the four wavefronts through LDS.  One workgroup takes 1024 slots
+// to one node; further passes of the same kernel take those nodes to the root.
+constexpr int TREE_SLOTS = 1024;  // per workgroup: 256 threads x 4
+
+__device__ __forceinline__ void tree_comb(dd& a, int& fa, dd b, int fb) {  // a <- a (+) b; a is the left node
+  if (fa && fb) a = dd_add(a, b);
+  else if (fb) a = b;
+  fa = fa | fb;
+}
+template <int CTRL>
+__device__ __forceinline__ int dpp_i(int v) {
+  return __builtin_amdgcn_update_dpp(0, v, CTRL, 0xF, 0xF, false);
+}
+template <int CTRL>
+__device__ __forceinline__ double dpp_d(double v) {
+  const long long b = __double_as_longlong(v);
+  const unsigned lo = (unsigned)dpp_i<CTRL>((int)b), hi = (unsigned)dpp_i<CTRL>((int)(b >> 32));
+  return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
+}
+__device__ __forceinline__ double swz16_d(double v) {  // lane ^ 16 within 32 lanes (bitmask mode)
+  const long long b = __double_as_longlong(v);
+  const unsigned lo = (unsigned)__builtin_amdgcn_ds_swizzle((int)b, 0x401F);
+  const unsigned hi = (unsigned)__builtin_amdgcn_ds_swizzle((int)(b >> 32), 0x401F);
+  return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
+}
+// after this, lane l with l % 64 == 0 holds the pairwise combination of the wavefront's 64 nodes
+__device__ __forceinline__ void wave_tree(dd& v, int& f) {
+  // distance d: lane l (l % 2d == 0) takes lane l + d as its right child.  row_shl:d
+  // (DPP control 0x100 + d) reads lane l + d within a row of 16.
+  tree_comb(v, f, mkdd(dpp_d<0x101>(v.hi), dpp_d<0x101>(v.lo)), dpp_i<0x101>(f));
+  tree_comb(v, f, mkdd(dpp_d<0x102>(v.hi), dpp_d<0x102>(v.lo)), dpp_i<0x102>(f));
+  tree_comb(v, f, mkdd(dpp_d<0x104>(v.hi), dpp_d<0x104>(v.lo)), dpp_i<0x104>(f));
+  tree_comb(v, f, mkdd(dpp_d<0x108>(v.hi), dpp_d<0x108>(v.lo)), dpp_i<0x108>(f));
+  tree_comb(v, f, mkdd(swz16_d(v.hi), swz16_d(v.lo)), __builtin_amdgcn_ds_swizzle(f, 0x401F));
+  tree_comb(v, f, mkdd(__shfl_down(v.hi, 32, 64), __shfl_down(v.lo, 32, 64)), __shfl_down(f, 32, 64));
+}
+
+// LEAF: the slots are the tree's leaves over the m chunk sums in (ph, pl) (depth D);
+// otherwise the first `count` slots are the nodes (ph, pl, pf) of the previous pass.
+template <bool LEAF>
+__global__ __launch_bounds__(256) void k_dot_tree(const double* __restrict__ ph, const double* __restrict__ pl,
+                                                  const int* __restrict__ pf, long count, int D,
+                                                  double* __restrict__ oh, double* __restrict__ ol,
+                                                  int* __restrict__ of) {
+  __shared__ double wh[4], wl[4];
+  __shared__ int wf[4];
+  const int t = threadIdx.x, w = t >> 6, lane = t & 63;
+  dd x[4];
+  int fx[4];
 #pragma unroll
-      for (int k = 0; k < 64; ++k)
-        if (k < cnt) acc = dd_add(acc, mkdd(lane_value(h, k), lane_value(l, k)));
+  for (int e = 0; e < 4; ++e) {
+    const long g = (long)blockIdx.x * TREE_SLOTS + 4 * t + e;
+    x[e] = mkdd(0.0);
+    fx[e] = 0;
+    if (LEAF) {
+      if (g < (1L << D)) {
+        long lo = 0, len = count;
+        for (int bit = D - 1; bit >= 0; --bit) {
+          const long half = len >> 1;
+          if ((g >> bit) & 1) { lo += half; len -= half; }
+          else len = half;
+        }
+        if (len == 1) {
+          x[e] = dd_add(mkdd(0.0), mkdd(ph[lo], pl[lo]));  // the leaf's fold: ZERO + p_lo
+          fx[e] = 1;
+        }
+      }
+    } else if (g < count) {
+      x[e] = mkdd(ph[g], pl[g]);
+      fx[e] = pf[g];
     }
   }
+  tree_comb(x[0], fx[0], x[1], fx[1]);
+  tree_comb(x[2], fx[2], x[3], fx[3]);
+  tree_comb(x[0], fx[0], x[2], fx[2]);
+  dd v = x[0];
+  int f = fx[0];
+  wave_tree(v, f);
   if (lane == 0) {
-    out_h[0] = acc.hi;
-    out_l[0] = acc.lo;
+    wh[w] = v.hi;
+    wl[w] = v.lo;
+    wf[w] = f;
+  }
+  __syncthreads();
+  if (t == 0) {
+    dd a = mkdd(wh[0], wl[0]), b = mkdd(wh[2], wl[2]);
+    int fa = wf[0], fb = wf[2];
+    tree_comb(a, fa, mkdd(wh[1], wl[1]), wf[1]);
+    tree_comb(b, fb, mkdd(wh[3], wl[3]), wf[3]);
+    tree_comb(a, fa, b, fb);
+    oh[blockIdx.x] = a.hi;
+    ol[blockIdx.x] = a.lo;
+    if (of) of[blockIdx.x] = fa;
+  }
+}
+
+// n < 1024: the serial dot of the single chunk (k_dot_chunks' p_0), ZERO for n = 0
+__global__ void k_dot_small(const double* ph, const double* pl, long n, double* oh, double* ol) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) {
+    oh[0] = n ? ph[0] : 0.0;
+    ol[0] = n ? pl[0] : 0.0;
   }
 }
 
@@ -250,19 +337,70 @@ unsigned nblk(long n, int t) { return (unsigned)((n + t - 1) / t); }
 
 }  // namespace
 
-int dot_dev(const double* ah, const double* al, const double* bh, const double* bl, long n, double* out_h,
-            double* out_l, hipStream_t s) {
-  const long np = std::max(1L, (n + DOT_CHUNK - 1) / DOT_CHUNK);
-  DevBuf part;
-  int rc;
-  if ((rc = part.alloc(2 * np * sizeof(double)))) return rc;
+// scratch of dot_dev: the m chunk sums (2 m doubles), then the tree passes' nodes (values
+// and flags, at most 2 (1 << D) / 1024 over all passes); dot_scratch_bytes(n) bytes
+constexpr long dot_tree_nodes(long m) {
+  int D = 0;
+  while ((1L << D) < m) ++D;
+  return 2 * ((1L << D) / TREE_SLOTS > 1 ? (1L << D) / TREE_SLOTS : 1);
+}
+constexpr size_t dot_scratch_bytes(long n) {
+  const long m = (n + DOT_CHUNK - 1) / DOT_CHUNK > 1 ? (n + DOT_CHUNK - 1) / DOT_CHUNK : 1;
+  const long nodes = dot_tree_nodes(m);
+  return (size_t)(2 * m + 2 * nodes) * sizeof(double) + (size_t)nodes * sizeof(int);
+}
+int dot_dev_scratch(const double* ah, const double* al, const double* bh, const double* bl, long n, double* out_h,
+                    double* out_l, void* scr, hipStream_t s) {
+  const long m = std::max(1L, (n + DOT_CHUNK - 1) / DOT_CHUNK);
+  double* ph = static_cast<double*>(scr);
+  double* pl = ph + m;
   if (n > 0) {
-    hipLaunchKernelGGL(k_dot_chunks, dim3(nblk(np, 64)), dim3(64), 0, s, ah, al, bh, bl, n, part.d(), part.d() + np);
+    hipLaunchKernelGGL(k_dot_chunks, dim3(nblk(m, 64)), dim3(64), 0, s, ah, al, bh, bl, n, ph, pl);
     HP_LAUNCH_CHECK();
   }
-  hipLaunchKernelGGL(k_fold, dim3(1), dim3(64), 0, s, part.d(), part.d() + np, np, n, out_h, out_l);
-  HP_LAUNCH_CHECK();
-  HP_HIP(hipStreamSynchronize(s));
+  if (n < DOT_CHUNK) {
+    hipLaunchKernelGGL(k_dot_small, dim3(1), dim3(64), 0, s, ph, pl, n, out_h, out_l);
+    HP_LAUNCH_CHECK();
+    return HP_OK;
+  }
+  int D = 0;
+  while ((1L << D) < m) ++D;
+  const long cap = dot_tree_nodes(m);
+  double* nh = pl + m;  // node values and flags of all passes, pass after pass
+  double* nl = nh + cap;
+  int* nf = reinterpret_cast<int*>(nl + cap);
+  long count = 1L << D, off = 0;  // slots of this pass; where its nodes go
+  const double *inh = ph, *inl = pl;
+  const int* inf = nullptr;
+  long in_count = m;
+  for (bool leaf = true;; leaf = false) {
+    const long grid = (count + TREE_SLOTS - 1) / TREE_SLOTS;
+    const bool last = grid == 1;
+    double* dh = last ? out_h : nh + off;
+    double* dl = last ? out_l : nl + off;
+    int* df = last ? nullptr : nf + off;
+    if (leaf)
+      hipLaunchKernelGGL(k_dot_tree<true>, dim3((unsigned)grid), dim3(256), 0, s, inh, inl, inf, in_count, D, dh, dl, df);
+    else
+      hipLaunchKernelGGL(k_dot_tree<false>, dim3((unsigned)grid), dim3(256), 0, s, inh, inl, inf, in_count, D, dh,
+                         dl, df);
+    HP_LAUNCH_CHECK();
+    if (last) break;
+    inh = dh;
+    inl = dl;
+    inf = df;
+    in_count = count = grid;
+    off += grid;
+  }
+  return HP_OK;
+}
+int dot_dev(const double* ah, const double* al, const double* bh, const double* bl, long n, double* out_h,
+            double* out_l, hipStream_t s) {
+  DevBuf scr;
+  int rc;
+  if ((rc = scr.alloc(dot_scratch_bytes(n)))) return rc;
+  if ((rc = dot_dev_scratch(ah, al, bh, bl, n, out_h, out_l, scr.p, s))) return rc;
+  HP_HIP(hipStreamSynchronize(s));  // the scratch is freed on return
   return HP_OK;
 }
 

@@ -1,8 +1,9 @@
 """GPU dot / gemv (crates/quad/src/linalg.rs:7-40) and Neumaier sums
 (crates/compensated-sum/src/lib.rs:6-66) against the oracle, bit for bit: both run the same
-dd operations in the same order (chunk sums, then their left fold or Neumaier sum), so every
-result must be identical, across the chunk boundaries (1023/1024/1025, 4095/4096/4097), on
-empty inputs and at a size of millions of elements."""
+dd operations in the same order (chunk sums, then rayon's halving tree or the Neumaier sum),
+so every result must be identical, across the chunk boundaries (1023/1024/1025,
+4095/4096/4097), at ragged chunk counts (3, 5, 1029: two tree passes), on empty inputs and at
+a size of millions of elements."""
 import numpy as np
 import pytest
 
@@ -25,7 +26,8 @@ def rand_dd(rng, n, spread=0.0):
     return x
 
 
-@pytest.mark.parametrize("n", [0, 1, 7, 1023, 1024, 1025, 2048, 5000, 1 << 20])
+@pytest.mark.parametrize("n", [0, 1, 7, 1023, 1024, 1025, 2048, 2555, 3072, 5000, 1 << 20, (1 << 20) + 5000,
+                               3 << 20])
 def test_dot_bit_exact(qp, n):
     rng = np.random.default_rng(n + 11)
     a, b = rand_dd(rng, n, 4), rand_dd(rng, n, 4)

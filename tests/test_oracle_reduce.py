@@ -4,8 +4,9 @@ tests (linalg.rs:368-374, lib.rs:70-102) and by exact rational arithmetic: every
 and every dd is a rational, so Python's Fraction gives the exact sums and dot products,
 against which the algorithms' error bounds are checked (dd dot: 2 n 2^-104 sum |a_i b_i|;
 Neumaier f64: 2u |S| + 4 n u^2 sum |v|; dd: 16u |S| + 8 n u^2 sum |v|).  dot's chunked branch is also
-checked to be the left fold of its chunk sums, and a planted mistake (the compensation
-branch swapped) fails the Neumaier bound."""
+checked to be rayon's halving tree over its chunk sums (split at len / 2 down to single
+chunks, each leaf ZERO + p_i, restated here in Python over the oracle's dd add), and a
+planted mistake (the compensation branch swapped) fails the Neumaier bound."""
 from fractions import Fraction
 
 import numpy as np
@@ -42,11 +43,42 @@ def test_dot_error_bound(n):
     absum = sum(abs(dd_frac(a[i]) * dd_frac(b[i])) for i in range(n))
     r = O.dot(a, b)
     assert abs(dd_frac(r) - exact) <= 2 * n * U * absum
-    if n >= 1024:  # the chunk sums, folded left to right from ZERO
+    if n >= 1024:  # the chunk sums, combined by the halving tree
+        assert np.array_equal(r, halving_tree(chunk_sums(a, b)))
+
+
+def chunk_sums(a, b):
+    return [O.dot(a[c0:c0 + 1024], b[c0:c0 + 1024]).reshape(1, 2) for c0 in range(0, len(a), 1024)]
+
+
+def halving_tree(p, lo=0, ln=None):
+    """rayon's reduce split at len // 2 down to single items: leaf ZERO + p, node left + right"""
+    ln = len(p) if ln is None else ln
+    if ln == 1:
+        return O.dd_binop("add", np.zeros((1, 2)), p[lo])[0]
+    h = ln // 2
+    return O.dd_binop("add", halving_tree(p, lo, h).reshape(1, 2), halving_tree(p, lo + h, ln - h).reshape(1, 2))[0]
+
+
+@pytest.mark.parametrize("m", [2, 3, 5, 6, 7, 11, 13, 64, 100])
+def test_dot_tree_shape(m):
+    """Ragged chunk counts: the tree differs from both the left fold and the level-by-level
+    pairing for some m (e.g. 3: p0 + (p1 + p2)); the oracle must take the halving tree."""
+    rng = np.random.default_rng(100 + m)
+    n = 1024 * m - (517 if m > 2 else 0)
+    a, b = rand_dd(rng, n, 1e3), rand_dd(rng, n)
+    b[::3] *= -1.0
+    r = O.dot(a, b)
+    p = chunk_sums(a, b)
+    assert np.array_equal(r, halving_tree(p))
+    if m == 3:  # p0 + (p1 + p2), not (p0 + p1) + p2
+        right = O.dd_binop("add", p[1], p[2])
+        assert np.array_equal(r, O.dd_binop("add", p[0], right)[0])
+    if m in (3, 5, 11, 64, 100):  # inputs on which the left fold gives other bits
         acc = np.zeros((1, 2))
-        for c0 in range(0, n, 1024):
-            acc = O.dd_binop("add", acc, O.dot(a[c0:c0 + 1024], b[c0:c0 + 1024]).reshape(1, 2))
-        assert np.array_equal(r, acc[0])
+        for q in p:
+            acc = O.dd_binop("add", acc, q)
+        assert not np.array_equal(r, acc[0])
 
 
 def test_gemv_rows_are_serial_dots():
