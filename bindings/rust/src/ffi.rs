@@ -131,6 +131,7 @@ unsafe extern "C" {
     pub fn hp_dev_solve_gen_eigen(f_hi: *const f64, f_lo: *const f64, s_hi: *const f64, s_lo: *const f64, n: usize, threshold: f64, evals_hi: *mut f64, evals_lo: *mut f64, c_hi: *mut f64, c_lo: *mut f64, n_basis: *mut usize, sweeps: *mut usize, stream: *mut c_void) -> c_int;
     pub fn hp_dot(a: *const f64, b: *const f64, n: usize, out_dd: *mut f64) -> c_int;
     pub fn hp_dev_dot(a_hi: *const f64, a_lo: *const f64, b_hi: *const f64, b_lo: *const f64, n: usize, out_hi: *mut f64, out_lo: *mut f64, stream: *mut c_void) -> c_int;
+    pub fn hp_dev_dot_noalloc(a_hi: *const f64, a_lo: *const f64, b_hi: *const f64, b_lo: *const f64, n: usize, out_hi: *mut f64, out_lo: *mut f64, stream: *mut c_void) -> c_int;
     pub fn hp_gemv(a: *const f64, rows: usize, cols: usize, x: *const f64, y: *mut f64) -> c_int;
     pub fn hp_dev_gemv(a_hi: *const f64, a_lo: *const f64, rows: usize, cols: usize, x_hi: *const f64, x_lo: *const f64, y_hi: *mut f64, y_lo: *mut f64, stream: *mut c_void) -> c_int;
     pub fn hp_neumaier_sum_f64(v: *const f64, n: usize, out: *mut f64) -> c_int;

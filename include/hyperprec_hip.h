@@ -328,6 +328,14 @@ int hp_dev_solve_gen_eigen(const double* f_hi, const double* f_lo, const double*
 int hp_dot(const double* a, const double* b, size_t n, double* out_dd);
 int hp_dev_dot(const double* a_hi, const double* a_lo, const double* b_hi, const double* b_lo, size_t n,
                double* out_hi, double* out_lo, void* stream);
+/* hp_dev_dot without any allocation, for callers that cannot allocate (code under HIP graph
+ * capture, tight loops, harnesses that forbid hipMalloc); no counterpart in the reference.
+ * Same kernels and bits as hp_dev_dot; its scratch (chunk sums, tree nodes) is a ~1 MiB
+ * device array the library holds on each GPU until the process ends, so n <= 2^26
+ * (HP_EINVAL above).  Enqueued on `stream` without waiting.  All calls on one GPU share
+ * the array: calls that overlap in time on one GPU (two streams, two threads) race. */
+int hp_dev_dot_noalloc(const double* a_hi, const double* a_lo, const double* b_hi, const double* b_lo, size_t n,
+                       double* out_hi, double* out_lo, void* stream);
 /* y = A x with A row-major rows x cols, replaces gemv (linalg.rs:29-40): y_i is the serial
  * dot of row i with x. */
 int hp_gemv(const double* a, size_t rows, size_t cols, const double* x, double* y);

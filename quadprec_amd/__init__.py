@@ -145,6 +145,7 @@ def lib():
                                          C.POINTER(C.c_size_t), C.POINTER(C.c_size_t), _vp]
     L.hp_dot.argtypes = [_dp, _dp, _sz, _dp]
     L.hp_dev_dot.argtypes = [_vp] * 4 + [_sz, _vp, _vp, _vp]
+    L.hp_dev_dot_noalloc.argtypes = [_vp] * 4 + [_sz, _vp, _vp, _vp]
     L.hp_gemv.argtypes = [_dp, _sz, _sz, _dp, _dp]
     L.hp_dev_gemv.argtypes = [_vp, _vp, _sz, _sz, _vp, _vp, _vp, _vp, _vp]
     for name in ("hp_neumaier_sum_f64", "hp_neumaier_sum", "hp_par_neumaier_sum_f64", "hp_par_neumaier_sum"):

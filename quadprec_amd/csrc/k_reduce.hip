@@ -10,6 +10,8 @@
 //   * par_neumaier_sum(_f64): one thread per 4096-element chunk, then the Neumaier sum of the
 //     partials in order; neumaier_sum(_f64): the whole sum in order.  The in-order sums run
 //     on one wavefront that loads 64 values at a time ahead of the chain.
+#include <mutex>
+
 #include "hp_arith.hpp"
 #include "hp_internal.hpp"
 
@@ -404,6 +406,39 @@ int dot_dev(const double* ah, const double* al, const double* bh, const double* 
   return HP_OK;
 }
 
+// ---------------------------------------------------------------- allocation-free dot
+// hp_dev_dot_noalloc runs dot_dev's kernels with their scratch (the chunk sums and the tree
+// nodes) in `scratch`, a file-scope device array sized for n <= NOALLOC_DOT_MAX_N, so the
+// call allocates nothing.  In host code `scratch` names a host-side stand-in: its device
+// address, per device, comes from hipGetSymbolAddress on the first call on that device and
+// is the only form passed to kernels.  Nothing in it must start at zero (every chunk sum and
+// node is written before it is read).  All calls on one device share the array: calls that
+// overlap in time on one device (two streams, two threads) race; calls in one stream are
+// ordered.
+constexpr long NOALLOC_DOT_MAX_N = 1L << 26;
+constexpr size_t NOALLOC_DOT_SCRATCH = dot_scratch_bytes(NOALLOC_DOT_MAX_N);  // ~1.0 MiB
+static __device__ __align__(256) unsigned char scratch[NOALLOC_DOT_SCRATCH];
+
+std::mutex g_scratch_mu;
+void* g_scratch_dev[64] = {};  // device address of `scratch` per device ordinal (under g_scratch_mu)
+
+int noalloc_scratch(void** out) {
+  int dev = 0;
+  HP_HIP(hipGetDevice(&dev));
+  if (dev < 0 || dev >= 64) {
+    set_error("hp_dev_dot_noalloc: device ordinal %d above 63", dev);
+    return HP_EDEVICE;
+  }
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_dev[dev]) {
+    void* p = nullptr;
+    HP_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    g_scratch_dev[dev] = p;
+  }
+  *out = g_scratch_dev[dev];
+  return HP_OK;
+}
+
 int gemv_dev(const double* ah, const double* al, long rows, long cols, const double* xh, const double* xl, double* yh,
              double* yl, hipStream_t s) {
   if (rows == 0) return HP_OK;
@@ -477,6 +512,22 @@ int hp_dot(const double* a, const double* b, size_t n, double* out_dd) {
   if ((rc = dot_dev(da.d(), da.d() + n, db.d(), db.d() + n, (long)n, r.d(), r.d() + 1, s))) return rc;
   HP_HIP(hipMemcpy(out_dd, r.p, 16, hipMemcpyDeviceToHost));
   return HP_OK;
+}
+
+int hp_dev_dot_noalloc(const double* a_hi, const double* a_lo, const double* b_hi, const double* b_lo, size_t n,
+                       double* out_hi, double* out_lo, void* stream) {
+  if (n > (size_t)NOALLOC_DOT_MAX_N) {
+    set_error("hp_dev_dot_noalloc: n = %zu above the scratch's maximum %ld (use hp_dev_dot)", n, NOALLOC_DOT_MAX_N);
+    return HP_EINVAL;
+  }
+  if (!out_hi || !out_lo || (n && (!a_hi || !a_lo || !b_hi || !b_lo))) {
+    set_error("hp_dev_dot_noalloc: null pointer");
+    return HP_EINVAL;
+  }
+  void* scr = nullptr;
+  int rc = noalloc_scratch(&scr);
+  if (rc) return rc;
+  return dot_dev_scratch(a_hi, a_lo, b_hi, b_lo, (long)n, out_hi, out_lo, scr, (hipStream_t)stream);
 }
 
 int hp_dev_dot(const double* a_hi, const double* a_lo, const double* b_hi, const double* b_lo, size_t n,
