@@ -24,6 +24,9 @@ pub const HP_FACTOR_F64: c_int = 0;
 pub const HP_FACTOR_DD: c_int = 1;
 pub const HP_FACTOR_DD_SPLIT: c_int = 2;
 
+pub const HP_BATCH_PER_INSTANCE: c_int = 1;
+pub const HP_BATCH_PANEL_STEPS: c_int = 2;
+
 pub const HP_NOT_CONVERGED: c_int = 0;
 pub const HP_CONVERGED_MEASURED: c_int = 1;
 pub const HP_CONVERGED_PREDICTED: c_int = 2;
@@ -117,8 +120,10 @@ unsafe extern "C" {
     pub fn hp_solve_spd_refined(factor_kind: c_int, s_dd: *const f64, b_dd: *const f64, n: usize, tol: f64, max_iter: c_int, x_dd: *mut f64, stats: *mut HpRefineStats) -> c_int;
     pub fn hp_set_split_factor_panel(nb: c_int) -> c_int;
     pub fn hp_dev_batch_cholesky_solve(sizes: *const c_int, count: usize, nrhs: c_int, s_hi: *const f64, s_lo: *const f64, l_hi: *mut f64, l_lo: *mut f64, b_hi: *const f64, b_lo: *const f64, x_hi: *mut f64, x_lo: *mut f64, status: *mut c_int, stream: *mut c_void) -> c_int;
+    pub fn hp_batch_set_schedule(schedule: c_int) -> c_int;
     pub fn hp_batch_graph_create(sizes: *const c_int, count: usize, nrhs: c_int, s_hi: *const f64, s_lo: *const f64, l_hi: *mut f64, l_lo: *mut f64, b_hi: *const f64, b_lo: *const f64, x_hi: *mut f64, x_lo: *mut f64, status: *mut c_int, stream: *mut c_void) -> *mut HpBatch;
     pub fn hp_batch_graph_launch(batch: *mut HpBatch, stream: *mut c_void) -> c_int;
+    pub fn hp_batch_graph_schedule(batch: *const HpBatch) -> c_int;
     pub fn hp_batch_graph_destroy(batch: *mut HpBatch);
     pub fn hp_overlap_s(centers: *const f64, alpha: *const f64, n: usize, s_dd: *mut f64) -> c_int;
     pub fn hp_dev_overlap_s(centers: *const f64, alpha: *const f64, n: usize, s_hi: *mut f64, s_lo: *mut f64, stream: *mut c_void) -> c_int;

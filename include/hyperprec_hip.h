@@ -239,11 +239,21 @@ int hp_set_split_factor_panel(int nb);
 
 /* ------------------------------------------------------------------ batched factor + solve */
 
-/* Many dd SPD systems of uneven size in one persistent launch: each instance is copied to
- * L and factored with cholesky_blocked(nb = 64) (bit-identical to hp_cholesky_blocked),
- * then L L^T X = B is solved for `nrhs` (1..8) right-hand sides (forward in reference
- * order, backward right-looking blocked).  Instances are taken largest first from an
- * atomic queue, one workgroup each.
+/* Many dd SPD systems of uneven size at once: each instance is copied to L and factored
+ * with cholesky_blocked(nb = 64) (bit-identical to hp_cholesky_blocked; the upper triangle
+ * keeps S), then L L^T X = B is solved for `nrhs` (1..8) right-hand sides by blocked
+ * forward and backward substitution (64-row diagonal blocks, pivots applied as dd
+ * reciprocals; not the reference's solve order, accurate to its error bound).  Two
+ * schedules, same bits:
+ *   HP_BATCH_PER_INSTANCE  one persistent launch, one workgroup per instance, taken
+ *                          largest first from an atomic queue;
+ *   HP_BATCH_PANEL_STEPS   panel step p of every instance at once: its diagonal blocks
+ *                          (one workgroup each), the rows below (8 per workgroup), the
+ *                          trailing 64 x 64 tiles (one per workgroup) across the whole chip,
+ *                          then one solve workgroup per instance.
+ * The autotuner times both on the first batch of a class (instance count, largest n and
+ * total n^3, each to the next power of two; nrhs) and keeps the faster for the class;
+ * hp_batch_set_schedule pins one.  An instance that breaks down leaves its L unspecified.
  * Layout (device): matrices packed in instance order, instance i at sum_{j<i} n_j^2 in each
  * of the hi / lo planes (s_*, l_*); right-hand sides packed likewise, instance i's r-th
  * vector at sum_{j<i} nrhs*n_j + r*n_i (b_*, x_*).  status[i] = 0, or p+1 when instance i
@@ -255,10 +265,16 @@ int hp_dev_batch_cholesky_solve(const int* sizes, size_t count, int nrhs, const 
 /* The same batch captured once in a hipGraph (queue reset by a kernel node + the
  * persistent kernel) and replayed by hp_batch_graph_launch on any stream. */
 typedef struct hp_batch hp_batch;
+enum hp_batch_schedule { HP_BATCH_PER_INSTANCE = 1, HP_BATCH_PANEL_STEPS = 2 };
+/* 0: the autotuner chooses (default); HP_BATCH_PER_INSTANCE / HP_BATCH_PANEL_STEPS: pinned
+ * for every later batch of the process. */
+int hp_batch_set_schedule(int schedule);
 hp_batch* hp_batch_graph_create(const int* sizes, size_t count, int nrhs, const double* s_hi, const double* s_lo,
                                 double* l_hi, double* l_lo, const double* b_hi, const double* b_lo, double* x_hi,
                                 double* x_lo, int* status, void* stream);
 int hp_batch_graph_launch(hp_batch* batch, void* stream);
+/* The schedule the graph captured (HP_BATCH_PER_INSTANCE or HP_BATCH_PANEL_STEPS). */
+int hp_batch_graph_schedule(const hp_batch* batch);
 void hp_batch_graph_destroy(hp_batch* batch);
 
 /* ------------------------------------------------------------------ overlap build */

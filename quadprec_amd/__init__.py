@@ -53,6 +53,7 @@ class RefineStats(C.Structure):
 
 
 HP_FACTOR_F64, HP_FACTOR_DD, HP_FACTOR_DD_SPLIT = 0, 1, 2
+HP_BATCH_AUTO, HP_BATCH_PER_INSTANCE, HP_BATCH_PANEL_STEPS = 0, 1, 2
 HP_NOT_CONVERGED, HP_CONVERGED_MEASURED, HP_CONVERGED_PREDICTED = 0, 1, 2
 
 
@@ -129,6 +130,9 @@ def lib():
     L.hp_batch_graph_create.restype = _vp
     L.hp_batch_graph_launch.argtypes = [_vp, _vp]
     L.hp_batch_graph_destroy.argtypes = [_vp]
+    L.hp_batch_set_schedule.argtypes = [C.c_int]
+    L.hp_batch_graph_schedule.argtypes = [_vp]
+    L.hp_batch_graph_schedule.restype = C.c_int
     L.hp_batch_graph_destroy.restype = None
     L.hp_assemble_overlap_mixed.argtypes = [_dp, np.ctypeslib.ndpointer(dtype=np.int32, flags="C_CONTIGUOUS"),
                                             np.ctypeslib.ndpointer(dtype=np.int64, flags="C_CONTIGUOUS"),

@@ -13,6 +13,11 @@
 // lo plane of the total; right-hand sides packed the same way, instance i's r-th vector at
 // rhs_off_i + r*n_i.  Per-instance status: 0 ok, p+1 = Cholesky breakdown at pivot p.
 #include <algorithm>
+#include <atomic>
+#include <cmath>
+#include <cstddef>
+#include <map>
+#include <mutex>
 #include <numeric>
 #include <vector>
 
@@ -57,10 +62,326 @@ union BatchLds {
 __device__ __forceinline__ dd ldm(const double* h, const double* l, long o) { return mkdd(h[o], l[o]); }
 __device__ __forceinline__ void stm(double* h, double* l, long o, dd v) { h[o] = v.hi; l[o] = v.lo; }
 
+// ---- the phases of cholesky_blocked(nb = 64) on one instance, shared by both schedules.
+// Src is where the not-yet-factored entries are read (S on the panel schedule's first step,
+// else the factor L = A itself); results go to A.
+
+// Diagonal block [jb0, jb0 + jb): the unblocked algorithm (linalg.rs:250-255, i.e. 42-96 on
+// the block) in the right-looking form that keeps each element's k order: the partial sums
+// S_ic accumulate l_ij l_cj after column j is final.  All NT threads; returns false (every
+// thread) on a breakdown, with status[inst] = pivot + 1.  On success the block's lower
+// triangle is written to A and stays in d.Lh / d.Ll.
+__device__ __forceinline__ bool bt_diag(const double* Srch, const double* Srcl, double* Ah, double* Al, long n, long jb0, int jb,
+                        DiagLds& d, double* s_inv, int* s_fail, int* status, int inst) {
+  const int tid = threadIdx.x;
+  for (int idx = tid; idx < jb * jb; idx += NT) {
+    int i = idx / jb, c = idx % jb;
+    if (c <= i) {
+      long o = (jb0 + i) * n + jb0 + c;
+      d.Lh[i][c] = Srch[o];
+      d.Ll[i][c] = Srcl[o];
+      d.Sh[i][c] = 0.0;
+      d.Sl[i][c] = 0.0;
+    }
+  }
+  if (tid == 0) *s_fail = 0;
+  __syncthreads();
+  for (int j = 0; j < jb; ++j) {
+    if (tid == 0) {
+      dd diag = dd_sub(mkdd(d.Lh[j][j], d.Ll[j][j]), mkdd(d.Sh[j][j], d.Sl[j][j]));
+      if (diag.hi <= 0.0) {
+        *s_fail = 1;
+        status[inst] = (int)(jb0 + j) + 1;
+      } else {
+        dd ljj = dd_sqrt(diag);
+        d.Lh[j][j] = ljj.hi;
+        d.Ll[j][j] = ljj.lo;
+        dd inv = dd_recip(ljj);
+        s_inv[0] = inv.hi;
+        s_inv[1] = inv.lo;
+      }
+    }
+    __syncthreads();
+    if (*s_fail) return false;
+    const dd inv = mkdd(s_inv[0], s_inv[1]);
+    for (int i = j + 1 + tid; i < jb; i += NT) {
+      dd v = dd_mul(dd_sub(mkdd(d.Lh[i][j], d.Ll[i][j]), mkdd(d.Sh[i][j], d.Sl[i][j])), inv);
+      d.Lh[i][j] = v.hi;
+      d.Ll[i][j] = v.lo;
+    }
+    __syncthreads();
+    const int m = jb - j - 1;
+    for (int idx = tid; idx < m * m; idx += NT) {
+      int ii = idx / m, cc = idx % m;
+      if (cc <= ii) {
+        int i = j + 1 + ii, c = j + 1 + cc;
+        dd v = dd_add(mkdd(d.Sh[i][c], d.Sl[i][c]), dd_mul(mkdd(d.Lh[i][j], d.Ll[i][j]), mkdd(d.Lh[c][j], d.Ll[c][j])));
+        d.Sh[i][c] = v.hi;
+        d.Sl[i][c] = v.lo;
+      }
+    }
+    __syncthreads();
+  }
+  for (int idx = tid; idx < jb * jb; idx += NT) {
+    int i = idx / jb, c = idx % jb;
+    if (c <= i) {
+      long o = (jb0 + i) * n + jb0 + c;
+      Ah[o] = d.Lh[i][c];
+      Al[o] = d.Ll[i][c];
+    }
+  }
+  return true;
+}
+
+// trsm_right_lt (linalg.rs:224-235) of rows row0, row0 + step, ... < row1 against the block's
+// L in d.Lh / d.Ll: one wavefront per row, lane = column, each product subtracted in k order.
+// A wavefront runs four rows' chains interleaved (independent, so their latencies overlap).
+__device__ __forceinline__ void bt_trsm_rows(const double* Srch, const double* Srcl, double* Ah, double* Al, long n, long jb0, int jb,
+                             const DiagLds& d, long row0, long row1, int step) {
+  const int lane = threadIdx.x & 63;
+  dd invk = mkdd(0.0);
+  if (lane < jb) invk = dd_recip(mkdd(d.Lh[lane][lane], d.Ll[lane][lane]));
+  for (long row = row0; row < row1; row += 4L * step) {
+    dd t[4], fin[4];
+    bool ok[4];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const long r = row + (long)q * step;
+      ok[q] = r < row1;
+      t[q] = (ok[q] && lane < jb) ? ldm(Srch, Srcl, r * n + jb0 + lane) : mkdd(0.0);
+      fin[q] = mkdd(0.0);
+    }
+#pragma unroll 1
+    for (int k = 0; k < jb; ++k) {
+      const dd lk = lane < jb ? mkdd(d.Lh[lane][k], d.Ll[lane][k]) : mkdd(0.0);
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        dd bk = dd_mul(t[q], invk);
+        if (lane == k) fin[q] = bk;
+        dd b = rlane(bk, k);
+        if (lane > k && lane < jb) t[q] = dd_sub(t[q], dd_mul(b, lk));
+      }
+    }
+#pragma unroll
+    for (int q = 0; q < 4; ++q)
+      if (ok[q] && lane < jb) stm(Ah, Al, (row + (long)q * step) * n + jb0 + lane, fin[q]);
+  }
+}
+
+// tile (ti, tc), tc <= ti, of the lower triangle of the trailing matrix from t0, row-major
+__device__ __forceinline__ void tile_of(long tile, long& ti, long& tc) {
+  ti = (long)((__builtin_sqrt(8.0 * (double)tile + 1.0) - 1.0) / 2.0);
+  while ((ti + 1) * (ti + 2) / 2 <= tile) ++ti;
+  while (ti * (ti + 1) / 2 > tile) --ti;
+  tc = tile - ti * (ti + 1) / 2;
+}
+
+// Trailing update (linalg.rs:268-279) of one 64 x 64 tile at rows i0, columns c0 by 256
+// threads (ht = 0..255, a 16 x 16 grid of 4 x 4 register blocks): s = sum_k p_ik p_ck over
+// the panel columns [jb0, jb0 + jb) from zero, then a = a - s (a read from Src).  The panel
+// goes through LDS in 16-deep slabs, the next slab's loads in flight (registers) while the
+// current one is multiplied.  As / Bs: this group's LDS staging, [plane][k][row].  Every
+// thread of the block calls it (the barriers), `active` false for a group without a tile.
+__device__ __forceinline__ void bt_tile(const double* Srch, const double* Srcl, double* Ah, double* Al, long n, long jb0, int jb,
+                        long i0, long c0, bool active, int ht, double (*As)[16][64], double (*Bs)[16][64]) {
+  const int tx = ht & 15, ty = ht >> 4;
+  const int lr = ht >> 2, lkc = (ht & 3) * 4;  // this thread's loads: row lr, slab columns lkc..lkc+3
+  dd acc[4][4];
+#pragma unroll
+  for (int r = 0; r < 4; ++r)
+#pragma unroll
+    for (int c = 0; c < 4; ++c) acc[r][c] = mkdd(0.0);
+  const bool oka = active && i0 + lr < n, okb = active && c0 + lr < n;
+  const double* pah = Ah + (i0 + lr) * n + jb0 + lkc;  // this thread's loads (valid when oka / okb)
+  const double* pal = Al + (i0 + lr) * n + jb0 + lkc;
+  const double* pbh = Ah + (c0 + lr) * n + jb0 + lkc;
+  const double* pbl = Al + (c0 + lr) * n + jb0 + lkc;
+  double ah[4], al[4], bh[4], bl[4];
+#pragma unroll
+  for (int qq = 0; qq < 4; ++qq) {
+    const bool in = lkc + qq < jb;
+    ah[qq] = (oka && in) ? pah[qq] : 0.0;
+    al[qq] = (oka && in) ? pal[qq] : 0.0;
+    bh[qq] = (okb && in) ? pbh[qq] : 0.0;
+    bl[qq] = (okb && in) ? pbl[qq] : 0.0;
+  }
+  for (int k0 = 0; k0 < jb; k0 += 16) {
+#pragma unroll
+    for (int qq = 0; qq < 4; ++qq) {
+      As[0][lkc + qq][lr] = ah[qq];
+      As[1][lkc + qq][lr] = al[qq];
+      Bs[0][lkc + qq][lr] = bh[qq];
+      Bs[1][lkc + qq][lr] = bl[qq];
+    }
+    __syncthreads();
+    if (k0 + 16 < jb) {  // the next slab's loads in flight during this slab's products
+#pragma unroll
+      for (int qq = 0; qq < 4; ++qq) {
+        const bool in = k0 + 16 + lkc + qq < jb;
+        ah[qq] = (oka && in) ? pah[k0 + 16 + qq] : 0.0;
+        al[qq] = (oka && in) ? pal[k0 + 16 + qq] : 0.0;
+        bh[qq] = (okb && in) ? pbh[k0 + 16 + qq] : 0.0;
+        bl[qq] = (okb && in) ? pbl[k0 + 16 + qq] : 0.0;
+      }
+    }
+    const int kmax = min(16, jb - k0);
+    if (active)
+      for (int kk = 0; kk < kmax; ++kk) {
+        dd av[4], bv[4];
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          av[r] = mkdd(As[0][kk][ty * 4 + r], As[1][kk][ty * 4 + r]);
+          bv[r] = mkdd(Bs[0][kk][tx * 4 + r], Bs[1][kk][tx * 4 + r]);
+        }
+#pragma unroll
+        for (int r = 0; r < 4; ++r)
+#pragma unroll
+          for (int c = 0; c < 4; ++c) acc[r][c] = dd_add(acc[r][c], dd_mul(av[r], bv[c]));
+      }
+    __syncthreads();
+  }
+  if (active) {
+#pragma unroll
+    for (int r = 0; r < 4; ++r)
+#pragma unroll
+      for (int c = 0; c < 4; ++c) {
+        long gi = i0 + ty * 4 + r, gc = c0 + tx * 4 + c;
+        if (gi < n && gc <= gi) {
+          long o = gi * n + gc;
+          stm(Ah, Al, o, dd_sub(ldm(Srch, Srcl, o), acc[r][c]));
+        }
+      }
+  }
+}
+
+// L L^T X = B for nrhs right-hand sides of one factored instance (all NT threads): blocked
+// forward and backward substitution, right-looking (each element takes the blocks'
+// contributions in block order, within a block in column order).  Each 64-row diagonal
+// block is staged in LDS (Dh / Dl) for the wavefronts' substitution chains, which multiply
+// by the pivots' dd reciprocals (formed per lane before the chain) instead of dividing; the
+// other rows take the block's contributions with two independent element chains
+// per thread.  Yh / Yl: the block's solved values.
+constexpr int SOLVE_EPT = 2;
+__device__ __forceinline__ void bt_solve(const double* Ah, const double* Al, long n, const double* bh, const double* bl, double* xh,
+                         double* xl, long ro, int R, double (*Yh)[NB], double (*Yl)[NB], double (*Dh)[NB + 1],
+                         double (*Dl)[NB + 1]) {
+  const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+  for (long idx = tid; idx < R * n; idx += NT) {
+    xh[ro + idx] = bh[ro + idx];
+    xl[ro + idx] = bl[ro + idx];
+  }
+  auto stage = [&](long b0, int bw) {  // L[b0 + i][b0 + c], c <= i, into Dh / Dl
+    for (int idx = tid; idx < bw * bw; idx += NT) {
+      const int i = idx / bw, c = idx % bw;
+      if (c <= i) {
+        const long o = (b0 + i) * n + b0 + c;
+        Dh[i][c] = Ah[o];
+        Dl[i][c] = Al[o];
+      }
+    }
+  };
+  __syncthreads();
+  for (long b0 = 0; b0 < n; b0 += NB) {
+    const int bw = (int)min((long)NB, n - b0);
+    stage(b0, bw);
+    __syncthreads();
+    if (wv < R) {  // wavefront r: diagonal block of right-hand side r, lane = row
+      double* Xh = xh + ro + (long)wv * n;
+      double* Xl = xl + ro + (long)wv * n;
+      dd y = lane < bw ? ldm(Xh, Xl, b0 + lane) : mkdd(0.0);
+      const dd rjj = dd_recip(lane < bw ? mkdd(Dh[lane][lane], Dl[lane][lane]) : mkdd(Dh[0][0], Dl[0][0]));
+      for (int j = 0; j < bw; ++j) {
+        dd yj = dd_mul(y, rjj);
+        if (lane == j) y = yj;
+        dd b = rlane(yj, j);
+        if (lane > j && lane < bw) y = dd_sub(y, dd_mul(mkdd(Dh[lane][j], Dl[lane][j]), b));
+      }
+      if (lane < bw) {
+        stm(Xh, Xl, b0 + lane, y);
+        Yh[wv][lane] = y.hi;
+        Yl[wv][lane] = y.lo;
+      }
+    }
+    __syncthreads();
+    const long rows = n - b0 - bw, tot = (long)R * rows;
+    for (long base = tid; base < tot; base += (long)SOLVE_EPT * NT) {
+      dd y[SOLVE_EPT];
+      long off[SOLVE_EPT];
+      int rr[SOLVE_EPT];
+#pragma unroll
+      for (int e = 0; e < SOLVE_EPT; ++e) {
+        const long idx = base + (long)e * NT;
+        rr[e] = idx < tot ? (int)(idx / rows) : 0;
+        const long i = idx < tot ? b0 + bw + idx % rows : b0 + bw;
+        off[e] = i;
+        y[e] = idx < tot ? ldm(xh + ro + (long)rr[e] * n, xl + ro + (long)rr[e] * n, i) : mkdd(0.0);
+      }
+#pragma unroll 1
+      for (int j = 0; j < bw; ++j) {
+#pragma unroll
+        for (int e = 0; e < SOLVE_EPT; ++e)
+          y[e] = dd_sub(y[e], dd_mul(ldm(Ah, Al, off[e] * n + b0 + j), mkdd(Yh[rr[e]][j], Yl[rr[e]][j])));
+      }
+#pragma unroll
+      for (int e = 0; e < SOLVE_EPT; ++e)
+        if (base + (long)e * NT < tot) stm(xh + ro + (long)rr[e] * n, xl + ro + (long)rr[e] * n, off[e], y[e]);
+    }
+    __syncthreads();
+  }
+  for (long b0 = ((n - 1) / NB) * NB; b0 >= 0; b0 -= NB) {
+    const int bw = (int)min((long)NB, n - b0);
+    stage(b0, bw);
+    __syncthreads();
+    if (wv < R) {
+      double* Xh = xh + ro + (long)wv * n;
+      double* Xl = xl + ro + (long)wv * n;
+      dd y = lane < bw ? ldm(Xh, Xl, b0 + lane) : mkdd(0.0);
+      const dd rjj = dd_recip(lane < bw ? mkdd(Dh[lane][lane], Dl[lane][lane]) : mkdd(Dh[0][0], Dl[0][0]));
+      for (int j = bw - 1; j >= 0; --j) {
+        dd xj = dd_mul(y, rjj);  // lane j's: its own pivot
+        if (lane == j) y = xj;
+        dd b = rlane(xj, j);
+        if (lane < j) y = dd_sub(y, dd_mul(mkdd(Dh[j][lane], Dl[j][lane]), b));
+      }
+      if (lane < bw) {
+        stm(Xh, Xl, b0 + lane, y);
+        Yh[wv][lane] = y.hi;
+        Yl[wv][lane] = y.lo;
+      }
+    }
+    __syncthreads();
+    const long tot = (long)R * b0;
+    for (long base = tid; base < tot; base += (long)SOLVE_EPT * NT) {
+      dd y[SOLVE_EPT];
+      long ii[SOLVE_EPT];
+      int rr[SOLVE_EPT];
+#pragma unroll
+      for (int e = 0; e < SOLVE_EPT; ++e) {
+        const long idx = base + (long)e * NT;
+        rr[e] = idx < tot ? (int)(idx / b0) : 0;
+        ii[e] = idx < tot ? idx % b0 : 0;
+        y[e] = idx < tot ? ldm(xh + ro + (long)rr[e] * n, xl + ro + (long)rr[e] * n, ii[e]) : mkdd(0.0);
+      }
+#pragma unroll 1
+      for (int j = bw - 1; j >= 0; --j) {
+#pragma unroll
+        for (int e = 0; e < SOLVE_EPT; ++e)
+          y[e] = dd_sub(y[e], dd_mul(ldm(Ah, Al, (b0 + j) * n + ii[e]), mkdd(Yh[rr[e]][j], Yl[rr[e]][j])));
+      }
+#pragma unroll
+      for (int e = 0; e < SOLVE_EPT; ++e)
+        if (base + (long)e * NT < tot) stm(xh + ro + (long)rr[e] * n, xl + ro + (long)rr[e] * n, ii[e], y[e]);
+    }
+    __syncthreads();
+  }
+}
+
 __global__ void k_batch_reset(int* counter) {
   if (threadIdx.x == 0 && blockIdx.x == 0) *counter = 0;
 }
 
+// ---- schedule 1: one workgroup per instance (persistent, instances from an atomic queue,
+// largest n^3 first); each workgroup copies S to L and runs the whole factor and solve.
 __global__ __launch_bounds__(NT) void k_batch_chol_solve(const BatchDesc* __restrict__ desc, const int* __restrict__ order,
                                                          int count, int* __restrict__ counter,
                                                          const double* __restrict__ sh, const double* __restrict__ sl,
@@ -73,7 +394,7 @@ __global__ __launch_bounds__(NT) void k_batch_chol_solve(const BatchDesc* __rest
   __shared__ int s_idx, s_fail;
   __shared__ double s_inv[2];
   __shared__ double Yh[MAXRHS][NB], Yl[MAXRHS][NB];
-  const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+  const int tid = threadIdx.x, wv = tid >> 6;
 
   for (;;) {
     if (tid == 0) s_idx = atomicAdd(counter, 1);
@@ -93,235 +414,169 @@ __global__ __launch_bounds__(NT) void k_batch_chol_solve(const BatchDesc* __rest
       Ah[i] = Sh[i];
       Al[i] = Sl[i];
     }
-    if (tid == 0) s_fail = 0;
     __syncthreads();
-
-    for (long jb0 = 0; jb0 < n; jb0 += NB) {
+    bool ok = true;
+    for (long jb0 = 0; jb0 < n && ok; jb0 += NB) {
       const int jb = (int)min((long)NB, n - jb0);
-      // ---- diagonal block: unblocked algorithm on the block (linalg.rs:250-255)
-      for (int idx = tid; idx < jb * jb; idx += NT) {
-        int i = idx / jb, c = idx % jb;
-        if (c <= i) {
-          long o = (jb0 + i) * n + jb0 + c;
-          lds.d.Lh[i][c] = Ah[o];
-          lds.d.Ll[i][c] = Al[o];
-          lds.d.Sh[i][c] = 0.0;
-          lds.d.Sl[i][c] = 0.0;
-        }
-      }
+      ok = bt_diag(Ah, Al, Ah, Al, n, jb0, jb, lds.d, s_inv, &s_fail, status, inst);
+      if (!ok) break;
+      bt_trsm_rows(Ah, Al, Ah, Al, n, jb0, jb, lds.d, jb0 + jb + wv, n, NT / 64);
       __syncthreads();
-      for (int j = 0; j < jb; ++j) {
-        if (tid == 0) {
-          dd diag = dd_sub(mkdd(lds.d.Lh[j][j], lds.d.Ll[j][j]), mkdd(lds.d.Sh[j][j], lds.d.Sl[j][j]));
-          if (diag.hi <= 0.0) {
-            s_fail = 1;
-            status[inst] = (int)(jb0 + j) + 1;
-          } else {
-            dd ljj = dd_sqrt(diag);
-            lds.d.Lh[j][j] = ljj.hi;
-            lds.d.Ll[j][j] = ljj.lo;
-            dd inv = dd_recip(ljj);
-            s_inv[0] = inv.hi;
-            s_inv[1] = inv.lo;
-          }
-        }
-        __syncthreads();
-        if (s_fail) break;
-        const dd inv = mkdd(s_inv[0], s_inv[1]);
-        for (int i = j + 1 + tid; i < jb; i += NT) {
-          dd v = dd_mul(dd_sub(mkdd(lds.d.Lh[i][j], lds.d.Ll[i][j]), mkdd(lds.d.Sh[i][j], lds.d.Sl[i][j])), inv);
-          lds.d.Lh[i][j] = v.hi;
-          lds.d.Ll[i][j] = v.lo;
-        }
-        __syncthreads();
-        const int m = jb - j - 1;
-        for (int idx = tid; idx < m * m; idx += NT) {
-          int ii = idx / m, cc = idx % m;
-          if (cc <= ii) {
-            int i = j + 1 + ii, c = j + 1 + cc;
-            dd v = dd_add(mkdd(lds.d.Sh[i][c], lds.d.Sl[i][c]),
-                          dd_mul(mkdd(lds.d.Lh[i][j], lds.d.Ll[i][j]), mkdd(lds.d.Lh[c][j], lds.d.Ll[c][j])));
-            lds.d.Sh[i][c] = v.hi;
-            lds.d.Sl[i][c] = v.lo;
-          }
-        }
-        __syncthreads();
-      }
-      if (s_fail) break;
-      for (int idx = tid; idx < jb * jb; idx += NT) {
-        int i = idx / jb, c = idx % jb;
-        if (c <= i) {
-          long o = (jb0 + i) * n + jb0 + c;
-          Ah[o] = lds.d.Lh[i][c];
-          Al[o] = lds.d.Ll[i][c];
-        }
-      }
-      // ---- trsm_right_lt on the rows below (linalg.rs:224-235): one wavefront per row
-      {
-        dd invk = mkdd(0.0);
-        if (lane < jb) invk = dd_recip(mkdd(lds.d.Lh[lane][lane], lds.d.Ll[lane][lane]));
-        for (long row = jb0 + jb + wv; row < n; row += NT / 64) {
-          dd t = mkdd(0.0), fin = mkdd(0.0);
-          if (lane < jb) t = ldm(Ah, Al, row * n + jb0 + lane);
-          for (int k = 0; k < jb; ++k) {
-            dd bk = dd_mul(t, invk);
-            if (lane == k) fin = bk;
-            dd b = rlane(bk, k);
-            if (lane > k && lane < jb) t = dd_sub(t, dd_mul(b, mkdd(lds.d.Lh[lane][k], lds.d.Ll[lane][k])));
-          }
-          if (lane < jb) stm(Ah, Al, row * n + jb0 + lane, fin);
-        }
-      }
-      __syncthreads();
-      // ---- trailing update (linalg.rs:268-279): s = sum_k p_ik p_ck from zero, a -= s.
-      // 64 x 64 tiles of the lower triangle; each 256-thread half of the workgroup owns one
+      // trailing tiles, two at a time (one per 256-thread half)
       const long t0 = jb0 + jb, rem = n - t0;
       if (rem > 0) {
         const long T = (rem + 63) / 64;
         const long ntiles = T * (T + 1) / 2;
-        const int half = tid >> 8, ht = tid & 255, tx = ht & 15, ty = ht >> 4;
+        const int half = tid >> 8, ht = tid & 255;
         for (long base = 0; base < ntiles; base += 2) {
           const long tile = base + half;
           long ti = 0, tc = 0;
-          if (tile < ntiles) {  // tile -> (ti, tc), tc <= ti, row-major over the triangle
-            ti = (long)((__builtin_sqrt(8.0 * (double)tile + 1.0) - 1.0) / 2.0);
-            while ((ti + 1) * (ti + 2) / 2 <= tile) ++ti;
-            while (ti * (ti + 1) / 2 > tile) --ti;
-            tc = tile - ti * (ti + 1) / 2;
-          }
-          const bool active = tile < ntiles;
-          const long i0 = t0 + ti * 64, c0 = t0 + tc * 64;
-          dd acc[4][4];
-#pragma unroll
-          for (int r = 0; r < 4; ++r)
-#pragma unroll
-            for (int c = 0; c < 4; ++c) acc[r][c] = mkdd(0.0);
-          for (int k0 = 0; k0 < jb; k0 += 16) {
-            {
-              int r = ht >> 2, kc = (ht & 3) * 4;
-#pragma unroll
-              for (int qq = 0; qq < 4; ++qq) {
-                int kk = k0 + kc + qq;
-                long gi = i0 + r, gc = c0 + r;
-                dd va = (active && gi < n && kk < jb) ? ldm(Ah, Al, gi * n + jb0 + kk) : mkdd(0.0);
-                dd vb = (active && gc < n && kk < jb) ? ldm(Ah, Al, gc * n + jb0 + kk) : mkdd(0.0);
-                lds.t.As[half][0][kc + qq][r] = va.hi;
-                lds.t.As[half][1][kc + qq][r] = va.lo;
-                lds.t.Bs[half][0][kc + qq][r] = vb.hi;
-                lds.t.Bs[half][1][kc + qq][r] = vb.lo;
-              }
-            }
-            __syncthreads();
-            const int kmax = min(16, jb - k0);
-            if (active) {
-              for (int kk = 0; kk < kmax; ++kk) {
-                dd av[4], bv[4];
-#pragma unroll
-                for (int r = 0; r < 4; ++r) {
-                  av[r] = mkdd(lds.t.As[half][0][kk][ty * 4 + r], lds.t.As[half][1][kk][ty * 4 + r]);
-                  bv[r] = mkdd(lds.t.Bs[half][0][kk][tx * 4 + r], lds.t.Bs[half][1][kk][tx * 4 + r]);
-                }
-#pragma unroll
-                for (int r = 0; r < 4; ++r)
-#pragma unroll
-                  for (int c = 0; c < 4; ++c) acc[r][c] = dd_add(acc[r][c], dd_mul(av[r], bv[c]));
-              }
-            }
-            __syncthreads();
-          }
-          if (active) {
-#pragma unroll
-            for (int r = 0; r < 4; ++r)
-#pragma unroll
-              for (int c = 0; c < 4; ++c) {
-                long gi = i0 + ty * 4 + r, gc = c0 + tx * 4 + c;
-                if (gi < n && gc <= gi) {
-                  long o = gi * n + gc;
-                  stm(Ah, Al, o, dd_sub(ldm(Ah, Al, o), acc[r][c]));
-                }
-              }
-          }
+          if (tile < ntiles) tile_of(tile, ti, tc);
+          bt_tile(Ah, Al, Ah, Al, n, jb0, jb, t0 + ti * 64, t0 + tc * 64, tile < ntiles, ht, lds.t.As[half],
+                  lds.t.Bs[half]);
         }
       }
       __syncthreads();
     }
-    if (s_fail) {
+    if (!ok) {
       __syncthreads();
       continue;
     }
     if (tid == 0) status[inst] = 0;
+    bt_solve(Ah, Al, n, bh, bl, xh, xl, ro, nrhs, Yh, Yl, lds.d.Lh, lds.d.Ll);
+  }
+}
 
-    // ---- solves: forward in the reference order, backward right-looking
-    const int R = nrhs;
-    for (int idx = tid; idx < R * n; idx += NT) {
-      xh[ro + idx] = bh[ro + idx];
-      xl[ro + idx] = bl[ro + idx];
+// ---- schedule 2: panel steps across the batch.  Step p runs three launches over the
+// instances still active (n > 64 p), which are a prefix of the order (n descending):
+//   k_bp_diag    one workgroup per instance: block p's factor;
+//   k_bp_trsm    eight rows per workgroup: trsm of the rows below block p;
+//   k_bp_update  one trailing 64 x 64 tile per workgroup (256 threads);
+// then k_bp_solve, one workgroup per instance, largest first.  Per-step prefix sums of row groups and tiles
+// map a workgroup to its (instance, piece).  Each element receives exactly the operations
+// of schedule 1, in the same order.  Step 0 reads S and writes L, and copies S's upper
+// triangle (the factor leaves it as it is), each piece the mirror of what it factors.
+constexpr int TRSM_ROWS = 32;  // rows per k_bp_trsm workgroup (four per wavefront, interleaved)
+
+__device__ __forceinline__ int piece_owner(const long long* pre, int A, long long g) {  // pre[j] <= g < pre[j+1]
+  int lo = 0, hi = A;
+  while (hi - lo > 1) {
+    const int mid = (lo + hi) >> 1;
+    if (pre[mid] <= g) lo = mid;
+    else hi = mid;
+  }
+  return lo;
+}
+
+__global__ void k_bp_init(int* status, int count) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < count) status[i] = 0;
+}
+
+template <bool FIRST>
+__global__ __launch_bounds__(NT) void k_bp_diag(const BatchDesc* __restrict__ desc, const int* __restrict__ order,
+                                                int* __restrict__ status, const double* __restrict__ sh,
+                                                const double* __restrict__ sl, double* __restrict__ lh,
+                                                double* __restrict__ ll, long jb0) {
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  DiagLds& d = *reinterpret_cast<DiagLds*>(smem_raw);
+  __shared__ int s_fail;
+  __shared__ double s_inv[2];
+  const int inst = order[blockIdx.x];
+  if (status[inst]) return;
+  const long n = desc[inst].n, mo = desc[inst].mat_off;
+  const int jb = (int)min((long)NB, n - jb0);
+  const double* Srch = (FIRST ? sh : lh) + mo;
+  const double* Srcl = (FIRST ? sl : ll) + mo;
+  if (FIRST)  // the block's strict upper triangle, as in S
+    for (int idx = threadIdx.x; idx < jb * jb; idx += NT) {
+      const int i = idx / jb, c = idx % jb;
+      if (c > i) {
+        const long o = (jb0 + i) * n + jb0 + c;
+        lh[mo + o] = sh[mo + o];
+        ll[mo + o] = sl[mo + o];
+      }
     }
-    __syncthreads();
-    for (long b0 = 0; b0 < n; b0 += NB) {
-      const int bw = (int)min((long)NB, n - b0);
-      if (wv < R) {  // wavefront r: diagonal block of right-hand side r, lane = row
-        double* Xh = xh + ro + (long)wv * n;
-        double* Xl = xl + ro + (long)wv * n;
-        dd y = lane < bw ? ldm(Xh, Xl, b0 + lane) : mkdd(0.0);
-        for (int j = 0; j < bw; ++j) {
-          dd ljj = ldm(Ah, Al, (b0 + (lane < bw ? lane : 0)) * n + b0 + (lane < bw ? lane : 0));
-          dd yj = dd_div(y, ljj);
-          if (lane == j) y = yj;
-          dd b = rlane(yj, j);
-          if (lane > j && lane < bw) y = dd_sub(y, dd_mul(ldm(Ah, Al, (b0 + lane) * n + b0 + j), b));
-        }
-        if (lane < bw) {
-          stm(Xh, Xl, b0 + lane, y);
-          Yh[wv][lane] = y.hi;
-          Yl[wv][lane] = y.lo;
-        }
-      }
-      __syncthreads();
-      for (long idx = tid; idx < (long)R * (n - b0 - bw); idx += NT) {
-        const int r = (int)(idx / (n - b0 - bw));
-        const long i = b0 + bw + idx % (n - b0 - bw);
-        double* Xh = xh + ro + (long)r * n;
-        double* Xl = xl + ro + (long)r * n;
-        dd y = ldm(Xh, Xl, i);
-        for (int j = 0; j < bw; ++j) y = dd_sub(y, dd_mul(ldm(Ah, Al, i * n + b0 + j), mkdd(Yh[r][j], Yl[r][j])));
-        stm(Xh, Xl, i, y);
-      }
-      __syncthreads();
-    }
-    for (long b0 = ((n - 1) / NB) * NB; b0 >= 0; b0 -= NB) {
-      const int bw = (int)min((long)NB, n - b0);
-      if (wv < R) {
-        double* Xh = xh + ro + (long)wv * n;
-        double* Xl = xl + ro + (long)wv * n;
-        dd y = lane < bw ? ldm(Xh, Xl, b0 + lane) : mkdd(0.0);
-        for (int j = bw - 1; j >= 0; --j) {
-          dd ljj = ldm(Ah, Al, (b0 + j) * n + b0 + j);
-          dd xj = dd_div(y, ljj);
-          if (lane == j) y = xj;
-          dd b = rlane(xj, j);
-          if (lane < j) y = dd_sub(y, dd_mul(ldm(Ah, Al, (b0 + j) * n + b0 + lane), b));
-        }
-        if (lane < bw) {
-          stm(Xh, Xl, b0 + lane, y);
-          Yh[wv][lane] = y.hi;
-          Yl[wv][lane] = y.lo;
-        }
-      }
-      __syncthreads();
-      for (long idx = tid; idx < (long)R * b0; idx += NT) {
-        const int r = (int)(idx / b0);
-        const long i = idx % b0;
-        double* Xh = xh + ro + (long)r * n;
-        double* Xl = xl + ro + (long)r * n;
-        dd y = ldm(Xh, Xl, i);
-        for (int j = bw - 1; j >= 0; --j)
-          y = dd_sub(y, dd_mul(ldm(Ah, Al, (b0 + j) * n + i), mkdd(Yh[r][j], Yl[r][j])));
-        stm(Xh, Xl, i, y);
-      }
-      __syncthreads();
+  bt_diag(Srch, Srcl, lh + mo, ll + mo, n, jb0, jb, d, s_inv, &s_fail, status, inst);
+}
+
+template <bool FIRST>
+__global__ __launch_bounds__(NT) void k_bp_trsm(const BatchDesc* __restrict__ desc, const int* __restrict__ order,
+                                                const int* __restrict__ status, const long long* __restrict__ pre,
+                                                int active, const double* __restrict__ sh, const double* __restrict__ sl,
+                                                double* __restrict__ lh, double* __restrict__ ll, long jb0) {
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  DiagLds& d = *reinterpret_cast<DiagLds*>(smem_raw);
+  const int j = piece_owner(pre, active, blockIdx.x);
+  const int inst = order[j];
+  if (status[inst]) return;
+  const long n = desc[inst].n, mo = desc[inst].mat_off;
+  // rows below a full block p exist, so jb = 64
+  for (int idx = threadIdx.x; idx < NB * NB; idx += NT) {
+    const int i = idx / NB, c = idx % NB;
+    if (c <= i) {
+      const long o = mo + (jb0 + i) * n + jb0 + c;
+      d.Lh[i][c] = lh[o];
+      d.Ll[i][c] = ll[o];
     }
   }
+  __syncthreads();
+  const long row0 = jb0 + NB + (long)(blockIdx.x - pre[j]) * TRSM_ROWS;
+  const long row1 = min(n, row0 + TRSM_ROWS);
+  const double* Srch = (FIRST ? sh : lh) + mo;
+  const double* Srcl = (FIRST ? sl : ll) + mo;
+  bt_trsm_rows(Srch, Srcl, lh + mo, ll + mo, n, jb0, NB, d, row0 + (threadIdx.x >> 6), row1, NT / 64);
+  if (FIRST) {  // the mirror of these rows' panel entries: S[jb0 + k][row], k < 64
+    const int lane = threadIdx.x & 63;
+    for (long row = row0 + (threadIdx.x >> 6); row < row1; row += NT / 64) {
+      const long o = mo + (jb0 + lane) * n + row;
+      lh[o] = sh[o];
+      ll[o] = sl[o];
+    }
+  }
+}
+
+template <bool FIRST>
+__global__ __launch_bounds__(256) void k_bp_update(const BatchDesc* __restrict__ desc, const int* __restrict__ order,
+                                                   const int* __restrict__ status, const long long* __restrict__ pre,
+                                                   int active, const double* __restrict__ sh,
+                                                   const double* __restrict__ sl, double* __restrict__ lh,
+                                                   double* __restrict__ ll, long jb0) {
+  __shared__ double As[2][16][64], Bs[2][16][64];
+  const int j = piece_owner(pre, active, blockIdx.x);
+  const int inst = order[j];
+  if (status[inst]) return;
+  const long n = desc[inst].n, mo = desc[inst].mat_off;
+  long ti, tc;
+  tile_of((long)blockIdx.x - pre[j], ti, tc);
+  const long t0 = jb0 + NB, i0 = t0 + ti * 64, c0 = t0 + tc * 64;
+  const double* Srch = (FIRST ? sh : lh) + mo;
+  const double* Srcl = (FIRST ? sl : ll) + mo;
+  bt_tile(Srch, Srcl, lh + mo, ll + mo, n, jb0, NB, i0, c0, true, threadIdx.x, As, Bs);
+  if (FIRST) {  // the mirror tile of S's upper triangle (its strict upper part on the diagonal)
+    for (int idx = threadIdx.x; idx < 64 * 64; idx += 256) {
+      const long r = c0 + idx / 64, c = i0 + idx % 64;
+      if (r < n && c < n && c > r) {
+        const long o = mo + r * n + c;
+        lh[o] = sh[o];
+        ll[o] = sl[o];
+      }
+    }
+  }
+}
+
+__global__ __launch_bounds__(NT) void k_bp_solve(const BatchDesc* __restrict__ desc, const int* __restrict__ order,
+                                                 int first, const int* __restrict__ status,
+                                                 const double* __restrict__ lh, const double* __restrict__ ll,
+                                                 const double* __restrict__ bh, const double* __restrict__ bl,
+                                                 double* __restrict__ xh, double* __restrict__ xl, int nrhs) {
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  DiagLds& d = *reinterpret_cast<DiagLds*>(smem_raw);  // Lh / Ll only: the staged diagonal block
+  __shared__ double Yh[MAXRHS][NB], Yl[MAXRHS][NB];
+  const int inst = order[first + blockIdx.x];  // largest first
+  if (status[inst]) return;
+  const long n = desc[inst].n, mo = desc[inst].mat_off, ro = desc[inst].rhs_off;
+  bt_solve(lh + mo, ll + mo, n, bh, bl, xh, xl, ro, nrhs, Yh, Yl, d.Lh, d.Ll);
 }
 
 }  // namespace
@@ -331,7 +586,12 @@ using namespace hp;
 
 struct hp_batch {
   int count = 0, nrhs = 0;
-  DevBuf desc, order, counter;
+  int schedule = 0;  // HP_BATCH_PER_INSTANCE or HP_BATCH_PANEL_STEPS
+  DevBuf desc, order, counter, pre;
+  // panel steps: per step p, the active instances (a prefix of `order`), and the offsets of
+  // its row-group and tile prefix sums in `pre` with their totals
+  std::vector<int> active;
+  std::vector<long long> trsm_off, upd_off, trsm_tot, upd_tot;
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
   ~hp_batch() {
@@ -341,6 +601,21 @@ struct hp_batch {
 };
 
 namespace {
+std::atomic<int> g_batch_schedule{0};  // 0: the autotuner chooses; else pinned (hp_batch_set_schedule)
+std::mutex g_batch_mu;
+std::map<uint64_t, int> g_batch_tuned;  // schedule per batch class (under g_batch_mu)
+
+uint64_t batch_class(const int* sizes, size_t count, int nrhs) {  // count, max n, n^3 total: powers of two
+  long long mx = 0;
+  double macs = 0;
+  for (size_t i = 0; i < count; ++i) {
+    mx = std::max<long long>(mx, sizes[i]);
+    macs += (double)sizes[i] * sizes[i] * sizes[i];
+  }
+  auto lg = [](double v) { return (uint64_t)std::max(0, (int)std::ceil(std::log2(std::max(v, 1.0)))); };
+  return (lg((double)count) << 24) | (lg((double)mx) << 16) | (lg(macs) << 4) | (uint64_t)nrhs;
+}
+
 int batch_prepare(hp_batch* b, const int* sizes, size_t count, int nrhs) {
   if (nrhs < 1 || nrhs > MAXRHS) {
     set_error("batch: nrhs must be in [1, %d]", MAXRHS);
@@ -348,6 +623,7 @@ int batch_prepare(hp_batch* b, const int* sizes, size_t count, int nrhs) {
   }
   std::vector<BatchDesc> d(count);
   long long mo = 0, ro = 0;
+  long long nmax = 0;
   for (size_t i = 0; i < count; ++i) {
     if (sizes[i] <= 0) {
       set_error("batch: instance %zu has size %d", i, sizes[i]);
@@ -358,44 +634,187 @@ int batch_prepare(hp_batch* b, const int* sizes, size_t count, int nrhs) {
     d[i].rhs_off = ro;
     mo += (long long)sizes[i] * sizes[i];
     ro += (long long)nrhs * sizes[i];
+    nmax = std::max<long long>(nmax, sizes[i]);
   }
   std::vector<int> ord(count);
   std::iota(ord.begin(), ord.end(), 0);
   std::stable_sort(ord.begin(), ord.end(), [&](int x, int y) { return sizes[x] > sizes[y]; });
+  // panel steps
+  const int steps = (int)((nmax + NB - 1) / NB);
+  std::vector<long long> pre;
+  b->active.assign(steps, 0);
+  b->trsm_off.assign(steps, 0);
+  b->upd_off.assign(steps, 0);
+  b->trsm_tot.assign(steps, 0);
+  b->upd_tot.assign(steps, 0);
+  for (int p = 0; p < steps; ++p) {
+    const long long jb0 = (long long)p * NB;
+    int A = 0;
+    while (A < (int)count && sizes[ord[A]] > jb0) ++A;
+    b->active[p] = A;
+    b->trsm_off[p] = (long long)pre.size();
+    long long acc = 0;
+    for (int j = 0; j < A; ++j) {
+      pre.push_back(acc);
+      const long long rows = std::max<long long>(0, sizes[ord[j]] - jb0 - NB);
+      acc += (rows + TRSM_ROWS - 1) / TRSM_ROWS;
+    }
+    pre.push_back(acc);
+    b->trsm_tot[p] = acc;
+    b->upd_off[p] = (long long)pre.size();
+    acc = 0;
+    for (int j = 0; j < A; ++j) {
+      pre.push_back(acc);
+      const long long rows = std::max<long long>(0, sizes[ord[j]] - jb0 - NB), T = (rows + 63) / 64;
+      acc += T * (T + 1) / 2;
+    }
+    pre.push_back(acc);
+    b->upd_tot[p] = acc;
+  }
   int rc;
   if ((rc = b->desc.alloc(count * sizeof(BatchDesc))) || (rc = b->order.alloc(count * sizeof(int))) ||
-      (rc = b->counter.alloc(sizeof(int))))
+      (rc = b->counter.alloc(sizeof(int))) || (rc = b->pre.alloc(std::max<size_t>(1, pre.size()) * sizeof(long long))))
     return rc;
   HP_HIP(hipMemcpy(b->desc.p, d.data(), count * sizeof(BatchDesc), hipMemcpyHostToDevice));
   HP_HIP(hipMemcpy(b->order.p, ord.data(), count * sizeof(int), hipMemcpyHostToDevice));
+  if (!pre.empty()) HP_HIP(hipMemcpy(b->pre.p, pre.data(), pre.size() * sizeof(long long), hipMemcpyHostToDevice));
   b->count = (int)count;
   b->nrhs = nrhs;
-  static bool attr = false;
-  if (!attr) {
-    HP_HIP(hipFuncSetAttribute((const void*)k_batch_chol_solve, hipFuncAttributeMaxDynamicSharedMemorySize,
-                               (int)sizeof(BatchLds)));
-    attr = true;
-  }
+  static std::once_flag attr_once;
+  static hipError_t attr_err = hipSuccess;
+  std::call_once(attr_once, [] {
+    hipError_t e[6] = {
+        hipFuncSetAttribute((const void*)k_batch_chol_solve, hipFuncAttributeMaxDynamicSharedMemorySize,
+                            (int)sizeof(BatchLds)),
+        hipFuncSetAttribute((const void*)k_bp_diag<true>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                            (int)sizeof(DiagLds)),
+        hipFuncSetAttribute((const void*)k_bp_diag<false>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                            (int)sizeof(DiagLds)),
+        hipFuncSetAttribute((const void*)k_bp_trsm<true>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                            (int)offsetof(DiagLds, Sh)),
+        hipFuncSetAttribute((const void*)k_bp_trsm<false>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                            (int)offsetof(DiagLds, Sh)),
+        hipFuncSetAttribute((const void*)k_bp_solve, hipFuncAttributeMaxDynamicSharedMemorySize,
+                            (int)offsetof(DiagLds, Sh))};
+    for (hipError_t x : e)
+      if (x != hipSuccess) attr_err = x;
+  });
+  HP_HIP(attr_err);
   return HP_OK;
 }
 
-int batch_enqueue(hp_batch* b, const double* s_hi, const double* s_lo, double* l_hi, double* l_lo, const double* b_hi,
-                  const double* b_lo, double* x_hi, double* x_lo, int* status, hipStream_t s) {
-  int dev = 0, cus = 0;
-  HP_HIP(hipGetDevice(&dev));
-  HP_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
-  const int grid = std::max(1, std::min(b->count, cus));  // one resident workgroup per CU
-  hipLaunchKernelGGL(k_batch_reset, dim3(1), dim3(64), 0, s, (int*)b->counter.p);
+int batch_enqueue(hp_batch* b, int schedule, const double* s_hi, const double* s_lo, double* l_hi, double* l_lo,
+                  const double* b_hi, const double* b_lo, double* x_hi, double* x_lo, int* status, hipStream_t s) {
+  const BatchDesc* desc = (const BatchDesc*)b->desc.p;
+  const int* order = (const int*)b->order.p;
+  if (schedule == HP_BATCH_PER_INSTANCE) {
+    int dev = 0, cus = 0;
+    HP_HIP(hipGetDevice(&dev));
+    HP_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+    const int grid = std::max(1, std::min(b->count, cus));  // one resident workgroup per CU
+    hipLaunchKernelGGL(k_batch_reset, dim3(1), dim3(64), 0, s, (int*)b->counter.p);
+    HP_LAUNCH_CHECK();
+    hipLaunchKernelGGL(k_batch_chol_solve, dim3(grid), dim3(NT), sizeof(BatchLds), s, desc, order, b->count,
+                       (int*)b->counter.p, s_hi, s_lo, l_hi, l_lo, b_hi, b_lo, x_hi, x_lo, b->nrhs, status);
+    HP_LAUNCH_CHECK();
+    return HP_OK;
+  }
+  const int steps = (int)b->active.size();
+  hipLaunchKernelGGL(k_bp_init, dim3((unsigned)((b->count + 255) / 256)), dim3(256), 0, s, status, b->count);
   HP_LAUNCH_CHECK();
-  hipLaunchKernelGGL(k_batch_chol_solve, dim3(grid), dim3(NT), sizeof(BatchLds), s, (const BatchDesc*)b->desc.p,
-                     (const int*)b->order.p, b->count, (int*)b->counter.p, s_hi, s_lo, l_hi, l_lo, b_hi, b_lo, x_hi,
-                     x_lo, b->nrhs, status);
+  const long long* pre = (const long long*)b->pre.p;
+  for (int p = 0; p < steps; ++p) {
+    const long jb0 = (long)p * NB;
+    const bool first = p == 0;
+    const int A = b->active[p];
+    if (first)
+      hipLaunchKernelGGL(k_bp_diag<true>, dim3(A), dim3(NT), sizeof(DiagLds), s, desc, order, status, s_hi, s_lo, l_hi,
+                         l_lo, jb0);
+    else
+      hipLaunchKernelGGL(k_bp_diag<false>, dim3(A), dim3(NT), sizeof(DiagLds), s, desc, order, status, s_hi, s_lo,
+                         l_hi, l_lo, jb0);
+    HP_LAUNCH_CHECK();
+    if (b->trsm_tot[p] > 0) {
+      const unsigned g = (unsigned)b->trsm_tot[p];
+      const size_t lds = offsetof(DiagLds, Sh);
+      if (first)
+        hipLaunchKernelGGL(k_bp_trsm<true>, dim3(g), dim3(NT), lds, s, desc, order, status, pre + b->trsm_off[p], A,
+                           s_hi, s_lo, l_hi, l_lo, jb0);
+      else
+        hipLaunchKernelGGL(k_bp_trsm<false>, dim3(g), dim3(NT), lds, s, desc, order, status, pre + b->trsm_off[p], A,
+                           s_hi, s_lo, l_hi, l_lo, jb0);
+      HP_LAUNCH_CHECK();
+    }
+    if (b->upd_tot[p] > 0) {
+      const unsigned g = (unsigned)b->upd_tot[p];
+      if (first)
+        hipLaunchKernelGGL(k_bp_update<true>, dim3(g), dim3(256), 0, s, desc, order, status, pre + b->upd_off[p], A,
+                           s_hi, s_lo, l_hi, l_lo, jb0);
+      else
+        hipLaunchKernelGGL(k_bp_update<false>, dim3(g), dim3(256), 0, s, desc, order, status, pre + b->upd_off[p], A,
+                           s_hi, s_lo, l_hi, l_lo, jb0);
+      HP_LAUNCH_CHECK();
+    }
+  }
+  // the solves, largest first.  (Solving each instance beside the later panel steps, on a
+  // second stream, was slower: the long solve workgroups hold CUs the steps need.)
+  hipLaunchKernelGGL(k_bp_solve, dim3(b->count), dim3(NT), offsetof(DiagLds, Sh), s, desc, order, 0, status, l_hi,
+                     l_lo, b_hi, b_lo, x_hi, x_lo, b->nrhs);
   HP_LAUNCH_CHECK();
+  return HP_OK;
+}
+
+// The schedule for this batch: pinned, tuned before for its class, or timed now (each
+// schedule once on the caller's buffers, HIP events on `s`; the results are overwritten by
+// the run that follows)
+int batch_choose(hp_batch* b, const int* sizes, const double* s_hi, const double* s_lo, double* l_hi, double* l_lo,
+                 const double* b_hi, const double* b_lo, double* x_hi, double* x_lo, int* status, hipStream_t s) {
+  const int pinned = g_batch_schedule.load();
+  if (pinned) {
+    b->schedule = pinned;
+    return HP_OK;
+  }
+  const uint64_t key = batch_class(sizes, (size_t)b->count, b->nrhs);
+  {
+    std::lock_guard<std::mutex> lock(g_batch_mu);
+    auto it = g_batch_tuned.find(key);
+    if (it != g_batch_tuned.end()) {
+      b->schedule = it->second;
+      return HP_OK;
+    }
+  }
+  hipEvent_t e[3];
+  for (hipEvent_t& x : e) HP_HIP(hipEventCreate(&x));
+  int rc;
+  HP_HIP(hipEventRecord(e[0], s));
+  if ((rc = batch_enqueue(b, HP_BATCH_PER_INSTANCE, s_hi, s_lo, l_hi, l_lo, b_hi, b_lo, x_hi, x_lo, status, s)))
+    return rc;
+  HP_HIP(hipEventRecord(e[1], s));
+  if ((rc = batch_enqueue(b, HP_BATCH_PANEL_STEPS, s_hi, s_lo, l_hi, l_lo, b_hi, b_lo, x_hi, x_lo, status, s)))
+    return rc;
+  HP_HIP(hipEventRecord(e[2], s));
+  HP_HIP(hipEventSynchronize(e[2]));
+  float t1 = 0.f, t2 = 0.f;
+  HP_HIP(hipEventElapsedTime(&t1, e[0], e[1]));
+  HP_HIP(hipEventElapsedTime(&t2, e[1], e[2]));
+  for (hipEvent_t& x : e) (void)hipEventDestroy(x);
+  b->schedule = t2 < t1 ? HP_BATCH_PANEL_STEPS : HP_BATCH_PER_INSTANCE;
+  std::lock_guard<std::mutex> lock(g_batch_mu);
+  g_batch_tuned[key] = b->schedule;
   return HP_OK;
 }
 }  // namespace
 
 extern "C" {
+
+int hp_batch_set_schedule(int schedule) {
+  if (schedule != 0 && schedule != HP_BATCH_PER_INSTANCE && schedule != HP_BATCH_PANEL_STEPS) {
+    set_error("batch schedule must be 0 (autotune), HP_BATCH_PER_INSTANCE or HP_BATCH_PANEL_STEPS");
+    return HP_EINVAL;
+  }
+  g_batch_schedule.store(schedule);
+  return HP_OK;
+}
 
 int hp_dev_batch_cholesky_solve(const int* sizes, size_t count, int nrhs, const double* s_hi, const double* s_lo,
                                 double* l_hi, double* l_lo, const double* b_hi, const double* b_lo, double* x_hi,
@@ -405,7 +824,8 @@ int hp_dev_batch_cholesky_solve(const int* sizes, size_t count, int nrhs, const 
   int rc = batch_prepare(&b, sizes, count, nrhs);
   if (rc) return rc;
   hipStream_t s = (hipStream_t)stream;
-  if ((rc = batch_enqueue(&b, s_hi, s_lo, l_hi, l_lo, b_hi, b_lo, x_hi, x_lo, status, s))) return rc;
+  if ((rc = batch_choose(&b, sizes, s_hi, s_lo, l_hi, l_lo, b_hi, b_lo, x_hi, x_lo, status, s))) return rc;
+  if ((rc = batch_enqueue(&b, b.schedule, s_hi, s_lo, l_hi, l_lo, b_hi, b_lo, x_hi, x_lo, status, s))) return rc;
   HP_HIP(hipStreamSynchronize(s));  // the descriptors are freed on return
   return HP_OK;
 }
@@ -416,14 +836,18 @@ hp_batch* hp_batch_graph_create(const int* sizes, size_t count, int nrhs, const 
   hp_batch* b = new hp_batch();
   hipStream_t s = (hipStream_t)stream;
   hipStream_t cap = nullptr;
-  if (count == 0 || batch_prepare(b, sizes, count, nrhs) != HP_OK) { delete b; return nullptr; }
+  if (count == 0 || batch_prepare(b, sizes, count, nrhs) != HP_OK ||
+      batch_choose(b, sizes, s_hi, s_lo, l_hi, l_lo, b_hi, b_lo, x_hi, x_lo, status, s) != HP_OK ||
+      hipStreamSynchronize(s) != hipSuccess) {
+    delete b;
+    return nullptr;
+  }
   if (hipStreamCreateWithFlags(&cap, hipStreamNonBlocking) != hipSuccess) { delete b; return nullptr; }
   bool ok = hipStreamBeginCapture(cap, hipStreamCaptureModeThreadLocal) == hipSuccess;
-  ok = ok && batch_enqueue(b, s_hi, s_lo, l_hi, l_lo, b_hi, b_lo, x_hi, x_lo, status, cap) == HP_OK;
+  ok = ok && batch_enqueue(b, b->schedule, s_hi, s_lo, l_hi, l_lo, b_hi, b_lo, x_hi, x_lo, status, cap) == HP_OK;
   ok = (hipStreamEndCapture(cap, &b->graph) == hipSuccess) && ok;
   ok = ok && hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0) == hipSuccess;
   (void)hipStreamDestroy(cap);
-  (void)s;
   if (!ok) {
     set_error("batch graph capture failed");
     delete b;
@@ -440,6 +864,8 @@ int hp_batch_graph_launch(hp_batch* b, void* stream) {
   HP_HIP(hipGraphLaunch(b->exec, (hipStream_t)stream));
   return HP_OK;
 }
+
+int hp_batch_graph_schedule(const hp_batch* b) { return b ? b->schedule : 0; }
 
 void hp_batch_graph_destroy(hp_batch* b) { delete b; }
 

@@ -88,9 +88,19 @@ def batch_cholesky_solve(sizes, nrhs, s, l, b, x, status, graph=False):
     return BatchGraph(h)
 
 
+def set_batch_schedule(schedule):
+    """0: the autotuner picks the batched schedule per batch class; HP_BATCH_PER_INSTANCE /
+    HP_BATCH_PANEL_STEPS: pinned for later batches."""
+    _check(lib().hp_batch_set_schedule(int(schedule)), "set_batch_schedule")
+
+
 class BatchGraph:
     def __init__(self, h):
         self.h = h
+
+    @property
+    def schedule(self):
+        return lib().hp_batch_graph_schedule(self.h)
 
     def launch(self):
         _check(lib().hp_batch_graph_launch(self.h, _stream()), "batch graph launch")

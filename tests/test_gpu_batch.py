@@ -1,9 +1,11 @@
-"""Batched uneven factor + solve (one persistent launch, hipGraph replay).
+"""Batched uneven factor + solve in both schedules (one workgroup per instance; panel steps
+across the chip), eager and as a replayed hipGraph.
 
 Each instance's factor must be bit-identical to the oracle's cholesky_blocked(nb=64)
-(crates/quad/src/linalg.rs:238-293) -- the batched kernel keeps the per-element order --
-and each solution must agree with the oracle's solve_cholesky (linalg.rs:123-126) to the
-dd accuracy the conditioning allows (the batched backward solve is reordered)."""
+(crates/quad/src/linalg.rs:238-293) -- both schedules keep the per-element order -- with
+the upper triangle left as S, and each solution must agree with the oracle's
+solve_cholesky (linalg.rs:123-126) to the dd accuracy the conditioning allows (the batched
+backward solve is reordered)."""
 import numpy as np
 import pytest
 
@@ -17,6 +19,13 @@ def qp():
     import quadprec_amd
     quadprec_amd.lib()
     return quadprec_amd
+
+
+@pytest.fixture(params=[1, 2], ids=["per_instance", "panel_steps"])
+def schedule(qp, request):
+    qp.dev.set_batch_schedule(request.param)
+    yield request.param
+    qp.dev.set_batch_schedule(0)
 
 
 def pack(qp, sizes, nrhs, seed0=100, lo=0.5):
@@ -58,6 +67,7 @@ def check(sizes, nrhs, mats, rhs, L, X, st):
         ref_l = O.cholesky_blocked(mats[i], n, 64)
         lower = np.tril(np.ones((n, n), dtype=bool)).ravel()
         assert np.array_equal(li[lower].view(np.uint64), ref_l[lower].view(np.uint64)), i
+        assert np.array_equal(li[~lower].view(np.uint64), mats[i][~lower].view(np.uint64)), i  # upper keeps S
         for r in range(nrhs):
             b = rhs[i][r * n:(r + 1) * n]
             x = X[ro + r * n:ro + (r + 1) * n]
@@ -69,7 +79,7 @@ def check(sizes, nrhs, mats, rhs, L, X, st):
         ro += nrhs * n
 
 
-def test_batch_factor_bitexact_and_solutions(qp):
+def test_batch_factor_bitexact_and_solutions(qp, schedule):
     sizes = [64, 65, 100, 130, 200, 37, 1, 128, 300, 257, 90, 150]
     nrhs = 4
     mats, rhs = pack(qp, sizes, nrhs)
@@ -77,19 +87,20 @@ def test_batch_factor_bitexact_and_solutions(qp):
     check(sizes, nrhs, mats, rhs, L, X, st)
 
 
-def test_batch_graph_replays_match_eager(qp):
+def test_batch_graph_replays_match_eager(qp, schedule):
     sizes = [70, 140, 33, 256, 190]
     nrhs = 3
     mats, rhs = pack(qp, sizes, nrhs, seed0=500)
     L0, X0, st0 = run_batch(qp, sizes, nrhs, mats, rhs)
     outs = run_batch(qp, sizes, nrhs, mats, rhs, graph=True, replays=3)
+    check(sizes, nrhs, mats, rhs, L0, X0, st0)
     for L, X, st in outs:
         assert np.array_equal(L.view(np.uint64), L0.view(np.uint64))
         assert np.array_equal(X.view(np.uint64), X0.view(np.uint64))
         assert np.array_equal(st, st0)
 
 
-def test_batch_reports_breakdown_per_instance(qp):
+def test_batch_reports_breakdown_per_instance(qp, schedule):
     sizes = [50, 80, 40]
     nrhs = 1
     mats, rhs = pack(qp, sizes, nrhs, seed0=900)
@@ -104,3 +115,24 @@ def test_batch_reports_breakdown_per_instance(qp):
         ref = e.pivot
     assert st[0] == 0 and st[2] == 0
     assert st[1] == ref + 1
+
+
+def test_autotuned_schedule_is_one_of_the_two(qp):
+    """With no pin, the first batch of a class times both schedules and the graph captures
+    the faster; either way the results are the reference's."""
+    import torch
+    qp.dev.set_batch_schedule(0)
+    sizes = [200, 64, 130, 90, 257]
+    nrhs = 2
+    mats, rhs = pack(qp, sizes, nrhs, seed0=700)
+    S = qp.dev.to_planes(np.concatenate(mats), 2)
+    B = qp.dev.to_planes(np.concatenate(rhs), 2)
+    L, X = torch.zeros_like(S), torch.zeros_like(B)
+    st = torch.full((len(sizes),), -7, dtype=torch.int32, device="cuda")
+    g = qp.dev.batch_cholesky_solve(sizes, nrhs, S, L, B, X, st, graph=True)
+    assert g.schedule in (1, 2)
+    L.zero_()
+    X.zero_()
+    g.launch()
+    torch.cuda.synchronize()
+    check(sizes, nrhs, mats, rhs, qp.dev.from_planes(L), qp.dev.from_planes(X), st.cpu().numpy())

@@ -183,16 +183,21 @@ def batch_case(qp, count=24, nrhs=4, seed=3):
     return [int(v) for v in sizes], nrhs, mats, rhs
 
 
-def test_config3_batch_at_size(qp):
+@pytest.mark.parametrize("schedule", [1, 2], ids=["per_instance", "panel_steps"])
+def test_config3_batch_at_size(qp, schedule):
     import torch
     sizes, nrhs, mats, rhs = batch_case(qp)
+    qp.dev.set_batch_schedule(schedule)
     S = qp.dev.to_planes(np.concatenate(mats), 2)
     B = qp.dev.to_planes(np.concatenate(rhs), 2)
     L = torch.zeros_like(S)
     X = torch.zeros_like(B)
     st = torch.full((len(sizes),), -7, dtype=torch.int32, device="cuda")
-    qp.dev.batch_cholesky_solve(sizes, nrhs, S, L, B, X, st)
-    torch.cuda.synchronize()
+    try:
+        qp.dev.batch_cholesky_solve(sizes, nrhs, S, L, B, X, st)
+        torch.cuda.synchronize()
+    finally:
+        qp.dev.set_batch_schedule(0)
     Lh, Xh, sth = qp.dev.from_planes(L), qp.dev.from_planes(X), st.cpu().numpy()
     mo = ro = 0
     worst_bwd = worst_ratio = 0.0
