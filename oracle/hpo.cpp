@@ -799,6 +799,26 @@ void hpo_solve_spd_f128(const double* a_dd, const double* b_dd, size_t n, hpo_so
 
 // ------------------------------------------------------------------ overlap crate
 static const double PI_F64 = 3.141592653589793;
+// libm exp, optionally moved by a deterministic pseudo-random number of ulps in
+// [-shift, shift] (a function of x's bits).  Shift 0 (the default) is libm's exp, as the
+// reference calls it; a non-zero shift stands in for another correctly-rounded-or-nearly
+// exp (the device library's) when the tests check the per-entry error bound of the
+// mixed-tier assembly.
+static int g_exp_ulp_shift = 0;
+void hpo_set_exp_ulp_shift(int ulps) { g_exp_ulp_shift = ulps; }
+static double oexp(double x) {
+  double e = std::exp(x);
+  if (g_exp_ulp_shift == 0 || !(e > 0.0) || std::isinf(e)) return e;
+  uint64_t b;
+  std::memcpy(&b, &x, 8);
+  b ^= b >> 29;
+  b *= 0xbf58476d1ce4e5b9ull;
+  b ^= b >> 32;
+  int k = (int)(b % (uint64_t)(2 * g_exp_ulp_shift + 1)) - g_exp_ulp_shift;
+  for (; k > 0; --k) e = std::nextafter(e, INFINITY);
+  for (; k < 0; ++k) e = std::nextafter(e, 0.0);
+  return e;
+}
 // overlap_primitive_f64: overlap/src/integral.rs:15-25
 double hpo_overlap_primitive_f64(double ai, const double* ci, double aj, const double* cj) {
   double gamma = ai + aj;
@@ -807,7 +827,7 @@ double hpo_overlap_primitive_f64(double ai, const double* ci, double aj, const d
   double r2 = dx * dx + dy * dy + dz * dz;  // integral.rs:354-360
   double sq = std::sqrt(PI_F64 / gamma);
   double prefactor = (sq * sq) * sq;         // powi(3)
-  return prefactor * std::exp(-mu * r2);
+  return prefactor * oexp(-mu * r2);
 }
 // overlap_primitive_f128: integral.rs:29-56 (f64 pi, f64 centre differences, libm exp)
 hpo_dd hpo_overlap_primitive_f128(double ai, const double* ci, double aj, const double* cj) {
@@ -820,7 +840,7 @@ hpo_dd hpo_overlap_primitive_f128(double ai, const double* ci, double aj, const 
   D ratio = div(pi, gamma);
   D prefactor = mul(ratio, dsqrt(ratio));
   D nmr2 = mul(sub(D{0.0, 0.0}, mu), r2);
-  D exp_hi = dd_from(std::exp(nmr2.hi));
+  D exp_hi = dd_from(oexp(nmr2.hi));
   D exp_val = mul(exp_hi, add(D{1.0, 0.0}, dd_from(nmr2.lo)));
   return to_c(mul(prefactor, exp_val));
 }
@@ -978,7 +998,7 @@ static void shell_pair_block(const ShellView& A, const ShellView& B, const doubl
         double dx = A.c[0] - B.c[0], dy = A.c[1] - B.c[1], dz = A.c[2] - B.c[2];
         double r2 = dx * dx + dy * dy + dz * dz;
         double sq = std::sqrt(PI_F64 / gamma);
-        double prefactor = (sq * sq) * sq * std::exp(-mu * r2);
+        double prefactor = (sq * sq) * sq * oexp(-mu * r2);
         double pa[3], pb[3];
         for (int k = 0; k < 3; ++k) {
           double pk = (ai * A.c[k] + aj * B.c[k]) / gamma;
@@ -1020,7 +1040,7 @@ static void shell_pair_block(const ShellView& A, const ShellView& B, const doubl
       D ratio = div(dd_from(PI_F64), gamma);
       D prefactor = mul(ratio, dsqrt(ratio));
       D nmr2 = mul(sub(D{0.0, 0.0}, mu), r2);
-      D exp_val = mul(dd_from(std::exp(nmr2.hi)), add(D{1.0, 0.0}, dd_from(nmr2.lo)));
+      D exp_val = mul(dd_from(oexp(nmr2.hi)), add(D{1.0, 0.0}, dd_from(nmr2.lo)));
       D total_pre = mul(prefactor, exp_val);
       D pa[3], pb[3];
       for (int k = 0; k < 3; ++k) {
@@ -1079,6 +1099,81 @@ size_t hpo_assemble_overlap_mixed(const double* centers, const int* l, const lon
   counts[1] = c1;
   counts[2] = c2;
   return n_ao;
+}
+
+// Per-entry envelope of assemble_overlap_mixed: env_uv = sum over the primitive pairs of
+// |term_pq|, the magnitudes of the terms whose signed sum is S_uv, each term formed in its
+// pair's tier arithmetic as the assembly forms it (f64, or dd for the F128 tier, where
+// P - A keeps the bits that f64 cancels).  A relative change d in each term's exp moves
+// S_uv by about d * env_uv, so env_uv scales a per-entry bound that is tight where the
+// contraction does not cancel and wider only where it does.
+void hpo_assemble_overlap_envelope(const double* centers, const int* l, const long* prim_off, const double* alpha,
+                                   const double* coeff, size_t n_shells, const double* params4, double* env) {
+  std::vector<long> off(n_shells + 1, 0);
+  for (size_t i = 0; i < n_shells; ++i) off[i + 1] = off[i] + (l[i] + 1) * (l[i] + 2) / 2;
+  const size_t n_ao = (size_t)off[n_shells];
+  std::memset(env, 0, sizeof(double) * n_ao * n_ao);
+#pragma omp parallel for schedule(dynamic, 4)
+  for (long ii = 0; ii < (long)n_shells; ++ii) {
+    const size_t i = (size_t)ii;
+    for (size_t j = 0; j <= i; ++j) {
+      ShellView A{centers + 3 * i, l[i], prim_off[i], prim_off[i + 1]};
+      ShellView B{centers + 3 * j, l[j], prim_off[j], prim_off[j + 1]};
+      const int tier = screen_pair(A, B, alpha, params4);
+      if (tier == 2) continue;
+      int ca[10][3], cb[10][3];
+      const int ni = cart_components(A.l, ca), nj = cart_components(B.l, cb), nb = B.l + 1;
+      double block[100] = {};
+      for (long p = A.p0; p < A.p1; ++p)
+        for (long q = B.p0; q < B.p1; ++q) {
+          double sx[16], sy[16], sz[16], w;
+          if (tier == 0) {
+            double ai = alpha[p], aj = alpha[q], gamma = ai + aj, mu = ai * aj / gamma;
+            double dx = A.c[0] - B.c[0], dy = A.c[1] - B.c[1], dz = A.c[2] - B.c[2];
+            double sq = std::sqrt(PI_F64 / gamma);
+            w = std::fabs(coeff[p] * coeff[q]) * sq * sq * sq * std::exp(-mu * (dx * dx + dy * dy + dz * dz));
+            double pa[3], pb[3];
+            for (int k = 0; k < 3; ++k) {
+              double pk = (ai * A.c[k] + aj * B.c[k]) / gamma;
+              pa[k] = pk - A.c[k];
+              pb[k] = pk - B.c[k];
+            }
+            os1d_f64(A.l, B.l, pa[0], pb[0], gamma, sx);
+            os1d_f64(A.l, B.l, pa[1], pb[1], gamma, sy);
+            os1d_f64(A.l, B.l, pa[2], pb[2], gamma, sz);
+          } else {
+            D ai = dd_from(alpha[p]), aj = dd_from(alpha[q]);
+            D gamma = add(ai, aj);
+            w = std::fabs(coeff[p] * coeff[q]) *
+                std::fabs(hpo_overlap_primitive_f128(alpha[p], A.c, alpha[q], B.c).hi);
+            D pa[3], pb[3], tx[16], ty[16], tz[16];
+            for (int k = 0; k < 3; ++k) {
+              D cik = dd_from(A.c[k]), cjk = dd_from(B.c[k]);
+              D pk = div(add(mul(ai, cik), mul(aj, cjk)), gamma);
+              pa[k] = sub(pk, cik);
+              pb[k] = sub(pk, cjk);
+            }
+            os1d_dd(A.l, B.l, pa[0], pb[0], gamma, tx);
+            os1d_dd(A.l, B.l, pa[1], pb[1], gamma, ty);
+            os1d_dd(A.l, B.l, pa[2], pb[2], gamma, tz);
+            for (int k = 0; k < (A.l + 1) * nb; ++k) {
+              sx[k] = tx[k].hi;
+              sy[k] = ty[k].hi;
+              sz[k] = tz[k].hi;
+            }
+          }
+          for (int u = 0; u < ni; ++u)
+            for (int v = 0; v < nj; ++v)
+              block[u * nj + v] +=
+                  w * std::fabs(sx[ca[u][0] * nb + cb[v][0]] * sy[ca[u][1] * nb + cb[v][1]] * sz[ca[u][2] * nb + cb[v][2]]);
+        }
+      for (int u = 0; u < ni; ++u)
+        for (int v = 0; v < nj; ++v) {
+          env[(off[i] + u) * n_ao + (off[j] + v)] = block[u * nj + v];
+          env[(off[j] + v) * n_ao + (off[i] + u)] = block[u * nj + v];
+        }
+    }
+  }
 }
 
 // Normalised s-type overlap built from the reference's dd operations.  The reference's

@@ -142,10 +142,19 @@ int hpo_overlap_fd_f256(const double* centers, const double* alpha, const double
 size_t hpo_assemble_overlap_mixed(const double* centers, const int* l, const long* prim_off, const double* alpha,
                                   const double* coeff, size_t n_shells, const double* params4, double* s,
                                   size_t* counts);
+/* Per-entry envelope of the same assembly: sum over primitive pairs of the magnitudes of
+ * the terms whose signed sum is S_uv (0 for skipped pairs).  Test infrastructure: the
+ * scale of the per-entry bound on the device's differences. */
+void hpo_assemble_overlap_envelope(const double* centers, const int* l, const long* prim_off, const double* alpha,
+                                   const double* coeff, size_t n_shells, const double* params4, double* env);
+/* Moves every exp of the overlap integrals by a deterministic pseudo-random number of ulps
+ * in [-ulps, ulps]; 0 restores libm's exp.  Test infrastructure only. */
+void hpo_set_exp_ulp_shift(int ulps);
 
 /* ---- reductions ---- */
 /* dot (crates/quad/src/linalg.rs:7-27): serial below 1024 elements, else 1024-element
- * chunk sums folded left to right (one of rayon's association orders) */
+ * chunk sums combined by rayon's halving tree (split at len/2 down to single chunks,
+ * each leaf ZERO + chunk sum) */
 hpo_dd hpo_dot(const double* a, const double* b, size_t n);
 /* gemv (linalg.rs:29-40): y_i = dot_serial(row_i, x) */
 void hpo_gemv(const double* a, size_t rows, size_t cols, const double* x, double* y);

@@ -105,6 +105,9 @@ def lib():
                                                  np.ctypeslib.ndpointer(dtype=np.int64, flags="C_CONTIGUOUS"),
                                                  _dp, _dp, _sz, _dp, _dp, C.POINTER(C.c_size_t)]
         L.hpo_assemble_overlap_mixed.restype = C.c_size_t
+        L.hpo_assemble_overlap_envelope.argtypes = L.hpo_assemble_overlap_mixed.argtypes[:7] + [_dp]
+        L.hpo_assemble_overlap_envelope.restype = None
+        L.hpo_set_exp_ulp_shift.argtypes = [C.c_int]
         L.hpo_dot.argtypes = [_dp, _dp, _sz]
         L.hpo_dot.restype = DD
         L.hpo_gemv.argtypes = [_dp, _sz, _sz, _dp, _dp]
@@ -445,6 +448,20 @@ def assemble_overlap_mixed(shells, params=DEFAULT_SCREEN):
     counts = (C.c_size_t * 3)()
     lib().hpo_assemble_overlap_mixed(centers, ls, off, alpha, coeff, len(shells), _c(params), s, counts)
     return s[:n_ao * n_ao].reshape(n_ao, n_ao), tuple(counts)
+
+
+def assemble_overlap_envelope(shells, params=DEFAULT_SCREEN):
+    """Per entry, the sum of |term| over the primitive pairs whose signed sum is S_uv."""
+    centers, ls, off, alpha, coeff = flatten_basis(shells)
+    n_ao = int(sum((l + 1) * (l + 2) // 2 for l in ls))
+    env = np.zeros(max(n_ao * n_ao, 1))
+    lib().hpo_assemble_overlap_envelope(centers, ls, off, alpha, coeff, len(shells), _c(params), env)
+    return env[:n_ao * n_ao].reshape(n_ao, n_ao)
+
+
+def set_exp_ulp_shift(ulps):
+    """Moves the overlap integrals' exp by up to +-ulps (deterministic); 0 is libm's exp."""
+    lib().hpo_set_exp_ulp_shift(int(ulps))
 
 
 # ---------------------------------------------------------------- reductions

@@ -57,17 +57,22 @@ def mp_block(shell_a, shell_b):
     ca, cb = cart(la), cart(lb)
     blk = np.zeros((len(ca), len(cb)))
     absb = np.zeros((len(ca), len(cb)))
+    envb = np.zeros((len(ca), len(cb)))
     for u, lu in enumerate(ca):
         for v, lv in enumerate(cb):
             tot = mp.mpf(0)
             ab = mp.mpf(0)
+            env = mp.mpf(0)
             for ai, ci in pa:
                 for aj, cj in pb:
-                    tot += mp.mpf(ci) * mp.mpf(cj) * mp_prim(ai, A, lu, aj, B, lv)
+                    term = mp.mpf(ci) * mp.mpf(cj) * mp_prim(ai, A, lu, aj, B, lv)
+                    tot += term
+                    env += abs(term)
                     ab += abs(mp.mpf(ci) * mp.mpf(cj)) * mp_prim(ai, A, lu, aj, B, lv, absolute=True)
             blk[u, v] = float(tot)
             absb[u, v] = float(ab)
-    return blk, absb
+            envb[u, v] = float(env)
+    return blk, absb, envb
 
 
 # ---------------------------------------------------------------- reference tests
@@ -160,15 +165,20 @@ def test_blocks_match_mpmath(seed, diffuse):
     assert c_dd[1] == n * (n + 1) // 2
     if diffuse:
         assert counts[1] > 0
+    env_dd = O.assemble_overlap_envelope(shells, ALL_DD)
+    env64 = O.assemble_overlap_envelope(shells)
     offs = np.cumsum([0] + [(l + 1) * (l + 2) // 2 for _, l, _ in shells])
     for i in range(n):
         for j in range(i + 1):
-            blk, absb = mp_block(shells[i], shells[j])
+            blk, absb, envb = mp_block(shells[i], shells[j])
             floor = absb.max() if i == j else 0.0  # same centre: odd components are rounding noise
-            for s, ulps in ((s_dd, 64 * 2.0 ** -53), (s64, 1e-12)):
+            for s, env, ulps in ((s_dd, env_dd, 64 * 2.0 ** -53), (s64, env64, 1e-12)):
                 got = s[offs[i]:offs[i + 1], offs[j]:offs[j + 1]]
                 bar = ulps * (absb + floor) + 1e-300
                 assert np.all(np.abs(got - blk) <= bar), (i, j, ulps, float((np.abs(got - blk) / bar).max()))
+                # the envelope: the sum of the magnitudes of the contraction's terms
+                e = env[offs[i]:offs[i + 1], offs[j]:offs[j + 1]]
+                assert np.all(np.abs(e - envb) <= bar), (i, j, float((np.abs(e - envb) / bar).max()))
 
 
 def test_single_primitive_s_equals_s_restatement():
@@ -181,3 +191,48 @@ def test_single_primitive_s_equals_s_restatement():
     s_full, c_full = O.assemble_overlap_mixed(shells)
     s_s, c_s = O.assemble_overlap_s(centers.ravel(), alpha, coeff)
     assert np.array_equal(s_full.ravel(), np.asarray(s_s).ravel()) and tuple(c_full) == tuple(c_s)
+
+
+# ---------------------------------------------------------------- per-entry bound
+U = 2.0 ** -53
+MOLECULES = [(1, 6, False, O.DEFAULT_SCREEN), (2, 10, True, O.DEFAULT_SCREEN), (3, 14, True, (0.3, 50.0, 1e-10, 6.0)),
+             (4, 8, True, ALL_DD)]
+
+
+@pytest.mark.parametrize("seed,atoms,diffuse,params", MOLECULES)
+def test_exp_differences_stay_within_the_entry_bound(seed, atoms, diffuse, params):
+    """The device assembly differs from this one only through libm's exp (each faithful, so
+    they differ by at most 2 ulps).  Moving every exp here by up to 2 ulps changes each
+    entry by at most 16 ulps of its envelope (the sum of the magnitudes of its terms): the
+    per-entry bar of the GPU test.  The envelope bounds |S| and is zero exactly where S is."""
+    from test_gpu_overlap_shells import molecule
+    shells = molecule(seed, atoms, diffuse)
+    s, _ = O.assemble_overlap_mixed(shells, params)
+    env = O.assemble_overlap_envelope(shells, params)
+    assert np.all(env * (1 + 8 * U) >= np.abs(s))
+    assert np.array_equal(env == 0, s == 0) or np.all(s[env == 0] == 0)
+    try:
+        O.set_exp_ulp_shift(2)
+        s2, _ = O.assemble_overlap_mixed(shells, params)
+    finally:
+        O.set_exp_ulp_shift(0)
+    d = np.abs(s2 - s)
+    assert (d > 0).mean() > 0.3  # the shift reaches most entries
+    assert np.all(d <= 16 * U * env), float((d / (16 * U * env + 1e-300)).max())
+
+
+def test_entry_bound_catches_small_entry_errors():
+    """A small entry wrong in its fifth digit passes a bar of 8 ulps of max |S| and fails the
+    per-entry bar."""
+    from test_gpu_overlap_shells import molecule
+    shells = molecule(2, 10, True)
+    s, _ = O.assemble_overlap_mixed(shells)
+    env = O.assemble_overlap_envelope(shells)
+    small = np.argwhere((np.abs(s) > 0) & (np.abs(s) < 1e-8 * np.abs(s).max()) & (env < 2 * np.abs(s)))
+    assert len(small) > 0
+    u, v = small[0]
+    bad = s.copy()
+    bad[u, v] *= 1 + 1e-5
+    d = np.abs(bad - s)
+    assert d.max() <= 8 * U * np.abs(s).max()
+    assert not np.all(d <= 16 * U * env)
