@@ -25,9 +25,12 @@
 //     mod m_r; then S = sum_r u_r (M/m_r) is accumulated in six 40-bit limbs, each sum
 //     below 32 * 255 * 2^40 < 2^53, and C' = S - round(S/M) M.
 //
-// Layouts: residue planes are [r][row][p] with the row pitch Kp = K rounded up to 64
-// (zero-filled), so A (m x k) and B^T (n x k) are loaded identically and every K stage
-// is whole; the digits of C are [r][i][j] bytes with the row pitch Np = N rounded up to 4.
+// Layouts: residue planes are [r][p / 64][row][p % 64]: K is cut into 64-deep slabs (Kp =
+// K rounded up to 64, zero-filled) and a slab holds every row's 64 bytes back to back, so
+// one K stage of a 256-row tile is 16 KB contiguous and its loads use whole cache lines
+// (with rows of pitch Kp, each stage used half of each 128-byte line); A (m x k) and B^T
+// (n x k) are loaded identically.  The digits of C are [r][i][j] bytes with the row pitch
+// Np = N rounded up to 4.
 #include <cmath>
 #include <cstdint>
 #include <cstring>
@@ -334,8 +337,8 @@ __global__ __launch_bounds__(256) void k_scale_cols(const double* __restrict__ h
 }
 
 // ------------------------------------------------------------------ residues
-// Element (row, p) of the logical rows x K operand -> out[r][row][p] (int8, centred),
-// row pitch Kp (p in [K, Kp) written as 0).
+// Element (row, p) of the logical rows x K operand -> out[r][p / 64][row][p % 64] (int8,
+// centred; p in [K, Kp) written as 0).
 //
 // Rows contiguous (element at base[row*ld + p]): a wavefront takes 512 p of one row, each
 // lane p = p0 + 4 lane + e and p0 + 256 + 4 lane + e (e < 4), and stores one dword of
@@ -366,16 +369,17 @@ __global__ __launch_bounds__(256) void k_residues_rows(const double* __restrict_
         }
         int_limbs(h, l, sc, t[4 * g + e][2], t[4 * g + e][1], t[4 * g + e][0]);
       }
-    int8_t* o = out + (size_t)row * Kp + p0;
     for (int r = 0; r < R; ++r) {
       const double c1 = cs[r][0], c2 = cs[r][1], m = cs[r][2], im = cs[r][3];
 #pragma unroll
       for (int g = 0; g < 2; ++g) {
-        if (p0 + 256 * g >= Kp) continue;  // Kp % 4 == 0: the whole dword is in the row
+        const long p = p0 + 256 * g;
+        if (p >= Kp) continue;  // Kp % 64 == 0: the whole dword is in the slab
         int b[4];
 #pragma unroll
         for (int e = 0; e < 4; ++e) b[e] = residue8(t[4 * g + e][2], t[4 * g + e][1], t[4 * g + e][0], c1, c2, m, im);
-        *reinterpret_cast<uint32_t*>(o + (size_t)r * plane + 256 * g) = pack4(b[0], b[1], b[2], b[3]);
+        *reinterpret_cast<uint32_t*>(out + (size_t)r * plane + (size_t)(p >> 6) * rows * 64 + row * 64 + (p & 63)) =
+            pack4(b[0], b[1], b[2], b[3]);
       }
     }
   }
@@ -387,7 +391,7 @@ __global__ __launch_bounds__(256) void k_residues_rows(const double* __restrict_
 // loaded while this tile's residues are formed (the blocks of a CU would otherwise load
 // and compute in phase).  The residues of one modulus are transposed through LDS (two
 // buffers, alternating by modulus, so one barrier per modulus) and leave as 16-byte
-// stores, 64 bytes per row.
+// stores: the tile is 4 KB contiguous in its slab.
 __global__ __launch_bounds__(256) void k_residues_cols(const double* __restrict__ hi, const double* __restrict__ lo,
                                                        long rows, long K, long Kp, long ld,
                                                        const int* __restrict__ escale, int W,
@@ -422,7 +426,7 @@ __global__ __launch_bounds__(256) void k_residues_cols(const double* __restrict_
     for (int e = 0; e < 16; ++e) int_limbs(hv[e], lv[e], sc, t[e][2], t[e][1], t[e][0]);
     if (row0 + stride < rows) fetch(row0 + stride);
     const bool sok = row0 + srow < rows;
-    int8_t* so = out + (size_t)(row0 + srow) * Kp + p0 + 16 * sch;
+    int8_t* so = out + (size_t)blockIdx.x * rows * 64 + (size_t)(row0 + srow) * 64 + 16 * sch;
     for (int r = 0; r < R; ++r) {
       const double c1 = cs[r][0], c2 = cs[r][1], m = cs[r][2], im = cs[r][3];
       v4i_t pk;
@@ -516,14 +520,15 @@ __global__ __launch_bounds__(512) void k_gemm_i8_mod(const int8_t* __restrict__ 
   const bool isA = t < 256;
   const long srow = (isA ? i0 : j0) + (t & 255);
   const long slim = isA ? M : N;
-  const int8_t* ps = (isA ? A : B) + srow * K;
+  const int8_t* ps = (isA ? A : B) + srow * 64;
   int8_t (*Ss)[GM][GKP] = isA ? As : Bs;
   v4i_t rs[NV];
   auto fetch = [&](long k0) {
 #pragma unroll
     for (int v = 0; v < NV; ++v) {
       const long kk = k0 + 16 * v;
-      rs[v] = (srow < slim && kk < K) ? *reinterpret_cast<const v4i_t*>(ps + kk) : v4i_t{0, 0, 0, 0};
+      rs[v] = (srow < slim && kk < K) ? *reinterpret_cast<const v4i_t*>(ps + (kk >> 6) * slim * 64 + (kk & 63))
+                                      : v4i_t{0, 0, 0, 0};
     }
   };
   const long nst = (K + GK - 1) / GK;
@@ -587,6 +592,7 @@ __global__ __launch_bounds__(512) void k_gemm_i8_mod_dl(const int8_t* __restrict
   // this lane's part of the 32 load instructions of a stage (wavefront w issues 4w..4w+3):
   // instruction q covers 16 rows of operand q >> 4 starting at row 16 (q & 15)
   const int8_t* src[4];
+  long sstr[4];  // bytes per K slab of the operand
   int dst[4];
 #pragma unroll
   for (int j = 0; j < 4; ++j) {
@@ -594,14 +600,15 @@ __global__ __launch_bounds__(512) void k_gemm_i8_mod_dl(const int8_t* __restrict
     const int row = rb + (lane >> 2);
     const int chunk = (lane & 3) ^ ((row >> 2) & 3);
     const long grow = op == 0 ? min(i0 + row, M - 1) : min(j0 + row, N - 1);
-    src[j] = (op == 0 ? A : B) + grow * K + 16 * chunk;
+    src[j] = (op == 0 ? A : B) + grow * 64 + 16 * chunk;
+    sstr[j] = (op == 0 ? M : N) * 64;
     dst[j] = op * (256 * DL_GK) + rb * DL_GK;
   }
   auto issue = [&](long st) {
     int8_t* slot = smem + (size_t)(st % DL_RING) * DL_SLOT;
 #pragma unroll
     for (int j = 0; j < 4; ++j)
-      __builtin_amdgcn_global_load_lds(src[j] + st * DL_GK, (__attribute__((address_space(3))) void*)(slot + dst[j]),
+      __builtin_amdgcn_global_load_lds(src[j] + st * sstr[j], (__attribute__((address_space(3))) void*)(slot + dst[j]),
                                        16, 0, 0);
   };
   auto frag = [&](const int8_t* tile, int row, int c) {
@@ -701,6 +708,7 @@ __device__ __forceinline__ void gemm_dlp_body(const int8_t* __restrict__ A, cons
   const long i0 = ty * GM, j0 = tx * GN;
 
   const int8_t* src[LPW];
+  long sstr[LPW];  // bytes per K slab of the operand
   int dst[LPW];
 #pragma unroll
   for (int j = 0; j < LPW; ++j) {
@@ -708,14 +716,15 @@ __device__ __forceinline__ void gemm_dlp_body(const int8_t* __restrict__ A, cons
     const int row = rb + (lane >> 2);
     const int chunk = (lane & 3) ^ ((row >> 2) & 3);
     const long grow = op == 0 ? min(i0 + row, M - 1) : min(j0 + row, N - 1);
-    src[j] = (op == 0 ? A : B) + grow * K + 16 * chunk;
+    src[j] = (op == 0 ? A : B) + grow * 64 + 16 * chunk;
+    sstr[j] = (op == 0 ? M : N) * 64;
     dst[j] = op * (256 * DL_GK) + rb * DL_GK;
   }
   auto issue = [&](long st) {
     int8_t* slot = smem + (size_t)(st % DL_RING) * DL_SLOT;
 #pragma unroll
     for (int j = 0; j < LPW; ++j)
-      __builtin_amdgcn_global_load_lds(src[j] + st * DL_GK, (__attribute__((address_space(3))) void*)(slot + dst[j]),
+      __builtin_amdgcn_global_load_lds(src[j] + st * sstr[j], (__attribute__((address_space(3))) void*)(slot + dst[j]),
                                        16, 0, 0);
   };
   // this wavefront's loads of the next stage done (at most `ahead` later stages in flight),
@@ -842,18 +851,20 @@ __global__ __launch_bounds__(512) void k_gemm_i8_mod_rs(const int8_t* __restrict
   // piece j (< 4) of this lane: chunk lane & 3 of row 16 (4w + j mod 16) + lane / 4 of operand
   // (4w + j) / 16
   const int8_t* src[4];
+  long sstr[4];  // bytes per K slab of the operand
   int dst[4];
 #pragma unroll
   for (int j = 0; j < 4; ++j) {
     const int q = 4 * w + j, op = q >> 4, rb = (q & 15) * 16;
     const int row = rb + (lane >> 2), c = lane & 3;
-    src[j] = (op == 0 ? A + min(i0 + row, M - 1) * K : B + min(j0 + row, N - 1) * K) + 16 * c;
+    src[j] = (op == 0 ? A + min(i0 + row, M - 1) * 64 : B + min(j0 + row, N - 1) * 64) + 16 * c;
+    sstr[j] = (op == 0 ? M : N) * 64;
     dst[j] = op * (256 * DL_GK) + row * DL_GK + ((c ^ ((row >> 2) & 3)) << 4);
   }
   auto load = [&](long st, v4i_t (&R)[4]) {  // stages past the end re-read the last one (in range)
-    const long s = min(st, nst - 1) * DL_GK;
+    const long s = min(st, nst - 1);
 #pragma unroll
-    for (int j = 0; j < 4; ++j) R[j] = *reinterpret_cast<const v4i_t*>(src[j] + s);
+    for (int j = 0; j < 4; ++j) R[j] = *reinterpret_cast<const v4i_t*>(src[j] + s * sstr[j]);
   };
   auto store = [&](long st, const v4i_t (&R)[4]) {
     int8_t* slot = smem + (size_t)(st & 1) * DL_SLOT;
