@@ -106,6 +106,15 @@ unsafe extern "C" {
     pub fn hp_mf4_elementwise(op: c_int, a: *const f64, b: *const f64, out: *mut f64, n: usize) -> c_int;
     pub fn hp_mf4_from_dd(dd: *const f64, mf4: *mut f64, n: usize) -> c_int;
     pub fn hp_dd_from_mf4(mf4: *const f64, dd: *mut f64, n: usize) -> c_int;
+    pub fn hp_mfn_gemm(nl: c_int, a: *const f64, b: *const f64, c: *mut f64, m: usize, n: usize, k: usize) -> c_int;
+    pub fn hp_mfn_gemm_atb(nl: c_int, a: *const f64, b: *const f64, c: *mut f64, m: usize, n: usize, k: usize) -> c_int;
+    pub fn hp_dev_mfn_gemm(nl: c_int, trans_a: c_int, a: *const f64, b: *const f64, c: *mut f64, m: usize, n: usize, k: usize, stream: *mut c_void) -> c_int;
+    pub fn hp_mfn_gemv(nl: c_int, a: *const f64, x: *const f64, y: *mut f64, n: usize, m: usize) -> c_int;
+    pub fn hp_mfn_dot(nl: c_int, a: *const f64, b: *const f64, n: usize, out: *mut f64) -> c_int;
+    pub fn hp_mfn_cholesky(nl: c_int, a: *mut f64, n: usize, pivot: *mut usize) -> c_int;
+    pub fn hp_mfn_cholesky_solve(nl: c_int, l: *const f64, b: *const f64, x: *mut f64, n: usize) -> c_int;
+    pub fn hp_mfn_elementwise(nl: c_int, op: c_int, a: *const f64, b: *const f64, out: *mut f64, n: usize) -> c_int;
+    pub fn hp_mfn_convert(nl_from: c_int, nl_to: c_int, in_: *const f64, out: *mut f64, n: usize) -> c_int;
     pub fn hp_cholesky(a: *mut f64, n: usize, pivot: *mut usize, value: *mut f64) -> c_int;
     pub fn hp_cholesky_blocked(a: *mut f64, n: usize, nb: usize, pivot: *mut usize, value: *mut f64) -> c_int;
     pub fn hp_dev_cholesky(a_hi: *mut f64, a_lo: *mut f64, n: usize, nb: usize, unblocked: c_int, pivot: *mut usize, value: *mut f64, stream: *mut c_void) -> c_int;

@@ -6,8 +6,8 @@
 //! include/hyperprec_hip.h (raw items in `ffi`).  A caller switches with
 //! `use hyperprec_hip as quad;` (or `as quadprec_solver` for the solver functions).
 //!
-//! Layout: the functions pass `&[f128]` / `&[MultiFloat<4>]` as plain `f64` arrays
-//! ({hi, lo} and four limbs per value).  The reference's structs are not `#[repr(C)]`
+//! Layout: the functions pass `&[f128]` / `&[MultiFloat<N>]` as plain `f64` arrays
+//! ({hi, lo} and N limbs per value).  The reference's structs are not `#[repr(C)]`
 //! (crates/quad/src/dd.rs:6-11, multifloat/core.rs:4-8); the drop-in patch adds
 //! `#[repr(C)]` to both (same fields, same size: non-breaking).  The size checks below
 //! catch a layout change at compile time.
@@ -17,7 +17,8 @@
 //! `CholeskyError` / `SolveError`; a HIP failure (no representation in the reference API)
 //! panics with the library's message.
 //!
-//! `MultiFloat<N>` entry points exist for N = 4 (f256) only; other N panic with a message.
+//! The `MultiFloat<N>` functions take any N in 1..=8 (f256 = N 4, f512 = N 8), as the
+//! reference's mul does (multifloat/mul.rs:21, N <= 8); a larger N panics with a message.
 
 pub mod ffi;
 
@@ -25,7 +26,7 @@ use quad::{f128, CholeskyError, MultiFloat};
 use std::os::raw::c_int;
 
 const _: () = assert!(std::mem::size_of::<f128>() == 16 && std::mem::align_of::<f128>() == 8);
-const _: () = assert!(std::mem::size_of::<MultiFloat<4>>() == 32);
+const _: () = assert!(std::mem::size_of::<MultiFloat<4>>() == 32 && std::mem::size_of::<MultiFloat<8>>() == 64);
 
 fn last_error() -> String {
     unsafe { std::ffi::CStr::from_ptr(ffi::hp_last_error()) }.to_string_lossy().into_owned()
@@ -45,12 +46,14 @@ fn dd(p: &[f128]) -> *const f64 {
 fn dd_mut(p: &mut [f128]) -> *mut f64 {
     p.as_mut_ptr() as *mut f64
 }
-fn mf4<const N: usize>(p: &[MultiFloat<N>]) -> *const f64 {
-    assert!(N == 4, "hyperprec_hip provides MultiFloat<4> (f256) only, not MultiFloat<{N}>");
+fn limbs<const N: usize>() -> c_int {
+    assert!((1..=8).contains(&N), "hyperprec_hip provides MultiFloat<N> for N in 1..=8, not {N}");
+    N as c_int
+}
+fn mf<const N: usize>(p: &[MultiFloat<N>]) -> *const f64 {
     p.as_ptr() as *const f64
 }
-fn mf4_mut<const N: usize>(p: &mut [MultiFloat<N>]) -> *mut f64 {
-    assert!(N == 4, "hyperprec_hip provides MultiFloat<4> (f256) only, not MultiFloat<{N}>");
+fn mf_mut<const N: usize>(p: &mut [MultiFloat<N>]) -> *mut f64 {
     p.as_mut_ptr() as *mut f64
 }
 
@@ -70,22 +73,22 @@ pub fn gemm_atb(a: &[f128], b: &[f128], c: &mut [f128], m: usize, n: usize, k: u
     check(unsafe { ffi::hp_gemm_atb(dd(a), dd(b), dd_mut(c), m, n, k) });
 }
 
-/// mf_gemm::<4> (crates/quad/src/multifloat/linalg.rs:39-61), bit-identical.
+/// mf_gemm::<N> (crates/quad/src/multifloat/linalg.rs:39-61), bit-identical.
 pub fn mf_gemm<const N: usize>(a: &[MultiFloat<N>], b: &[MultiFloat<N>], c: &mut [MultiFloat<N>], m: usize, n: usize,
                                k: usize) {
     assert_eq!(a.len(), m * k);
     assert_eq!(b.len(), k * n);
     assert_eq!(c.len(), m * n);
-    check(unsafe { ffi::hp_mf_gemm(mf4(a), mf4(b), mf4_mut(c), m, n, k) });
+    check(unsafe { ffi::hp_mfn_gemm(limbs::<N>(), mf(a), mf(b), mf_mut(c), m, n, k) });
 }
 
-/// mf_gemm_atb::<4> (multifloat/linalg.rs:64-85), bit-identical.
+/// mf_gemm_atb::<N> (multifloat/linalg.rs:64-85), bit-identical.
 pub fn mf_gemm_atb<const N: usize>(a: &[MultiFloat<N>], b: &[MultiFloat<N>], c: &mut [MultiFloat<N>], m: usize,
                                    n: usize, k: usize) {
     assert_eq!(a.len(), k * m);
     assert_eq!(b.len(), k * n);
     assert_eq!(c.len(), m * n);
-    check(unsafe { ffi::hp_mf_gemm_atb(mf4(a), mf4(b), mf4_mut(c), m, n, k) });
+    check(unsafe { ffi::hp_mfn_gemm_atb(limbs::<N>(), mf(a), mf(b), mf_mut(c), m, n, k) });
 }
 
 /// In-place unblocked Cholesky (linalg.rs:42-96): lower triangle = L, upper keeps A.
@@ -174,11 +177,11 @@ pub fn jacobi_eigen(a: &[f128], n: usize, max_iter: usize) -> Result<(Vec<f128>,
     Ok((evals, evecs))
 }
 
-/// mf_cholesky::<4> (multifloat/linalg.rs:89-141).
+/// mf_cholesky::<N> (multifloat/linalg.rs:89-141).
 pub fn mf_cholesky<const N: usize>(a: &mut [MultiFloat<N>], n: usize) -> Result<(), usize> {
     assert_eq!(a.len(), n * n);
     let mut pivot = 0usize;
-    let rc = unsafe { ffi::hp_mf_cholesky(mf4_mut(a), n, &mut pivot) };
+    let rc = unsafe { ffi::hp_mfn_cholesky(limbs::<N>(), mf_mut(a), n, &mut pivot) };
     if rc == ffi::HP_ENOTSPD {
         return Err(pivot);
     }
@@ -186,22 +189,30 @@ pub fn mf_cholesky<const N: usize>(a: &mut [MultiFloat<N>], n: usize) -> Result<
     Ok(())
 }
 
-/// mf_cholesky_solve::<4> (multifloat/linalg.rs:145-172).
+/// mf_cholesky_solve::<N> (multifloat/linalg.rs:145-172).
 pub fn mf_cholesky_solve<const N: usize>(l: &[MultiFloat<N>], b: &[MultiFloat<N>], n: usize) -> Vec<MultiFloat<N>> {
     assert_eq!(l.len(), n * n);
     assert_eq!(b.len(), n);
     let mut x = vec![MultiFloat::<N> { limbs: [0.0; N] }; n];
-    check(unsafe { ffi::hp_mf_cholesky_solve(mf4(l), mf4(b), mf4_mut(&mut x), n) });
+    check(unsafe { ffi::hp_mfn_cholesky_solve(limbs::<N>(), mf(l), mf(b), mf_mut(&mut x), n) });
     x
 }
 
-/// mf_gemv::<4> (multifloat/linalg.rs:17-36), A n x m.
+/// mf_gemv::<N> (multifloat/linalg.rs:17-36), A n x m.
 pub fn mf_gemv<const N: usize>(a: &[MultiFloat<N>], x: &[MultiFloat<N>], y: &mut [MultiFloat<N>], n: usize,
                                m: usize) {
     assert_eq!(a.len(), n * m);
     assert_eq!(x.len(), m);
     assert_eq!(y.len(), n);
-    check(unsafe { ffi::hp_mf_gemv(mf4(a), mf4(x), mf4_mut(y), n, m) });
+    check(unsafe { ffi::hp_mfn_gemv(limbs::<N>(), mf(a), mf(x), mf_mut(y), n, m) });
+}
+
+/// mf_dot::<N> (multifloat/linalg.rs:7-14).
+pub fn mf_dot<const N: usize>(a: &[MultiFloat<N>], b: &[MultiFloat<N>]) -> MultiFloat<N> {
+    assert_eq!(a.len(), b.len());
+    let mut out = MultiFloat::<N> { limbs: [0.0; N] };
+    check(unsafe { ffi::hp_mfn_dot(limbs::<N>(), mf(a), mf(b), a.len(), out.limbs.as_mut_ptr()) });
+    out
 }
 
 /// The normalised s-type Gaussian overlap matrix in dd (the single-primitive s-s case of

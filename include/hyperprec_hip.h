@@ -148,6 +148,38 @@ int hp_mf4_elementwise(int op, const double* a, const double* b, double* out, si
 int hp_mf4_from_dd(const double* dd, double* mf4, size_t n);
 int hp_dd_from_mf4(const double* mf4, double* dd, size_t n);
 
+/* ---- MultiFloat<N>, N = 1..8 (f512 = MultiFloat<8>; N = 4 is f256 and takes the kernels
+ * above).  The reference's generic expansion arithmetic (multifloat/{core,add,mul,div,
+ * math}.rs) and linear algebra (multifloat/linalg.rs), bit-identical per element.  Values
+ * are AoS arrays of nl doubles (limb 0 most significant), as [MultiFloat<N>] is in memory;
+ * nl outside 1..8 is HP_EINVAL. */
+/* C = A*B, A m x k, B k x n.  Replaces mf_gemm::<N> (multifloat/linalg.rs:39-61). */
+int hp_mfn_gemm(int nl, const double* a, const double* b, double* c, size_t m, size_t n, size_t k);
+/* C = A^T B, A stored k x m.  Replaces mf_gemm_atb::<N> (multifloat/linalg.rs:64-85). */
+int hp_mfn_gemm_atb(int nl, const double* a, const double* b, double* c, size_t m, size_t n, size_t k);
+/* The same on device limb planes (plane l of an r x c matrix at offset l*r*c), enqueued
+ * on `stream` (a hipStream_t; NULL is the default stream). */
+int hp_dev_mfn_gemm(int nl, int trans_a, const double* a, const double* b, double* c, size_t m, size_t n, size_t k,
+                    void* stream);
+/* y = A x, A n x m.  Replaces mf_gemv::<N> (multifloat/linalg.rs:17-36). */
+int hp_mfn_gemv(int nl, const double* a, const double* x, double* y, size_t n, size_t m);
+/* *out = sum a_i b_i from ZERO in index order.  Replaces mf_dot::<N> (multifloat/linalg.rs:7-14). */
+int hp_mfn_dot(int nl, const double* a, const double* b, size_t n, double* out);
+/* In-place Cholesky, replaces mf_cholesky::<N> (multifloat/linalg.rs:89-141): HP_ENOTSPD
+ * with *pivot at a column whose diag.limbs[0] <= 0 (the columns before it factored, as the
+ * reference leaves them; the upper triangle keeps A). */
+int hp_mfn_cholesky(int nl, double* a, size_t n, size_t* pivot);
+/* L L^T x = b, replaces mf_cholesky_solve::<N> (multifloat/linalg.rs:145-172). */
+int hp_mfn_cholesky_solve(int nl, const double* l, const double* b, double* x, size_t n);
+/* out_i = op(a_i, b_i): 0 a + b, 1 a - b, 2 a * b, 3 a / b (add.rs:19-114, mul.rs:12-111,
+ * div.rs:44-77), 4 a.recip() (div.rs:14-41), 5 a.sqrt() (math.rs:22-40), 6 a.renorm()
+ * (core.rs:231, renormalize); b is unused (may be NULL) for ops 4-6. */
+int hp_mfn_elementwise(int nl, int op, const double* a, const double* b, double* out, size_t n);
+/* MultiFloat<nl_to> from MultiFloat<nl_from>: the first min(nl_from, nl_to) limbs, the rest
+ * zero: from_f128 / to_f128 / from_f256 / to_f256 (multifloat/conv.rs:119-157; f128 is the
+ * 2-limb case). */
+int hp_mfn_convert(int nl_from, int nl_to, const double* in, double* out, size_t n);
+
 /* ------------------------------------------------------------------ Cholesky and solves */
 
 /* In-place unblocked Cholesky A = L L^T of an n x n dd matrix (row-major, AoS).

@@ -151,6 +151,18 @@ void hpo_assemble_overlap_envelope(const double* centers, const int* l, const lo
  * in [-ulps, ulps]; 0 restores libm's exp.  Test infrastructure only. */
 void hpo_set_exp_ulp_shift(int ulps);
 
+/* ---- MultiFloat<N>, N = 1..8 (hpo_mfn.cpp): the generic expansion arithmetic and linear
+ * algebra of multifloat/{core,add,mul,div,math,linalg}.rs.  AoS arrays of N doubles per
+ * value.  Return -1 for N outside 1..8. */
+/* op: 0 add, 1 sub, 2 mul, 3 div, 4 recip(a), 5 sqrt(a), 6 renormalize(a) */
+int hpo_mfn_elementwise(int nl, int op, const double* a, const double* b, double* out, size_t n);
+/* mf_gemm / mf_gemm_atb (linalg.rs:39-85) for rows row0 .. row0+nrows-1 of C */
+int hpo_mfn_gemm_rows(int nl, int trans_a, const double* a, const double* b, double* c, size_t m, size_t n, size_t k,
+                      size_t row0, size_t nrows);
+int hpo_mfn_gemv(int nl, const double* a, const double* x, double* y, size_t n, size_t m); /* linalg.rs:17-36 */
+long hpo_mfn_cholesky(int nl, double* a, size_t n); /* linalg.rs:89-141: 1 + failing column, 0 on success */
+int hpo_mfn_cholesky_solve(int nl, const double* l, const double* b, double* x, size_t n); /* 145-172 */
+
 /* ---- reductions ---- */
 /* dot (crates/quad/src/linalg.rs:7-27): serial below 1024 elements, else 1024-element
  * chunk sums combined by rayon's halving tree (split at len/2 down to single chunks,

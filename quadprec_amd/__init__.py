@@ -169,6 +169,15 @@ def lib():
     L.hp_mf_gemv.argtypes = [_dp, _dp, _dp, _sz, _sz]
     L.hp_mf4_elementwise.argtypes = [C.c_int, _dp, _dp, _dp, _sz]
     L.hp_mf4_from_dd.argtypes = [_dp, _dp, _sz]
+    for name in ("hp_mfn_gemm", "hp_mfn_gemm_atb"):
+        getattr(L, name).argtypes = [C.c_int, _dp, _dp, _dp, _sz, _sz, _sz]
+    L.hp_dev_mfn_gemm.argtypes = [C.c_int, C.c_int, _vp, _vp, _vp, _sz, _sz, _sz, _vp]
+    L.hp_mfn_gemv.argtypes = [C.c_int, _dp, _dp, _dp, _sz, _sz]
+    L.hp_mfn_dot.argtypes = [C.c_int, _dp, _dp, _sz, _dp]
+    L.hp_mfn_cholesky.argtypes = [C.c_int, _dp, _sz, C.POINTER(C.c_size_t)]
+    L.hp_mfn_cholesky_solve.argtypes = [C.c_int, _dp, _dp, _dp, _sz]
+    L.hp_mfn_elementwise.argtypes = [C.c_int, C.c_int, _dp, _dp, _dp, _sz]
+    L.hp_mfn_convert.argtypes = [C.c_int, C.c_int, _dp, _dp, _sz]
     L.hp_dd_from_mf4.argtypes = [_dp, _dp, _sz]
     L.hp_overlap_s.argtypes = [_dp, _dp, _sz, _dp]
     L.hp_dev_overlap_s.argtypes = [_vp, _vp, _sz, _vp, _vp, _vp]
@@ -504,6 +513,86 @@ def dd_from_mf4(v):
     v = np.ascontiguousarray(v, dtype=np.float64).reshape(-1, 4)
     out = np.zeros((max(len(v), 1), 2))
     _check(lib().hp_dd_from_mf4(v if len(v) else np.zeros(4), out, len(v)), "dd_from_mf4")
+    return out[:len(v)]
+
+
+# ---------------------------------------------------------------- MultiFloat<N>, N = 1..8
+MFN_OPS = {"add": 0, "sub": 1, "mul": 2, "div": 3, "recip": 4, "sqrt": 5, "renormalize": 6}
+
+
+def _mfn(v, nl, count, name):
+    v = np.ascontiguousarray(v, dtype=np.float64).reshape(-1, nl)
+    if v.shape[0] != count:
+        raise AssertionError(f"{name}: {v.shape[0]} values, expected {count}")
+    return v if count else np.zeros((1, nl))
+
+
+def mfn_gemm(nl, a, b, m, n, k, trans_a=False):
+    """mf_gemm::<nl> (multifloat/linalg.rs:39-61), or mf_gemm_atb::<nl> (64-85, A stored
+    k x m) when trans_a: C (m*n, nl)."""
+    a, b = _mfn(a, nl, m * k, "a"), _mfn(b, nl, k * n, "b")
+    c = np.zeros((max(m * n, 1), nl))
+    name = "hp_mfn_gemm_atb" if trans_a else "hp_mfn_gemm"
+    _check(getattr(lib(), name)(nl, a, b, c, m, n, k), name)
+    return c[:m * n]
+
+
+def mfn_gemv(nl, a, x, n, m):
+    """mf_gemv::<nl> (multifloat/linalg.rs:17-36): y = A x, A n x m; (n, nl)."""
+    a, x = _mfn(a, nl, n * m, "a"), _mfn(x, nl, m, "x")
+    y = np.zeros((max(n, 1), nl))
+    _check(lib().hp_mfn_gemv(nl, a, x, y, n, m), "mfn_gemv")
+    return y[:n]
+
+
+def mfn_dot(nl, a, b):
+    """mf_dot::<nl> (multifloat/linalg.rs:7-14): (nl,)."""
+    a = np.ascontiguousarray(a, dtype=np.float64).reshape(-1, nl)
+    b = _mfn(b, nl, a.shape[0], "b")
+    out = np.zeros(nl)
+    _check(lib().hp_mfn_dot(nl, a if a.size else np.zeros(nl), b, a.shape[0], out), "mfn_dot")
+    return out
+
+
+def mfn_cholesky(nl, a, n):
+    """mf_cholesky::<nl> (multifloat/linalg.rs:89-141): the factored copy (n*n, nl); raises
+    NotPositiveDefinite (with .pivot) like the reference's Err."""
+    a = _mfn(a, nl, n * n, "a").copy()
+    piv = C.c_size_t(0)
+    rc = lib().hp_mfn_cholesky(nl, a, n, C.byref(piv))
+    if rc == HP_ENOTSPD:
+        e = NotPositiveDefinite(lib().hp_last_error().decode(errors="replace"))
+        e.pivot = piv.value
+        raise e
+    _check(rc, "mfn_cholesky")
+    return a[:n * n]
+
+
+def mfn_cholesky_solve(nl, l, b, n):
+    """mf_cholesky_solve::<nl> (multifloat/linalg.rs:145-172): x (n, nl)."""
+    l, b = _mfn(l, nl, n * n, "l"), _mfn(b, nl, n, "b")
+    x = np.zeros((max(n, 1), nl))
+    _check(lib().hp_mfn_cholesky_solve(nl, l, b, x, n), "mfn_cholesky_solve")
+    return x[:n]
+
+
+def mfn_elementwise(nl, op, a, b=None):
+    """op(a_i, b_i) over (count, nl) arrays: "add", "sub", "mul", "div" (add.rs, mul.rs,
+    div.rs), "recip", "sqrt", "renormalize" (b unused)."""
+    a = np.ascontiguousarray(a, dtype=np.float64).reshape(-1, nl)
+    cnt = a.shape[0]
+    bb = _mfn(b, nl, cnt, "b") if b is not None else np.zeros((1, nl))
+    out = np.zeros((max(cnt, 1), nl))
+    _check(lib().hp_mfn_elementwise(nl, MFN_OPS[op], a if cnt else np.zeros(nl), bb, out, cnt), "mfn_elementwise")
+    return out[:cnt]
+
+
+def mfn_convert(nl_from, nl_to, v):
+    """MultiFloat<nl_to> from MultiFloat<nl_from> (multifloat/conv.rs:119-157): leading limbs
+    kept, the rest zero; (count, nl_to)."""
+    v = np.ascontiguousarray(v, dtype=np.float64).reshape(-1, nl_from)
+    out = np.zeros((max(len(v), 1), nl_to))
+    _check(lib().hp_mfn_convert(nl_from, nl_to, v if len(v) else np.zeros(nl_from), out, len(v)), "mfn_convert")
     return out[:len(v)]
 
 

@@ -124,7 +124,10 @@ int PhaseTimer::collect(double* sum, int max, int* ncalls) {
 using namespace hp;
 
 namespace {
-int gemm_host(int trans_a, int limbs, const double* a, const double* b, double* c, size_t m, size_t n, size_t k) {
+// limbs 2 with dd: the f128 kernel; otherwise MultiFloat<limbs> (MultiFloat<2> is not f128:
+// its add and mul are the expansion algorithms)
+int gemm_host(int trans_a, int limbs, bool dd, const double* a, const double* b, double* c, size_t m, size_t n,
+              size_t k) {
   if (!a || !b || !c) {
     if (m * n == 0) return HP_OK;
     set_error("gemm: null pointer");
@@ -137,11 +140,11 @@ int gemm_host(int trans_a, int limbs, const double* a, const double* b, double* 
       (rc = dc.alloc(m * n * limbs * sizeof(double))))
     return rc;
   if ((rc = upload_planes(a, m * k, limbs, da.d(), s)) || (rc = upload_planes(b, k * n, limbs, db.d(), s))) return rc;
-  if (limbs == 2)
+  if (dd)
     rc = launch_gemm_dd_plain(trans_a, da.d(), da.d() + m * k, db.d(), db.d() + k * n, dc.d(), dc.d() + m * n, m, n,
                               k, s);
   else
-    rc = launch_mf4_gemm(trans_a, da.d(), db.d(), dc.d(), m, n, k, s);
+    rc = launch_mfn_gemm(limbs, trans_a, da.d(), db.d(), dc.d(), m, n, k, s);
   if (rc) return rc;
   return download_planes(dc.d(), m * n, limbs, c, s);
 }
@@ -213,16 +216,24 @@ int hp_get_phase_times(double* ms, int max) {
 int hp_version(void) { return 200; }
 
 int hp_gemm(const double* a, const double* b, double* c, size_t m, size_t n, size_t k) {
-  return gemm_host(0, 2, a, b, c, m, n, k);
+  return gemm_host(0, 2, true, a, b, c, m, n, k);
 }
 int hp_gemm_atb(const double* a, const double* b, double* c, size_t m, size_t n, size_t k) {
-  return gemm_host(1, 2, a, b, c, m, n, k);
+  return gemm_host(1, 2, true, a, b, c, m, n, k);
 }
 int hp_mf_gemm(const double* a, const double* b, double* c, size_t m, size_t n, size_t k) {
-  return gemm_host(0, 4, a, b, c, m, n, k);
+  return gemm_host(0, 4, false, a, b, c, m, n, k);
 }
 int hp_mf_gemm_atb(const double* a, const double* b, double* c, size_t m, size_t n, size_t k) {
-  return gemm_host(1, 4, a, b, c, m, n, k);
+  return gemm_host(1, 4, false, a, b, c, m, n, k);
+}
+int hp_mfn_gemm(int nl, const double* a, const double* b, double* c, size_t m, size_t n, size_t k) {
+  if (nl < 1 || nl > 8) return launch_mfn_gemm(nl, 0, a, b, c, m, n, k, nullptr);  // the error
+  return gemm_host(0, nl, false, a, b, c, m, n, k);
+}
+int hp_mfn_gemm_atb(int nl, const double* a, const double* b, double* c, size_t m, size_t n, size_t k) {
+  if (nl < 1 || nl > 8) return launch_mfn_gemm(nl, 1, a, b, c, m, n, k, nullptr);
+  return gemm_host(1, nl, false, a, b, c, m, n, k);
 }
 
 int hp_dev_gemm(int trans_a, const double* a_hi, const double* a_lo, const double* b_hi, const double* b_lo,

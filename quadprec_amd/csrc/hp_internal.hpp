@@ -121,5 +121,11 @@ int split_gemm_variant(size_t m, size_t n, size_t k, int R);
 int set_split_gemm_variant(size_t m, size_t n, size_t k, int R, int variant);
 int launch_mf4_gemm(int trans_a, const double* a, const double* b, double* c, size_t m, size_t n,
                     size_t k, hipStream_t s);
+// MultiFloat<nl> GEMM on device limb planes (nl = 1..8; 4 takes the f256 kernel)
+int launch_mfn_gemm(int nl, int trans_a, const double* a, const double* b, double* c, size_t m, size_t n, size_t k,
+                    hipStream_t s);
+// MultiFloat<4> elementwise op on device limb planes with the generic op codes (0 add, 1 sub,
+// 2 mul, 3 div, 4 recip, 5 sqrt, 6 renormalize)
+int launch_mfn_elem4(int op, const double* a, const double* b, double* out, size_t n, hipStream_t s);
 
 }  // namespace hp

@@ -194,6 +194,29 @@ __global__ void k_mf_elem(int op, const double* a, const double* b, double* out,
   stv(out, n, i, r);
 }
 
+// elementwise with the generic MultiFloat<N> op codes: 0 add, 1 sub, 2 mul, 3 div,
+// 4 recip(a), 5 sqrt(a), 6 renormalize(a)
+__global__ void k_mf4_elem_n(int op, const double* a, const double* b, double* out, long n) {
+  const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const mf4 x = ldv(a, n, i);
+  mf4 r = x;
+  switch (op) {
+    case 0: r = mf_add(x, ldv(b, n, i)); break;
+    case 1: r = mf_sub(x, ldv(b, n, i)); break;
+    case 2: r = mf_mul(x, ldv(b, n, i)); break;
+    case 3: r = mf_div(x, ldv(b, n, i)); break;
+    case 4: r = mf_recip(x); break;
+    case 5: r = mf_sqrt(x); break;
+    default: {
+      double l[4] = {x.l0, x.l1, x.l2, x.l3};
+      mf_renormalize(l);
+      r = mkmf(l[0], l[1], l[2], l[3]);
+    }
+  }
+  stv(out, n, i, r);
+}
+
 // MultiFloat<4>::from_f128 / to_f128 (multifloat/conv.rs:120-131) over arrays: limbs
 // (hi, lo, 0, 0), and (limb 0, limb 1) with limbs 2 and 3 dropped; AoS in and out
 __global__ void k_mf4_from_dd(const double2* __restrict__ in, double4* __restrict__ out, long n) {
@@ -208,6 +231,13 @@ __global__ void k_dd_from_mf4(const double4* __restrict__ in, double2* __restric
 unsigned blocks_of(long n, int t) { return (unsigned)((n + t - 1) / t); }
 
 }  // namespace
+
+int launch_mfn_elem4(int op, const double* a, const double* b, double* out, size_t n, hipStream_t s) {
+  hipLaunchKernelGGL(k_mf4_elem_n, dim3(blocks_of((long)n, 256)), dim3(256), 0, s, op, a, b, out, (long)n);
+  HP_LAUNCH_CHECK();
+  return HP_OK;
+}
+
 }  // namespace hp
 
 using namespace hp;

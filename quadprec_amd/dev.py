@@ -126,6 +126,15 @@ def mf_gemm(a, b, c, m, n, k, trans_a=False):
     return c
 
 
+def mfn_gemm(nl, a, b, c, m, n, k, trans_a=False):
+    """MultiFloat<nl> C = op(A) B on limb planes: a (nl, m*k), b (nl, k*n), c (nl, m*n)."""
+    _need(a, nl * m * k, "a")
+    _need(b, nl * k * n, "b")
+    _need(c, nl * m * n, "c")
+    _check(lib().hp_dev_mfn_gemm(nl, int(trans_a), _ptr(a), _ptr(b), _ptr(c), m, n, k, _stream()), "dev.mfn_gemm")
+    return c
+
+
 # ---------------------------------------------------------------- adjoints
 class OverlapAdjoint:
     """Saved state of one forward evaluation of F1 = log det S and F2 = b^T S^-1 b for the

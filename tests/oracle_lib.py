@@ -105,6 +105,12 @@ def lib():
                                                  np.ctypeslib.ndpointer(dtype=np.int64, flags="C_CONTIGUOUS"),
                                                  _dp, _dp, _sz, _dp, _dp, C.POINTER(C.c_size_t)]
         L.hpo_assemble_overlap_mixed.restype = C.c_size_t
+        L.hpo_mfn_elementwise.argtypes = [C.c_int, C.c_int, _dp, _dp, _dp, _sz]
+        L.hpo_mfn_gemm_rows.argtypes = [C.c_int, C.c_int, _dp, _dp, _dp, _sz, _sz, _sz, _sz, _sz]
+        L.hpo_mfn_gemv.argtypes = [C.c_int, _dp, _dp, _dp, _sz, _sz]
+        L.hpo_mfn_cholesky.argtypes = [C.c_int, _dp, _sz]
+        L.hpo_mfn_cholesky.restype = C.c_long
+        L.hpo_mfn_cholesky_solve.argtypes = [C.c_int, _dp, _dp, _dp, _sz]
         L.hpo_assemble_overlap_envelope.argtypes = L.hpo_assemble_overlap_mixed.argtypes[:7] + [_dp]
         L.hpo_assemble_overlap_envelope.restype = None
         L.hpo_set_exp_ulp_shift.argtypes = [C.c_int]
@@ -183,6 +189,53 @@ def mf4_unop(op, a):
     out = np.empty_like(a)
     lib().hpo_mf4_unop_n(_MUN[op], a, out, a.size // 4)
     return out
+
+
+# ------------------------------------------------- MultiFloat<N>, N = 1..8 (hpo_mfn.cpp)
+_MFN_OP = {"add": 0, "sub": 1, "mul": 2, "div": 3, "recip": 4, "sqrt": 5, "renormalize": 6}
+
+
+def mfn_op(nl, op, a, b=None):
+    """Elementwise MultiFloat<nl> op on (count, nl) arrays."""
+    a = _c(np.asarray(a, dtype=np.float64).reshape(-1, nl))
+    b = a if b is None else _c(np.broadcast_to(np.asarray(b, dtype=np.float64).reshape(-1, nl), a.shape))
+    out = np.empty_like(a)
+    assert lib().hpo_mfn_elementwise(nl, _MFN_OP[op], a, b, out, a.shape[0]) == 0
+    return out
+
+
+def mfn_gemm(nl, a, b, m, n, k, trans_a=False, row0=0, nrows=None):
+    """mf_gemm::<nl> (mf_gemm_atb when trans_a: A stored k x m); rows row0 .. row0+nrows-1."""
+    c = np.zeros((m * n, nl))
+    nrows = m - row0 if nrows is None else nrows
+    assert lib().hpo_mfn_gemm_rows(nl, int(trans_a), _c(a), _c(b), c, m, n, k, row0, nrows) == 0
+    return c
+
+
+def mfn_gemv(nl, a, x, n, m):
+    y = np.zeros((n, nl))
+    assert lib().hpo_mfn_gemv(nl, _c(a), _c(x), y, n, m) == 0
+    return y
+
+
+def mfn_dot(nl, a, b):
+    """mf_dot (linalg.rs:7-14): the one-row gemv."""
+    return mfn_gemv(nl, a, b, 1, len(np.asarray(a).reshape(-1, nl)))[0]
+
+
+def mfn_cholesky(nl, a, n):
+    """mf_cholesky::<nl> in place on a copy; raises ValueError at a non-positive pivot."""
+    a = _c(a).copy()
+    r = lib().hpo_mfn_cholesky(nl, a, n)
+    if r:
+        raise ValueError(f"Cholesky failed: matrix is not positive definite (column {r - 1})")
+    return a
+
+
+def mfn_cholesky_solve(nl, l, b, n):
+    x = np.zeros((n, nl))
+    assert lib().hpo_mfn_cholesky_solve(nl, _c(l), _c(b), x, n) == 0
+    return x
 
 
 # ---------------------------------------------------------------- linalg

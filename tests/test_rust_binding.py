@@ -109,5 +109,5 @@ def test_wrappers_call_only_declared_functions():
     assert used and used <= set(rust_decls()), sorted(used - set(rust_decls()))
     for f in ("gemm", "gemm_atb", "mf_gemm", "mf_gemm_atb", "cholesky", "cholesky_blocked", "forward_solve",
               "backward_solve", "solve_cholesky", "matvec", "cond_estimate", "dot", "gemv", "jacobi_eigen",
-              "solve_spd", "solve_spd_f128"):
+              "solve_spd", "solve_spd_f128", "mf_dot"):
         assert re.search(r"pub fn " + f + r"\b", open(LIBRS).read()), f
