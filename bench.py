@@ -366,6 +366,49 @@ def bench_mf4(qp, torch, dev, steps, cpu):
     return out
 
 
+MFN_NL, MFN_N = 8, 512
+# FP64 operations per f512 MAC, the op-counting restatement's mean over four accumulation
+# chains of these inputs (tests/test_oracle_mfn.py re-derives it)
+MFN8_OPS_PER_MAC_MEAN = 1966
+
+
+def bench_mfn(qp, torch, dev, steps, cpu):
+    """f512 = MultiFloat<8> mf_gemm C = A*B at 512^3 (multifloat/linalg.rs:39-61; operands
+    from gen_mfn_normal(8, 1 / 2, .)), the reference's per-element order (bit-identical,
+    tests/test_gpu_mfn.py)."""
+    nl, n = MFN_NL, MFN_N
+    a = qp.gen_mfn_normal(nl, 1, n * n)
+    b = qp.gen_mfn_normal(nl, 2, n * n)
+    da, db = qp.dev.to_planes(a, nl, dev), qp.dev.to_planes(b, nl, dev)
+    dc = torch.empty((nl, n * n), dtype=torch.float64, device=dev)
+    stream = torch.cuda.current_stream()
+    ms, kern = time_steps(lambda: qp.dev.mfn_gemm(nl, da, db, dc, n, n, n), max(2, min(steps, 5)), 1, stream, torch)
+    macs = float(n) ** 3
+    ops = MFN8_OPS_PER_MAC_MEAN * macs / (kern * 1e-3) / 1e12
+    out = {"workload": f"f512 (MultiFloat<8>) mf_gemm C=A*B, m=n=k={n}, gen_mfn_normal seeds 1, 2",
+           "ms_per_step": round(ms, 3), "value": round(2.0 * macs / (ms * 1e-3) / 1e9, 3),
+           "unit": "f512 GFLOP/s (2 per MAC)",
+           "roofline": {"bound": "valu", "kernel": "k_mfn_gemm<8>", "kernel_ms": round(kern, 3),
+                        "ops_per_mac": MFN8_OPS_PER_MAC_MEAN, "achieved": round(2.0 * ops, 3),
+                        "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s", "frac": round(2.0 * ops / FP64_PEAK_TFLOPS, 4),
+                        "note": "FP64 VALU ops per MAC from the op-counting restatement on these inputs (mean), "
+                                "FMA-equivalent (2 flops each); the sorts and compactions add selects and compares "
+                                "beyond the counted arithmetic"}}
+    if cpu:
+        sys.path.insert(0, os.path.join(ROOT, "tests"))
+        import oracle_lib as O
+        threads = O.lib().hpo_num_threads()
+        rows = threads
+        t0 = time.perf_counter()
+        O.mfn_gemm(nl, a, b, n, n, n, row0=0, nrows=rows)
+        t = time.perf_counter() - t0
+        out["cpu_baseline"] = {"value": round(2.0 * rows * n * n / t / 1e9, 4), "unit": "f512 GFLOP/s",
+                               "cores": threads, "kind": "port",
+                               "sample": f"{rows} of {n} rows ({t:.1f} s), oracle C++ restatement of the generic "
+                                         "MultiFloat<N> code, OpenMP rows"}
+    return out
+
+
 def bench_eigen(qp, torch, dev, cpu):
     """The consumer of S: solve_gen_eigen F C = S C eps (crates/solver/src/ortho.rs:201-248)
     with S the config-0 overlap matrix (lattice, seed 0, exponents in [0.01, 10]) and F a
@@ -960,6 +1003,7 @@ def main():
     ap.add_argument("--no-overlap", action="store_true", help="skip the mixed-tier overlap assembly")
     ap.add_argument("--no-reduce", action="store_true", help="skip dot / gemv")
     ap.add_argument("--no-mfchol", action="store_true", help="skip the MF4 Cholesky")
+    ap.add_argument("--no-mfn", action="store_true", help="skip the f512 (MultiFloat<8>) gemm")
     ap.add_argument("--no-config0", action="store_true", help="skip config 0 (N=512 overlap, factor, solve)")
     ap.add_argument("--no-overlap-s", action="store_true", help="skip the dd overlap build")
     ap.add_argument("--no-atb", action="store_true", help="skip gemm_atb at 4096^3")
@@ -1069,6 +1113,8 @@ def main():
         out["batched_solves"] = bench_batch(qp, torch, dev, not args.no_cpu)
     if not args.no_mf4:
         out["mf4_gemm_1024"] = bench_mf4(qp, torch, dev, args.steps, not args.no_cpu)
+    if not args.no_mfn:
+        out["f512_gemm_512"] = bench_mfn(qp, torch, dev, args.steps, not args.no_cpu)
     if not args.no_adjoint:
         out["adjoint_n2048"] = bench_adjoint(qp, torch, dev, not args.no_cpu)
     if not args.no_eigen:

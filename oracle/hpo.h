@@ -162,6 +162,8 @@ int hpo_mfn_gemm_rows(int nl, int trans_a, const double* a, const double* b, dou
 int hpo_mfn_gemv(int nl, const double* a, const double* x, double* y, size_t n, size_t m); /* linalg.rs:17-36 */
 long hpo_mfn_cholesky(int nl, double* a, size_t n); /* linalg.rs:89-141: 1 + failing column, 0 on success */
 int hpo_mfn_cholesky_solve(int nl, const double* l, const double* b, double* x, size_t n); /* 145-172 */
+/* FP64 operations of one MAC s = s + a * b (add.rs + mul.rs) on the given values */
+long hpo_count_mfn_mac(int nl, const double* s, const double* a, const double* b);
 
 /* ---- reductions ---- */
 /* dot (crates/quad/src/linalg.rs:7-27): serial below 1024 elements, else 1024-element

@@ -27,16 +27,23 @@ struct MFN {
   double l[N];
 };
 
+// FP64 operations performed by the error-free transforms (the only arithmetic of add and
+// mul), counted for the work figure of the MultiFloat<N> GEMM
+thread_local long t_ops = 0;
+
 inline void tsum(double a, double b, double& s, double& e) {  // core.rs:19-25
+  t_ops += 6;
   s = a + b;
   double v = s - a;
   e = (a - (s - v)) + (b - v);
 }
 inline void qtsum(double a, double b, double& s, double& e) {  // core.rs:29-33
+  t_ops += 3;
   s = a + b;
   e = b - (s - a);
 }
 inline void tprod(double a, double b, double& p, double& e) {  // core.rs:37-41
+  t_ops += 2;
   p = a * b;
   e = std::fma(a, b, -p);
 }
@@ -328,6 +335,15 @@ void cholesky_solve(const double* l, const double* b, double* x, size_t n) {
   for (size_t i = 0; i < n; ++i) st<N>(x, i, xv[i]);
 }
 
+template <int N>
+long count_mac(const double* s, const double* a, const double* b) {
+  MFN<N> x = ld<N>(s, 0), y = ld<N>(a, 0), z = ld<N>(b, 0);
+  t_ops = 0;
+  x = add<N>(x, mul<N>(y, z));
+  (void)x;
+  return t_ops;
+}
+
 }  // namespace
 
 #define HPO_MFN_DISPATCH(N, CALL)  \
@@ -367,6 +383,14 @@ int hpo_mfn_gemv(int nl, const double* a, const double* x, double* y, size_t n, 
 long hpo_mfn_cholesky(int nl, double* a, size_t n) {
   long r = 0;
 #define C_(K) r = cholesky<K>(a, n)
+  HPO_MFN_DISPATCH(nl, C_)
+#undef C_
+  return r;
+}
+// FP64 operations of one s = s + a * b (the GEMM's MAC) on these values
+long hpo_count_mfn_mac(int nl, const double* s, const double* a, const double* b) {
+  long r = 0;
+#define C_(K) r = count_mac<K>(s, a, b)
   HPO_MFN_DISPATCH(nl, C_)
 #undef C_
   return r;

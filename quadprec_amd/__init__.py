@@ -831,4 +831,15 @@ def gen_mf4_normal(seed, count):
     return out
 
 
+def gen_mfn_normal(nl, seed, count):
+    """(count, nl) MultiFloat<nl> values: limb 0 N(0,1), each further limb a uniform fraction
+    of half an ulp of the one above (non-overlapping, every limb populated)."""
+    rng = np.random.default_rng(seed)
+    out = np.empty((count, nl))
+    out[:, 0] = rng.standard_normal(count)
+    for i in range(1, nl):
+        out[:, i] = out[:, i - 1] * 2.0 ** -53 * rng.uniform(-0.5, 0.5, count)
+    return out
+
+
 from . import dev  # noqa: E402,F401  (device entry points on torch tensors)

@@ -111,6 +111,8 @@ def lib():
         L.hpo_mfn_cholesky.argtypes = [C.c_int, _dp, _sz]
         L.hpo_mfn_cholesky.restype = C.c_long
         L.hpo_mfn_cholesky_solve.argtypes = [C.c_int, _dp, _dp, _dp, _sz]
+        L.hpo_count_mfn_mac.argtypes = [C.c_int, _dp, _dp, _dp]
+        L.hpo_count_mfn_mac.restype = C.c_long
         L.hpo_assemble_overlap_envelope.argtypes = L.hpo_assemble_overlap_mixed.argtypes[:7] + [_dp]
         L.hpo_assemble_overlap_envelope.restype = None
         L.hpo_set_exp_ulp_shift.argtypes = [C.c_int]
@@ -221,6 +223,11 @@ def mfn_gemv(nl, a, x, n, m):
 def mfn_dot(nl, a, b):
     """mf_dot (linalg.rs:7-14): the one-row gemv."""
     return mfn_gemv(nl, a, b, 1, len(np.asarray(a).reshape(-1, nl)))[0]
+
+
+def count_mfn_mac(nl, s, a, b):
+    """FP64 operations of one MultiFloat<nl> MAC s + a * b on these values."""
+    return lib().hpo_count_mfn_mac(nl, _c(s), _c(a), _c(b))
 
 
 def mfn_cholesky(nl, a, n):

@@ -204,3 +204,36 @@ def test_n1_and_renormalize_shapes():
         assert abs(to_mp(r[i]) - to_mp(x[i])) <= mp.mpf(2) ** -400 * mp.fsum([abs(v) for v in x[i]])
         nz = r[i][r[i] != 0]
         assert np.all(np.abs(nz[1:]) <= np.abs(nz[:-1]) * 2.0 ** -52)
+
+
+def test_mac_op_count_matches_the_mf4_counter():
+    """The FP64 operation count of one MAC (the GEMM's work figure) agrees at N = 4 with the
+    MultiFloat<4> oracle's counting-double instantiation, value for value."""
+    rng = np.random.default_rng(8)
+    s, a, b = rand_mf(rng, 40, 4), rand_mf(rng, 40, 4), rand_mf(rng, 40, 4)
+    for i in range(40):
+        assert O.count_mfn_mac(4, s[i], a[i], b[i]) == O.lib().hpo_count_mf4_mac(O._c(s[i]), O._c(a[i]), O._c(b[i]))
+    counts = {nl: np.mean([O.count_mfn_mac(nl, *rand_mf(rng, 3, nl)) for _ in range(20)]) for nl in (2, 3, 8)}
+    assert counts[2] < counts[3] < 515 < counts[8]
+
+
+def test_f512_mac_op_count_matches_the_bench_constant():
+    """bench.py prices the f512 512^3 gemm at MFN8_OPS_PER_MAC_MEAN FP64 operations per MAC:
+    re-derive that mean from the op-counting restatement on the same inputs (four
+    accumulation chains of 512 MACs)."""
+    import os
+    import re
+    import quadprec_amd as qp
+    src = open(os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "bench.py")).read()
+    const = int(re.search(r"MFN8_OPS_PER_MAC_MEAN = (\d+)", src).group(1))
+    n = 512
+    a, b = qp.gen_mfn_normal(8, 1, n * n), qp.gen_mfn_normal(8, 2, n * n)
+    tot = cnt = 0
+    for i, j in ((0, 0), (17, 400), (255, 3), (511, 511)):
+        s = np.zeros(8)
+        for p in range(n):
+            ai, bj = a[i * n + p], b[p * n + j]
+            tot += O.count_mfn_mac(8, s, ai, bj)
+            cnt += 1
+            s = O.mfn_op(8, "add", s, O.mfn_op(8, "mul", ai, bj))[0]
+    assert abs(tot / cnt - const) <= 0.01 * const, tot / cnt
