@@ -316,11 +316,28 @@ __device__ __forceinline__ void bt_solve(const double* Ah, const double* Al, lon
         off[e] = i;
         y[e] = idx < tot ? ldm(xh + ro + (long)rr[e] * n, xl + ro + (long)rr[e] * n, i) : mkdd(0.0);
       }
+      // each element's row of L is strided by n across the lanes: the next eight columns'
+      // loads go out together, ahead of their eight chained steps
 #pragma unroll 1
-      for (int j = 0; j < bw; ++j) {
+      for (int j0 = 0; j0 < bw; j0 += 8) {
+        double lh[SOLVE_EPT][8], ll[SOLVE_EPT][8];
 #pragma unroll
         for (int e = 0; e < SOLVE_EPT; ++e)
-          y[e] = dd_sub(y[e], dd_mul(ldm(Ah, Al, off[e] * n + b0 + j), mkdd(Yh[rr[e]][j], Yl[rr[e]][j])));
+#pragma unroll
+          for (int q = 0; q < 8; ++q) {
+            const bool ok = j0 + q < bw;
+            const long o = off[e] * n + b0 + j0 + (ok ? q : 0);
+            lh[e][q] = ok ? Ah[o] : 0.0;
+            ll[e][q] = ok ? Al[o] : 0.0;
+          }
+#pragma unroll
+        for (int q = 0; q < 8; ++q) {
+          if (j0 + q < bw) {
+#pragma unroll
+            for (int e = 0; e < SOLVE_EPT; ++e)
+              y[e] = dd_sub(y[e], dd_mul(mkdd(lh[e][q], ll[e][q]), mkdd(Yh[rr[e]][j0 + q], Yl[rr[e]][j0 + q])));
+          }
+        }
       }
 #pragma unroll
       for (int e = 0; e < SOLVE_EPT; ++e)
@@ -452,7 +469,7 @@ __global__ __launch_bounds__(NT) void k_batch_chol_solve(const BatchDesc* __rest
 //   k_bp_diag    one workgroup per instance: block p's factor;
 //   k_bp_trsm    eight rows per workgroup: trsm of the rows below block p;
 //   k_bp_update  one trailing 64 x 64 tile per workgroup (256 threads);
-// then k_bp_solve, one workgroup per instance, largest first.  Per-step prefix sums of row groups and tiles
+// then the block-step solves below.  Per-step prefix sums of row groups and tiles
 // map a workgroup to its (instance, piece).  Each element receives exactly the operations
 // of schedule 1, in the same order.  Step 0 reads S and writes L, and copies S's upper
 // triangle (the factor leaves it as it is), each piece the mirror of what it factors.
@@ -565,18 +582,132 @@ __global__ __launch_bounds__(256) void k_bp_update(const BatchDesc* __restrict__
   }
 }
 
-__global__ __launch_bounds__(NT) void k_bp_solve(const BatchDesc* __restrict__ desc, const int* __restrict__ order,
-                                                 int first, const int* __restrict__ status,
-                                                 const double* __restrict__ lh, const double* __restrict__ ll,
-                                                 const double* __restrict__ bh, const double* __restrict__ bl,
-                                                 double* __restrict__ xh, double* __restrict__ xl, int nrhs) {
+
+// ---- the panel-step solves: L L^T X = B for every instance in block steps across the
+// batch, the same operations in the same order as bt_solve (results identical to schedule
+// 1's).  X = B first; then per block b ascending: k_bs_diag<true> (one workgroup per active
+// instance: the block's forward substitution chains, one wavefront per right-hand side,
+// lane = row) and k_bs_update<true> (64 rows below the block per workgroup: each element
+// takes the block's 64 contributions in column order); then per block b descending the
+// same with the backward chains and the 64-row groups above the block.  A 64 x 64 tile of
+// L is staged in LDS with coalesced row loads (rows of the block's columns forward, rows of
+// the block's rows backward), so no lane walks a row of L.
+struct SolveLds {
+  double Th[NB][NB + 1], Tl[NB][NB + 1];
+  double Yh[MAXRHS][NB], Yl[MAXRHS][NB];
+};
+
+__global__ void k_bs_copy(const BatchDesc* __restrict__ desc, const int* __restrict__ status,
+                          const double* __restrict__ bh, const double* __restrict__ bl, double* __restrict__ xh,
+                          double* __restrict__ xl, int nrhs) {
+  const int inst = blockIdx.x;
+  if (status[inst]) return;
+  const long n = desc[inst].n, ro = desc[inst].rhs_off;
+  for (long idx = threadIdx.x; idx < (long)nrhs * n; idx += blockDim.x) {
+    xh[ro + idx] = bh[ro + idx];
+    xl[ro + idx] = bl[ro + idx];
+  }
+}
+
+template <bool FWD>
+__global__ __launch_bounds__(NT) void k_bs_diag(const BatchDesc* __restrict__ desc, const int* __restrict__ order,
+                                                const int* __restrict__ status, const double* __restrict__ lh,
+                                                const double* __restrict__ ll, double* __restrict__ xh,
+                                                double* __restrict__ xl, int nrhs, long b0) {
   extern __shared__ __align__(16) unsigned char smem_raw[];
   DiagLds& d = *reinterpret_cast<DiagLds*>(smem_raw);  // Lh / Ll only: the staged diagonal block
-  __shared__ double Yh[MAXRHS][NB], Yl[MAXRHS][NB];
-  const int inst = order[first + blockIdx.x];  // largest first
+  double (*Dh)[NB + 1] = d.Lh;
+  double (*Dl)[NB + 1] = d.Ll;
+  const int inst = order[blockIdx.x];
   if (status[inst]) return;
   const long n = desc[inst].n, mo = desc[inst].mat_off, ro = desc[inst].rhs_off;
-  bt_solve(lh + mo, ll + mo, n, bh, bl, xh, xl, ro, nrhs, Yh, Yl, d.Lh, d.Ll);
+  const int bw = (int)min((long)NB, n - b0);
+  const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+  for (int idx = tid; idx < bw * bw; idx += NT) {
+    const int i = idx / bw, c = idx % bw;
+    if (c <= i) {
+      const long o = mo + (b0 + i) * n + b0 + c;
+      Dh[i][c] = lh[o];
+      Dl[i][c] = ll[o];
+    }
+  }
+  __syncthreads();
+  for (int r = wv; r < nrhs; r += NT / 64) {
+    double* Xh = xh + ro + (long)r * n;
+    double* Xl = xl + ro + (long)r * n;
+    dd y = lane < bw ? ldm(Xh, Xl, b0 + lane) : mkdd(0.0);
+    const dd rjj = dd_recip(lane < bw ? mkdd(Dh[lane][lane], Dl[lane][lane]) : mkdd(Dh[0][0], Dl[0][0]));
+    if (FWD) {
+      for (int j = 0; j < bw; ++j) {
+        dd yj = dd_mul(y, rjj);
+        if (lane == j) y = yj;
+        dd bj = rlane(yj, j);
+        if (lane > j && lane < bw) y = dd_sub(y, dd_mul(mkdd(Dh[lane][j], Dl[lane][j]), bj));
+      }
+    } else {
+      for (int j = bw - 1; j >= 0; --j) {
+        dd xj = dd_mul(y, rjj);
+        if (lane == j) y = xj;
+        dd bj = rlane(xj, j);
+        if (lane < j) y = dd_sub(y, dd_mul(mkdd(Dh[j][lane], Dl[j][lane]), bj));
+      }
+    }
+    if (lane < bw) stm(Xh, Xl, b0 + lane, y);
+  }
+}
+
+// 64 rows (below the block forward, above it backward) of one instance take the block's
+// contributions: forward x_i -= sum_j l[i][b0+j] y_j (j ascending), backward
+// x_i -= sum_j l[b0+j][i] x_{b0+j} (j descending); thread (row, right-hand-side group)
+template <bool FWD>
+__global__ __launch_bounds__(256) void k_bs_update(const BatchDesc* __restrict__ desc, const int* __restrict__ order,
+                                                   const int* __restrict__ status, const long long* __restrict__ pre,
+                                                   int active, const double* __restrict__ lh,
+                                                   const double* __restrict__ ll, double* __restrict__ xh,
+                                                   double* __restrict__ xl, int nrhs, long b0) {
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  SolveLds& sl = *reinterpret_cast<SolveLds*>(smem_raw);
+  double (*Th)[NB + 1] = sl.Th;  // forward [row][col], backward [col][row]
+  double (*Tl)[NB + 1] = sl.Tl;
+  double (*Yh)[NB] = sl.Yh;
+  double (*Yl)[NB] = sl.Yl;
+  const int j = piece_owner(pre, active, blockIdx.x);
+  const int inst = order[j];
+  if (status[inst]) return;
+  const long n = desc[inst].n, mo = desc[inst].mat_off, ro = desc[inst].rhs_off;
+  const int bw = (int)min((long)NB, n - b0);
+  const long c = (long)blockIdx.x - pre[j];
+  const long i0 = FWD ? b0 + bw + c * NB : c * NB;
+  const int rows = (int)min((long)NB, (FWD ? n : b0) - i0);
+  const int tid = threadIdx.x;
+  for (int idx = tid; idx < NB * NB; idx += 256) {
+    const int a = idx / NB, e = idx % NB;  // forward: row a, column e; backward: block row a, row e
+    const bool ok = FWD ? (a < rows && e < bw) : (a < bw && e < rows);
+    const long o = mo + (FWD ? (i0 + a) * n + b0 + e : (b0 + a) * n + i0 + e);
+    Th[a][e] = ok ? lh[o] : 0.0;
+    Tl[a][e] = ok ? ll[o] : 0.0;
+  }
+  for (int idx = tid; idx < nrhs * NB; idx += 256) {
+    const int r = idx / NB, e = idx % NB;
+    const long o = ro + (long)r * n + b0 + e;
+    Yh[r][e] = e < bw ? xh[o] : 0.0;
+    Yl[r][e] = e < bw ? xl[o] : 0.0;
+  }
+  __syncthreads();
+  const int row = tid & 63;
+  if (row >= rows) return;
+  for (int r = tid >> 6; r < nrhs; r += 4) {
+    const long o = ro + (long)r * n + i0 + row;
+    dd y = ldm(xh, xl, o);
+    if (FWD) {
+#pragma unroll 4
+      for (int q = 0; q < bw; ++q) y = dd_sub(y, dd_mul(mkdd(Th[row][q], Tl[row][q]), mkdd(Yh[r][q], Yl[r][q])));
+    } else {
+#pragma unroll 4
+      for (int q = bw - 1; q >= 0; --q) y = dd_sub(y, dd_mul(mkdd(Th[q][row], Tl[q][row]), mkdd(Yh[r][q], Yl[r][q])));
+    }
+    stm(xh, xl, o, y);
+  }
 }
 
 }  // namespace
@@ -592,6 +723,9 @@ struct hp_batch {
   // its row-group and tile prefix sums in `pre` with their totals
   std::vector<int> active;
   std::vector<long long> trsm_off, upd_off, trsm_tot, upd_tot;
+  // the block-step solves: per block b, the 64-row groups below (forward) and above
+  // (backward) the block over the instances active at b
+  std::vector<long long> fs_off, fs_tot, bs_off, bs_tot;
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
   ~hp_batch() {
@@ -647,6 +781,10 @@ int batch_prepare(hp_batch* b, const int* sizes, size_t count, int nrhs) {
   b->upd_off.assign(steps, 0);
   b->trsm_tot.assign(steps, 0);
   b->upd_tot.assign(steps, 0);
+  b->fs_off.assign(steps, 0);
+  b->fs_tot.assign(steps, 0);
+  b->bs_off.assign(steps, 0);
+  b->bs_tot.assign(steps, 0);
   for (int p = 0; p < steps; ++p) {
     const long long jb0 = (long long)p * NB;
     int A = 0;
@@ -670,6 +808,22 @@ int batch_prepare(hp_batch* b, const int* sizes, size_t count, int nrhs) {
     }
     pre.push_back(acc);
     b->upd_tot[p] = acc;
+    b->fs_off[p] = (long long)pre.size();
+    acc = 0;
+    for (int j = 0; j < A; ++j) {
+      pre.push_back(acc);
+      acc += (std::max<long long>(0, sizes[ord[j]] - jb0 - NB) + NB - 1) / NB;
+    }
+    pre.push_back(acc);
+    b->fs_tot[p] = acc;
+    b->bs_off[p] = (long long)pre.size();
+    acc = 0;
+    for (int j = 0; j < A; ++j) {
+      pre.push_back(acc);
+      acc += jb0 / NB;
+    }
+    pre.push_back(acc);
+    b->bs_tot[p] = acc;
   }
   int rc;
   if ((rc = b->desc.alloc(count * sizeof(BatchDesc))) || (rc = b->order.alloc(count * sizeof(int))) ||
@@ -683,7 +837,7 @@ int batch_prepare(hp_batch* b, const int* sizes, size_t count, int nrhs) {
   static std::once_flag attr_once;
   static hipError_t attr_err = hipSuccess;
   std::call_once(attr_once, [] {
-    hipError_t e[6] = {
+    hipError_t e[9] = {
         hipFuncSetAttribute((const void*)k_batch_chol_solve, hipFuncAttributeMaxDynamicSharedMemorySize,
                             (int)sizeof(BatchLds)),
         hipFuncSetAttribute((const void*)k_bp_diag<true>, hipFuncAttributeMaxDynamicSharedMemorySize,
@@ -694,8 +848,14 @@ int batch_prepare(hp_batch* b, const int* sizes, size_t count, int nrhs) {
                             (int)offsetof(DiagLds, Sh)),
         hipFuncSetAttribute((const void*)k_bp_trsm<false>, hipFuncAttributeMaxDynamicSharedMemorySize,
                             (int)offsetof(DiagLds, Sh)),
-        hipFuncSetAttribute((const void*)k_bp_solve, hipFuncAttributeMaxDynamicSharedMemorySize,
-                            (int)offsetof(DiagLds, Sh))};
+        hipFuncSetAttribute((const void*)k_bs_diag<true>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                            (int)offsetof(DiagLds, Sh)),
+        hipFuncSetAttribute((const void*)k_bs_diag<false>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                            (int)offsetof(DiagLds, Sh)),
+        hipFuncSetAttribute((const void*)k_bs_update<true>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                            (int)sizeof(SolveLds)),
+        hipFuncSetAttribute((const void*)k_bs_update<false>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                            (int)sizeof(SolveLds))};
     for (hipError_t x : e)
       if (x != hipSuccess) attr_err = x;
   });
@@ -756,11 +916,36 @@ int batch_enqueue(hp_batch* b, int schedule, const double* s_hi, const double* s
       HP_LAUNCH_CHECK();
     }
   }
-  // the solves, largest first.  (Solving each instance beside the later panel steps, on a
-  // second stream, was slower: the long solve workgroups hold CUs the steps need.)
-  hipLaunchKernelGGL(k_bp_solve, dim3(b->count), dim3(NT), offsetof(DiagLds, Sh), s, desc, order, 0, status, l_hi,
-                     l_lo, b_hi, b_lo, x_hi, x_lo, b->nrhs);
+  // the solves in block steps across the batch (one workgroup per instance took up to 3 ms
+  // for an n = 1024 instance with four right-hand sides: a single CU's FP64 rate)
+  hipLaunchKernelGGL(k_bs_copy, dim3(b->count), dim3(256), 0, s, desc, static_cast<const int*>(status), b_hi, b_lo,
+                     x_hi, x_lo, b->nrhs);
   HP_LAUNCH_CHECK();
+  for (int fwd = 1; fwd >= 0; --fwd)
+    for (int q = 0; q < steps; ++q) {
+      const int p = fwd ? q : steps - 1 - q;
+      const long b0 = (long)p * NB;
+      const int A = b->active[p];
+      const long long off = fwd ? b->fs_off[p] : b->bs_off[p], tot = fwd ? b->fs_tot[p] : b->bs_tot[p];
+      if (fwd) {
+        hipLaunchKernelGGL(k_bs_diag<true>, dim3(A), dim3(NT), offsetof(DiagLds, Sh), s, desc, order, static_cast<const int*>(status),
+                           static_cast<const double*>(l_hi), static_cast<const double*>(l_lo), x_hi, x_lo, b->nrhs,
+                           b0);
+        if (tot > 0)
+          hipLaunchKernelGGL(k_bs_update<true>, dim3((unsigned)tot), dim3(256), sizeof(SolveLds), s, desc, order,
+                             static_cast<const int*>(status), pre + off, A, static_cast<const double*>(l_hi),
+                             static_cast<const double*>(l_lo), x_hi, x_lo, b->nrhs, b0);
+      } else {
+        hipLaunchKernelGGL(k_bs_diag<false>, dim3(A), dim3(NT), offsetof(DiagLds, Sh), s, desc, order, static_cast<const int*>(status),
+                           static_cast<const double*>(l_hi), static_cast<const double*>(l_lo), x_hi, x_lo, b->nrhs,
+                           b0);
+        if (tot > 0)
+          hipLaunchKernelGGL(k_bs_update<false>, dim3((unsigned)tot), dim3(256), sizeof(SolveLds), s, desc, order,
+                             static_cast<const int*>(status), pre + off, A, static_cast<const double*>(l_hi),
+                             static_cast<const double*>(l_lo), x_hi, x_lo, b->nrhs, b0);
+      }
+      HP_LAUNCH_CHECK();
+    }
   return HP_OK;
 }
 
