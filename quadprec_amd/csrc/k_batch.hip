@@ -553,7 +553,8 @@ __global__ __launch_bounds__(NT) void k_bp_trsm(const BatchDesc* __restrict__ de
   }
 }
 
-template <bool FIRST>
+// PART 1: the tiles of the next panel's column block (tc = 0); PART 2: the others
+template <bool FIRST, int PART>
 __global__ __launch_bounds__(256) void k_bp_update(const BatchDesc* __restrict__ desc, const int* __restrict__ order,
                                                    const int* __restrict__ status, const long long* __restrict__ pre,
                                                    int active, const double* __restrict__ sh,
@@ -565,7 +566,14 @@ __global__ __launch_bounds__(256) void k_bp_update(const BatchDesc* __restrict__
   if (status[inst]) return;
   const long n = desc[inst].n, mo = desc[inst].mat_off;
   long ti, tc;
-  tile_of((long)blockIdx.x - pre[j], ti, tc);
+  if (PART == 1) {
+    ti = (long)blockIdx.x - pre[j];
+    tc = 0;
+  } else {  // the lower triangle of tiles from (1, 1)
+    tile_of((long)blockIdx.x - pre[j], ti, tc);
+    ++ti;
+    ++tc;
+  }
   const long t0 = jb0 + NB, i0 = t0 + ti * 64, c0 = t0 + tc * 64;
   const double* Srch = (FIRST ? sh : lh) + mo;
   const double* Srcl = (FIRST ? sl : ll) + mo;
@@ -722,15 +730,23 @@ struct hp_batch {
   // panel steps: per step p, the active instances (a prefix of `order`), and the offsets of
   // its row-group and tile prefix sums in `pre` with their totals
   std::vector<int> active;
-  std::vector<long long> trsm_off, upd_off, trsm_tot, upd_tot;
+  std::vector<long long> trsm_off, trsm_tot, next_off, next_tot, rest_off, rest_tot;
   // the block-step solves: per block b, the 64-row groups below (forward) and above
   // (backward) the block over the instances active at b
   std::vector<long long> fs_off, fs_tot, bs_off, bs_tot;
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
+  // panel steps: the second stream (the tiles beyond the next panel) and its events
+  hipStream_t s2 = nullptr;
+  std::vector<hipEvent_t> ev;
   ~hp_batch() {
     if (exec) (void)hipGraphExecDestroy(exec);
     if (graph) (void)hipGraphDestroy(graph);
+    if (s2) {
+      (void)hipStreamSynchronize(s2);
+      (void)hipStreamDestroy(s2);
+    }
+    for (hipEvent_t e : ev) (void)hipEventDestroy(e);
   }
 };
 
@@ -778,9 +794,11 @@ int batch_prepare(hp_batch* b, const int* sizes, size_t count, int nrhs) {
   std::vector<long long> pre;
   b->active.assign(steps, 0);
   b->trsm_off.assign(steps, 0);
-  b->upd_off.assign(steps, 0);
   b->trsm_tot.assign(steps, 0);
-  b->upd_tot.assign(steps, 0);
+  b->next_off.assign(steps, 0);
+  b->next_tot.assign(steps, 0);
+  b->rest_off.assign(steps, 0);
+  b->rest_tot.assign(steps, 0);
   b->fs_off.assign(steps, 0);
   b->fs_tot.assign(steps, 0);
   b->bs_off.assign(steps, 0);
@@ -799,15 +817,24 @@ int batch_prepare(hp_batch* b, const int* sizes, size_t count, int nrhs) {
     }
     pre.push_back(acc);
     b->trsm_tot[p] = acc;
-    b->upd_off[p] = (long long)pre.size();
+    b->next_off[p] = (long long)pre.size();
     acc = 0;
     for (int j = 0; j < A; ++j) {
       pre.push_back(acc);
       const long long rows = std::max<long long>(0, sizes[ord[j]] - jb0 - NB), T = (rows + 63) / 64;
-      acc += T * (T + 1) / 2;
+      acc += T;
     }
     pre.push_back(acc);
-    b->upd_tot[p] = acc;
+    b->next_tot[p] = acc;
+    b->rest_off[p] = (long long)pre.size();
+    acc = 0;
+    for (int j = 0; j < A; ++j) {
+      pre.push_back(acc);
+      const long long rows = std::max<long long>(0, sizes[ord[j]] - jb0 - NB), T = (rows + 63) / 64;
+      acc += T * (T - 1) / 2;
+    }
+    pre.push_back(acc);
+    b->rest_tot[p] = acc;
     b->fs_off[p] = (long long)pre.size();
     acc = 0;
     for (int j = 0; j < A; ++j) {
@@ -880,13 +907,29 @@ int batch_enqueue(hp_batch* b, int schedule, const double* s_hi, const double* s
     return HP_OK;
   }
   const int steps = (int)b->active.size();
+  // lookahead on two streams: per step p, `s` runs the diagonal block, the row solves and
+  // the next panel's column block of the trailing update (tiles (i, p+1)), which is all
+  // step p+1 waits for; `s2` runs the rest of the trailing update beside step p+1's
+  // diagonal block and row solves (they touch column block p+1 only).  Every tile still
+  // takes its panels' updates in order: the next column block's tiles wait for the
+  // previous step's rest, and the rest of step p waits for step p's row solves.
+  if (!b->s2) {
+    HP_HIP(hipStreamCreateWithFlags(&b->s2, hipStreamNonBlocking));
+    b->ev.assign(2 * steps + 2, nullptr);
+    for (hipEvent_t& e : b->ev) HP_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+  }
+  hipStream_t s2 = b->s2;
+  hipEvent_t ev_fork = b->ev[2 * steps], ev_join = b->ev[2 * steps + 1];
   hipLaunchKernelGGL(k_bp_init, dim3((unsigned)((b->count + 255) / 256)), dim3(256), 0, s, status, b->count);
   HP_LAUNCH_CHECK();
+  HP_HIP(hipEventRecord(ev_fork, s));
+  HP_HIP(hipStreamWaitEvent(s2, ev_fork, 0));
   const long long* pre = (const long long*)b->pre.p;
   for (int p = 0; p < steps; ++p) {
     const long jb0 = (long)p * NB;
     const bool first = p == 0;
     const int A = b->active[p];
+    hipEvent_t ev_trsm = b->ev[2 * p], ev_rest = b->ev[2 * p + 1];
     if (first)
       hipLaunchKernelGGL(k_bp_diag<true>, dim3(A), dim3(NT), sizeof(DiagLds), s, desc, order, status, s_hi, s_lo, l_hi,
                          l_lo, jb0);
@@ -905,17 +948,33 @@ int batch_enqueue(hp_batch* b, int schedule, const double* s_hi, const double* s
                            s_hi, s_lo, l_hi, l_lo, jb0);
       HP_LAUNCH_CHECK();
     }
-    if (b->upd_tot[p] > 0) {
-      const unsigned g = (unsigned)b->upd_tot[p];
+    HP_HIP(hipEventRecord(ev_trsm, s));
+    if (p > 0) HP_HIP(hipStreamWaitEvent(s, b->ev[2 * p - 1], 0));  // the previous step's rest
+    if (b->next_tot[p] > 0) {
+      const unsigned g = (unsigned)b->next_tot[p];
       if (first)
-        hipLaunchKernelGGL(k_bp_update<true>, dim3(g), dim3(256), 0, s, desc, order, status, pre + b->upd_off[p], A,
-                           s_hi, s_lo, l_hi, l_lo, jb0);
+        hipLaunchKernelGGL((k_bp_update<true, 1>), dim3(g), dim3(256), 0, s, desc, order, status,
+                           pre + b->next_off[p], A, s_hi, s_lo, l_hi, l_lo, jb0);
       else
-        hipLaunchKernelGGL(k_bp_update<false>, dim3(g), dim3(256), 0, s, desc, order, status, pre + b->upd_off[p], A,
-                           s_hi, s_lo, l_hi, l_lo, jb0);
+        hipLaunchKernelGGL((k_bp_update<false, 1>), dim3(g), dim3(256), 0, s, desc, order, status,
+                           pre + b->next_off[p], A, s_hi, s_lo, l_hi, l_lo, jb0);
       HP_LAUNCH_CHECK();
     }
+    HP_HIP(hipStreamWaitEvent(s2, ev_trsm, 0));
+    if (b->rest_tot[p] > 0) {
+      const unsigned g = (unsigned)b->rest_tot[p];
+      if (first)
+        hipLaunchKernelGGL((k_bp_update<true, 2>), dim3(g), dim3(256), 0, s2, desc, order, status,
+                           pre + b->rest_off[p], A, s_hi, s_lo, l_hi, l_lo, jb0);
+      else
+        hipLaunchKernelGGL((k_bp_update<false, 2>), dim3(g), dim3(256), 0, s2, desc, order, status,
+                           pre + b->rest_off[p], A, s_hi, s_lo, l_hi, l_lo, jb0);
+      HP_LAUNCH_CHECK();
+    }
+    HP_HIP(hipEventRecord(ev_rest, s2));
   }
+  HP_HIP(hipEventRecord(ev_join, s2));
+  HP_HIP(hipStreamWaitEvent(s, ev_join, 0));
   // the solves in block steps across the batch (one workgroup per instance took up to 3 ms
   // for an n = 1024 instance with four right-hand sides: a single CU's FP64 rate)
   hipLaunchKernelGGL(k_bs_copy, dim3(b->count), dim3(256), 0, s, desc, static_cast<const int*>(status), b_hi, b_lo,
