@@ -33,9 +33,9 @@ DD_MAC_OPS = 29                  # FP64 ops per reference dd MAC (dd add 20 + dd
 I8_KERNELS = {32: "k_gemm_i8_mod<32>", 64: "k_gemm_i8_mod<64>", 128: "k_gemm_i8_mod<128>", 3064: "k_gemm_i8_mod_dl",
               4064: "k_gemm_i8_mod_dlp8", 5064: "k_gemm_i8_mod_dlp4", 6064: "k_gemm_i8_mod_rs"}
 # HBM bytes one launch at 4096^3 x 31 moduli moves, from rocprofv3 FETCH_SIZE / WRITE_SIZE
-# counters scaled by a device copy of known size (k_gemm_i8_mod_dlp8: read 3.126e9 + write
-# 0.520e9); null for a kernel these were not measured on
-I8_TRAFFIC_BYTES = {4064: 3.646e9}
+# counters scaled by a device copy of known size (k_gemm_i8_mod_dlp8 on the 64-deep K-slab
+# residue layout: read 3.121e9 + write 0.520e9); null for a kernel these were not measured on
+I8_TRAFFIC_BYTES = {4064: 3.641e9}
 
 
 def oracle():
