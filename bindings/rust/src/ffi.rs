@@ -140,6 +140,7 @@ unsafe extern "C" {
     pub fn hp_dev_assemble_overlap_mixed(centers: *const f64, l: *const c_int, prim_off: *const i64, alpha: *const f64, coeff: *const f64, n_shells: usize, params4: *const f64, s_dev: *mut f64, n_ao: usize, counts: *mut usize, stream: *mut c_void) -> c_int;
     pub fn hp_jacobi_eigen(a: *const f64, n: usize, max_iter: usize, evals: *mut f64, evecs: *mut f64) -> c_int;
     pub fn hp_dev_jacobi_eigen(a_hi: *const f64, a_lo: *const f64, n: usize, max_sweeps: usize, stop_rule: c_int, evals_hi: *mut f64, evals_lo: *mut f64, vecs_hi: *mut f64, vecs_lo: *mut f64, vec_rows: c_int, sweeps: *mut usize, stream: *mut c_void) -> c_int;
+    pub fn hp_eigen_set_block_min(min_n: usize) -> c_int;
     pub fn hp_canonical_orthogonalize(s_dd: *const f64, n: usize, threshold: f64, x_dd: *mut f64, n_orth: *mut usize) -> c_int;
     pub fn hp_solve_gen_eigen(f_dd: *const f64, s_dd: *const f64, n: usize, threshold: f64, evals_dd: *mut f64, c_dd: *mut f64, n_basis: *mut usize) -> c_int;
     pub fn hp_dev_solve_gen_eigen(f_hi: *const f64, f_lo: *const f64, s_hi: *const f64, s_lo: *const f64, n: usize, threshold: f64, evals_hi: *mut f64, evals_lo: *mut f64, c_hi: *mut f64, c_lo: *mut f64, n_basis: *mut usize, sweeps: *mut usize, stream: *mut c_void) -> c_int;

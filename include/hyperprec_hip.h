@@ -353,6 +353,12 @@ int hp_jacobi_eigen(const double* a, size_t n, size_t max_iter, double* evals, d
 int hp_dev_jacobi_eigen(const double* a_hi, const double* a_lo, size_t n, size_t max_sweeps, int stop_rule,
                         double* evals_hi, double* evals_lo, double* vecs_hi, double* vecs_lo, int vec_rows,
                         size_t* sweeps, void* stream);
+/* From n = min_n on (default 768; 0 restores it, SIZE_MAX disables it) the eigensolvers
+ * above (and canonical_orthogonalize, solve_gen_eigen) run block Jacobi: 16 x 16 blocks
+ * paired in round-robin order, each 32 x 32 subproblem taken through one inner parallel
+ * sweep, A <- Q^T A Q and V <- V Q as dd block products.  Same stop rules and results to
+ * dd rounding; a sweep makes n/16 passes over A instead of n.  Process-wide. */
+int hp_eigen_set_block_min(size_t min_n);
 /* canonical_orthogonalize (crates/solver/src/ortho.rs:158-193): diagonalise S (stop rule 1),
  * keep eigenvalues above threshold in ascending order, X = U diag(1/sqrt(lambda)).
  * x_dd: capacity n x n dd, receives n x n_orth (row-major). */

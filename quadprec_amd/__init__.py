@@ -169,6 +169,7 @@ def lib():
     L.hp_mf_gemv.argtypes = [_dp, _dp, _dp, _sz, _sz]
     L.hp_mf4_elementwise.argtypes = [C.c_int, _dp, _dp, _dp, _sz]
     L.hp_mf4_from_dd.argtypes = [_dp, _dp, _sz]
+    L.hp_eigen_set_block_min.argtypes = [_sz]
     for name in ("hp_mfn_gemm", "hp_mfn_gemm_atb"):
         getattr(L, name).argtypes = [C.c_int, _dp, _dp, _dp, _sz, _sz, _sz]
     L.hp_dev_mfn_gemm.argtypes = [C.c_int, C.c_int, _vp, _vp, _vp, _sz, _sz, _sz, _vp]
@@ -452,6 +453,12 @@ def solve_gen_eigen(f, s, n, threshold):
     m = C.c_size_t(0)
     _check(lib().hp_solve_gen_eigen(f, s, n, threshold, ev, c, C.byref(m)), "solve_gen_eigen")
     return ev[:m.value], c[:n * m.value], m.value
+
+
+def eigen_set_block_min(n):
+    """From n = min_n on the Jacobi eigensolvers run the block form (0: the default, 768;
+    a huge value disables it).  Process-wide."""
+    _check(lib().hp_eigen_set_block_min(int(n)), "eigen_set_block_min")
 
 
 # ---------------------------------------------------------------- MultiFloat<4> linear algebra

@@ -21,6 +21,8 @@
 // bit for bit: eigenvalues to a few units of 2^-104 |A|, eigenvectors up to sign (and up
 // to rotations inside a degenerate eigenspace).
 #include <algorithm>
+#include <atomic>
+#include <climits>
 #include <cmath>
 #include <cstring>
 #include <numeric>
@@ -48,6 +50,28 @@ struct Rot {
   int pad;
 };
 
+// The reference's rotation of a pair (linalg.rs:323-333): c, s and the new diagonal; false
+// (the identity) when a_pq is zero, or below 1e-150 of the diagonal gap (|tau| > 1e150,
+// where tau^2 would overflow and the rotation is the identity to dd precision)
+__device__ __forceinline__ bool jac_rotation(dd app, dd aqq, dd apq, dd& c, dd& s, dd& np, dd& nq) {
+  c = mkdd(1.0);
+  s = mkdd(0.0);
+  np = app;
+  nq = aqq;
+  if (apq.hi == 0.0) return false;
+  const dd one = mkdd(1.0);
+  const dd tau = dd_div(dd_sub(aqq, app), dd_mul(mkdd(2.0), apq));
+  if (!(__builtin_fabs(tau.hi) <= 1e150)) return false;
+  const dd tau_abs = dd_abs(tau);
+  const dd t_denom = dd_add(tau_abs, dd_sqrt(dd_add(one, dd_mul(tau, tau))));
+  const dd t = tau.hi >= 0.0 ? dd_div(one, t_denom) : dd_div(dd_neg(one), t_denom);
+  c = dd_div(one, dd_sqrt(dd_add(one, dd_mul(t, t))));
+  s = dd_mul(t, c);
+  np = dd_sub(app, dd_mul(t, apq));
+  nq = dd_add(aqq, dd_mul(t, apq));
+  return true;
+}
+
 // the rotation of pair i (positions 2i, 2i+1) from A's diagonal 2 x 2 block
 __device__ __forceinline__ Rot pair_rot(const double* __restrict__ ah, const double* __restrict__ al, long N2,
                                         long i) {
@@ -56,25 +80,9 @@ __device__ __forceinline__ Rot pair_rot(const double* __restrict__ ah, const dou
   const dd aqq = mkdd(ah[q * N2 + q], al[q * N2 + q]);
   const dd apq = mkdd(ah[p * N2 + q], al[p * N2 + q]);
   Rot r;
-  r.active = 0;
   r.pad = 0;
-  dd c = mkdd(1.0), s = mkdd(0.0), np = app, nq = aqq;
-  if (apq.hi != 0.0) {
-    const dd one = mkdd(1.0);
-    const dd tau = dd_div(dd_sub(aqq, app), dd_mul(mkdd(2.0), apq));
-    // |tau| > 1e150: a_pq is below 1e-150 of the diagonal gap (tau^2 would overflow); such a
-    // rotation is the identity to dd precision, so the pair is left alone
-    if (__builtin_fabs(tau.hi) <= 1e150) {
-      const dd tau_abs = dd_abs(tau);
-      const dd t_denom = dd_add(tau_abs, dd_sqrt(dd_add(one, dd_mul(tau, tau))));
-      const dd t = tau.hi >= 0.0 ? dd_div(one, t_denom) : dd_div(dd_neg(one), t_denom);
-      c = dd_div(one, dd_sqrt(dd_add(one, dd_mul(t, t))));
-      s = dd_mul(t, c);
-      np = dd_sub(app, dd_mul(t, apq));
-      nq = dd_add(aqq, dd_mul(t, apq));
-      r.active = 1;
-    }
-  }
+  dd c, s, np, nq;
+  r.active = jac_rotation(app, aqq, apq, c, s, np, nq) ? 1 : 0;
   r.ch = c.hi; r.cl = c.lo; r.sh = s.hi; r.sl = s.lo;
   r.ph = np.hi; r.pl = np.lo; r.qh = nq.hi; r.ql = nq.lo;
   return r;
@@ -331,6 +339,246 @@ __global__ void k_jac_extract(const double* __restrict__ ah, const double* __res
 
 unsigned blocks(long n, int t = 256) { return (unsigned)((n + t - 1) / t); }
 
+// ---- block Jacobi (large n).  A is cut into 16 x 16 blocks, nb of them per side (n padded
+// with zero rows and columns to a multiple of 32, whose rotations are the identity).  An
+// outer sweep runs nb - 1 steps of the round-robin order over the blocks; at a step each of
+// the nb / 2 block pairs (P, R) takes its 32 x 32 subproblem [A_PP A_PR; A_RP A_RR] through
+// one inner sweep of the scalar parallel Jacobi (31 steps of 16 disjoint rotations, their
+// angles to binary64 accuracy: jac_rotation_fast) in LDS, accumulating them into an orthogonal Q
+// (k_bj_sub); then A <- Q^T A Q and V <- V Q with Q block diagonal over the pairs, as
+// 32 x 32 dd products (k_bj_apply, k_bj_vapply).  Every index pair meets in at least one
+// subproblem per outer sweep, as in a scalar sweep, and a sweep makes nb - 1 passes over A
+// instead of n - 1.
+constexpr int BJ_B = 16, BJ_S = 2 * BJ_B;
+
+// round-robin (circle method) pairing of n players, round t in [0, n-1), match k in
+// [0, n/2): every pair exactly once in n - 1 rounds; p < q
+__host__ __device__ __forceinline__ void rr_match(int t, int k, int n, int& p, int& q) {
+  int a, b;
+  if (k == 0) {
+    a = t;
+    b = n - 1;
+  } else {
+    a = (t + k) % (n - 1);
+    b = (t - k + (n - 1)) % (n - 1);
+  }
+  p = a < b ? a : b;
+  q = a < b ? b : a;
+}
+
+// The block form's rotation: the angle to binary64 accuracy (tau from the dd difference of
+// the diagonal, t in binary64), then c = 1 / sqrt(1 + t^2) and s = t c in dd, so the
+// rotation is orthogonal to dd precision and annihilates a_pq to about 2^-53 of itself; the
+// pair's 2 x 2 block is then rotated in full (not replaced by the closed-form diagonal).
+// A Jacobi rotation need not take the exact angle: any orthogonal rotation keeps the
+// similarity exact, and the sweeps drive the remaining a_pq down.  One dd division and one
+// dd square root instead of three and two.
+__device__ __forceinline__ bool jac_rotation_fast(dd app, dd aqq, dd apq, dd& c, dd& s) {
+  c = mkdd(1.0);
+  s = mkdd(0.0);
+  if (apq.hi == 0.0) return false;
+  const dd d = dd_sub(aqq, app);
+  const double tau = (d.hi + d.lo) / (2.0 * (apq.hi + apq.lo));
+  if (!(__builtin_fabs(tau) <= 1e150)) return false;
+  const double t = (tau >= 0.0 ? 1.0 : -1.0) / (__builtin_fabs(tau) + sqrt(1.0 + tau * tau));
+  const dd td = mkdd(t);
+  c = dd_div(mkdd(1.0), dd_sqrt(dd_add(mkdd(1.0), dd_mul(td, td))));
+  s = dd_mul(td, c);
+  return true;
+}
+
+// global index of local row i of pair (P, R)'s subproblem
+__device__ __forceinline__ long bj_g(int i, int P, int R) { return i < BJ_B ? (long)P * BJ_B + i : (long)R * BJ_B + i - BJ_B; }
+
+// 2 x 2 block (rows p0, q0; columns p1, q1) of a 32 x 32 LDS matrix through the rotations
+// (c0, s0) on the rows and (c1, s1) on the columns: R0^T X R1 with R = [c s; -s c];
+// rows_first computes (R0^T X) R1, else R0^T (X R1): the two are transposes of each other
+// operation for operation, which keeps a symmetric matrix exactly symmetric
+__device__ __forceinline__ void bj_rot2(dd& x00, dd& x01, dd& x10, dd& x11, dd c0, dd s0, dd c1, dd s1,
+                                         bool rows_first) {
+  if (rows_first) {
+    const dd y00 = dd_sub(dd_mul(c0, x00), dd_mul(s0, x10)), y01 = dd_sub(dd_mul(c0, x01), dd_mul(s0, x11));
+    const dd y10 = dd_add(dd_mul(s0, x00), dd_mul(c0, x10)), y11 = dd_add(dd_mul(s0, x01), dd_mul(c0, x11));
+    x00 = dd_sub(dd_mul(c1, y00), dd_mul(s1, y01));
+    x01 = dd_add(dd_mul(s1, y00), dd_mul(c1, y01));
+    x10 = dd_sub(dd_mul(c1, y10), dd_mul(s1, y11));
+    x11 = dd_add(dd_mul(s1, y10), dd_mul(c1, y11));
+  } else {
+    const dd y00 = dd_sub(dd_mul(c1, x00), dd_mul(s1, x01)), y01 = dd_add(dd_mul(s1, x00), dd_mul(c1, x01));
+    const dd y10 = dd_sub(dd_mul(c1, x10), dd_mul(s1, x11)), y11 = dd_add(dd_mul(s1, x10), dd_mul(c1, x11));
+    x00 = dd_sub(dd_mul(c0, y00), dd_mul(s0, y10));
+    x01 = dd_sub(dd_mul(c0, y01), dd_mul(s0, y11));
+    x10 = dd_add(dd_mul(s0, y00), dd_mul(c0, y10));
+    x11 = dd_add(dd_mul(s0, y01), dd_mul(c0, y11));
+  }
+}
+
+// one inner sweep of pair w's subproblem; Q (row-major 32 x 32 dd) to q_h / q_l + w * 1024.
+// S is double-buffered (each step reads one copy and writes the other), and every Q entry
+// is read and written by the one thread that owns it, so a step takes two barriers.
+__global__ __launch_bounds__(256) void k_bj_sub(const double* __restrict__ ah, const double* __restrict__ al, long Nb,
+                                                const int2* __restrict__ pairs, double* __restrict__ q_h,
+                                                double* __restrict__ q_l) {
+  __shared__ double Sh[2][BJ_S][BJ_S + 1], Sl[2][BJ_S][BJ_S + 1];
+  __shared__ double Qh[BJ_S][BJ_S + 1], Ql[BJ_S][BJ_S + 1];
+  __shared__ double rc[4][BJ_B];  // c.hi, c.lo, s.hi, s.lo per match
+  __shared__ int ract[BJ_B];
+  const int tid = threadIdx.x;
+  const int2 pr = pairs[blockIdx.x];
+  for (int e = tid; e < BJ_S * BJ_S; e += 256) {
+    const int i = e / BJ_S, j = e % BJ_S;
+    const long o = bj_g(i, pr.x, pr.y) * Nb + bj_g(j, pr.x, pr.y);
+    Sh[0][i][j] = ah[o];
+    Sl[0][i][j] = al[o];
+    Qh[i][j] = i == j ? 1.0 : 0.0;
+    Ql[i][j] = 0.0;
+  }
+  __syncthreads();
+  const int I = tid >> 4, J = tid & 15;  // this thread's 2 x 2 block (matches I, J)
+  for (int t = 0; t < BJ_S - 1; ++t) {
+    const int cur = t & 1, nxt = cur ^ 1;
+    if (tid < BJ_B) {
+      int p, q;
+      rr_match(t, tid, BJ_S, p, q);
+      dd c, sn;
+      ract[tid] = jac_rotation_fast(mkdd(Sh[cur][p][p], Sl[cur][p][p]), mkdd(Sh[cur][q][q], Sl[cur][q][q]),
+                                    mkdd(Sh[cur][p][q], Sl[cur][p][q]), c, sn) ? 1 : 0;
+      rc[0][tid] = c.hi; rc[1][tid] = c.lo; rc[2][tid] = sn.hi; rc[3][tid] = sn.lo;
+    }
+    __syncthreads();
+    int p0, q0, p1, q1;
+    rr_match(t, I, BJ_S, p0, q0);
+    rr_match(t, J, BJ_S, p1, q1);
+    dd x00 = mkdd(Sh[cur][p0][p1], Sl[cur][p0][p1]), x01 = mkdd(Sh[cur][p0][q1], Sl[cur][p0][q1]);
+    dd x10 = mkdd(Sh[cur][q0][p1], Sl[cur][q0][p1]), x11 = mkdd(Sh[cur][q0][q1], Sl[cur][q0][q1]);
+    if (ract[I] || ract[J]) {
+      bj_rot2(x00, x01, x10, x11, mkdd(rc[0][I], rc[1][I]), mkdd(rc[2][I], rc[3][I]), mkdd(rc[0][J], rc[1][J]),
+              mkdd(rc[2][J], rc[3][J]), I > J);
+      if (I == J) x10 = x01;  // the pair's own block: symmetric exactly
+    }
+    Sh[nxt][p0][p1] = x00.hi; Sl[nxt][p0][p1] = x00.lo; Sh[nxt][p0][q1] = x01.hi; Sl[nxt][p0][q1] = x01.lo;
+    Sh[nxt][q0][p1] = x10.hi; Sl[nxt][q0][p1] = x10.lo; Sh[nxt][q0][q1] = x11.hi; Sl[nxt][q0][q1] = x11.lo;
+    // Q <- Q R: rows 2J, 2J+1 of the columns of match I (this thread's entries only)
+    if (ract[I]) {
+      const dd cq = mkdd(rc[0][I], rc[1][I]), sq = mkdd(rc[2][I], rc[3][I]);
+#pragma unroll
+      for (int e = 0; e < 2; ++e) {
+        const int r = 2 * J + e;
+        const dd a = mkdd(Qh[r][p0], Ql[r][p0]), b = mkdd(Qh[r][q0], Ql[r][q0]);
+        const dd na = dd_sub(dd_mul(cq, a), dd_mul(sq, b)), nb2 = dd_add(dd_mul(sq, a), dd_mul(cq, b));
+        Qh[r][p0] = na.hi; Ql[r][p0] = na.lo;
+        Qh[r][q0] = nb2.hi; Ql[r][q0] = nb2.lo;
+      }
+    }
+    __syncthreads();
+  }
+  double* oh = q_h + (size_t)blockIdx.x * BJ_S * BJ_S;
+  double* ol = q_l + (size_t)blockIdx.x * BJ_S * BJ_S;
+  for (int e = tid; e < BJ_S * BJ_S; e += 256) {
+    oh[e] = Qh[e / BJ_S][e % BJ_S];
+    ol[e] = Ql[e / BJ_S][e % BJ_S];
+  }
+}
+
+// X <- Q_i^T X Q_j for the 32 x 32 block of A on pair i's rows and pair j's columns, i <= j
+// (the linear block index over the upper triangle of pairs), the transpose written to the
+// mirror (i < j); the diagonal blocks keep their upper triangle and mirror it
+__global__ __launch_bounds__(256) void k_bj_apply(double* __restrict__ ah, double* __restrict__ al, long Nb,
+                                                  const int2* __restrict__ pairs, const double* __restrict__ q_h,
+                                                  const double* __restrict__ q_l) {
+  __shared__ double Xh[BJ_S][BJ_S + 1], Xl[BJ_S][BJ_S + 1];
+  __shared__ double Ah[BJ_S][BJ_S + 1], Al[BJ_S][BJ_S + 1];  // Q_i
+  __shared__ double Bh[BJ_S][BJ_S + 1], Bl[BJ_S][BJ_S + 1];  // Q_j
+  long j = (long)((__builtin_sqrt(8.0 * (double)blockIdx.x + 1.0) - 1.0) / 2.0);
+  while ((j + 1) * (j + 2) / 2 <= (long)blockIdx.x) ++j;
+  while (j * (j + 1) / 2 > (long)blockIdx.x) --j;
+  const long i = (long)blockIdx.x - j * (j + 1) / 2;  // i <= j
+  const int2 pi = pairs[i], pj = pairs[j];
+  const int tid = threadIdx.x;
+  for (int e = tid; e < BJ_S * BJ_S; e += 256) {
+    const int r = e / BJ_S, c = e % BJ_S;
+    const long o = bj_g(r, pi.x, pi.y) * Nb + bj_g(c, pj.x, pj.y);
+    Xh[r][c] = ah[o];
+    Xl[r][c] = al[o];
+    Ah[r][c] = q_h[i * BJ_S * BJ_S + e];
+    Al[r][c] = q_l[i * BJ_S * BJ_S + e];
+    Bh[r][c] = q_h[j * BJ_S * BJ_S + e];
+    Bl[r][c] = q_l[j * BJ_S * BJ_S + e];
+  }
+  __syncthreads();
+  const int r = tid >> 3, c0 = (tid & 7) * 4;  // this thread's four entries of row r
+  dd acc[4];
+#pragma unroll
+  for (int e = 0; e < 4; ++e) acc[e] = mkdd(0.0);
+  for (int k = 0; k < BJ_S; ++k) {  // T = X Q_j
+    const dd x = mkdd(Xh[r][k], Xl[r][k]);
+#pragma unroll
+    for (int e = 0; e < 4; ++e) acc[e] = dd_add(acc[e], dd_mul(x, mkdd(Bh[k][c0 + e], Bl[k][c0 + e])));
+  }
+  __syncthreads();
+#pragma unroll
+  for (int e = 0; e < 4; ++e) {
+    Xh[r][c0 + e] = acc[e].hi;
+    Xl[r][c0 + e] = acc[e].lo;
+    acc[e] = mkdd(0.0);
+  }
+  __syncthreads();
+  for (int k = 0; k < BJ_S; ++k) {  // Z = Q_i^T T
+    const dd qa = mkdd(Ah[k][r], Al[k][r]);
+#pragma unroll
+    for (int e = 0; e < 4; ++e) acc[e] = dd_add(acc[e], dd_mul(qa, mkdd(Xh[k][c0 + e], Xl[k][c0 + e])));
+  }
+  const long gr = bj_g(r, pi.x, pi.y);
+#pragma unroll
+  for (int e = 0; e < 4; ++e) {
+    const int c = c0 + e;
+    const long gc = bj_g(c, pj.x, pj.y);
+    if (i < j || r <= c) {
+      ah[gr * Nb + gc] = acc[e].hi;
+      al[gr * Nb + gc] = acc[e].lo;
+      if (i < j || r < c) {
+        ah[gc * Nb + gr] = acc[e].hi;
+        al[gc * Nb + gr] = acc[e].lo;
+      }
+    }
+  }
+}
+
+// V <- V Q_j on pair j's 32 columns for rows 32 blockIdx.x .. +31
+__global__ __launch_bounds__(256) void k_bj_vapply(double* __restrict__ vh, double* __restrict__ vl, long Nb,
+                                                   const int2* __restrict__ pairs, const double* __restrict__ q_h,
+                                                   const double* __restrict__ q_l) {
+  __shared__ double Th[BJ_S][BJ_S + 1], Tl[BJ_S][BJ_S + 1];
+  __shared__ double Bh[BJ_S][BJ_S + 1], Bl[BJ_S][BJ_S + 1];
+  const long j = blockIdx.y, r0 = (long)blockIdx.x * BJ_S;
+  const int2 pj = pairs[j];
+  const int tid = threadIdx.x;
+  for (int e = tid; e < BJ_S * BJ_S; e += 256) {
+    const int r = e / BJ_S, c = e % BJ_S;
+    const long o = (r0 + r) * Nb + bj_g(c, pj.x, pj.y);
+    Th[r][c] = vh[o];
+    Tl[r][c] = vl[o];
+    Bh[r][c] = q_h[j * BJ_S * BJ_S + e];
+    Bl[r][c] = q_l[j * BJ_S * BJ_S + e];
+  }
+  __syncthreads();
+  const int r = tid >> 3, c0 = (tid & 7) * 4;
+  dd acc[4];
+#pragma unroll
+  for (int e = 0; e < 4; ++e) acc[e] = mkdd(0.0);
+  for (int k = 0; k < BJ_S; ++k) {
+    const dd x = mkdd(Th[r][k], Tl[r][k]);
+#pragma unroll
+    for (int e = 0; e < 4; ++e) acc[e] = dd_add(acc[e], dd_mul(x, mkdd(Bh[k][c0 + e], Bl[k][c0 + e])));
+  }
+#pragma unroll
+  for (int e = 0; e < 4; ++e) {
+    const long o = (r0 + r) * Nb + bj_g(c0 + e, pj.x, pj.y);
+    vh[o] = acc[e].hi;
+    vl[o] = acc[e].lo;
+  }
+}
+
 // X[i][k] = V[i][sel[k]] * recip(sqrt(lambda_k)) (ortho.rs:185-190), V's columns the eigenvectors
 __global__ void k_ortho_x(const double* __restrict__ vh, const double* __restrict__ vl, long n,
                           const long* __restrict__ sel, const double* __restrict__ lam, long m, double* __restrict__ xh,
@@ -373,11 +621,137 @@ __global__ void k_gather_cols(const double* __restrict__ ih, const double* __res
 //   else:           sweep until max |a_ab| < 1e-60 (ortho.rs's test), at most max_sweeps.
 // evals: n dd (diagonal order of the original indices); evecs: n x n dd, row o =
 // eigenvector o when vec_rows, else column o.
+// smallest n that takes the block form (hp_eigen_set_block_min)
+constexpr long BJ_DEFAULT_MIN = 768;
+std::atomic<long> g_bj_min{BJ_DEFAULT_MIN};
+
+// The block form of jacobi_eigen_dev (same stop rules, outputs and sweep count meaning).
+int block_jacobi_dev(const double* a_h, const double* a_l, long n, bool stop_frobenius, long max_sweeps,
+                     double* e_h, double* e_l, double* x_h, double* x_l, bool vec_rows, long* sweeps_run,
+                     hipStream_t s) {
+  const long Nb = (n + BJ_S - 1) / BJ_S * BJ_S, nb = Nb / BJ_B, mp = nb / 2, steps = nb - 1;
+  if (Nb > 65536) {
+    set_error("jacobi_eigen: n = %ld too large (at most 65536)", n);
+    return HP_EINVAL;
+  }
+  const size_t nn = (size_t)Nb * Nb;
+  DevBuf A, V, Q, red, posb, pairb;
+  int rc;
+  if ((rc = A.alloc(2 * nn * sizeof(double))) || (rc = V.alloc(2 * nn * sizeof(double))) ||
+      (rc = Q.alloc(4 * (size_t)mp * BJ_S * BJ_S * sizeof(double))) || (rc = red.alloc(16)) ||
+      (rc = posb.alloc(n * sizeof(long))) || (rc = pairb.alloc((size_t)steps * mp * sizeof(int2))))
+    return rc;
+  double *ah = A.d(), *al = ah + nn, *vh = V.d(), *vl = vh + nn;
+  // two Q buffers (steps alternate), so the V update of a step can run on a second stream
+  // beside the next step's subproblems
+  const size_t qsz = (size_t)mp * BJ_S * BJ_S;
+  double* qbh[2] = {Q.d(), Q.d() + 2 * qsz};
+  double* qbl[2] = {Q.d() + qsz, Q.d() + 3 * qsz};
+  std::vector<int2> pr((size_t)steps * mp);
+  for (long t = 0; t < steps; ++t)
+    for (long k = 0; k < mp; ++k) {
+      int p, q;
+      rr_match((int)t, (int)k, (int)nb, p, q);
+      pr[t * mp + k] = make_int2(p, q);
+    }
+  HP_HIP(hipMemcpyAsync(pairb.p, pr.data(), pr.size() * sizeof(int2), hipMemcpyHostToDevice, s));
+  hipLaunchKernelGGL(k_jac_init, dim3(blocks((long)nn)), dim3(256), 0, s, a_h, a_l, n, Nb, ah, al, vh, vl);
+  HP_LAUNCH_CHECK();
+  auto off = [&](double* fro, double* mxv) -> int {
+    HP_HIP(hipMemsetAsync(red.p, 0, 16, s));
+    unsigned long long* mx = reinterpret_cast<unsigned long long*>(red.d() + 1);
+    hipLaunchKernelGGL(k_jac_off, dim3(blocks(Nb), (unsigned)Nb), dim3(256), 0, s, ah, al, Nb, red.d(), mx);
+    HP_LAUNCH_CHECK();
+    double host[2];
+    HP_HIP(hipMemcpyAsync(host, red.p, 16, hipMemcpyDeviceToHost, s));
+    HP_HIP(hipStreamSynchronize(s));
+    *fro = std::sqrt(host[0]);
+    std::memcpy(mxv, &host[1], 8);
+    return HP_OK;
+  };
+  // one outer sweep as a graph, replayed (everything is in place)
+  struct Graph {
+    hipStream_t cs = nullptr, vs = nullptr;
+    std::vector<hipEvent_t> ev;
+    hipGraphExec_t exec = nullptr;
+    ~Graph() {
+      if (exec) (void)hipGraphExecDestroy(exec);
+      if (vs) (void)hipStreamDestroy(vs);
+      if (cs) (void)hipStreamDestroy(cs);
+      for (hipEvent_t e : ev) (void)hipEventDestroy(e);
+    }
+  } gr;
+  HP_HIP(hipStreamCreateWithFlags(&gr.cs, hipStreamNonBlocking));
+  HP_HIP(hipStreamCreateWithFlags(&gr.vs, hipStreamNonBlocking));
+  gr.ev.assign(2 * steps + 2, nullptr);
+  for (hipEvent_t& e : gr.ev) HP_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+  HP_HIP(hipStreamBeginCapture(gr.cs, hipStreamCaptureModeThreadLocal));
+  HP_HIP(hipEventRecord(gr.ev[2 * steps], gr.cs));
+  HP_HIP(hipStreamWaitEvent(gr.vs, gr.ev[2 * steps], 0));
+  const int2* pd = static_cast<const int2*>(pairb.p);
+  // step t: Q_t into buffer t & 1; the V update of step t (second stream) reads it, so
+  // the subproblems of step t + 2, which rewrite that buffer, wait for it
+  for (long t = 0; t < steps; ++t) {
+    double *qh = qbh[t & 1], *ql = qbl[t & 1];
+    if (t >= 2) HP_HIP(hipStreamWaitEvent(gr.cs, gr.ev[2 * (t - 2) + 1], 0));
+    hipLaunchKernelGGL(k_bj_sub, dim3((unsigned)mp), dim3(256), 0, gr.cs, static_cast<const double*>(ah),
+                       static_cast<const double*>(al), Nb, pd + t * mp, qh, ql);
+    HP_HIP(hipEventRecord(gr.ev[2 * t], gr.cs));
+    hipLaunchKernelGGL(k_bj_apply, dim3((unsigned)(mp * (mp + 1) / 2)), dim3(256), 0, gr.cs, ah, al, Nb, pd + t * mp,
+                       static_cast<const double*>(qh), static_cast<const double*>(ql));
+    HP_HIP(hipStreamWaitEvent(gr.vs, gr.ev[2 * t], 0));
+    hipLaunchKernelGGL(k_bj_vapply, dim3((unsigned)(Nb / BJ_S), (unsigned)mp), dim3(256), 0, gr.vs, vh, vl, Nb,
+                       pd + t * mp, static_cast<const double*>(qh), static_cast<const double*>(ql));
+    HP_HIP(hipEventRecord(gr.ev[2 * t + 1], gr.vs));
+  }
+  HP_HIP(hipEventRecord(gr.ev[2 * steps + 1], gr.vs));
+  HP_HIP(hipStreamWaitEvent(gr.cs, gr.ev[2 * steps + 1], 0));
+  {
+    hipGraph_t g = nullptr;
+    HP_HIP(hipStreamEndCapture(gr.cs, &g));
+    hipError_t e = hipGraphInstantiate(&gr.exec, g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    HP_HIP(e);
+  }
+  bool converged = false;
+  long run = 0;
+  for (long it = 0; it < max_sweeps; ++it) {
+    double fro = 0.0, mxv = 0.0;
+    if ((rc = off(&fro, &mxv))) return rc;
+    if (stop_frobenius ? fro < 1e-30 : mxv < 1e-60) {
+      converged = true;
+      break;
+    }
+    HP_HIP(hipGraphLaunch(gr.exec, s));
+    ++run;
+  }
+  if (!converged && !stop_frobenius) {
+    double fro = 0.0, mxv = 0.0;
+    if ((rc = off(&fro, &mxv))) return rc;
+    converged = mxv < 1e-60;
+  }
+  if (sweeps_run) *sweeps_run = run;
+  if (!converged && stop_frobenius) {
+    set_error("Jacobi eigenvalue solver did not converge within max iterations");
+    return HP_ENOCONV;
+  }
+  std::vector<long> pos(n);
+  std::iota(pos.begin(), pos.end(), 0L);  // natural order: position = index
+  HP_HIP(hipMemcpyAsync(posb.p, pos.data(), n * sizeof(long), hipMemcpyHostToDevice, s));
+  hipLaunchKernelGGL(k_jac_extract, dim3(blocks(n * n)), dim3(256), 0, s, ah, al, vh, vl, n, Nb,
+                     static_cast<const long*>(posb.p), e_h, e_l, x_h, x_l, vec_rows ? 1 : 0);
+  HP_LAUNCH_CHECK();
+  HP_HIP(hipStreamSynchronize(s));
+  return HP_OK;
+}
+
 int jacobi_eigen_dev(const double* a_h, const double* a_l, long n, bool stop_frobenius, long max_sweeps,
                      double* e_h, double* e_l, double* x_h, double* x_l, bool vec_rows, long* sweeps_run,
                      hipStream_t s) {
   if (sweeps_run) *sweeps_run = 0;
   if (n <= 0) return HP_OK;
+  if (n >= g_bj_min.load())
+    return block_jacobi_dev(a_h, a_l, n, stop_frobenius, max_sweeps, e_h, e_l, x_h, x_l, vec_rows, sweeps_run, s);
   const long N2 = n + (n & 1), m = N2 / 2;
   if (N2 > 65534) {
     set_error("jacobi_eigen: n = %ld too large (at most 65534)", n);
@@ -634,6 +1008,11 @@ int solve_gen_eigen_dev(const double* f_h, const double* f_l, const double* s_h,
 using namespace hp;
 
 extern "C" {
+
+int hp_eigen_set_block_min(size_t n) {
+  g_bj_min.store(n == 0 ? BJ_DEFAULT_MIN : (long)std::min<size_t>(n, (size_t)LONG_MAX));
+  return HP_OK;
+}
 
 int hp_jacobi_eigen(const double* a, size_t n, size_t max_iter, double* evals, double* evecs) {
   if (n == 0) return HP_OK;

@@ -27,6 +27,14 @@ def qp():
     return quadprec_amd
 
 
+@pytest.fixture(params=[0, 1], ids=["pair_steps", "block_jacobi"])
+def path(qp, request):
+    """the scalar round-robin form (the default below n = 768) or the block form, forced"""
+    qp.eigen_set_block_min(2 ** 62 if request.param == 0 else 1)
+    yield request.param
+    qp.eigen_set_block_min(0)
+
+
 def dd(x):
     x = np.asarray(x, dtype=np.float64).ravel()
     return np.stack([x, np.zeros_like(x)], axis=1)
@@ -80,7 +88,7 @@ def test_jacobi_2x2(qp):  # linalg.rs:495-509
     assert abs(val(d)[0]) < 1e-28
 
 
-def test_jacobi_3x3(qp):  # linalg.rs:511-530
+def test_jacobi_3x3(qp, path):  # linalg.rs:511-530
     a = dd([4, 1, 0, 1, 3, 1, 0, 1, 2])
     ev, vec = qp.jacobi_eigen(a, 3, 200)
     for i in range(3):
@@ -128,7 +136,7 @@ def check_pencil(f, s, n, ev, c, m, bar):
     return worst
 
 
-def test_solve_gen_eigen_2x2(qp):  # ortho.rs:304-344
+def test_solve_gen_eigen_2x2(qp, path):  # ortho.rs:304-344
     f, s = dd([2.0, 1.0, 1.0, 3.0]), dd([1.0, 0.2, 0.2, 1.0])
     ev, c, m = qp.solve_gen_eigen(f, s, 2, 1e-8)
     assert m == 2
@@ -146,7 +154,7 @@ def test_solve_gen_eigen_identity_overlap(qp):  # ortho.rs:346-398
 
 # ---------------------------------------------------------------- against the oracle
 @pytest.mark.parametrize("n,seed", [(1, 0), (2, 1), (7, 2), (40, 3), (65, 4), (128, 5)])
-def test_eigen_matches_oracle(qp, n, seed):
+def test_eigen_matches_oracle(qp, path, n, seed):
     a = random_sym(n, seed)
     ev, vec = qp.jacobi_eigen(a, n, 60)
     ref, _ = O.jacobi_eigen(a, n, 60)
@@ -164,7 +172,7 @@ def test_eigen_matches_oracle(qp, n, seed):
     assert orth <= 8 * n * U
 
 
-def test_eigen_degenerate_and_diagonal(qp):
+def test_eigen_degenerate_and_diagonal(qp, path):
     n = 9
     rng = np.random.default_rng(7)
     q, _ = np.linalg.qr(rng.standard_normal((n, n)))
@@ -179,7 +187,7 @@ def test_eigen_degenerate_and_diagonal(qp):
     assert np.array_equal(val(ev), np.arange(5.0)) and np.array_equal(val(vec).reshape(5, 5), np.eye(5))
 
 
-def test_eigen_reports_no_convergence(qp):
+def test_eigen_reports_no_convergence(qp, path):
     a = random_sym(6, 8)
     with pytest.raises(qp.NotConverged):
         qp.jacobi_eigen(a, 6, 1)  # one check, one sweep, no further check (linalg.rs:305-359)
@@ -196,7 +204,7 @@ def overlap(qp, n, seed, lo, spacing=1.4):
 # threshold 5e-5: three directions are dropped
 @pytest.mark.parametrize("n,seed,lo,thr,spacing,dropped", [(64, 0, 0.01, 1e-8, 1.4, 0), (125, 1, 0.05, 1e-6, 1.4, 0),
                                                            (33, 2, 0.3, 1e-8, 1.4, 0), (64, 0, 0.01, 5e-5, 0.4, 3)])
-def test_canonical_orthogonalize_overlap(qp, n, seed, lo, thr, spacing, dropped):
+def test_canonical_orthogonalize_overlap(qp, path, n, seed, lo, thr, spacing, dropped):
     s = overlap(qp, n, seed, lo, spacing)
     x, m = qp.canonical_orthogonalize(s, n, thr)
     xr, mr = O.canonical_orthogonalize(s, n, thr)
@@ -227,7 +235,7 @@ def test_canonical_orthogonalize_overlap(qp, n, seed, lo, thr, spacing, dropped)
 
 
 @pytest.mark.parametrize("n,seed", [(24, 3), (48, 4)])
-def test_solve_gen_eigen_matches_oracle(qp, n, seed):
+def test_solve_gen_eigen_matches_oracle(qp, path, n, seed):
     s = overlap(qp, n, seed, 0.1)
     f = random_sym(n, 100 + seed)
     thr = 1e-8
