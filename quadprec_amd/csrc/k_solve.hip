@@ -15,6 +15,7 @@
 #include <chrono>
 #include <cmath>
 #include <cstring>
+#include <type_traits>
 #include <vector>
 
 #include "hp_chol.hpp"
@@ -161,61 +162,92 @@ __global__ __launch_bounds__(64) void k_diag_block_inverse(Mat l, long n, double
 // 64 x 64 diagonal blocks:
 //   workgroup b < nblk:        z_b = inv(L_bb)   (y_b - sum_{j<b} L_bj z_j)
 //   workgroup nblk + c:        x_b = inv(L_bb)^T (z_b - sum_{j>b} L_jb^T x_j),  b = nblk-1-c
-// Each block publishes its result with a flag: wavefront 0 stores the block, takes one
-// agent-scope release fence and sets the flag; a consumer polls flags with relaxed loads,
-// takes one acquire fence, then every block whose flag is already set, so a block
-// far down the chain keeps up with the front.  Every wait is on a lower-numbered
-// workgroup: under in-order dispatch the smallest-numbered resident workgroup never waits
-// on one that is not running, so the chain always advances, whatever the residency.  Each
-// wait is also bounded (flag_wait): past the limit it raises *fail and the workgroups
-// leave (the host reports a device error).  Summation order: eight lanes per row, four
-// interleaved chains per lane, combined by lane shuffles (these solves only feed the
-// refinement).
+// Results travel between workgroups as self-validating values: z and x start filled with
+// EMPTY (a signalling-NaN pattern that no arithmetic result carries: IEEE operations
+// return quiet NaNs), a block's values are published with relaxed agent-scope atomic
+// stores, and a consumer polls them with relaxed agent-scope atomic loads until none of a
+// block's values is EMPTY.  Every value is its own atomic, so no flags, fences or second
+// round trip are needed between a block's stores and its use.  Each of the 8 wavefronts
+// polls one of the next 8 dependency blocks per round (64 lanes, one element each), so a
+// workgroup that is behind takes up to 8 ready blocks per round.  Every wait is on a
+// lower-numbered workgroup: under in-order dispatch the smallest-numbered resident
+// workgroup never waits on one that is not running, so the chain always advances,
+// whatever the residency.  Each wait is also bounded: past SPIN_LIMIT polls it raises
+// *fail and the workgroups leave (the host reports a device error).  The consumer of x
+// (k_add_absmax) refills z and x with EMPTY for the next solve.
+// Summation: eight lanes per row, four interleaved chains per lane, each a compensated
+// multiply-accumulate (dd products with the low cross terms, two_sum of the leading
+// parts, the errors summed in a second double), renormalised at the end; these solves only
+// feed the refinement, so neither the reference's order nor its operations are needed.
 constexpr int SPIN_LIMIT = 1 << 22;
+constexpr unsigned long long EMPTY = 0x7FF0DEADBEEF0001ull;
 
-// Flags are polled with relaxed agent-scope loads (coherent, no cache invalidation); the
-// consumer's workgroup takes one acquire fence after the barrier that follows a successful
-// wait.  (An acquire load per poll would invalidate the caches on every spin.)
-__device__ __forceinline__ int ld_flag(const int* f) {
-  return __hip_atomic_load(f, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+__device__ __forceinline__ double ld_coherent(const double* p) {
+  return __builtin_bit_cast(double, __hip_atomic_load((const unsigned long long*)p, __ATOMIC_RELAXED,
+                                                      __HIP_MEMORY_SCOPE_AGENT));
 }
-__device__ bool flag_wait(const int* f, int epoch, int* fail) {
-  for (int spins = 0;; ++spins) {
-    if (ld_flag(f) >= epoch) return true;
-    if ((spins & 255) == 255 && __hip_atomic_load(fail, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) return false;
-    if (spins > SPIN_LIMIT) {
-      __hip_atomic_store(fail, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-      return false;
-    }
-    __builtin_amdgcn_s_sleep(1);
-  }
+__device__ __forceinline__ void st_coherent(double* p, double v) {
+  __hip_atomic_store((unsigned long long*)p, __builtin_bit_cast(unsigned long long, v), __ATOMIC_RELAXED,
+                     __HIP_MEMORY_SCOPE_AGENT);
 }
+__device__ __forceinline__ bool is_empty(double v) { return __builtin_bit_cast(unsigned long long, v) == EMPTY; }
 
-// the sum over the eight lanes 8r..8r+7 that share a row (each holds 8 of its 64 terms)
+// compensated multiply-accumulate: a (h + l, unnormalised) += x y
+struct AccDD {
+  double h, l;
+};
+__device__ __forceinline__ void cacc_mac(AccDD& a, dd x, dd y) {
+  const double p = x.hi * y.hi;
+  double e = __builtin_fma(x.hi, y.hi, -p);
+  e = __builtin_fma(x.hi, y.lo, e);
+  e = __builtin_fma(x.lo, y.hi, e);
+  double s, t;
+  two_sum(a.h, p, s, t);
+  a.h = s;
+  a.l = a.l + (t + e);
+}
+__device__ __forceinline__ void cacc_add(AccDD& a, AccDD b) {
+  double s, t;
+  two_sum(a.h, b.h, s, t);
+  a.h = s;
+  a.l = a.l + (b.l + t);
+}
+__device__ __forceinline__ void cacc_mac(double& a, double x, double y) { a = __builtin_fma(x, y, a); }
+__device__ __forceinline__ void cacc_add(double& a, double b) { a = a + b; }
+__device__ __forceinline__ dd cacc_value(AccDD a) { return renorm(a.h, a.l); }
+__device__ __forceinline__ double cacc_value(double a) { return a; }
+__device__ __forceinline__ AccDD cacc_shfl_xor(AccDD a, int off) {
+  return AccDD{__shfl_xor(a.h, off, 64), __shfl_xor(a.l, off, 64)};
+}
+__device__ __forceinline__ double cacc_shfl_xor(double a, int off) { return __shfl_xor(a, off, 64); }
+
 template <class A>
-__device__ __forceinline__ typename A::T octet_sum(typename A::T q) {
+using AccOf = typename std::conditional<A::LIMBS == 2, AccDD, double>::type;
+
+// the four chains of a lane, then the eight lanes 8r..8r+7 that share a row
+template <class A>
+__device__ __forceinline__ typename A::T row_total(AccOf<A> (&p)[4]) {
+  cacc_add(p[0], p[1]);
+  cacc_add(p[2], p[3]);
+  cacc_add(p[0], p[2]);
 #pragma unroll
-  for (int off = 1; off < 8; off <<= 1) {
-    typename A::T o;
-    if constexpr (A::LIMBS == 2) o = mkdd(__shfl_xor(q.hi, off, 64), __shfl_xor(q.lo, off, 64));
-    else o = __shfl_xor(q, off, 64);
-    q = A::add(q, o);
-  }
-  return q;
+  for (int off = 1; off < 8; off <<= 1) cacc_add(p[0], cacc_shfl_xor(p[0], off));
+  return cacc_value(p[0]);
 }
 
-// 512 threads: row r = t / 8 of the block, part = t % 8 holds terms 8 part .. 8 part + 7 of
-// each 64-term sum, as four 2-term chains
 template <class A>
 __global__ __launch_bounds__(512) void k_inv_solve(Mat l, const double* __restrict__ ih, const double* __restrict__ il,
                                                    const double* yh, const double* yl, double* zh, double* zl,
-                                                   double* xh, double* xl, long n, int nblk, int* flags, int epoch,
-                                                   int* fail, unsigned long long* red) {
+                                                   double* xh, double* xl, long n, int nblk, int* fail,
+                                                   unsigned long long* red) {
   using T = typename A::T;
+  using Acc = AccOf<A>;
   constexpr int TPL = 8;  // terms per lane
+  constexpr int NW = 8;   // wavefronts, one polled block each
+  __shared__ double Vh[NW][BW], Vl[A::LIMBS == 2 ? NW : 1][BW];
   __shared__ double Th[BW], Tl[BW], Oh[BW], Ol[BW];
-  __shared__ int s_ready[2], s_ok[2], s_zok;
-  const int t = threadIdx.x, part = t & 7, kb = part * TPL, r = t >> 3;
+  __shared__ int s_valid[NW], s_abort;
+  const int t = threadIdx.x, part = t & 7, kb = part * TPL, r = t >> 3, w = t >> 6, lane = t & 63;
   const bool fwd = (int)blockIdx.x < nblk;
   const int b = fwd ? (int)blockIdx.x : 2 * nblk - 1 - (int)blockIdx.x;
   const long b0 = (long)b * BW;
@@ -230,8 +262,49 @@ __global__ __launch_bounds__(512) void k_inv_solve(Mat l, const double* __restri
     const int k = kb + u;
     iv[u] = A::ldv(ih, il, (long)(base + (fwd ? (size_t)r * BW + k : (size_t)k * BW + r)));
   }
-  const int* dep = flags + (fwd ? 0 : nblk);
-  T p[4] = {A::zero(), A::zero(), A::zero(), A::zero()};
+  // Polls blocks first, first + step, ... (at most min(NW, count)) of (ph, pl) into the
+  // slots Vh[q] until the first is complete; returns how many leading blocks are, or 0
+  // when the wait was abandoned.  Called by the whole workgroup.
+  auto poll = [&](const double* ph, const double* pl, int first, int step, int count) -> int {
+    const int m = min(NW, count);
+    for (int spins = 0;; ++spins) {
+      if (w < m) {
+        const long e = (long)(first + w * step) * BW + lane;
+        bool v = true;
+        double h = 0.0, lo = 0.0;
+        if (e < n) {
+          h = ld_coherent(ph + e);
+          v = !is_empty(h);
+          if constexpr (A::LIMBS == 2) {
+            lo = ld_coherent(pl + e);
+            v = v && !is_empty(lo);
+          }
+        }
+        Vh[w][lane] = h;
+        if constexpr (A::LIMBS == 2) Vl[w][lane] = lo;
+        const bool all = __all(v);
+        if (lane == 0) s_valid[w] = all;
+      }
+      if (t == 0) {
+        s_abort = 0;
+        if ((spins & 255) == 255 && __hip_atomic_load(fail, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) s_abort = 1;
+        if (spins > SPIN_LIMIT) {
+          __hip_atomic_store(fail, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+          s_abort = 1;
+        }
+      }
+      __syncthreads();
+      int ready = 0;
+      while (ready < m && s_valid[ready]) ++ready;
+      if (ready > 0) return ready;
+      if (s_abort) return 0;
+      __syncthreads();  // every wavefront has read s_valid before the next poll rewrites it
+      __builtin_amdgcn_s_sleep(1);
+    }
+  };
+  Acc p[4];
+#pragma unroll
+  for (int q = 0; q < 4; ++q) p[q] = Acc{};
   // this lane's factor entries of dependency block j: fwd L[b0+r][j0+kb+u] (row r of L_bj),
   // bwd L[j0+kb+u][b0+r] (column r of L_jb); zero outside the matrix
   auto load_l = [&](int j, T (&lv)[TPL]) {
@@ -245,98 +318,86 @@ __global__ __launch_bounds__(512) void k_inv_solve(Mat l, const double* __restri
   };
   const double* vh = fwd ? zh : xh;  // the dependencies' results
   const double* vl = fwd ? zl : xl;
-  auto take = [&](int j, const T (&lv)[TPL]) {
-    const long j0 = (long)j * BW;
-    const int bwj = (int)min((long)BW, n - j0);
-    T vv[TPL];
-#pragma unroll
-    for (int u = 0; u < TPL; ++u) vv[u] = kb + u < bwj ? A::ldv(vh, vl, j0 + kb + u) : A::zero();
-#pragma unroll
-    for (int u = 0; u < TPL; ++u) p[u & 3] = A::add(p[u & 3], A::mul(lv[u], vv[u]));
-  };
   int next = fwd ? 0 : nblk - 1;  // next dependency to take: fwd ascending below b, bwd descending above b
   const int step = fwd ? 1 : -1;
-  bool ok = true;
+  int remaining = fwd ? b : nblk - 1 - b;
   T lv[TPL];
-  for (int round = 0; fwd ? next < b : next > b; ++round) {
-    load_l(next, lv);  // the factor does not depend on the flags: in flight during the wait
-    if (t == 0) {
-      int k = next;
-      bool w = flag_wait(dep + k, epoch, fail);
-      if (w) {
-        k += step;
-        while ((fwd ? k < b : k > b) && ld_flag(dep + k) >= epoch) k += step;
+  while (remaining > 0) {
+    load_l(next, lv);  // the factor does not depend on the results: in flight during the poll
+    const int ready = poll(vh, vl, next, step, remaining);
+    if (ready == 0) return;
+    for (int q = 0; q < ready; ++q) {
+      if (q > 0) load_l(next + q * step, lv);
+#pragma unroll
+      for (int u = 0; u < TPL; ++u) {
+        T vv;
+        if constexpr (A::LIMBS == 2) vv = mkdd(Vh[q][kb + u], Vl[q][kb + u]);
+        else vv = Vh[q][kb + u];
+        cacc_mac(p[u & 3], lv[u], vv);  // lanes past the matrix hold zero factor entries
       }
-      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");  // the flagged blocks' stores are visible
-      s_ready[round & 1] = k;
-      s_ok[round & 1] = w;
     }
-    __syncthreads();
-    const int ready = s_ready[round & 1];
-    if (!s_ok[round & 1]) { ok = false; break; }
-    for (int j = next; j != ready; j += step) {
-      if (j != next) load_l(j, lv);
-      take(j, lv);
-    }
-    next = ready;
+    next += ready * step;
+    remaining -= ready;
+    __syncthreads();  // the slots are rewritten by the next poll
   }
-  if (!ok) return;
-  if (!fwd) {  // z_b of the forward pass
-    if (t == 0) {
-      s_zok = flag_wait(flags + b, epoch, fail);
-      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
-    }
-    __syncthreads();
-    if (!s_zok) return;
+  T own = A::zero();  // bwd: z_b of the forward pass
+  if (!fwd) {
+    if (poll(zh, zl, b, 1, 1) == 0) return;
+    if constexpr (A::LIMBS == 2) own = mkdd(Vh[0][r], Vl[0][r]);
+    else own = Vh[0][r];
   }
-  const T sum = octet_sum<A>(A::add(A::add(p[0], p[1]), A::add(p[2], p[3])));
+  const T sum = row_total<A>(p);
   if (part == 0) {
     T v = A::zero();
-    if (live) v = A::sub(fwd ? A::ldv(yh, yl, b0 + r) : A::ldv(zh, zl, b0 + r), sum);
+    if (live) v = A::sub(fwd ? A::ldv(yh, yl, b0 + r) : own, sum);
     if constexpr (A::LIMBS == 2) { Th[r] = v.hi; Tl[r] = v.lo; }
     else Th[r] = v;
   }
   __syncthreads();
-  T q[4] = {A::zero(), A::zero(), A::zero(), A::zero()};
+  Acc q[4];
+#pragma unroll
+  for (int u = 0; u < 4; ++u) q[u] = Acc{};
 #pragma unroll
   for (int u = 0; u < TPL; ++u) {
     T tv;
     if constexpr (A::LIMBS == 2) tv = mkdd(Th[kb + u], Tl[kb + u]);
     else tv = Th[kb + u];
-    q[u & 3] = A::add(q[u & 3], A::mul(iv[u], tv));
+    cacc_mac(q[u & 3], iv[u], tv);
   }
-  const T o = octet_sum<A>(A::add(A::add(q[0], q[1]), A::add(q[2], q[3])));
+  const T o = row_total<A>(q);
   if (part == 0) {
     if constexpr (A::LIMBS == 2) { Oh[r] = o.hi; Ol[r] = o.lo; }
     else Oh[r] = o;
   }
   __syncthreads();
-  if (t < 64) {  // wavefront 0 stores the block and publishes it: one release for the block
-    if (t < bw) {
-      T v;
-      if constexpr (A::LIMBS == 2) v = mkdd(Oh[t], Ol[t]);
-      else v = Oh[t];
-      if (fwd) A::stv(zh, zl, b0 + t, v);
-      else A::stv(xh, xl, b0 + t, v);
-    }
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
-    if (t == 0) __hip_atomic_store(flags + (fwd ? 0 : nblk) + b, epoch, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  if (t < bw) {  // publish the block: one coherent store per value
+    double* oh = fwd ? zh : xh;
+    double* ol = fwd ? zl : xl;
+    st_coherent(oh + b0 + t, Oh[t]);
+    if constexpr (A::LIMBS == 2) st_coherent(ol + b0 + t, Ol[t]);
   }
 }
 
 // x += dx (dd += dd, or dd += f64 when dl is null), and red[0] = max|dx.hi|, red[1] =
 // max|x.hi| after the update (bit patterns of non-negative doubles order as integers);
-// red is cleared by the solve kernel that precedes this one on the stream
-__global__ __launch_bounds__(256) void k_add_absmax(double* xh, double* xl, const double* dh, const double* dl, long n,
-                                                    unsigned long long* red) {
+// red is cleared by the solve kernel that precedes this one on the stream.  dx and the
+// solve's intermediate z (zh, zl; zl may be null) are refilled with EMPTY for the next solve.
+__global__ __launch_bounds__(256) void k_add_absmax(double* xh, double* xl, double* dh, double* dl, long n,
+                                                    unsigned long long* red, double* zh, double* zl) {
   __shared__ double m0[256], m1[256];
+  const double empty = __builtin_bit_cast(double, EMPTY);
   double a0 = 0.0, a1 = 0.0;
   for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
-    dd r = dl ? dd_add(mkdd(xh[i], xl[i]), mkdd(dh[i], dl[i])) : dd_add_f(mkdd(xh[i], xl[i]), dh[i]);
+    const double d0 = dh[i];
+    dd r = dl ? dd_add(mkdd(xh[i], xl[i]), mkdd(d0, dl[i])) : dd_add_f(mkdd(xh[i], xl[i]), d0);
     xh[i] = r.hi;
     xl[i] = r.lo;
-    a0 = fmax(a0, fabs(dh[i]));
+    a0 = fmax(a0, fabs(d0));
     a1 = fmax(a1, fabs(r.hi));
+    dh[i] = empty;
+    if (dl) dl[i] = empty;
+    zh[i] = empty;
+    if (zl) zl[i] = empty;
   }
   m0[threadIdx.x] = a0;
   m1[threadIdx.x] = a1;
@@ -352,6 +413,10 @@ __global__ __launch_bounds__(256) void k_add_absmax(double* xh, double* xl, cons
     atomicMax(red, __builtin_bit_cast(unsigned long long, m0[0]));
     atomicMax(red + 1, __builtin_bit_cast(unsigned long long, m1[0]));
   }
+}
+__global__ void k_fill_empty(double* p, long n) {
+  long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = __builtin_bit_cast(double, EMPTY);
 }
 
 // ---------------------------------------------------------------- qd residual
@@ -446,15 +511,15 @@ int diag_inverses(Mat l, long n, double* ih, double* il, hipStream_t s) {
   HP_LAUNCH_CHECK();
   return HP_OK;
 }
-// L L^T x = y through the persistent kernel; z (n) is scratch, flags (2 nblk ints, zero
-// before the first call) and fail belong to the caller, epoch grows by one per call
+// L L^T x = y through the persistent kernel; z (n) is scratch, z and x hold EMPTY on entry
+// (k_fill_empty, or k_add_absmax after the previous solve), fail belongs to the caller
 template <class A>
 int solve_inv_persistent(Mat l, const double* ih, const double* il, const double* yh, const double* yl, double* zh,
-                         double* zl, double* xh, double* xl, long n, int* flags, int epoch, int* fail,
-                         unsigned long long* red, hipStream_t s) {
+                         double* zl, double* xh, double* xl, long n, int* fail, unsigned long long* red,
+                         hipStream_t s) {
   const int nblk = (int)((n + BW - 1) / BW);
   hipLaunchKernelGGL(k_inv_solve<A>, dim3((unsigned)(2 * nblk)), dim3(512), 0, s, l, ih, il, yh, yl, zh, zl, xh, xl, n,
-                     nblk, flags, epoch, fail, red);
+                     nblk, fail, red);
   HP_LAUNCH_CHECK();
   return HP_OK;
 }
@@ -563,9 +628,8 @@ int refine_loop(int kind, Mat Lm, const double* ivh, const double* ivl, const do
                 const double* bh, const double* bl, long n, double tol, int max_iter, double* xh, double* xl,
                 int* iters, double* last_out, int* converged, hipStream_t s) {
   int rc;
-  const long nblk = (n + BW - 1) / BW;
   DevBuf vec, sc;
-  if ((rc = vec.alloc(6 * n * sizeof(double))) || (rc = sc.alloc(64 + (2 * nblk + 1) * sizeof(int)))) return rc;
+  if ((rc = vec.alloc(6 * n * sizeof(double))) || (rc = sc.alloc(64))) return rc;
   double* rh = vec.d();
   double* rlo = rh + n;
   double* dh = rh + 2 * n;  // the correction (dd, or f64 in dh for the f64 factor)
@@ -574,10 +638,11 @@ int refine_loop(int kind, Mat Lm, const double* ivh, const double* ivl, const do
   double* zl = rh + 5 * n;
   unsigned long long* red = (unsigned long long*)sc.p;  // max|dx|, max|x|
   int* fail = (int*)((char*)sc.p + 16);
-  int* flags = (int*)((char*)sc.p + 64);
-  HP_HIP(hipMemsetAsync(sc.p, 0, 64 + (2 * nblk + 1) * sizeof(int), s));
+  HP_HIP(hipMemsetAsync(sc.p, 0, 64, s));
   HP_HIP(hipMemsetAsync(xh, 0, n * sizeof(double), s));
   HP_HIP(hipMemsetAsync(xl, 0, n * sizeof(double), s));
+  hipLaunchKernelGGL(k_fill_empty, dim3(g1(4 * n)), dim3(256), 0, s, dh, 4 * n);  // dx and z (contiguous)
+  HP_LAUNCH_CHECK();
   double last = 1.0, best = HUGE_VAL;
   int it = 0;
   *converged = HP_NOT_CONVERGED;
@@ -593,15 +658,14 @@ int refine_loop(int kind, Mat Lm, const double* ivh, const double* ivl, const do
     if (kind == HP_FACTOR_F64) {
       hipLaunchKernelGGL(k_dd_to_f64, dim3(g1(n)), dim3(256), 0, s, yh_, yl_, dh, n);  // f64 right-hand side
       HP_LAUNCH_CHECK();
-      if ((rc = solve_inv_persistent<ArF64>(Lm, ivh, nullptr, dh, nullptr, zh, nullptr, dl, nullptr, n, flags,
-                                            it + 1, fail, red, s)))
+      if ((rc = solve_inv_persistent<ArF64>(Lm, ivh, nullptr, dh, nullptr, zh, nullptr, dl, nullptr, n, fail, red, s)))
         return rc;
       hipLaunchKernelGGL(k_add_absmax, dim3(std::min<unsigned>(g1(n), 1024)), dim3(256), 0, s, xh, xl, dl, nullptr, n,
-                         red);
+                         red, zh, nullptr);
     } else {
-      if ((rc = solve_inv_persistent<ArDD>(Lm, ivh, ivl, yh_, yl_, zh, zl, dh, dl, n, flags, it + 1, fail, red, s)))
-        return rc;
-      hipLaunchKernelGGL(k_add_absmax, dim3(std::min<unsigned>(g1(n), 1024)), dim3(256), 0, s, xh, xl, dh, dl, n, red);
+      if ((rc = solve_inv_persistent<ArDD>(Lm, ivh, ivl, yh_, yl_, zh, zl, dh, dl, n, fail, red, s))) return rc;
+      hipLaunchKernelGGL(k_add_absmax, dim3(std::min<unsigned>(g1(n), 1024)), dim3(256), 0, s, xh, xl, dh, dl, n, red,
+                         zh, zl);
     }
     HP_LAUNCH_CHECK();
     struct { unsigned long long m[2]; int fail; } h{};
