@@ -67,61 +67,113 @@ __device__ __forceinline__ void stm(double* h, double* l, long o, dd v) { h[o] =
 // else the factor L = A itself); results go to A.
 
 // Diagonal block [jb0, jb0 + jb): the unblocked algorithm (linalg.rs:250-255, i.e. 42-96 on
-// the block) in the right-looking form that keeps each element's k order: the partial sums
-// S_ic accumulate l_ij l_cj after column j is final.  All NT threads; returns false (every
-// thread) on a breakdown, with status[inst] = pivot + 1.  On success the block's lower
-// triangle is written to A and stays in d.Lh / d.Ll.
+// the block) in the right-looking form that keeps each element's k order: the partial sum
+// S_ic starts at zero and accumulates l_ij l_cj (one dd multiply and one dd add) once column
+// j is final; column c is finished as l_cc = sqrt(a_cc - S_cc), l_ic = (a_ic - S_ic) *
+// recip(l_cc).  One barrier per column: wavefront 0 (lane = row) finishes column j+1 --
+// it adds column j's product to S(., j+1) itself, takes the pivot on lane j+1 and
+// broadcasts its reciprocal with v_readlane -- while the seven other wavefronts add column
+// j's products to the columns beyond j+1.  Their partial sums live in registers: element e
+// of the packed triangle (e = q(q+1)/2 + p, 0 <= p <= q, is (jb-1-p, jb-1-q)) belongs to
+// thread 64 + e mod 448 (the columns still active form a prefix of the packing, so the work
+// stays spread evenly), and an element is handed to wavefront 0 through a two-column LDS
+// buffer when its column is next but one.  Returns false (every thread) on a breakdown,
+// with status[inst] = pivot + 1.  On success the block's lower triangle is written to A and
+// stays in d.Lh / d.Ll (d.Sh / d.Sl are not used).
+constexpr int DG_OWN = 5;  // partial sums per trailing thread: 5 * 448 >= 62 * 63 / 2
 __device__ __forceinline__ bool bt_diag(const double* Srch, const double* Srcl, double* Ah, double* Al, long n, long jb0, int jb,
-                        DiagLds& d, double* s_inv, int* s_fail, int* status, int inst) {
-  const int tid = threadIdx.x;
+                        DiagLds& d, double* /*s_inv*/, int* s_fail, int* status, int inst) {
+  __shared__ double Hh[2][NB], Hl[2][NB];  // partial sums of column c for wavefront 0, by the parity of c
+  const int tid = threadIdx.x, lane = tid & 63;
   for (int idx = tid; idx < jb * jb; idx += NT) {
     int i = idx / jb, c = idx % jb;
     if (c <= i) {
       long o = (jb0 + i) * n + jb0 + c;
       d.Lh[i][c] = Srch[o];
       d.Ll[i][c] = Srcl[o];
-      d.Sh[i][c] = 0.0;
-      d.Sl[i][c] = 0.0;
     }
   }
   if (tid == 0) *s_fail = 0;
+  const int cnt_all = (jb - 2) * (jb - 1) / 2;
+  dd sv[DG_OWN];
+  int ei[DG_OWN], ec[DG_OWN];
+  if (tid >= 64) {
+#pragma unroll
+    for (int u = 0; u < DG_OWN; ++u) {
+      const int e = tid - 64 + u * (NT - 64);
+      int q = (int)((__builtin_sqrtf(8.0f * (float)e + 1.0f) - 1.0f) * 0.5f);
+      while (q * (q + 1) / 2 > e) --q;
+      while ((q + 1) * (q + 2) / 2 <= e) ++q;
+      ei[u] = jb - 1 - (e - q * (q + 1) / 2);
+      ec[u] = jb - 1 - q;
+      if (e >= cnt_all) {
+        ei[u] = 0;
+        ec[u] = -1;  // no element
+      }
+      sv[u] = mkdd(0.0, 0.0);
+    }
+  }
   __syncthreads();
-  for (int j = 0; j < jb; ++j) {
-    if (tid == 0) {
-      dd diag = dd_sub(mkdd(d.Lh[j][j], d.Ll[j][j]), mkdd(d.Sh[j][j], d.Sl[j][j]));
+  dd lcol = mkdd(0.0, 0.0);  // wavefront 0: l(lane, j) of the last finished column
+  auto finish_column = [&](int c, dd sc) {
+    dd inv = mkdd(0.0, 0.0);
+    int bad = 0;
+    if (lane == c) {
+      const dd diag = dd_sub(mkdd(d.Lh[c][c], d.Ll[c][c]), sc);
       if (diag.hi <= 0.0) {
         *s_fail = 1;
-        status[inst] = (int)(jb0 + j) + 1;
+        status[inst] = (int)(jb0 + c) + 1;
+        bad = 1;
       } else {
-        dd ljj = dd_sqrt(diag);
-        d.Lh[j][j] = ljj.hi;
-        d.Ll[j][j] = ljj.lo;
-        dd inv = dd_recip(ljj);
-        s_inv[0] = inv.hi;
-        s_inv[1] = inv.lo;
+        const dd ljj = dd_sqrt(diag);
+        d.Lh[c][c] = ljj.hi;
+        d.Ll[c][c] = ljj.lo;
+        inv = dd_recip(ljj);
       }
     }
-    __syncthreads();
-    if (*s_fail) return false;
-    const dd inv = mkdd(s_inv[0], s_inv[1]);
-    for (int i = j + 1 + tid; i < jb; i += NT) {
-      dd v = dd_mul(dd_sub(mkdd(d.Lh[i][j], d.Ll[i][j]), mkdd(d.Sh[i][j], d.Sl[i][j])), inv);
-      d.Lh[i][j] = v.hi;
-      d.Ll[i][j] = v.lo;
+    if (__builtin_amdgcn_readlane(bad, c)) return;
+    inv = rlane(inv, c);
+    if (lane > c && lane < jb) {
+      lcol = dd_mul(dd_sub(mkdd(d.Lh[lane][c], d.Ll[lane][c]), sc), inv);
+      d.Lh[lane][c] = lcol.hi;
+      d.Ll[lane][c] = lcol.lo;
     }
-    __syncthreads();
-    const int m = jb - j - 1;
-    for (int idx = tid; idx < m * m; idx += NT) {
-      int ii = idx / m, cc = idx % m;
-      if (cc <= ii) {
-        int i = j + 1 + ii, c = j + 1 + cc;
-        dd v = dd_add(mkdd(d.Sh[i][c], d.Sl[i][c]), dd_mul(mkdd(d.Lh[i][j], d.Ll[i][j]), mkdd(d.Lh[c][j], d.Ll[c][j])));
-        d.Sh[i][c] = v.hi;
-        d.Sl[i][c] = v.lo;
+  };
+  if (tid < 64) finish_column(0, mkdd(0.0, 0.0));
+  __syncthreads();
+  for (int j = 0; j + 1 < jb; ++j) {
+    if (*s_fail) return false;
+    if (tid < 64) {
+      const int c = j + 1;
+      const dd ljc = rlane(lcol, c);  // l(j+1, j)
+      dd sc = mkdd(0.0, 0.0);
+      if (lane >= c && lane < jb) {
+        const dd sin = c == 1 ? mkdd(0.0, 0.0) : mkdd(Hh[c & 1][lane], Hl[c & 1][lane]);
+        sc = dd_add(sin, dd_mul(lcol, ljc));
+      }
+      finish_column(c, sc);
+    } else {
+      dd li[DG_OWN], lc[DG_OWN];
+#pragma unroll
+      for (int u = 0; u < DG_OWN; ++u) {  // every operand read first
+        li[u] = mkdd(d.Lh[ei[u]][j], d.Ll[ei[u]][j]);
+        const int cc = max(ec[u], 0);
+        lc[u] = mkdd(d.Lh[cc][j], d.Ll[cc][j]);
+      }
+#pragma unroll
+      for (int u = 0; u < DG_OWN; ++u) {
+        if (ec[u] >= j + 2) {  // columns beyond j+1 take column j
+          sv[u] = dd_add(sv[u], dd_mul(li[u], lc[u]));
+          if (ec[u] == j + 2) {  // complete but for column j+1, which wavefront 0 adds
+            Hh[ec[u] & 1][ei[u]] = sv[u].hi;
+            Hl[ec[u] & 1][ei[u]] = sv[u].lo;
+          }
+        }
       }
     }
     __syncthreads();
   }
+  if (*s_fail) return false;
   for (int idx = tid; idx < jb * jb; idx += NT) {
     int i = idx / jb, c = idx % jb;
     if (c <= i) {
@@ -473,7 +525,7 @@ __global__ __launch_bounds__(NT) void k_batch_chol_solve(const BatchDesc* __rest
 // map a workgroup to its (instance, piece).  Each element receives exactly the operations
 // of schedule 1, in the same order.  Step 0 reads S and writes L, and copies S's upper
 // triangle (the factor leaves it as it is), each piece the mirror of what it factors.
-constexpr int TRSM_ROWS = 32;  // rows per k_bp_trsm workgroup (four per wavefront, interleaved)
+constexpr int TRSM_ROWS = 128;  // rows per k_bp_trsm workgroup, one thread each
 
 __device__ __forceinline__ int piece_owner(const long long* pre, int A, long long g) {  // pre[j] <= g < pre[j+1]
   int lo = 0, hi = A;
@@ -517,40 +569,97 @@ __global__ __launch_bounds__(NT) void k_bp_diag(const BatchDesc* __restrict__ de
   bt_diag(Srch, Srcl, lh + mo, ll + mo, n, jb0, jb, d, s_inv, &s_fail, status, inst);
 }
 
+// trsm_right_lt (linalg.rs:224-235) of TRSM_ROWS rows below the step's diagonal block, one
+// thread per row, the block's lower triangle (packed) and the pivots' reciprocals in LDS,
+// read at uniform addresses.  Entry j of a row is t_j = a_j - sum_{k<j} b_k l_jk (k
+// ascending), b_j = t_j * recip(l_jj), as in bt_trsm_rows (same operations and order, so
+// bit-identical).  Left-looking in four phases of 16 columns: phase P holds its 16 entries in
+// registers, subtracts the earlier phases' finished values b_k (k < 16P, read back from
+// the row it stored) in k order, then finishes its own 16 columns as a fully unrolled
+// triangle.  Exactly the 2016 useful multiply-subtracts per row are issued, and a row
+// holds 16 entries in registers instead of 64.
 template <bool FIRST>
-__global__ __launch_bounds__(NT) void k_bp_trsm(const BatchDesc* __restrict__ desc, const int* __restrict__ order,
-                                                const int* __restrict__ status, const long long* __restrict__ pre,
-                                                int active, const double* __restrict__ sh, const double* __restrict__ sl,
-                                                double* __restrict__ lh, double* __restrict__ ll, long jb0) {
-  extern __shared__ __align__(16) unsigned char smem_raw[];
-  DiagLds& d = *reinterpret_cast<DiagLds*>(smem_raw);
+__global__ __launch_bounds__(TRSM_ROWS) void k_bp_trsm(const BatchDesc* __restrict__ desc, const int* __restrict__ order,
+                                                       const int* __restrict__ status, const long long* __restrict__ pre,
+                                                       int active, const double* __restrict__ sh,
+                                                       const double* __restrict__ sl, double* lh, double* ll, long jb0) {
+  __shared__ double2 Lp[NB * (NB + 1) / 2];  // (j, k), k <= j, at j (j + 1) / 2 + k
+  __shared__ double2 Inv[NB];
   const int j = piece_owner(pre, active, blockIdx.x);
   const int inst = order[j];
   if (status[inst]) return;
   const long n = desc[inst].n, mo = desc[inst].mat_off;
-  // rows below a full block p exist, so jb = 64
-  for (int idx = threadIdx.x; idx < NB * NB; idx += NT) {
-    const int i = idx / NB, c = idx % NB;
-    if (c <= i) {
-      const long o = mo + (jb0 + i) * n + jb0 + c;
-      d.Lh[i][c] = lh[o];
-      d.Ll[i][c] = ll[o];
+  const int tid = threadIdx.x;
+  // rows below a full block p exist, so the block is 64 wide
+  for (int r = 0; r < NB; ++r)
+    if (tid <= r) {
+      const long o = mo + (jb0 + r) * n + jb0 + tid;
+      Lp[r * (r + 1) / 2 + tid] = make_double2(lh[o], ll[o]);
     }
+  __syncthreads();
+  if (tid < NB) {
+    const double2 v = Lp[tid * (tid + 1) / 2 + tid];
+    const dd r = dd_recip(mkdd(v.x, v.y));
+    Inv[tid] = make_double2(r.hi, r.lo);
   }
   __syncthreads();
-  const long row0 = jb0 + NB + (long)(blockIdx.x - pre[j]) * TRSM_ROWS;
-  const long row1 = min(n, row0 + TRSM_ROWS);
+  const long row = jb0 + NB + (long)(blockIdx.x - pre[j]) * TRSM_ROWS + tid;
+  if (row >= n) return;
   const double* Srch = (FIRST ? sh : lh) + mo;
   const double* Srcl = (FIRST ? sl : ll) + mo;
-  bt_trsm_rows(Srch, Srcl, lh + mo, ll + mo, n, jb0, NB, d, row0 + (threadIdx.x >> 6), row1, NT / 64);
-  if (FIRST) {  // the mirror of these rows' panel entries: S[jb0 + k][row], k < 64
-    const int lane = threadIdx.x & 63;
-    for (long row = row0 + (threadIdx.x >> 6); row < row1; row += NT / 64) {
-      const long o = mo + (jb0 + lane) * n + row;
+  double* Rh = lh + mo + row * n + jb0;  // this row's entries of the panel (the output)
+  double* Rl = ll + mo + row * n + jb0;
+  constexpr int W = 16;
+#pragma unroll 1
+  for (int P = 0; P < NB / W; ++P) {
+    dd t[W];
+#pragma unroll
+    for (int q = 0; q < W; ++q) t[q] = ldm(Srch, Srcl, row * n + jb0 + W * P + q);
+    // the earlier phases' finished values, k ascending, eight at a time
+#pragma unroll 1
+    for (int k0 = 0; k0 < W * P; k0 += 8) {
+      dd bk[8];
+#pragma unroll
+      for (int u = 0; u < 8; ++u) bk[u] = mkdd(Rh[k0 + u], Rl[k0 + u]);
+#pragma unroll
+      for (int u = 0; u < 8; ++u) {
+#pragma unroll
+        for (int q = 0; q < W; ++q) {
+          const int c = W * P + q;
+          const double2 l = Lp[c * (c + 1) / 2 + k0 + u];
+          t[q] = dd_sub(t[q], dd_mul(bk[u], mkdd(l.x, l.y)));
+        }
+        __builtin_amdgcn_sched_barrier(0);  // keeps the LDS operands of one k live at a time
+      }
+    }
+    // the phase's own triangle
+#pragma unroll
+    for (int kk = 0; kk < W; ++kk) {
+      const int k = W * P + kk;
+      const double2 iv = Inv[k];
+      const dd b = dd_mul(t[kk], mkdd(iv.x, iv.y));
+      t[kk] = b;
+#pragma unroll
+      for (int q = kk + 1; q < W; ++q) {
+        const int c = W * P + q;
+        const double2 l = Lp[c * (c + 1) / 2 + k];
+        t[q] = dd_sub(t[q], dd_mul(b, mkdd(l.x, l.y)));
+      }
+      __builtin_amdgcn_sched_barrier(0);
+    }
+#pragma unroll
+    for (int q = 0; q < W; ++q) {
+      Rh[W * P + q] = t[q].hi;
+      Rl[W * P + q] = t[q].lo;
+    }
+    __builtin_amdgcn_sched_barrier(0);  // the next phase's loads stay in the next phase
+  }
+  if (FIRST)  // the mirror of this row's panel entries: S[jb0 + k][row], k < 64
+    for (int k = 0; k < NB; ++k) {
+      const long o = mo + (jb0 + k) * n + row;
       lh[o] = sh[o];
       ll[o] = sl[o];
     }
-  }
 }
 
 // PART 1: the tiles of the next panel's column block (tc = 0); PART 2: the others
@@ -864,17 +973,13 @@ int batch_prepare(hp_batch* b, const int* sizes, size_t count, int nrhs) {
   static std::once_flag attr_once;
   static hipError_t attr_err = hipSuccess;
   std::call_once(attr_once, [] {
-    hipError_t e[9] = {
+    hipError_t e[7] = {
         hipFuncSetAttribute((const void*)k_batch_chol_solve, hipFuncAttributeMaxDynamicSharedMemorySize,
                             (int)sizeof(BatchLds)),
         hipFuncSetAttribute((const void*)k_bp_diag<true>, hipFuncAttributeMaxDynamicSharedMemorySize,
                             (int)sizeof(DiagLds)),
         hipFuncSetAttribute((const void*)k_bp_diag<false>, hipFuncAttributeMaxDynamicSharedMemorySize,
                             (int)sizeof(DiagLds)),
-        hipFuncSetAttribute((const void*)k_bp_trsm<true>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                            (int)offsetof(DiagLds, Sh)),
-        hipFuncSetAttribute((const void*)k_bp_trsm<false>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                            (int)offsetof(DiagLds, Sh)),
         hipFuncSetAttribute((const void*)k_bs_diag<true>, hipFuncAttributeMaxDynamicSharedMemorySize,
                             (int)offsetof(DiagLds, Sh)),
         hipFuncSetAttribute((const void*)k_bs_diag<false>, hipFuncAttributeMaxDynamicSharedMemorySize,
@@ -931,21 +1036,20 @@ int batch_enqueue(hp_batch* b, int schedule, const double* s_hi, const double* s
     const int A = b->active[p];
     hipEvent_t ev_trsm = b->ev[2 * p], ev_rest = b->ev[2 * p + 1];
     if (first)
-      hipLaunchKernelGGL(k_bp_diag<true>, dim3(A), dim3(NT), sizeof(DiagLds), s, desc, order, status, s_hi, s_lo, l_hi,
-                         l_lo, jb0);
-    else
-      hipLaunchKernelGGL(k_bp_diag<false>, dim3(A), dim3(NT), sizeof(DiagLds), s, desc, order, status, s_hi, s_lo,
+      hipLaunchKernelGGL(k_bp_diag<true>, dim3(A), dim3(NT), offsetof(DiagLds, Sh), s, desc, order, status, s_hi, s_lo,
                          l_hi, l_lo, jb0);
+    else
+      hipLaunchKernelGGL(k_bp_diag<false>, dim3(A), dim3(NT), offsetof(DiagLds, Sh), s, desc, order, status, s_hi,
+                         s_lo, l_hi, l_lo, jb0);
     HP_LAUNCH_CHECK();
     if (b->trsm_tot[p] > 0) {
       const unsigned g = (unsigned)b->trsm_tot[p];
-      const size_t lds = offsetof(DiagLds, Sh);
       if (first)
-        hipLaunchKernelGGL(k_bp_trsm<true>, dim3(g), dim3(NT), lds, s, desc, order, status, pre + b->trsm_off[p], A,
-                           s_hi, s_lo, l_hi, l_lo, jb0);
+        hipLaunchKernelGGL(k_bp_trsm<true>, dim3(g), dim3(TRSM_ROWS), 0, s, desc, order, status, pre + b->trsm_off[p],
+                           A, s_hi, s_lo, l_hi, l_lo, jb0);
       else
-        hipLaunchKernelGGL(k_bp_trsm<false>, dim3(g), dim3(NT), lds, s, desc, order, status, pre + b->trsm_off[p], A,
-                           s_hi, s_lo, l_hi, l_lo, jb0);
+        hipLaunchKernelGGL(k_bp_trsm<false>, dim3(g), dim3(TRSM_ROWS), 0, s, desc, order, status, pre + b->trsm_off[p],
+                           A, s_hi, s_lo, l_hi, l_lo, jb0);
       HP_LAUNCH_CHECK();
     }
     HP_HIP(hipEventRecord(ev_trsm, s));

@@ -169,93 +169,170 @@ __global__ __launch_bounds__(512) void k_diag_factor(Mat a, Mat S, long r0, int 
 
 // ---------------------------------------------------------------- diagonal block, split-dd
 // The split-dd factorisation's diagonal block (reference order not required).  a[r0:r0+jb,
-// r0:r0+jb] (jb <= 64) is reduced in place in LDS, column by column: every element takes
-// one dd multiply and one dd subtraction per column, T(i, c) -= L(i, j) L(c, j), and column
-// c is finished as L(i, c) = T(i, c) / L(c, c) with the pivot's root and reciprocal from a
-// dd reciprocal square root.  Per column the chain is one multiply-subtract, the
-// reciprocal square root, one broadcast and one multiply.  Wavefront 0 finishes column
-// j+1 while the other seven apply column j to the columns beyond j+1 (through the packed
-// triangle index table).  One 64x65 dd tile (66.5 KB).  On a breakdown the block is left
-// as it was and err records the pivot (linalg.rs:54-57).
-__global__ __launch_bounds__(512) void k_diag_factor_split(Mat a, long r0, int jb, ErrRec* err, double* inv_hi,
-                                                           double* inv_lo) {
+// r0:r0+jb] (jb <= 64) is reduced column by column: every element takes one compensated
+// multiply-subtract per column, T(i, c) -= L(i, j) L(c, j), and column c is finished as
+// L(i, c) = T(i, c) * inv(L(c, c)) with the pivot's reciprocal square root from
+// rsqrt_dd_fast.  Wavefront 0 (raised wave priority) finishes column j+1 while the seven
+// trailing wavefronts apply column j to the columns beyond j+1.  Each trailing thread
+// keeps its elements of those columns in registers for the whole block: element e of the
+// packed triangle (e = q(q+1)/2 + p, 0 <= p <= q, is (jb-1-p, jb-1-q)) belongs to thread
+// e mod 448, so the columns still active after column j (a prefix of the packing) stay
+// spread evenly over the threads.  When an element's column is next but one, its owner
+// hands it to wavefront 0 through a two-column LDS buffer.  Per column the chain is one
+// multiply-subtract, the reciprocal square root, one broadcast and one multiply; the
+// trailing work is ~12 FP64 operations per element and column, operands read from the
+// finished columns of L in LDS.  On a breakdown the block is left as it was and err
+// records the pivot (linalg.rs:54-57).
+struct DDacc {  // h + l, not normalised
+  double h, l;
+};
+// a -= x y: the product's leading part by two_sum into a.h, its FMA error and the two cross
+// terms into a.l
+__device__ __forceinline__ void dd_fms_acc(DDacc& a, dd x, dd y) {
+  const double p = x.hi * y.hi;
+  double e = __builtin_fma(x.hi, y.hi, -p);
+  e = __builtin_fma(x.hi, y.lo, e);
+  e = __builtin_fma(x.lo, y.hi, e);
+  double s, t;
+  two_sum(a.h, -p, s, t);
+  a.h = s;
+  a.l = a.l + (t - e);
+}
+__device__ __forceinline__ dd dd_of(DDacc a) {
+  dd r;
+  two_sum(a.h, a.l, r.hi, r.lo);
+  return r;
+}
+// inv = 1/sqrt(d) and root = sqrt(d) for d > 0, relative error below 2^-103 (2^-103.2 and
+// 2^-102.9 worst over 2e6 samples of every binade, subnormals included, against binary128):
+// d is scaled by 4^-k into [1, 4) (exact), y = v_rsq_f64 refined by two binary64 Newton
+// steps (each at least doubles the correct bits, so a seed better than 2^-14 reaches
+// binary64 rounding), then y (1 + e/2 + 3e^2/8) with e = 1 - d y^2 formed exactly enough:
+// y^2 = p + pe by TwoProd, 1 - d.hi p by one FMA (|result| ~ 2^-52, rounding ~2^-105), the
+// small terms d.hi pe and d.lo p subtracted by FMA.
+__device__ __forceinline__ void rsqrt_dd_fast(dd d, dd& root, dd& inv) {
+  const int k = (__builtin_amdgcn_frexp_exp(d.hi) - 1) >> 1;  // d.hi = m 2^(2k + {1, 2}) ...
+  const dd s = mkdd(__builtin_ldexp(d.hi, -2 * k), __builtin_ldexp(d.lo, -2 * k));  // s.hi in [1, 4)
+  double y = __builtin_amdgcn_rsq(s.hi);
+  y = y * __builtin_fma(-(0.5 * s.hi) * y, y, 1.5);
+  y = y * __builtin_fma(-(0.5 * s.hi) * y, y, 1.5);
+  const double p = y * y;
+  const double pe = __builtin_fma(y, y, -p);
+  double e = __builtin_fma(-s.hi, p, 1.0);
+  e = __builtin_fma(-s.hi, pe, e);
+  e = __builtin_fma(-s.lo, p, e);
+  dd r;
+  quick_two_sum(y, y * (e * __builtin_fma(0.375, e, 0.5)), r.hi, r.lo);
+  inv = mkdd(__builtin_ldexp(r.hi, -k), __builtin_ldexp(r.lo, -k));
+  root = dd_mul(d, inv);
+}
+
+constexpr int DF_THREADS = 1024;
+constexpr int DF_NTR = DF_THREADS - 64;  // trailing threads
+constexpr int DF_OWN = 3;                // elements per trailing thread: 3 * 960 >= 62 * 63 / 2
+__device__ __forceinline__ dd ld2(const double2* p) {
+  const double2 v = *p;
+  return mkdd(v.x, v.y);
+}
+__global__ __launch_bounds__(DF_THREADS) void k_diag_factor_split(Mat a, long r0, int jb, ErrRec* err, double* inv_hi,
+                                                                  double* inv_lo) {
+  // the block, then the finished columns of L, as (hi, lo) pairs: one 16-byte LDS access per value
   extern __shared__ __align__(16) unsigned char smem[];
-  Lds2<ArDD>& T = *reinterpret_cast<Lds2<ArDD>*>(smem);
+  double2(*L)[DB + 1] = reinterpret_cast<double2(*)[DB + 1]>(smem);
+  __shared__ double2 H[2][DB];  // column hand-off to wavefront 0, by column parity
   __shared__ int fail;
   __shared__ double invs_h[DB], invs_l[DB];
-  // packed triangle: entry e = q(q+1)/2 + p, 0 <= p <= q, is the element (jb-1-p, jb-1-q);
-  // the columns beyond j+1 form the prefix q <= jb-3-j
-  __shared__ unsigned short tri[DB * (DB - 1) / 2];
   const int tid = threadIdx.x, lane = tid & 63;
   if (err->flag) return;
   for (int idx = tid; idx < jb * jb; idx += blockDim.x) {
     int i = idx / jb, c = idx % jb;
-    if (c <= i) lset<ArDD>(T, i, c, ArDD::ld(a, r0 + i, r0 + c));
-  }
-  for (int e = tid; e < DB * (DB - 1) / 2; e += blockDim.x) {
-    int q = (int)((__builtin_sqrtf(8.0f * e + 1.0f) - 1.0f) * 0.5f);
-    while (q * (q + 1) / 2 > e) --q;
-    while ((q + 1) * (q + 2) / 2 <= e) ++q;
-    tri[e] = (unsigned short)((e - q * (q + 1) / 2) | (q << 8));
+    if (c <= i) L[i][c] = make_double2(a.hi[(r0 + i) * a.ld + r0 + c], a.lo[(r0 + i) * a.ld + r0 + c]);
   }
   if (tid == 0) fail = 0;
   __syncthreads();
-  // wavefront 0: finish column c from t = T(lane, c) after every earlier column
+  // trailing threads: their elements of columns 2 .. jb-1 into registers
+  const int cnt_all = (jb - 2) * (jb - 1) / 2;
+  DDacc sv[DF_OWN];
+  int ei[DF_OWN], ec[DF_OWN];
+  dd t0 = mkdd(0.0, 0.0), t1 = mkdd(0.0, 0.0);
+  if (tid >= 64) {
+#pragma unroll
+    for (int u = 0; u < DF_OWN; ++u) {
+      const int e = tid - 64 + u * DF_NTR;
+      int q = (int)((__builtin_sqrtf(8.0f * (float)e + 1.0f) - 1.0f) * 0.5f);
+      while (q * (q + 1) / 2 > e) --q;
+      while ((q + 1) * (q + 2) / 2 <= e) ++q;
+      ei[u] = jb - 1 - (e - q * (q + 1) / 2);
+      ec[u] = jb - 1 - q;
+      sv[u] = DDacc{0.0, 0.0};
+      if (e < cnt_all) {
+        const dd v = ld2(&L[ei[u]][ec[u]]);
+        sv[u] = DDacc{v.hi, v.lo};
+      } else {
+        ei[u] = 0;
+        ec[u] = -1;  // no element
+      }
+    }
+  } else {
+    __builtin_amdgcn_s_setprio(3);
+    if (lane < jb) t0 = ld2(&L[lane][0]);
+    if (lane < jb && jb > 1) t1 = ld2(&L[lane][1]);
+  }
+  // wavefront 0: finish column c from t = T(lane, c) after every earlier column.  Lane c
+  // takes the reciprocal square root; one multiply by it then gives the root on lane c and
+  // the column below it on the other lanes.
   dd lcol = mkdd(0.0, 0.0);
   auto finish_column = [&](int c, dd t) {
     dd inv = mkdd(0.0, 0.0);
-    int bad = 0;
     if (lane == c) {
       if (ArDD::nonpos(t)) {
         err->flag = 1;
         err->pivot = r0 + c;
         err->value = ArDD::f64(t);
         fail = 1;
-        bad = 1;
       } else {
-        dd ljj;
-        ArDD::sqrt_recip_fast(t, ljj, inv);
-        lset<ArDD>(T, c, c, ljj);
+        dd root;
+        rsqrt_dd_fast(t, root, inv);
         invs_h[c] = inv.hi;
         invs_l[c] = inv.lo;
       }
     }
-    if (__builtin_amdgcn_readlane(bad, c)) return;
     inv = rl(inv, c);
-    if (lane > c && lane < jb) {
-      lcol = dd_mul(t, inv);
-      lset<ArDD>(T, lane, c, lcol);
-    }
+    const dd v = dd_mul(t, inv);
+    if (lane >= c && lane < jb) L[lane][c] = make_double2(v.hi, v.lo);
+    if (lane > c) lcol = v;
   };
-  if (tid < 64) finish_column(0, lane < jb ? lget<ArDD>(T, lane, 0) : mkdd(0.0, 0.0));
+  if (tid < 64) finish_column(0, t0);
   __syncthreads();
-  constexpr int NT = 512 - 64;  // threads of the trailing wavefronts
   for (int j = 0; j + 1 < jb; ++j) {
     if (fail) return;
     if (tid < 64) {
       const int c = j + 1;
-      dd t = mkdd(0.0, 0.0);
       const dd ljc = rl(lcol, c);  // L(j+1, j)
-      if (lane >= c && lane < jb) t = dd_sub(lget<ArDD>(T, lane, c), dd_mul(lcol, ljc));
+      dd t = mkdd(0.0, 0.0);
+      if (lane >= c && lane < jb) {
+        const dd tin = c == 1 ? t1 : ld2(&H[c & 1][lane]);
+        DDacc acc{tin.hi, tin.lo};
+        dd_fms_acc(acc, lcol, ljc);
+        t = dd_of(acc);
+      }
       finish_column(c, t);
     } else {
-      const int m = jb - j - 2;  // columns j+2 .. jb-1
-      const int cnt = m * (m + 1) / 2;
-      for (int e0 = tid - 64; e0 < cnt; e0 += 2 * NT) {
-        dd sv[2], li[2], lc[2];
-        int ii[2], cc[2];
+      dd li[DF_OWN], lc[DF_OWN];
 #pragma unroll
-        for (int u = 0; u < 2; ++u) {  // loads of both elements first (index clamped in range)
-          const unsigned short pq = tri[min(e0 + u * NT, cnt - 1)];
-          ii[u] = jb - 1 - (pq & 255);
-          cc[u] = jb - 1 - (pq >> 8);
-          sv[u] = lget<ArDD>(T, ii[u], cc[u]);
-          li[u] = lget<ArDD>(T, ii[u], j);
-          lc[u] = lget<ArDD>(T, cc[u], j);
+      for (int u = 0; u < DF_OWN; ++u) {  // every operand read first
+        li[u] = ld2(&L[ei[u]][j]);
+        lc[u] = ld2(&L[max(ec[u], 0)][j]);
+      }
+#pragma unroll
+      for (int u = 0; u < DF_OWN; ++u) {
+        if (ec[u] >= j + 2) {  // columns beyond j+1 take column j
+          dd_fms_acc(sv[u], li[u], lc[u]);
+          if (ec[u] == j + 2) {  // final but for column j+1, which wavefront 0 applies
+            const dd v = dd_of(sv[u]);
+            H[ec[u] & 1][ei[u]] = make_double2(v.hi, v.lo);
+          }
         }
-#pragma unroll
-        for (int u = 0; u < 2; ++u)
-          if (e0 + u * NT < cnt) lset<ArDD>(T, ii[u], cc[u], dd_sub(sv[u], dd_mul(li[u], lc[u])));
       }
     }
     __syncthreads();
@@ -263,7 +340,11 @@ __global__ __launch_bounds__(512) void k_diag_factor_split(Mat a, long r0, int j
   if (fail) return;
   for (int idx = tid; idx < jb * jb; idx += blockDim.x) {
     int i = idx / jb, c = idx % jb;
-    if (c <= i) ArDD::st(a, r0 + i, r0 + c, lget<ArDD>(T, i, c));
+    if (c <= i) {
+      const double2 v = L[i][c];
+      a.hi[(r0 + i) * a.ld + r0 + c] = v.x;
+      a.lo[(r0 + i) * a.ld + r0 + c] = v.y;
+    }
   }
   if (tid < jb) {
     inv_hi[tid] = invs_h[tid];
@@ -689,7 +770,7 @@ int chol_panel_split(Mat a, Mat S, long c0, long jb, long n, ErrRec* err, double
     const int q = (int)(q0 / DB);
     double* ih = inv + (size_t)q * 2 * DB;
     double* il = ih + DB;
-    hipLaunchKernelGGL(k_diag_factor_split, dim3(1), dim3(512), sizeof(Lds2<A>), s, a, c0 + q0, w, err, ih, il);
+    hipLaunchKernelGGL(k_diag_factor_split, dim3(1), dim3(DF_THREADS), sizeof(Lds2<A>), s, a, c0 + q0, w, err, ih, il);
     HP_LAUNCH_CHECK();
     HP_HIP(hipEventRecord(evq[q], s));
     const long rb = c0 + q0 + w, re = c0 + jb;

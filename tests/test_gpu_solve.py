@@ -129,3 +129,28 @@ def test_split_factor_panel_widths(qp, nb, n):
         qp.set_split_factor_panel(256)
     assert rel_err(x2, x1) < 1e-22, rel_err(x2, x1)
     assert rel_err(x3, x4) < 1e-30, rel_err(x3, x4)
+
+
+def test_split_factor_pivots_across_binades(qp):
+    """The split factor's pivots (the scaled dd reciprocal square root of its diagonal-block
+    kernel) over d spanning 2^-600 .. 2^600 with random dd low parts.  S = diag(d): one
+    correction from x = 0 (max_iter=1) is x = L^-T L^-1 b with L = diag(sqrt(d)), so its
+    error is the pivots' (plus the block inverses' dd divisions).  The bar, 2^-99 per
+    component against the oracle's dd 1/d, would fail for a pivot accurate only to binary64
+    or to half of dd."""
+    rng = np.random.default_rng(11)
+    n = 300
+    hi = np.ldexp(1.0 + rng.random(n), rng.integers(-600, 600, n))
+    lo = hi * rng.uniform(-1.0, 1.0, n) * 2.0 ** -53
+    s_ = hi + lo
+    d = np.stack([s_, lo - (s_ - hi)], axis=1)
+    smat = np.zeros((n * n, 2))
+    smat[np.arange(n) * (n + 1)] = d
+    b = np.zeros((n, 2))
+    b[:, 0] = 1.0
+    x, st = qp.solve_spd_refined(smat, b, n, qp.HP_FACTOR_DD_SPLIT, tol=0.0, max_iter=1)
+    ref = O.dd_binop("div", b, d)
+    diff = O.dd_binop("sub", x, ref)
+    err = np.abs(diff[:, 0] + diff[:, 1]) / np.abs(ref[:, 0])
+    print(f"worst relative error {err.max():.3e} (2^{np.log2(max(err.max(), 1e-300)):.1f})")
+    assert err.max() <= 2.0 ** -99, err.max()
