@@ -77,11 +77,12 @@ int hp_mf_gemm_atb(const double* a, const double* b, double* c, size_t m, size_t
 /* Device dd GEMM on SoA planes.  trans_a = 0: A is m x k (gemm); 1: A is k x m (gemm_atb).
  *   HP_GEMM_PLAIN     the reference's per-element order on FP64 VALU (bit-identical);
  *   HP_GEMM_SPLIT_I8  split precision on INT8 matrix cores: rows of A / columns of B are
- *                     scaled to W-bit integers (W ~ 108 bits for `slices` = 31), the
+ *                     scaled to W-bit integers (W = 105 bits for `slices` = 30 at k = 4096), the
  *                     integer product is formed exactly from `slices` int8 GEMMs modulo
  *                     coprime moduli <= 256 and rebuilt by the Chinese remainder theorem.
  *                     Error <= 2^(1-W) (max_p|A_ip| sum_p|B_pj| + sum_p|A_ip| max_p|B_pj|)
- *                     + 2^-106 |C_ij|.  slices in [8, 32]; 0 selects 31.
+ *                     + 2^-106 |C_ij|.  slices in [8, 32]; 0 selects the fewest moduli
+ *                     reaching dd accuracy (W >= 105) at this k (30 at k = 4096).
  *                     The bound is normwise per row of A and column of B, NOT
  *                     componentwise: an entry far below that norm (graded or cancelling
  *                     data) keeps only ~W bits of the norm, where the plain kernel keeps

@@ -591,7 +591,7 @@ int chol_split(Mat a, long n, long nb, Mat S, ErrRec* err, double*, double*, hip
       o.slot = 1;
       o.variant = 4064;
       if ((rc = split_gemm_general(P, Q, a.hi + r1 * a.ld + t0, a.lo + r1 * a.ld + t0, (size_t)a.ld,
-                                   (size_t)(rem - w2), (size_t)w2, (size_t)jb, 31, s3, o)))
+                                   (size_t)(rem - w2), (size_t)w2, (size_t)jb, 0, s3, o)))
         return rc;
     }
     HP_HIP(hipEventRecord(evA, s));
@@ -604,7 +604,7 @@ int chol_split(Mat a, long n, long nb, Mat S, ErrRec* err, double*, double*, hip
       SplitOpts opt;
       opt.lower_only = opt.subtract = opt.same_operand = true;
       if ((rc = split_gemm_general(P, P, a.hi + r0 * a.ld + r0, a.lo + r0 * a.ld + r0, (size_t)a.ld, (size_t)m,
-                                   (size_t)m, (size_t)jb, 31, s2, opt)))
+                                   (size_t)m, (size_t)jb, 0, s2, opt)))
         return rc;
       HP_HIP(hipEventRecord(evB, s2));
       evB_live = true;

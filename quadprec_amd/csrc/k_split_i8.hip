@@ -46,16 +46,20 @@ namespace hp {
 
 constexpr int CRT_MAXR = 32;
 constexpr int CRT_LIMBS = 8;  // host big integers: 256 bits as 8 x 32-bit limbs
-constexpr int CRT_L40 = 6;    // device: 240 bits as 6 x 40-bit limbs held in doubles (M < 2^239.4)
+constexpr int CRT_L40 = 6;    // device: 240 bits as 6 x 40-bit limbs held in doubles (M < 2^240)
 
 typedef int v4i_t __attribute__((ext_vector_type(4)));
 typedef int v16i_t __attribute__((ext_vector_type(16)));
 
-// The moduli: 2^8 and 31 pairwise-coprime values below it (primes and products of primes
-// that no other modulus shares), largest first: the first R of them serve R moduli.
+// The moduli, largest first (the first R of them serve R moduli): 2^8 and the next 30 values
+// of a pairwise-coprime set grown greedily from 256 down (primes and products of primes
+// that no other modulus shares: 255 = 3*5*17, 253 = 11*23, 247 = 13*19, 217 = 7*31), then 79,
+// the largest value coprime to them all that keeps the product of the 32 below 2^240 (the
+// CRT's six 40-bit limbs and 30 balanced digit bytes).  With 217, 30 moduli give W = 105 at
+// k = 4096 (dd accuracy), where the next prime, 211, left them at 104.
 static const uint32_t kModuli[CRT_MAXR] = {256, 255, 253, 251, 247, 241, 239, 233, 229, 227, 223,
-                                           211, 199, 197, 193, 191, 181, 179, 173, 167, 163, 157,
-                                           151, 149, 139, 137, 131, 127, 113, 109, 107, 103};
+                                           217, 211, 199, 197, 193, 191, 181, 179, 173, 167, 163,
+                                           157, 151, 149, 139, 137, 131, 127, 113, 109, 79};
 
 struct CrtTable {
   int R;                                   // number of moduli in use
@@ -1183,7 +1187,7 @@ int split_gemm_general(const SplitOperand& A, const SplitOperand& B, double* chi
     return HP_EINVAL;
   }
   if (m == 0 || n == 0) return HP_OK;
-  if (R <= 0) R = 31;
+  if (R <= 0) R = split_slices_for_bits(104, k);  // dd accuracy at this k
   if (R < 8 || R > CRT_MAXR) {
     set_error("split int8 gemm: moduli count %d outside [8, %d]", R, CRT_MAXR);
     return HP_EINVAL;
@@ -1241,6 +1245,10 @@ int split_gemm_general(const SplitOperand& A, const SplitOperand& B, double* chi
                                (int)gemm_i8_rs_lds()));
   }
   if (g_work.tabR != R) {
+    if (log2_M(R) >= 240.0) {
+      set_error("split int8 gemm: the product of %d moduli does not fit 240 bits", R);
+      return HP_EINVAL;
+    }
     std::unique_ptr<CrtTable> staged(new CrtTable());  // per call: slots rebuild tables independently
     CrtTable& host = *staged;
     build_table(R, host);

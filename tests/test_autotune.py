@@ -14,8 +14,8 @@ LIB = os.path.join(ROOT, "quadprec_amd", "libhyperprec_hip.so")
 
 
 def moduli(count=32):
-    return [256, 255, 253, 251, 247, 241, 239, 233, 229, 227, 223, 211, 199, 197, 193, 191, 181, 179, 173,
-            167, 163, 157, 151, 149, 139, 137, 131, 127, 113, 109, 107, 103][:count]
+    return [256, 255, 253, 251, 247, 241, 239, 233, 229, 227, 223, 217, 211, 199, 197, 193, 191, 181, 179,
+            173, 167, 163, 157, 151, 149, 139, 137, 131, 127, 113, 109, 79][:count]
 
 
 def width(R, k):
@@ -44,7 +44,22 @@ def test_fewest_moduli_for_the_requested_bits(k):
 
 
 @pytest.mark.skipif(not os.path.exists(LIB), reason="library not built")
-def test_dd_accuracy_at_4096_takes_31_moduli():
+def test_dd_accuracy_at_4096_takes_30_moduli():
     import quadprec_amd as qp
-    assert qp.split_slices_for_bits(4096, 104) == 31   # W = 108 (30 moduli give 104)
+    assert width(30, 4096) == 105 and width(29, 4096) == 102
+    assert qp.split_slices_for_bits(4096, 104) == 30   # W = 105 (29 moduli give 102)
     assert qp.split_slices_for_bits(4096, 53) == 16    # W = 54 (15 moduli give 50)
+
+
+def test_no_coprime_modulus_left_out():
+    """The first 31 moduli are a pairwise-coprime set grown greedily from 256 down (every value
+    in (109, 256] coprime to all of them is in it); the 32nd is the largest value coprime to
+    those 31 that keeps the product of all 32 below 2^240, the width of the CRT's limb and
+    digit formats."""
+    ms = moduli()
+    for v in range(110, 257):
+        if v not in ms[:31]:
+            assert any(math.gcd(v, m) > 1 for m in ms[:31]), v
+    p31 = math.prod(ms[:31])
+    fits = [v for v in range(2, 110) if all(math.gcd(v, m) == 1 for m in ms[:31]) and p31 * v < 2 ** 240]
+    assert ms[31] == max(fits) and math.prod(ms) < 2 ** 240
