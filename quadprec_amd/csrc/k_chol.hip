@@ -358,7 +358,8 @@ __global__ __launch_bounds__(DF_THREADS) void k_diag_factor_split(Mat a, long r0
 //   SUB (trsm_right_lt, linalg.rs:224-235):  t = b_ij; t = t - b_ik*l_jk (k<j); b_ij = t*inv_j
 //   ACC (unblocked, linalg.rs:78-82):        t = S_ij; t = t + l_ik*l_jk (k<j); l_ij = (a_ij - t)*inv_j
 // One wavefront per row, lane j = panel column j; the rows of a workgroup share the L tile
-// (16 for binary64; 4 for dd, whose chain needs more registers than 16 rows leave).
+// (16 for binary64; 4 for dd, so that a small set of rows, whose chains are latency-bound,
+// spreads over more CUs).
 template <class A>
 constexpr int row_recur_rows() { return A::LIMBS == 2 ? 4 : 16; }
 template <class A, bool ACC>
@@ -383,14 +384,25 @@ __global__ __launch_bounds__(64 * row_recur_rows<A>()) void k_row_recur(Mat a, M
       if (ACC) { t = A::ld(S, row, c0 + lane); av = A::ld(a, row, c0 + lane); }
       else t = A::ld(a, row, c0 + lane);
     }
+    // the factor entry of the next step is read from LDS one step ahead, and the update is
+    // taken by a select (same operations, every lane), so neither the LDS latency nor a
+    // branch sits on the step's dependent chain
+    T lk = lget<A>(L, lane, 0);
     for (int k = 0; k < w; ++k) {
+      const T lnext = lget<A>(L, lane, min(k + 1, DB - 1));
       T bk = ACC ? A::mul(A::sub(av, t), inv) : A::mul(t, inv);  // meaningful on lane k
       if (lane == k) fin = bk;
       T b = rl(bk, k);
-      if (lane > k && lane < w) {
-        T p = A::mul(b, lget<A>(L, lane, k));
-        t = ACC ? A::add(t, p) : A::sub(t, p);
+      const T p = A::mul(b, lk);
+      const T u = ACC ? A::add(t, p) : A::sub(t, p);
+      if constexpr (A::LIMBS == 2) {
+        const bool up = lane > k && lane < w;
+        t.hi = up ? u.hi : t.hi;
+        t.lo = up ? u.lo : t.lo;
+      } else {
+        t = (lane > k && lane < w) ? u : t;
       }
+      lk = lnext;
     }
     if (lane < w) A::st(a, row, c0 + lane, fin);
   }
