@@ -209,7 +209,7 @@ def bench_solves(qp, torch, dev, cpu):
                     "bound": "latency", "work_bound_ms": round(wb, 3), "frac": round(wb / best[1], 4),
                     "note": "frac = the factor's work at the FP64 VALU and INT8 peaks over factor_ms; the "
                             "factor is held by the serial chain of 64 diagonal 64x64 sub-block factorisations "
-                            "(k_diag_factor_split, ~58 us each)"}
+                            "(k_diag_factor_split, ~35-55 us each)"}
         # x* and the errors (host, after the timed runs)
         s_host = O.overlap_s_dd(c, a)
         t0 = time.perf_counter()
