@@ -167,6 +167,126 @@ __global__ __launch_bounds__(512) void k_diag_factor(Mat a, Mat S, long r0, int 
   }
 }
 
+// The same block factor with the partial sums starting from zero (cholesky_blocked's
+// diagonal block) and held in registers instead of an LDS plane: element e of the packed
+// triangle (e = q(q+1)/2 + p, 0 <= p <= q, is (jb-1-p, jb-1-q)) belongs to trailing thread
+// e mod 448 for the whole block (the columns still active form a prefix of the packing, so
+// the work stays spread evenly), and reaches wavefront 0 through a two-column LDS buffer
+// when its column is next but one.  Same operations in the same order as k_diag_factor
+// (bit-identical); one LDS tile (66.5 KB for dd) instead of two, so the block fits on a CU
+// beside the trailing updates of the other stream.
+template <class A>
+__global__ __launch_bounds__(512) void k_diag_factor_rs(Mat a, long r0, int jb, ErrRec* err, double* inv_hi,
+                                                        double* inv_lo) {
+  using T = typename A::T;
+  constexpr int OWN = 5;  // 5 * 448 >= 62 * 63 / 2
+  extern __shared__ __align__(16) unsigned char smem[];
+  Lds2<A>& L = *reinterpret_cast<Lds2<A>*>(smem);
+  __shared__ double Hh[2][DB], Hl[A::LIMBS == 2 ? 2 : 1][DB];
+  __shared__ int fail;
+  __shared__ double invs_h[DB], invs_l[DB];
+  const int tid = threadIdx.x, lane = tid & 63;
+  if (err->flag) return;
+  for (int idx = tid; idx < jb * jb; idx += blockDim.x) {
+    int i = idx / jb, c = idx % jb;
+    if (c <= i) lset<A>(L, i, c, A::ld(a, r0 + i, r0 + c));
+  }
+  if (tid == 0) fail = 0;
+  const int cnt_all = (jb - 2) * (jb - 1) / 2;
+  T sv[OWN];
+  int ei[OWN], ec[OWN];
+  if (tid >= 64) {
+#pragma unroll
+    for (int u = 0; u < OWN; ++u) {
+      const int e = tid - 64 + u * (512 - 64);
+      int q = (int)((__builtin_sqrtf(8.0f * (float)e + 1.0f) - 1.0f) * 0.5f);
+      while (q * (q + 1) / 2 > e) --q;
+      while ((q + 1) * (q + 2) / 2 <= e) ++q;
+      ei[u] = jb - 1 - (e - q * (q + 1) / 2);
+      ec[u] = jb - 1 - q;
+      if (e >= cnt_all) {
+        ei[u] = 0;
+        ec[u] = -1;  // no element
+      }
+      sv[u] = A::zero();
+    }
+  }
+  __syncthreads();
+  // wavefront 0: finish column c whose partial sums (lane i: row i) are in sc
+  T lcol = A::zero();
+  auto finish_column = [&](int c, T sc) {
+    T inv = A::zero();
+    int bad = 0;
+    if (lane == c) {
+      T diag = A::sub(lget<A>(L, c, c), sc);
+      if (A::nonpos(diag)) {
+        err->flag = 1;
+        err->pivot = r0 + c;
+        err->value = A::f64(diag);
+        fail = 1;
+        bad = 1;
+      } else {
+        const T ljj = A::sqrt(diag);
+        inv = A::recip(ljj);
+        lset<A>(L, c, c, ljj);
+        if constexpr (A::LIMBS == 2) { invs_h[c] = inv.hi; invs_l[c] = inv.lo; }
+        else invs_h[c] = inv;
+      }
+    }
+    if (__builtin_amdgcn_readlane(bad, c)) return;
+    inv = rl(inv, c);
+    if (lane > c && lane < jb) {
+      lcol = A::mul(A::sub(lget<A>(L, lane, c), sc), inv);
+      lset<A>(L, lane, c, lcol);
+    }
+  };
+  if (tid < 64) finish_column(0, A::zero());
+  __syncthreads();
+  for (int j = 0; j + 1 < jb; ++j) {
+    if (fail) return;
+    if (tid < 64) {
+      const int c = j + 1;
+      T sc = A::zero();
+      const T ljc = rl(lcol, c);  // L(j+1, j)
+      if (lane >= c && lane < jb) {
+        T sin;
+        if (c == 1) sin = A::zero();
+        else if constexpr (A::LIMBS == 2) sin = mkdd(Hh[c & 1][lane], Hl[c & 1][lane]);
+        else sin = Hh[c & 1][lane];
+        sc = A::add(sin, A::mul(lcol, ljc));
+      }
+      finish_column(c, sc);
+    } else {
+      T li[OWN], lc[OWN];
+#pragma unroll
+      for (int u = 0; u < OWN; ++u) {  // every operand read first
+        li[u] = lget<A>(L, ei[u], j);
+        lc[u] = lget<A>(L, max(ec[u], 0), j);
+      }
+#pragma unroll
+      for (int u = 0; u < OWN; ++u) {
+        if (ec[u] >= j + 2) {  // columns beyond j+1 take column j
+          sv[u] = A::add(sv[u], A::mul(li[u], lc[u]));
+          if (ec[u] == j + 2) {  // complete but for column j+1, which wavefront 0 adds
+            if constexpr (A::LIMBS == 2) { Hh[ec[u] & 1][ei[u]] = sv[u].hi; Hl[ec[u] & 1][ei[u]] = sv[u].lo; }
+            else Hh[ec[u] & 1][ei[u]] = sv[u];
+          }
+        }
+      }
+    }
+    __syncthreads();
+  }
+  if (fail) return;
+  for (int idx = tid; idx < jb * jb; idx += blockDim.x) {
+    int i = idx / jb, c = idx % jb;
+    if (c <= i) A::st(a, r0 + i, r0 + c, lget<A>(L, i, c));
+  }
+  if (tid < jb) {
+    inv_hi[tid] = invs_h[tid];
+    if constexpr (A::LIMBS == 2) inv_lo[tid] = invs_l[tid];
+  }
+}
+
 // ---------------------------------------------------------------- diagonal block, split-dd
 // The split-dd factorisation's diagonal block (reference order not required).  a[r0:r0+jb,
 // r0:r0+jb] (jb <= 64) is reduced column by column: every element takes one compensated
@@ -362,22 +482,39 @@ __global__ __launch_bounds__(DF_THREADS) void k_diag_factor_split(Mat a, long r0
 // spreads over more CUs).
 template <class A>
 constexpr int row_recur_rows() { return A::LIMBS == 2 ? 4 : 16; }
+// the block's strict lower triangle, packed: (i, k), k < i, at i (i - 1) / 2 + k
+constexpr int TRI_S = DB * (DB - 1) / 2;
+template <class A>
+struct LdsTri {
+  double h[TRI_S];
+  double l[A::LIMBS == 2 ? TRI_S : 1];
+};
+template <class A>
+__device__ __forceinline__ typename A::T tget(const LdsTri<A>& s, int e) {
+  if constexpr (A::LIMBS == 2) return mkdd(s.h[e], s.l[e]);
+  else return s.h[e];
+}
 template <class A, bool ACC>
 __global__ __launch_bounds__(64 * row_recur_rows<A>()) void k_row_recur(Mat a, Mat S, long c0, int w, long row_begin, long row_end,
                                                    const double* inv_hi, const double* inv_lo, const ErrRec* err) {
   using T = typename A::T;
   extern __shared__ __align__(16) unsigned char smem[];
-  Lds2<A>& L = *reinterpret_cast<Lds2<A>*>(smem);
+  LdsTri<A>& L = *reinterpret_cast<LdsTri<A>*>(smem);  // 31.5 KB for dd: fits beside the trailing updates
   if (err->flag) return;
   const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
   for (int idx = tid; idx < w * w; idx += blockDim.x) {
     int i = idx / w, c = idx % w;
-    if (c < i) lset<A>(L, i, c, A::ld(a, c0 + i, c0 + c));
+    if (c < i) {
+      const T v = A::ld(a, c0 + i, c0 + c);
+      if constexpr (A::LIMBS == 2) { L.h[i * (i - 1) / 2 + c] = v.hi; L.l[i * (i - 1) / 2 + c] = v.lo; }
+      else L.h[i * (i - 1) / 2 + c] = v;
+    }
   }
   __syncthreads();
   T inv = A::zero();
   if (lane < w) inv = A::ldv(inv_hi, inv_lo, lane);
   constexpr int RPB = row_recur_rows<A>();
+  const int rowoff = lane * (lane - 1) / 2;  // this lane's row of the packed triangle
   for (long row = row_begin + (long)blockIdx.x * RPB + wv; row < row_end; row += (long)gridDim.x * RPB) {
     T t = A::zero(), av = A::zero(), fin = A::zero();
     if (lane < w) {
@@ -386,10 +523,10 @@ __global__ __launch_bounds__(64 * row_recur_rows<A>()) void k_row_recur(Mat a, M
     }
     // the factor entry of the next step is read from LDS one step ahead, and the update is
     // taken by a select (same operations, every lane), so neither the LDS latency nor a
-    // branch sits on the step's dependent chain
-    T lk = lget<A>(L, lane, 0);
+    // branch sits on the step's dependent chain; lanes at or left of the pivot read entry 0
+    T lk = tget<A>(L, lane > 0 ? rowoff : 0);
     for (int k = 0; k < w; ++k) {
-      const T lnext = lget<A>(L, lane, min(k + 1, DB - 1));
+      const T lnext = tget<A>(L, k + 1 < lane ? rowoff + k + 1 : 0);
       T bk = ACC ? A::mul(A::sub(av, t), inv) : A::mul(t, inv);  // meaningful on lane k
       if (lane == k) fin = bk;
       T b = rl(bk, k);
@@ -658,7 +795,7 @@ static int set_lds_attrs() {
   const int big = (int)(2 * sizeof(Lds2<A>));
   const int one = (int)sizeof(Lds2<A>);
   HP_HIP(hipFuncSetAttribute((const void*)k_diag_factor<A, true>, hipFuncAttributeMaxDynamicSharedMemorySize, big));
-  HP_HIP(hipFuncSetAttribute((const void*)k_diag_factor<A, false>, hipFuncAttributeMaxDynamicSharedMemorySize, big));
+  HP_HIP(hipFuncSetAttribute((const void*)k_diag_factor_rs<A>, hipFuncAttributeMaxDynamicSharedMemorySize, one));
   if constexpr (A::LIMBS == 2)
     HP_HIP(hipFuncSetAttribute((const void*)k_diag_factor_split, hipFuncAttributeMaxDynamicSharedMemorySize, one));
   HP_HIP(hipFuncSetAttribute((const void*)k_row_recur<A, true>, hipFuncAttributeMaxDynamicSharedMemorySize, one));
@@ -702,8 +839,8 @@ static int row_recur(Mat a, Mat S, long c0, int w, long rb, long re, double* inv
   if (re <= rb) return HP_OK;
   constexpr int RPB = row_recur_rows<A>();
   unsigned g = (unsigned)std::min<long>((re - rb + RPB - 1) / RPB, 8192 / RPB);
-  hipLaunchKernelGGL((k_row_recur<A, ACC>), dim3(g), dim3(64 * RPB), sizeof(Lds2<A>), s, a, S, c0, w, rb, re, inv_h,
-                     inv_l, (const ErrRec*)err);
+  hipLaunchKernelGGL((k_row_recur<A, ACC>), dim3(g), dim3(64 * RPB), sizeof(LdsTri<A>), s, a, S, c0, w, rb, re,
+                     inv_h, inv_l, (const ErrRec*)err);
   HP_LAUNCH_CHECK();
   return HP_OK;
 }
@@ -739,8 +876,8 @@ int chol_panel_exact(Mat a, Mat S, long c0, long jb, long n, ErrRec* err, double
   int rc = ensure_attrs<A>();
   if (rc) return rc;
   if (jb <= DB) {
-    hipLaunchKernelGGL((k_diag_factor<A, false>), dim3(1), dim3(512), 2 * sizeof(Lds2<A>), s, a, S, c0, (int)jb,
-                       err, inv_h, inv_l);
+    hipLaunchKernelGGL((k_diag_factor_rs<A>), dim3(1), dim3(512), sizeof(Lds2<A>), s, a, c0, (int)jb, err, inv_h,
+                       inv_l);
     HP_LAUNCH_CHECK();
   } else if ((rc = factor_unblocked_region<A>(a, S, c0, jb, err, inv_h, inv_l, s))) {
     return rc;
