@@ -279,14 +279,16 @@ __device__ __forceinline__ uint32_t pack4(int a, int b, int c, int d) {
   return (uint32_t)(a & 255) | ((uint32_t)(b & 255) << 8) | ((uint32_t)(c & 255) << 16) | ((uint32_t)d << 24);
 }
 // CRT digit (v y) mod m in [0, m) of an exact int32 GEMM sum v: |v y| < 2^39, so v y is
-// exact and floor(v y / m) is off by at most one (at exact multiples), which the
-// corrections absorb.
-__device__ __forceinline__ uint8_t crt_digit(int v, double m, double inv_m, double y) {
+// exact and floor(v y / m) is off by at most one (at exact multiples), so the remainder r
+// is an exact integer in [-m, 2m); it is brought into [0, m) in 32-bit integers (t = r + m
+// in [0, 3m), then twice t = min(t, t - m) as unsigned: a wrap below zero is the larger).
+__device__ __forceinline__ uint32_t crt_digit(int v, double m, double inv_m, double y, uint32_t mi) {
   const double x = (double)v * y;
-  double r = __builtin_fma(__builtin_floor(x * inv_m), -m, x);
-  r = r < 0.0 ? r + m : r;
-  r = r >= m ? r - m : r;
-  return (uint8_t)(int)r;
+  const int r = (int)__builtin_fma(__builtin_floor(x * inv_m), -m, x);
+  uint32_t t = (uint32_t)(r + (int)mi);
+  t = min(t, t - mi);
+  t = min(t, t - mi);
+  return t;
 }
 
 // ------------------------------------------------------------------ scale exponents
@@ -470,22 +472,43 @@ __device__ __forceinline__ void reduce_acc(v16i_t (&acc)[NA][4], uint32_t m, dou
         acc[a][b][q] = v - (int32_t)m * (int32_t)__builtin_floor((double)v * inv_m);
       }
 }
-// wavefront (wm, wn) holds rows wm*32*NA + 32a and columns wn*128 + 32b of the block tile
+// wavefront (wm, wn) holds rows wm*32*NA + 32a and columns wn*128 + 32b of the block tile.
+// Register q of a lane holds row 8 (q / 4) + q % 4 of its 4-row group: the row's address is
+// uniform (scalar), the lane adds a 32-bit offset fixed for the kernel, and a tile wholly
+// inside C stores without bounds checks.
 template <int NA>
 __device__ __forceinline__ void store_digits(const v16i_t (&acc)[NA][4], uint8_t* __restrict__ C, long M, long N,
                                              long ldc, long i0, long j0, int wm, int wn, int lane, double m,
                                              double inv_m, double y) {
+  const uint32_t mi = (uint32_t)m;
+  const bool full = i0 + GM <= M && j0 + GN <= N;
+  // per lane: 32-bit offset of its column and 4-row group within a 64 x 32 sub-tile
+  const uint32_t voff = (uint32_t)(4 * (lane >> 5) * ldc + (lane & 31));
+  const int wmu = __builtin_amdgcn_readfirstlane(wm), wnu = __builtin_amdgcn_readfirstlane(wn);  // wavefront-uniform
 #pragma unroll
   for (int a = 0; a < NA; ++a)
 #pragma unroll
     for (int b = 0; b < 4; ++b) {
-      const long gj = j0 + wn * 128 + b * 32 + (lane & 31);
-      const long gib = i0 + wm * 32 * NA + a * 32 + 4 * (lane >> 5);
+      const long ri = i0 + wmu * 32 * NA + a * 32, cj = j0 + wnu * 128 + b * 32;  // uniform
+      uint8_t* sb = C + ri * ldc + cj;
+      if (full) {
+        uint32_t u[16];  // the group's digits first (independent chains interleave), then the stores
 #pragma unroll
-      for (int q = 0; q < 16; ++q) {
-        const long gi = gib + 8 * (q >> 2) + (q & 3);
-        const uint8_t u = crt_digit(acc[a][b][q], m, inv_m, y);
-        if (gi < M && gj < N) C[gi * ldc + gj] = u;
+        for (int q = 0; q < 16; ++q) u[q] = crt_digit(acc[a][b][q], m, inv_m, y, mi);
+#pragma unroll
+        for (int q = 0; q < 16; ++q) {
+          uint8_t* row = sb + (long)(8 * (q >> 2) + (q & 3)) * ldc;  // uniform
+          row[voff] = (uint8_t)u[q];
+        }
+      } else {
+        const long gj = cj + (lane & 31);
+        const long gib = ri + 4 * (lane >> 5);
+#pragma unroll
+        for (int q = 0; q < 16; ++q) {
+          const long gi = gib + 8 * (q >> 2) + (q & 3);
+          const uint8_t u = (uint8_t)crt_digit(acc[a][b][q], m, inv_m, y, mi);
+          if (gi < M && gj < N) C[gi * ldc + gj] = u;
+        }
       }
     }
 }
