@@ -720,8 +720,8 @@ __global__ __launch_bounds__(256) void k_fwd_update(Mat l, double* yh, double* y
 template <class A>
 __global__ __launch_bounds__(256) void k_bwd_exact(Mat l, double* xh, double* xl, long n) {
   using T = typename A::T;
-  __shared__ double Ph[256], Pl[A::LIMBS == 2 ? 256 : 1];
-  __shared__ double Xi[2];
+  constexpr int G = 8;  // the chain's group of products (read ahead up to one group past a chunk)
+  __shared__ double Ph[256 + G], Pl[A::LIMBS == 2 ? 256 + G : 1];
   const int tid = threadIdx.x;
   for (long i = n - 1; i >= 0; --i) {
     T s = A::zero();
@@ -734,14 +734,35 @@ __global__ __launch_bounds__(256) void k_bwd_exact(Mat l, double* xh, double* xl
         else Ph[tid] = p;
       }
       __syncthreads();
-      if (tid == 0) {
-        int cnt = (int)min((long)256, n - jc);
-        for (int q = 0; q < cnt; ++q) {
-          T p;
-          if constexpr (A::LIMBS == 2) p = mkdd(Ph[q], Pl[q]);
-          else p = Ph[q];
-          s = A::sub(s, p);
+      if (tid == 0) {  // the chain, in groups of G products: each group is read from LDS
+                       // while the previous one is subtracted (two register sets, no copies)
+        const int cnt = (int)min((long)256, n - jc);
+        auto rd = [&](int q) -> T {
+          if constexpr (A::LIMBS == 2) return mkdd(Ph[q], Pl[q]);
+          else return Ph[q];
+        };
+        T ga[G], gb[G];
+#pragma unroll
+        for (int u = 0; u < G; ++u) ga[u] = rd(u);
+        int q = 0;
+        for (; q + 2 * G <= cnt; q += 2 * G) {
+#pragma unroll
+          for (int u = 0; u < G; ++u) gb[u] = rd(q + G + u);
+          __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+          for (int u = 0; u < G; ++u) s = A::sub(s, ga[u]);
+#pragma unroll
+          for (int u = 0; u < G; ++u) ga[u] = rd(q + 2 * G + u);  // q + 2G + u < cnt + G
+          __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+          for (int u = 0; u < G; ++u) s = A::sub(s, gb[u]);
         }
+        if (q + G <= cnt) {
+#pragma unroll
+          for (int u = 0; u < G; ++u) s = A::sub(s, ga[u]);
+          q += G;
+        }
+        for (; q < cnt; ++q) s = A::sub(s, rd(q));
       }
       __syncthreads();
     }
