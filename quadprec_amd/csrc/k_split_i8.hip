@@ -346,9 +346,14 @@ __global__ __launch_bounds__(256) void k_scale_cols(const double* __restrict__ h
 // Element (row, p) of the logical rows x K operand -> out[r][p / 64][row][p % 64] (int8,
 // centred; p in [K, Kp) written as 0).
 //
-// Rows contiguous (element at base[row*ld + p]): a wavefront takes 512 p of one row, each
-// lane p = p0 + 4 lane + e and p0 + 256 + 4 lane + e (e < 4), and stores one dword of
-// four residues per modulus.
+// Rows contiguous (element at base[row*ld + p]): a block takes one 64-deep slab of 32 rows
+// (row groups by = blockIdx.y + n gridDim.y); wavefront w the rows 8w + lane/16 and
+// 8w + 4 + lane/16, lane the four p from 4 (lane % 16).  A wavefront's store of one modulus
+// is then 256 contiguous bytes (four rows of the slab), where a layout with a wavefront
+// along one row's p left four 64-byte pieces in four slabs per store and ran the kernel
+// about 20 % slower.  VEC (ld even, hi and lo 16-byte aligned): the four p of a lane come
+// as two 16-byte loads.
+template <bool VEC>
 __global__ __launch_bounds__(256) void k_residues_rows(const double* __restrict__ hi, const double* __restrict__ lo,
                                                        long rows, long K, long Kp, long ld,
                                                        const int* __restrict__ escale, int W,
@@ -357,97 +362,109 @@ __global__ __launch_bounds__(256) void k_residues_rows(const double* __restrict_
   const int R = tab->R;
   if ((int)threadIdx.x < 4 * R) cs[threadIdx.x >> 2][threadIdx.x & 3] = tab->split[threadIdx.x >> 2][threadIdx.x & 3];
   __syncthreads();
-  const int lane = threadIdx.x & 63;
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
   const size_t plane = (size_t)rows * Kp;
-  const long p0 = (long)blockIdx.x * 512 + 4 * lane;
-  for (long row = (long)blockIdx.y * 4 + (threadIdx.x >> 6); row < rows; row += (long)gridDim.y * 4) {
-    const int sc = W - max(escale[row], -1100);
+  const long slab = blockIdx.x, p = slab * 64 + 4 * (lane & 15);
+  for (long rg = blockIdx.y; rg * 32 < rows; rg += gridDim.y) {
+    const long rbase = rg * 32 + 8 * w + (lane >> 4);
     double t[8][3];
 #pragma unroll
-    for (int g = 0; g < 2; ++g)
+    for (int g = 0; g < 2; ++g) {
+      const long row = rbase + 4 * g;
+      double h[4] = {0.0, 0.0, 0.0, 0.0}, l[4] = {0.0, 0.0, 0.0, 0.0};
+      if (row < rows) {
+        const double* hr = hi + row * ld + p;
+        const double* lr = lo + row * ld + p;
+        if (VEC && p + 4 <= K) {
 #pragma unroll
-      for (int e = 0; e < 4; ++e) {
-        const long p = p0 + 256 * g + e;
-        double h = 0.0, l = 0.0;
-        if (p < K) {
-          h = hi[row * ld + p];
-          l = lo[row * ld + p];
+          for (int q = 0; q < 2; ++q) {
+            const double2 a = *reinterpret_cast<const double2*>(hr + 2 * q);
+            const double2 b = *reinterpret_cast<const double2*>(lr + 2 * q);
+            h[2 * q] = a.x; h[2 * q + 1] = a.y; l[2 * q] = b.x; l[2 * q + 1] = b.y;
+          }
+        } else {
+#pragma unroll
+          for (int e = 0; e < 4; ++e)
+            if (p + e < K) { h[e] = hr[e]; l[e] = lr[e]; }
         }
-        int_limbs(h, l, sc, t[4 * g + e][2], t[4 * g + e][1], t[4 * g + e][0]);
       }
+      const int sc = row < rows ? W - max(escale[row], -1100) : 0;
+#pragma unroll
+      for (int e = 0; e < 4; ++e) int_limbs(h[e], l[e], sc, t[4 * g + e][2], t[4 * g + e][1], t[4 * g + e][0]);
+    }
     for (int r = 0; r < R; ++r) {
       const double c1 = cs[r][0], c2 = cs[r][1], m = cs[r][2], im = cs[r][3];
 #pragma unroll
       for (int g = 0; g < 2; ++g) {
-        const long p = p0 + 256 * g;
-        if (p >= Kp) continue;  // Kp % 64 == 0: the whole dword is in the slab
+        const long row = rbase + 4 * g;
         int b[4];
 #pragma unroll
         for (int e = 0; e < 4; ++e) b[e] = residue8(t[4 * g + e][2], t[4 * g + e][1], t[4 * g + e][0], c1, c2, m, im);
-        *reinterpret_cast<uint32_t*>(out + (size_t)r * plane + (size_t)(p >> 6) * rows * 64 + row * 64 + (p & 63)) =
-            pack4(b[0], b[1], b[2], b[3]);
+        if (row < rows)
+          *reinterpret_cast<uint32_t*>(out + (size_t)r * plane + (size_t)slab * rows * 64 + row * 64 + (p & 63)) =
+              pack4(b[0], b[1], b[2], b[3]);
       }
     }
   }
 }
 
 // Rows strided (element at base[p*ld + row], the columns of a row-major B): a block takes
-// 64 rows x 64 p per tile, lane = row (coalesced dd loads), wavefront w the 16 p from 16w,
-// and walks row tiles row0 = 64 blockIdx.y + n 64 gridDim.y; the next tile's dd inputs are
-// loaded while this tile's residues are formed (the blocks of a CU would otherwise load
-// and compute in phase).  The residues of one modulus are transposed through LDS (two
-// buffers, alternating by modulus, so one barrier per modulus) and leave as 16-byte
-// stores: the tile is 4 KB contiguous in its slab.
+// one 64-deep slab of 16 rows (row groups n = blockIdx.y + i gridDim.y).  The tile comes in
+// coalesced along the rows (thread t: row t % 16, p t / 16 + 16u; 128 contiguous bytes per
+// p), goes through LDS, and is then worked and stored as in k_residues_rows: wavefront w
+// the rows 4w + lane/16, lane the four p from 4 (lane % 16), one 256-byte store per
+// wavefront and modulus.  (Forming the residues in the load layout needs a transpose of
+// the 30 residue bytes per element instead, one barrier per modulus.)
 __global__ __launch_bounds__(256) void k_residues_cols(const double* __restrict__ hi, const double* __restrict__ lo,
                                                        long rows, long K, long Kp, long ld,
                                                        const int* __restrict__ escale, int W,
                                                        const CrtTable* __restrict__ tab, int8_t* __restrict__ out) {
+  constexpr int TS = 66;  // LDS row pitch in doubles: 16-byte aligned rows
   __shared__ __align__(16) double cs[CRT_MAXR][4];
-  __shared__ __align__(16) uint8_t tb[2][64][80];
+  __shared__ __align__(16) double Th[16][TS], Tl[16][TS];
   const int R = tab->R;
   if ((int)threadIdx.x < 4 * R) cs[threadIdx.x >> 2][threadIdx.x & 3] = tab->split[threadIdx.x >> 2][threadIdx.x & 3];
-  __syncthreads();
-  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-  const int srow = threadIdx.x >> 2, sch = threadIdx.x & 3;  // store: 16 bytes of row srow
+  const int t = threadIdx.x, lane = t & 63, w = t >> 6;
   const size_t plane = (size_t)rows * Kp;
-  const long p0 = (long)blockIdx.x * 64, stride = (long)gridDim.y * 64;
-  double hv[16], lv[16];
-  auto fetch = [&](long r0) {
-    const long row = r0 + lane;
+  const long slab = blockIdx.x, p0 = slab * 64;
+  const int jl = t & 15, pl = t >> 4;                 // load layout
+  const int jr = 4 * w + (lane >> 4), pc = 4 * (lane & 15);  // work layout
+  for (long rg = blockIdx.y; rg * 16 < rows; rg += gridDim.y) {
+    const long j0 = rg * 16;
+    double hv[4], lv[4];
 #pragma unroll
-    for (int e = 0; e < 16; ++e) {
-      const long p = p0 + 16 * w + e;
-      const bool ok = row < rows && p < K;
-      hv[e] = ok ? hi[p * ld + row] : 0.0;
-      lv[e] = ok ? lo[p * ld + row] : 0.0;
+    for (int u = 0; u < 4; ++u) {
+      const long p = p0 + pl + 16 * u, j = j0 + jl;
+      const bool ok = j < rows && p < K;
+      hv[u] = ok ? hi[p * ld + j] : 0.0;
+      lv[u] = ok ? lo[p * ld + j] : 0.0;
     }
-  };
-  long row0 = (long)blockIdx.y * 64;
-  if (row0 < rows) fetch(row0);
-  for (; row0 < rows; row0 += stride) {
-    const long row = row0 + lane;
-    const int sc = row < rows ? W - max(escale[row], -1100) : 0;
-    double t[16][3];
+    __syncthreads();  // the previous tile's LDS reads are done (and cs is in place)
 #pragma unroll
-    for (int e = 0; e < 16; ++e) int_limbs(hv[e], lv[e], sc, t[e][2], t[e][1], t[e][0]);
-    if (row0 + stride < rows) fetch(row0 + stride);
-    const bool sok = row0 + srow < rows;
-    int8_t* so = out + (size_t)blockIdx.x * rows * 64 + (size_t)(row0 + srow) * 64 + 16 * sch;
+    for (int u = 0; u < 4; ++u) {
+      Th[jl][pl + 16 * u] = hv[u];
+      Tl[jl][pl + 16 * u] = lv[u];
+    }
+    __syncthreads();
+    const long row = j0 + jr;
+    const int sc = row < rows ? W - max(escale[row], -1100) : 0;
+    double tl[4][3];
+#pragma unroll
+    for (int q = 0; q < 2; ++q) {
+      const double2 a = *reinterpret_cast<const double2*>(&Th[jr][pc + 2 * q]);
+      const double2 b = *reinterpret_cast<const double2*>(&Tl[jr][pc + 2 * q]);
+      int_limbs(a.x, b.x, sc, tl[2 * q][2], tl[2 * q][1], tl[2 * q][0]);
+      int_limbs(a.y, b.y, sc, tl[2 * q + 1][2], tl[2 * q + 1][1], tl[2 * q + 1][0]);
+    }
     for (int r = 0; r < R; ++r) {
       const double c1 = cs[r][0], c2 = cs[r][1], m = cs[r][2], im = cs[r][3];
-      v4i_t pk;
+      int bq[4];
 #pragma unroll
-      for (int d = 0; d < 4; ++d) {
-        int b[4];
-#pragma unroll
-        for (int e = 0; e < 4; ++e) b[e] = residue8(t[4 * d + e][2], t[4 * d + e][1], t[4 * d + e][0], c1, c2, m, im);
-        pk[d] = (int)pack4(b[0], b[1], b[2], b[3]);
-      }
-      *reinterpret_cast<v4i_t*>(&tb[r & 1][lane][16 * w]) = pk;
-      __syncthreads();  // buffer r & 1 complete; buffer (r + 1) & 1 was read before this point
-      if (sok) *reinterpret_cast<v4i_t*>(so + (size_t)r * plane) = *reinterpret_cast<const v4i_t*>(&tb[r & 1][srow][16 * sch]);
+      for (int e = 0; e < 4; ++e) bq[e] = residue8(tl[e][2], tl[e][1], tl[e][0], c1, c2, m, im);
+      if (row < rows)
+        *reinterpret_cast<uint32_t*>(out + (size_t)r * plane + (size_t)slab * rows * 64 + row * 64 + pc) =
+            pack4(bq[0], bq[1], bq[2], bq[3]);
     }
-    __syncthreads();  // the next row tile's first modulus reuses buffer 0
   }
 }
 
@@ -1295,11 +1312,6 @@ int split_gemm_general(const SplitOperand& A, const SplitOperand& B, double* chi
   hipLaunchKernelGGL(k_fill_int, dim3((unsigned)((m + n + 255) / 256)), dim3(256), 0, s, eA, (long)(m + n), -100000);
   HP_LAUNCH_CHECK();
   auto gy = [](size_t blocks) { return (unsigned)std::min<size_t>(blocks, 65535); };
-  // k_residues_cols: about 512 blocks (two per CU at its register use), each walking row tiles
-  auto cols_gy = [&](long rows) {
-    const size_t tiles = (size_t)(rows + 63) / 64, gx = Kp / 64;
-    return (unsigned)std::max<size_t>(1, std::min<size_t>(tiles, (512 + gx - 1) / gx));
-  };
   auto scales_and_residues = [&](const SplitOperand& X, long rows, int* e, int8_t* out) -> int {
     if (X.contig)
       hipLaunchKernelGGL(k_scale_rows, dim3(gy((rows + 3) / 4)), dim3(256), 0, s, X.hi, rows, (long)k, X.ld, e);
@@ -1307,11 +1319,17 @@ int split_gemm_general(const SplitOperand& A, const SplitOperand& B, double* chi
       hipLaunchKernelGGL(k_scale_cols, dim3((unsigned)((rows + 63) / 64), (unsigned)((k + 255) / 256)), dim3(256), 0,
                          s, X.hi, rows, (long)k, X.ld, e);
     HP_LAUNCH_CHECK();
-    if (X.contig)
-      hipLaunchKernelGGL(k_residues_rows, dim3((unsigned)((Kp + 511) / 512), gy((rows + 3) / 4)), dim3(256), 0, s,
-                         X.hi, X.lo, rows, (long)k, (long)Kp, X.ld, e, W, g_work.tab, out);
-    else
-      hipLaunchKernelGGL(k_residues_cols, dim3((unsigned)(Kp / 64), cols_gy(rows)), dim3(256), 0, s, X.hi,
+    if (X.contig) {
+      const bool vec = X.ld % 2 == 0 && ((uintptr_t)X.hi | (uintptr_t)X.lo) % 16 == 0;
+      const dim3 grid((unsigned)(Kp / 64), gy((rows + 31) / 32));
+      if (vec)
+        hipLaunchKernelGGL(k_residues_rows<true>, grid, dim3(256), 0, s, X.hi, X.lo, rows, (long)k, (long)Kp, X.ld,
+                           e, W, g_work.tab, out);
+      else
+        hipLaunchKernelGGL(k_residues_rows<false>, grid, dim3(256), 0, s, X.hi, X.lo, rows, (long)k, (long)Kp, X.ld,
+                           e, W, g_work.tab, out);
+    } else
+      hipLaunchKernelGGL(k_residues_cols, dim3((unsigned)(Kp / 64), gy((rows + 15) / 16)), dim3(256), 0, s, X.hi,
                          X.lo, rows, (long)k, (long)Kp, X.ld, e, W, g_work.tab, out);
     HP_LAUNCH_CHECK();
     return HP_OK;
