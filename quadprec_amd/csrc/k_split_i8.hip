@@ -21,16 +21,17 @@
 //   * Residues: T(h 2^s) + T(l 2^s) = t2 2^80 + t1 2^40 + t0 (signed limbs, |t| < 2^41),
 //     v = t0 + t1 (2^40 mod m) + t2 (2^80 mod m) with the two coefficients centred
 //     (|v| < 2^48.2), and the centred residue is v - m floor(v/m + 1/2).
-//   * CRT: the GEMM epilogue emits the digit u_r = (C'_r y_r) mod m_r, y_r = (M/m_r)^-1
-//     mod m_r; then S = sum_r u_r (M/m_r) is accumulated in six 40-bit limbs, each sum
-//     below 32 * 255 * 2^40 < 2^53, and C' = S - round(S/M) M.
+//   * CRT: the GEMM epilogue emits the centred digit c_r = C'_r mod m_r (in [-m/2, m/2],
+//     as int8); then S = sum_r c_r T_r with T_r = (y_r M/m_r) mod M, y_r = (M/m_r)^-1 mod
+//     m_r, is accumulated in six 40-bit limbs and C' = S - round(S/M) M.
 //
 // Layouts: residue planes are [r][p / 64][row][p % 64]: K is cut into 64-deep slabs (Kp =
 // K rounded up to 64, zero-filled) and a slab holds every row's 64 bytes back to back, so
 // one K stage of a 256-row tile is 16 KB contiguous and its loads use whole cache lines
 // (with rows of pitch Kp, each stage used half of each 128-byte line); A (m x k) and B^T
-// (n x k) are loaded identically.  The digits of C are [r][i][j] bytes with the row pitch
-// Np = N rounded up to 4.
+// (n x k) are loaded identically.  The digits of C are bytes in 4-row groups, [r][i/4][j][i%4]
+// (group pitch 4 Np, Np = N rounded up to 4; M rounded up to 4 rows per plane): the four
+// rows of one column that an MFMA lane holds are one dword.
 #include <cmath>
 #include <cstdint>
 #include <cstring>
@@ -72,8 +73,8 @@ struct CrtTable {
   double m40[CRT_L40];                     // M in 40-bit limbs
   double inv_M;                            // 1 / M, rounded
   // k_crt's INT8 operand: entry [kb][rho][kk] is byte b(rho) (in [-128, 127]) of the balanced
-  // base-256 digits of M/m_r for k = 32 kb + kk = r < R, of Z for k = R, 0 above; rho -> b
-  // is crt_byte_of_row (bytes 0..29)
+  // base-256 digits of T_r = y_r M/m_r mod M, centred into [-M/2, M/2], for k = 32 kb + kk =
+  // r < R, 0 above; rho -> b is crt_byte_of_row (bytes 0..29)
   int8_t mrb[2][32][32];
 };
 
@@ -200,36 +201,33 @@ int crt_byte_of_row(int rho) {
   return q < 15 ? 5 * (2 * (q / 5) + h) + q % 5 : -1;
 }
 
-// the balanced digits of M/m_r and of Z = 128 sum_r M/m_r reduced into [-M/2, M/2]
+// the balanced digits of T_r = y_r M/m_r (below M, as y_r < m_r), centred into [-M/2, M/2]
 bool build_crt_bytes(int R, CrtTable& T) {
-  int8_t bal[CRT_MAXR + 1][30];
-  uint32_t M[CRT_LIMBS] = {1, 0, 0, 0, 0, 0, 0, 0}, Z[CRT_LIMBS] = {0, 0, 0, 0, 0, 0, 0, 0};
+  int8_t bal[CRT_MAXR][30];
+  uint32_t M[CRT_LIMBS] = {1, 0, 0, 0, 0, 0, 0, 0};
   for (int r = 0; r < R; ++r) big_mul_small(M, kModuli[r]);
   for (int r = 0; r < R; ++r) {
-    uint32_t Mr[CRT_LIMBS] = {1, 0, 0, 0, 0, 0, 0, 0};
+    uint32_t Tr[CRT_LIMBS] = {1, 0, 0, 0, 0, 0, 0, 0};
     for (int q = 0; q < R; ++q)
-      if (q != r) big_mul_small(Mr, kModuli[q]);
-    if (!balanced_bytes(Mr, false, bal[r], 30)) return false;
-    big_add(Z, Mr);
+      if (q != r) big_mul_small(Tr, kModuli[q]);
+    big_mul_small(Tr, (uint32_t)T.y[r]);
+    uint32_t twoT[CRT_LIMBS];
+    std::memcpy(twoT, Tr, sizeof(twoT));
+    big_add(twoT, Tr);
+    const bool neg = big_ge(twoT, M) && std::memcmp(twoT, M, sizeof(M)) != 0;  // T_r > M/2: use T_r - M
+    if (neg) {
+      uint32_t t[CRT_LIMBS];
+      std::memcpy(t, M, sizeof(t));
+      big_sub(t, Tr);
+      std::memcpy(Tr, t, sizeof(t));
+    }
+    if (!balanced_bytes(Tr, neg, bal[r], 30)) return false;
   }
-  big_mul_small(Z, 128);  // < 2^244
-  while (big_ge(Z, M)) big_sub(Z, M);
-  uint32_t twoZ[CRT_LIMBS];
-  std::memcpy(twoZ, Z, sizeof(twoZ));
-  big_add(twoZ, Z);
-  const bool neg = big_ge(twoZ, M) && std::memcmp(twoZ, M, sizeof(M)) != 0;  // Z > M/2: use Z - M
-  if (neg) {
-    uint32_t t[CRT_LIMBS];
-    std::memcpy(t, M, sizeof(t));
-    big_sub(t, Z);
-    std::memcpy(Z, t, sizeof(t));
-  }
-  if (!balanced_bytes(Z, neg, bal[R], 30)) return false;
   for (int kb = 0; kb < 2; ++kb)
     for (int rho = 0; rho < 32; ++rho)
       for (int kk = 0; kk < 32; ++kk) {
         const int k = 32 * kb + kk, b = crt_byte_of_row(rho);
-        T.mrb[kb][rho][kk] = (b >= 0 && k <= R) ? bal[k][b] : 0;
+        T.mrb[kb][rho][kk] = (b >= 0 && k < R) ? bal[k][b] : 0;
       }
   return true;
 }
@@ -278,17 +276,16 @@ __device__ __forceinline__ int residue8(double t2, double t1, double t0, double 
 __device__ __forceinline__ uint32_t pack4(int a, int b, int c, int d) {
   return (uint32_t)(a & 255) | ((uint32_t)(b & 255) << 8) | ((uint32_t)(c & 255) << 16) | ((uint32_t)d << 24);
 }
-// CRT digit (v y) mod m in [0, m) of an exact int32 GEMM sum v: |v y| < 2^39, so v y is
-// exact and floor(v y / m) is off by at most one (at exact multiples), so the remainder r
-// is an exact integer in [-m, 2m); it is brought into [0, m) in 32-bit integers (t = r + m
-// in [0, 3m), then twice t = min(t, t - m) as unsigned: a wrap below zero is the larger).
-__device__ __forceinline__ uint32_t crt_digit(int v, double m, double inv_m, double y, uint32_t mi) {
-  const double x = (double)v * y;
-  const int r = (int)__builtin_fma(__builtin_floor(x * inv_m), -m, x);
-  uint32_t t = (uint32_t)(r + (int)mi);
-  t = min(t, t - mi);
-  t = min(t, t - mi);
-  return t;
+// Centred CRT digit v - m rint(v/m) (in [-m/2, m/2]) of an exact int32 GEMM sum v.  The FMA
+// forms v (1/m) + 1.5 2^52 with one rounding, to an integer: 1/m is rounded by at most 2^-53
+// relatively, so v (1/m) is within 2^-28 of v/m (|v| < 2^31, m >= 79), while v/m is at least
+// 1/(2m) from a half-integer for odd m (for m = 256 it is exact and a tie gives +-128, either
+// of which is the residue as an int8).  The rounding therefore picks the nearest integer and
+// the remainder is exact.
+__device__ __forceinline__ int crt_digit(int v, double m, double inv_m) {
+  const double x = (double)v;
+  const double q = __builtin_fma(x, inv_m, 0x1.8p52) - 0x1.8p52;
+  return (int)__builtin_fma(q, -m, x);
 }
 
 // ------------------------------------------------------------------ scale exponents
@@ -490,41 +487,39 @@ __device__ __forceinline__ void reduce_acc(v16i_t (&acc)[NA][4], uint32_t m, dou
       }
 }
 // wavefront (wm, wn) holds rows wm*32*NA + 32a and columns wn*128 + 32b of the block tile.
-// Register q of a lane holds row 8 (q / 4) + q % 4 of its 4-row group: the row's address is
-// uniform (scalar), the lane adds a 32-bit offset fixed for the kernel, and a tile wholly
-// inside C stores without bounds checks.
+// Registers 4G..4G+3 of a lane hold rows 8G + 4 (lane / 32) + 0..3 of column lane % 32: one
+// dword of the 4-row-group layout, stored as such (lanes 0-31 and 32-63 each write 128
+// contiguous bytes).  The group's address is uniform (scalar) plus a 32-bit lane offset
+// fixed for the kernel, and a tile wholly inside C stores without bounds checks.
 template <int NA>
 __device__ __forceinline__ void store_digits(const v16i_t (&acc)[NA][4], uint8_t* __restrict__ C, long M, long N,
                                              long ldc, long i0, long j0, int wm, int wn, int lane, double m,
-                                             double inv_m, double y) {
-  const uint32_t mi = (uint32_t)m;
+                                             double inv_m) {
   const bool full = i0 + GM <= M && j0 + GN <= N;
-  // per lane: 32-bit offset of its column and 4-row group within a 64 x 32 sub-tile
-  const uint32_t voff = (uint32_t)(4 * (lane >> 5) * ldc + (lane & 31));
+  const uint32_t voff = (uint32_t)(4 * (lane >> 5) * ldc + 4 * (lane & 31));
   const int wmu = __builtin_amdgcn_readfirstlane(wm), wnu = __builtin_amdgcn_readfirstlane(wn);  // wavefront-uniform
 #pragma unroll
   for (int a = 0; a < NA; ++a)
 #pragma unroll
     for (int b = 0; b < 4; ++b) {
-      const long ri = i0 + wmu * 32 * NA + a * 32, cj = j0 + wnu * 128 + b * 32;  // uniform
-      uint8_t* sb = C + ri * ldc + cj;
+      const long ri = i0 + wmu * 32 * NA + a * 32, cj = j0 + wnu * 128 + b * 32;  // uniform, ri % 32 == 0
+      uint32_t wd[4];  // the digits of all four groups first (independent chains interleave)
+#pragma unroll
+      for (int G = 0; G < 4; ++G)
+        wd[G] = pack4(crt_digit(acc[a][b][4 * G], m, inv_m), crt_digit(acc[a][b][4 * G + 1], m, inv_m),
+                      crt_digit(acc[a][b][4 * G + 2], m, inv_m), crt_digit(acc[a][b][4 * G + 3], m, inv_m));
       if (full) {
-        uint32_t u[16];  // the group's digits first (independent chains interleave), then the stores
 #pragma unroll
-        for (int q = 0; q < 16; ++q) u[q] = crt_digit(acc[a][b][q], m, inv_m, y, mi);
-#pragma unroll
-        for (int q = 0; q < 16; ++q) {
-          uint8_t* row = sb + (long)(8 * (q >> 2) + (q & 3)) * ldc;  // uniform
-          row[voff] = (uint8_t)u[q];
+        for (int G = 0; G < 4; ++G) {
+          uint8_t* grp = C + ((ri >> 2) + 2 * G) * 4 * ldc + 4 * cj;  // uniform
+          *reinterpret_cast<uint32_t*>(grp + voff) = wd[G];
         }
       } else {
         const long gj = cj + (lane & 31);
-        const long gib = ri + 4 * (lane >> 5);
 #pragma unroll
-        for (int q = 0; q < 16; ++q) {
-          const long gi = gib + 8 * (q >> 2) + (q & 3);
-          const uint8_t u = (uint8_t)crt_digit(acc[a][b][q], m, inv_m, y, mi);
-          if (gi < M && gj < N) C[gi * ldc + gj] = u;
+        for (int G = 0; G < 4; ++G) {
+          const long gi = ri + 8 * G + 4 * (lane >> 5);  // first row of the lane's group
+          if (gi < M && gj < N) *reinterpret_cast<uint32_t*>(C + (gi >> 2) * 4 * ldc + 4 * gj) = wd[G];
         }
       }
     }
@@ -547,7 +542,7 @@ __global__ __launch_bounds__(512) void k_gemm_i8_mod(const int8_t* __restrict__ 
   const double inv_m = tab->inv_m[r];
   A += (size_t)r * M * K;
   B += (size_t)r * N * K;
-  C += (size_t)r * M * ldc;
+  C += (size_t)r * ((M + 3) & ~3L) * ldc;
   const int t = threadIdx.x, lane = t & 63, w = t >> 6;
   const int wm = w >> 1, wn = w & 1;  // 8 wavefronts: 4 x 2, each 64 rows x 128 columns
   const long i0 = (long)blockIdx.y * GM, j0 = (long)blockIdx.x * GN;
@@ -599,7 +594,7 @@ __global__ __launch_bounds__(512) void k_gemm_i8_mod(const int8_t* __restrict__ 
     if (((st + 1) * GK) % KCH == 0 && st + 1 < nst) reduce_acc(acc, m, inv_m);
     __syncthreads();  // the other buffer is overwritten next stage
   }
-  store_digits(acc, C, M, N, ldc, i0, j0, wm, wn, lane, (double)m, inv_m, tab->y[r]);
+  store_digits(acc, C, M, N, ldc, i0, j0, wm, wn, lane, (double)m, inv_m);
 }
 
 template <int GK>
@@ -628,7 +623,7 @@ __global__ __launch_bounds__(512) void k_gemm_i8_mod_dl(const int8_t* __restrict
   const double inv_m = tab->inv_m[r];
   A += (size_t)r * M * K;
   B += (size_t)r * N * K;
-  C += (size_t)r * M * ldc;
+  C += (size_t)r * ((M + 3) & ~3L) * ldc;
   const int t = threadIdx.x, lane = t & 63, w = t >> 6;
   const int wm = w >> 1, wn = w & 1;
   const long i0 = (long)blockIdx.y * GM, j0 = (long)blockIdx.x * GN;
@@ -696,7 +691,7 @@ __global__ __launch_bounds__(512) void k_gemm_i8_mod_dl(const int8_t* __restrict
     }
     if (((st + 1) * DL_GK) % KCH == 0 && st + 1 < nst) reduce_acc(acc, m, inv_m);
   }
-  store_digits(acc, C, M, N, ldc, i0, j0, wm, wn, lane, (double)m, inv_m, tab->y[r]);
+  store_digits(acc, C, M, N, ldc, i0, j0, wm, wn, lane, (double)m, inv_m);
 }
 
 // XCD-aware tile order.  Workgroups are dispatched to the 8 XCDs round-robin by linear id,
@@ -746,7 +741,7 @@ __device__ __forceinline__ void gemm_dlp_body(const int8_t* __restrict__ A, cons
   const double inv_m = tab->inv_m[r];
   A += (size_t)r * M * K;
   B += (size_t)r * N * K;
-  C += (size_t)r * M * ldc;
+  C += (size_t)r * ((M + 3) & ~3L) * ldc;
   const int t = threadIdx.x, lane = t & 63, w = t >> 6;
   const int wm = w >> 1, wn = w & 1;
   const long i0 = ty * GM, j0 = tx * GN;
@@ -861,7 +856,7 @@ __device__ __forceinline__ void gemm_dlp_body(const int8_t* __restrict__ A, cons
     mfmas(a1, b1);
     if (NA == 2 && ((st + 1) * DL_GK) % KCH == 0 && st + 1 < nst) reduce_acc(acc, m, inv_m);
   }
-  store_digits(acc, C, M, N, ldc, i0, j0, wm, wn, lane, (double)m, inv_m, tab->y[r]);
+  store_digits(acc, C, M, N, ldc, i0, j0, wm, wn, lane, (double)m, inv_m);
 }
 
 // Register-staged product (8 wavefronts of 64 x 128): plain global loads (about twice the
@@ -886,7 +881,7 @@ __global__ __launch_bounds__(512) void k_gemm_i8_mod_rs(const int8_t* __restrict
   const double inv_m = tab->inv_m[r];
   A += (size_t)r * M * K;
   B += (size_t)r * N * K;
-  C += (size_t)r * M * ldc;
+  C += (size_t)r * ((M + 3) & ~3L) * ldc;
   const int t = threadIdx.x, lane = t & 63, w = t >> 6;
   const int wm = w >> 1, wn = w & 1;
   const long i0 = ty * GM, j0 = tx * GN;
@@ -978,7 +973,7 @@ __global__ __launch_bounds__(512) void k_gemm_i8_mod_rs(const int8_t* __restrict
   phase(st, R1);
   phase(st + 1, R2);
   compute(st + 2);
-  store_digits(acc, C, M, N, ldc, i0, j0, wm, wn, lane, (double)m, inv_m, tab->y[r]);
+  store_digits(acc, C, M, N, ldc, i0, j0, wm, wn, lane, (double)m, inv_m);
 }
 constexpr size_t gemm_i8_rs_lds() { return 2 * DL_SLOT; }
 
@@ -1006,9 +1001,10 @@ __device__ __forceinline__ dd dd_add_d(dd a, double b) {
 }
 
 // C' = S - round(S/M) M from limbs s[0..5] (weights 2^(40q), signed integers below 2^51.2
-// in magnitude) of an S congruent to sum_r u_r (M/m_r) mod M with |S| < 26 M.  |C'| < M/16,
-// so S/M is within 1/16 of an integer, and the Horner estimate (relative error ~2^-32 of M
-// here) rounds to the right quotient.  The subtraction and the carry pass are exact and
+// in magnitude) of an S congruent to sum_r c_r T_r mod M with |S| <= 32 * 128 * M/2 = 2^11 M.
+// |C'| < M/16, so S/M is within 1/16 of an integer, and the Horner estimate (absolute error
+// below 2^-35 of M here) rounds to the right quotient; the quotient times M's limbs stays
+// below 2^51, so the limb updates are exact.  The subtraction and the carry pass are exact and
 // give the unique normalised limbs of C' (s[0..4] in [0, 2^40)), so the dd value of C' 2^sh
 // that follows depends on C' only; every step is exact up to the final sum of the six
 // limbs into a dd.
@@ -1042,24 +1038,23 @@ __device__ __forceinline__ dd crt_finish(double (&s)[CRT_L40], const double* mt,
   return neg ? dd_neg(res) : res;
 }
 
-// CRT on the INT8 matrix cores.  With the digit byte read as s_r = u_r - 128 (its top bit
-// flipped, as int8) and Z = 128 sum_r M/m_r reduced into [-M/2, M/2], S = sum_r s_r M/m_r + Z
-// is congruent to sum_r u_r M/m_r mod M.  In balanced base-256 digits (bytes in [-128, 127])
-// of M/m_r and Z, S = sum_b P_b 256^b with P_b = sum_k T[b][k] D[k][j]: an int8 x int8 ->
-// int32 product over k = modulus (k = R carries the constant digit 1 that picks up Z), one
-// v_mfma_i32_32x32x32_i8 per 32 elements and 32 moduli.  |P_b| <= 33 * 2^14 < 2^19.1, so the
+// CRT on the INT8 matrix cores.  The digit bytes c_r (int8) and the balanced base-256 digits
+// (bytes in [-128, 127]) of T_r give S = sum_r c_r T_r = sum_b P_b 256^b with P_b = sum_k
+// T[b][k] D[k][j]: an int8 x int8 -> int32 product over k = modulus, one
+// v_mfma_i32_32x32x32_i8 per 32 elements and 32 moduli.  |P_b| <= 32 * 2^14 = 2^19, so the
 // 40-bit limbs sum_{j<5} P_(5g+j) 2^(8j) are exact doubles below 2^51.1.
 //
-// A block takes 256 elements of a row per step.  Wavefront w loads plane quads w and w + 4
-// (and those of the second block of 32 moduli), lane l the dword of elements 4l..4l+3 of
-// each plane: 256 contiguous bytes per load, issued unconditionally (clamped addresses,
-// selected afterwards).  Each lane turns its 4 dwords into 4 dwords of 4 planes of one
-// element (byte transpose) and stores them to LDS rows of ROWD = 8 KB + 1 dwords (KB
-// blocks of 32 moduli), element e of the tile in row crt_row(e), a permutation within
-// groups of 4: the 32 lanes of every store and of every ds_read_b32 of the product's
-// column operand hit 32 distinct banks.  Product rows (crt_byte_of_row) put the limbs
-// 2g + h in lane half h, so one v_permlane32_swap per dword leaves each lane with the six
-// limbs of one element: lane l of wavefront w finishes element 64 w + l of the tile.
+// A block takes a 4-row group of 64 columns (256 elements) per step.  Wavefront w loads plane
+// quads w and w + 4, lane l the dword of column j0 + l (its four rows) of each plane: 256
+// contiguous bytes per load, issued unconditionally (clamped addresses, selected
+// afterwards).  Each lane turns its 4 dwords into 4 dwords of 4 planes of one element (byte
+// transpose) and stores them to LDS rows of ROWD = 8 KB + 1 dwords (KB blocks of 32
+// moduli): element (row e, column l) of the tile is element 64 e + l, in row
+// crt_row(64 e + l), a permutation within groups of 4, so the 32 lanes of every store and of
+// every ds_read_b32 of the product's column operand hit 32 distinct banks.  Product rows
+// (crt_byte_of_row) put the limbs 2g + h in lane half h, so one v_permlane32_swap per dword
+// leaves each lane with the six limbs of one element: lane l of wavefront w finishes
+// element 64 w + l, row w of the group and column j0 + l (512 contiguous bytes of C).
 constexpr int CRT_TILE = 256;
 __device__ __forceinline__ int crt_row(int e) { return (e & ~3) | (((e & 3) + (e >> 5)) & 3); }
 
@@ -1079,31 +1074,38 @@ __global__ __launch_bounds__(256) void k_crt(const uint8_t* __restrict__ Cr, lon
 #pragma unroll
   for (int kb = 0; kb < KB; ++kb) tf[kb] = *reinterpret_cast<const v4i_t*>(&tab->mrb[kb][lane & 31][16 * h]);
   __syncthreads();
-  const size_t plane = (size_t)M * Np;
-  const long j0 = (long)blockIdx.x * CRT_TILE, j = j0 + 64 * w + lane;
-  const long jq = j0 + 4 * lane, jc = min(jq, Np - 4);  // Np >= 4, a multiple of 4
+  const long G = (M + 3) >> 2;  // 4-row groups
+  const size_t plane = (size_t)G * 4 * Np;
+  const long j0 = (long)blockIdx.x * 64, j = j0 + lane, jc = min(j, N - 1);
   int wrow[4];
 #pragma unroll
-  for (int e = 0; e < 4; ++e) wrow[e] = crt_row(4 * lane + e) * ROWD;
+  for (int e = 0; e < 4; ++e) wrow[e] = crt_row(64 * e + lane) * ROWD;
   const int rrow[2] = {crt_row(64 * w + (lane & 31)) * ROWD + 4 * h, crt_row(64 * w + 32 + (lane & 31)) * ROWD + 4 * h};
-  // rows i = blockIdx.y + n gridDim.y; the next row's digits are loaded before this row's
-  // finish.  In the lower mode, rows above the tile's first column are skipped.
+  size_t poff[KB][2][4];  // the planes a lane loads (uniform), clamped to the last one
+#pragma unroll
+  for (int kb = 0; kb < KB; ++kb)
+#pragma unroll
+    for (int p = 0; p < 2; ++p)
+#pragma unroll
+      for (int k = 0; k < 4; ++k) poff[kb][p][k] = (size_t)min(32 * kb + 4 * (w + 4 * p) + k, R - 1) * plane;
+  // groups g = blockIdx.y + n gridDim.y; the next group's digits are loaded before this
+  // group's finish.  In the lower mode, groups wholly above the tile's first column are
+  // skipped (j0 is a multiple of 4).
   const long gy = gridDim.y;
-  long i = blockIdx.y;
-  if (lower && i < j0) i += (j0 - i + gy - 1) / gy * gy;
+  long g = blockIdx.y;
+  if (lower && 4 * g < j0) g += (j0 / 4 - g + gy - 1) / gy * gy;
   uint32_t d[KB][2][4];
-  auto fetch = [&](long ii) {
-    const uint8_t* row = Cr + (size_t)ii * Np + jc;
+  auto fetch = [&](long gg) {
+    const uint8_t* grp = Cr + (size_t)gg * 4 * Np + 4 * jc;
 #pragma unroll
     for (int kb = 0; kb < KB; ++kb)
 #pragma unroll
       for (int p = 0; p < 2; ++p)
 #pragma unroll
-        for (int k = 0; k < 4; ++k)
-          d[kb][p][k] = *reinterpret_cast<const uint32_t*>(row + (size_t)min(32 * kb + 4 * (w + 4 * p) + k, R - 1) * plane);
+        for (int k = 0; k < 4; ++k) d[kb][p][k] = *reinterpret_cast<const uint32_t*>(grp + poff[kb][p][k]);
   };
-  if (i < M) fetch(i);
-  for (; i < M; i += gy) {
+  if (g < G) fetch(g);
+  for (; g < G; g += gy) {
 #pragma unroll
     for (int kb = 0; kb < KB; ++kb)
 #pragma unroll
@@ -1113,7 +1115,7 @@ __global__ __launch_bounds__(256) void k_crt(const uint8_t* __restrict__ Cr, lon
 #pragma unroll
         for (int k = 0; k < 4; ++k) {
           const int r = 32 * kb + 4 * rq + k;
-          v[k] = r < R ? (jq < Np ? d[kb][p][k] ^ 0x80808080u : 0u) : (r == R ? 0x01010101u : 0u);
+          v[k] = (r < R && j < N) ? d[kb][p][k] : 0u;
         }
 #pragma unroll
         for (int e = 0; e < 4; ++e)
@@ -1139,8 +1141,8 @@ __global__ __launch_bounds__(256) void k_crt(const uint8_t* __restrict__ Cr, lon
         y[c][g] = __builtin_fma((double)acc[5 * g + 4], 0x1p32, __builtin_fma((double)b, 0x1p16, (double)a));
       }
     }
-    __syncthreads();  // dig is restaged for the next row
-    if (i + gy < M) fetch(i + gy);
+    __syncthreads();  // dig is restaged for the next group
+    if (g + gy < G) fetch(g + gy);
     double sl[CRT_L40];
 #pragma unroll
     for (int g = 0; g < 3; ++g) {
@@ -1150,7 +1152,8 @@ __global__ __launch_bounds__(256) void k_crt(const uint8_t* __restrict__ Cr, lon
       sl[2 * g] = __builtin_bit_cast(double, (uint64_t)lw[0] | ((uint64_t)hw[0] << 32));
       sl[2 * g + 1] = __builtin_bit_cast(double, (uint64_t)lw[1] | ((uint64_t)hw[1] << 32));
     }
-    if (j >= N || (lower && j > i)) continue;
+    const long i = 4 * g + w;
+    if (i >= M || j >= N || (lower && j > i)) continue;
     dd res = (er[i] >= E_NONFINITE || fc[j] >= E_NONFINITE) ? mkdd(__builtin_nan(""), __builtin_nan(""))
                                                               : crt_finish(sl, mt, er[i] - 2 * W + fc[j]);
     const size_t co = (size_t)i * ldc + j;
@@ -1252,7 +1255,7 @@ int split_gemm_general(const SplitOperand& A, const SplitOperand& B, double* chi
     return HP_EINVAL;
   }
   const size_t Kp = (k + DL_GK - 1) / DL_GK * DL_GK, Np = (n + 3) / 4 * 4;
-  const size_t resA = (size_t)R * m * Kp, resB = (size_t)R * n * Kp, resC = (size_t)R * m * Np;
+  const size_t resA = (size_t)R * m * Kp, resB = (size_t)R * n * Kp, resC = (size_t)R * ((m + 3) / 4 * 4) * Np;
   const size_t need = resA + resB + resC + (m + n) * sizeof(int) + 1024;
   SplitWork& g_work = hp::g_work[opt.slot];
   std::lock_guard<std::mutex> lock(g_work.mu);
@@ -1410,17 +1413,13 @@ int split_gemm_general(const SplitOperand& A, const SplitOperand& B, double* chi
   }
   if (int rc = launch(variant)) return rc;
   tm.mark(s);
-  // about 2048 blocks (8 per CU), each taking rows blockIdx.y + n gridDim.y
-  const unsigned crt_gx = (unsigned)((n + CRT_TILE - 1) / CRT_TILE);
-  const unsigned crt_gy = (unsigned)std::max<size_t>(1, std::min<size_t>(m, (2048 + crt_gx - 1) / crt_gx));
-  if (R < 32)
-    hipLaunchKernelGGL(k_crt<1>, dim3(crt_gx, crt_gy), dim3(256), 0, s, rC,
-                       (long)m, (long)n, (long)Np, eA, fB, W, g_work.tab, chi, clo, (long)ldc, low,
-                       opt.subtract ? 1 : 0);
-  else
-    hipLaunchKernelGGL(k_crt<2>, dim3(crt_gx, crt_gy), dim3(256), 0, s, rC,
-                       (long)m, (long)n, (long)Np, eA, fB, W, g_work.tab, chi, clo, (long)ldc, low,
-                       opt.subtract ? 1 : 0);
+  // about 2048 blocks (8 per CU), each taking the 4-row groups blockIdx.y + n gridDim.y
+  const unsigned crt_gx = (unsigned)((n + 63) / 64);
+  const unsigned crt_gy = (unsigned)std::max<size_t>(1, std::min<size_t>((m + 3) / 4, (2048 + crt_gx - 1) / crt_gx));
+  // one block of 32 moduli: R <= 32 (the constant digit row that uncentred digits needed
+  // is gone, so a second block is never used)
+  hipLaunchKernelGGL(k_crt<1>, dim3(crt_gx, crt_gy), dim3(256), 0, s, rC, (long)m, (long)n, (long)Np, eA, fB, W,
+                     g_work.tab, chi, clo, (long)ldc, low, opt.subtract ? 1 : 0);
   HP_LAUNCH_CHECK();
   tm.mark(s);
   HP_HIP(hipEventRecord(g_work.done, s));
