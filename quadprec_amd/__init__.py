@@ -803,8 +803,11 @@ def gemm_plan(m, n, k, bits=0, trans_a=False):
 
 
 def split_gemm_set_variant(m, n, k, variant, slices=0):
-    """Pin the split GEMM's int8 kernel for a shape class: 32, 64 or 128 (K stages of that
-    depth staged through registers) or 3064 (direct-to-LDS ring of 64-deep stages); 0 re-tunes."""
+    """Pin the split GEMM's int8 kernel for a shape class (m, n, k and the moduli count, slices
+    = 0 meaning the dd-accuracy default the GEMM itself takes): 32, 64 or 128 (K stages of that
+    depth staged through registers), 3064 (direct-to-LDS ring of 64-deep stages), 4064 / 5064
+    (that ring software-pipelined, 8 / 4 wavefronts), 6064 (register-staged, three stages
+    ahead) or 7064 (4064 as a persistent kernel); 0 re-tunes."""
     _check(lib().hp_split_gemm_set_variant(m, n, k, slices, variant), "split_gemm_set_variant")
 
 

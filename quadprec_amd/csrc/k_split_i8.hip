@@ -38,6 +38,7 @@
 #include <map>
 #include <memory>
 #include <mutex>
+#include <type_traits>
 #include <vector>
 
 #include "hp_arith.hpp"
@@ -699,10 +700,9 @@ __global__ __launch_bounds__(512) void k_gemm_i8_mod_dl(const int8_t* __restrict
 // logical range [c B/8, (c+1) B/8); within a modulus the logical order walks 4 x 8 patches of
 // 256 x 256 tiles, so the 32 blocks resident on an XCD share 4 A row tiles and 8 B column
 // tiles (12 tiles streamed through its L2 per K stage instead of 18 in launch order).
-__device__ __forceinline__ void xcd_tile(long& tx, long& ty, int& tz) {
-  const unsigned gx = gridDim.x, gy = gridDim.y, T = gx * gy, nb = T * gridDim.z;
-  unsigned b = blockIdx.x + gx * (blockIdx.y + gy * blockIdx.z);
-  if (nb % 8 == 0) b = (b % 8) * (nb / 8) + b / 8;
+// logical tile b (moduli outermost, then 4 x 8 patches of tiles) -> tile (tx, ty) of modulus tz
+__device__ __forceinline__ void logical_tile(unsigned b, unsigned gx, unsigned gy, long& tx, long& ty, int& tz) {
+  const unsigned T = gx * gy;
   tz = (int)(b / T);
   const unsigned u = b % T;
   if (gx % 8 == 0 && gy % 4 == 0) {
@@ -713,6 +713,12 @@ __device__ __forceinline__ void xcd_tile(long& tx, long& ty, int& tz) {
     ty = (long)(u / gx);
     tx = (long)(u % gx);
   }
+}
+__device__ __forceinline__ void xcd_tile(long& tx, long& ty, int& tz) {
+  const unsigned gx = gridDim.x, gy = gridDim.y, nb = gx * gy * gridDim.z;
+  unsigned b = blockIdx.x + gx * (blockIdx.y + gy * blockIdx.z);
+  if (nb % 8 == 0) b = (b % 8) * (nb / 8) + b / 8;
+  logical_tile(b, gx, gy, tx, ty, tz);
 }
 
 // s_waitcnt immediate that waits until at most n vector-memory operations are outstanding
@@ -859,6 +865,198 @@ __device__ __forceinline__ void gemm_dlp_body(const int8_t* __restrict__ A, cons
   store_digits(acc, C, M, N, ldc, i0, j0, wm, wn, lane, (double)m, inv_m);
 }
 
+// Persistent form of the software-pipelined product with 8 wavefronts of 64 x 128: one block
+// per CU (the grid is a multiple of 8 blocks) walks tiles.  The hardware hands block ids to
+// the 8 XCDs in turn, so block b runs on XCD b % 8, which takes the logical range [c L,
+// (c+1) L) of the tile order above (L = total / 8 rounded up), its blocks stepping through it
+// side by side: at any time an XCD works on one 4 x 8 patch.  The next tile's first three
+// K stages are issued as soon as every wavefront has read the ring for the last time, before
+// the digit epilogue of the current tile, so their load latency is spent on the epilogue.
+// Vector memory operations complete in issue order, so the next tile's first wait counts
+// the epilogue's 32 stores issued after those loads; a tile that is not wholly inside C
+// stores first (its store count varies) and then issues the next tile's loads.
+__device__ __forceinline__ void gemm_dlp_persistent(const int8_t* __restrict__ A0, const int8_t* __restrict__ B0,
+                                                    uint8_t* __restrict__ C0, long M, long N, long K, long ldc,
+                                                    const CrtTable* __restrict__ tab, int lower, int R) {
+  constexpr int NA = 2, NW = 8, LPW = 32 / NW;
+  extern __shared__ __align__(16) int8_t smem[];
+  const int t = threadIdx.x, lane = t & 63, w = t >> 6;
+  const int wm = w >> 1, wn = w & 1;
+  const unsigned gx = (unsigned)((N + GN - 1) / GN), gy = (unsigned)((M + GM - 1) / GM);
+  const unsigned total = gx * gy * (unsigned)R;
+  const unsigned P = gridDim.x / 8, c = blockIdx.x % 8, L = (total + 7) / 8;
+  const unsigned uend = min(total, (c + 1) * L);
+  // the next tile of this block at or after logical index uu that the lower mode keeps
+  auto next_valid = [&](unsigned uu, long& tx, long& ty, int& r) -> unsigned {
+    for (; uu < uend; uu += P) {
+      logical_tile(uu, gx, gy, tx, ty, r);
+      if (!(lower && tx > ty)) return uu;
+    }
+    return uend;
+  };
+  long tx, ty;
+  int r;
+  unsigned u = next_valid(c * L + blockIdx.x / 8, tx, ty, r);
+  if (u >= uend) return;
+
+  const int8_t* src[LPW];
+  long sstr[LPW];
+  int dst[LPW];
+  auto setup = [&](long i0, long j0, int rr) {
+    const int8_t* A = A0 + (size_t)rr * M * K;
+    const int8_t* B = B0 + (size_t)rr * N * K;
+#pragma unroll
+    for (int j = 0; j < LPW; ++j) {
+      const int q = LPW * w + j, op = q >> 4, rb = (q & 15) * 16;
+      const int row = rb + (lane >> 2);
+      const int chunk = (lane & 3) ^ ((row >> 2) & 3);
+      const long grow = op == 0 ? min(i0 + row, M - 1) : min(j0 + row, N - 1);
+      src[j] = (op == 0 ? A : B) + grow * 64 + 16 * chunk;
+      sstr[j] = (op == 0 ? M : N) * 64;
+      dst[j] = op * (256 * DL_GK) + rb * DL_GK;
+    }
+  };
+  auto issue = [&](long st) {
+    int8_t* slot = smem + (size_t)(st % DL_RING) * DL_SLOT;
+#pragma unroll
+    for (int j = 0; j < LPW; ++j)
+      __builtin_amdgcn_global_load_lds(src[j] + st * sstr[j], (__attribute__((address_space(3))) void*)(slot + dst[j]),
+                                       16, 0, 0);
+  };
+  auto bar = []() { asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory"); };
+  auto admit = [&](long ahead) {
+    if (ahead >= 2) __builtin_amdgcn_s_waitcnt(waitcnt_vm(2 * LPW));
+    else if (ahead == 1) __builtin_amdgcn_s_waitcnt(waitcnt_vm(LPW));
+    else __builtin_amdgcn_s_waitcnt(waitcnt_vm(0));
+    bar();
+  };
+  // the waits of a tile whose first three stages were issued before the previous tile's 32
+  // stores: the stores sit between stage 2 and stage 3 in issue order, so the waits for
+  // stages 0, 1 and 2 allow 32 more operations in flight
+  bool after_stores = false;
+  auto admit_after_stores = [&](long ahead) {
+    if (ahead >= 2) __builtin_amdgcn_s_waitcnt(waitcnt_vm(2 * LPW + 32));
+    else if (ahead == 1) __builtin_amdgcn_s_waitcnt(waitcnt_vm(LPW + 32));
+    else __builtin_amdgcn_s_waitcnt(waitcnt_vm(32));
+    bar();
+  };
+  auto admit_k = [&](long ahead, long st) {  // the wait that admits stage st + 1
+    if (after_stores && st < 2) admit_after_stores(ahead);
+    else admit(ahead);
+  };
+  auto read_frags = [&](long st, int ks, v4i_t (&af)[NA], v4i_t (&bf)[4]) {
+    const int8_t* at = smem + (size_t)(st % DL_RING) * DL_SLOT;
+    const int8_t* bt = at + 256 * DL_GK;
+    const int cc = (ks >> 4) + (lane >> 5);
+#pragma unroll
+    for (int a = 0; a < NA; ++a) {
+      const int row = wm * 32 * NA + a * 32 + (lane & 31);
+      af[a] = *reinterpret_cast<const v4i_t*>(at + row * DL_GK + ((cc ^ ((row >> 2) & 3)) << 4));
+    }
+#pragma unroll
+    for (int b = 0; b < 4; ++b) {
+      const int row = wn * 128 + b * 32 + (lane & 31);
+      bf[b] = *reinterpret_cast<const v4i_t*>(bt + row * DL_GK + ((cc ^ ((row >> 2) & 3)) << 4));
+    }
+  };
+  v16i_t acc[NA][4];
+  auto mfmas = [&](const v4i_t (&af)[NA], const v4i_t (&bf)[4]) {
+#pragma unroll
+    for (int a = 0; a < NA; ++a)
+#pragma unroll
+      for (int b = 0; b < 4; ++b) acc[a][b] = __builtin_amdgcn_mfma_i32_32x32x32_i8(af[a], bf[b], acc[a][b], 0, 0, 0);
+  };
+
+  const long nst = K / DL_GK;
+  constexpr int NR = NA + 4;
+  setup(ty * GM, tx * GN, r);
+  for (long st = 0; st < 3 && st < nst; ++st) issue(st);
+  while (true) {
+    const long i0 = ty * GM, j0 = tx * GN;
+    const uint32_t m = tab->mod[r];
+    const double inv_m = tab->inv_m[r];
+#pragma unroll
+    for (int a = 0; a < NA; ++a)
+#pragma unroll
+      for (int b = 0; b < 4; ++b)
+#pragma unroll
+        for (int q = 0; q < 16; ++q) acc[a][b][q] = 0;
+    if (after_stores) admit_after_stores(min(nst - 1, 2L));
+    else admit(min(nst - 1, 2L));
+    if (nst > 3) issue(3);
+    v4i_t a0[NA], b0[4], a1[NA], b1[4];
+    read_frags(0, 0, a0, b0);
+    // one steady-state stage; AS: its wait allows for the previous tile's stores (the first
+    // two stages after a full tile), a separate copy so the main loop has no branch
+    auto steady = [&](long st, auto AS) {
+      read_frags(st, 32, a1, b1);
+      mfmas(a0, b0);
+#pragma unroll
+      for (int i = 0; i < NR; ++i) {
+        __builtin_amdgcn_sched_group_barrier(0x008, 1, 0);
+        __builtin_amdgcn_sched_group_barrier(0x100, 1, 0);
+      }
+      __builtin_amdgcn_sched_group_barrier(0x008, 4 * NA - NR, 0);
+      __builtin_amdgcn_sched_barrier(0);
+      if constexpr (decltype(AS)::value) admit_after_stores(2);
+      else admit(2);
+      __builtin_amdgcn_sched_barrier(0);
+      issue(st + 4);
+      read_frags(st + 1, 0, a0, b0);
+      mfmas(a1, b1);
+#pragma unroll
+      for (int i = 0; i < NR; ++i) {
+        __builtin_amdgcn_sched_group_barrier(0x008, 1, 0);
+        __builtin_amdgcn_sched_group_barrier(0x100, 1, 0);
+      }
+#pragma unroll
+      for (int i = 0; i < 4 * NA - NR; ++i) {
+        __builtin_amdgcn_sched_group_barrier(0x008, 1, 0);
+        __builtin_amdgcn_sched_group_barrier(0x020, 1, 0);
+      }
+      __builtin_amdgcn_sched_barrier(0);
+      if (((st + 1) * DL_GK) % KCH == 0) reduce_acc(acc, m, inv_m);
+    };
+    long st = 0;
+    if (after_stores)
+      for (; st < 2 && st + 4 < nst; ++st) steady(st, std::true_type{});
+    for (; st + 4 < nst; ++st) steady(st, std::false_type{});
+    for (; st < nst; ++st) {
+      read_frags(st, 32, a1, b1);
+      mfmas(a0, b0);
+      if (st + 1 < nst) {
+        admit_k(nst - 2 - st, st);
+        read_frags(st + 1, 0, a0, b0);
+      }
+      mfmas(a1, b1);
+      if (((st + 1) * DL_GK) % KCH == 0 && st + 1 < nst) reduce_acc(acc, m, inv_m);
+    }
+    uint8_t* C = C0 + (size_t)r * ((M + 3) & ~3L) * ldc;
+    long ntx, nty;
+    int nr;
+    const unsigned nu = next_valid(u + P, ntx, nty, nr);
+    if (nu >= uend) {
+      store_digits(acc, C, M, N, ldc, i0, j0, wm, wn, lane, (double)m, inv_m);
+      break;
+    }
+    bar();  // every wavefront has read the ring for the last time: it takes the next tile
+    after_stores = i0 + GM <= M && j0 + GN <= N;  // store_digits' unchecked path: 32 stores
+    if (after_stores) {
+      setup(nty * GM, ntx * GN, nr);
+      for (long s2 = 0; s2 < 3 && s2 < nst; ++s2) issue(s2);
+      store_digits(acc, C, M, N, ldc, i0, j0, wm, wn, lane, (double)m, inv_m);
+    } else {
+      store_digits(acc, C, M, N, ldc, i0, j0, wm, wn, lane, (double)m, inv_m);
+      setup(nty * GM, ntx * GN, nr);
+      for (long s2 = 0; s2 < 3 && s2 < nst; ++s2) issue(s2);
+    }
+    u = nu;
+    tx = ntx;
+    ty = nty;
+    r = nr;
+  }
+}
+
 // Register-staged product (8 wavefronts of 64 x 128): plain global loads (about twice the
 // throughput of direct-to-LDS loads on this access pattern) fetch each stage three stages
 // ahead into three register sets (stage s in set s % 3; the loop is unrolled by three so
@@ -981,6 +1179,11 @@ __global__ __launch_bounds__(512) void k_gemm_i8_mod_dlp8(const int8_t* __restri
                                                           uint8_t* __restrict__ C, long M, long N, long K, long ldc,
                                                           const CrtTable* __restrict__ tab, int lower) {
   gemm_dlp_body<2>(A, B, C, M, N, K, ldc, tab, lower);
+}
+__global__ __launch_bounds__(512) void k_gemm_i8_mod_dlpp(const int8_t* __restrict__ A, const int8_t* __restrict__ B,
+                                                          uint8_t* __restrict__ C, long M, long N, long K, long ldc,
+                                                          const CrtTable* __restrict__ tab, int lower, int R) {
+  gemm_dlp_persistent(A, B, C, M, N, K, ldc, tab, lower, R);
 }
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 1))) void k_gemm_i8_mod_dlp4(
     const int8_t* __restrict__ A, const int8_t* __restrict__ B, uint8_t* __restrict__ C, long M, long N, long K,
@@ -1163,6 +1366,16 @@ __global__ __launch_bounds__(256) void k_crt(const uint8_t* __restrict__ Cr, lon
   }
 }
 
+// blocks of the persistent int8 kernel: one per CU of the current device, a multiple of 8
+// (one share per XCD)
+unsigned persistent_blocks() {
+  int dev = 0, cus = 0;
+  if (hipGetDevice(&dev) != hipSuccess ||
+      hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus < 8)
+    cus = 256;
+  return (unsigned)(cus / 8 * 8);
+}
+
 // ------------------------------------------------------------------ workspace cache
 // A workspace (residue planes, CRT digits, scales) is shared by all calls of its slot.  A
 // call records `done` after its last kernel; the next call of the slot on the same stream
@@ -1197,15 +1410,16 @@ uint64_t shape_key(size_t m, size_t n, size_t k, int R) {
 
 int set_split_gemm_variant(size_t m, size_t n, size_t k, int R, int variant) {
   if (variant != 0 && variant != 32 && variant != 64 && variant != 128 && variant != 3064 && variant != 4064 &&
-      variant != 5064 && variant != 6064) {
+      variant != 5064 && variant != 6064 && variant != 7064) {
     set_error("split gemm variant must be 0 (autotune), 32, 64, 128 (K stages of that depth staged through "
               "registers), 3064 (direct-to-LDS ring of 64-deep stages), 4064 / 5064 (the same ring, "
-              "software-pipelined, 8 wavefronts of 64 x 128 / 4 wavefronts of 128 x 128, K <= 65536) or 6064 "
-              "(register-staged, three stages ahead, K % 192 == 0)");
+              "software-pipelined, 8 wavefronts of 64 x 128 / 4 wavefronts of 128 x 128, K <= 65536), 6064 "
+              "(register-staged, three stages ahead, K % 192 == 0) or 7064 (4064 as a persistent kernel, one "
+              "block per CU, the next tile's first stages loaded during the digit epilogue)");
     return HP_EINVAL;
   }
   std::lock_guard<std::mutex> lock(g_tune_mu);
-  uint64_t key = shape_key(m, n, k, R <= 0 ? 31 : R);
+  uint64_t key = shape_key(m, n, k, R <= 0 ? split_slices_for_bits(104, k) : R);
   if (variant == 0) g_tuned.erase(key);
   else g_tuned[key] = variant;
   return HP_OK;
@@ -1213,7 +1427,7 @@ int set_split_gemm_variant(size_t m, size_t n, size_t k, int R, int variant) {
 
 int split_gemm_variant(size_t m, size_t n, size_t k, int R) {
   std::lock_guard<std::mutex> lock(g_tune_mu);
-  auto it = g_tuned.find(shape_key(m, n, k, R <= 0 ? 31 : R));
+  auto it = g_tuned.find(shape_key(m, n, k, R <= 0 ? split_slices_for_bits(104, k) : R));
   return it == g_tuned.end() ? 0 : it->second;
 }
 
@@ -1284,6 +1498,8 @@ int split_gemm_general(const SplitOperand& A, const SplitOperand& B, double* chi
                                (int)gemm_i8_dl_lds()));
     HP_HIP(hipFuncSetAttribute((const void*)k_gemm_i8_mod_dlp4, hipFuncAttributeMaxDynamicSharedMemorySize,
                                (int)gemm_i8_dl_lds()));
+    HP_HIP(hipFuncSetAttribute((const void*)k_gemm_i8_mod_dlpp, hipFuncAttributeMaxDynamicSharedMemorySize,
+                               (int)gemm_i8_dl_lds()));
     HP_HIP(hipFuncSetAttribute((const void*)k_gemm_i8_mod_rs, hipFuncAttributeMaxDynamicSharedMemorySize,
                                (int)gemm_i8_rs_lds()));
   }
@@ -1350,7 +1566,11 @@ int split_gemm_general(const SplitOperand& A, const SplitOperand& B, double* chi
   auto launch = [&](int variant) -> int {
     if (variant == 5064 && Kp > (size_t)KCH) variant = 4064;  // no intermediate reduction in that kernel
     if (variant == 6064 && Kp % (3 * DL_GK) != 0) variant = 4064;  // whole three-stage trips only
-    if (variant == 6064)
+    if (variant == 7064) {
+      const unsigned nb = std::min<unsigned>((unsigned)(gg.x * gg.y * gg.z + 7) / 8 * 8, persistent_blocks());
+      hipLaunchKernelGGL(k_gemm_i8_mod_dlpp, dim3(nb), dim3(512), gemm_i8_dl_lds(), s, rA, rB, rC, (long)m, (long)n,
+                         (long)Kp, (long)Np, g_work.tab, low, R);
+    } else if (variant == 6064)
       hipLaunchKernelGGL(k_gemm_i8_mod_rs, gg, dim3(512), gemm_i8_rs_lds(), s, rA, rB, rC, (long)m, (long)n, (long)Kp,
                          (long)Np, g_work.tab, low);
     else if (variant == 5064)
@@ -1384,7 +1604,7 @@ int split_gemm_general(const SplitOperand& A, const SplitOperand& B, double* chi
     if (it != g_tuned.end()) variant = it->second;
   }
   if (!variant) {
-    const int cand[4] = {64, 128, 4064, 6064};
+    const int cand[5] = {64, 128, 4064, 6064, 7064};
     float best = 1e30f;
     hipEvent_t e0, e1;
     HP_HIP(hipEventCreate(&e0));

@@ -65,7 +65,7 @@ def test_split_i8_vs_f256(qp, m, n, k):
     assert e_split <= max(4 * e_plain, 2.0 ** -104)
 
 
-@pytest.mark.parametrize("variant", [32, 64, 128, 3064, 4064, 5064, 6064])
+@pytest.mark.parametrize("variant", [32, 64, 128, 3064, 4064, 5064, 6064, 7064])
 def test_split_i8_every_k_stage_variant(qp, variant):
     m, n, k = 300, 270, 576  # 9 stages of 64: whole three-stage trips for 6064
     a = qp.gen_dd_normal(41, m * k)
@@ -110,7 +110,8 @@ def test_split_i8_exact_on_small_integers(qp):
     assert np.array_equal(c[:, 0], ex.astype(np.float64)) and not c[:, 1].any()
 
 
-@pytest.mark.parametrize("variant,k", [(128, 70000), (3064, 70000), (4064, 70000), (5064, 70000), (6064, 69888)])
+@pytest.mark.parametrize("variant,k", [(128, 70000), (3064, 70000), (4064, 70000), (5064, 70000), (6064, 69888),
+                                       (7064, 70000)])
 def test_split_i8_exact_beyond_one_reduction_chunk(qp, variant, k):
     """k = 70000: the int32 sums are reduced mod m once after 65536 products; K is padded to
     70016 with zero residues and N = 47 leaves a partial dword of CRT digits per row
@@ -129,6 +130,26 @@ def test_split_i8_exact_beyond_one_reduction_chunk(qp, variant, k):
         qp.split_gemm_set_variant(m, n, k, 0)
     ex = (a[:, 0].reshape(m, k).astype(np.int64) @ b[:, 0].reshape(k, n).astype(np.int64)).ravel()
     assert np.array_equal(c[:, 0], ex.astype(np.float64)) and not c[:, 1].any()
+
+
+@pytest.mark.parametrize("m,n,k", [(1100, 1030, 64), (1100, 1030, 128), (1100, 1030, 192), (1100, 1030, 256),
+                                   (1100, 1030, 1000), (2048, 2048, 256)])
+def test_split_i8_persistent_matches_tiled(qp, m, n, k):
+    """The persistent int8 kernel (7064: one block per CU walking tiles, the next tile's first
+    K stages loaded during the digit epilogue) forms the same digits as the one-tile-per-block
+    kernel 4064, hence the same dd product bit for bit: full tiles (their stores issued between
+    the next tile's loads and its waits), ragged edge tiles (stores first), 1 to 16 K stages,
+    several tiles per block, and the 4 x 8 patch order (2048^2)."""
+    a = qp.gen_dd_normal(61, m * k)
+    b = qp.gen_dd_normal(62, k * n)
+    out = {}
+    for v in (4064, 7064):
+        qp.split_gemm_set_variant(m, n, k, v)
+        try:
+            out[v] = dev_gemm(qp, a, b, m, n, k, qp.HP_GEMM_SPLIT_I8)
+        finally:
+            qp.split_gemm_set_variant(m, n, k, 0)
+    assert np.array_equal(out[4064], out[7064])
 
 
 @pytest.mark.parametrize("slices", [24, 28, 32])
