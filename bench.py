@@ -31,11 +31,13 @@ INT8_PEAK_TOPS = 5000.0          # MI355X dense INT8 matrix peak
 DD_MAC_OPS = 29                  # FP64 ops per reference dd MAC (dd add 20 + dd mul 9)
 # int8 GEMM kernel per autotuner variant code
 I8_KERNELS = {32: "k_gemm_i8_mod<32>", 64: "k_gemm_i8_mod<64>", 128: "k_gemm_i8_mod<128>", 3064: "k_gemm_i8_mod_dl",
-              4064: "k_gemm_i8_mod_dlp8", 5064: "k_gemm_i8_mod_dlp4", 6064: "k_gemm_i8_mod_rs"}
+              4064: "k_gemm_i8_mod_dlp8", 5064: "k_gemm_i8_mod_dlp4", 6064: "k_gemm_i8_mod_rs",
+              7064: "k_gemm_i8_mod_dlpp"}
 # HBM bytes one launch at 4096^3 x 30 moduli moves, from rocprofv3 FETCH_SIZE / WRITE_SIZE
-# counters scaled by a device copy of known size (k_gemm_i8_mod_dlp8 on the 64-deep K-slab
-# residue layout: read 3.020e9 + write 0.503e9); null for a kernel these were not measured on
-I8_TRAFFIC_BYTES = {4064: 3.523e9}
+# counters scaled by a device copy of known size (k_gemm_i8_mod_dlp8 and its persistent form
+# on the 64-deep K-slab residue layout: read 3.020e9 + write 0.503e9); null for a kernel
+# these were not measured on
+I8_TRAFFIC_BYTES = {4064: 3.523e9, 7064: 3.524e9}
 
 
 def oracle():
@@ -1008,6 +1010,8 @@ def main():
     ap.add_argument("--no-overlap-s", action="store_true", help="skip the dd overlap build")
     ap.add_argument("--no-atb", action="store_true", help="skip gemm_atb at 4096^3")
     ap.add_argument("--no-autotune", action="store_true", help="skip the autotuner's accuracy sweep")
+    ap.add_argument("--i8-variant", type=int, default=0, metavar="V",
+                    help="pin the headline's int8 GEMM kernel (variant code, e.g. 4064 or 7064; 0: the tuner's pick)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only copy MIB MiB device-to-device three times (a plain copy kernel of known size, "
                          "to calibrate rocprofv3's FETCH_SIZE / WRITE_SIZE counters) and exit")
@@ -1041,6 +1045,9 @@ def main():
     # split path (headline); per-phase HIP events on the launch stream, accumulated over the
     # timed steps without a host wait per call
     qp.set_phase_timing(2)
+
+    if args.i8_variant:  # pin the int8 kernel instead of the tuner's choice
+        qp.split_gemm_set_variant(M, N, K, args.i8_variant, slices=SLICES)
 
     def split_step():
         qp.dev.gemm(da, db, dc, M, N, K, mode=qp.HP_GEMM_AUTO, slices=DD_BITS)

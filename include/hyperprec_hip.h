@@ -508,11 +508,15 @@ int hp_overlap_adjoint(const double* centers, const double* alpha, const double*
 void hp_set_phase_timing(int on);
 int hp_collect_phase_times(double* ms_sum, int max, int* calls);
 /* Int8 kernel the split GEMM's autotuner chose for this shape class (0 = not tuned yet).
- * The first split GEMM of a shape class times each variant on its real operands. */
+ * A shape class is (m, n, k) bucketed to powers of two and the moduli count; slices = 0
+ * means the count the GEMM takes by default at this k (dd accuracy).  The first split GEMM
+ * of a shape class times each candidate twice on its real operands and keeps the fastest. */
 int hp_split_gemm_variant(size_t m, size_t n, size_t k, int slices);
 /* Pins the int8 kernel for a shape class: 32, 64 or 128 (K stages of that depth staged
- * through registers) or 3064 (operand tiles streamed straight into a 4-deep LDS ring of
- * 64-deep stages); 0 clears it (re-tune). */
+ * through registers), 3064 (operand tiles streamed straight into a 4-deep LDS ring of
+ * 64-deep stages), 4064 / 5064 (that ring software-pipelined, 8 / 4 wavefronts), 6064
+ * (register-staged, three stages ahead) or 7064 (4064 as a persistent kernel, one block per
+ * CU); 0 clears it (re-tune). */
 int hp_split_gemm_set_variant(size_t m, size_t n, size_t k, int slices, int variant);
 int hp_get_phase_times(double* ms, int max);
 

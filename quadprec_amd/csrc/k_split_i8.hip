@@ -1613,13 +1613,15 @@ int split_gemm_general(const SplitOperand& A, const SplitOperand& B, double* chi
       if ((c == 128 && Kp < 128) || (c == 6064 && Kp % (3 * DL_GK) != 0)) continue;
       int rc = launch(c);  // warm
       if (rc) return rc;
-      HP_HIP(hipEventRecord(e0, s));
-      if ((rc = launch(c))) return rc;
-      HP_HIP(hipEventRecord(e1, s));
-      HP_HIP(hipEventSynchronize(e1));
-      float ms = 0.f;
-      HP_HIP(hipEventElapsedTime(&ms, e0, e1));
-      if (ms < best) { best = ms; variant = c; }
+      for (int rep = 0; rep < 2; ++rep) {  // the faster of two runs: one run varies by a few percent
+        HP_HIP(hipEventRecord(e0, s));
+        if ((rc = launch(c))) return rc;
+        HP_HIP(hipEventRecord(e1, s));
+        HP_HIP(hipEventSynchronize(e1));
+        float ms = 0.f;
+        HP_HIP(hipEventElapsedTime(&ms, e0, e1));
+        if (ms < best) { best = ms; variant = c; }
+      }
     }
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
