@@ -365,15 +365,32 @@ HP_HD void mf_accumulate_front(double (&w)[8], double v) {
     two_sum(w[IDX], v, s, e);
     w[IDX] = s;
 #pragma unroll
-    for (int k = IDX + 1; k < F; ++k) {
-      if (e != 0.0) {
-        double s2, e2;
-        two_sum(w[k], e, s2, e2);
-        w[k] = s2;
-        e = e2;
-      }
+    for (int k = IDX + 1; k < F; ++k) {  // unguarded: the same bits (mf_accumulate_finite)
+      double s2, e2;
+      two_sum(w[k], e, s2, e2);
+      w[k] = s2;
+      e = e2;
     }
     w[F] = 0.0 + e;
+  }
+}
+// mf_accumulate without the e != 0 guard, for finite values in slots that never hold -0.0.
+// Slots start at +0.0 and a two_sum's sum is -0.0 only when both addends are -0.0, so no
+// slot becomes -0.0; for such a finite slot w, two_sum(w, +-0.0) returns (w, +0.0) (w + 0
+// is w, (w - w) + (+-0 - +0) is +0), so the levels the guard would skip leave every slot
+// and the carried error unchanged: the same bits as mf_accumulate, without the compare
+// and selects per level.
+template <int IDX>
+HP_HD void mf_accumulate_finite(double (&w)[8], double v) {
+  double s, e;
+  two_sum(w[IDX], v, s, e);
+  w[IDX] = s;
+#pragma unroll
+  for (int k = IDX + 1; k < 8; ++k) {
+    double s2, e2;
+    two_sum(w[k], e, s2, e2);
+    w[k] = s2;
+    e = e2;
   }
 }
 // (i, j) pairs with i + j < 4 in the reference's loop order (mul.rs:212-224); FAST: every
@@ -399,9 +416,20 @@ HP_HD mf4 mf_mul_impl(mf4 a, mf4 b) {
   } else {
     HP_MFMUL_TERM(0, 0) HP_MFMUL_TERM(0, 1) HP_MFMUL_TERM(0, 2) HP_MFMUL_TERM(0, 3)
   }
-  HP_MFMUL_TERM(1, 0) HP_MFMUL_TERM(1, 1) HP_MFMUL_TERM(1, 2)
-  HP_MFMUL_TERM(2, 0) HP_MFMUL_TERM(2, 1)
-  HP_MFMUL_TERM(3, 0)
+#define HP_MFMUL_FIN(I, J)             \
+  two_prod(A[I], B[J], p, e);          \
+  mf_accumulate_finite<(I) + (J)>(w, p); \
+  mf_accumulate_finite<(I) + (J) + 1>(w, e);
+  if constexpr (FAST) {  // all values finite, the slots never -0.0 (see mf_accumulate_finite)
+    HP_MFMUL_FIN(1, 0) HP_MFMUL_FIN(1, 1) HP_MFMUL_FIN(1, 2)
+    HP_MFMUL_FIN(2, 0) HP_MFMUL_FIN(2, 1)
+    HP_MFMUL_FIN(3, 0)
+  } else {
+    HP_MFMUL_TERM(1, 0) HP_MFMUL_TERM(1, 1) HP_MFMUL_TERM(1, 2)
+    HP_MFMUL_TERM(2, 0) HP_MFMUL_TERM(2, 1)
+    HP_MFMUL_TERM(3, 0)
+  }
+#undef HP_MFMUL_FIN
 #undef HP_MFMUL_FRONT
 #undef HP_MFMUL_TERM
   double o[4];
