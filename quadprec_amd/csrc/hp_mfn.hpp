@@ -182,15 +182,18 @@ HP_HD mfn<N> mfn_sub(const mfn<N>& a, const mfn<N>& b) {
 }
 
 // mul.rs:31-42: error-free accumulate of v into w[IDX], the error cascading upward while
-// non-zero, up to the 2N-slot limit
-template <int IDX, int LIMIT>
+// non-zero, up to the 2N-slot limit.  GUARD = false drops the e != 0 test, which gives the
+// same bits when every value is finite and no slot holds -0.0: the slots start at +0.0 and
+// a two_sum's sum is -0.0 only when both addends are, so none becomes -0.0, and for such a
+// finite slot w, two_sum(w, +-0.0) returns (w, +0.0), leaving every later level unchanged.
+template <int IDX, int LIMIT, bool GUARD = true>
 HP_HD void mfn_accumulate(double (&w)[LIMIT], double v) {
   double s, e;
   two_sum(w[IDX], v, s, e);
   w[IDX] = s;
 #pragma unroll
   for (int k = IDX + 1; k < LIMIT; ++k) {
-    if (e != 0.0) {
+    if (!GUARD || e != 0.0) {
       double s2, e2;
       two_sum(w[k], e, s2, e2);
       w[k] = s2;
@@ -200,9 +203,10 @@ HP_HD void mfn_accumulate(double (&w)[LIMIT], double v) {
 }
 // mul.rs:12-111: the products a_i b_j with i + j < N in the reference's loop order (i
 // outer, j inner), each two_prod's product into slot i + j and its error into i + j + 1,
-// then the 2N slots renormalised to N limbs
-template <int N>
-HP_HD mfn<N> mfn_mul(const mfn<N>& a, const mfn<N>& b) {
+// then the 2N slots renormalised to N limbs.  FAST: every limb below 2^500 in magnitude, so
+// no product, sum or error overflows and the cascades run unguarded (mfn_accumulate).
+template <int N, bool FAST>
+HP_HD mfn<N> mfn_mul_impl(const mfn<N>& a, const mfn<N>& b) {
   constexpr int LIMIT = 2 * N;
   double w[LIMIT];
 #pragma unroll
@@ -213,13 +217,22 @@ HP_HD mfn<N> mfn_mul(const mfn<N>& a, const mfn<N>& b) {
       constexpr int J = decltype(jc)::value;
       double p, e;
       two_prod(a.l[I], b.l[J], p, e);
-      mfn_accumulate<I + J, LIMIT>(w, p);
-      mfn_accumulate<I + J + 1, LIMIT>(w, e);  // I + J + 1 <= N < 2N
+      mfn_accumulate<I + J, LIMIT, !FAST>(w, p);
+      mfn_accumulate<I + J + 1, LIMIT, !FAST>(w, e);  // I + J + 1 <= N < 2N
     });
   });
   mfn<N> r;
   mfn_renormalize_from<N, LIMIT>(w, r.l);
   return r;
+}
+template <int N>
+HP_HD mfn<N> mfn_mul(const mfn<N>& a, const mfn<N>& b) {
+  constexpr double LIM = 0x1p500;
+  bool fast = true;
+#pragma unroll
+  for (int i = 0; i < N; ++i) fast &= (__builtin_fabs(a.l[i]) < LIM) & (__builtin_fabs(b.l[i]) < LIM);
+  if (fast) return mfn_mul_impl<N, true>(a, b);
+  return mfn_mul_impl<N, false>(a, b);
 }
 
 constexpr int mfn_ceil_log2(int n) {  // (n as f64).log2().ceil() for n = 1..8
