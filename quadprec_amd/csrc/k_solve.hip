@@ -535,6 +535,78 @@ int solve_inv_persistent(Mat l, const double* ih, const double* il, const double
 // panel is factored.  Every element receives the panels' updates in panel order.  All
 // updates subtract in place: S (a partial-sum plane in the other factorisations) is not
 // read or written.
+// The split-dd factorisation's update of the next panel's diagonal block (its lower
+// triangle): a(t0 + i, t0 + j) -= sum_k a(t0 + i, c0 + k) a(t0 + j, c0 + k), j <= i < w,
+// k < K.  cholesky_blocked's summation order is not needed here, so each 16 x 16 tile takes
+// a workgroup of 1024 threads.  The tile's 16 + 16 rows come through LDS in 64-deep k chunks
+// (coalesced 512-byte row pieces, the next chunk loaded into registers while the current
+// one is summed); thread group g (threadIdx / 256) sums the 16 k of its quarter of each
+// chunk with the compensated multiply-accumulate, the quarters are added in a fixed order,
+// and the dd total is subtracted.  The reference-order update (a 256-long dd chain per
+// element on 136 workgroups of 256) took ~40 us at w = K = 256, as did one reading its rows
+// straight from global memory 8 bytes at a time.
+constexpr int SYD_KC = 64;
+__global__ __launch_bounds__(1024) void k_diag_syrk_split(Mat a, long t0, long w, long c0, long K,
+                                                          const ErrRec* err) {
+  __shared__ double Ph[2][16][SYD_KC + 1], Pl[2][16][SYD_KC + 1];  // [0]: rows i, [1]: rows j
+  __shared__ double ph[4][256], pl[4][256];
+  if (err->flag) return;
+  // triangular tile index: block b -> (ti, tc), tc <= ti, b = ti (ti + 1) / 2 + tc
+  const long b = blockIdx.x;
+  long ti = (long)((__builtin_sqrt(8.0 * (double)b + 1.0) - 1.0) * 0.5);
+  while ((ti + 1) * (ti + 2) / 2 <= b) ++ti;
+  while (ti * (ti + 1) / 2 > b) --ti;
+  const long tc = b - ti * (ti + 1) / 2;
+  const int t = threadIdx.x & 255, g = threadIdx.x >> 8;
+  const int ty = t >> 4, tx = t & 15;
+  const long i = ti * 16 + ty, j = tc * 16 + tx;
+  const bool live = i < w && j <= i;
+  // loader: thread T stages rows lr (0..31: 16 rows i, then 16 rows j) at k offset lk, hi and lo
+  const int T = threadIdx.x, lr = T >> 5, lk = 2 * (T & 31);
+  const long grow = t0 + (lr < 16 ? ti * 16 + lr : tc * 16 + (lr - 16));
+  const bool rok = (lr < 16 ? ti * 16 + lr : tc * 16 + (lr - 16)) < w;
+  double v[4];
+  auto fetch = [&](long k0) {
+#pragma unroll
+    for (int q = 0; q < 2; ++q) {
+      const long k = k0 + lk + q;
+      const bool ok = rok && k < K;
+      const long o = grow * a.ld + c0 + k;
+      v[2 * q] = ok ? a.hi[o] : 0.0;
+      v[2 * q + 1] = ok ? a.lo[o] : 0.0;
+    }
+  };
+  AccDD acc{0.0, 0.0};
+  fetch(0);
+  for (long k0 = 0; k0 < K; k0 += SYD_KC) {
+    __syncthreads();  // the previous chunk has been read
+#pragma unroll
+    for (int q = 0; q < 2; ++q) {
+      Ph[lr >> 4][lr & 15][lk + q] = v[2 * q];
+      Pl[lr >> 4][lr & 15][lk + q] = v[2 * q + 1];
+    }
+    __syncthreads();
+    if (k0 + SYD_KC < K) fetch(k0 + SYD_KC);
+    if (live) {
+#pragma unroll
+      for (int kk = 16 * g; kk < 16 * g + 16; ++kk)
+        cacc_mac(acc, mkdd(Ph[0][ty][kk], Pl[0][ty][kk]), mkdd(Ph[1][tx][kk], Pl[1][tx][kk]));
+    }
+  }
+  ph[g][t] = acc.h;
+  pl[g][t] = acc.l;
+  __syncthreads();
+  if (g == 0 && live) {
+    AccDD sum{ph[0][t], pl[0][t]};
+#pragma unroll
+    for (int q = 1; q < 4; ++q) cacc_add(sum, AccDD{ph[q][t], pl[q][t]});
+    const long o = (t0 + i) * a.ld + t0 + j;
+    const dd r = dd_sub(mkdd(a.hi[o], a.lo[o]), cacc_value(sum));
+    a.hi[o] = r.hi;
+    a.lo[o] = r.lo;
+  }
+}
+
 int chol_split(Mat a, long n, long nb, Mat S, ErrRec* err, double*, double*, hipStream_t s) {
   if (nb <= 0) nb = 256;
   nb = std::max<long>(64, (nb / 64) * 64);
@@ -581,7 +653,12 @@ int chol_split(Mat a, long n, long nb, Mat S, ErrRec* err, double*, double*, hip
     // next panel's columns take this panel's update: its diagonal block on s (FP64 VALU),
     // the rows below on s3 (where the next panel's rows below are solved) as a split GEMM
     // in the second workspace slot, beside the trailing SYRK's
-    if ((rc = rank_update_sumsub<ArDD>(a, S, t0, w2, t0, w2, c0, jb, err, s))) return rc;
+    {
+      const long nt = (w2 + 15) / 16;
+      hipLaunchKernelGGL(k_diag_syrk_split, dim3((unsigned)(nt * (nt + 1) / 2)), dim3(1024), 0, s, a, t0, w2, c0, jb,
+                         (const ErrRec*)err);
+      HP_LAUNCH_CHECK();
+    }
     if (rem > w2) {
       const long r1 = t0 + w2;
       SplitOperand P{a.hi + r1 * a.ld + c0, a.lo + r1 * a.ld + c0, a.ld, true};
