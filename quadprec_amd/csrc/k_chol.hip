@@ -928,6 +928,8 @@ int chol_panel_exact(Mat a, Mat S, long c0, long jb, long n, ErrRec* err, double
 // in slot q of inv (2 x 64 doubles per slot).  evq: 16 events, [q] after sub-panel q's
 // diagonal block, [8 + q] after its rows within the panel; evdone is recorded on `side`
 // once the panel is complete and `s` waits for it.
+int diag_syrk_split(Mat a, long t0, long w, long c0, long K, ErrRec* err, hipStream_t s);  // k_solve.hip
+
 template <class A>
 int chol_panel_split(Mat a, Mat S, long c0, long jb, long n, ErrRec* err, double* inv, hipStream_t s,
                      hipStream_t side, hipEvent_t* evq, hipEvent_t evdone) {
@@ -947,7 +949,9 @@ int chol_panel_split(Mat a, Mat S, long c0, long jb, long n, ErrRec* err, double
     if (re > rb) {
       if ((rc = row_recur<A, false>(a, S, c0 + q0, w, rb, re, ih, il, err, s))) return rc;
       HP_HIP(hipEventRecord(evq[8 + q], s));
-      if ((rc = rank_update<A, RU_SUBINC, true>(a, S, rb, re - rb, rb, re - rb, rb, c0 + q0, w, err, s))) return rc;
+      // the panel's later columns of those rows (lower triangle), its summation order free:
+      // partial sums in parallel instead of one 64-long dd chain per element
+      if ((rc = diag_syrk_split(a, rb, re - rb, c0 + q0, w, err, s))) return rc;
     }
     if (rem > 0) {
       HP_HIP(hipStreamWaitEvent(side, evq[q], 0));

@@ -607,6 +607,16 @@ __global__ __launch_bounds__(1024) void k_diag_syrk_split(Mat a, long t0, long w
   }
 }
 
+// a(t0 + i, t0 + j) -= sum_k a(t0 + i, c0 + k) a(t0 + j, c0 + k), j <= i < w, k < K, on s
+int diag_syrk_split(Mat a, long t0, long w, long c0, long K, ErrRec* err, hipStream_t s) {
+  if (w <= 0 || K <= 0) return HP_OK;
+  const long nt = (w + 15) / 16;
+  hipLaunchKernelGGL(k_diag_syrk_split, dim3((unsigned)(nt * (nt + 1) / 2)), dim3(1024), 0, s, a, t0, w, c0, K,
+                     (const ErrRec*)err);
+  HP_LAUNCH_CHECK();
+  return HP_OK;
+}
+
 int chol_split(Mat a, long n, long nb, Mat S, ErrRec* err, double*, double*, hipStream_t s) {
   if (nb <= 0) nb = 256;
   nb = std::max<long>(64, (nb / 64) * 64);
@@ -653,12 +663,7 @@ int chol_split(Mat a, long n, long nb, Mat S, ErrRec* err, double*, double*, hip
     // next panel's columns take this panel's update: its diagonal block on s (FP64 VALU),
     // the rows below on s3 (where the next panel's rows below are solved) as a split GEMM
     // in the second workspace slot, beside the trailing SYRK's
-    {
-      const long nt = (w2 + 15) / 16;
-      hipLaunchKernelGGL(k_diag_syrk_split, dim3((unsigned)(nt * (nt + 1) / 2)), dim3(1024), 0, s, a, t0, w2, c0, jb,
-                         (const ErrRec*)err);
-      HP_LAUNCH_CHECK();
-    }
+    if ((rc = diag_syrk_split(a, t0, w2, c0, jb, err, s))) return rc;
     if (rem > w2) {
       const long r1 = t0 + w2;
       SplitOperand P{a.hi + r1 * a.ld + c0, a.lo + r1 * a.ld + c0, a.ld, true};
