@@ -928,7 +928,10 @@ int chol_panel_exact(Mat a, Mat S, long c0, long jb, long n, ErrRec* err, double
 // in slot q of inv (2 x 64 doubles per slot).  evq: 16 events, [q] after sub-panel q's
 // diagonal block, [8 + q] after its rows within the panel; evdone is recorded on `side`
 // once the panel is complete and `s` waits for it.
-int diag_syrk_split(Mat a, long t0, long w, long c0, long K, ErrRec* err, hipStream_t s);  // k_solve.hip
+// k_solve.hip: order-free updates a(r, c) -= sum_k a(r, c0 + k) a(c, c0 + k) of the lower
+// triangle of the block at (t0, t0), width w, or of the rows ri0.. x columns cj0..
+int diag_syrk_split(Mat a, long t0, long w, long c0, long K, ErrRec* err, hipStream_t s);
+int rect_syrk_split(Mat a, long ri0, long M, long cj0, long N, long c0, long K, ErrRec* err, hipStream_t s);
 
 template <class A>
 int chol_panel_split(Mat a, Mat S, long c0, long jb, long n, ErrRec* err, double* inv, hipStream_t s,
@@ -958,8 +961,7 @@ int chol_panel_split(Mat a, Mat S, long c0, long jb, long n, ErrRec* err, double
       if ((rc = row_recur<A, false>(a, S, c0 + q0, w, c0 + jb, n, ih, il, err, side))) return rc;
       if (re > rb) {  // L[rb:re, sub-panel q] comes from the row solve on s
         HP_HIP(hipStreamWaitEvent(side, evq[8 + q], 0));
-        if ((rc = rank_update<A, RU_SUBINC, false>(a, S, c0 + jb, rem, rb, re - rb, rb, c0 + q0, w, err, side)))
-          return rc;
+        if ((rc = rect_syrk_split(a, c0 + jb, rem, rb, re - rb, c0 + q0, w, err, side))) return rc;
       }
     }
   }

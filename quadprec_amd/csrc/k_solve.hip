@@ -535,36 +535,46 @@ int solve_inv_persistent(Mat l, const double* ih, const double* il, const double
 // panel is factored.  Every element receives the panels' updates in panel order.  All
 // updates subtract in place: S (a partial-sum plane in the other factorisations) is not
 // read or written.
-// The split-dd factorisation's update of the next panel's diagonal block (its lower
-// triangle): a(t0 + i, t0 + j) -= sum_k a(t0 + i, c0 + k) a(t0 + j, c0 + k), j <= i < w,
-// k < K.  cholesky_blocked's summation order is not needed here, so each 16 x 16 tile takes
-// a workgroup of 1024 threads.  The tile's 16 + 16 rows come through LDS in 64-deep k chunks
-// (coalesced 512-byte row pieces, the next chunk loaded into registers while the current
-// one is summed); thread group g (threadIdx / 256) sums the 16 k of its quarter of each
-// chunk with the compensated multiply-accumulate, the quarters are added in a fixed order,
-// and the dd total is subtracted.  The reference-order update (a 256-long dd chain per
-// element on 136 workgroups of 256) took ~40 us at w = K = 256, as did one reading its rows
+// Updates of the split-dd factorisation whose summation order is free:
+//   a(ri0 + i, cj0 + j) -= sum_k a(ri0 + i, c0 + k) a(cj0 + j, c0 + k),  i < M, j < N, k < K
+// (TRI: ri0 == cj0, M == N, j <= i only): the next panel's diagonal block at each panel
+// boundary, and within a panel the later columns of each sub-block's rows, both within the
+// panel (TRI) and below it (the side stream).  Each 16 x 16 tile takes a workgroup of 1024
+// threads.  The tile's 16 + 16 rows come through LDS in 64-deep k chunks (coalesced
+// 512-byte row pieces, the next chunk loaded into registers while the current one is
+// summed); thread group g (threadIdx / 256) sums the 16 k of its quarter of each chunk with
+// the compensated multiply-accumulate, the quarters are added in a fixed order, and the dd
+// total is subtracted.  The reference-order update (one 256-long dd chain per element on
+// 136 workgroups of 256) took ~40 us at M = N = K = 256, as did one reading its rows
 // straight from global memory 8 bytes at a time.
 constexpr int SYD_KC = 64;
-__global__ __launch_bounds__(1024) void k_diag_syrk_split(Mat a, long t0, long w, long c0, long K,
-                                                          const ErrRec* err) {
+template <bool TRI>
+__global__ __launch_bounds__(1024) void k_syrk_split(Mat a, long ri0, long M, long cj0, long N, long c0, long K,
+                                                     const ErrRec* err) {
   __shared__ double Ph[2][16][SYD_KC + 1], Pl[2][16][SYD_KC + 1];  // [0]: rows i, [1]: rows j
   __shared__ double ph[4][256], pl[4][256];
   if (err->flag) return;
-  // triangular tile index: block b -> (ti, tc), tc <= ti, b = ti (ti + 1) / 2 + tc
   const long b = blockIdx.x;
-  long ti = (long)((__builtin_sqrt(8.0 * (double)b + 1.0) - 1.0) * 0.5);
-  while ((ti + 1) * (ti + 2) / 2 <= b) ++ti;
-  while (ti * (ti + 1) / 2 > b) --ti;
-  const long tc = b - ti * (ti + 1) / 2;
+  long ti, tc;
+  if constexpr (TRI) {  // triangular tile index: b = ti (ti + 1) / 2 + tc, tc <= ti
+    ti = (long)((__builtin_sqrt(8.0 * (double)b + 1.0) - 1.0) * 0.5);
+    while ((ti + 1) * (ti + 2) / 2 <= b) ++ti;
+    while (ti * (ti + 1) / 2 > b) --ti;
+    tc = b - ti * (ti + 1) / 2;
+  } else {
+    const long ntc = (N + 15) / 16;
+    ti = b / ntc;
+    tc = b % ntc;
+  }
   const int t = threadIdx.x & 255, g = threadIdx.x >> 8;
   const int ty = t >> 4, tx = t & 15;
   const long i = ti * 16 + ty, j = tc * 16 + tx;
-  const bool live = i < w && j <= i;
-  // loader: thread T stages rows lr (0..31: 16 rows i, then 16 rows j) at k offset lk, hi and lo
+  const bool live = i < M && j < N && (!TRI || j <= i);
+  // loader: thread T stages row lr (0..31: 16 rows i, then 16 rows j) at k offset lk, hi and lo
   const int T = threadIdx.x, lr = T >> 5, lk = 2 * (T & 31);
-  const long grow = t0 + (lr < 16 ? ti * 16 + lr : tc * 16 + (lr - 16));
-  const bool rok = (lr < 16 ? ti * 16 + lr : tc * 16 + (lr - 16)) < w;
+  const long lrow = lr < 16 ? ti * 16 + lr : tc * 16 + (lr - 16);
+  const bool rok = lrow < (lr < 16 ? M : N);
+  const long grow = (lr < 16 ? ri0 : cj0) + lrow;
   double v[4];
   auto fetch = [&](long k0) {
 #pragma unroll
@@ -600,18 +610,31 @@ __global__ __launch_bounds__(1024) void k_diag_syrk_split(Mat a, long t0, long w
     AccDD sum{ph[0][t], pl[0][t]};
 #pragma unroll
     for (int q = 1; q < 4; ++q) cacc_add(sum, AccDD{ph[q][t], pl[q][t]});
-    const long o = (t0 + i) * a.ld + t0 + j;
+    const long o = (ri0 + i) * a.ld + cj0 + j;
     const dd r = dd_sub(mkdd(a.hi[o], a.lo[o]), cacc_value(sum));
     a.hi[o] = r.hi;
     a.lo[o] = r.lo;
   }
 }
 
-// a(t0 + i, t0 + j) -= sum_k a(t0 + i, c0 + k) a(t0 + j, c0 + k), j <= i < w, k < K, on s
+// the TRI form on the block at (t0, t0) of width w, on s
 int diag_syrk_split(Mat a, long t0, long w, long c0, long K, ErrRec* err, hipStream_t s) {
   if (w <= 0 || K <= 0) return HP_OK;
   const long nt = (w + 15) / 16;
-  hipLaunchKernelGGL(k_diag_syrk_split, dim3((unsigned)(nt * (nt + 1) / 2)), dim3(1024), 0, s, a, t0, w, c0, K,
+  hipLaunchKernelGGL(k_syrk_split<true>, dim3((unsigned)(nt * (nt + 1) / 2)), dim3(1024), 0, s, a, t0, w, t0, w, c0,
+                     K, (const ErrRec*)err);
+  HP_LAUNCH_CHECK();
+  return HP_OK;
+}
+// the rectangular form (rows ri0.., columns cj0..), on s
+int rect_syrk_split(Mat a, long ri0, long M, long cj0, long N, long c0, long K, ErrRec* err, hipStream_t s) {
+  if (M <= 0 || N <= 0 || K <= 0) return HP_OK;
+  const long nb = ((M + 15) / 16) * ((N + 15) / 16);
+  if (nb > 0x7fffffffL) {
+    set_error("split factorisation: update of %ld x %ld too large", M, N);
+    return HP_EINVAL;
+  }
+  hipLaunchKernelGGL(k_syrk_split<false>, dim3((unsigned)nb), dim3(1024), 0, s, a, ri0, M, cj0, N, c0, K,
                      (const ErrRec*)err);
   HP_LAUNCH_CHECK();
   return HP_OK;
