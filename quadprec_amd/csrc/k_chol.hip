@@ -494,9 +494,15 @@ __device__ __forceinline__ typename A::T tget(const LdsTri<A>& s, int e) {
   if constexpr (A::LIMBS == 2) return mkdd(s.h[e], s.l[e]);
   else return s.h[e];
 }
-template <class A, bool ACC>
+// CF (split-dd factorisation only, where the order is free; dd, SUB): each lane keeps its
+// entry as an unnormalised pair and subtracts b l_jk with the compensated multiply-subtract
+// (the product's leading part by two_sum, its FMA error and the cross terms in the second
+// double, ~12 FP64 operations against 29 for a dd multiply and subtract); the pair is
+// renormalised by two_sum when its column is next.
+template <class A, bool ACC, bool CF = false>
 __global__ __launch_bounds__(64 * row_recur_rows<A>()) void k_row_recur(Mat a, Mat S, long c0, int w, long row_begin, long row_end,
                                                    const double* inv_hi, const double* inv_lo, const ErrRec* err) {
+  static_assert(!CF || (A::LIMBS == 2 && !ACC), "the compensated form is the dd SUB mode");
   using T = typename A::T;
   extern __shared__ __align__(16) unsigned char smem[];
   LdsTri<A>& L = *reinterpret_cast<LdsTri<A>*>(smem);  // 31.5 KB for dd: fits beside the trailing updates
@@ -516,6 +522,37 @@ __global__ __launch_bounds__(64 * row_recur_rows<A>()) void k_row_recur(Mat a, M
   constexpr int RPB = row_recur_rows<A>();
   const int rowoff = lane * (lane - 1) / 2;  // this lane's row of the packed triangle
   for (long row = row_begin + (long)blockIdx.x * RPB + wv; row < row_end; row += (long)gridDim.x * RPB) {
+    if constexpr (CF) {
+      double th = 0.0, tl = 0.0;
+      dd fin = mkdd(0.0, 0.0);
+      if (lane < w) {
+        const dd v = A::ld(a, row, c0 + lane);
+        th = v.hi;
+        tl = v.lo;
+      }
+      dd lk = tget<A>(L, lane > 0 ? rowoff : 0);
+      for (int k = 0; k < w; ++k) {
+        const dd lnext = tget<A>(L, k + 1 < lane ? rowoff + k + 1 : 0);
+        dd tk;
+        two_sum(th, tl, tk.hi, tk.lo);
+        const dd bk = A::mul(tk, inv);  // meaningful on lane k
+        if (lane == k) fin = bk;
+        const dd b = rl(bk, k);
+        const double p = b.hi * lk.hi;
+        double e = __builtin_fma(b.hi, lk.hi, -p);
+        e = __builtin_fma(b.hi, lk.lo, e);
+        e = __builtin_fma(b.lo, lk.hi, e);
+        double sh, st;
+        two_sum(th, -p, sh, st);
+        const double nl = tl + (st - e);
+        const bool up = lane > k && lane < w;
+        th = up ? sh : th;
+        tl = up ? nl : tl;
+        lk = lnext;
+      }
+      if (lane < w) A::st(a, row, c0 + lane, fin);
+      continue;
+    }
     T t = A::zero(), av = A::zero(), fin = A::zero();
     if (lane < w) {
       if (ACC) { t = A::ld(S, row, c0 + lane); av = A::ld(a, row, c0 + lane); }
@@ -821,6 +858,9 @@ static int set_lds_attrs() {
     HP_HIP(hipFuncSetAttribute((const void*)k_diag_factor_split, hipFuncAttributeMaxDynamicSharedMemorySize, one));
   HP_HIP(hipFuncSetAttribute((const void*)k_row_recur<A, true>, hipFuncAttributeMaxDynamicSharedMemorySize, one));
   HP_HIP(hipFuncSetAttribute((const void*)k_row_recur<A, false>, hipFuncAttributeMaxDynamicSharedMemorySize, one));
+  if constexpr (A::LIMBS == 2)
+    HP_HIP(hipFuncSetAttribute((const void*)k_row_recur<A, false, true>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                               one));
   HP_HIP(hipFuncSetAttribute((const void*)k_fwd_diag<A>, hipFuncAttributeMaxDynamicSharedMemorySize, one));
   return HP_OK;
 }
@@ -854,13 +894,13 @@ static int rank_update(Mat a, Mat S, long ri0, long M, long cj0, long N, long bj
   return rank_update_rb<A, MODE, TRI, 1>(a, S, ri0, M, cj0, N, bj0, ka0, K, err, s);
 }
 
-template <class A, bool ACC>
+template <class A, bool ACC, bool CF = false>
 static int row_recur(Mat a, Mat S, long c0, int w, long rb, long re, double* inv_h, double* inv_l, ErrRec* err,
                      hipStream_t s) {
   if (re <= rb) return HP_OK;
   constexpr int RPB = row_recur_rows<A>();
   unsigned g = (unsigned)std::min<long>((re - rb + RPB - 1) / RPB, 8192 / RPB);
-  hipLaunchKernelGGL((k_row_recur<A, ACC>), dim3(g), dim3(64 * RPB), sizeof(LdsTri<A>), s, a, S, c0, w, rb, re,
+  hipLaunchKernelGGL((k_row_recur<A, ACC, CF>), dim3(g), dim3(64 * RPB), sizeof(LdsTri<A>), s, a, S, c0, w, rb, re,
                      inv_h, inv_l, (const ErrRec*)err);
   HP_LAUNCH_CHECK();
   return HP_OK;
@@ -950,7 +990,7 @@ int chol_panel_split(Mat a, Mat S, long c0, long jb, long n, ErrRec* err, double
     HP_HIP(hipEventRecord(evq[q], s));
     const long rb = c0 + q0 + w, re = c0 + jb;
     if (re > rb) {
-      if ((rc = row_recur<A, false>(a, S, c0 + q0, w, rb, re, ih, il, err, s))) return rc;
+      if ((rc = row_recur<A, false, true>(a, S, c0 + q0, w, rb, re, ih, il, err, s))) return rc;
       HP_HIP(hipEventRecord(evq[8 + q], s));
       // the panel's later columns of those rows (lower triangle), its summation order free:
       // partial sums in parallel instead of one 64-long dd chain per element
@@ -958,7 +998,7 @@ int chol_panel_split(Mat a, Mat S, long c0, long jb, long n, ErrRec* err, double
     }
     if (rem > 0) {
       HP_HIP(hipStreamWaitEvent(side, evq[q], 0));
-      if ((rc = row_recur<A, false>(a, S, c0 + q0, w, c0 + jb, n, ih, il, err, side))) return rc;
+      if ((rc = row_recur<A, false, true>(a, S, c0 + q0, w, c0 + jb, n, ih, il, err, side))) return rc;
       if (re > rb) {  // L[rb:re, sub-panel q] comes from the row solve on s
         HP_HIP(hipStreamWaitEvent(side, evq[8 + q], 0));
         if ((rc = rect_syrk_split(a, c0 + jb, rem, rb, re - rb, c0 + q0, w, err, side))) return rc;
